@@ -1,0 +1,153 @@
+/* insilica_hip.h -- the C ABI of libinsilica_hip.so: the stacked-layer TRiSK dynamical core with its split-explicit
+ * RK3 step, FCT tracer transport, the one-layer SWE solver and the air-sea coupling of InSilica World V2, as
+ * hand-written gfx950 HIP kernels.
+ *
+ * The reference drives these operations from Python through CuPy RawKernel launches (src/insilica/gpu.py:41-83); here
+ * each Python driver method of the reference (Layers.step, Transport.step_many, SWE.step, Ocean.couple, ...) is one
+ * entry point, which launches this library's own (fused) kernels in order on the given stream.
+ *
+ * Conventions (the reference's own, so tests and diagnostics read the same arrays):
+ *   - layered arrays are layer-major, x[l * n + i]; transport rows are y = tracer * L + layer, (rows, N);
+ *   - index tables are int32, padded with -1 to width 6 per cell and 3 per vertex; signs are int8;
+ *   - every pointer is a device pointer; optional arrays are NULL, as the reference passes NULL;
+ *   - nothing allocates and nothing synchronises: every buffer comes in, all work goes on `stream` (a hipStream_t;
+ *     NULL is the default stream), so a caller may capture any sequence of calls in a hipGraph;
+ *   - every entry point returns 0, a hipError_t code (launch failure), or ISW_EBADARG for sizes or options it does not
+ *     support (checked on the host before anything is launched).
+ *
+ * Precision: `prec` ISW_FP64 is the plain form (every operation in FP64, no contraction: the reference's "double"
+ * path, --fmad=false); ISW_MIXED is the reference's mixed form (src/insilica/gpu.py:16-19,46-47,77-80): divisions by
+ * fixed grid quantities become products with stored reciprocals, the PV flux gathers FP32 packed neighbour records
+ * with FP32 weights, the transport stencil runs on FP32 weights and an FP32 copy of the field, and multiply-adds may
+ * fuse; the state, the fast sub-step accumulation, the mass updates and the sums stay FP64.
+ */
+#ifndef INSILICA_HIP_H
+#define INSILICA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISW_FP64 0
+#define ISW_MIXED 1
+#define ISW_EBADARG 9001
+#define ISW_LMAX 8
+#define ISW_KSOIL 16
+
+/* The mesh on the device (reference: src/insilica/gpu.py:55-80 DeviceGrid; tables from src/insilica/grid/trisk.py).
+ * rA, rde, rAV: 1/areaCell, 1/de, 1/areaV; wf, d2wtf: FP32 copies of w and d2wt (read by the mixed form only).
+ * d2idx/d2wt: the transport's curvature stencil, 8 per edge (reference: src/insilica/gpu.py:86-108). */
+typedef struct isw_grid {
+  int32_t N, E, NV, n_eoe;
+  const int32_t *eI, *eJ, *nbr, *cellEdge, *cellEdgeAsc, *ev1, *ev2, *vEdge, *vCell, *eoeStart, *eoe, *d2idx;
+  const int8_t *cellSign, *vEdgeSign, *vSlot;
+  const double *le, *de, *areaCell, *areaV, *fV, *kite, *w, *d2wt;
+  const double *rA, *rde, *rAV;
+  const float *wf, *d2wtf;
+} isw_grid;
+
+/* One stack of layers and its preallocated scratch (reference: src/insilica/layers.py:161-218 Layers.__init__).
+ * air != 0: air layers (h is the mass per area, theta per layer); else Boussinesq layers (h in m).
+ * gp (L-1), drag (L), stress (L) are values in the struct. fV (NV): the Coriolis parameter with the rotation applied.
+ * b (N): the bottom. wall (E, nullable): 0 on the edges no water crosses. Gc (N): the fast mode's coefficient per cell.
+ * Scratch sizes -- (L,N): B, scale, h1, h2, D1, hold, pibar, S, muf, mup, phi; (L,NV): q, Z1;
+ * (L,E): Tp, F, u1, u2, L1, L2, hedge, uf, accL; (E): G, He, ubs, ub0, ub, acc, diss; (N): eta0, eta, eta_prev;
+ * pk: 3*L*E doubles (plain form); pkf: 3*L*E floats (mixed form; NULL otherwise). */
+typedef struct isw_layers {
+  int32_t L, air, prec, pad_;
+  double grav, cp, kappa, p0, rd, nu4, floor_, div_damp, thin, shelf, vmix, vmix_cd, thin_wind;
+  double gp[ISW_LMAX], drag[ISW_LMAX], stress[ISW_LMAX];
+  const double *fV, *b, *wall;
+  double *Gc;
+  double *B, *q, *Tp, *F, *scale, *G, *He, *ubs, *ub0, *ub, *acc, *eta0, *eta, *eta_prev, *h1, *h2, *u1, *u2;
+  double *D1, *Z1, *L1, *L2, *hold, *hedge, *diss, *pibar, *S, *muf, *mup, *phi, *uf, *accL, *pk;
+  float *pkf;
+} isw_layers;
+
+/* FCT transport scratch for up to rows_max rows (reference: src/insilica/transport.py:19-30 Transport.__init__).
+ * (rows,N): td, p1, p2, Rp, Rm; (rows,E): Hf, A; pf: (rows,N) floats for the mixed form's stencil copy, else NULL. */
+typedef struct isw_transport {
+  int32_t L, rows_max, prec, pad_;
+  double beta;
+  double *td, *p1, *p2, *Rp, *Rm, *Hf, *A;
+  float *pf;
+} isw_transport;
+
+/* Layers.step (reference: src/insilica/layers.py:311-338; kernels/layers.cu): one time-split RK3 step in place on
+ * h (L,N) and u (L,E); theta (L,N) for air, else NULL; Fm (L,E, nullable) receives the last stage's l*dt*F.
+ * Stage s takes ceil(c_s * nfast) fast sub-steps (c = 1/3, 1/2, 1). Then the interfacial stress (vmix). */
+int32_t isw_layers_step(const isw_grid* g, const isw_layers* lay, double* h, double* u, double* theta, double* Fm,
+                        double dt, int32_t nfast, void* stream);
+
+/* Layers._lap (reference: layers.py:249-253; kernels/trisk.cu:14-45): out = grad(div u) - k x grad(curl u), (L,E). */
+int32_t isw_layers_lap(const isw_grid* g, const isw_layers* lay, const double* u, double* out, void* stream);
+
+/* Layers.update_gfast (reference: layers.py:230-235; kernels/layers.cu:234-250 air_gfast): lay->Gc from h, theta. */
+int32_t isw_layers_gfast(const isw_grid* g, const isw_layers* lay, const double* h, const double* theta, void* stream);
+
+/* Layers.exchange (reference: layers.py:340-358; kernels/layers.cu:454-565,599-621): move wx (L-1,N) across the
+ * interfaces with tracer psi (L,N), volume M (L,N) and momentum; theta (nullable) for air, which takes the kinetic
+ * energy lost as heat along with nu4dt times the last step's del^4 loss; seq != 0 takes donors in passing order. */
+int32_t isw_layers_exchange(const isw_grid* g, const isw_layers* lay, const double* wx, double* h, double* u,
+                            double* M, double* psi, double* theta, int32_t seq, double nu4dt, void* stream);
+
+/* vmix_heat (reference: kernels/layers.cu:599-621; world.py:317-318): each edge's kinetic-energy loss diss (E) back
+ * as heat, half to each cell, spread over the air column by mass. */
+int32_t isw_vmix_heat(const isw_grid* g, const isw_layers* lay, const double* h, const double* diss, double* theta,
+                      void* stream);
+
+/* Transport.mass_after (reference: transport.py:32-35; kernels/trisk.cu:48-59): M1 = M0 moved by Fm, (L,N). */
+int32_t isw_mass_update(const isw_grid* g, int32_t L, const double* M0, const double* Fm, double* M1, void* stream);
+
+/* Transport.step_many (reference: transport.py:40-71; kernels/fused.cu:124-241): the rows tracer rows stacked in psi
+ * (rows, N; rows a multiple of tr->L, row y moved by layer y % L's fluxes) moved in place by the mass fluxes Fm (L,E)
+ * from M0 to M1 (L,N): 3rd-order fluxes in RK3 stages with Zalesak FCT (order 3), or upwind (order 1). */
+int32_t isw_transport_step(const isw_grid* g, const isw_transport* tr, const double* M0, const double* M1,
+                           const double* Fm, double* psi, int32_t rows, int32_t order, void* stream);
+
+/* SWE.step (reference: src/insilica/swe.py:124-132; kernels/fused.cu:27-120): Wicker-Skamarock RK3 with the
+ * positivity limiter, L independent layers. Scratch from `lay` (F, B, q, scale, h1, h2, u1, u2, D1, Z1, L1, L2);
+ * grav, drag (L), nu4, b (N) from `lay`; Fm (L,E, nullable) receives the last stage's l*dt*F. */
+int32_t isw_swe_step(const isw_grid* g, const isw_layers* lay, double* h, double* u, double* Fm, double dt,
+                     void* stream);
+
+/* Ocean.couple (reference: src/insilica/ocean.py:193-202; kernels/ocean.cu:17-42 couple_stress): the exact wind
+ * stress between the air's bottom layer (La layers: ha (La,N), ua (La,E)) and the ocean's top layer (ho, uo);
+ * diss (E) gets the kinetic energy removed, tau_acc (E) += tau^2 dt; wall (E) nullable; cd (N) nullable. */
+int32_t isw_couple_stress(const isw_grid* g, int32_t La, const double* ha, double* ua, const double* ho, double* uo,
+                          double c, double rho_w, double dt, double* diss, double* tau_acc, const double* wall,
+                          double land_drag, const double* cd, void* stream);
+
+/* Ocean.ground's land column (reference: ocean.py:177-182; kernels/ocean.cu:434-465 land_column): implicit soil
+ * conduction under the skin T[0] (N) of land cells, K <= 16 layers Tg (K,N) of thickness dz (K, device), one FP64
+ * Thomas solve per column (both forms). */
+int32_t isw_land_column(const isw_grid* g, int32_t K, const double* land, double* T, const double* cap, double* Tg,
+                        const double* dz, double c_soil, double k_soil, double dt, void* stream);
+
+/* Ocean.update_cap (reference: ocean.py:188-191; kernels/ocean.cu:468-471 ocn_cap). land nullable. */
+int32_t isw_ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* cap, const double* land, double c_land,
+                    void* stream);
+
+/* gpu.dsum (reference: src/insilica/gpu.py:111-126; kernels/trisk.cu:61-81 block_sum): the fixed-order tree sum of
+ * x (or of x*y, y nullable) over n values into out[0]: 256 threads x 8 values per block, a halving tree, passes
+ * repeated to one block. scratch: at least ceil(n / 2048) + ceil(n / 2048^2) + 2 doubles. */
+int32_t isw_dsum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream);
+
+/* A compensated, deterministic sum of x (or x*y) into out[0]: per-block Neumaier partial sums with their error terms
+ * in a fixed order, then one fixed-order compensated pass over the partials. scratch: 2 * ceil(n / 2048) doubles.
+ * The budgets' sum (no counterpart in the reference, whose totals use dsum). */
+int32_t isw_csum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream);
+
+/* Fill n doubles with v (a kernel, not hipMemsetAsync, so that captured graphs replay it safely). */
+int32_t isw_fill(double* x, int64_t n, double v, void* stream);
+
+/* Library version and the gfx target it was built for. */
+const char* isw_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,259 @@
+// The C ABI of libinsilica_hip.so (include/insilica_hip.h): argument checks on the host, then dispatch to the plain
+// FP64 (isw_f64) or the mixed (isw_mix) instantiation of core.hip; and the deterministic sums, which are FP64 in
+// both forms.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "insilica_hip.h"
+
+#define ISW_DECLARE(NS)                                                                                              \
+  namespace NS {                                                                                                     \
+  int layers_step(const isw_grid*, const isw_layers*, double*, double*, double*, double*, double, int, hipStream_t); \
+  int layers_lap(const isw_grid*, const isw_layers*, const double*, double*, hipStream_t);                          \
+  int layers_gfast(const isw_grid*, const isw_layers*, const double*, const double*, hipStream_t);                  \
+  int layers_exchange(const isw_grid*, const isw_layers*, const double*, double*, double*, double*, double*,        \
+                      double*, int, double, hipStream_t);                                                            \
+  int vmix_heat(const isw_grid*, const isw_layers*, const double*, const double*, double*, hipStream_t);            \
+  int mass_update(const isw_grid*, int, const double*, const double*, double*, hipStream_t);                        \
+  int transport_step(const isw_grid*, const isw_transport*, const double*, const double*, const double*, double*,   \
+                     int, int, hipStream_t);                                                                         \
+  int swe_step(const isw_grid*, const isw_layers*, double*, double*, double*, double, hipStream_t);                 \
+  int couple_stress(const isw_grid*, int, const double*, double*, const double*, double*, double, double, double,   \
+                    double*, double*, const double*, double, const double*, hipStream_t);                           \
+  int land_column(const isw_grid*, int, const double*, double*, const double*, double*, const double*, double,      \
+                  double, double, hipStream_t);                                                                      \
+  int ocn_cap(const isw_grid*, const double*, double, double*, const double*, double, hipStream_t);                 \
+  int fill(double*, long, double, hipStream_t);                                                                      \
+  }
+ISW_DECLARE(isw_f64)
+ISW_DECLARE(isw_mix)
+
+namespace {
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+bool grid_ok(const isw_grid* g) {
+  return g && g->N > 0 && g->E > 0 && g->NV > 0 && g->eI && g->eJ && g->cellEdge && g->cellEdgeAsc && g->cellSign &&
+         g->le && g->de && g->areaCell;
+}
+
+bool layers_ok(const isw_grid* g, const isw_layers* s) {
+  if (!grid_ok(g) || !s || s->L < 1 || s->L > ISW_LMAX) return false;
+  if (s->prec != ISW_FP64 && s->prec != ISW_MIXED) return false;
+  if (s->prec == ISW_MIXED && (!s->pkf || !g->wf || !g->rA || !g->rde || !g->rAV)) return false;
+  if (s->prec == ISW_FP64 && !s->pk) return false;
+  return s->fV && s->b && s->Gc && s->B && s->q && s->Tp && s->G && s->He && s->ubs && s->scale && s->h1 && s->h2 &&
+         s->u1 && s->u2 && s->D1 && s->Z1 && s->L1 && s->L2 && s->hedge && s->diss && s->hold &&
+         (!s->air || (s->pibar && s->S && s->muf && s->phi && s->uf && s->accL)) &&
+         (s->air || (s->ub0 && s->ub && s->acc && s->eta0 && s->eta && s->eta_prev));
+}
+
+}  // namespace
+
+#define DISPATCH(prec, call) ((prec) == ISW_MIXED ? isw_mix::call : isw_f64::call)
+
+extern "C" {
+
+int32_t isw_layers_step(const isw_grid* g, const isw_layers* lay, double* h, double* u, double* theta, double* Fm,
+                        double dt, int32_t nfast, void* stream) {
+  if (!layers_ok(g, lay) || !h || !u || nfast < 1 || !(dt > 0) || (lay->air && !theta)) return ISW_EBADARG;
+  return DISPATCH(lay->prec, layers_step(g, lay, h, u, lay->air ? theta : nullptr, Fm, dt, nfast, S(stream)));
+}
+
+int32_t isw_layers_lap(const isw_grid* g, const isw_layers* lay, const double* u, double* out, void* stream) {
+  if (!layers_ok(g, lay) || !u || !out) return ISW_EBADARG;
+  return DISPATCH(lay->prec, layers_lap(g, lay, u, out, S(stream)));
+}
+
+int32_t isw_layers_gfast(const isw_grid* g, const isw_layers* lay, const double* h, const double* theta,
+                         void* stream) {
+  if (!layers_ok(g, lay) || !h || (lay->air && !theta)) return ISW_EBADARG;
+  return DISPATCH(lay->prec, layers_gfast(g, lay, h, theta, S(stream)));
+}
+
+int32_t isw_layers_exchange(const isw_grid* g, const isw_layers* lay, const double* wx, double* h, double* u,
+                            double* M, double* psi, double* theta, int32_t seq, double nu4dt, void* stream) {
+  if (!layers_ok(g, lay) || !wx || !h || !u || !M || !psi) return ISW_EBADARG;
+  return DISPATCH(lay->prec, layers_exchange(g, lay, wx, h, u, M, psi, theta, seq, nu4dt, S(stream)));
+}
+
+int32_t isw_vmix_heat(const isw_grid* g, const isw_layers* lay, const double* h, const double* diss, double* theta,
+                      void* stream) {
+  if (!layers_ok(g, lay) || !h || !diss || !theta) return ISW_EBADARG;
+  return DISPATCH(lay->prec, vmix_heat(g, lay, h, diss, theta, S(stream)));
+}
+
+int32_t isw_mass_update(const isw_grid* g, int32_t L, const double* M0, const double* Fm, double* M1, void* stream) {
+  if (!grid_ok(g) || L < 1 || L > ISW_LMAX || !M0 || !Fm || !M1) return ISW_EBADARG;
+  return isw_f64::mass_update(g, L, M0, Fm, M1, S(stream));
+}
+
+int32_t isw_transport_step(const isw_grid* g, const isw_transport* tr, const double* M0, const double* M1,
+                           const double* Fm, double* psi, int32_t rows, int32_t order, void* stream) {
+  if (!grid_ok(g) || !tr || tr->L < 1 || rows < 1 || rows > tr->rows_max || rows % tr->L || !M0 || !M1 || !Fm ||
+      !psi || (order != 1 && order != 3) || !g->nbr || !g->d2idx || !tr->td)
+    return ISW_EBADARG;
+  if (order == 3 && (!tr->p1 || !tr->p2 || !tr->Rp || !tr->Rm || !tr->Hf || !tr->A)) return ISW_EBADARG;
+  if (tr->prec == ISW_MIXED && (!tr->pf || !g->d2wtf)) return ISW_EBADARG;
+  if (tr->prec == ISW_FP64 && !g->d2wt) return ISW_EBADARG;
+  return DISPATCH(tr->prec, transport_step(g, tr, M0, M1, Fm, psi, rows, order, S(stream)));
+}
+
+int32_t isw_swe_step(const isw_grid* g, const isw_layers* lay, double* h, double* u, double* Fm, double dt,
+                     void* stream) {
+  if (!grid_ok(g) || !lay || lay->L < 1 || lay->L > ISW_LMAX || !h || !u || !lay->F || !lay->B || !lay->q ||
+      !lay->scale || !lay->h1 || !lay->h2 || !lay->u1 || !lay->u2 || !lay->b || !lay->fV || !g->eoe || !g->w ||
+      (lay->nu4 != 0 && (!lay->D1 || !lay->Z1 || !lay->L1 || !lay->L2)))
+    return ISW_EBADARG;
+  if (lay->prec == ISW_MIXED && !g->rAV) return ISW_EBADARG;
+  return DISPATCH(lay->prec, swe_step(g, lay, h, u, Fm, dt, S(stream)));
+}
+
+int32_t isw_couple_stress(const isw_grid* g, int32_t La, const double* ha, double* ua, const double* ho, double* uo,
+                          double c, double rho_w, double dt, double* diss, double* tau_acc, const double* wall,
+                          double land_drag, const double* cd, void* stream) {
+  if (!grid_ok(g) || La < 1 || La > ISW_LMAX || !ha || !ua || !ho || !uo || !diss || !tau_acc || !(dt > 0))
+    return ISW_EBADARG;
+  return isw_f64::couple_stress(g, La, ha, ua, ho, uo, c, rho_w, dt, diss, tau_acc, wall, land_drag, cd, S(stream));
+}
+
+int32_t isw_land_column(const isw_grid* g, int32_t K, const double* land, double* T, const double* cap, double* Tg,
+                        const double* dz, double c_soil, double k_soil, double dt, void* stream) {
+  if (!g || g->N < 1 || K < 1 || K > ISW_KSOIL || !land || !T || !cap || !Tg || !dz) return ISW_EBADARG;
+  return isw_f64::land_column(g, K, land, T, cap, Tg, dz, c_soil, k_soil, dt, S(stream));
+}
+
+int32_t isw_ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* cap, const double* land, double c_land,
+                    void* stream) {
+  if (!g || g->N < 1 || !h || !cap) return ISW_EBADARG;
+  return isw_f64::ocn_cap(g, h, rho_cw, cap, land, c_land, S(stream));
+}
+
+int32_t isw_fill(double* x, int64_t n, double v, void* stream) {
+  if (n < 0 || (n > 0 && !x)) return ISW_EBADARG;
+  return isw_f64::fill(x, (long)n, v, S(stream));
+}
+
+const char* isw_version(void) { return "insilica_hip 0.1 gfx950"; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------------------------
+// Deterministic sums (both forms FP64).
+// ---------------------------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int SUM_THREADS = 256, SUM_CHUNK = 8;
+
+// block_sum (trisk.cu:66-81): thread t adds x[base + t + 256 k], k = 0..7, then a halving tree in LDS.
+__global__ __launch_bounds__(SUM_THREADS) void k_block_sum(long long n, const double* __restrict__ x,
+                                                           const double* __restrict__ y, double* __restrict__ out) {
+  __shared__ double s[SUM_THREADS];
+  const long long base = (long long)blockIdx.x * SUM_THREADS * SUM_CHUNK;
+  double acc = 0;
+  for (int k = 0; k < SUM_CHUNK; k++) {
+    const long long i = base + threadIdx.x + (long long)k * SUM_THREADS;
+    if (i < n) acc += y ? x[i] * y[i] : x[i];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = SUM_THREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+// Neumaier's compensated addition of v into (s, c).
+__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
+  const double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  s = t;
+}
+
+// Pass 1: each block's compensated partial (sum, error) over 2048 consecutive values, in a fixed order: thread t
+// sums its 8 values, then thread 0 folds the 256 (sum, error) pairs in thread order.
+__global__ __launch_bounds__(SUM_THREADS) void k_csum_blocks(long long n, const double* __restrict__ x,
+                                                             const double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double ss[SUM_THREADS], sc[SUM_THREADS];
+  const long long base = (long long)blockIdx.x * SUM_THREADS * SUM_CHUNK;
+  double s = 0, c = 0;
+  for (int k = 0; k < SUM_CHUNK; k++) {
+    const long long i = base + (long long)threadIdx.x * SUM_CHUNK + k;
+    if (i < n) neumaier(s, c, y ? x[i] * y[i] : x[i]);
+  }
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S0 = 0, C0 = 0;
+    for (int t = 0; t < SUM_THREADS; t++) {
+      neumaier(S0, C0, ss[t]);
+      C0 += sc[t];
+    }
+    part[2 * blockIdx.x] = S0;
+    part[2 * blockIdx.x + 1] = C0;
+  }
+}
+
+// Pass 2: one block folds the partials in block order, each wave-sized slice by one thread, then thread 0.
+__global__ __launch_bounds__(SUM_THREADS) void k_csum_final(long long nb, const double* __restrict__ part,
+                                                            double* __restrict__ out) {
+  __shared__ double ss[SUM_THREADS], sc[SUM_THREADS];
+  const long long per = (nb + SUM_THREADS - 1) / SUM_THREADS;
+  double s = 0, c = 0;
+  for (long long k = 0; k < per; k++) {
+    const long long b = (long long)threadIdx.x * per + k;
+    if (b < nb) {
+      neumaier(s, c, part[2 * b]);
+      c += part[2 * b + 1];
+    }
+  }
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S0 = 0, C0 = 0;
+    for (int t = 0; t < SUM_THREADS; t++) {
+      neumaier(S0, C0, ss[t]);
+      C0 += sc[t];
+    }
+    out[0] = S0 + C0;
+    out[1] = C0;
+  }
+}
+
+}  // namespace
+
+extern "C" int32_t isw_dsum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream) {
+  if (n < 1 || !x || !scratch || !out) return ISW_EBADARG;
+  hipStream_t st = S(stream);
+  const long long per = (long long)SUM_THREADS * SUM_CHUNK;
+  long long m = n;
+  const double *xx = x, *yy = y;
+  double* buf[2] = {scratch, scratch + (n + per - 1) / per + 1};
+  int which = 0;
+  for (;;) {
+    const long long blocks = (m + per - 1) / per;
+    double* dst = blocks == 1 ? out : buf[which];
+    k_block_sum<<<dim3((unsigned)blocks), SUM_THREADS, 0, st>>>(m, xx, yy, dst);
+    if (blocks == 1) break;
+    xx = dst;
+    yy = nullptr;
+    m = blocks;
+    which ^= 1;
+  }
+  return (int32_t)hipGetLastError();
+}
+
+extern "C" int32_t isw_csum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream) {
+  if (n < 1 || !x || !scratch || !out) return ISW_EBADARG;
+  hipStream_t st = S(stream);
+  const long long per = (long long)SUM_THREADS * SUM_CHUNK;
+  const long long nb = (n + per - 1) / per;
+  k_csum_blocks<<<dim3((unsigned)nb), SUM_THREADS, 0, st>>>(n, x, y, scratch);
+  k_csum_final<<<dim3(1), SUM_THREADS, 0, st>>>(nb, scratch, out);
+  return (int32_t)hipGetLastError();
+}

@@ -1,0 +1,1247 @@
+// The TRiSK layered core, FCT transport, SWE solver and air-sea coupling as gfx950 HIP kernels, with the host-side
+// sequencing of each C-ABI entry point (the reference's Python drivers: layers.py, transport.py, swe.py, ocean.py).
+//
+// This file is compiled twice: with ISW_FORM_MIXED=0 and -ffp-contract=off into namespace isw_f64 (the plain FP64 form:
+// every operation rounds on its own, as the reference's --fmad=false build), and with ISW_FORM_MIXED=1 and
+// -ffp-contract=fast into namespace isw_mix (the reference's mixed form). capi.hip dispatches on the `prec` field.
+//
+// Design (MI355X): one thread per cell, edge or vertex, 256-thread blocks (four 64-wide waves), layer-major arrays so
+// consecutive threads touch consecutive words. The reference's one-kernel-per-loop structure is fused wherever a
+// value can be recomputed bit for bit instead of stored and re-read: the Bernoulli (cell) and PV (vertex) passes run
+// in one launch; the air fast sub-steps read the stage's start state directly (no copies); the stage flux
+// F = h_e * mean(u) is recomputed inside the limiter and the divergence instead of being stored, the limited
+// flux's volume Fm is written by each edge's eI cell, and the wall mask is applied inside the kernels that write.
+// Every sum keeps the reference's order (a cell's edges in cellEdge or cellEdgeAsc order, a vertex's three in table
+// order, an edge's eoe neighbours ascending), so the FP64 form matches the reference's double path value for value.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "insilica_hip.h"
+
+#if ISW_FORM_MIXED
+#define NS isw_mix
+#define DIVBY(x, r) ((x) * (r))
+typedef float fast;
+#else
+#define NS isw_f64
+#define DIVBY(x, r) ((x) / (r))
+typedef double fast;
+#endif
+
+namespace NS {
+
+constexpr int BS = 256;
+#define TID ((int)(blockIdx.x * blockDim.x + threadIdx.x))
+
+static inline dim3 g1(long n) { return dim3((unsigned)((n + BS - 1) / BS)); }
+static inline dim3 g2(long n, int y) { return dim3((unsigned)((n + BS - 1) / BS), (unsigned)y); }
+
+// Settings of a layer stack passed by value (reference: Layers.__init__, src/insilica/layers.py:161-218).
+struct LP {
+  int L, air;
+  double grav, cp, kappa, p0, floor_, div_damp, thin, shelf, nu4;
+  double gp[ISW_LMAX], drag[ISW_LMAX], stress[ISW_LMAX];
+};
+
+static LP make_lp(const isw_layers* s) {
+  LP p;
+  p.L = s->L; p.air = s->air;
+  p.grav = s->grav; p.cp = s->cp; p.kappa = s->kappa; p.p0 = s->p0; p.floor_ = s->floor_; p.div_damp = s->div_damp;
+  p.thin = s->thin; p.shelf = s->shelf; p.nu4 = s->nu4;
+  for (int k = 0; k < ISW_LMAX; k++) { p.gp[k] = s->gp[k]; p.drag[k] = s->drag[k]; p.stress[k] = s->stress[k]; }
+  return p;
+}
+
+// What the DIVBY divisions take: the quantity (plain form) or its stored reciprocal (mixed form).
+struct Div {
+  const double *A, *de, *AV;
+};
+static Div divs(const isw_grid* g) {
+#if ISW_FORM_MIXED
+  return Div{g->rA, g->rde, g->rAV};
+#else
+  return Div{g->areaCell, g->de, g->areaV};
+#endif
+}
+
+__global__ void k_copy(long n, const double* __restrict__ a, double* __restrict__ b) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_fill(long n, double v, double* __restrict__ x) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// A layer's thickness at an edge (layers.cu:42-45), ob NULL.
+__device__ __forceinline__ double edge_h(double a, double b, double shelf) {
+  const double m = a < b ? a : b;
+  return shelf > 0 && m < shelf ? m : 0.5 * (a + b);
+}
+
+__device__ __forceinline__ double cell_ke(const isw_grid& g, const double* __restrict__ ul, int i) {
+  double ke = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int e = g.cellEdge[i * 6 + m];
+    if (e < 0) break;
+    ke += g.le[e] * g.de[e] * 0.25 * ul[e] * ul[e];
+  }
+  return ke;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Mesh operators: the vector Laplacian (trisk.cu:14-45). Cells and vertices in one launch.
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_div_curl(isw_grid g, int L, int nbc, const double* __restrict__ u,
+                                                 double* __restrict__ D, double* __restrict__ Z) {
+  if ((int)blockIdx.x < nbc) {
+    const int t = TID;
+    if (t >= g.N * L) return;
+    const int l = t / g.N, i = t - l * g.N;
+    const double* ul = u + (size_t)l * g.E;
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const int e = g.cellEdge[i * 6 + k];
+      if (e < 0) break;
+      s += (double)g.cellSign[i * 6 + k] * g.le[e] * ul[e];
+    }
+    D[t] = s / g.areaCell[i];
+  } else {
+    const int t = (int)((blockIdx.x - nbc) * blockDim.x + threadIdx.x);
+    if (t >= g.NV * L) return;
+    const int l = t / g.NV, v = t - l * g.NV;
+    const double* ul = u + (size_t)l * g.E;
+    double z = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int e = g.vEdge[v * 3 + k];
+      z += (double)g.vEdgeSign[v * 3 + k] * g.de[e] * ul[e];
+    }
+    Z[t] = z / g.areaV[v];
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_edge_lap(isw_grid g, int L, const double* __restrict__ D,
+                                                 const double* __restrict__ Z, double* __restrict__ out) {
+  const int t = TID;
+  if (t >= g.E * L) return;
+  const int l = t / g.E, e = t - l * g.E;
+  const double* Dl = D + (size_t)l * g.N;
+  const double* Zl = Z + (size_t)l * g.NV;
+  out[t] = (Dl[g.eJ[e]] - Dl[g.eI[e]]) / g.de[e] - (Zl[g.ev2[e]] - Zl[g.ev1[e]]) / g.le[e];
+}
+
+static int lap(const isw_grid* g, int L, const double* u, double* D, double* Z, double* out, hipStream_t st) {
+  const int nbc = (g->N * L + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
+  k_div_curl<<<dim3(nbc + nbv), BS, 0, st>>>(*g, L, nbc, u, D, Z);
+  k_edge_lap<<<g1((long)g->E * L), BS, 0, st>>>(*g, L, D, Z, out);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Column kernels at the step's start (layers.cu:33-74,165-175,234-250).
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_air_stiff(int N, LP p, const double* __restrict__ h,
+                                                  const double* __restrict__ th, double* __restrict__ S) {
+  const int i = TID;
+  if (i >= N) return;
+  double pr = 0;
+  for (int k = 0; k < p.L; k++) {
+    pr += p.grav * h[(size_t)k * N + i];
+    const double dth = th[(size_t)k * N + i] - (k + 1 < p.L ? th[(size_t)(k + 1) * N + i] : 0.0);
+    const double dpi = p.kappa * p.cp * pow(pr / p.p0, p.kappa) / pr;
+    S[(size_t)k * N + i] = dth > 0 ? dth * p.grav * dpi : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_air_gfast(int N, LP p, double rgas, const double* __restrict__ h,
+                                                  const double* __restrict__ th, double* __restrict__ Gc) {
+  const int i = TID;
+  if (i >= N) return;
+  const int L = p.L;
+  double mu = 0;
+  for (int k = 0; k < L; k++) mu += h[(size_t)k * N + i];
+  int kb = L - 1;
+  while (kb > 0 && h[(size_t)kb * N + i] < 2 * p.floor_) kb--;
+  const double Ts = th[(size_t)kb * N + i] * pow(p.grav * mu / p.p0, p.kappa);
+  double gc = rgas * Ts / mu;
+  double above = 0;
+  for (int j = 0; j < L - 1; j++) {
+    above += h[(size_t)j * N + i];
+    const double dth = th[(size_t)j * N + i] - th[(size_t)(j + 1) * N + i];
+    if (dth > 0) gc += dth * rgas * pow(p.grav * above / p.p0, p.kappa) * above / (mu * mu);
+  }
+  Gc[i] = gc;
+}
+
+__global__ __launch_bounds__(BS) void k_lay_eta(int N, int L, const double* __restrict__ h,
+                                                const double* __restrict__ b, double* __restrict__ eta) {
+  const int i = TID;
+  if (i >= N) return;
+  double z = b[i];
+  for (int k = L - 1; k >= 0; k--) z += h[(size_t)k * N + i];
+  eta[i] = z;
+}
+
+__global__ __launch_bounds__(BS) void k_lay_ubar(isw_grid g, int L, double shelf, const double* __restrict__ h,
+                                                 const double* __restrict__ u, const double* __restrict__ wall,
+                                                 double* __restrict__ ubar) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int N = g.N, eI = g.eI[e], eJ = g.eJ[e];
+  double H = 0, U = 0;
+  for (int k = 0; k < L; k++) {
+    const double he = edge_h(h[(size_t)k * N + eI], h[(size_t)k * N + eJ], shelf);
+    H += he;
+    U += he * u[(size_t)k * g.E + e];
+  }
+  ubar[e] = wall && wall[e] == 0 ? 0.0 : U / H;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Per stage, pass 1: the Bernoulli/Montgomery potential per cell (air_bern / lay_bern, layers.cu:83-158) and the PV
+// at vertices (swe_vertex, fused.cu:82-98), in one launch: blocks [0, nbc) take cells, the rest (vertex, layer).
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_bern_pv(isw_grid g, LP p, Div dv, int nbc, const double* __restrict__ h,
+                                                const double* __restrict__ u, const double* __restrict__ th,
+                                                const double* __restrict__ b, const double* __restrict__ S,
+                                                const double* __restrict__ fV, double* __restrict__ B,
+                                                double* __restrict__ pibar, double* __restrict__ q) {
+  const int N = g.N, E = g.E, L = p.L;
+  if ((int)blockIdx.x < nbc) {
+    const int i = TID;
+    if (i >= N) return;
+    if (p.air) {
+      double xb[ISW_LMAX], Mk[ISW_LMAX];
+      double pr = 0, xt = 0;
+      for (int k = 0; k < L; k++) {
+        const double hk = h[(size_t)k * N + i];
+        const double pt = pr;
+        pr = pr + p.grav * hk;
+        const double x = pow(pr / p.p0, p.kappa);
+        xb[k] = x;
+        pibar[(size_t)k * N + i] = p.cp * (pr * x - pt * xt) / ((p.kappa + 1) * (pr - pt));
+        xt = x;
+      }
+      double M = p.grav * b[i] + th[(size_t)(L - 1) * N + i] * p.cp * xb[L - 1];
+      Mk[L - 1] = M;
+      for (int k = L - 2; k >= 0; k--) {
+        M = M + (th[(size_t)k * N + i] - th[(size_t)(k + 1) * N + i]) * p.cp * xb[k];
+        Mk[k] = M;
+      }
+      double P[ISW_LMAX], Phi[ISW_LMAX];
+      double a = 0;
+      for (int k = 0; k < L; k++) { a += h[(size_t)k * N + i]; P[k] = a; }
+      double ph = 0;
+      for (int k = L - 1; k >= 0; k--) { ph += S[(size_t)k * N + i] * P[k]; Phi[k] = ph; }
+      for (int k = 0; k < L; k++) {
+        const double ke = cell_ke(g, u + (size_t)k * E, i);
+        B[(size_t)k * N + i] = Mk[k] - Phi[k] + DIVBY(ke, dv.A[i]);
+      }
+    } else {
+      double z[ISW_LMAX];
+      double s = b[i];
+      for (int k = L - 1; k >= 0; k--) { s += h[(size_t)k * N + i]; z[k] = s; }
+      double Mp = 0;
+      for (int k = 0; k < L; k++) {
+        if (k > 0) Mp += p.gp[k - 1] * z[k];
+        const double ke = cell_ke(g, u + (size_t)k * E, i);
+        B[(size_t)k * N + i] = Mp + ke / g.areaCell[i];
+      }
+    }
+  } else {
+    const int t = (int)((blockIdx.x - nbc) * blockDim.x + threadIdx.x);
+    if (t >= g.NV * L) return;
+    const int l = t / g.NV, v = t - l * g.NV;
+    const double* hl = h + (size_t)l * N;
+    const double* ul = u + (size_t)l * E;
+    double hv = 0, z = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int c = g.vCell[v * 3 + k];
+      hv += g.kite[c * 6 + g.vSlot[v * 3 + k]] * hl[c] * 1.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int e = g.vEdge[v * 3 + k];
+      z += (double)g.vEdgeSign[v * 3 + k] * g.de[e] * ul[e];
+    }
+    q[t] = (DIVBY(z, dv.AV[v]) + fV[v]) / DIVBY(hv, dv.AV[v]);
+  }
+}
+
+// Per stage, pass 2 (lay_edge_hq, layers.cu:254-263): each edge's h_e and q_e; [edge][layer][h_e, u, q_e] packed for
+// the PV flux's neighbour gathers.
+__global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf, const double* __restrict__ h,
+                                                const double* __restrict__ q, const double* __restrict__ u,
+                                                double* __restrict__ he, fast* __restrict__ pk) {
+  const int t = TID;
+  if (t >= g.E * L) return;
+  const int l = t / g.E, e = t - l * g.E;
+  const int N = g.N, NV = g.NV;
+  const double hv = edge_h(h[(size_t)l * N + g.eI[e]], h[(size_t)l * N + g.eJ[e]], shelf);
+  const double qv = 0.5 * (q[(size_t)l * NV + g.ev1[e]] + q[(size_t)l * NV + g.ev2[e]]);
+  he[t] = hv;
+  fast* pp = pk + ((size_t)e * L + l) * 3;
+  pp[0] = (fast)hv; pp[1] = (fast)u[t]; pp[2] = (fast)qv;
+}
+
+// Per stage, pass 3 (lay_slow, layers.cu:269-359): the slow momentum tendency of every layer of an edge (W-weight PV
+// flux with the thin-layer cap, -grad B, drag, exact-exponential stress, pibar grad theta for air, del^4), its
+// thickness-weighted column mean G, T' = T - G, H_e and the column wind. The eoe neighbours are walked once for all
+// layers (each layer's sum still runs in ascending neighbour order).
+__global__ __launch_bounds__(BS) void k_slow(isw_grid g, LP p, Div dv, double dts, const fast* __restrict__ pk,
+                                             const fast* __restrict__ wf, const double* __restrict__ hedge,
+                                             const double* __restrict__ u, const double* __restrict__ B,
+                                             const double* __restrict__ lap4, const double* __restrict__ th,
+                                             const double* __restrict__ pibar, double* __restrict__ Tp,
+                                             double* __restrict__ G, double* __restrict__ He,
+                                             double* __restrict__ ubs) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int N = g.N, E = g.E, L = p.L;
+  const int eI = g.eI[e], eJ = g.eJ[e];
+  fast Q[ISW_LMAX], cap[ISW_LMAX], qf[ISW_LMAX];
+  for (int k = 0; k < L; k++) {
+    const double he = hedge[(size_t)k * E + e];
+    cap[k] = (fast)(p.thin * he);
+    qf[k] = pk[((size_t)e * L + k) * 3 + 2];
+    Q[k] = 0;
+  }
+  const int m0 = g.eoeStart[e], m1 = g.eoeStart[e + 1];
+  for (int m = m0; m < m1; m++) {
+    const fast* pp = pk + (size_t)g.eoe[m] * L * 3;
+    const fast w = wf[m];
+    for (int k = 0; k < L; k++) {
+      fast h2 = pp[3 * k];
+      if (h2 > cap[k]) h2 = cap[k];
+      const fast F2 = h2 * pp[3 * k + 1];
+      Q[k] += w * F2 * (fast)0.5 * (qf[k] + pp[3 * k + 2]);
+    }
+  }
+  double T[ISW_LMAX], hev[ISW_LMAX], sqk[ISW_LMAX];
+  double H = 0, U = 0;
+  for (int k = 0; k < L; k++) {
+    const size_t o = (size_t)k * E + e;
+    const double he = hedge[o];
+    const double ue = u[o];
+    const double shk = p.shelf;
+    const double sq0 =
+        shk > 0 && he < shk ? (he - 2 * p.floor_ > 0 ? (he - 2 * p.floor_) / (shk - 2 * p.floor_) : 0.0) : 1.0;
+    const double sq = th ? 1.0 : sq0;
+    sqk[k] = sq0;
+    const double* Bl = B + (size_t)k * N;
+    double d = Q[k] - DIVBY(sq * (Bl[eJ] - Bl[eI]), dv.de[e]) - (p.drag[k] + (1 - sq) / 21600.0) * ue;
+    const double ck = p.stress[k];
+    if (ck != 0) {
+      const double r = ck / (he > p.floor_ ? he : p.floor_);
+      d = d - ((r * dts > 1e-8) ? (1 - exp(-r * dts)) / dts : r) * ue;
+    }
+    if (th) {
+      const double* tl = th + (size_t)k * N;
+      const double* pl = pibar + (size_t)k * N;
+      d = d + DIVBY(0.5 * (pl[eI] + pl[eJ]) * (tl[eJ] - tl[eI]), dv.de[e]);
+    }
+    if (p.nu4 != 0) d = d - p.nu4 * lap4[o];
+    T[k] = d;
+    hev[k] = he;
+    H += he;
+    U += he * ue;
+  }
+  if (th && p.shelf > 0) {
+    double Tn[ISW_LMAX];
+    for (int k = 0; k < L; k++) {
+      Tn[k] = T[k];
+      if (sqk[k] >= 1) continue;
+      double w = 0, s = 0;
+      for (int j = k - 1; j >= 0; j--) if (sqk[j] >= 1) { w += hev[j]; s += hev[j] * T[j]; break; }
+      for (int j = k + 1; j < L; j++) if (sqk[j] >= 1) { w += hev[j]; s += hev[j] * T[j]; break; }
+      if (w > 0) Tn[k] = sqk[k] * T[k] + (1 - sqk[k]) * s / w;
+    }
+    for (int k = 0; k < L; k++) T[k] = Tn[k];
+  }
+  double GT = 0;
+  for (int k = 0; k < L; k++) GT += hev[k] * T[k];
+  const double gg = GT / H;
+  for (int k = 0; k < L; k++) Tp[(size_t)k * E + e] = T[k] - gg;
+  G[e] = gg;
+  He[e] = H;
+  ubs[e] = U / H;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Air fast sub-steps (layers.cu:182-224): mass then wind, forward-backward, from the stage's start.
+// The first sub-step reads the step's start (mu from h0, u from u0) and starts acc; the last writes the stage's wind.
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_af_muphi(isw_grid g, LP p, Div dv, double tau, const double* __restrict__ hedge,
+                                                 const double* __restrict__ uf, const double* mu_src,
+                                                 double* mu, const double* __restrict__ S,
+                                                 double* __restrict__ phi) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int N = g.N, E = g.E, L = p.L;
+  int ce[6];
+  double cs[6];
+  int deg = 6;
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    ce[m] = g.cellEdge[i * 6 + m];
+    cs[m] = (double)g.cellSign[i * 6 + m];
+  }
+  if (ce[5] < 0) deg = 5;
+  double P[ISW_LMAX];
+  double a = 0;
+  const double rA = dv.A[i];
+  for (int k = 0; k < L; k++) {
+    const double* hl = hedge + (size_t)k * E;
+    const double* ul = uf + (size_t)k * E;
+    const size_t o = (size_t)k * N + i;
+    const double m0 = mu_src[o];
+    double div = 0;
+    for (int m = 0; m < deg; m++) {
+      const int e = ce[m];
+      div += cs[m] * g.le[e] * hl[e] * ul[e];
+    }
+    const double m1 = m0 - DIVBY(tau * div, rA);
+    mu[o] = m1;
+    a += m1 + p.div_damp * (m1 - m0);
+    P[k] = a;
+  }
+  double ph = 0;
+  for (int k = L - 1; k >= 0; k--) { ph += S[(size_t)k * N + i] * P[k]; phi[(size_t)k * N + i] = ph; }
+}
+
+__global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, double tau, int first,
+                                             const double* __restrict__ phi, const double* __restrict__ Tp,
+                                             const double* __restrict__ G, const double* uin, double* acc,
+                                             double* uout) {
+  const int t = TID;
+  if (t >= g.E * L) return;
+  const int l = t / g.E, e = t - l * g.E;
+  const double v = uin[t];
+  acc[t] = first ? 0.0 + v : acc[t] + v;
+  const double* pl = phi + (size_t)l * g.N;
+  uout[t] = v + tau * (DIVBY(-(pl[g.eJ[e]] - pl[g.eI[e]]), dv.de[e]) + Tp[t] + G[e]);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Ocean barotropic sub-steps (layers.cu:366-388) with the wall mask of the reference's lay_mask calls fused in.
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_bt_eta(isw_grid g, Div dv, double tau, const double* __restrict__ He,
+                                               const double* __restrict__ ub, const double* eta_src, double* eta,
+                                               double* __restrict__ eta_prev) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const double e0 = eta_src[i];
+  eta_prev[i] = e0;
+  double div = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int e = g.cellEdge[i * 6 + m];
+    if (e < 0) break;
+    div += (double)g.cellSign[i * 6 + m] * g.le[e] * He[e] * ub[e];
+  }
+  eta[i] = e0 - DIVBY(tau * div, dv.A[i]);
+}
+
+__global__ __launch_bounds__(BS) void k_bt_u(isw_grid g, Div dv, double tau, double gamma, int first,
+                                             const double* __restrict__ eta, const double* __restrict__ eta_prev,
+                                             const double* __restrict__ G, const double* __restrict__ Gc,
+                                             const double* __restrict__ wall, const double* ub_src, double* ub,
+                                             double* acc) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const double v = ub_src[e];
+  double a = first ? 0.0 + v : acc[e] + v;
+  const int i = g.eI[e], j = g.eJ[e];
+  const double ej = Gc[j] * (eta[j] + gamma * (eta[j] - eta_prev[j]));
+  const double ei = Gc[i] * (eta[i] + gamma * (eta[i] - eta_prev[i]));
+  double un = v + tau * (DIVBY(-(ej - ei), dv.de[e]) + G[e]);
+  if (wall && wall[e] == 0) { un = 0; a = 0; }
+  ub[e] = un;
+  acc[e] = a;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The stage's layer fluxes, limiter and update (layers.cu:220-224,392-447). F is recomputed where it is read:
+// air F = h_e acc / n (af_flux); ocean F = h_e (u - ubar*) + h_e ubar_avg (lay_flux), 0 on walls.
+// ---------------------------------------------------------------------------------------------------------------
+struct FluxSrc {
+  int air;
+  double inv_n, shelf;
+  const double *hedge, *acc;                  // air
+  const double *hs, *us, *ubs, *bacc, *wall;  // ocean
+};
+
+__device__ __forceinline__ double stage_flux(const FluxSrc& f, const isw_grid& g, int l, int e) {
+  const size_t o = (size_t)l * g.E + e;
+  if (f.air) return f.hedge[o] * f.acc[o] * f.inv_n;
+  if (f.wall && f.wall[e] == 0) return 0.0;
+  const double he = edge_h(f.hs[(size_t)l * g.N + g.eI[e]], f.hs[(size_t)l * g.N + g.eJ[e]], f.shelf);
+  return he * (f.us[o] - f.ubs[e]) + he * (f.bacc[e] * f.inv_n);
+}
+
+__global__ __launch_bounds__(BS) void k_scale(isw_grid g, FluxSrc fs, double dts, double floor_,
+                                              const double* __restrict__ h0, double* __restrict__ scale, int L) {
+  const int t = TID;
+  if (t >= g.N * L) return;
+  const int l = t / g.N, i = t - l * g.N;
+  double out = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int e = g.cellEdgeAsc[i * 6 + m];
+    if (e < 0) break;
+    const double f = stage_flux(fs, g, l, e) * g.le[e] * dts;
+    if (g.eI[e] == i) { if (f > 0) out += f; }
+    else if (!(f > 0)) out -= f;
+  }
+  const double avail = h0[t] - floor_;
+  const double cap = (avail > 0 ? avail : 0.0) * g.areaCell[i];
+  scale[t] = out > cap ? cap / out : 1.0;
+}
+
+// lay_limit + lay_h: the limited fluxes summed into the new thickness; each edge's eI cell writes its Fm.
+__global__ __launch_bounds__(BS) void k_limit_h(isw_grid g, Div dv, FluxSrc fs, double cdt, double dt,
+                                                const double* __restrict__ scale, const double* h0, double* hn,
+                                                double* __restrict__ Fm, int L) {
+  const int t = TID;
+  if (t >= g.N * L) return;
+  const int l = t / g.N, i = t - l * g.N;
+  const double* sl = scale + (size_t)l * g.N;
+  double div = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int e = g.cellEdge[i * 6 + m];
+    if (e < 0) break;
+    double f = stage_flux(fs, g, l, e);
+    const int eI = g.eI[e];
+    f *= f > 0 ? sl[eI] : sl[g.eJ[e]];
+    div += (double)g.cellSign[i * 6 + m] * g.le[e] * f;
+    if (Fm && eI == i) Fm[(size_t)l * g.E + e] = f * g.le[e] * dt;
+  }
+  const double v0 = h0[t];
+  hn[t] = v0 - DIVBY(cdt * div, dv.A[i]);
+}
+
+__global__ __launch_bounds__(BS) void k_lay_u(int E, int L, double cdt, const double* u0, const double* __restrict__ Tp,
+                                              const double* __restrict__ ub_end, const double* __restrict__ ub0,
+                                              const double* __restrict__ wall, double* un) {
+  const int t = TID;
+  if (t >= E * L) return;
+  const int e = t % E;
+  const double v = u0[t] + cdt * Tp[t] + (ub_end[e] - ub0[e]);
+  un[t] = wall && wall[e] == 0 ? 0.0 : v;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Interfacial stress, its heat, and the exchange between layers (layers.cu:454-621).
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_lay_vmix(isw_grid g, int L, double c, double dt, double cd,
+                                                 const double* __restrict__ h, const double* __restrict__ wall,
+                                                 double* __restrict__ u, double* __restrict__ diss) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int N = g.N, E = g.E, eI = g.eI[e], eJ = g.eJ[e];
+  double d = 0;
+  for (int k = 0; k + 1 < L; k++) {
+    const double ha = 0.5 * (h[(size_t)k * N + eI] + h[(size_t)k * N + eJ]);
+    const double hb = 0.5 * (h[(size_t)(k + 1) * N + eI] + h[(size_t)(k + 1) * N + eJ]);
+    double* ua = u + (size_t)k * E + e;
+    double* ub = u + (size_t)(k + 1) * E + e;
+    const double mean = (ha * *ua + hb * *ub) / (ha + hb);
+    const double du0 = *ua - *ub;
+    double du1 = du0 * exp(-c * (1 / ha + 1 / hb) * dt);
+    if (cd > 0) du1 = du1 / (1 + cd * fabs(du0) * (1 / ha + 1 / hb) * dt);
+    *ua = mean + hb / (ha + hb) * du1;
+    *ub = mean - ha / (ha + hb) * du1;
+    d += 0.5 * ha * hb / (ha + hb) * (du0 * du0 - du1 * du1);
+  }
+  diss[e] = d;
+  if (wall && wall[e] == 0)
+    for (int k = 0; k < L; k++) u[(size_t)k * E + e] = 0;
+}
+
+__global__ __launch_bounds__(BS) void k_vmix_heat(isw_grid g, LP p, const double* __restrict__ h,
+                                                  const double* __restrict__ diss, double* __restrict__ th) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int N = g.N, L = p.L;
+  double q = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int e = g.cellEdge[i * 6 + m];
+    if (e < 0) break;
+    q += 0.5 * g.le[e] * g.de[e] * diss[e];
+  }
+  if (q == 0) return;
+  q /= g.areaCell[i];
+  double mu = 0;
+  for (int k = 0; k < L; k++) mu += h[(size_t)k * N + i];
+  double pr = 0, xt = 0;
+  for (int k = 0; k < L; k++) {
+    const double pt = pr;
+    pr = pr + p.grav * h[(size_t)k * N + i];
+    const double x = pow(pr / p.p0, p.kappa);
+    const double pib = p.cp * (pr * x - pt * xt) / ((p.kappa + 1) * (pr - pt));
+    th[(size_t)k * N + i] += q / (mu * pib);
+    xt = x;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_xch_cell(int N, int L, int seq, const double* __restrict__ wx,
+                                                 const double* __restrict__ areaCell, double* __restrict__ h,
+                                                 double* __restrict__ M, double* __restrict__ psi) {
+  const int i = TID;
+  if (i >= N) return;
+  double tr[ISW_LMAX], dm[ISW_LMAX];
+  for (int k = 0; k < L; k++) { tr[k] = 0; dm[k] = 0; }
+  if (seq) {
+    double m[ISW_LMAX], t[ISW_LMAX];
+    for (int k = 0; k < L; k++) { const size_t o = (size_t)k * N + i; m[k] = M[o]; t[k] = M[o] * psi[o]; }
+    for (int pass = 0; pass < 2; pass++)
+      for (int n = 0; n + 1 < L; n++) {
+        const int k = pass == 0 ? L - 2 - n : n;
+        const double v = wx[(size_t)k * N + i];
+        if (pass == 0 ? !(v > 0) : !(v < 0)) continue;
+        const int d = v > 0 ? k + 1 : k;
+        const double vol = v * areaCell[i], pp = t[d] / m[d];
+        m[k] += vol; m[k + 1] -= vol; t[k] += vol * pp; t[k + 1] -= vol * pp;
+      }
+    for (int k = 0; k < L; k++) {
+      const size_t o = (size_t)k * N + i;
+      h[o] = h[o] + (m[k] - M[o]) / areaCell[i];
+      M[o] = m[k];
+      psi[o] = t[k] / m[k];
+    }
+    return;
+  }
+  for (int k = 0; k + 1 < L; k++) {
+    const double v = wx[(size_t)k * N + i];
+    const double pp = v > 0 ? psi[(size_t)(k + 1) * N + i] : psi[(size_t)k * N + i];
+    const double vol = v * areaCell[i];
+    dm[k] += vol; dm[k + 1] -= vol;
+    tr[k] += vol * pp; tr[k + 1] -= vol * pp;
+  }
+  for (int k = 0; k < L; k++) {
+    const size_t o = (size_t)k * N + i;
+    const double m1 = M[o] + dm[k];
+    psi[o] = (M[o] * psi[o] + tr[k]) / m1;
+    M[o] = m1;
+    h[o] = h[o] + dm[k] / areaCell[i];
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_xch_edge(isw_grid g, int L, const double* __restrict__ wx,
+                                                 const double* __restrict__ hold, double* __restrict__ u,
+                                                 double* __restrict__ diss, const double* __restrict__ lap4,
+                                                 double nu4dt, double hthin, double floor_,
+                                                 const double* __restrict__ wall) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int N = g.N, E = g.E, eI = g.eI[e], eJ = g.eJ[e];
+  double mom[ISW_LMAX], hn[ISW_LMAX];
+  double k0 = 0, v4 = 0;
+  for (int k = 0; k < L; k++) {
+    const double he = 0.5 * (hold[(size_t)k * N + eI] + hold[(size_t)k * N + eJ]);
+    const double uk = u[(size_t)k * E + e];
+    mom[k] = he * uk;
+    hn[k] = he;
+    k0 += he * uk * uk;
+    if (lap4) v4 += he * uk * lap4[(size_t)k * E + e];
+  }
+  for (int pass = 0; pass < 2; pass++)
+    for (int n = 0; n + 1 < L; n++) {
+      const int k = pass == 0 ? L - 2 - n : n;
+      const double v = 0.5 * (wx[(size_t)k * N + eI] + wx[(size_t)k * N + eJ]);
+      if (pass == 0 ? !(v > 0) : !(v < 0)) continue;
+      const int d = v > 0 ? k + 1 : k;
+      const double ud = hn[d] > floor_ ? mom[d] / hn[d] : u[(size_t)d * E + e];
+      mom[k] += v * ud; mom[k + 1] -= v * ud;
+      hn[k] += v; hn[k + 1] -= v;
+    }
+  double un[ISW_LMAX];
+  for (int k = 0; k < L; k++) un[k] = mom[k] / hn[k];
+  if (hthin > 0)
+    for (int k = 0; k < L; k++) {
+      const double hm = fmin(hold[(size_t)k * N + eI], hold[(size_t)k * N + eJ]);
+      if (hm >= hthin) continue;
+      double w = (hthin - hm) / (hthin - floor_);
+      if (w > 1) w = 1;
+      double hs = 0, ms = 0;
+      if (k > 0) { hs += hn[k - 1]; ms += hn[k - 1] * un[k - 1]; }
+      if (k + 1 < L) { hs += hn[k + 1]; ms += hn[k + 1] * un[k + 1]; }
+      if (hs <= 0) continue;
+      w = fmin(w, hs / (hn[k] + hs));
+      const double du = w * (ms / hs - un[k]);
+      un[k] += du;
+      const double dn = hn[k] * du / hs;
+      if (k > 0) un[k - 1] -= dn;
+      if (k + 1 < L) un[k + 1] -= dn;
+    }
+  double k1 = 0;
+  const bool walled = wall && wall[e] == 0;
+  for (int k = 0; k < L; k++) {
+    u[(size_t)k * E + e] = walled ? 0.0 : un[k];
+    k1 += hn[k] * un[k] * un[k];
+  }
+  if (diss) diss[e] = 0.5 * (k0 - k1) + nu4dt * v4;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Transport (fused.cu:124-241; trisk.cu:48-59). Rows y = tracer * Lm + layer; a row's masses and fluxes are layer
+// y % Lm's. blockIdx.y is the row.
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_mass_update(isw_grid g, const double* __restrict__ M0,
+                                                    const double* __restrict__ Fm, double* __restrict__ M1) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int l = blockIdx.y;
+  const size_t o = (size_t)l * g.N + i;
+  const double* f = Fm + (size_t)l * g.E;
+  double m = M0[o];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int e = g.cellEdgeAsc[i * 6 + k];
+    if (e < 0) break;
+    if (g.eI[e] == i) m -= f[e]; else m += f[e];
+  }
+  M1[o] = m;
+}
+
+__device__ __forceinline__ double upwind(const double* Fml, const double* p, const isw_grid& g, int e) {
+  const double f = Fml[e];
+  return f * (f >= 0 ? p[g.eI[e]] : p[g.eJ[e]]);
+}
+
+// 3rd-order flux with upwind weight beta (fused.cu:131-138): f (mean - a - b) + |f| (b - a) beta; the curvature
+// stencil on `psf` (the field itself in the plain form, its FP32 copy in the mixed form) with weights wt.
+__device__ __forceinline__ double hoflux(const double* Fml, const double* ps, const fast* psf, const fast* wt,
+                                         const isw_grid& g, int e, double beta) {
+  const double f = Fml[e];
+  const int o = e * 8;
+  const int* idx = g.d2idx;
+  const fast a = wt[o] * psf[idx[o]] + wt[o + 1] * psf[idx[o + 1]] + wt[o + 2] * psf[idx[o + 2]] +
+                 wt[o + 3] * psf[idx[o + 3]];
+  const fast b = wt[o + 4] * psf[idx[o + 4]] + wt[o + 5] * psf[idx[o + 5]] + wt[o + 6] * psf[idx[o + 6]] +
+                 wt[o + 7] * psf[idx[o + 7]];
+  return f * (0.5 * (ps[g.eI[e]] + ps[g.eJ[e]]) - (double)(a + b)) + fabs(f) * ((double)(b - a) * beta);
+}
+
+__global__ __launch_bounds__(BS) void k_tr_td(isw_grid g, int Lm, const double* __restrict__ M0,
+                                              const double* __restrict__ psi, const double* __restrict__ Fm,
+                                              const double* __restrict__ M1, double* __restrict__ td,
+                                              float* __restrict__ pf) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int y = blockIdx.y, l = y % Lm;
+  const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
+  const double* p = psi + (size_t)y * g.N;
+  const double* Fml = Fm + (size_t)l * g.E;
+  double t = M0[om] * psi[o];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int e = g.cellEdgeAsc[i * 6 + k];
+    if (e < 0) break;
+    const double f = upwind(Fml, p, g, e);
+    if (g.eI[e] == i) t -= f; else t += f;
+  }
+  td[o] = t / M1[om];
+  if (pf) pf[o] = (float)psi[o];
+}
+
+__global__ __launch_bounds__(BS) void k_tr_hflux(isw_grid g, int Lm, double frac, double beta,
+                                                 const double* __restrict__ src, const fast* __restrict__ srcf,
+                                                 const fast* __restrict__ wt, const double* __restrict__ Fm,
+                                                 double* __restrict__ Hf) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int y = blockIdx.y, l = y % Lm;
+  Hf[(size_t)y * g.E + e] =
+      frac * hoflux(Fm + (size_t)l * g.E, src + (size_t)y * g.N, srcf + (size_t)y * g.N, wt, g, e, beta);
+}
+
+__global__ __launch_bounds__(BS) void k_tr_stage(isw_grid g, int Lm, double frac, const double* __restrict__ M0,
+                                                 const double* __restrict__ psi, const double* __restrict__ Hf,
+                                                 const double* __restrict__ Fm, double* __restrict__ out,
+                                                 float* __restrict__ pf) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int y = blockIdx.y, l = y % Lm;
+  const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
+  const double* Hl = Hf + (size_t)y * g.E;
+  const double* Fml = Fm + (size_t)l * g.E;
+  double t = M0[om] * psi[o];
+  double mx = M0[om];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int e = g.cellEdgeAsc[i * 6 + k];
+    if (e < 0) break;
+    const double f = Hl[e];
+    const double m = frac * Fml[e];
+    if (g.eI[e] == i) { t -= f; mx -= m; } else { t += f; mx += m; }
+  }
+  const double r = t / mx;
+  out[o] = r;
+  if (pf) pf[o] = (float)r;
+}
+
+__global__ __launch_bounds__(BS) void k_tr_aflux(isw_grid g, int Lm, double beta, const double* __restrict__ psi,
+                                                 const double* __restrict__ p2, const fast* __restrict__ p2f,
+                                                 const fast* __restrict__ wt, const double* __restrict__ Fm,
+                                                 double* __restrict__ A) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int y = blockIdx.y, l = y % Lm;
+  const double* Fml = Fm + (size_t)l * g.E;
+  A[(size_t)y * g.E + e] = hoflux(Fml, p2 + (size_t)y * g.N, p2f + (size_t)y * g.N, wt, g, e, beta) -
+                           upwind(Fml, psi + (size_t)y * g.N, g, e);
+}
+
+__global__ __launch_bounds__(BS) void k_tr_ratios(isw_grid g, int Lm, const double* __restrict__ psi,
+                                                  const double* __restrict__ td, const double* __restrict__ M1,
+                                                  const double* __restrict__ A, double* __restrict__ Rp,
+                                                  double* __restrict__ Rm) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int y = blockIdx.y, l = y % Lm;
+  const double* p = psi + (size_t)y * g.N;
+  const double* t = td + (size_t)y * g.N;
+  const double* Al = A + (size_t)y * g.E;
+  double hi = fmax(p[i], t[i]), lo = fmin(p[i], t[i]);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int j = g.nbr[i * 6 + k];
+    if (j < 0) break;
+    hi = fmax(hi, fmax(p[j], t[j]));
+    lo = fmin(lo, fmin(p[j], t[j]));
+  }
+  const double m1 = M1[(size_t)l * g.N + i];
+  const double qmax = (hi - t[i]) * m1, qmin = (t[i] - lo) * m1;
+  double Pp = 0, Pm = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int e = g.cellEdgeAsc[i * 6 + k];
+    if (e < 0) break;
+    const double a = Al[e];
+    if (g.eI[e] == i) { if (a > 0) Pm += a; else Pp -= a; }
+    else { if (a > 0) Pp += a; else Pm -= a; }
+  }
+  Rp[(size_t)y * g.N + i] = Pp > 0 ? fmin(1.0, qmax / Pp) : 0.0;
+  Rm[(size_t)y * g.N + i] = Pm > 0 ? fmin(1.0, qmin / Pm) : 0.0;
+}
+
+__global__ __launch_bounds__(BS) void k_tr_apply(isw_grid g, int Lm, const double* __restrict__ td,
+                                                 const double* __restrict__ M1, const double* __restrict__ A,
+                                                 const double* __restrict__ Rp, const double* __restrict__ Rm,
+                                                 double* __restrict__ out) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int y = blockIdx.y, l = y % Lm;
+  const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
+  const double* Al = A + (size_t)y * g.E;
+  const double* rp = Rp + (size_t)y * g.N;
+  const double* rm = Rm + (size_t)y * g.N;
+  double t = td[o] * M1[om];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int e = g.cellEdgeAsc[i * 6 + k];
+    if (e < 0) break;
+    const double a = Al[e];
+    const double c = a > 0 ? fmin(rp[g.eJ[e]], rm[g.eI[e]]) : fmin(rp[g.eI[e]], rm[g.eJ[e]]);
+    if (g.eI[e] == i) t -= c * a; else t += c * a;
+  }
+  out[o] = t / M1[om];
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// SWE (fused.cu:27-120): five passes per stage.
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_swe_scale(isw_grid g, double dts, const double* __restrict__ h,
+                                                  const double* __restrict__ u, double* __restrict__ scale) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int l = blockIdx.y;
+  const double* hl = h + (size_t)l * g.N;
+  const double* ul = u + (size_t)l * g.E;
+  double out = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int e = g.cellEdgeAsc[i * 6 + k];
+    if (e < 0) break;
+    const double f = 0.5 * (hl[g.eI[e]] + hl[g.eJ[e]]) * ul[e] * g.le[e] * dts;
+    if (g.eI[e] == i) { if (f > 0) out += f; }
+    else if (!(f > 0)) out -= f;
+  }
+  const double hv = hl[i];
+  const double cap = 0.5 * (hv > 0 ? hv : 0.0) * g.areaCell[i];
+  scale[(size_t)l * g.N + i] = out > cap ? cap / out : 1.0;
+}
+
+__global__ __launch_bounds__(BS) void k_swe_flux(isw_grid g, double dt, const double* __restrict__ h,
+                                                 const double* __restrict__ u, const double* __restrict__ scale,
+                                                 double* __restrict__ F, double* __restrict__ Fm) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int l = blockIdx.y;
+  const size_t o = (size_t)l * g.E + e;
+  const double* hl = h + (size_t)l * g.N;
+  double f = 0.5 * (hl[g.eI[e]] + hl[g.eJ[e]]) * u[o];
+  f *= f > 0 ? scale[(size_t)l * g.N + g.eI[e]] : scale[(size_t)l * g.N + g.eJ[e]];
+  F[o] = f;
+  if (Fm) Fm[o] = f * g.le[e] * dt;
+}
+
+__global__ __launch_bounds__(BS) void k_swe_cell(isw_grid g, double grav, double c, const double* __restrict__ h,
+                                                 const double* __restrict__ b, const double* __restrict__ F,
+                                                 const double* __restrict__ u, const double* h0, double* hn,
+                                                 double* __restrict__ B) {
+  const int i = TID;
+  if (i >= g.N) return;
+  const int l = blockIdx.y;
+  const double* Fl = F + (size_t)l * g.E;
+  const double* ul = u + (size_t)l * g.E;
+  double div = 0, ke = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int e = g.cellEdge[i * 6 + k];
+    if (e < 0) break;
+    div += (double)g.cellSign[i * 6 + k] * g.le[e] * Fl[e];
+    ke += g.le[e] * g.de[e] * 0.25 * ul[e] * ul[e];
+  }
+  const size_t o = (size_t)l * g.N + i;
+  const double dh = 0.0 - div / g.areaCell[i];
+  B[o] = grav * (h[o] + b[i]) + 0.0 + ke / g.areaCell[i];
+  hn[o] = h0[o] + c * dh;
+}
+
+__global__ __launch_bounds__(BS) void k_swe_vertex(isw_grid g, Div dv, const double* __restrict__ h,
+                                                   const double* __restrict__ u, const double* __restrict__ fV,
+                                                   double* __restrict__ q) {
+  const int v = TID;
+  if (v >= g.NV) return;
+  const int l = blockIdx.y;
+  const double* hl = h + (size_t)l * g.N;
+  const double* ul = u + (size_t)l * g.E;
+  double hv = 0, z = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int c = g.vCell[v * 3 + k];
+    hv += g.kite[c * 6 + g.vSlot[v * 3 + k]] * hl[c] * 1.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int e = g.vEdge[v * 3 + k];
+    z += (double)g.vEdgeSign[v * 3 + k] * g.de[e] * ul[e];
+  }
+  q[(size_t)l * g.NV + v] = (DIVBY(z, dv.AV[v]) + fV[v]) / DIVBY(hv, dv.AV[v]);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Air-sea stress, the land column, the mixed layer's capacity (ocean.cu:17-42,434-471).
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_couple_stress(isw_grid g, int La, double c0, double rho_w, double dt,
+                                                      double land_drag0, const double* __restrict__ ha,
+                                                      double* __restrict__ ua, const double* __restrict__ ho,
+                                                      double* __restrict__ uo, double* __restrict__ diss,
+                                                      double* __restrict__ tau_acc, const double* __restrict__ wall,
+                                                      const double* __restrict__ cd) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int N = g.N, E = g.E, b = La - 1, eI = g.eI[e], eJ = g.eJ[e];
+  double c = c0, land_drag = land_drag0;
+  if (cd) { c = 0.5 * (cd[eI] + cd[eJ]); land_drag = 1; }
+  const double ma = 0.5 * (ha[(size_t)b * N + eI] + ha[(size_t)b * N + eJ]);
+  const size_t oa = (size_t)b * E + e;
+  const double a0 = ua[oa];
+  double a1;
+  if (wall && wall[e] == 0) {
+    const double du1 = a0 * exp(-land_drag * c / ma * dt);
+    a1 = du1;
+    diss[e] = 0.5 * ma * (a0 * a0 - du1 * du1);
+  } else {
+    const double o0 = uo[e];
+    const double mo = rho_w * 0.5 * (ho[eI] + ho[eJ]);
+    const double mean = (ma * a0 + mo * o0) / (ma + mo);
+    const double du0 = a0 - o0;
+    const double du1 = du0 * exp(-c * (1 / ma + 1 / mo) * dt);
+    a1 = mean + mo / (ma + mo) * du1;
+    uo[e] = mean - ma / (ma + mo) * du1;
+    diss[e] = 0.5 * ma * mo / (ma + mo) * (du0 * du0 - du1 * du1);
+  }
+  ua[oa] = a1;
+  const double tau = ma * (a0 - a1) / dt;
+  tau_acc[e] += tau * tau * dt;
+}
+
+__global__ __launch_bounds__(BS) void k_land_column(int N, int K, double c_soil, double k_soil, double dt,
+                                                    const double* __restrict__ land, double* __restrict__ T,
+                                                    const double* __restrict__ cap, double* __restrict__ Tg,
+                                                    const double* __restrict__ dz) {
+  const int i = TID;
+  if (i >= N) return;
+  if (!(land[i] > 0.5)) return;
+  const int n = K + 1;
+  double C[ISW_KSOIL + 1], G[ISW_KSOIL + 1], x[ISW_KSOIL + 1], cp_[ISW_KSOIL + 1], dp[ISW_KSOIL + 1];
+  C[0] = cap[i];
+  x[0] = T[i];
+  double half = cap[i] / c_soil / 2;
+  for (int k = 1; k < n; k++) {
+    const double d = dz[k - 1];
+    C[k] = c_soil * d;
+    x[k] = Tg[(size_t)(k - 1) * N + i];
+    G[k - 1] = dt * k_soil / (half + d / 2);
+    half = d / 2;
+  }
+  G[n - 1] = 0;
+  for (int k = 0; k < n; k++) {
+    const double a = k ? G[k - 1] : 0.0, c = G[k];
+    const double den = C[k] + a + c - (k ? a * cp_[k - 1] : 0.0);
+    cp_[k] = c / den;
+    dp[k] = (C[k] * x[k] + (k ? a * dp[k - 1] : 0.0)) / den;
+  }
+  x[n - 1] = dp[n - 1];
+  for (int k = n - 2; k >= 0; k--) x[k] = dp[k] + cp_[k] * x[k + 1];
+  T[i] = x[0];
+  for (int k = 1; k < n; k++) Tg[(size_t)(k - 1) * N + i] = x[k];
+}
+
+__global__ __launch_bounds__(BS) void k_ocn_cap(int N, double rho_cw, double c_land, const double* __restrict__ h,
+                                                double* __restrict__ cap, const double* __restrict__ land) {
+  const int i = TID;
+  if (i >= N) return;
+  cap[i] = land && land[i] > 0.5 ? (cap[i] > 0 ? cap[i] : c_land) : rho_cw * h[i];
+}
+
+// ===============================================================================================================
+// Host sequencing (the reference's Python drivers), all on one stream.
+// ===============================================================================================================
+
+// Layers._stage (layers.py:255-309).
+static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div& dv, const double* hs,
+                  const double* us, const double* h0, const double* u0, double c, double dt, int nfast, double* hn,
+                  double* un, double* Fm, const double* theta, hipStream_t st) {
+  const int N = g->N, E = g->E, L = s->L;
+  const int nbc = (N + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
+  k_bern_pv<<<dim3(nbc + nbv), BS, 0, st>>>(*g, p, dv, nbc, hs, us, theta, s->b, s->S, s->fV, s->B, s->pibar, s->q);
+#if ISW_FORM_MIXED
+  fast* pk = s->pkf;
+  const fast* wf = g->wf;
+#else
+  fast* pk = s->pk;
+  const fast* wf = g->w;
+#endif
+  k_edge_hq<<<g1((long)E * L), BS, 0, st>>>(*g, L, s->shelf, hs, s->q, us, s->hedge, pk);
+  k_slow<<<g1(E), BS, 0, st>>>(*g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
+                               s->air ? theta : nullptr, s->air ? s->pibar : nullptr, s->Tp, s->G, s->He, s->ubs);
+  int n = (int)std::ceil(c * nfast);
+  if (n < 1) n = 1;
+  const double tau = c * dt / n;
+  FluxSrc fs{};
+  fs.inv_n = 1.0 / n;
+  fs.shelf = s->shelf;
+  if (s->air) {
+    for (int it = 0; it < n; it++) {
+      const bool first = it == 0, last = it == n - 1;
+      k_af_muphi<<<g1(N), BS, 0, st>>>(*g, p, dv, tau, s->hedge, first ? u0 : s->uf, first ? h0 : s->muf, s->muf,
+                                       s->S, s->phi);
+      k_af_u<<<g1((long)E * L), BS, 0, st>>>(*g, L, dv, tau, first ? 1 : 0, s->phi, s->Tp, s->G, first ? u0 : s->uf,
+                                             s->accL, last ? un : s->uf);
+    }
+    fs.air = 1;
+    fs.hedge = s->hedge;
+    fs.acc = s->accL;
+  } else {
+    for (int it = 0; it < n; it++) {
+      const bool first = it == 0;
+      k_bt_eta<<<g1(N), BS, 0, st>>>(*g, dv, tau, s->He, first ? s->ub0 : s->ub, first ? s->eta0 : s->eta, s->eta,
+                                     s->eta_prev);
+      k_bt_u<<<g1(E), BS, 0, st>>>(*g, dv, tau, s->div_damp, first ? 1 : 0, s->eta, s->eta_prev, s->G, s->Gc,
+                                   s->wall, first ? s->ub0 : s->ub, s->ub, s->acc);
+    }
+    fs.air = 0;
+    fs.hs = hs;
+    fs.us = us;
+    fs.ubs = s->ubs;
+    fs.bacc = s->acc;
+    fs.wall = s->wall;
+  }
+  k_scale<<<g1((long)N * L), BS, 0, st>>>(*g, fs, c * dt, s->floor_, h0, s->scale, L);
+  k_limit_h<<<g1((long)N * L), BS, 0, st>>>(*g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L);
+  if (!s->air) k_lay_u<<<g1((long)E * L), BS, 0, st>>>(E, L, c * dt, u0, s->Tp, s->ub, s->ub0, s->wall, un);
+}
+
+int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, double* theta, double* Fm, double dt,
+                int nfast, hipStream_t st) {
+  const LP p = make_lp(s);
+  const Div dv = divs(g);
+  const int N = g->N, L = s->L;
+  if (s->air) {
+    k_air_stiff<<<g1(N), BS, 0, st>>>(N, p, h, theta, s->S);
+  } else {
+    k_lay_eta<<<g1(N), BS, 0, st>>>(N, L, h, s->b, s->eta0);
+    k_lay_ubar<<<g1(g->E), BS, 0, st>>>(*g, L, s->shelf, h, u, s->wall, s->ub0);
+  }
+  if (s->nu4 != 0) {
+    lap(g, L, u, s->D1, s->Z1, s->L1, st);
+    lap(g, L, s->L1, s->D1, s->Z1, s->L2, st);
+  }
+  stage(g, s, p, dv, h, u, h, u, 1.0 / 3.0, dt, nfast, s->h1, s->u1, nullptr, theta, st);
+  stage(g, s, p, dv, s->h1, s->u1, h, u, 1.0 / 2.0, dt, nfast, s->h2, s->u2, nullptr, theta, st);
+  stage(g, s, p, dv, s->h2, s->u2, h, u, 1.0, dt, nfast, h, u, Fm, theta, st);
+  if ((s->vmix != 0 || s->vmix_cd != 0) && L > 1) {
+    k_lay_vmix<<<g1(g->E), BS, 0, st>>>(*g, L, s->vmix, dt, s->vmix_cd, h, s->wall, u, s->diss);
+    if (s->air) k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, p, h, s->diss, theta);
+  }
+  return (int)hipGetLastError();
+}
+
+int layers_lap(const isw_grid* g, const isw_layers* s, const double* u, double* out, hipStream_t st) {
+  lap(g, s->L, u, s->D1, s->Z1, out, st);
+  return (int)hipGetLastError();
+}
+
+int layers_gfast(const isw_grid* g, const isw_layers* s, const double* h, const double* theta, hipStream_t st) {
+  if (s->air) k_air_gfast<<<g1(g->N), BS, 0, st>>>(g->N, make_lp(s), s->rd, h, theta, s->Gc);
+  return (int)hipGetLastError();
+}
+
+int layers_exchange(const isw_grid* g, const isw_layers* s, const double* wx, double* h, double* u, double* M,
+                    double* psi, double* theta, int seq, double nu4dt, hipStream_t st) {
+  const int N = g->N, L = s->L;
+  k_copy<<<g1((long)N * L), BS, 0, st>>>((long)N * L, h, s->hold);
+  const bool heat = s->air && theta;
+  k_xch_cell<<<g1(N), BS, 0, st>>>(N, L, seq, wx, g->areaCell, h, M, psi);
+  const bool v4 = heat && s->nu4 != 0;
+  k_xch_edge<<<g1(g->E), BS, 0, st>>>(*g, L, wx, s->hold, u, heat ? s->diss : nullptr, v4 ? s->L2 : nullptr,
+                                      v4 ? nu4dt : 0.0, heat ? s->thin_wind : 0.0, s->floor_, s->wall);
+  if (heat) k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, make_lp(s), h, s->diss, theta);
+  return (int)hipGetLastError();
+}
+
+int vmix_heat(const isw_grid* g, const isw_layers* s, const double* h, const double* diss, double* theta,
+              hipStream_t st) {
+  k_vmix_heat<<<g1(g->N), BS, 0, st>>>(*g, make_lp(s), h, diss, theta);
+  return (int)hipGetLastError();
+}
+
+int mass_update(const isw_grid* g, int L, const double* M0, const double* Fm, double* M1, hipStream_t st) {
+  k_mass_update<<<g2(g->N, L), BS, 0, st>>>(*g, M0, Fm, M1);
+  return (int)hipGetLastError();
+}
+
+// Transport.step_many (transport.py:40-71) on rows stacked in psi; the result is written back into psi.
+int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0, const double* M1, const double* Fm,
+                   double* psi, int rows, int order, hipStream_t st) {
+  const int N = g->N, E = g->E, Lm = tr->L;
+#if ISW_FORM_MIXED
+  float* pf = tr->pf;
+  const fast* wt = g->d2wtf;
+#else
+  float* pf = nullptr;
+  const fast* wt = g->d2wt;
+#endif
+  k_tr_td<<<g2(N, rows), BS, 0, st>>>(*g, Lm, M0, psi, Fm, M1, tr->td, pf);
+  if (order == 1) {
+    k_copy<<<g1((long)N * rows), BS, 0, st>>>((long)N * rows, tr->td, psi);
+    return (int)hipGetLastError();
+  }
+#if ISW_FORM_MIXED
+  const fast* psf = pf;
+  const fast* p2f = pf;
+#else
+  const fast* psf = psi;
+  const fast* p2f = tr->p2;
+#endif
+  k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf);
+  k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf);
+#if ISW_FORM_MIXED
+  const fast* p1f = pf;
+#else
+  const fast* p1f = tr->p1;
+#endif
+  k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf);
+  k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
+  k_tr_aflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
+  k_tr_ratios<<<g2(N, rows), BS, 0, st>>>(*g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm);
+  k_tr_apply<<<g2(N, rows), BS, 0, st>>>(*g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi);
+  return (int)hipGetLastError();
+}
+
+// SWE.step (swe.py:105-132): the edge pass (swe_edge, fused.cu:101-120) with each layer's drag taken from the layer
+// settings passed by value.
+__global__ __launch_bounds__(BS) void k_swe_edge_lp(isw_grid g, LP p, double c, const double* __restrict__ F,
+                                                    const double* __restrict__ q, const double* __restrict__ B,
+                                                    const double* __restrict__ u, const double* __restrict__ lap4,
+                                                    const double* u0, double* un) {
+  const int e = TID;
+  if (e >= g.E) return;
+  const int l = blockIdx.y;
+  const double* Fl = F + (size_t)l * g.E;
+  const double* ql = q + (size_t)l * g.NV;
+  const double* Bl = B + (size_t)l * g.N;
+  const double qa = 0.5 * (ql[g.ev1[e]] + ql[g.ev2[e]]);
+  double Q = 0;
+  for (int k = g.eoeStart[e]; k < g.eoeStart[e + 1]; k++) {
+    const int e2 = g.eoe[k];
+    const double qb = 0.5 * (ql[g.ev1[e2]] + ql[g.ev2[e2]]);
+    Q += g.w[k] * Fl[e2] * 0.5 * (qa + qb);
+  }
+  const size_t o = (size_t)l * g.E + e;
+  double d = Q - (Bl[g.eJ[e]] - Bl[g.eI[e]]) / g.de[e] - p.drag[l] * u[o];
+  if (p.nu4 != 0) d = d - p.nu4 * lap4[o];
+  un[o] = u0[o] + c * d;
+}
+
+int swe_step(const isw_grid* g, const isw_layers* s, double* h, double* u, double* Fm, double dt, hipStream_t st) {
+  const LP p = make_lp(s);
+  const Div dv = divs(g);
+  const int N = g->N, E = g->E, NV = g->NV, L = s->L;
+  if (s->nu4 != 0) {
+    lap(g, L, u, s->D1, s->Z1, s->L1, st);
+    lap(g, L, s->L1, s->D1, s->Z1, s->L2, st);
+  }
+  const double cs[3] = {dt / 3, dt / 2, dt};
+  const double* hsv[3] = {h, s->h1, s->h2};
+  const double* usv[3] = {u, s->u1, s->u2};
+  double* hnv[3] = {s->h1, s->h2, h};
+  double* unv[3] = {s->u1, s->u2, u};
+  for (int k = 0; k < 3; k++) {
+    const double c = cs[k];
+    k_swe_scale<<<g2(N, L), BS, 0, st>>>(*g, c, hsv[k], usv[k], s->scale);
+    k_swe_flux<<<g2(E, L), BS, 0, st>>>(*g, k == 2 ? dt : 0.0, hsv[k], usv[k], s->scale, s->F, k == 2 ? Fm : nullptr);
+    k_swe_cell<<<g2(N, L), BS, 0, st>>>(*g, s->grav, c, hsv[k], s->b, s->F, usv[k], h, hnv[k], s->B);
+    k_swe_vertex<<<g2(NV, L), BS, 0, st>>>(*g, dv, hsv[k], usv[k], s->fV, s->q);
+    k_swe_edge_lp<<<g2(E, L), BS, 0, st>>>(*g, p, c, s->F, s->q, s->B, usv[k], s->L2, u, unv[k]);
+  }
+  return (int)hipGetLastError();
+}
+
+int couple_stress(const isw_grid* g, int La, const double* ha, double* ua, const double* ho, double* uo, double c,
+                  double rho_w, double dt, double* diss, double* tau_acc, const double* wall, double land_drag,
+                  const double* cd, hipStream_t st) {
+  k_couple_stress<<<g1(g->E), BS, 0, st>>>(*g, La, c, rho_w, dt, land_drag, ha, ua, ho, uo, diss, tau_acc, wall, cd);
+  return (int)hipGetLastError();
+}
+
+int land_column(const isw_grid* g, int K, const double* land, double* T, const double* cap, double* Tg,
+                const double* dz, double c_soil, double k_soil, double dt, hipStream_t st) {
+  k_land_column<<<g1(g->N), BS, 0, st>>>(g->N, K, c_soil, k_soil, dt, land, T, cap, Tg, dz);
+  return (int)hipGetLastError();
+}
+
+int ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* cap, const double* land, double c_land,
+            hipStream_t st) {
+  k_ocn_cap<<<g1(g->N), BS, 0, st>>>(g->N, rho_cw, c_land, h, cap, land);
+  return (int)hipGetLastError();
+}
+
+int fill(double* x, long n, double v, hipStream_t st) {
+  if (n > 0) k_fill<<<g1(n), BS, 0, st>>>(n, v, x);
+  return (int)hipGetLastError();
+}
+
+}  // namespace NS

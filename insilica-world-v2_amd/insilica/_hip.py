@@ -1,0 +1,111 @@
+"""ctypes bindings of libinsilica_hip.so (include/insilica_hip.h).
+
+The library is loaded from insilica/_lib (built in-tree by insilica/build.py). There is no fallback: if the library
+or a GPU is missing, every operation raises.
+"""
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / '_lib' / 'libinsilica_hip.so'
+ISW_FP64, ISW_MIXED, ISW_EBADARG, LMAX, KSOIL = 0, 1, 9001, 8, 16
+
+_i32p, _i8p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int8)
+_f64p, _f32p = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
+
+
+class IswGrid(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int32) for k in ('N', 'E', 'NV', 'n_eoe')]
+                + [(k, _i32p) for k in ('eI', 'eJ', 'nbr', 'cellEdge', 'cellEdgeAsc', 'ev1', 'ev2', 'vEdge', 'vCell',
+                                        'eoeStart', 'eoe', 'd2idx')]
+                + [(k, _i8p) for k in ('cellSign', 'vEdgeSign', 'vSlot')]
+                + [(k, _f64p) for k in ('le', 'de', 'areaCell', 'areaV', 'fV', 'kite', 'w', 'd2wt', 'rA', 'rde', 'rAV')]
+                + [(k, _f32p) for k in ('wf', 'd2wtf')])
+
+
+LAYER_SCRATCH = ('B', 'q', 'Tp', 'F', 'scale', 'G', 'He', 'ubs', 'ub0', 'ub', 'acc', 'eta0', 'eta', 'eta_prev', 'h1',
+                 'h2', 'u1', 'u2', 'D1', 'Z1', 'L1', 'L2', 'hold', 'hedge', 'diss', 'pibar', 'S', 'muf', 'mup', 'phi',
+                 'uf', 'accL', 'pk')
+
+
+class IswLayers(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int32) for k in ('L', 'air', 'prec', 'pad_')]
+                + [(k, ctypes.c_double) for k in ('grav', 'cp', 'kappa', 'p0', 'rd', 'nu4', 'floor_', 'div_damp', 'thin',
+                                                  'shelf', 'vmix', 'vmix_cd', 'thin_wind')]
+                + [(k, ctypes.c_double * LMAX) for k in ('gp', 'drag', 'stress')]
+                + [(k, _f64p) for k in ('fV', 'b', 'wall', 'Gc')]
+                + [(k, _f64p) for k in LAYER_SCRATCH]
+                + [('pkf', _f32p)])
+
+
+class IswTransport(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int32) for k in ('L', 'rows_max', 'prec', 'pad_')]
+                + [('beta', ctypes.c_double)]
+                + [(k, _f64p) for k in ('td', 'p1', 'p2', 'Rp', 'Rm', 'Hf', 'A')]
+                + [('pf', _f32p)])
+
+
+class HipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it is missing: build it with `python __graft_entry__.py` or insilica.build)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HipError(f'{LIB_PATH} is missing: build the HIP library first (python -c "import __graft_entry__ as '
+                           f'g; g.build()")')
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, i32, i64, d = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        sig = {
+            'isw_layers_step': [vp, vp, vp, vp, vp, vp, d, i32, vp],
+            'isw_layers_lap': [vp, vp, vp, vp, vp],
+            'isw_layers_gfast': [vp, vp, vp, vp, vp],
+            'isw_layers_exchange': [vp, vp, vp, vp, vp, vp, vp, vp, i32, d, vp],
+            'isw_vmix_heat': [vp, vp, vp, vp, vp, vp],
+            'isw_mass_update': [vp, i32, vp, vp, vp, vp],
+            'isw_transport_step': [vp, vp, vp, vp, vp, vp, i32, i32, vp],
+            'isw_swe_step': [vp, vp, vp, vp, vp, d, vp],
+            'isw_couple_stress': [vp, i32, vp, vp, vp, vp, d, d, d, vp, vp, vp, d, vp, vp],
+            'isw_land_column': [vp, i32, vp, vp, vp, vp, vp, d, d, d, vp],
+            'isw_ocn_cap': [vp, vp, d, vp, vp, d, vp],
+            'isw_dsum': [vp, vp, i64, vp, vp, vp],
+            'isw_csum': [vp, vp, i64, vp, vp, vp],
+            'isw_fill': [vp, i64, d, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = ctypes.c_int32, args
+        L.isw_version.restype, L.isw_version.argtypes = ctypes.c_char_p, []
+        _lib = L
+    return _lib
+
+
+EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_exchange', 'isw_vmix_heat',
+           'isw_mass_update', 'isw_transport_step', 'isw_swe_step', 'isw_couple_stress', 'isw_land_column',
+           'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_version')
+
+
+def check(rc, what):
+    if rc != 0:
+        if rc == ISW_EBADARG:
+            raise HipError(f'{what}: bad arguments (ISW_EBADARG)')
+        raise HipError(f'{what}: HIP error {rc}')
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None passes NULL)."""
+    return None if t is None else t.data_ptr()
+
+
+def stream():
+    """The current torch stream as a hipStream_t."""
+    import torch
+    return torch.cuda.current_stream().cuda_stream

@@ -1,0 +1,66 @@
+"""Build the native libraries in-tree: libinsilica_hip.so (hipcc, gfx950) and libinsilica_mesh.so (g++).
+
+core.hip is compiled twice: the plain FP64 form with -ffp-contract=off (every product rounds on its own) and the
+mixed form with -ffp-contract=fast. All HIP code is compiled with packed-FP32 instructions disabled.
+"""
+import os
+import subprocess
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = ROOT / 'csrc'
+INCLUDE = ROOT.parent / 'include'
+LIBDIR = PKG / '_lib'
+OBJDIR = ROOT / '_build'
+ARCH = os.environ.get('ISW_OFFLOAD_ARCH', 'gfx950')
+HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+
+HIP_FLAGS = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}', '-I', str(INCLUDE),
+             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wno-unused-command-line-argument']
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError('build failed: ' + ' '.join(cmd) + '\n' + r.stdout + r.stderr)
+    return r
+
+
+def build_hip(verbose=False):
+    """Compile csrc/*.hip into insilica/_lib/libinsilica_hip.so."""
+    OBJDIR.mkdir(exist_ok=True)
+    LIBDIR.mkdir(exist_ok=True)
+    objs = []
+    units = [('core.hip', 'core_f64.o', ['-DISW_FORM_MIXED=0', '-ffp-contract=off']),
+             ('core.hip', 'core_mix.o', ['-DISW_FORM_MIXED=1', '-ffp-contract=fast']),
+             ('capi.hip', 'capi.o', ['-ffp-contract=off'])]
+    for src, obj, extra in units:
+        out = OBJDIR / obj
+        _run([HIPCC, '-c', *HIP_FLAGS, *extra, str(CSRC / src), '-o', str(out)])
+        objs.append(str(out))
+        if verbose:
+            print('compiled', src, '->', obj, flush=True)
+    tmp = LIBDIR / 'libinsilica_hip.so.tmp'
+    _run([HIPCC, '-shared', f'--offload-arch={ARCH}', '-o', str(tmp), *objs])
+    os.replace(tmp, LIBDIR / 'libinsilica_hip.so')
+    return LIBDIR / 'libinsilica_hip.so'
+
+
+def build_mesh():
+    """Compile the Lloyd loop (host C++, OpenMP) into insilica/_lib/libinsilica_mesh.so."""
+    LIBDIR.mkdir(exist_ok=True)
+    tmp = LIBDIR / 'libinsilica_mesh.so.tmp'
+    _run(['g++', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(tmp),
+          str(CSRC / 'mesh_lloyd.cpp')])
+    os.replace(tmp, LIBDIR / 'libinsilica_mesh.so')
+    return LIBDIR / 'libinsilica_mesh.so'
+
+
+def build_all(verbose=False):
+    build_mesh()
+    return build_hip(verbose)
+
+
+if __name__ == '__main__':
+    print(build_all(verbose=True))

@@ -1,0 +1,192 @@
+"""Initial states: (h (L, N), b (N), u (L, E)) NumPy arrays, plus extras (theta, Ts, q) for air.
+
+The one-layer benchmarks are Williamson et al. (1992) 2 and 5 and the Galewsky et al. (2004) jet, built the way the
+reference builds them (src/insilica/cases.py:15-104): winds sampled at the Voronoi edge midpoint xm and projected on
+the edge normal; the same closed forms. The layered starts (a gravity-wave bump, a two-layer internal wave, the
+balanced two-layer jet of the Phillips problem) and the three-layer aquaplanet air (src/insilica/cases.py:227-275)
+likewise. The two benchmark workloads of this package are built here too:
+
+  * `jet_bumps`: one layer 8 km deep in a balanced 20 cos(lat) m/s zonal jet with three seeded 100 m, 500 km
+    Gaussian bumps;
+  * `coupled_stack`: three air layers over two ocean layers with a seeded 30% land mask and random-phase
+    spherical-harmonic perturbations of degree <= 20 (+-2 K on theta, +-5 m/s on the wind).
+"""
+import math
+
+import numpy as np
+
+from .constants import CP_DRY, GRAV, KAPPA, OMEGA, P0, R_PLANET
+from .grid import planet_of
+
+MU_REF = 1.0e5 / GRAV             # kg/m^2: the column mass of 1000 hPa
+AQUA_P = (0.22e5, 0.60e5)         # Pa: the three air layers' starting interfaces
+SURFACE_DRAG = 0.15 * MU_REF / 86400.0
+
+
+def edge_east_north(g):
+    """Latitude, longitude and the east and north components of each edge normal, at the edge midpoints xm."""
+    x = g.xm
+    lat = np.arcsin(np.clip(x[:, 2], -1, 1))
+    lon = np.arctan2(x[:, 1], x[:, 0])
+    en = -np.sin(lon) * g.ne[:, 0] + np.cos(lon) * g.ne[:, 1]
+    nn = (-np.sin(lat) * np.cos(lon) * g.ne[:, 0] - np.sin(lat) * np.sin(lon) * g.ne[:, 1] + np.cos(lat) * g.ne[:, 2])
+    return lat, lon, en, nn
+
+
+def great_circle(g, lon_c, lat_c):
+    c = np.array([math.cos(lat_c) * math.cos(lon_c), math.cos(lat_c) * math.sin(lon_c), math.sin(lat_c)])
+    return R_PLANET * np.arccos(np.clip(g.pos @ c, -1, 1))
+
+
+def cosine_bell(g, lon_c=3 * math.pi / 2, lat_c=0.0, radius=R_PLANET / 3, height=1000.0):
+    """Williamson 1's cosine bell: height/2 (1 + cos(pi r / R)) within great-circle distance R."""
+    d = great_circle(g, lon_c, lat_c)
+    return np.where(d < radius, 0.5 * height * (1 + np.cos(math.pi * d / radius)), 0.0)
+
+
+def default_dt(n):
+    """450 s at n = 4, halved per level."""
+    return 450.0 * 2.0 ** (4 - n)
+
+
+def williamson(g, case):
+    """One layer (h, b, u) of Williamson 2 (steady zonal flow) or 5 (zonal flow over a mountain)."""
+    a, Om, G = planet_of(g)
+    lat_e, _, en, _ = edge_east_north(g)
+    s = np.sin(g.lat)
+    b = np.zeros(g.N)
+    if case == 'w2':
+        u0, gh0 = 2 * math.pi * a / (12 * 86400), 2.94e4
+        h = (gh0 - (a * Om * u0 + u0 * u0 / 2) * s * s) / G
+    elif case == 'w5':
+        u0, h0, Rm, lc, pc = 20.0, 5960.0, math.pi / 9, -math.pi / 2, math.pi / 6
+        dl = np.arctan2(np.sin(g.lon - lc), np.cos(g.lon - lc))
+        b = 2000 * (1 - np.minimum(Rm, np.hypot(dl, g.lat - pc)) / Rm)
+        h = h0 - (a * Om * u0 + u0 * u0 / 2) * s * s / G - b
+    else:
+        raise ValueError(f'unknown case {case!r}')
+    return h, b, u0 * np.cos(lat_e) * en
+
+
+def jet_bumps(g, seed=0, H0=8000.0, u0=20.0, amp=100.0, radius=5.0e5, nbumps=3):
+    """One layer in a balanced zonal jet u = u0 cos(lat), h = H0 - (a Om u0 + u0^2/2) sin^2(lat) / g (the Williamson 2
+    form), plus `nbumps` Gaussian bumps of `amp` m and e-folding `radius` m at seeded centres."""
+    a, Om, G = planet_of(g)
+    rng = np.random.default_rng(seed)
+    s = np.sin(g.lat)
+    h = H0 - (a * Om * u0 + u0 * u0 / 2) * s * s / G
+    for _ in range(nbumps):
+        lon_c, lat_c = rng.uniform(-math.pi, math.pi), math.asin(rng.uniform(-0.9, 0.9))
+        d = great_circle(g, lon_c, lat_c) * (a / R_PLANET)
+        h = h + amp * np.exp(-(d / radius) ** 2)
+    lat_e, _, en, _ = edge_east_north(g)
+    return h[None], np.zeros(g.N), (u0 * np.cos(lat_e) * en)[None]
+
+
+def gravity_wave(g, H, amp=10.0, radius=1.0e6):
+    r = great_circle(g, 0.0, 0.0)
+    return (H + amp * np.exp(-(r / radius) ** 2))[None], np.zeros(g.N), np.zeros((1, g.E))
+
+
+def internal_wave(g, h1, h2, amp=50.0, radius=1.0e6):
+    d = amp * np.exp(-(great_circle(g, 0.0, 0.0) / radius) ** 2)
+    return np.stack([h1 - d, h2 + d]), np.zeros(g.N), np.zeros((2, g.E))
+
+
+def internal_wave_speed(gprime, h1, h2, grav=GRAV):
+    """The internal mode's exact speed for two Boussinesq layers under a free surface."""
+    H = h1 + h2
+    return math.sqrt(grav * H / 2 * (1 - math.sqrt(1 - 4 * gprime * h1 * h2 / (grav * H * H))))
+
+
+def zonal_jet_layers(g, H, ujets, gprime, M=20000):
+    """Two layers with zonal winds ujets[k](lat), each in gradient-wind balance integrated from the south pole."""
+    a = R_PLANET
+    dphi = math.pi / M
+    ph = -math.pi / 2 + (np.arange(1, M + 1) - 0.5) * dphi
+    Ms = []
+    for uj in ujets:
+        uu = uj(ph)
+        Ms.append(np.concatenate([[0.0], np.cumsum(-a * uu * (2 * OMEGA * np.sin(ph) + np.tan(ph) * uu / a) * dphi)]))
+    x = (g.lat + math.pi / 2) / dphi
+    k = np.minimum(M - 1, np.floor(x).astype(int))
+    m0, m1 = (m[k] + (x - k) * (m[k + 1] - m[k]) for m in Ms)
+    A = g.areaCell / np.sum(g.areaCell)
+    eta_v, z1_v = m0 / GRAV, (m1 - m0) / gprime
+    eta = eta_v + H[0] + H[1] - np.sum(A * eta_v)
+    z1 = z1_v + H[1] - np.sum(A * z1_v)
+    lat_e, _, en, _ = edge_east_north(g)
+    return np.stack([eta - z1, z1]), np.zeros(g.N), np.stack([uj(lat_e) * en for uj in ujets])
+
+
+def phillips(g, shear_factor, H=(5000.0, 5000.0), Ld=5.0e5, width_deg=7.0):
+    """The balanced (unperturbed) Phillips jet: equal layers, the upper a Gaussian jet at 45N of shear_factor times
+    the critical shear 2 beta Ld^2, the lower at rest. Returns (h, b, u, gprime)."""
+    lat0 = math.pi / 4
+    f0, beta = 2 * OMEGA * math.sin(lat0), 2 * OMEGA * math.cos(lat0) / R_PLANET
+    He = H[0] * H[1] / (H[0] + H[1])
+    gprime = (Ld * f0) ** 2 / He
+    U0 = shear_factor * 2 * beta * Ld ** 2
+    sig = math.radians(width_deg)
+    jet = lambda phi: U0 * np.exp(-((np.asarray(phi) - lat0) / sig) ** 2)
+    h, b, u = zonal_jet_layers(g, H, (jet, lambda phi: 0.0 * np.asarray(phi)), gprime)
+    return h, b, u, gprime
+
+
+def aqua_air(g):
+    """Three air layers at rest over a slab aquaplanet: interfaces at 220 and 600 hPa, theta 415 / 342.5 /
+    290 - 30 sin^2(lat) K, surface 288 - 40 (sin^2(lat) - 1/3) K (no colder than 274 K), vapor 10/50/70% of
+    saturation at each layer's mid-pressure."""
+    s2 = np.sin(g.lat) ** 2
+    edges = np.array([0.0, *AQUA_P, MU_REF * GRAV])
+    h = np.repeat((np.diff(edges) / GRAV)[:, None], g.N, axis=1)
+    theta = np.stack([np.full(g.N, 415.0), np.full(g.N, 342.5), 290.0 - 30.0 * s2])
+    Ts = np.maximum(288.0 - 40.0 * (s2 - 1.0 / 3.0), 274.0)
+    pm = (0.5 * (edges[:-1] + edges[1:]))[:, None]
+    T = theta * (pm / P0) ** KAPPA
+    qs = 0.622 * 611.0 * np.exp(17.27 * (T - 273.15) / (T - 35.85)) / pm
+    q = np.array([0.1, 0.5, 0.7])[:, None] * qs
+    return h, np.zeros(g.N), np.zeros((3, g.E)), dict(theta=theta, Ts=Ts, q=q)
+
+
+def harmonic_field(g, rng, lmax=20, pos=None):
+    """A random-phase field of spherical-harmonic degree <= lmax at unit positions pos (default: the cells), scaled
+    to a maximum of 1: sum over l, m of a_lm P_l^m(sin lat) cos(m lon + phase), with Gaussian a_lm ~ 1/(l+1)."""
+    from scipy.special import sph_harm_y
+    pos = g.pos if pos is None else pos
+    lat = np.arcsin(np.clip(pos[:, 2], -1, 1))
+    lon = np.arctan2(pos[:, 1], pos[:, 0])
+    theta_c = math.pi / 2 - lat
+    f = np.zeros(len(pos))
+    for l in range(1, lmax + 1):
+        for m in range(0, l + 1):
+            a, ph = rng.standard_normal() / (l + 1), rng.uniform(0, 2 * math.pi)
+            Y = sph_harm_y(l, m, theta_c, lon)
+            f += a * np.real(Y * np.exp(1j * ph))
+    return f / np.max(np.abs(f))
+
+
+def land_mask(g, rng, frac=0.3, smooth=2):
+    """A seeded land mask covering `frac` of the cells: a random field smoothed over `smooth` neighbour rings, the
+    top `frac` of cells by value."""
+    v = rng.standard_normal(g.N)
+    nb = np.where(g.nbr >= 0, g.nbr, np.arange(g.N)[:, None])
+    for _ in range(smooth):
+        v = (v + v[nb].sum(1)) / (1 + nb.shape[1])
+    cut = np.quantile(v, 1 - frac)
+    return (v > cut).astype(float)
+
+
+def coupled_stack(g, seed=0, lmax=20, dtheta=2.0, du=5.0, land_frac=0.3):
+    """Three air layers (the aquaplanet start) over two ocean layers (a 100 m mixed layer over the abyss, 4 km deep),
+    a seeded 30% land mask, and random-phase harmonic perturbations of degree <= lmax: +-dtheta K on every air
+    layer's theta and +-du m/s on every air layer's wind (sampled at the edge midpoints along the normals)."""
+    rng = np.random.default_rng(seed)
+    h, b, u, x = aqua_air(g)
+    theta = x['theta'] + dtheta * np.stack([harmonic_field(g, rng, lmax) for _ in range(3)])
+    lat_e, _, en, nn = edge_east_north(g)
+    ue = np.stack([harmonic_field(g, rng, lmax, pos=g.xm) for _ in range(3)])
+    vn = np.stack([harmonic_field(g, rng, lmax, pos=g.xm) for _ in range(3)])
+    u = du * (ue * en + vn * nn) / math.sqrt(2.0)
+    land = land_mask(g, rng, land_frac)
+    return dict(h=h, b=b, u=u, theta=theta, q=x['q'], Ts=x['Ts'], land=land)

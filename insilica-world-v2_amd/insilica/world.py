@@ -1,0 +1,214 @@
+"""A world: a stack of layers on the layered core carrying its tracers, with an ocean under air worlds; the
+reference's `World` (src/insilica/world.py:55-611) for the dynamical core, FCT transport and the air-sea coupling.
+
+State (float64, layer-major, resident on the GPU): h (L, N), u (L, E), M (L, N) (the layer's volume or mass per cell,
+moved by the same fluxes as h so tracer budgets close exactly) and the tracer rows. A one-layer world carries psi (the
+cosine bell); an air world carries theta, vapor q and three gases (CO2, N2O, CH4 mole fractions) as one stacked
+(5 L, N) tensor, so transport moves all of them in one call with no copies. A tick is one time-split RK3 step of the
+layers, one FCT transport step with that step's fluxes, then for a coupled world the air-sea stress (its kinetic
+energy back as the air's heat), the land column, and on every `ocean_every`-th tick one ocean step.
+
+Presets: 'w2', 'w5' (Williamson 2 and 5, one layer), 'jet' (one 8 km layer in a balanced 20 cos(lat) m/s jet with
+three seeded 100 m bumps, dt 600 s), 'coupled_stack' (three air layers over two ocean layers with a seeded 30% land
+mask and random-phase harmonic perturbations, dt 450 s, 15 fast sub-steps, an hourly ocean step).
+
+Ticks are captured as HIP graphs per tick kind (air only / with the ocean step) and replayed (torch.cuda.CUDAGraph);
+`step(graph=False)` launches every kernel directly. Both give bit-identical results.
+"""
+import hashlib
+import math
+
+import numpy as np
+import torch
+
+from . import cases
+from . import grid as gridmod
+from .constants import CP_DRY, GRAV, KAPPA, PLANET, R_DRY, SOIL_DZ, C_SOIL, K_SOIL
+from .gpu import DeviceGrid, csum, dsum
+from .layers import DIV_DAMP, Layers
+from .ocean import Ocean, visc4
+from .transport import BETA, Transport
+
+THIN_WIND = 100.0
+GAS_TRACERS = (('co2', 400e-6), ('n2o', 0.32e-6), ('ch4', 1.8e-6))
+PRESETS = ('w2', 'w5', 'jet', 'coupled_stack')
+
+
+class World:
+    def __init__(self, n=4, preset='w2', params=None, precision=None, state=None):
+        self.n, self.preset = n, preset
+        P = self.params = dict(params or {})
+        if preset not in PRESETS:
+            raise ValueError(f'unknown preset {preset!r}; this build has {PRESETS}')
+        self.planet = dict(PLANET, **P.get('planet', {}))
+        self.grav = self.planet['grav']
+        self.grid = gridmod.for_planet(gridmod.get(n), **self.planet)
+        self.dg = dg = DeviceGrid(self.grid, precision)
+        g = self.grid
+        self.air = preset == 'coupled_stack'
+        self.sea = None
+        if self.air:
+            built = cases.coupled_stack(g, seed=P.get('seed', 0), land_frac=P.get('land_frac', 0.3))
+            h, b, u = built['h'] * (GRAV / self.grav), built['b'], built['u']
+            self.L = 3
+            self.dt = float(P.get('dt', 900.0 * 2.0 ** (5 - n)))
+            nu4 = P.get('nu4', visc4(g, P.get('visc4_U', 15.0)))
+            floor = P.get('floor', 10.0)
+            self.core = Layers(dg, 3, b=b, grav=self.grav, gprime=[0.0, 0.0], drag=P.get('drag', 0.0),
+                               stress=P.get('stress', [0.0, 0.0, 0.0]), nu4=nu4, floor=floor,
+                               rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP),
+                               air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY), vmix=P.get('vmix', 0.0),
+                               thin_wind=P.get('thin_wind', THIN_WIND))
+            self.land = built['land'] if P.get('land', True) else None
+            gx = [np.full((3, g.N), v) for _, v in GAS_TRACERS]
+            self.rows = dg.asarray(np.concatenate([built['theta'], built['q']] + gx))
+            self.theta, self.q = self.rows[0:3], self.rows[3:6]
+            self.gx = self.rows[6:15].view(3, 3, g.N)
+            self.tr = Transport(dg, layers=3, beta=P.get('beta', BETA), tracers=5)
+            self.sea = Ocean(dg, dict(gprime=(0.016,), h_layers=(100.0,), **P.get('ocean', {})), Tsurf=built['Ts'],
+                             land=self.land, grav=self.grav,
+                             soil=dict(dz=SOIL_DZ, c=C_SOIL, k=K_SOIL) if self.land is not None else None)
+            self.surface_drag = P.get('surface_drag', cases.SURFACE_DRAG)
+            ratio = self.sea.cfg['dt'] / self.dt
+            if abs(ratio - round(ratio)) > 1e-9 or round(ratio) < 1:
+                raise ValueError(f'the ocean step {self.sea.cfg["dt"]} s must be a whole number of air steps ({self.dt} s)')
+            self.ocean_every = int(round(ratio))
+        else:
+            if preset in ('w2', 'w5'):
+                h, b, u = cases.williamson(g, preset)
+                h, u = h[None], u[None]
+                self.dt = float(P.get('dt', cases.default_dt(n)))
+            else:
+                h, b, u = cases.jet_bumps(g, seed=P.get('seed', 0))
+                self.dt = float(P.get('dt', 600.0))
+            if state is not None:
+                h, b, u = state[:3]
+            h = np.asarray(h, dtype=float).reshape(-1, g.N)
+            u = np.asarray(u, dtype=float).reshape(-1, g.E)
+            self.L = h.shape[0]
+            nu4 = P.get('nu4', visc4(g, P['visc4_U']) if 'visc4_U' in P else 0.0)
+            gp = P.get('gprime', [0.0] * (self.L - 1))
+            self.core = Layers(dg, self.L, b=b, grav=self.grav, gprime=gp, drag=P.get('drag', 0.0),
+                               stress=P.get('stress', 0.0), nu4=nu4, floor=P.get('floor', 1.0),
+                               rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP))
+            self.tracer = P.get('tracer', True)
+            self.rows = dg.asarray(np.tile(cases.cosine_bell(g), (self.L, 1)))
+            self.psi = self.rows
+            self.tr = Transport(dg, layers=self.L, beta=P.get('beta', BETA), tracers=1)
+        self.b = self.core.b
+        self.h, self.u = dg.asarray(h), dg.asarray(u)
+        self.M = self.h * dg.areaCell[None, :]
+        if nu4:
+            lam = self.core.lap_max()
+            if nu4 * lam * lam * self.dt > 1.0:
+                raise ValueError(f'nu4 {nu4:.3g} m^4/s is unstable at dt {self.dt:g} s on n = {n}')
+        if self.air:
+            self.core.update_gfast(self.h, self.theta)
+        self.nfast = int(P.get('nfast', math.ceil(1.2 * self.core.substeps(self.h, self.dt))))
+        P['nfast'] = self.nfast
+        self.tick = 0
+        self._Fm, self._M1 = dg.zeros(self.L, dg.E), dg.zeros(self.L, dg.N)
+        self._graphs, self._tick0 = {}, 0
+        self.STATE = ('h', 'u', 'M', 'rows') + (('sea.h', 'sea.u', 'sea.M', 'sea.rows', 'sea.Tg')
+                                                 if self.sea is not None else ())
+
+    # ---- the tick ----
+    def _kind(self, tick):
+        if self.sea is None or (tick + 1) % self.ocean_every:
+            return 0
+        return 1
+
+    def _tick(self, kind=0):
+        """One tick's work on fixed buffers (nothing is allocated), so the sequence can be captured as a graph."""
+        self.core.step(self.h, self.u, self.dt, self.nfast, self._Fm, theta=self.theta if self.air else None)
+        self.tr.mass_after(self.M, self._Fm, self._M1)
+        if self.air or self.tracer:
+            self.tr.step_rows(self.M, self._M1, self._Fm, self.rows, order=3)
+        self.M.copy_(self._M1)
+        if self.sea is not None:
+            self.sea.couple(self.h, self.u, self.surface_drag, self.dt)
+            self.core.vmix_heat(self.h, self.sea.diss, self.theta)
+            self.sea.ground(self.dt)
+            if kind:
+                self.sea.step(mix=False)
+
+    def launch_tick(self, graph=True):
+        """Queue one tick on the current stream without waiting for it."""
+        kind = self._kind(self.tick)
+        gr = self._graphs.get(kind)
+        if graph and gr is None and self.tick > self._tick0:
+            gr = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gr):
+                self._tick(kind)
+            self._graphs[kind] = gr
+        if graph and gr is not None:
+            gr.replay()
+        else:
+            self._tick(kind)
+        self.tick += 1
+
+    def step(self, k=1, graph=True):
+        """k ticks; the first runs directly, later ones replay a captured graph per tick kind."""
+        for _ in range(k):
+            self.launch_tick(graph)
+        torch.cuda.synchronize()
+        return self
+
+    # ---- diagnostics ----
+    def totals(self):
+        """Budgets, each a compensated deterministic sum: per layer volume (or mass) sum h A, M, and each tracer row's
+        sum M psi; for a coupled world the ocean's volume, heat content sum M T and salt sum M S per layer."""
+        dg, A = self.dg, self.dg.areaCell
+        out = {}
+        for k in range(self.L):
+            out[f'volume{k}'] = csum(dg, self.h[k], A)
+            out[f'M{k}'] = csum(dg, self.M[k])
+        for y in range(self.rows.shape[0]):
+            out[f'tracer{y}'] = csum(dg, self.M[y % self.L], self.rows[y])
+        if self.sea is not None:
+            s = self.sea
+            for k in range(s.L):
+                out[f'ocean_volume{k}'] = csum(dg, s.h[k], A)
+                out[f'ocean_M{k}'] = csum(dg, s.M[k])
+            for y in range(s.rows.shape[0]):
+                out[f'ocean_tracer{y}'] = csum(dg, s.M[y % s.L], s.rows[y])
+        return out
+
+    def energy(self):
+        return self.core.energy(self.h, self.u, self.theta if self.air else None)
+
+    def state_arrays(self):
+        out = {}
+        for k in self.STATE:
+            obj = self
+            for part in k.split('.'):
+                obj = getattr(obj, part)
+            out[k] = obj
+        return out
+
+    def state_hash(self):
+        h = hashlib.sha1()
+        for k, v in self.state_arrays().items():
+            h.update(v.detach().cpu().numpy().tobytes())
+        h.update(str(self.tick).encode())
+        return h.hexdigest()
+
+    def field(self, name, layer=0):
+        """A cell field (torch float64 on the device)."""
+        if name == 'h':
+            return self.h[layer]
+        if name == 'eta':
+            return self.b + torch.sum(self.h, dim=0)
+        if name == 'psi' and not self.air:
+            return self.psi[layer]
+        if name == 'theta' and self.air:
+            return self.theta[layer]
+        if name == 'q' and self.air:
+            return self.q[layer] * 1000.0
+        if name == 'ps' and self.air:
+            return self.grav * torch.sum(self.h, dim=0) / 100.0
+        if name == 'sst' and self.sea is not None:
+            return self.sea.T[0] - 273.15
+        if name == 'sss' and self.sea is not None:
+            return self.sea.S[0]
+        raise KeyError(name)

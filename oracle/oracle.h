@@ -1,0 +1,71 @@
+/* CPU oracle of the TRiSK layered core, FCT transport, the one-layer SWE solver and the air-sea coupling of
+ * InSilica World V2 -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a scalar C restatement of the reference's double-precision path, kernel by kernel and in the reference's
+ * summation order, used by the tests (as the parity checker of the HIP library) and by bench.py's CPU-baseline leg.
+ * Nothing in the product links, loads or calls it. Built with gcc -O2 -ffp-contract=off -fopenmp, so every product
+ * rounds on its own as the reference's --fmad=false build does; loops over independent cells/edges/vertices are
+ * OpenMP-parallel, which changes no result.
+ *
+ * Arrays are layer-major x[l * n + i], index tables int32 (width 6 per cell, 3 per vertex, pad -1), signs int8.
+ */
+#ifndef ISW_ORACLE_H
+#define ISW_ORACLE_H
+#include <stdint.h>
+
+#define OG_LMAX 8
+
+typedef struct og_grid {
+  int32_t N, E, NV, n_eoe;
+  const int32_t *eI, *eJ, *nbr, *cellEdge, *cellEdgeAsc, *ev1, *ev2, *vEdge, *vCell, *eoeStart, *eoe, *d2idx;
+  const int8_t *cellSign, *vEdgeSign, *vSlot;
+  const double *le, *de, *areaCell, *areaV, *fV, *kite, *w, *d2wt;
+} og_grid;
+
+/* Settings of one stack of layers (reference: src/insilica/layers.py:161-218 Layers.__init__). */
+typedef struct og_layers_cfg {
+  int32_t L, air;
+  double grav, cp, kappa, p0, rd, nu4, floor_, div_damp, thin, shelf, vmix, vmix_cd, thin_wind;
+  double gp[OG_LMAX], drag[OG_LMAX], stress[OG_LMAX];
+  const double *fV;   /* (NV) the Coriolis parameter with rotation applied */
+  const double *b;    /* (N) bottom */
+  const double *wall; /* (E) or NULL */
+} og_layers_cfg;
+
+typedef struct og_layers og_layers; /* opaque: the settings plus preallocated scratch */
+
+og_layers* og_layers_new(const og_grid* g, const og_layers_cfg* cfg);
+void og_layers_free(og_layers* s);
+double* og_layers_Gc(og_layers* s);  /* (N) the fast mode's coefficient (grav at creation) */
+double* og_layers_L2(og_layers* s);  /* (L,E) del^4 of the last step's start wind */
+double* og_layers_diss(og_layers* s); /* (E) */
+void og_layers_gfast(og_layers* s, const double* h, const double* theta);
+int og_layers_substeps(og_layers* s, const double* h, double dt);
+void og_layers_step(og_layers* s, double* h, double* u, double dt, int nfast, double* Fm, double* theta);
+void og_layers_lap(og_layers* s, const double* u, double* out);
+void og_layers_exchange(og_layers* s, const double* wx, double* h, double* u, double* M, double* psi, double* theta,
+                        int seq, double nu4dt);
+
+/* Transport (reference: src/insilica/transport.py) on R = n_tracers * L rows stacked in psi (rows, N). */
+void og_mass_update(const og_grid* g, int L, const double* M0, const double* Fm, double* M1);
+void og_transport(const og_grid* g, int L, int rows, double beta, const double* M0, const double* M1,
+                  const double* Fm, double* psi, int order);
+
+/* SWE.step (reference: src/insilica/swe.py:124-132): L independent layers, b (N), drag (L). */
+void og_swe_step(const og_grid* g, int L, double grav, const double* b, const double* drag, double nu4, double* h,
+                 double* u, double dt, double* Fm);
+
+/* couple_stress, land_column, ocn_cap, vmix_heat (reference: kernels/ocean.cu:17-42,434-471; layers.cu:599-621). */
+void og_couple_stress(const og_grid* g, int La, const double* ha, double* ua, const double* ho, double* uo, double c,
+                      double rho_w, double dt, double* diss, double* tau_acc, const double* wall, double land_drag,
+                      const double* cd);
+void og_land_column(const og_grid* g, int K, const double* land, double* T, const double* cap, double* Tg,
+                    const double* dz, double c_soil, double k_soil, double dt);
+void og_ocn_cap(const og_grid* g, const double* h, double rho_cw, double* cap, const double* land, double c_land);
+void og_vmix_heat(const og_grid* g, int L, const double* h, const double* diss, double grav, double cp, double kappa,
+                  double p0, double* theta);
+
+/* gpu.dsum's fixed-order tree (reference: kernels/trisk.cu:61-81, gpu.py:111-126). */
+double og_dsum(const double* x, const double* y, int64_t n);
+
+#endif

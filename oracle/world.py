@@ -1,0 +1,99 @@
+"""The oracle's world tick -- TEST INFRASTRUCTURE ONLY.
+
+Restates the tick order of the reference's World._tick (src/insilica/world.py:286-358) for the parts this build
+covers (the layered core, FCT transport, the air-sea stress with its heat, the land column, the hourly ocean step
+without its column mixing) and the Ocean.step order (src/insilica/ocean.py:238-252), on the CPU oracle. Built from a
+product World's settings and starting state, so that the two can be stepped side by side.
+"""
+import numpy as np
+
+from . import Grid, Layers, couple_stress, land_column, mass_update, ocn_cap, transport, vmix_heat
+
+
+def _np(t):
+    return np.ascontiguousarray(t.detach().cpu().numpy()).copy()
+
+
+class OracleWorld:
+    def __init__(self, w):
+        """Mirror of the product World w (its grid, settings and current state)."""
+        self.og = og = Grid(w.grid)
+        self.L, self.dt, self.nfast, self.air, self.ocean_every = w.L, w.dt, w.nfast, w.air, getattr(w, 'ocean_every', 0)
+        c = w.core
+        self.core = Layers(og, c.L, b=_np(c.b), grav=c.grav, gprime=c.gprime or (), drag=c.drag, stress=c.stress,
+                           nu4=c.nu4, floor=c.floor, rotation=1.0, div_damp=c.div_damp,
+                           air=dict(cp=c.cpa, kappa=c.kap, rd=c.rda) if c.air else None, vmix=c.vmix, thin=c.thin,
+                           wall=_np(c.wall) if c.wall is not None else None, thin_wind=c.thin_wind, shelf=c.shelf,
+                           vmix_cd=c.vmix_cd)
+        self.core.fV[:] = _np(c.fV)
+        self.core.Gc[:] = _np(c.Gc)
+        self.h, self.u, self.M, self.rows = _np(w.h), _np(w.u), _np(w.M), _np(w.rows)
+        self.moved = w.air or getattr(w, 'tracer', True)
+        self.beta = w.tr.beta
+        self.tick = w.tick
+        self.Fm, self.M1 = np.zeros((self.L, og.E)), np.zeros((self.L, og.N))
+        self.sea = None
+        if w.sea is not None:
+            s = w.sea
+            self.surface_drag = w.surface_drag
+            self.rho, self.cw, self.c_land, self.land_drag = s.cfg['rho'], s.cfg['cw'], s.c_land, s.cfg['land_drag']
+            sc = s.core
+            self.score = Layers(og, sc.L, b=_np(sc.b), grav=sc.grav, gprime=sc.gprime, drag=sc.drag, stress=sc.stress,
+                                nu4=sc.nu4, floor=sc.floor, div_damp=sc.div_damp, vmix=sc.vmix, thin=sc.thin,
+                                wall=_np(sc.wall) if sc.wall is not None else None, shelf=sc.shelf, vmix_cd=sc.vmix_cd)
+            self.score.fV[:] = _np(sc.fV)
+            self.score.Gc[:] = _np(sc.Gc)
+            self.sea = dict(h=_np(s.h), u=_np(s.u), M=_np(s.M), rows=_np(s.rows), cap=_np(s.cap),
+                            tau_acc=_np(s.tau_acc), diss=_np(s.diss), dt=s.cfg['dt'], nfast=s.nfast, L=s.L,
+                            land=_np(s.land) if s.land is not None else None,
+                            Tg=_np(s.Tg) if s.soil is not None else None,
+                            dz=np.asarray(s.soil['dz'], dtype=float) if s.soil is not None else None,
+                            c_soil=s.soil['c'] if s.soil is not None else 0.0,
+                            k_soil=s.soil['k'] if s.soil is not None else 0.0,
+                            Fm=np.zeros((s.L, og.E)), M1=np.zeros((s.L, og.N)))
+
+    def _kind(self, tick):
+        if self.sea is None or (tick + 1) % self.ocean_every:
+            return 0
+        return 1
+
+    def step(self, k=1):
+        for _ in range(k):
+            self._tick(self._kind(self.tick))
+            self.tick += 1
+        return self
+
+    def _tick(self, kind):
+        og, L = self.og, self.L
+        theta = self.rows[0:L] if self.air else None   # a view: the air's theta is the first L rows
+        self.core.step(self.h, self.u, self.dt, self.nfast, self.Fm, theta=theta)
+        mass_update(og, L, self.M, self.Fm, self.M1)
+        if self.moved:
+            transport(og, L, self.M, self.M1, self.Fm, self.rows, beta=self.beta, order=3)
+        self.M[...] = self.M1
+        if self.sea is not None:
+            s = self.sea
+            couple_stress(og, L, self.h, self.u, s['h'], s['u'], self.surface_drag, self.rho, self.dt, s['diss'],
+                          s['tau_acc'], wall=_wall(self.score), land_drag=self.land_drag)
+            vmix_heat(og, L, self.h, s['diss'], theta, grav=self.core.grav, cp=self.core.cpa, kappa=self.core.kap)
+            if s['Tg'] is not None and s['land'] is not None:
+                land_column(og, len(s['dz']), s['land'], s['rows'], s['cap'], s['Tg'], s['dz'], s['c_soil'],
+                            s['k_soil'], self.dt)
+            if kind:
+                self.score.step(s['h'], s['u'], s['dt'], s['nfast'], s['Fm'])
+                mass_update(og, s['L'], s['M'], s['Fm'], s['M1'])
+                transport(og, s['L'], s['M'], s['M1'], s['Fm'], s['rows'], order=3)
+                s['M'][...] = s['M1']
+                ocn_cap(og, s['h'], self.rho * self.cw, s['cap'], s['land'], self.c_land)
+
+    def state(self):
+        out = dict(h=self.h, u=self.u, M=self.M, rows=self.rows)
+        if self.sea is not None:
+            for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap'):
+                if self.sea[k] is not None:
+                    out['sea.' + k] = self.sea[k]
+        return out
+
+
+def _wall(layers):
+    return layers.wall
