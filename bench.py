@@ -1,0 +1,129 @@
+"""Benchmark: model-hours/s of the coupled air-ocean world at level 6 on one GPU, mixed and plain FP64 forms.
+
+Workload: the level-6 mesh (40,962 cells, 122,880 edges), three air layers over two ocean layers with a seeded 30%
+land mask and random-phase harmonic perturbations (degree <= 20, +-2 K, +-5 m/s); split-explicit RK3 at dt 450 s with
+15 fast sub-steps; FCT transport of theta, vapor and three gases (15 rows) in the air and of T and S in the ocean; the
+exact air-sea stress with its heat, the land column, and one ocean step (dt 3600 s) per model hour. A step here is one
+model hour: 8 ticks, replayed from captured HIP graphs, with the state resident in HBM. Synthetic data.
+
+Printed: one JSON line with the mixed form's model-hours/s as `value`, the plain FP64 form's beside it, the dominant
+kernel's roofline (algorithmic bytes per launch over its event-timed average duration, against 8 TB/s), and the CPU
+oracle's rate on a bounded sample of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (str(ROOT), str(ROOT / 'insilica-world-v2_amd')):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=12, help='model hours timed')
+    ap.add_argument('--warmup', type=int, default=2, help='model hours before timing')
+    ap.add_argument('--level', type=int, default=6)
+    ap.add_argument('--no-fp64', action='store_true', help='skip the plain FP64 form')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline')
+    ap.add_argument('--cpu-hours', type=float, default=1.0, help='model hours of the CPU sample')
+    return ap.parse_args()
+
+
+def make_world(level, precision):
+    from insilica.world import World
+    return World(level, 'coupled_stack', precision=precision, params=dict(nfast=15))
+
+
+def time_world(w, steps, warmup):
+    import torch
+    ticks = w.ocean_every
+    w.step(warmup * ticks)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    w.step(steps * ticks)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_profile(w, hours=1):
+    """Per-kernel event times and algorithmic bytes over `hours` model hours of directly launched ticks."""
+    import torch
+    from insilica import _hip
+    torch.cuda.synchronize()
+    _hip.timing_start()
+    w.step(hours * w.ocean_every, graph=False)
+    torch.cuda.synchronize()
+    return _hip.timing_stop()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f'bench.py runs on one GPU only (asked for --gpus {a.gpus}): this build has no multi-GPU path; '
+              f'run with --gpus 1', file=sys.stderr, flush=True)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    print(f'building level-{a.level} worlds', file=sys.stderr, flush=True)
+    wm = make_world(a.level, 'mixed')
+    el_m = time_world(wm, a.steps, a.warmup)
+    print(f'mixed: {a.steps} model hours in {el_m:.3f} s', file=sys.stderr, flush=True)
+    prof = kernel_profile(wm)
+    fp64 = None
+    if not a.no_fp64:
+        wd = make_world(a.level, 'double')
+        el_d = time_world(wd, a.steps, a.warmup)
+        fp64 = a.steps / el_d
+        print(f'fp64: {a.steps} model hours in {el_d:.3f} s', file=sys.stderr, flush=True)
+        del wd
+    # dominant kernel by summed time
+    name, k = max(prof.items(), key=lambda kv: kv[1]['ms'])
+    avg_s = k['ms'] / k['launches'] / 1e3
+    bytes_per = k['bytes'] / k['launches']
+    achieved = bytes_per / avg_s / 1e9
+    total_ms = sum(v['ms'] for v in prof.values())
+    total_bytes = sum(v['bytes'] for v in prof.values())
+    roofline = dict(bound='hbm', kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=bytes_per,
+                    avg_us=round(avg_s * 1e6, 3), launches_per_hour=k['launches'],
+                    all_kernels_GBs=round(total_bytes / (total_ms / 1e3) / 1e9, 1),
+                    kernel_ms_per_hour=round(total_ms, 3))
+    cpu = None
+    if not a.no_cpu:
+        from oracle.world import OracleWorld
+        cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+        o = OracleWorld(wm)
+        ticks = max(1, int(round(a.cpu_hours * wm.ocean_every)))
+        t0 = time.perf_counter()
+        o.step(ticks)
+        el = time.perf_counter() - t0
+        cpu = dict(value=round(ticks / wm.ocean_every / el, 4), unit='model-hours/s', cores=cores, kind='port',
+                   sample=f'{ticks} ticks ({ticks / wm.ocean_every:g} model hours, one ocean step per hour) of the '
+                          f'same level-{a.level} coupled world, FP64 oracle (C, OpenMP), {el:.1f} s')
+    ms = el_m / a.steps * 1e3
+    line = dict(metric='model-hours/s at level 6 (FP64 vs mixed, 1 GPU)', value=round(a.steps / el_m, 3),
+                unit='model-hours/s', n_gpus=1, steps=a.steps, warmup=a.warmup, ms_per_step=round(ms, 3),
+                higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f64 state, f32 stencils (mixed)',
+                data='synthetic',
+                config=dict(workload=f'coupled_stack level {a.level}: 3 air + 2 ocean layers, dt 450 s, nfast 15, '
+                                     f'15 air + 4 ocean tracer rows, 30% land; step = 1 model hour (8 ticks)',
+                            cells=wm.grid.N, edges=wm.grid.E, parallelism='1 GPU'),
+                fp64_value=round(fp64, 3) if fp64 else None,
+                mixed_over_fp64=round(a.steps / el_m / fp64, 3) if fp64 else None,
+                roofline=roofline, cpu_baseline=cpu,
+                kernels={k2: dict(launches=v['launches'], ms=round(v['ms'], 4),
+                                  GBs=round(v['bytes'] / (v['ms'] / 1e3) / 1e9, 1) if v['ms'] > 0 else None)
+                         for k2, v in sorted(prof.items(), key=lambda kv: -kv[1]['ms'])})
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

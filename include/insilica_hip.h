@@ -136,12 +136,24 @@ int32_t isw_ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* c
 int32_t isw_dsum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream);
 
 /* A compensated, deterministic sum of x (or x*y) into out[0]: per-block Neumaier partial sums with their error terms
- * in a fixed order, then one fixed-order compensated pass over the partials. scratch: 2 * ceil(n / 2048) doubles.
- * The budgets' sum (no counterpart in the reference, whose totals use dsum). */
+ * in a fixed order, then one fixed-order compensated pass over the partials; out[0] the sum, out[1] its remaining
+ * correction (out holds 2 doubles). scratch: 2 * ceil(n / 2048) doubles. The budgets' sum (no counterpart in the
+ * reference, whose totals use dsum). */
 int32_t isw_csum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream);
 
 /* Fill n doubles with v (a kernel, not hipMemsetAsync, so that captured graphs replay it safely). */
 int32_t isw_fill(double* x, int64_t n, double v, void* stream);
+
+/* Measurement mode (no counterpart in the reference, whose tools/profile.py wraps its kernels in CUDA event pairs
+ * from Python; this is that instrument inside the library). isw_timing_start() switches it on: from then every kernel
+ * launched directly (not under stream capture) by the entry points above is bracketed by two HIP events on its stream
+ * and tagged with its name and algorithmic bytes (each distinct array the kernel touches counted once at its stored
+ * width, twice if read and written, index tables included). isw_timing_stop() switches it off, waits for the events
+ * and writes one line per kernel, "name launches total_ms total_bytes", into out (cap bytes); it returns the text's
+ * length, or minus the size needed if cap is too small (the records are then dropped). Host-side bookkeeping only:
+ * it creates events, not device memory. Not thread-safe; not for use while a graph is being captured. */
+int32_t isw_timing_start(void);
+int32_t isw_timing_stop(char* out, int32_t cap);
 
 /* Library version and the gfx target it was built for. */
 const char* isw_version(void);

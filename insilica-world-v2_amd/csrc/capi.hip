@@ -4,7 +4,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
 
+#include "diag.hpp"
 #include "insilica_hip.h"
 
 #define ISW_DECLARE(NS)                                                                                              \
@@ -256,4 +262,89 @@ extern "C" int32_t isw_csum(const double* x, const double* y, int64_t n, double*
   k_csum_blocks<<<dim3((unsigned)nb), SUM_THREADS, 0, st>>>(n, x, y, scratch);
   k_csum_final<<<dim3(1), SUM_THREADS, 0, st>>>(nb, scratch, out);
   return (int32_t)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Measurement mode (diag.hpp): per-launch event pairs, aggregated per kernel name.
+// ---------------------------------------------------------------------------------------------------------------
+namespace isw_diag {
+namespace {
+struct Rec {
+  const char* name;
+  double bytes;
+  hipEvent_t a, b;
+};
+bool g_on = false;
+std::vector<Rec> g_recs;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t take() {
+  if (!g_pool.empty()) {
+    hipEvent_t e = g_pool.back();
+    g_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void recycle() {
+  for (auto& r : g_recs) {
+    if (r.a) g_pool.push_back(r.a);
+    if (r.b) g_pool.push_back(r.b);
+  }
+  g_recs.clear();
+}
+}  // namespace
+
+bool enabled() { return g_on; }
+
+int begin(const char* name, double bytes, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+  Rec r{name, bytes, take(), take()};
+  if (!r.a || !r.b) return -1;
+  hipEventRecord(r.a, st);
+  g_recs.push_back(r);
+  return (int)g_recs.size() - 1;
+}
+
+void end(int tok, hipStream_t st) { hipEventRecord(g_recs[tok].b, st); }
+}  // namespace isw_diag
+
+extern "C" int32_t isw_timing_start(void) {
+  isw_diag::recycle();
+  isw_diag::g_on = true;
+  return 0;
+}
+
+extern "C" int32_t isw_timing_stop(char* out, int32_t cap) {
+  using namespace isw_diag;
+  g_on = false;
+  struct Agg {
+    long count = 0;
+    double ms = 0, bytes = 0;
+  };
+  std::map<std::string, Agg> agg;
+  for (auto& r : g_recs) {
+    if (hipEventSynchronize(r.b) != hipSuccess) return -1;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) return -1;
+    Agg& a = agg[r.name];
+    a.count++;
+    a.ms += ms;
+    a.bytes += r.bytes;
+  }
+  recycle();
+  std::string text;
+  char line[256];
+  for (auto& kv : agg) {
+    std::snprintf(line, sizeof line, "%s %ld %.6f %.1f\n", kv.first.c_str(), kv.second.count, kv.second.ms,
+                  kv.second.bytes);
+    text += line;
+  }
+  if (!out || cap <= (int32_t)text.size()) return -(int32_t)(text.size() + 1);
+  std::memcpy(out, text.c_str(), text.size() + 1);
+  return (int32_t)text.size();
 }

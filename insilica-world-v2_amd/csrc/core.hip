@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "insilica_hip.h"
+#include "diag.hpp"
 
 #if ISW_FORM_MIXED
 #define NS isw_mix
@@ -135,12 +136,6 @@ __global__ __launch_bounds__(BS) void k_edge_lap(isw_grid g, int L, const double
   out[t] = (Dl[g.eJ[e]] - Dl[g.eI[e]]) / g.de[e] - (Zl[g.ev2[e]] - Zl[g.ev1[e]]) / g.le[e];
 }
 
-static int lap(const isw_grid* g, int L, const double* u, double* D, double* Z, double* out, hipStream_t st) {
-  const int nbc = (g->N * L + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
-  k_div_curl<<<dim3(nbc + nbv), BS, 0, st>>>(*g, L, nbc, u, D, Z);
-  k_edge_lap<<<g1((long)g->E * L), BS, 0, st>>>(*g, L, D, Z, out);
-  return 0;
-}
 
 // ---------------------------------------------------------------------------------------------------------------
 // Column kernels at the step's start (layers.cu:33-74,165-175,234-250).
@@ -1016,163 +1011,7 @@ __global__ __launch_bounds__(BS) void k_ocn_cap(int N, double rho_cw, double c_l
   cap[i] = land && land[i] > 0.5 ? (cap[i] > 0 ? cap[i] : c_land) : rho_cw * h[i];
 }
 
-// ===============================================================================================================
-// Host sequencing (the reference's Python drivers), all on one stream.
-// ===============================================================================================================
-
-// Layers._stage (layers.py:255-309).
-static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div& dv, const double* hs,
-                  const double* us, const double* h0, const double* u0, double c, double dt, int nfast, double* hn,
-                  double* un, double* Fm, const double* theta, hipStream_t st) {
-  const int N = g->N, E = g->E, L = s->L;
-  const int nbc = (N + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
-  k_bern_pv<<<dim3(nbc + nbv), BS, 0, st>>>(*g, p, dv, nbc, hs, us, theta, s->b, s->S, s->fV, s->B, s->pibar, s->q);
-#if ISW_FORM_MIXED
-  fast* pk = s->pkf;
-  const fast* wf = g->wf;
-#else
-  fast* pk = s->pk;
-  const fast* wf = g->w;
-#endif
-  k_edge_hq<<<g1((long)E * L), BS, 0, st>>>(*g, L, s->shelf, hs, s->q, us, s->hedge, pk);
-  k_slow<<<g1(E), BS, 0, st>>>(*g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
-                               s->air ? theta : nullptr, s->air ? s->pibar : nullptr, s->Tp, s->G, s->He, s->ubs);
-  int n = (int)std::ceil(c * nfast);
-  if (n < 1) n = 1;
-  const double tau = c * dt / n;
-  FluxSrc fs{};
-  fs.inv_n = 1.0 / n;
-  fs.shelf = s->shelf;
-  if (s->air) {
-    for (int it = 0; it < n; it++) {
-      const bool first = it == 0, last = it == n - 1;
-      k_af_muphi<<<g1(N), BS, 0, st>>>(*g, p, dv, tau, s->hedge, first ? u0 : s->uf, first ? h0 : s->muf, s->muf,
-                                       s->S, s->phi);
-      k_af_u<<<g1((long)E * L), BS, 0, st>>>(*g, L, dv, tau, first ? 1 : 0, s->phi, s->Tp, s->G, first ? u0 : s->uf,
-                                             s->accL, last ? un : s->uf);
-    }
-    fs.air = 1;
-    fs.hedge = s->hedge;
-    fs.acc = s->accL;
-  } else {
-    for (int it = 0; it < n; it++) {
-      const bool first = it == 0;
-      k_bt_eta<<<g1(N), BS, 0, st>>>(*g, dv, tau, s->He, first ? s->ub0 : s->ub, first ? s->eta0 : s->eta, s->eta,
-                                     s->eta_prev);
-      k_bt_u<<<g1(E), BS, 0, st>>>(*g, dv, tau, s->div_damp, first ? 1 : 0, s->eta, s->eta_prev, s->G, s->Gc,
-                                   s->wall, first ? s->ub0 : s->ub, s->ub, s->acc);
-    }
-    fs.air = 0;
-    fs.hs = hs;
-    fs.us = us;
-    fs.ubs = s->ubs;
-    fs.bacc = s->acc;
-    fs.wall = s->wall;
-  }
-  k_scale<<<g1((long)N * L), BS, 0, st>>>(*g, fs, c * dt, s->floor_, h0, s->scale, L);
-  k_limit_h<<<g1((long)N * L), BS, 0, st>>>(*g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L);
-  if (!s->air) k_lay_u<<<g1((long)E * L), BS, 0, st>>>(E, L, c * dt, u0, s->Tp, s->ub, s->ub0, s->wall, un);
-}
-
-int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, double* theta, double* Fm, double dt,
-                int nfast, hipStream_t st) {
-  const LP p = make_lp(s);
-  const Div dv = divs(g);
-  const int N = g->N, L = s->L;
-  if (s->air) {
-    k_air_stiff<<<g1(N), BS, 0, st>>>(N, p, h, theta, s->S);
-  } else {
-    k_lay_eta<<<g1(N), BS, 0, st>>>(N, L, h, s->b, s->eta0);
-    k_lay_ubar<<<g1(g->E), BS, 0, st>>>(*g, L, s->shelf, h, u, s->wall, s->ub0);
-  }
-  if (s->nu4 != 0) {
-    lap(g, L, u, s->D1, s->Z1, s->L1, st);
-    lap(g, L, s->L1, s->D1, s->Z1, s->L2, st);
-  }
-  stage(g, s, p, dv, h, u, h, u, 1.0 / 3.0, dt, nfast, s->h1, s->u1, nullptr, theta, st);
-  stage(g, s, p, dv, s->h1, s->u1, h, u, 1.0 / 2.0, dt, nfast, s->h2, s->u2, nullptr, theta, st);
-  stage(g, s, p, dv, s->h2, s->u2, h, u, 1.0, dt, nfast, h, u, Fm, theta, st);
-  if ((s->vmix != 0 || s->vmix_cd != 0) && L > 1) {
-    k_lay_vmix<<<g1(g->E), BS, 0, st>>>(*g, L, s->vmix, dt, s->vmix_cd, h, s->wall, u, s->diss);
-    if (s->air) k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, p, h, s->diss, theta);
-  }
-  return (int)hipGetLastError();
-}
-
-int layers_lap(const isw_grid* g, const isw_layers* s, const double* u, double* out, hipStream_t st) {
-  lap(g, s->L, u, s->D1, s->Z1, out, st);
-  return (int)hipGetLastError();
-}
-
-int layers_gfast(const isw_grid* g, const isw_layers* s, const double* h, const double* theta, hipStream_t st) {
-  if (s->air) k_air_gfast<<<g1(g->N), BS, 0, st>>>(g->N, make_lp(s), s->rd, h, theta, s->Gc);
-  return (int)hipGetLastError();
-}
-
-int layers_exchange(const isw_grid* g, const isw_layers* s, const double* wx, double* h, double* u, double* M,
-                    double* psi, double* theta, int seq, double nu4dt, hipStream_t st) {
-  const int N = g->N, L = s->L;
-  k_copy<<<g1((long)N * L), BS, 0, st>>>((long)N * L, h, s->hold);
-  const bool heat = s->air && theta;
-  k_xch_cell<<<g1(N), BS, 0, st>>>(N, L, seq, wx, g->areaCell, h, M, psi);
-  const bool v4 = heat && s->nu4 != 0;
-  k_xch_edge<<<g1(g->E), BS, 0, st>>>(*g, L, wx, s->hold, u, heat ? s->diss : nullptr, v4 ? s->L2 : nullptr,
-                                      v4 ? nu4dt : 0.0, heat ? s->thin_wind : 0.0, s->floor_, s->wall);
-  if (heat) k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, make_lp(s), h, s->diss, theta);
-  return (int)hipGetLastError();
-}
-
-int vmix_heat(const isw_grid* g, const isw_layers* s, const double* h, const double* diss, double* theta,
-              hipStream_t st) {
-  k_vmix_heat<<<g1(g->N), BS, 0, st>>>(*g, make_lp(s), h, diss, theta);
-  return (int)hipGetLastError();
-}
-
-int mass_update(const isw_grid* g, int L, const double* M0, const double* Fm, double* M1, hipStream_t st) {
-  k_mass_update<<<g2(g->N, L), BS, 0, st>>>(*g, M0, Fm, M1);
-  return (int)hipGetLastError();
-}
-
-// Transport.step_many (transport.py:40-71) on rows stacked in psi; the result is written back into psi.
-int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0, const double* M1, const double* Fm,
-                   double* psi, int rows, int order, hipStream_t st) {
-  const int N = g->N, E = g->E, Lm = tr->L;
-#if ISW_FORM_MIXED
-  float* pf = tr->pf;
-  const fast* wt = g->d2wtf;
-#else
-  float* pf = nullptr;
-  const fast* wt = g->d2wt;
-#endif
-  k_tr_td<<<g2(N, rows), BS, 0, st>>>(*g, Lm, M0, psi, Fm, M1, tr->td, pf);
-  if (order == 1) {
-    k_copy<<<g1((long)N * rows), BS, 0, st>>>((long)N * rows, tr->td, psi);
-    return (int)hipGetLastError();
-  }
-#if ISW_FORM_MIXED
-  const fast* psf = pf;
-  const fast* p2f = pf;
-#else
-  const fast* psf = psi;
-  const fast* p2f = tr->p2;
-#endif
-  k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf);
-  k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf);
-#if ISW_FORM_MIXED
-  const fast* p1f = pf;
-#else
-  const fast* p1f = tr->p1;
-#endif
-  k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf);
-  k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
-  k_tr_aflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
-  k_tr_ratios<<<g2(N, rows), BS, 0, st>>>(*g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm);
-  k_tr_apply<<<g2(N, rows), BS, 0, st>>>(*g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi);
-  return (int)hipGetLastError();
-}
-
-// SWE.step (swe.py:105-132): the edge pass (swe_edge, fused.cu:101-120) with each layer's drag taken from the layer
-// settings passed by value.
+// SWE's edge pass (swe_edge, fused.cu:101-120) with each layer's drag from the layer settings passed by value.
 __global__ __launch_bounds__(BS) void k_swe_edge_lp(isw_grid g, LP p, double c, const double* __restrict__ F,
                                                     const double* __restrict__ q, const double* __restrict__ B,
                                                     const double* __restrict__ u, const double* __restrict__ lap4,
@@ -1196,10 +1035,233 @@ __global__ __launch_bounds__(BS) void k_swe_edge_lp(isw_grid g, LP p, double c, 
   un[o] = u0[o] + c * d;
 }
 
+// ===============================================================================================================
+// Host sequencing (the reference's Python drivers), all on one stream. Every launch goes through ISW_LAUNCH with the
+// kernel's algorithmic bytes for the measurement mode (diag.hpp).
+// ===============================================================================================================
+
+// Byte counts: D doubles, I int32, C int8, R fast (the PV-flux and stencil operands' width).
+constexpr double D = 8, I = 4, C = 1, R = sizeof(fast);
+
+static double flux_bytes(const FluxSrc& f, double N, double E, double L) {
+  return f.air ? 2 * L * E * D : L * N * D + L * E * D + 2 * E * D + (f.wall ? E * D : 0);
+}
+
+static void lap(const isw_grid* g, int L, const double* u, double* D1, double* Z1, double* out, hipStream_t st) {
+  const double N = g->N, E = g->E, NV = g->NV, l = L;
+  const int nbc = (g->N * L + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
+  ISW_LAUNCH("div_curl", l * E * D + 6 * N * (I + C) + E * D + N * D + l * N * D + 3 * NV * (I + C) + E * D + NV * D +
+                             l * NV * D,
+             k_div_curl<<<dim3(nbc + nbv), BS, 0, st>>>(*g, L, nbc, u, D1, Z1));
+  ISW_LAUNCH("edge_lap", l * N * D + l * NV * D + 4 * E * I + 2 * E * D + l * E * D,
+             k_edge_lap<<<g1((long)g->E * L), BS, 0, st>>>(*g, L, D1, Z1, out));
+}
+
+// Layers._stage (layers.py:255-309).
+static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div& dv, const double* hs,
+                  const double* us, const double* h0, const double* u0, double c, double dt, int nfast, double* hn,
+                  double* un, double* Fm, const double* theta, hipStream_t st) {
+  const int N = g->N, E = g->E, L = s->L;
+  const double n_ = N, e_ = E, v_ = g->NV, l_ = L, air = s->air ? 1 : 0;
+  const int nbc = (N + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
+  ISW_LAUNCH("bern_pv", l_ * n_ * D * (1 + 3 * air) + l_ * e_ * D + n_ * D + 6 * n_ * I + 2 * e_ * D + n_ * D +
+                            l_ * n_ * D + 3 * v_ * (2 * I + 2 * C) + 6 * n_ * D + 2 * v_ * D + l_ * v_ * D,
+             k_bern_pv<<<dim3(nbc + nbv), BS, 0, st>>>(*g, p, dv, nbc, hs, us, theta, s->b, s->S, s->fV, s->B,
+                                                      s->pibar, s->q));
+#if ISW_FORM_MIXED
+  fast* pk = s->pkf;
+  const fast* wf = g->wf;
+#else
+  fast* pk = s->pk;
+  const fast* wf = g->w;
+#endif
+  ISW_LAUNCH("edge_hq", l_ * n_ * D + l_ * v_ * D + l_ * e_ * D + 4 * e_ * I + l_ * e_ * D + 3 * l_ * e_ * R,
+             k_edge_hq<<<g1((long)E * L), BS, 0, st>>>(*g, L, s->shelf, hs, s->q, us, s->hedge, pk));
+  const double neoe = g->n_eoe;
+  ISW_LAUNCH("slow", 3 * l_ * e_ * R + neoe * (R + I) + (e_ + 1) * I + 2 * l_ * e_ * D + l_ * n_ * D + 2 * e_ * I +
+                         e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + air * 2 * l_ * n_ * D + l_ * e_ * D + 3 * e_ * D,
+             k_slow<<<g1(E), BS, 0, st>>>(*g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
+                                          s->air ? theta : nullptr, s->air ? s->pibar : nullptr, s->Tp, s->G, s->He,
+                                          s->ubs));
+  int n = (int)std::ceil(c * nfast);
+  if (n < 1) n = 1;
+  const double tau = c * dt / n;
+  FluxSrc fs{};
+  fs.inv_n = 1.0 / n;
+  fs.shelf = s->shelf;
+  if (s->air) {
+    for (int it = 0; it < n; it++) {
+      const bool first = it == 0, last = it == n - 1;
+      ISW_LAUNCH("af_muphi", 2 * l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 4 * l_ * n_ * D,
+                 k_af_muphi<<<g1(N), BS, 0, st>>>(*g, p, dv, tau, s->hedge, first ? u0 : s->uf, first ? h0 : s->muf,
+                                                  s->muf, s->S, s->phi));
+      ISW_LAUNCH("af_u", l_ * n_ * D + l_ * e_ * D + e_ * D + 2 * e_ * I + e_ * D + 2 * l_ * e_ * D +
+                             (first ? 1 : 2) * l_ * e_ * D,
+                 k_af_u<<<g1((long)E * L), BS, 0, st>>>(*g, L, dv, tau, first ? 1 : 0, s->phi, s->Tp, s->G,
+                                                        first ? u0 : s->uf, s->accL, last ? un : s->uf));
+    }
+    fs.air = 1;
+    fs.hedge = s->hedge;
+    fs.acc = s->accL;
+  } else {
+    for (int it = 0; it < n; it++) {
+      const bool first = it == 0;
+      ISW_LAUNCH("bt_eta", 2 * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 3 * n_ * D,
+                 k_bt_eta<<<g1(N), BS, 0, st>>>(*g, dv, tau, s->He, first ? s->ub0 : s->ub, first ? s->eta0 : s->eta,
+                                                s->eta, s->eta_prev));
+      ISW_LAUNCH("bt_u", 3 * n_ * D + e_ * D + 2 * e_ * I + e_ * D + (s->wall ? e_ * D : 0) + 2 * e_ * D +
+                             (first ? 1 : 2) * e_ * D,
+                 k_bt_u<<<g1(E), BS, 0, st>>>(*g, dv, tau, s->div_damp, first ? 1 : 0, s->eta, s->eta_prev, s->G,
+                                              s->Gc, s->wall, first ? s->ub0 : s->ub, s->ub, s->acc));
+    }
+    fs.air = 0;
+    fs.hs = hs;
+    fs.us = us;
+    fs.ubs = s->ubs;
+    fs.bacc = s->acc;
+    fs.wall = s->wall;
+  }
+  const double fb = flux_bytes(fs, n_, e_, l_);
+  ISW_LAUNCH("scale", fb + 6 * n_ * I + 2 * e_ * I + e_ * D + l_ * n_ * D + n_ * D + l_ * n_ * D,
+             k_scale<<<g1((long)N * L), BS, 0, st>>>(*g, fs, c * dt, s->floor_, h0, s->scale, L));
+  ISW_LAUNCH("limit_h", fb + 6 * n_ * (I + C) + 2 * e_ * I + e_ * D + 3 * l_ * n_ * D + n_ * D +
+                            (Fm ? l_ * e_ * D : 0),
+             k_limit_h<<<g1((long)N * L), BS, 0, st>>>(*g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L));
+  if (!s->air)
+    ISW_LAUNCH("lay_u", 3 * l_ * e_ * D + 2 * e_ * D + (s->wall ? e_ * D : 0),
+               k_lay_u<<<g1((long)E * L), BS, 0, st>>>(E, L, c * dt, u0, s->Tp, s->ub, s->ub0, s->wall, un));
+}
+
+int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, double* theta, double* Fm, double dt,
+                int nfast, hipStream_t st) {
+  const LP p = make_lp(s);
+  const Div dv = divs(g);
+  const int N = g->N, L = s->L;
+  const double n_ = N, e_ = g->E, l_ = L;
+  if (s->air) {
+    ISW_LAUNCH("air_stiff", 3 * l_ * n_ * D, k_air_stiff<<<g1(N), BS, 0, st>>>(N, p, h, theta, s->S));
+  } else {
+    ISW_LAUNCH("lay_eta", (l_ + 2) * n_ * D, k_lay_eta<<<g1(N), BS, 0, st>>>(N, L, h, s->b, s->eta0));
+    ISW_LAUNCH("lay_ubar", l_ * n_ * D + l_ * e_ * D + 2 * e_ * I + (s->wall ? e_ * D : 0) + e_ * D,
+               k_lay_ubar<<<g1(g->E), BS, 0, st>>>(*g, L, s->shelf, h, u, s->wall, s->ub0));
+  }
+  if (s->nu4 != 0) {
+    lap(g, L, u, s->D1, s->Z1, s->L1, st);
+    lap(g, L, s->L1, s->D1, s->Z1, s->L2, st);
+  }
+  stage(g, s, p, dv, h, u, h, u, 1.0 / 3.0, dt, nfast, s->h1, s->u1, nullptr, theta, st);
+  stage(g, s, p, dv, s->h1, s->u1, h, u, 1.0 / 2.0, dt, nfast, s->h2, s->u2, nullptr, theta, st);
+  stage(g, s, p, dv, s->h2, s->u2, h, u, 1.0, dt, nfast, h, u, Fm, theta, st);
+  if ((s->vmix != 0 || s->vmix_cd != 0) && L > 1) {
+    ISW_LAUNCH("lay_vmix", l_ * n_ * D + 2 * e_ * I + 2 * l_ * e_ * D + e_ * D + (s->wall ? e_ * D : 0),
+               k_lay_vmix<<<g1(g->E), BS, 0, st>>>(*g, L, s->vmix, dt, s->vmix_cd, h, s->wall, u, s->diss));
+    if (s->air)
+      ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
+                 k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, p, h, s->diss, theta));
+  }
+  return (int)hipGetLastError();
+}
+
+int layers_lap(const isw_grid* g, const isw_layers* s, const double* u, double* out, hipStream_t st) {
+  lap(g, s->L, u, s->D1, s->Z1, out, st);
+  return (int)hipGetLastError();
+}
+
+int layers_gfast(const isw_grid* g, const isw_layers* s, const double* h, const double* theta, hipStream_t st) {
+  if (s->air)
+    ISW_LAUNCH("air_gfast", 3.0 * s->L * g->N * D,
+               k_air_gfast<<<g1(g->N), BS, 0, st>>>(g->N, make_lp(s), s->rd, h, theta, s->Gc));
+  return (int)hipGetLastError();
+}
+
+int layers_exchange(const isw_grid* g, const isw_layers* s, const double* wx, double* h, double* u, double* M,
+                    double* psi, double* theta, int seq, double nu4dt, hipStream_t st) {
+  const int N = g->N, L = s->L;
+  const double n_ = N, e_ = g->E, l_ = L;
+  ISW_LAUNCH("copy", 2 * l_ * n_ * D, k_copy<<<g1((long)N * L), BS, 0, st>>>((long)N * L, h, s->hold));
+  const bool heat = s->air && theta;
+  ISW_LAUNCH("xch_cell", (l_ - 1) * n_ * D + n_ * D + 6 * l_ * n_ * D,
+             k_xch_cell<<<g1(N), BS, 0, st>>>(N, L, seq, wx, g->areaCell, h, M, psi));
+  const bool v4 = heat && s->nu4 != 0;
+  ISW_LAUNCH("xch_edge", (l_ - 1) * n_ * D + l_ * n_ * D + 2 * e_ * I + 2 * l_ * e_ * D + (heat ? e_ * D : 0) +
+                             (v4 ? l_ * e_ * D : 0) + (s->wall ? e_ * D : 0),
+             k_xch_edge<<<g1(g->E), BS, 0, st>>>(*g, L, wx, s->hold, u, heat ? s->diss : nullptr,
+                                                 v4 ? s->L2 : nullptr, v4 ? nu4dt : 0.0, heat ? s->thin_wind : 0.0,
+                                                 s->floor_, s->wall));
+  if (heat)
+    ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
+               k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, make_lp(s), h, s->diss, theta));
+  return (int)hipGetLastError();
+}
+
+int vmix_heat(const isw_grid* g, const isw_layers* s, const double* h, const double* diss, double* theta,
+              hipStream_t st) {
+  const double n_ = g->N, e_ = g->E, l_ = s->L;
+  ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
+             k_vmix_heat<<<g1(g->N), BS, 0, st>>>(*g, make_lp(s), h, diss, theta));
+  return (int)hipGetLastError();
+}
+
+int mass_update(const isw_grid* g, int L, const double* M0, const double* Fm, double* M1, hipStream_t st) {
+  const double n_ = g->N, e_ = g->E, l_ = L;
+  ISW_LAUNCH("mass_update", 2 * l_ * n_ * D + l_ * e_ * D + 6 * n_ * I + e_ * I,
+             k_mass_update<<<g2(g->N, L), BS, 0, st>>>(*g, M0, Fm, M1));
+  return (int)hipGetLastError();
+}
+
+// Transport.step_many (transport.py:40-71) on rows stacked in psi; the result is written back into psi.
+int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0, const double* M1, const double* Fm,
+                   double* psi, int rows, int order, hipStream_t st) {
+  const int N = g->N, E = g->E, Lm = tr->L;
+  const double n_ = N, e_ = E, l_ = Lm, r_ = rows, mixed = ISW_FORM_MIXED ? 1 : 0;
+#if ISW_FORM_MIXED
+  float* pf = tr->pf;
+  const fast* wt = g->d2wtf;
+#else
+  float* pf = nullptr;
+  const fast* wt = g->d2wt;
+#endif
+  const double cellE = 6 * n_ * I + 2 * e_ * I;  // cellEdgeAsc, eI/eJ
+  const double sten = 8 * e_ * (R + I) + l_ * e_ * D + 2 * e_ * I;
+  ISW_LAUNCH("tr_td", 2 * l_ * n_ * D + r_ * n_ * D + l_ * e_ * D + cellE + r_ * n_ * D + mixed * r_ * n_ * 4,
+             k_tr_td<<<g2(N, rows), BS, 0, st>>>(*g, Lm, M0, psi, Fm, M1, tr->td, pf));
+  if (order == 1) {
+    ISW_LAUNCH("copy", 2 * r_ * n_ * D, k_copy<<<g1((long)N * rows), BS, 0, st>>>((long)N * rows, tr->td, psi));
+    return (int)hipGetLastError();
+  }
+#if ISW_FORM_MIXED
+  const fast* psf = pf;
+  const fast* p1f = pf;
+  const fast* p2f = pf;
+#else
+  const fast* psf = psi;
+  const fast* p1f = tr->p1;
+  const fast* p2f = tr->p2;
+#endif
+  ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
+             k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf));
+  ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
+                             mixed * r_ * n_ * 4,
+             k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf));
+  ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
+             k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf));
+  ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
+                             mixed * r_ * n_ * 4,
+             k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf));
+  ISW_LAUNCH("tr_aflux", 2 * r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
+             k_tr_aflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A));
+  ISW_LAUNCH("tr_ratios", 2 * r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 6 * n_ * I + cellE + 2 * r_ * n_ * D,
+             k_tr_ratios<<<g2(N, rows), BS, 0, st>>>(*g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm));
+  ISW_LAUNCH("tr_apply", r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 2 * r_ * n_ * D + cellE + r_ * n_ * D,
+             k_tr_apply<<<g2(N, rows), BS, 0, st>>>(*g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi));
+  return (int)hipGetLastError();
+}
+
 int swe_step(const isw_grid* g, const isw_layers* s, double* h, double* u, double* Fm, double dt, hipStream_t st) {
   const LP p = make_lp(s);
   const Div dv = divs(g);
   const int N = g->N, E = g->E, NV = g->NV, L = s->L;
+  const double n_ = N, e_ = E, v_ = NV, l_ = L;
   if (s->nu4 != 0) {
     lap(g, L, u, s->D1, s->Z1, s->L1, st);
     lap(g, L, s->L1, s->D1, s->Z1, s->L2, st);
@@ -1211,11 +1273,21 @@ int swe_step(const isw_grid* g, const isw_layers* s, double* h, double* u, doubl
   double* unv[3] = {s->u1, s->u2, u};
   for (int k = 0; k < 3; k++) {
     const double c = cs[k];
-    k_swe_scale<<<g2(N, L), BS, 0, st>>>(*g, c, hsv[k], usv[k], s->scale);
-    k_swe_flux<<<g2(E, L), BS, 0, st>>>(*g, k == 2 ? dt : 0.0, hsv[k], usv[k], s->scale, s->F, k == 2 ? Fm : nullptr);
-    k_swe_cell<<<g2(N, L), BS, 0, st>>>(*g, s->grav, c, hsv[k], s->b, s->F, usv[k], h, hnv[k], s->B);
-    k_swe_vertex<<<g2(NV, L), BS, 0, st>>>(*g, dv, hsv[k], usv[k], s->fV, s->q);
-    k_swe_edge_lp<<<g2(E, L), BS, 0, st>>>(*g, p, c, s->F, s->q, s->B, usv[k], s->L2, u, unv[k]);
+    ISW_LAUNCH("swe_scale", l_ * n_ * D + l_ * e_ * D + 6 * n_ * I + 2 * e_ * I + e_ * D + n_ * D + l_ * n_ * D,
+               k_swe_scale<<<g2(N, L), BS, 0, st>>>(*g, c, hsv[k], usv[k], s->scale));
+    ISW_LAUNCH("swe_flux", l_ * n_ * D + l_ * e_ * D + 2 * e_ * I + l_ * n_ * D + l_ * e_ * D +
+                               (k == 2 && Fm ? l_ * e_ * D + e_ * D : 0),
+               k_swe_flux<<<g2(E, L), BS, 0, st>>>(*g, k == 2 ? dt : 0.0, hsv[k], usv[k], s->scale, s->F,
+                                                   k == 2 ? Fm : nullptr));
+    ISW_LAUNCH("swe_cell", 2 * l_ * n_ * D + n_ * D + 2 * l_ * e_ * D + 6 * n_ * (I + C) + 2 * e_ * D + n_ * D +
+                               2 * l_ * n_ * D,
+               k_swe_cell<<<g2(N, L), BS, 0, st>>>(*g, s->grav, c, hsv[k], s->b, s->F, usv[k], h, hnv[k], s->B));
+    ISW_LAUNCH("swe_vertex", l_ * n_ * D + l_ * e_ * D + 3 * v_ * (2 * I + 2 * C) + 6 * n_ * D + e_ * D +
+                                 2 * v_ * D + l_ * v_ * D,
+               k_swe_vertex<<<g2(NV, L), BS, 0, st>>>(*g, dv, hsv[k], usv[k], s->fV, s->q));
+    ISW_LAUNCH("swe_edge", l_ * e_ * D + l_ * v_ * D + l_ * n_ * D + l_ * e_ * D + g->n_eoe * (D + I) +
+                               (e_ + 1) * I + 4 * e_ * I + e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + 2 * l_ * e_ * D,
+               k_swe_edge_lp<<<g2(E, L), BS, 0, st>>>(*g, p, c, s->F, s->q, s->B, usv[k], s->L2, u, unv[k]));
   }
   return (int)hipGetLastError();
 }
@@ -1223,24 +1295,32 @@ int swe_step(const isw_grid* g, const isw_layers* s, double* h, double* u, doubl
 int couple_stress(const isw_grid* g, int La, const double* ha, double* ua, const double* ho, double* uo, double c,
                   double rho_w, double dt, double* diss, double* tau_acc, const double* wall, double land_drag,
                   const double* cd, hipStream_t st) {
-  k_couple_stress<<<g1(g->E), BS, 0, st>>>(*g, La, c, rho_w, dt, land_drag, ha, ua, ho, uo, diss, tau_acc, wall, cd);
+  const double n_ = g->N, e_ = g->E;
+  ISW_LAUNCH("couple_stress", 2 * n_ * D + 4 * e_ * D + 2 * e_ * I + e_ * D + 2 * e_ * D + (wall ? e_ * D : 0) +
+                                  (cd ? n_ * D : 0),
+             k_couple_stress<<<g1(g->E), BS, 0, st>>>(*g, La, c, rho_w, dt, land_drag, ha, ua, ho, uo, diss, tau_acc,
+                                                      wall, cd));
   return (int)hipGetLastError();
 }
 
 int land_column(const isw_grid* g, int K, const double* land, double* T, const double* cap, double* Tg,
                 const double* dz, double c_soil, double k_soil, double dt, hipStream_t st) {
-  k_land_column<<<g1(g->N), BS, 0, st>>>(g->N, K, c_soil, k_soil, dt, land, T, cap, Tg, dz);
+  const double n_ = g->N;
+  ISW_LAUNCH("land_column", n_ * D + 2 * n_ * D + n_ * D + 2.0 * K * n_ * D + K * D,
+             k_land_column<<<g1(g->N), BS, 0, st>>>(g->N, K, c_soil, k_soil, dt, land, T, cap, Tg, dz));
   return (int)hipGetLastError();
 }
 
 int ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* cap, const double* land, double c_land,
             hipStream_t st) {
-  k_ocn_cap<<<g1(g->N), BS, 0, st>>>(g->N, rho_cw, c_land, h, cap, land);
+  const double n_ = g->N;
+  ISW_LAUNCH("ocn_cap", n_ * D + 2 * n_ * D + (land ? n_ * D : 0),
+             k_ocn_cap<<<g1(g->N), BS, 0, st>>>(g->N, rho_cw, c_land, h, cap, land));
   return (int)hipGetLastError();
 }
 
 int fill(double* x, long n, double v, hipStream_t st) {
-  if (n > 0) k_fill<<<g1(n), BS, 0, st>>>(n, v, x);
+  if (n > 0) ISW_LAUNCH("fill", 8.0 * n, k_fill<<<g1(n), BS, 0, st>>>(n, v, x));
   return (int)hipGetLastError();
 }
 

@@ -80,13 +80,15 @@ def lib():
             f = getattr(L, name)
             f.restype, f.argtypes = ctypes.c_int32, args
         L.isw_version.restype, L.isw_version.argtypes = ctypes.c_char_p, []
+        L.isw_timing_start.restype, L.isw_timing_start.argtypes = ctypes.c_int32, []
+        L.isw_timing_stop.restype, L.isw_timing_stop.argtypes = ctypes.c_int32, [ctypes.c_char_p, ctypes.c_int32]
         _lib = L
     return _lib
 
 
 EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_exchange', 'isw_vmix_heat',
            'isw_mass_update', 'isw_transport_step', 'isw_swe_step', 'isw_couple_stress', 'isw_land_column',
-           'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_version')
+           'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_timing_start', 'isw_timing_stop', 'isw_version')
 
 
 def check(rc, what):
@@ -109,3 +111,24 @@ def stream():
     """The current torch stream as a hipStream_t."""
     import torch
     return torch.cuda.current_stream().cuda_stream
+
+
+def timing_start():
+    """Switch the library's per-kernel event timing on (direct launches only)."""
+    lib().isw_timing_start()
+
+
+def timing_stop():
+    """Switch it off; per kernel name: dict(launches, ms, bytes) summed over the launches since timing_start()."""
+    cap = 1 << 20
+    buf = ctypes.create_string_buffer(cap)
+    n = lib().isw_timing_stop(buf, cap)
+    if n == -1:
+        raise HipError('isw_timing_stop: event query failed')
+    if n < 0:
+        raise HipError(f'isw_timing_stop: the report needs {-n} bytes')
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms, by = line.split()
+        out[name] = dict(launches=int(cnt), ms=float(ms), bytes=float(by))
+    return out

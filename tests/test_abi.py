@@ -1,0 +1,45 @@
+"""The C ABI (CPU): libinsilica_hip.so loads, exports every entry point include/insilica_hip.h declares, and rejects
+bad arguments on the host with ISW_EBADARG before any HIP call (no GPU needed for that path)."""
+import ctypes
+import re
+from pathlib import Path
+
+from insilica import _hip
+
+HEADER = Path(__file__).resolve().parent.parent / 'include' / 'insilica_hip.h'
+
+
+def declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r'^\S.*?\b(isw_[a-z0-9_]+)\(', text, flags=re.M)))
+
+
+def test_exports_every_declared_symbol():
+    names = declared()
+    assert 'isw_layers_step' in names and 'isw_transport_step' in names
+    lib = _hip.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_hip.EXPORTS)
+    assert lib.isw_version().decode().endswith('gfx950')
+
+
+def test_bad_arguments_rejected_on_host():
+    lib = _hip.lib()
+    E = _hip.ISW_EBADARG
+    assert lib.isw_layers_step(None, None, None, None, None, None, 1.0, 1, None) == E
+    assert lib.isw_transport_step(None, None, None, None, None, None, 3, 3, None) == E
+    assert lib.isw_swe_step(None, None, None, None, None, 1.0, None) == E
+    assert lib.isw_dsum(None, None, 0, None, None, None) == E
+    assert lib.isw_csum(None, None, 10, None, None, None) == E
+    assert lib.isw_fill(None, 5, 0.0, None) == E
+    assert lib.isw_land_column(None, 3, None, None, None, None, None, 1.0, 1.0, 1.0, None) == E
+    # a grid struct with sizes but a layer count beyond the maximum, and a transport with rows not a multiple of L
+    g = _hip.IswGrid()
+    g.N, g.E, g.NV = 12, 30, 20
+    lay = _hip.IswLayers()
+    lay.L = 9
+    assert lib.isw_layers_step(ctypes.byref(g), ctypes.byref(lay), None, None, None, None, 1.0, 1, None) == E
+    tr = _hip.IswTransport()
+    tr.L, tr.rows_max = 2, 4
+    assert lib.isw_transport_step(ctypes.byref(g), ctypes.byref(tr), None, None, None, None, 3, 3, None) == E

@@ -208,12 +208,14 @@ def test_couple_land_cap_fp64(g4):
     uo = rng.uniform(-1, 1, g.E)
     wall = (rng.random(g.E) > 0.1).astype(float)
     cd = rng.uniform(1e-3, 3e-2, g.N)
+    # the device inputs are held in named tensors until the kernels have run: a temporary's memory may be reused
+    had, hod, walld, cdd = dg.asarray(ha), dg.asarray(ho), dg.asarray(wall), dg.asarray(cd)
     for use_cd in (False, True):
         d = [dg.asarray(x) for x in (ua, uo, np.zeros(g.E), np.ones(g.E))]
         o = [x.copy() for x in (ua, uo, np.zeros(g.E), np.ones(g.E))]
-        _hip.call('isw_couple_stress', dg.ref, 3, _hip.ptr(dg.asarray(ha)), _hip.ptr(d[0]), _hip.ptr(dg.asarray(ho)),
-                  _hip.ptr(d[1]), 0.017, 1025.0, 450.0, _hip.ptr(d[2]), _hip.ptr(d[3]), _hip.ptr(dg.asarray(wall)), 1.0,
-                  _hip.ptr(dg.asarray(cd) if use_cd else None), _hip.stream())
+        _hip.call('isw_couple_stress', dg.ref, 3, _hip.ptr(had), _hip.ptr(d[0]), _hip.ptr(hod), _hip.ptr(d[1]),
+                  0.017, 1025.0, 450.0, _hip.ptr(d[2]), _hip.ptr(d[3]), _hip.ptr(walld), 1.0,
+                  _hip.ptr(cdd if use_cd else None), _hip.stream())
         oracle.couple_stress(og, 3, ha, o[0], ho, o[1], 0.017, 1025.0, 450.0, o[2], o[3], wall=wall, land_drag=1.0,
                              cd=cd if use_cd else None)
         torch.cuda.synchronize()
@@ -225,16 +227,16 @@ def test_couple_land_cap_fp64(g4):
     cap = rng.uniform(1e5, 4e6, g.N)
     Tg = rng.uniform(260, 300, (K, g.N))
     dz = 0.05 * 1.5 ** np.arange(K)
-    Td, Tgd = dg.asarray(T), dg.asarray(Tg)
-    _hip.call('isw_land_column', dg.ref, K, _hip.ptr(dg.asarray(land)), _hip.ptr(Td), _hip.ptr(dg.asarray(cap)),
-              _hip.ptr(Tgd), _hip.ptr(dg.asarray(dz)), 2.0e6, 1.0, 450.0, _hip.stream())
+    Td, Tgd, landd, capd0, dzd = dg.asarray(T), dg.asarray(Tg), dg.asarray(land), dg.asarray(cap), dg.asarray(dz)
+    _hip.call('isw_land_column', dg.ref, K, _hip.ptr(landd), _hip.ptr(Td), _hip.ptr(capd0), _hip.ptr(Tgd),
+              _hip.ptr(dzd), 2.0e6, 1.0, 450.0, _hip.stream())
     To, Tgo = T.copy(), Tg.copy()
     oracle.land_column(og, K, land, To, cap, Tgo, dz, 2.0e6, 1.0, 450.0)
     torch.cuda.synchronize()
     assert relmax(_np(Td), To) <= TOL1 and relmax(_np(Tgd), Tgo) <= TOL1
     capd, capo = dg.asarray(np.where(land > 0.5, 0.0, 1.0)), np.where(land > 0.5, 0.0, 1.0)
-    _hip.call('isw_ocn_cap', dg.ref, _hip.ptr(dg.asarray(ho)), 1025.0 * 3990.0, _hip.ptr(capd),
-              _hip.ptr(dg.asarray(land)), 2.0e6, _hip.stream())
+    _hip.call('isw_ocn_cap', dg.ref, _hip.ptr(hod), 1025.0 * 3990.0, _hip.ptr(capd), _hip.ptr(landd), 2.0e6,
+              _hip.stream())
     oracle.ocn_cap(og, ho, 1025.0 * 3990.0, capo, land, 2.0e6)
     torch.cuda.synchronize()
     assert np.array_equal(_np(capd), capo)
