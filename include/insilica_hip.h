@@ -54,6 +54,7 @@ typedef struct isw_grid {
  * b (N): the bottom. wall (E, nullable): 0 on the edges no water crosses. Gc (N): the fast mode's coefficient per cell.
  * Scratch sizes -- (L,N): B, scale, h1, h2, D1, hold, pibar, S, muf, mup, phi; (L,NV): q, Z1;
  * (L,E): Tp, F, u1, u2, L1, L2, hedge, uf, accL; (E): G, He, ubs, ub0, ub, acc, diss; (N): eta0, eta, eta_prev;
+ * ufb (L,E) and ubb (E): the second buffers of the fast sub-steps' winds (the sub-steps alternate between two);
  * pk: 3*L*E doubles (plain form); pkf: 3*L*E floats (mixed form; NULL otherwise). */
 typedef struct isw_layers {
   int32_t L, air, prec, pad_;
@@ -62,7 +63,7 @@ typedef struct isw_layers {
   const double *fV, *b, *wall;
   double *Gc;
   double *B, *q, *Tp, *F, *scale, *G, *He, *ubs, *ub0, *ub, *acc, *eta0, *eta, *eta_prev, *h1, *h2, *u1, *u2;
-  double *D1, *Z1, *L1, *L2, *hold, *hedge, *diss, *pibar, *S, *muf, *mup, *phi, *uf, *accL, *pk;
+  double *D1, *Z1, *L1, *L2, *hold, *hedge, *diss, *pibar, *S, *muf, *mup, *phi, *uf, *accL, *pk, *ufb, *ubb;
   float *pkf;
 } isw_layers;
 

@@ -51,8 +51,8 @@ bool layers_ok(const isw_grid* g, const isw_layers* s) {
   if (s->prec == ISW_FP64 && !s->pk) return false;
   return s->fV && s->b && s->Gc && s->B && s->q && s->Tp && s->G && s->He && s->ubs && s->scale && s->h1 && s->h2 &&
          s->u1 && s->u2 && s->D1 && s->Z1 && s->L1 && s->L2 && s->hedge && s->diss && s->hold &&
-         (!s->air || (s->pibar && s->S && s->muf && s->phi && s->uf && s->accL)) &&
-         (s->air || (s->ub0 && s->ub && s->acc && s->eta0 && s->eta && s->eta_prev));
+         (!s->air || (s->pibar && s->S && s->muf && s->mup && s->uf && s->ufb && s->accL)) &&
+         (s->air || (s->ub0 && s->ub && s->ubb && s->acc && s->eta0 && s->eta && s->eta_prev));
 }
 
 }  // namespace

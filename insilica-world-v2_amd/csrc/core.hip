@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 
 #include "insilica_hip.h"
 #include "diag.hpp"
@@ -37,6 +38,30 @@ constexpr int BS = 256;
 
 static inline dim3 g1(long n) { return dim3((unsigned)((n + BS - 1) / BS)); }
 static inline dim3 g2(long n, int y) { return dim3((unsigned)((n + BS - 1) / BS), (unsigned)y); }
+
+// Passes over cells or vertices (40,962 / 81,920 at level 6) launch 64-thread blocks (one wave each) so that they
+// still spread over all 256 CUs; column passes launch (64, L) blocks: 64 cells or edges, one thread row per layer.
+constexpr int BS1 = 64;
+static inline dim3 c1(long n) { return dim3((unsigned)((n + BS1 - 1) / BS1)); }
+static inline dim3 c2(long n, int y) { return dim3((unsigned)((n + BS1 - 1) / BS1), (unsigned)y); }
+
+// XCD-aware block order: the dispatcher deals consecutive blocks to the 8 XCDs in turn; renumbered so that each XCD
+// takes one contiguous range of blocks, neighbouring cells (Hilbert-ordered) then gather through the same L2.
+__device__ __forceinline__ int xblock() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+#define XTID ((int)(xblock() * blockDim.x + threadIdx.x))
+
+// A cell's six table entries (edges or neighbours; pads -1 last) in three 8-byte loads, so that every gather that
+// depends on them can be issued at once; sums then skip the pads in table order, as the reference's loops do.
+__device__ __forceinline__ void ring6(const int* __restrict__ tab, int i, int (&e)[6]) {
+  const int2* p = reinterpret_cast<const int2*>(tab + (size_t)i * 6);
+  const int2 a = p[0], b = p[1], c = p[2];
+  e[0] = a.x; e[1] = a.y; e[2] = b.x; e[3] = b.y; e[4] = c.x; e[5] = c.y;
+}
+__device__ __forceinline__ int ok0(int e) { return e < 0 ? 0 : e; }
 
 // Settings of a layer stack passed by value (reference: Layers.__init__, src/insilica/layers.py:161-218).
 struct LP {
@@ -82,14 +107,18 @@ __device__ __forceinline__ double edge_h(double a, double b, double shelf) {
   return shelf > 0 && m < shelf ? m : 0.5 * (a + b);
 }
 
-__device__ __forceinline__ double cell_ke(const isw_grid& g, const double* __restrict__ ul, int i) {
-  double ke = 0;
+// A cell's kinetic energy times its area, sum over its edges of l d u^2 / 4, in table order.
+__device__ __forceinline__ double cell_ke(const isw_grid& g, const double* __restrict__ ul, const int (&e)[6]) {
+  double t[6];
 #pragma unroll
   for (int m = 0; m < 6; m++) {
-    const int e = g.cellEdge[i * 6 + m];
-    if (e < 0) break;
-    ke += g.le[e] * g.de[e] * 0.25 * ul[e] * ul[e];
+    const int ek = ok0(e[m]);
+    t[m] = g.le[ek] * g.de[ek] * 0.25 * ul[ek] * ul[ek];
   }
+  double ke = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++)
+    if (e[m] >= 0) ke += t[m];
   return ke;
 }
 
@@ -103,13 +132,15 @@ __global__ __launch_bounds__(BS) void k_div_curl(isw_grid g, int L, int nbc, con
     if (t >= g.N * L) return;
     const int l = t / g.N, i = t - l * g.N;
     const double* ul = u + (size_t)l * g.E;
+    int e[6];
+    ring6(g.cellEdge, i, e);
+    double tm[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) tm[k] = (double)g.cellSign[i * 6 + k] * g.le[ok0(e[k])] * ul[ok0(e[k])];
     double s = 0;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      const int e = g.cellEdge[i * 6 + k];
-      if (e < 0) break;
-      s += (double)g.cellSign[i * 6 + k] * g.le[e] * ul[e];
-    }
+    for (int k = 0; k < 6; k++)
+      if (e[k] >= 0) s += tm[k];
     D[t] = s / g.areaCell[i];
   } else {
     const int t = (int)((blockIdx.x - nbc) * blockDim.x + threadIdx.x);
@@ -128,7 +159,7 @@ __global__ __launch_bounds__(BS) void k_div_curl(isw_grid g, int L, int nbc, con
 
 __global__ __launch_bounds__(BS) void k_edge_lap(isw_grid g, int L, const double* __restrict__ D,
                                                  const double* __restrict__ Z, double* __restrict__ out) {
-  const int t = TID;
+  const int t = XTID;
   if (t >= g.E * L) return;
   const int l = t / g.E, e = t - l * g.E;
   const double* Dl = D + (size_t)l * g.N;
@@ -142,7 +173,7 @@ __global__ __launch_bounds__(BS) void k_edge_lap(isw_grid g, int L, const double
 // ---------------------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BS) void k_air_stiff(int N, LP p, const double* __restrict__ h,
                                                   const double* __restrict__ th, double* __restrict__ S) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= N) return;
   double pr = 0;
   for (int k = 0; k < p.L; k++) {
@@ -155,7 +186,7 @@ __global__ __launch_bounds__(BS) void k_air_stiff(int N, LP p, const double* __r
 
 __global__ __launch_bounds__(BS) void k_air_gfast(int N, LP p, double rgas, const double* __restrict__ h,
                                                   const double* __restrict__ th, double* __restrict__ Gc) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= N) return;
   const int L = p.L;
   double mu = 0;
@@ -175,7 +206,7 @@ __global__ __launch_bounds__(BS) void k_air_gfast(int N, LP p, double rgas, cons
 
 __global__ __launch_bounds__(BS) void k_lay_eta(int N, int L, const double* __restrict__ h,
                                                 const double* __restrict__ b, double* __restrict__ eta) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= N) return;
   double z = b[i];
   for (int k = L - 1; k >= 0; k--) z += h[(size_t)k * N + i];
@@ -185,7 +216,7 @@ __global__ __launch_bounds__(BS) void k_lay_eta(int N, int L, const double* __re
 __global__ __launch_bounds__(BS) void k_lay_ubar(isw_grid g, int L, double shelf, const double* __restrict__ h,
                                                  const double* __restrict__ u, const double* __restrict__ wall,
                                                  double* __restrict__ ubar) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int N = g.N, eI = g.eI[e], eJ = g.eJ[e];
   double H = 0, U = 0;
@@ -199,73 +230,72 @@ __global__ __launch_bounds__(BS) void k_lay_ubar(isw_grid g, int L, double shelf
 
 // ---------------------------------------------------------------------------------------------------------------
 // Per stage, pass 1: the Bernoulli/Montgomery potential per cell (air_bern / lay_bern, layers.cu:83-158) and the PV
-// at vertices (swe_vertex, fused.cu:82-98), in one launch: blocks [0, nbc) take cells, the rest (vertex, layer).
+// at vertices (swe_vertex, fused.cu:82-98), in one launch of (64, L) blocks: blocks [0, nbc) take 64 cells, the rest
+// 64 vertices; thread row y is layer y. For air each thread takes one interface's Exner power (the expensive `pow`),
+// shared with the column through LDS; every thread then replays the column's sums in the reference's order.
 // ---------------------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_bern_pv(isw_grid g, LP p, Div dv, int nbc, const double* __restrict__ h,
-                                                const double* __restrict__ u, const double* __restrict__ th,
-                                                const double* __restrict__ b, const double* __restrict__ S,
-                                                const double* __restrict__ fV, double* __restrict__ B,
-                                                double* __restrict__ pibar, double* __restrict__ q) {
+__global__ __launch_bounds__(512) void k_bern_pv(isw_grid g, LP p, Div dv, int nbc, const double* __restrict__ h,
+                                                 const double* __restrict__ u, const double* __restrict__ th,
+                                                 const double* __restrict__ b, const double* __restrict__ S,
+                                                 const double* __restrict__ fV, double* __restrict__ B,
+                                                 double* __restrict__ pibar, double* __restrict__ q) {
   const int N = g.N, E = g.E, L = p.L;
+  const int j = threadIdx.x, k = threadIdx.y;
   if ((int)blockIdx.x < nbc) {
-    const int i = TID;
-    if (i >= N) return;
+    __shared__ double sx[ISW_LMAX][BS1];
+    const int i = blockIdx.x * BS1 + j;
+    const bool live = i < N;
+    const int ic = live ? i : 0;
+    int e[6];
+    ring6(g.cellEdge, ic, e);
+    const double ke = cell_ke(g, u + (size_t)k * E, e);
     if (p.air) {
-      double xb[ISW_LMAX], Mk[ISW_LMAX];
-      double pr = 0, xt = 0;
-      for (int k = 0; k < L; k++) {
-        const double hk = h[(size_t)k * N + i];
-        const double pt = pr;
-        pr = pr + p.grav * hk;
-        const double x = pow(pr / p.p0, p.kappa);
-        xb[k] = x;
-        pibar[(size_t)k * N + i] = p.cp * (pr * x - pt * xt) / ((p.kappa + 1) * (pr - pt));
-        xt = x;
+      double pr = 0, pt = 0;
+      for (int m = 0; m <= k; m++) {
+        pt = pr;
+        pr = pr + p.grav * h[(size_t)m * N + ic];
       }
-      double M = p.grav * b[i] + th[(size_t)(L - 1) * N + i] * p.cp * xb[L - 1];
-      Mk[L - 1] = M;
-      for (int k = L - 2; k >= 0; k--) {
-        M = M + (th[(size_t)k * N + i] - th[(size_t)(k + 1) * N + i]) * p.cp * xb[k];
-        Mk[k] = M;
-      }
-      double P[ISW_LMAX], Phi[ISW_LMAX];
+      const double x = pow(pr / p.p0, p.kappa);
+      sx[k][j] = x;
+      __syncthreads();
+      if (!live) return;
+      const double xt = k ? sx[k - 1][j] : 0.0;
+      pibar[(size_t)k * N + i] = p.cp * (pr * x - pt * xt) / ((p.kappa + 1) * (pr - pt));
+      double M = p.grav * b[i] + th[(size_t)(L - 1) * N + i] * p.cp * sx[L - 1][j];
+      for (int m = L - 2; m >= k; m--) M = M + (th[(size_t)m * N + i] - th[(size_t)(m + 1) * N + i]) * p.cp * sx[m][j];
+      double P[ISW_LMAX];
       double a = 0;
-      for (int k = 0; k < L; k++) { a += h[(size_t)k * N + i]; P[k] = a; }
+      for (int m = 0; m < L; m++) { a += h[(size_t)m * N + i]; P[m] = a; }
       double ph = 0;
-      for (int k = L - 1; k >= 0; k--) { ph += S[(size_t)k * N + i] * P[k]; Phi[k] = ph; }
-      for (int k = 0; k < L; k++) {
-        const double ke = cell_ke(g, u + (size_t)k * E, i);
-        B[(size_t)k * N + i] = Mk[k] - Phi[k] + DIVBY(ke, dv.A[i]);
-      }
+      for (int m = L - 1; m >= k; m--) ph += S[(size_t)m * N + i] * P[m];
+      B[(size_t)k * N + i] = M - ph + DIVBY(ke, dv.A[i]);
     } else {
+      if (!live) return;
       double z[ISW_LMAX];
       double s = b[i];
-      for (int k = L - 1; k >= 0; k--) { s += h[(size_t)k * N + i]; z[k] = s; }
+      for (int m = L - 1; m >= 0; m--) { s += h[(size_t)m * N + i]; z[m] = s; }
       double Mp = 0;
-      for (int k = 0; k < L; k++) {
-        if (k > 0) Mp += p.gp[k - 1] * z[k];
-        const double ke = cell_ke(g, u + (size_t)k * E, i);
-        B[(size_t)k * N + i] = Mp + ke / g.areaCell[i];
-      }
+      for (int m = 1; m <= k; m++) Mp += p.gp[m - 1] * z[m];
+      B[(size_t)k * N + i] = Mp + ke / g.areaCell[i];
     }
   } else {
-    const int t = (int)((blockIdx.x - nbc) * blockDim.x + threadIdx.x);
-    if (t >= g.NV * L) return;
-    const int l = t / g.NV, v = t - l * g.NV;
+    const int v = ((int)blockIdx.x - nbc) * BS1 + j;
+    if (v >= g.NV) return;
+    const int l = k;
     const double* hl = h + (size_t)l * N;
     const double* ul = u + (size_t)l * E;
     double hv = 0, z = 0;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const int c = g.vCell[v * 3 + k];
-      hv += g.kite[c * 6 + g.vSlot[v * 3 + k]] * hl[c] * 1.0;
+    for (int m = 0; m < 3; m++) {
+      const int c = g.vCell[v * 3 + m];
+      hv += g.kite[c * 6 + g.vSlot[v * 3 + m]] * hl[c] * 1.0;
     }
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const int e = g.vEdge[v * 3 + k];
-      z += (double)g.vEdgeSign[v * 3 + k] * g.de[e] * ul[e];
+    for (int m = 0; m < 3; m++) {
+      const int ee = g.vEdge[v * 3 + m];
+      z += (double)g.vEdgeSign[v * 3 + m] * g.de[ee] * ul[ee];
     }
-    q[t] = (DIVBY(z, dv.AV[v]) + fV[v]) / DIVBY(hv, dv.AV[v]);
+    q[(size_t)l * g.NV + v] = (DIVBY(z, dv.AV[v]) + fV[v]) / DIVBY(hv, dv.AV[v]);
   }
 }
 
@@ -274,7 +304,7 @@ __global__ __launch_bounds__(BS) void k_bern_pv(isw_grid g, LP p, Div dv, int nb
 __global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf, const double* __restrict__ h,
                                                 const double* __restrict__ q, const double* __restrict__ u,
                                                 double* __restrict__ he, fast* __restrict__ pk) {
-  const int t = TID;
+  const int t = XTID;
   if (t >= g.E * L) return;
   const int l = t / g.E, e = t - l * g.E;
   const int N = g.N, NV = g.NV;
@@ -287,134 +317,166 @@ __global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf,
 
 // Per stage, pass 3 (lay_slow, layers.cu:269-359): the slow momentum tendency of every layer of an edge (W-weight PV
 // flux with the thin-layer cap, -grad B, drag, exact-exponential stress, pibar grad theta for air, del^4), its
-// thickness-weighted column mean G, T' = T - G, H_e and the column wind. The eoe neighbours are walked once for all
-// layers (each layer's sum still runs in ascending neighbour order).
-__global__ __launch_bounds__(BS) void k_slow(isw_grid g, LP p, Div dv, double dts, const fast* __restrict__ pk,
-                                             const fast* __restrict__ wf, const double* __restrict__ hedge,
-                                             const double* __restrict__ u, const double* __restrict__ B,
-                                             const double* __restrict__ lap4, const double* __restrict__ th,
-                                             const double* __restrict__ pibar, double* __restrict__ Tp,
-                                             double* __restrict__ G, double* __restrict__ He,
-                                             double* __restrict__ ubs) {
-  const int e = TID;
-  if (e >= g.E) return;
+// thickness-weighted column mean G, T' = T - G, H_e and the column wind. (64, L) blocks: thread (x, y) is layer y of
+// one of 64 edges; the column's sums run through LDS in the reference's layer order. The ~10 eoe neighbours' indices
+// and weights are loaded before the sum so that their packed records are gathered together.
+__global__ __launch_bounds__(512) void k_slow(isw_grid g, LP p, Div dv, double dts, const fast* __restrict__ pk,
+                                              const fast* __restrict__ wf, const double* __restrict__ hedge,
+                                              const double* __restrict__ u, const double* __restrict__ B,
+                                              const double* __restrict__ lap4, const double* __restrict__ th,
+                                              const double* __restrict__ pibar, double* __restrict__ Tp,
+                                              double* __restrict__ G, double* __restrict__ He,
+                                              double* __restrict__ ubs) {
+  __shared__ double sT[ISW_LMAX][BS1], sh[ISW_LMAX][BS1], su[ISW_LMAX][BS1], sq[ISW_LMAX][BS1];
   const int N = g.N, E = g.E, L = p.L;
-  const int eI = g.eI[e], eJ = g.eJ[e];
-  fast Q[ISW_LMAX], cap[ISW_LMAX], qf[ISW_LMAX];
-  for (int k = 0; k < L; k++) {
-    const double he = hedge[(size_t)k * E + e];
-    cap[k] = (fast)(p.thin * he);
-    qf[k] = pk[((size_t)e * L + k) * 3 + 2];
-    Q[k] = 0;
+  const int j = threadIdx.x, k = threadIdx.y;
+  const int e = xblock() * BS1 + j;
+  const bool live = e < E;
+  const int ec = live ? e : 0;
+  const int eI = g.eI[ec], eJ = g.eJ[ec];
+  const size_t o = (size_t)k * E + ec;
+  const double he = hedge[o];
+  const double ue = u[o];
+  const fast cap = (fast)(p.thin * he), qf = pk[((size_t)ec * L + k) * 3 + 2];
+  fast Q = 0;
+  const int m0 = g.eoeStart[ec], m1 = g.eoeStart[ec + 1];
+  constexpr int NB = 10;
+  int nb[NB];
+  fast w[NB];
+#pragma unroll
+  for (int m = 0; m < NB; m++) {
+    const int mm = m0 + m < m1 ? m0 + m : (m1 > m0 ? m0 : 0);
+    nb[m] = g.eoe[mm];
+    w[m] = wf[mm];
   }
-  const int m0 = g.eoeStart[e], m1 = g.eoeStart[e + 1];
-  for (int m = m0; m < m1; m++) {
-    const fast* pp = pk + (size_t)g.eoe[m] * L * 3;
-    const fast w = wf[m];
-    for (int k = 0; k < L; k++) {
-      fast h2 = pp[3 * k];
-      if (h2 > cap[k]) h2 = cap[k];
-      const fast F2 = h2 * pp[3 * k + 1];
-      Q[k] += w * F2 * (fast)0.5 * (qf[k] + pp[3 * k + 2]);
-    }
+  fast h2v[NB], u2v[NB], q2v[NB];
+#pragma unroll
+  for (int m = 0; m < NB; m++) {
+    const fast* pp = pk + ((size_t)nb[m] * L + k) * 3;
+    h2v[m] = pp[0]; u2v[m] = pp[1]; q2v[m] = pp[2];
   }
-  double T[ISW_LMAX], hev[ISW_LMAX], sqk[ISW_LMAX];
-  double H = 0, U = 0;
-  for (int k = 0; k < L; k++) {
-    const size_t o = (size_t)k * E + e;
-    const double he = hedge[o];
-    const double ue = u[o];
-    const double shk = p.shelf;
-    const double sq0 =
-        shk > 0 && he < shk ? (he - 2 * p.floor_ > 0 ? (he - 2 * p.floor_) / (shk - 2 * p.floor_) : 0.0) : 1.0;
-    const double sq = th ? 1.0 : sq0;
-    sqk[k] = sq0;
-    const double* Bl = B + (size_t)k * N;
-    double d = Q[k] - DIVBY(sq * (Bl[eJ] - Bl[eI]), dv.de[e]) - (p.drag[k] + (1 - sq) / 21600.0) * ue;
-    const double ck = p.stress[k];
-    if (ck != 0) {
-      const double r = ck / (he > p.floor_ ? he : p.floor_);
-      d = d - ((r * dts > 1e-8) ? (1 - exp(-r * dts)) / dts : r) * ue;
-    }
-    if (th) {
-      const double* tl = th + (size_t)k * N;
-      const double* pl = pibar + (size_t)k * N;
-      d = d + DIVBY(0.5 * (pl[eI] + pl[eJ]) * (tl[eJ] - tl[eI]), dv.de[e]);
-    }
-    if (p.nu4 != 0) d = d - p.nu4 * lap4[o];
-    T[k] = d;
-    hev[k] = he;
-    H += he;
-    U += he * ue;
+#pragma unroll
+  for (int m = 0; m < NB; m++) {
+    if (m0 + m >= m1) break;
+    fast h2 = h2v[m];
+    if (h2 > cap) h2 = cap;
+    const fast F2 = h2 * u2v[m];
+    Q += w[m] * F2 * (fast)0.5 * (qf + q2v[m]);
   }
-  if (th && p.shelf > 0) {
-    double Tn[ISW_LMAX];
-    for (int k = 0; k < L; k++) {
-      Tn[k] = T[k];
-      if (sqk[k] >= 1) continue;
-      double w = 0, s = 0;
-      for (int j = k - 1; j >= 0; j--) if (sqk[j] >= 1) { w += hev[j]; s += hev[j] * T[j]; break; }
-      for (int j = k + 1; j < L; j++) if (sqk[j] >= 1) { w += hev[j]; s += hev[j] * T[j]; break; }
-      if (w > 0) Tn[k] = sqk[k] * T[k] + (1 - sqk[k]) * s / w;
-    }
-    for (int k = 0; k < L; k++) T[k] = Tn[k];
+  for (int m = m0 + NB; m < m1; m++) {             // (meshes with more neighbours per edge than 10)
+    const fast* pp = pk + ((size_t)g.eoe[m] * L + k) * 3;
+    fast h2 = pp[0];
+    if (h2 > cap) h2 = cap;
+    const fast F2 = h2 * pp[1];
+    Q += wf[m] * F2 * (fast)0.5 * (qf + pp[2]);
   }
-  double GT = 0;
-  for (int k = 0; k < L; k++) GT += hev[k] * T[k];
+  const double shk = p.shelf;
+  const double sq0 =
+      shk > 0 && he < shk ? (he - 2 * p.floor_ > 0 ? (he - 2 * p.floor_) / (shk - 2 * p.floor_) : 0.0) : 1.0;
+  const double sqv = th ? 1.0 : sq0;
+  const double* Bl = B + (size_t)k * N;
+  double d = Q - DIVBY(sqv * (Bl[eJ] - Bl[eI]), dv.de[ec]) - (p.drag[k] + (1 - sqv) / 21600.0) * ue;
+  const double ck = p.stress[k];
+  if (ck != 0) {
+    const double r = ck / (he > p.floor_ ? he : p.floor_);
+    d = d - ((r * dts > 1e-8) ? (1 - exp(-r * dts)) / dts : r) * ue;
+  }
+  if (th) {
+    const double* tl = th + (size_t)k * N;
+    const double* pl = pibar + (size_t)k * N;
+    d = d + DIVBY(0.5 * (pl[eI] + pl[eJ]) * (tl[eJ] - tl[eI]), dv.de[ec]);
+  }
+  if (p.nu4 != 0) d = d - p.nu4 * lap4[o];
+  sT[k][j] = d;
+  sh[k][j] = he;
+  su[k][j] = ue;
+  sq[k][j] = sq0;
+  __syncthreads();
+  if (!live) return;
+  double T = d;
+  if (th && p.shelf > 0 && sq0 < 1) {
+    double wgt = 0, s = 0;
+    for (int m = k - 1; m >= 0; m--) if (sq[m][j] >= 1) { wgt += sh[m][j]; s += sh[m][j] * sT[m][j]; break; }
+    for (int m = k + 1; m < L; m++) if (sq[m][j] >= 1) { wgt += sh[m][j]; s += sh[m][j] * sT[m][j]; break; }
+    if (wgt > 0) T = sq0 * d + (1 - sq0) * s / wgt;
+  }
+  // the column mean takes every layer's (blended) tendency: a blend reads only unblended neighbours, so each
+  // layer's value is recomputed here from LDS in the same way
+  double H = 0, U = 0, GT = 0;
+  for (int m = 0; m < L; m++) {
+    H += sh[m][j];
+    U += sh[m][j] * su[m][j];
+  }
+  for (int m = 0; m < L; m++) {
+    double Tm = sT[m][j];
+    if (th && p.shelf > 0 && sq[m][j] < 1) {
+      double wgt = 0, s = 0;
+      for (int q2 = m - 1; q2 >= 0; q2--) if (sq[q2][j] >= 1) { wgt += sh[q2][j]; s += sh[q2][j] * sT[q2][j]; break; }
+      for (int q2 = m + 1; q2 < L; q2++) if (sq[q2][j] >= 1) { wgt += sh[q2][j]; s += sh[q2][j] * sT[q2][j]; break; }
+      if (wgt > 0) Tm = sq[m][j] * sT[m][j] + (1 - sq[m][j]) * s / wgt;
+    }
+    GT += sh[m][j] * Tm;
+  }
   const double gg = GT / H;
-  for (int k = 0; k < L; k++) Tp[(size_t)k * E + e] = T[k] - gg;
-  G[e] = gg;
-  He[e] = H;
-  ubs[e] = U / H;
+  Tp[o] = T - gg;
+  if (k == 0) {
+    G[e] = gg;
+    He[e] = H;
+    ubs[e] = U / H;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
 // Air fast sub-steps (layers.cu:182-224): mass then wind, forward-backward, from the stage's start.
 // The first sub-step reads the step's start (mu from h0, u from u0) and starts acc; the last writes the stage's wind.
 // ---------------------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_af_muphi(isw_grid g, LP p, Div dv, double tau, const double* __restrict__ hedge,
-                                                 const double* __restrict__ uf, const double* mu_src,
-                                                 double* mu, const double* __restrict__ S,
-                                                 double* __restrict__ phi) {
-  const int i = TID;
-  if (i >= g.N) return;
+// The mass sub-step: (64, L) blocks, thread (x, y) updates layer y's mass of one of 64 cells (its six edges gathered
+// at once); the column's forward-weighted masses and the linear potential Phi* are then summed through LDS in the
+// reference's order (P from the top, Phi from the bottom) by every thread for its own layer.
+__global__ __launch_bounds__(512) void k_af_muphi(isw_grid g, LP p, Div dv, double tau, const double* __restrict__ hedge,
+                                                  const double* __restrict__ uf, const double* mu_src, double* mu,
+                                                  const double* __restrict__ S, double* __restrict__ phi) {
+  __shared__ double sa[ISW_LMAX][BS1], sS[ISW_LMAX][BS1];
   const int N = g.N, E = g.E, L = p.L;
-  int ce[6];
-  double cs[6];
-  int deg = 6;
+  const int j = threadIdx.x, k = threadIdx.y;
+  const int i = xblock() * BS1 + j;
+  const bool live = i < N;
+  const int ic = live ? i : 0;
+  int e[6];
+  ring6(g.cellEdge, ic, e);
+  const double* hl = hedge + (size_t)k * E;
+  const double* ul = uf + (size_t)k * E;
+  double t[6];
 #pragma unroll
   for (int m = 0; m < 6; m++) {
-    ce[m] = g.cellEdge[i * 6 + m];
-    cs[m] = (double)g.cellSign[i * 6 + m];
+    const int ek = ok0(e[m]);
+    t[m] = (double)g.cellSign[ic * 6 + m] * g.le[ek] * hl[ek] * ul[ek];
   }
-  if (ce[5] < 0) deg = 5;
+  double div = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++)
+    if (e[m] >= 0) div += t[m];
+  const size_t o = (size_t)k * N + ic;
+  const double m0 = mu_src[o];
+  const double m1 = m0 - DIVBY(tau * div, dv.A[ic]);
+  sa[k][j] = m1 + p.div_damp * (m1 - m0);
+  sS[k][j] = S[o];
+  __syncthreads();
+  if (!live) return;
+  mu[o] = m1;
   double P[ISW_LMAX];
   double a = 0;
-  const double rA = dv.A[i];
-  for (int k = 0; k < L; k++) {
-    const double* hl = hedge + (size_t)k * E;
-    const double* ul = uf + (size_t)k * E;
-    const size_t o = (size_t)k * N + i;
-    const double m0 = mu_src[o];
-    double div = 0;
-    for (int m = 0; m < deg; m++) {
-      const int e = ce[m];
-      div += cs[m] * g.le[e] * hl[e] * ul[e];
-    }
-    const double m1 = m0 - DIVBY(tau * div, rA);
-    mu[o] = m1;
-    a += m1 + p.div_damp * (m1 - m0);
-    P[k] = a;
-  }
+  for (int m = 0; m < L; m++) { a += sa[m][j]; P[m] = a; }
   double ph = 0;
-  for (int k = L - 1; k >= 0; k--) { ph += S[(size_t)k * N + i] * P[k]; phi[(size_t)k * N + i] = ph; }
+  for (int m = L - 1; m >= k; m--) ph += sS[m][j] * P[m];
+  phi[o] = ph;
 }
 
 __global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, double tau, int first,
                                              const double* __restrict__ phi, const double* __restrict__ Tp,
                                              const double* __restrict__ G, const double* uin, double* acc,
                                              double* uout) {
-  const int t = TID;
+  const int t = XTID;
   if (t >= g.E * L) return;
   const int l = t / g.E, e = t - l * g.E;
   const double v = uin[t];
@@ -429,17 +491,22 @@ __global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, double t
 __global__ __launch_bounds__(BS) void k_bt_eta(isw_grid g, Div dv, double tau, const double* __restrict__ He,
                                                const double* __restrict__ ub, const double* eta_src, double* eta,
                                                double* __restrict__ eta_prev) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const double e0 = eta_src[i];
   eta_prev[i] = e0;
-  double div = 0;
+  int e[6];
+  ring6(g.cellEdge, i, e);
+  double t[6];
 #pragma unroll
   for (int m = 0; m < 6; m++) {
-    const int e = g.cellEdge[i * 6 + m];
-    if (e < 0) break;
-    div += (double)g.cellSign[i * 6 + m] * g.le[e] * He[e] * ub[e];
+    const int ek = ok0(e[m]);
+    t[m] = (double)g.cellSign[i * 6 + m] * g.le[ek] * He[ek] * ub[ek];
   }
+  double div = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++)
+    if (e[m] >= 0) div += t[m];
   eta[i] = e0 - DIVBY(tau * div, dv.A[i]);
 }
 
@@ -448,7 +515,7 @@ __global__ __launch_bounds__(BS) void k_bt_u(isw_grid g, Div dv, double tau, dou
                                              const double* __restrict__ G, const double* __restrict__ Gc,
                                              const double* __restrict__ wall, const double* ub_src, double* ub,
                                              double* acc) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const double v = ub_src[e];
   double a = first ? 0.0 + v : acc[e] + v;
@@ -482,17 +549,25 @@ __device__ __forceinline__ double stage_flux(const FluxSrc& f, const isw_grid& g
 
 __global__ __launch_bounds__(BS) void k_scale(isw_grid g, FluxSrc fs, double dts, double floor_,
                                               const double* __restrict__ h0, double* __restrict__ scale, int L) {
-  const int t = TID;
+  const int t = XTID;
   if (t >= g.N * L) return;
   const int l = t / g.N, i = t - l * g.N;
+  int e[6];
+  ring6(g.cellEdgeAsc, i, e);
+  double f[6];
+  bool own[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int ek = ok0(e[m]);
+    f[m] = stage_flux(fs, g, l, ek) * g.le[ek] * dts;
+    own[m] = g.eI[ek] == i;
+  }
   double out = 0;
 #pragma unroll
   for (int m = 0; m < 6; m++) {
-    const int e = g.cellEdgeAsc[i * 6 + m];
-    if (e < 0) break;
-    const double f = stage_flux(fs, g, l, e) * g.le[e] * dts;
-    if (g.eI[e] == i) { if (f > 0) out += f; }
-    else if (!(f > 0)) out -= f;
+    if (e[m] < 0) break;
+    if (own[m]) { if (f[m] > 0) out += f[m]; }
+    else if (!(f[m] > 0)) out -= f[m];
   }
   const double avail = h0[t] - floor_;
   const double cap = (avail > 0 ? avail : 0.0) * g.areaCell[i];
@@ -503,20 +578,30 @@ __global__ __launch_bounds__(BS) void k_scale(isw_grid g, FluxSrc fs, double dts
 __global__ __launch_bounds__(BS) void k_limit_h(isw_grid g, Div dv, FluxSrc fs, double cdt, double dt,
                                                 const double* __restrict__ scale, const double* h0, double* hn,
                                                 double* __restrict__ Fm, int L) {
-  const int t = TID;
+  const int t = XTID;
   if (t >= g.N * L) return;
   const int l = t / g.N, i = t - l * g.N;
   const double* sl = scale + (size_t)l * g.N;
+  int e[6];
+  ring6(g.cellEdge, i, e);
+  double f[6], sc[6], le[6];
+  int own[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) {
+    const int ek = ok0(e[m]);
+    f[m] = stage_flux(fs, g, l, ek);
+    const int eI = g.eI[ek];
+    own[m] = eI == i;
+    sc[m] = f[m] > 0 ? sl[eI] : sl[g.eJ[ek]];
+    le[m] = g.le[ek];
+  }
   double div = 0;
 #pragma unroll
   for (int m = 0; m < 6; m++) {
-    const int e = g.cellEdge[i * 6 + m];
-    if (e < 0) break;
-    double f = stage_flux(fs, g, l, e);
-    const int eI = g.eI[e];
-    f *= f > 0 ? sl[eI] : sl[g.eJ[e]];
-    div += (double)g.cellSign[i * 6 + m] * g.le[e] * f;
-    if (Fm && eI == i) Fm[(size_t)l * g.E + e] = f * g.le[e] * dt;
+    if (e[m] < 0) break;
+    const double fl = f[m] * sc[m];
+    div += (double)g.cellSign[i * 6 + m] * le[m] * fl;
+    if (Fm && own[m]) Fm[(size_t)l * g.E + e[m]] = fl * le[m] * dt;
   }
   const double v0 = h0[t];
   hn[t] = v0 - DIVBY(cdt * div, dv.A[i]);
@@ -525,7 +610,7 @@ __global__ __launch_bounds__(BS) void k_limit_h(isw_grid g, Div dv, FluxSrc fs, 
 __global__ __launch_bounds__(BS) void k_lay_u(int E, int L, double cdt, const double* u0, const double* __restrict__ Tp,
                                               const double* __restrict__ ub_end, const double* __restrict__ ub0,
                                               const double* __restrict__ wall, double* un) {
-  const int t = TID;
+  const int t = XTID;
   if (t >= E * L) return;
   const int e = t % E;
   const double v = u0[t] + cdt * Tp[t] + (ub_end[e] - ub0[e]);
@@ -538,7 +623,7 @@ __global__ __launch_bounds__(BS) void k_lay_u(int E, int L, double cdt, const do
 __global__ __launch_bounds__(BS) void k_lay_vmix(isw_grid g, int L, double c, double dt, double cd,
                                                  const double* __restrict__ h, const double* __restrict__ wall,
                                                  double* __restrict__ u, double* __restrict__ diss) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int N = g.N, E = g.E, eI = g.eI[e], eJ = g.eJ[e];
   double d = 0;
@@ -562,16 +647,18 @@ __global__ __launch_bounds__(BS) void k_lay_vmix(isw_grid g, int L, double c, do
 
 __global__ __launch_bounds__(BS) void k_vmix_heat(isw_grid g, LP p, const double* __restrict__ h,
                                                   const double* __restrict__ diss, double* __restrict__ th) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int N = g.N, L = p.L;
+  int e[6];
+  ring6(g.cellEdge, i, e);
+  double t[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) t[m] = 0.5 * g.le[ok0(e[m])] * g.de[ok0(e[m])] * diss[ok0(e[m])];
   double q = 0;
 #pragma unroll
-  for (int m = 0; m < 6; m++) {
-    const int e = g.cellEdge[i * 6 + m];
-    if (e < 0) break;
-    q += 0.5 * g.le[e] * g.de[e] * diss[e];
-  }
+  for (int m = 0; m < 6; m++)
+    if (e[m] >= 0) q += t[m];
   if (q == 0) return;
   q /= g.areaCell[i];
   double mu = 0;
@@ -590,7 +677,7 @@ __global__ __launch_bounds__(BS) void k_vmix_heat(isw_grid g, LP p, const double
 __global__ __launch_bounds__(BS) void k_xch_cell(int N, int L, int seq, const double* __restrict__ wx,
                                                  const double* __restrict__ areaCell, double* __restrict__ h,
                                                  double* __restrict__ M, double* __restrict__ psi) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= N) return;
   double tr[ISW_LMAX], dm[ISW_LMAX];
   for (int k = 0; k < L; k++) { tr[k] = 0; dm[k] = 0; }
@@ -635,7 +722,7 @@ __global__ __launch_bounds__(BS) void k_xch_edge(isw_grid g, int L, const double
                                                  double* __restrict__ diss, const double* __restrict__ lap4,
                                                  double nu4dt, double hthin, double floor_,
                                                  const double* __restrict__ wall) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int N = g.N, E = g.E, eI = g.eI[e], eJ = g.eJ[e];
   double mom[ISW_LMAX], hn[ISW_LMAX];
@@ -692,17 +779,22 @@ __global__ __launch_bounds__(BS) void k_xch_edge(isw_grid g, int L, const double
 // ---------------------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BS) void k_mass_update(isw_grid g, const double* __restrict__ M0,
                                                     const double* __restrict__ Fm, double* __restrict__ M1) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int l = blockIdx.y;
   const size_t o = (size_t)l * g.N + i;
   const double* f = Fm + (size_t)l * g.E;
+  int e[6];
+  ring6(g.cellEdgeAsc, i, e);
+  double fv[6];
+  bool own[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) { fv[k] = f[ok0(e[k])]; own[k] = g.eI[ok0(e[k])] == i; }
   double m = M0[o];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const int e = g.cellEdgeAsc[i * 6 + k];
-    if (e < 0) break;
-    if (g.eI[e] == i) m -= f[e]; else m += f[e];
+    if (e[k] < 0) break;
+    if (own[k]) m -= fv[k]; else m += fv[k];
   }
   M1[o] = m;
 }
@@ -730,19 +822,23 @@ __global__ __launch_bounds__(BS) void k_tr_td(isw_grid g, int Lm, const double* 
                                               const double* __restrict__ psi, const double* __restrict__ Fm,
                                               const double* __restrict__ M1, double* __restrict__ td,
                                               float* __restrict__ pf) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int y = blockIdx.y, l = y % Lm;
   const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
   const double* p = psi + (size_t)y * g.N;
   const double* Fml = Fm + (size_t)l * g.E;
+  int e[6];
+  ring6(g.cellEdgeAsc, i, e);
+  double fv[6];
+  bool own[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) { fv[k] = upwind(Fml, p, g, ok0(e[k])); own[k] = g.eI[ok0(e[k])] == i; }
   double t = M0[om] * psi[o];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const int e = g.cellEdgeAsc[i * 6 + k];
-    if (e < 0) break;
-    const double f = upwind(Fml, p, g, e);
-    if (g.eI[e] == i) t -= f; else t += f;
+    if (e[k] < 0) break;
+    if (own[k]) t -= fv[k]; else t += fv[k];
   }
   td[o] = t / M1[om];
   if (pf) pf[o] = (float)psi[o];
@@ -752,7 +848,7 @@ __global__ __launch_bounds__(BS) void k_tr_hflux(isw_grid g, int Lm, double frac
                                                  const double* __restrict__ src, const fast* __restrict__ srcf,
                                                  const fast* __restrict__ wt, const double* __restrict__ Fm,
                                                  double* __restrict__ Hf) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int y = blockIdx.y, l = y % Lm;
   Hf[(size_t)y * g.E + e] =
@@ -763,21 +859,29 @@ __global__ __launch_bounds__(BS) void k_tr_stage(isw_grid g, int Lm, double frac
                                                  const double* __restrict__ psi, const double* __restrict__ Hf,
                                                  const double* __restrict__ Fm, double* __restrict__ out,
                                                  float* __restrict__ pf) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int y = blockIdx.y, l = y % Lm;
   const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
   const double* Hl = Hf + (size_t)y * g.E;
   const double* Fml = Fm + (size_t)l * g.E;
+  int e[6];
+  ring6(g.cellEdgeAsc, i, e);
+  double fv[6], mv[6];
+  bool own[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int ek = ok0(e[k]);
+    fv[k] = Hl[ek];
+    mv[k] = frac * Fml[ek];
+    own[k] = g.eI[ek] == i;
+  }
   double t = M0[om] * psi[o];
   double mx = M0[om];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const int e = g.cellEdgeAsc[i * 6 + k];
-    if (e < 0) break;
-    const double f = Hl[e];
-    const double m = frac * Fml[e];
-    if (g.eI[e] == i) { t -= f; mx -= m; } else { t += f; mx += m; }
+    if (e[k] < 0) break;
+    if (own[k]) { t -= fv[k]; mx -= mv[k]; } else { t += fv[k]; mx += mv[k]; }
   }
   const double r = t / mx;
   out[o] = r;
@@ -788,7 +892,7 @@ __global__ __launch_bounds__(BS) void k_tr_aflux(isw_grid g, int Lm, double beta
                                                  const double* __restrict__ p2, const fast* __restrict__ p2f,
                                                  const fast* __restrict__ wt, const double* __restrict__ Fm,
                                                  double* __restrict__ A) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int y = blockIdx.y, l = y % Lm;
   const double* Fml = Fm + (size_t)l * g.E;
@@ -800,29 +904,39 @@ __global__ __launch_bounds__(BS) void k_tr_ratios(isw_grid g, int Lm, const doub
                                                   const double* __restrict__ td, const double* __restrict__ M1,
                                                   const double* __restrict__ A, double* __restrict__ Rp,
                                                   double* __restrict__ Rm) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int y = blockIdx.y, l = y % Lm;
   const double* p = psi + (size_t)y * g.N;
   const double* t = td + (size_t)y * g.N;
   const double* Al = A + (size_t)y * g.E;
+  int nb[6], e[6];
+  ring6(g.nbr, i, nb);
+  ring6(g.cellEdgeAsc, i, e);
+  double pv[6], tv[6], av[6];
+  bool own[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    pv[k] = p[ok0(nb[k])];
+    tv[k] = t[ok0(nb[k])];
+    av[k] = Al[ok0(e[k])];
+    own[k] = g.eI[ok0(e[k])] == i;
+  }
   double hi = fmax(p[i], t[i]), lo = fmin(p[i], t[i]);
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const int j = g.nbr[i * 6 + k];
-    if (j < 0) break;
-    hi = fmax(hi, fmax(p[j], t[j]));
-    lo = fmin(lo, fmin(p[j], t[j]));
+    if (nb[k] < 0) break;
+    hi = fmax(hi, fmax(pv[k], tv[k]));
+    lo = fmin(lo, fmin(pv[k], tv[k]));
   }
   const double m1 = M1[(size_t)l * g.N + i];
   const double qmax = (hi - t[i]) * m1, qmin = (t[i] - lo) * m1;
   double Pp = 0, Pm = 0;
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const int e = g.cellEdgeAsc[i * 6 + k];
-    if (e < 0) break;
-    const double a = Al[e];
-    if (g.eI[e] == i) { if (a > 0) Pm += a; else Pp -= a; }
+    if (e[k] < 0) break;
+    const double a = av[k];
+    if (own[k]) { if (a > 0) Pm += a; else Pp -= a; }
     else { if (a > 0) Pp += a; else Pm -= a; }
   }
   Rp[(size_t)y * g.N + i] = Pp > 0 ? fmin(1.0, qmax / Pp) : 0.0;
@@ -833,21 +947,30 @@ __global__ __launch_bounds__(BS) void k_tr_apply(isw_grid g, int Lm, const doubl
                                                  const double* __restrict__ M1, const double* __restrict__ A,
                                                  const double* __restrict__ Rp, const double* __restrict__ Rm,
                                                  double* __restrict__ out) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int y = blockIdx.y, l = y % Lm;
   const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
   const double* Al = A + (size_t)y * g.E;
   const double* rp = Rp + (size_t)y * g.N;
   const double* rm = Rm + (size_t)y * g.N;
+  int e[6];
+  ring6(g.cellEdgeAsc, i, e);
+  double av[6], cv[6];
+  bool own[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int ek = ok0(e[k]);
+    const int eI = g.eI[ek], eJ = g.eJ[ek];
+    av[k] = Al[ek];
+    cv[k] = av[k] > 0 ? fmin(rp[eJ], rm[eI]) : fmin(rp[eI], rm[eJ]);
+    own[k] = eI == i;
+  }
   double t = td[o] * M1[om];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const int e = g.cellEdgeAsc[i * 6 + k];
-    if (e < 0) break;
-    const double a = Al[e];
-    const double c = a > 0 ? fmin(rp[g.eJ[e]], rm[g.eI[e]]) : fmin(rp[g.eI[e]], rm[g.eJ[e]]);
-    if (g.eI[e] == i) t -= c * a; else t += c * a;
+    if (e[k] < 0) break;
+    if (own[k]) t -= cv[k] * av[k]; else t += cv[k] * av[k];
   }
   out[o] = t / M1[om];
 }
@@ -857,7 +980,7 @@ __global__ __launch_bounds__(BS) void k_tr_apply(isw_grid g, int Lm, const doubl
 // ---------------------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BS) void k_swe_scale(isw_grid g, double dts, const double* __restrict__ h,
                                                   const double* __restrict__ u, double* __restrict__ scale) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int l = blockIdx.y;
   const double* hl = h + (size_t)l * g.N;
@@ -879,7 +1002,7 @@ __global__ __launch_bounds__(BS) void k_swe_scale(isw_grid g, double dts, const 
 __global__ __launch_bounds__(BS) void k_swe_flux(isw_grid g, double dt, const double* __restrict__ h,
                                                  const double* __restrict__ u, const double* __restrict__ scale,
                                                  double* __restrict__ F, double* __restrict__ Fm) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int l = blockIdx.y;
   const size_t o = (size_t)l * g.E + e;
@@ -894,7 +1017,7 @@ __global__ __launch_bounds__(BS) void k_swe_cell(isw_grid g, double grav, double
                                                  const double* __restrict__ b, const double* __restrict__ F,
                                                  const double* __restrict__ u, const double* h0, double* hn,
                                                  double* __restrict__ B) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= g.N) return;
   const int l = blockIdx.y;
   const double* Fl = F + (size_t)l * g.E;
@@ -916,7 +1039,7 @@ __global__ __launch_bounds__(BS) void k_swe_cell(isw_grid g, double grav, double
 __global__ __launch_bounds__(BS) void k_swe_vertex(isw_grid g, Div dv, const double* __restrict__ h,
                                                    const double* __restrict__ u, const double* __restrict__ fV,
                                                    double* __restrict__ q) {
-  const int v = TID;
+  const int v = XTID;
   if (v >= g.NV) return;
   const int l = blockIdx.y;
   const double* hl = h + (size_t)l * g.N;
@@ -944,7 +1067,7 @@ __global__ __launch_bounds__(BS) void k_couple_stress(isw_grid g, int La, double
                                                       double* __restrict__ uo, double* __restrict__ diss,
                                                       double* __restrict__ tau_acc, const double* __restrict__ wall,
                                                       const double* __restrict__ cd) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int N = g.N, E = g.E, b = La - 1, eI = g.eI[e], eJ = g.eJ[e];
   double c = c0, land_drag = land_drag0;
@@ -976,7 +1099,7 @@ __global__ __launch_bounds__(BS) void k_land_column(int N, int K, double c_soil,
                                                     const double* __restrict__ land, double* __restrict__ T,
                                                     const double* __restrict__ cap, double* __restrict__ Tg,
                                                     const double* __restrict__ dz) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= N) return;
   if (!(land[i] > 0.5)) return;
   const int n = K + 1;
@@ -1006,7 +1129,7 @@ __global__ __launch_bounds__(BS) void k_land_column(int N, int K, double c_soil,
 
 __global__ __launch_bounds__(BS) void k_ocn_cap(int N, double rho_cw, double c_land, const double* __restrict__ h,
                                                 double* __restrict__ cap, const double* __restrict__ land) {
-  const int i = TID;
+  const int i = XTID;
   if (i >= N) return;
   cap[i] = land && land[i] > 0.5 ? (cap[i] > 0 ? cap[i] : c_land) : rho_cw * h[i];
 }
@@ -1016,7 +1139,7 @@ __global__ __launch_bounds__(BS) void k_swe_edge_lp(isw_grid g, LP p, double c, 
                                                     const double* __restrict__ q, const double* __restrict__ B,
                                                     const double* __restrict__ u, const double* __restrict__ lap4,
                                                     const double* u0, double* un) {
-  const int e = TID;
+  const int e = XTID;
   if (e >= g.E) return;
   const int l = blockIdx.y;
   const double* Fl = F + (size_t)l * g.E;
@@ -1063,10 +1186,10 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
                   double* un, double* Fm, const double* theta, hipStream_t st) {
   const int N = g->N, E = g->E, L = s->L;
   const double n_ = N, e_ = E, v_ = g->NV, l_ = L, air = s->air ? 1 : 0;
-  const int nbc = (N + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
+  const int nbc = (N + BS1 - 1) / BS1, nbv = (g->NV + BS1 - 1) / BS1;
   ISW_LAUNCH("bern_pv", l_ * n_ * D * (1 + 3 * air) + l_ * e_ * D + n_ * D + 6 * n_ * I + 2 * e_ * D + n_ * D +
                             l_ * n_ * D + 3 * v_ * (2 * I + 2 * C) + 6 * n_ * D + 2 * v_ * D + l_ * v_ * D,
-             k_bern_pv<<<dim3(nbc + nbv), BS, 0, st>>>(*g, p, dv, nbc, hs, us, theta, s->b, s->S, s->fV, s->B,
+             k_bern_pv<<<dim3(nbc + nbv), dim3(BS1, L), 0, st>>>(*g, p, dv, nbc, hs, us, theta, s->b, s->S, s->fV, s->B,
                                                       s->pibar, s->q));
 #if ISW_FORM_MIXED
   fast* pk = s->pkf;
@@ -1080,7 +1203,7 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   const double neoe = g->n_eoe;
   ISW_LAUNCH("slow", 3 * l_ * e_ * R + neoe * (R + I) + (e_ + 1) * I + 2 * l_ * e_ * D + l_ * n_ * D + 2 * e_ * I +
                          e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + air * 2 * l_ * n_ * D + l_ * e_ * D + 3 * e_ * D,
-             k_slow<<<g1(E), BS, 0, st>>>(*g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
+             k_slow<<<c1(E), dim3(BS1, L), 0, st>>>(*g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
                                           s->air ? theta : nullptr, s->air ? s->pibar : nullptr, s->Tp, s->G, s->He,
                                           s->ubs));
   int n = (int)std::ceil(c * nfast);
@@ -1093,8 +1216,8 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     for (int it = 0; it < n; it++) {
       const bool first = it == 0, last = it == n - 1;
       ISW_LAUNCH("af_muphi", 2 * l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 4 * l_ * n_ * D,
-                 k_af_muphi<<<g1(N), BS, 0, st>>>(*g, p, dv, tau, s->hedge, first ? u0 : s->uf, first ? h0 : s->muf,
-                                                  s->muf, s->S, s->phi));
+                 k_af_muphi<<<c1(N), dim3(BS1, L), 0, st>>>(*g, p, dv, tau, s->hedge, first ? u0 : s->uf,
+                                                             first ? h0 : s->muf, s->muf, s->S, s->phi));
       ISW_LAUNCH("af_u", l_ * n_ * D + l_ * e_ * D + e_ * D + 2 * e_ * I + e_ * D + 2 * l_ * e_ * D +
                              (first ? 1 : 2) * l_ * e_ * D,
                  k_af_u<<<g1((long)E * L), BS, 0, st>>>(*g, L, dv, tau, first ? 1 : 0, s->phi, s->Tp, s->G,
@@ -1107,8 +1230,8 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     for (int it = 0; it < n; it++) {
       const bool first = it == 0;
       ISW_LAUNCH("bt_eta", 2 * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 3 * n_ * D,
-                 k_bt_eta<<<g1(N), BS, 0, st>>>(*g, dv, tau, s->He, first ? s->ub0 : s->ub, first ? s->eta0 : s->eta,
-                                                s->eta, s->eta_prev));
+                 k_bt_eta<<<c1(N), BS1, 0, st>>>(*g, dv, tau, s->He, first ? s->ub0 : s->ub,
+                                                 first ? s->eta0 : s->eta, s->eta, s->eta_prev));
       ISW_LAUNCH("bt_u", 3 * n_ * D + e_ * D + 2 * e_ * I + e_ * D + (s->wall ? e_ * D : 0) + 2 * e_ * D +
                              (first ? 1 : 2) * e_ * D,
                  k_bt_u<<<g1(E), BS, 0, st>>>(*g, dv, tau, s->div_damp, first ? 1 : 0, s->eta, s->eta_prev, s->G,
@@ -1123,10 +1246,10 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   }
   const double fb = flux_bytes(fs, n_, e_, l_);
   ISW_LAUNCH("scale", fb + 6 * n_ * I + 2 * e_ * I + e_ * D + l_ * n_ * D + n_ * D + l_ * n_ * D,
-             k_scale<<<g1((long)N * L), BS, 0, st>>>(*g, fs, c * dt, s->floor_, h0, s->scale, L));
+             k_scale<<<c1((long)N * L), BS1, 0, st>>>(*g, fs, c * dt, s->floor_, h0, s->scale, L));
   ISW_LAUNCH("limit_h", fb + 6 * n_ * (I + C) + 2 * e_ * I + e_ * D + 3 * l_ * n_ * D + n_ * D +
                             (Fm ? l_ * e_ * D : 0),
-             k_limit_h<<<g1((long)N * L), BS, 0, st>>>(*g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L));
+             k_limit_h<<<c1((long)N * L), BS1, 0, st>>>(*g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L));
   if (!s->air)
     ISW_LAUNCH("lay_u", 3 * l_ * e_ * D + 2 * e_ * D + (s->wall ? e_ * D : 0),
                k_lay_u<<<g1((long)E * L), BS, 0, st>>>(E, L, c * dt, u0, s->Tp, s->ub, s->ub0, s->wall, un));
@@ -1139,9 +1262,9 @@ int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, do
   const int N = g->N, L = s->L;
   const double n_ = N, e_ = g->E, l_ = L;
   if (s->air) {
-    ISW_LAUNCH("air_stiff", 3 * l_ * n_ * D, k_air_stiff<<<g1(N), BS, 0, st>>>(N, p, h, theta, s->S));
+    ISW_LAUNCH("air_stiff", 3 * l_ * n_ * D, k_air_stiff<<<c1(N), BS1, 0, st>>>(N, p, h, theta, s->S));
   } else {
-    ISW_LAUNCH("lay_eta", (l_ + 2) * n_ * D, k_lay_eta<<<g1(N), BS, 0, st>>>(N, L, h, s->b, s->eta0));
+    ISW_LAUNCH("lay_eta", (l_ + 2) * n_ * D, k_lay_eta<<<c1(N), BS1, 0, st>>>(N, L, h, s->b, s->eta0));
     ISW_LAUNCH("lay_ubar", l_ * n_ * D + l_ * e_ * D + 2 * e_ * I + (s->wall ? e_ * D : 0) + e_ * D,
                k_lay_ubar<<<g1(g->E), BS, 0, st>>>(*g, L, s->shelf, h, u, s->wall, s->ub0));
   }
@@ -1157,7 +1280,7 @@ int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, do
                k_lay_vmix<<<g1(g->E), BS, 0, st>>>(*g, L, s->vmix, dt, s->vmix_cd, h, s->wall, u, s->diss));
     if (s->air)
       ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
-                 k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, p, h, s->diss, theta));
+                 k_vmix_heat<<<c1(N), BS1, 0, st>>>(*g, p, h, s->diss, theta));
   }
   return (int)hipGetLastError();
 }
@@ -1170,7 +1293,7 @@ int layers_lap(const isw_grid* g, const isw_layers* s, const double* u, double* 
 int layers_gfast(const isw_grid* g, const isw_layers* s, const double* h, const double* theta, hipStream_t st) {
   if (s->air)
     ISW_LAUNCH("air_gfast", 3.0 * s->L * g->N * D,
-               k_air_gfast<<<g1(g->N), BS, 0, st>>>(g->N, make_lp(s), s->rd, h, theta, s->Gc));
+               k_air_gfast<<<c1(g->N), BS1, 0, st>>>(g->N, make_lp(s), s->rd, h, theta, s->Gc));
   return (int)hipGetLastError();
 }
 
@@ -1181,7 +1304,7 @@ int layers_exchange(const isw_grid* g, const isw_layers* s, const double* wx, do
   ISW_LAUNCH("copy", 2 * l_ * n_ * D, k_copy<<<g1((long)N * L), BS, 0, st>>>((long)N * L, h, s->hold));
   const bool heat = s->air && theta;
   ISW_LAUNCH("xch_cell", (l_ - 1) * n_ * D + n_ * D + 6 * l_ * n_ * D,
-             k_xch_cell<<<g1(N), BS, 0, st>>>(N, L, seq, wx, g->areaCell, h, M, psi));
+             k_xch_cell<<<c1(N), BS1, 0, st>>>(N, L, seq, wx, g->areaCell, h, M, psi));
   const bool v4 = heat && s->nu4 != 0;
   ISW_LAUNCH("xch_edge", (l_ - 1) * n_ * D + l_ * n_ * D + 2 * e_ * I + 2 * l_ * e_ * D + (heat ? e_ * D : 0) +
                              (v4 ? l_ * e_ * D : 0) + (s->wall ? e_ * D : 0),
@@ -1190,7 +1313,7 @@ int layers_exchange(const isw_grid* g, const isw_layers* s, const double* wx, do
                                                  s->floor_, s->wall));
   if (heat)
     ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
-               k_vmix_heat<<<g1(N), BS, 0, st>>>(*g, make_lp(s), h, s->diss, theta));
+               k_vmix_heat<<<c1(N), BS1, 0, st>>>(*g, make_lp(s), h, s->diss, theta));
   return (int)hipGetLastError();
 }
 
@@ -1198,14 +1321,14 @@ int vmix_heat(const isw_grid* g, const isw_layers* s, const double* h, const dou
               hipStream_t st) {
   const double n_ = g->N, e_ = g->E, l_ = s->L;
   ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
-             k_vmix_heat<<<g1(g->N), BS, 0, st>>>(*g, make_lp(s), h, diss, theta));
+             k_vmix_heat<<<c1(g->N), BS1, 0, st>>>(*g, make_lp(s), h, diss, theta));
   return (int)hipGetLastError();
 }
 
 int mass_update(const isw_grid* g, int L, const double* M0, const double* Fm, double* M1, hipStream_t st) {
   const double n_ = g->N, e_ = g->E, l_ = L;
   ISW_LAUNCH("mass_update", 2 * l_ * n_ * D + l_ * e_ * D + 6 * n_ * I + e_ * I,
-             k_mass_update<<<g2(g->N, L), BS, 0, st>>>(*g, M0, Fm, M1));
+             k_mass_update<<<c2(g->N, L), BS1, 0, st>>>(*g, M0, Fm, M1));
   return (int)hipGetLastError();
 }
 
@@ -1224,7 +1347,7 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
   const double cellE = 6 * n_ * I + 2 * e_ * I;  // cellEdgeAsc, eI/eJ
   const double sten = 8 * e_ * (R + I) + l_ * e_ * D + 2 * e_ * I;
   ISW_LAUNCH("tr_td", 2 * l_ * n_ * D + r_ * n_ * D + l_ * e_ * D + cellE + r_ * n_ * D + mixed * r_ * n_ * 4,
-             k_tr_td<<<g2(N, rows), BS, 0, st>>>(*g, Lm, M0, psi, Fm, M1, tr->td, pf));
+             k_tr_td<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, M0, psi, Fm, M1, tr->td, pf));
   if (order == 1) {
     ISW_LAUNCH("copy", 2 * r_ * n_ * D, k_copy<<<g1((long)N * rows), BS, 0, st>>>((long)N * rows, tr->td, psi));
     return (int)hipGetLastError();
@@ -1242,18 +1365,18 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
              k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf));
   ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
                              mixed * r_ * n_ * 4,
-             k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf));
+             k_tr_stage<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf));
   ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
              k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf));
   ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
                              mixed * r_ * n_ * 4,
-             k_tr_stage<<<g2(N, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf));
+             k_tr_stage<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf));
   ISW_LAUNCH("tr_aflux", 2 * r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
              k_tr_aflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A));
   ISW_LAUNCH("tr_ratios", 2 * r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 6 * n_ * I + cellE + 2 * r_ * n_ * D,
-             k_tr_ratios<<<g2(N, rows), BS, 0, st>>>(*g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm));
+             k_tr_ratios<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm));
   ISW_LAUNCH("tr_apply", r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 2 * r_ * n_ * D + cellE + r_ * n_ * D,
-             k_tr_apply<<<g2(N, rows), BS, 0, st>>>(*g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi));
+             k_tr_apply<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi));
   return (int)hipGetLastError();
 }
 
@@ -1274,17 +1397,17 @@ int swe_step(const isw_grid* g, const isw_layers* s, double* h, double* u, doubl
   for (int k = 0; k < 3; k++) {
     const double c = cs[k];
     ISW_LAUNCH("swe_scale", l_ * n_ * D + l_ * e_ * D + 6 * n_ * I + 2 * e_ * I + e_ * D + n_ * D + l_ * n_ * D,
-               k_swe_scale<<<g2(N, L), BS, 0, st>>>(*g, c, hsv[k], usv[k], s->scale));
+               k_swe_scale<<<c2(N, L), BS1, 0, st>>>(*g, c, hsv[k], usv[k], s->scale));
     ISW_LAUNCH("swe_flux", l_ * n_ * D + l_ * e_ * D + 2 * e_ * I + l_ * n_ * D + l_ * e_ * D +
                                (k == 2 && Fm ? l_ * e_ * D + e_ * D : 0),
                k_swe_flux<<<g2(E, L), BS, 0, st>>>(*g, k == 2 ? dt : 0.0, hsv[k], usv[k], s->scale, s->F,
                                                    k == 2 ? Fm : nullptr));
     ISW_LAUNCH("swe_cell", 2 * l_ * n_ * D + n_ * D + 2 * l_ * e_ * D + 6 * n_ * (I + C) + 2 * e_ * D + n_ * D +
                                2 * l_ * n_ * D,
-               k_swe_cell<<<g2(N, L), BS, 0, st>>>(*g, s->grav, c, hsv[k], s->b, s->F, usv[k], h, hnv[k], s->B));
+               k_swe_cell<<<c2(N, L), BS1, 0, st>>>(*g, s->grav, c, hsv[k], s->b, s->F, usv[k], h, hnv[k], s->B));
     ISW_LAUNCH("swe_vertex", l_ * n_ * D + l_ * e_ * D + 3 * v_ * (2 * I + 2 * C) + 6 * n_ * D + e_ * D +
                                  2 * v_ * D + l_ * v_ * D,
-               k_swe_vertex<<<g2(NV, L), BS, 0, st>>>(*g, dv, hsv[k], usv[k], s->fV, s->q));
+               k_swe_vertex<<<c2(NV, L), BS1, 0, st>>>(*g, dv, hsv[k], usv[k], s->fV, s->q));
     ISW_LAUNCH("swe_edge", l_ * e_ * D + l_ * v_ * D + l_ * n_ * D + l_ * e_ * D + g->n_eoe * (D + I) +
                                (e_ + 1) * I + 4 * e_ * I + e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + 2 * l_ * e_ * D,
                k_swe_edge_lp<<<g2(E, L), BS, 0, st>>>(*g, p, c, s->F, s->q, s->B, usv[k], s->L2, u, unv[k]));
@@ -1307,7 +1430,7 @@ int land_column(const isw_grid* g, int K, const double* land, double* T, const d
                 const double* dz, double c_soil, double k_soil, double dt, hipStream_t st) {
   const double n_ = g->N;
   ISW_LAUNCH("land_column", n_ * D + 2 * n_ * D + n_ * D + 2.0 * K * n_ * D + K * D,
-             k_land_column<<<g1(g->N), BS, 0, st>>>(g->N, K, c_soil, k_soil, dt, land, T, cap, Tg, dz));
+             k_land_column<<<c1(g->N), BS1, 0, st>>>(g->N, K, c_soil, k_soil, dt, land, T, cap, Tg, dz));
   return (int)hipGetLastError();
 }
 
@@ -1315,7 +1438,7 @@ int ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* cap, cons
             hipStream_t st) {
   const double n_ = g->N;
   ISW_LAUNCH("ocn_cap", n_ * D + 2 * n_ * D + (land ? n_ * D : 0),
-             k_ocn_cap<<<g1(g->N), BS, 0, st>>>(g->N, rho_cw, c_land, h, cap, land));
+             k_ocn_cap<<<c1(g->N), BS1, 0, st>>>(g->N, rho_cw, c_land, h, cap, land));
   return (int)hipGetLastError();
 }
 

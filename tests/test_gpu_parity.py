@@ -112,6 +112,38 @@ def test_layers_air_fp64(g4):
             assert d <= tol, f'step {it}: {name} differs by {d:.2e}'
 
 
+@pytest.mark.parametrize('air', [True, False])
+def test_layers_one_substep_fp64(g4, air):
+    """nfast = 1: the last stage's single fast sub-step writes its output over its own input wind (u0 == un)."""
+    import oracle
+    from insilica import cases
+    from insilica.constants import CP_DRY, KAPPA, R_DRY
+    from insilica.gpu import DeviceGrid
+    print(f'case: one sub-step, air={air}', flush=True)
+    g = g4
+    dg, og = DeviceGrid(g, 'double'), oracle.Grid(g)
+    if air:
+        st = cases.coupled_stack(g, seed=6)
+        kw = dict(layers=3, gprime=[0.0, 0.0], floor=10.0, air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY))
+        h, u, th = st['h'], st['u'], st['theta']
+    else:
+        h, b, u, gp = cases.phillips(g, 1.0)
+        kw = dict(layers=2, b=b, gprime=[gp], floor=1.0)
+        th = None
+    lay, olay = _pair_layers(g, dg, og, **kw)
+    hd, ud = dg.asarray(h), dg.asarray(u)
+    ho, uo = np.ascontiguousarray(h), np.ascontiguousarray(u)
+    td, to = (dg.asarray(th), np.ascontiguousarray(th)) if air else (None, None)
+    if air:
+        lay.update_gfast(hd, td)
+        olay.update_gfast(ho, to)
+    for it in range(3):
+        lay.step(hd, ud, 60.0, 1, theta=td)
+        olay.step(ho, uo, 60.0, 1, theta=to)
+    torch.cuda.synchronize()
+    assert relmax(_np(hd), ho) <= TOLN and relmax(_np(ud), uo) <= TOLN
+
+
 def test_exchange_fp64(g4):
     """Layers.exchange (both orders) on three air layers with theta: h, M, psi, u, theta."""
     import oracle
