@@ -300,17 +300,16 @@ void recycle() {
 
 bool enabled() { return g_on; }
 
-int begin(const char* name, double bytes, hipStream_t st) {
+bool begin(const char* name, double bytes, hipStream_t st, hipEvent_t* a, hipEvent_t* b) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return -1;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
   Rec r{name, bytes, take(), take()};
-  if (!r.a || !r.b) return -1;
-  hipEventRecord(r.a, st);
+  if (!r.a || !r.b) return false;
   g_recs.push_back(r);
-  return (int)g_recs.size() - 1;
+  *a = r.a;
+  *b = r.b;
+  return true;
 }
-
-void end(int tok, hipStream_t st) { hipEventRecord(g_recs[tok].b, st); }
 }  // namespace isw_diag
 
 extern "C" int32_t isw_timing_start(void) {

@@ -1175,9 +1175,9 @@ static void lap(const isw_grid* g, int L, const double* u, double* D1, double* Z
   const int nbc = (g->N * L + BS - 1) / BS, nbv = (g->NV * L + BS - 1) / BS;
   ISW_LAUNCH("div_curl", l * E * D + 6 * N * (I + C) + E * D + N * D + l * N * D + 3 * NV * (I + C) + E * D + NV * D +
                              l * NV * D,
-             k_div_curl<<<dim3(nbc + nbv), BS, 0, st>>>(*g, L, nbc, u, D1, Z1));
+             k_div_curl, dim3(nbc + nbv), BS, *g, L, nbc, u, D1, Z1);
   ISW_LAUNCH("edge_lap", l * N * D + l * NV * D + 4 * E * I + 2 * E * D + l * E * D,
-             k_edge_lap<<<g1((long)g->E * L), BS, 0, st>>>(*g, L, D1, Z1, out));
+             k_edge_lap, g1((long)g->E * L), BS, *g, L, D1, Z1, out);
 }
 
 // Layers._stage (layers.py:255-309).
@@ -1189,8 +1189,8 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   const int nbc = (N + BS1 - 1) / BS1, nbv = (g->NV + BS1 - 1) / BS1;
   ISW_LAUNCH("bern_pv", l_ * n_ * D * (1 + 3 * air) + l_ * e_ * D + n_ * D + 6 * n_ * I + 2 * e_ * D + n_ * D +
                             l_ * n_ * D + 3 * v_ * (2 * I + 2 * C) + 6 * n_ * D + 2 * v_ * D + l_ * v_ * D,
-             k_bern_pv<<<dim3(nbc + nbv), dim3(BS1, L), 0, st>>>(*g, p, dv, nbc, hs, us, theta, s->b, s->S, s->fV, s->B,
-                                                      s->pibar, s->q));
+             k_bern_pv, dim3(nbc + nbv), dim3(BS1, L), *g, p, dv, nbc, hs, us, theta, s->b, s->S, s->fV, s->B,
+                                                      s->pibar, s->q);
 #if ISW_FORM_MIXED
   fast* pk = s->pkf;
   const fast* wf = g->wf;
@@ -1199,13 +1199,13 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   const fast* wf = g->w;
 #endif
   ISW_LAUNCH("edge_hq", l_ * n_ * D + l_ * v_ * D + l_ * e_ * D + 4 * e_ * I + l_ * e_ * D + 3 * l_ * e_ * R,
-             k_edge_hq<<<g1((long)E * L), BS, 0, st>>>(*g, L, s->shelf, hs, s->q, us, s->hedge, pk));
+             k_edge_hq, g1((long)E * L), BS, *g, L, s->shelf, hs, s->q, us, s->hedge, pk);
   const double neoe = g->n_eoe;
   ISW_LAUNCH("slow", 3 * l_ * e_ * R + neoe * (R + I) + (e_ + 1) * I + 2 * l_ * e_ * D + l_ * n_ * D + 2 * e_ * I +
                          e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + air * 2 * l_ * n_ * D + l_ * e_ * D + 3 * e_ * D,
-             k_slow<<<c1(E), dim3(BS1, L), 0, st>>>(*g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
+             k_slow, c1(E), dim3(BS1, L), *g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
                                           s->air ? theta : nullptr, s->air ? s->pibar : nullptr, s->Tp, s->G, s->He,
-                                          s->ubs));
+                                          s->ubs);
   int n = (int)std::ceil(c * nfast);
   if (n < 1) n = 1;
   const double tau = c * dt / n;
@@ -1216,12 +1216,12 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     for (int it = 0; it < n; it++) {
       const bool first = it == 0, last = it == n - 1;
       ISW_LAUNCH("af_muphi", 2 * l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 4 * l_ * n_ * D,
-                 k_af_muphi<<<c1(N), dim3(BS1, L), 0, st>>>(*g, p, dv, tau, s->hedge, first ? u0 : s->uf,
-                                                             first ? h0 : s->muf, s->muf, s->S, s->phi));
+                 k_af_muphi, c1(N), dim3(BS1, L), *g, p, dv, tau, s->hedge, first ? u0 : s->uf,
+                                                             first ? h0 : s->muf, s->muf, s->S, s->phi);
       ISW_LAUNCH("af_u", l_ * n_ * D + l_ * e_ * D + e_ * D + 2 * e_ * I + e_ * D + 2 * l_ * e_ * D +
                              (first ? 1 : 2) * l_ * e_ * D,
-                 k_af_u<<<g1((long)E * L), BS, 0, st>>>(*g, L, dv, tau, first ? 1 : 0, s->phi, s->Tp, s->G,
-                                                        first ? u0 : s->uf, s->accL, last ? un : s->uf));
+                 k_af_u, g1((long)E * L), BS, *g, L, dv, tau, first ? 1 : 0, s->phi, s->Tp, s->G,
+                                                        first ? u0 : s->uf, s->accL, last ? un : s->uf);
     }
     fs.air = 1;
     fs.hedge = s->hedge;
@@ -1230,12 +1230,12 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     for (int it = 0; it < n; it++) {
       const bool first = it == 0;
       ISW_LAUNCH("bt_eta", 2 * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 3 * n_ * D,
-                 k_bt_eta<<<c1(N), BS1, 0, st>>>(*g, dv, tau, s->He, first ? s->ub0 : s->ub,
-                                                 first ? s->eta0 : s->eta, s->eta, s->eta_prev));
+                 k_bt_eta, c1(N), BS1, *g, dv, tau, s->He, first ? s->ub0 : s->ub,
+                                                 first ? s->eta0 : s->eta, s->eta, s->eta_prev);
       ISW_LAUNCH("bt_u", 3 * n_ * D + e_ * D + 2 * e_ * I + e_ * D + (s->wall ? e_ * D : 0) + 2 * e_ * D +
                              (first ? 1 : 2) * e_ * D,
-                 k_bt_u<<<g1(E), BS, 0, st>>>(*g, dv, tau, s->div_damp, first ? 1 : 0, s->eta, s->eta_prev, s->G,
-                                              s->Gc, s->wall, first ? s->ub0 : s->ub, s->ub, s->acc));
+                 k_bt_u, g1(E), BS, *g, dv, tau, s->div_damp, first ? 1 : 0, s->eta, s->eta_prev, s->G,
+                                              s->Gc, s->wall, first ? s->ub0 : s->ub, s->ub, s->acc);
     }
     fs.air = 0;
     fs.hs = hs;
@@ -1246,13 +1246,13 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   }
   const double fb = flux_bytes(fs, n_, e_, l_);
   ISW_LAUNCH("scale", fb + 6 * n_ * I + 2 * e_ * I + e_ * D + l_ * n_ * D + n_ * D + l_ * n_ * D,
-             k_scale<<<c1((long)N * L), BS1, 0, st>>>(*g, fs, c * dt, s->floor_, h0, s->scale, L));
+             k_scale, c1((long)N * L), BS1, *g, fs, c * dt, s->floor_, h0, s->scale, L);
   ISW_LAUNCH("limit_h", fb + 6 * n_ * (I + C) + 2 * e_ * I + e_ * D + 3 * l_ * n_ * D + n_ * D +
                             (Fm ? l_ * e_ * D : 0),
-             k_limit_h<<<c1((long)N * L), BS1, 0, st>>>(*g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L));
+             k_limit_h, c1((long)N * L), BS1, *g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L);
   if (!s->air)
     ISW_LAUNCH("lay_u", 3 * l_ * e_ * D + 2 * e_ * D + (s->wall ? e_ * D : 0),
-               k_lay_u<<<g1((long)E * L), BS, 0, st>>>(E, L, c * dt, u0, s->Tp, s->ub, s->ub0, s->wall, un));
+               k_lay_u, g1((long)E * L), BS, E, L, c * dt, u0, s->Tp, s->ub, s->ub0, s->wall, un);
 }
 
 int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, double* theta, double* Fm, double dt,
@@ -1262,11 +1262,11 @@ int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, do
   const int N = g->N, L = s->L;
   const double n_ = N, e_ = g->E, l_ = L;
   if (s->air) {
-    ISW_LAUNCH("air_stiff", 3 * l_ * n_ * D, k_air_stiff<<<c1(N), BS1, 0, st>>>(N, p, h, theta, s->S));
+    ISW_LAUNCH("air_stiff", 3 * l_ * n_ * D, k_air_stiff, c1(N), BS1, N, p, h, theta, s->S);
   } else {
-    ISW_LAUNCH("lay_eta", (l_ + 2) * n_ * D, k_lay_eta<<<c1(N), BS1, 0, st>>>(N, L, h, s->b, s->eta0));
+    ISW_LAUNCH("lay_eta", (l_ + 2) * n_ * D, k_lay_eta, c1(N), BS1, N, L, h, s->b, s->eta0);
     ISW_LAUNCH("lay_ubar", l_ * n_ * D + l_ * e_ * D + 2 * e_ * I + (s->wall ? e_ * D : 0) + e_ * D,
-               k_lay_ubar<<<g1(g->E), BS, 0, st>>>(*g, L, s->shelf, h, u, s->wall, s->ub0));
+               k_lay_ubar, g1(g->E), BS, *g, L, s->shelf, h, u, s->wall, s->ub0);
   }
   if (s->nu4 != 0) {
     lap(g, L, u, s->D1, s->Z1, s->L1, st);
@@ -1277,10 +1277,10 @@ int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, do
   stage(g, s, p, dv, s->h2, s->u2, h, u, 1.0, dt, nfast, h, u, Fm, theta, st);
   if ((s->vmix != 0 || s->vmix_cd != 0) && L > 1) {
     ISW_LAUNCH("lay_vmix", l_ * n_ * D + 2 * e_ * I + 2 * l_ * e_ * D + e_ * D + (s->wall ? e_ * D : 0),
-               k_lay_vmix<<<g1(g->E), BS, 0, st>>>(*g, L, s->vmix, dt, s->vmix_cd, h, s->wall, u, s->diss));
+               k_lay_vmix, g1(g->E), BS, *g, L, s->vmix, dt, s->vmix_cd, h, s->wall, u, s->diss);
     if (s->air)
       ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
-                 k_vmix_heat<<<c1(N), BS1, 0, st>>>(*g, p, h, s->diss, theta));
+                 k_vmix_heat, c1(N), BS1, *g, p, h, s->diss, theta);
   }
   return (int)hipGetLastError();
 }
@@ -1293,7 +1293,7 @@ int layers_lap(const isw_grid* g, const isw_layers* s, const double* u, double* 
 int layers_gfast(const isw_grid* g, const isw_layers* s, const double* h, const double* theta, hipStream_t st) {
   if (s->air)
     ISW_LAUNCH("air_gfast", 3.0 * s->L * g->N * D,
-               k_air_gfast<<<c1(g->N), BS1, 0, st>>>(g->N, make_lp(s), s->rd, h, theta, s->Gc));
+               k_air_gfast, c1(g->N), BS1, g->N, make_lp(s), s->rd, h, theta, s->Gc);
   return (int)hipGetLastError();
 }
 
@@ -1301,19 +1301,19 @@ int layers_exchange(const isw_grid* g, const isw_layers* s, const double* wx, do
                     double* psi, double* theta, int seq, double nu4dt, hipStream_t st) {
   const int N = g->N, L = s->L;
   const double n_ = N, e_ = g->E, l_ = L;
-  ISW_LAUNCH("copy", 2 * l_ * n_ * D, k_copy<<<g1((long)N * L), BS, 0, st>>>((long)N * L, h, s->hold));
+  ISW_LAUNCH("copy", 2 * l_ * n_ * D, k_copy, g1((long)N * L), BS, (long)N * L, h, s->hold);
   const bool heat = s->air && theta;
   ISW_LAUNCH("xch_cell", (l_ - 1) * n_ * D + n_ * D + 6 * l_ * n_ * D,
-             k_xch_cell<<<c1(N), BS1, 0, st>>>(N, L, seq, wx, g->areaCell, h, M, psi));
+             k_xch_cell, c1(N), BS1, N, L, seq, wx, g->areaCell, h, M, psi);
   const bool v4 = heat && s->nu4 != 0;
   ISW_LAUNCH("xch_edge", (l_ - 1) * n_ * D + l_ * n_ * D + 2 * e_ * I + 2 * l_ * e_ * D + (heat ? e_ * D : 0) +
                              (v4 ? l_ * e_ * D : 0) + (s->wall ? e_ * D : 0),
-             k_xch_edge<<<g1(g->E), BS, 0, st>>>(*g, L, wx, s->hold, u, heat ? s->diss : nullptr,
+             k_xch_edge, g1(g->E), BS, *g, L, wx, s->hold, u, heat ? s->diss : nullptr,
                                                  v4 ? s->L2 : nullptr, v4 ? nu4dt : 0.0, heat ? s->thin_wind : 0.0,
-                                                 s->floor_, s->wall));
+                                                 s->floor_, s->wall);
   if (heat)
     ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
-               k_vmix_heat<<<c1(N), BS1, 0, st>>>(*g, make_lp(s), h, s->diss, theta));
+               k_vmix_heat, c1(N), BS1, *g, make_lp(s), h, s->diss, theta);
   return (int)hipGetLastError();
 }
 
@@ -1321,14 +1321,14 @@ int vmix_heat(const isw_grid* g, const isw_layers* s, const double* h, const dou
               hipStream_t st) {
   const double n_ = g->N, e_ = g->E, l_ = s->L;
   ISW_LAUNCH("vmix_heat", 6 * n_ * I + 3 * e_ * D + n_ * D + 3 * l_ * n_ * D,
-             k_vmix_heat<<<c1(g->N), BS1, 0, st>>>(*g, make_lp(s), h, diss, theta));
+             k_vmix_heat, c1(g->N), BS1, *g, make_lp(s), h, diss, theta);
   return (int)hipGetLastError();
 }
 
 int mass_update(const isw_grid* g, int L, const double* M0, const double* Fm, double* M1, hipStream_t st) {
   const double n_ = g->N, e_ = g->E, l_ = L;
   ISW_LAUNCH("mass_update", 2 * l_ * n_ * D + l_ * e_ * D + 6 * n_ * I + e_ * I,
-             k_mass_update<<<c2(g->N, L), BS1, 0, st>>>(*g, M0, Fm, M1));
+             k_mass_update, c2(g->N, L), BS1, *g, M0, Fm, M1);
   return (int)hipGetLastError();
 }
 
@@ -1347,9 +1347,9 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
   const double cellE = 6 * n_ * I + 2 * e_ * I;  // cellEdgeAsc, eI/eJ
   const double sten = 8 * e_ * (R + I) + l_ * e_ * D + 2 * e_ * I;
   ISW_LAUNCH("tr_td", 2 * l_ * n_ * D + r_ * n_ * D + l_ * e_ * D + cellE + r_ * n_ * D + mixed * r_ * n_ * 4,
-             k_tr_td<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, M0, psi, Fm, M1, tr->td, pf));
+             k_tr_td, c2(N, rows), BS1, *g, Lm, M0, psi, Fm, M1, tr->td, pf);
   if (order == 1) {
-    ISW_LAUNCH("copy", 2 * r_ * n_ * D, k_copy<<<g1((long)N * rows), BS, 0, st>>>((long)N * rows, tr->td, psi));
+    ISW_LAUNCH("copy", 2 * r_ * n_ * D, k_copy, g1((long)N * rows), BS, (long)N * rows, tr->td, psi);
     return (int)hipGetLastError();
   }
 #if ISW_FORM_MIXED
@@ -1362,21 +1362,21 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
   const fast* p2f = tr->p2;
 #endif
   ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf));
+             k_tr_hflux, g2(E, rows), BS, *g, Lm, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf);
   ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
                              mixed * r_ * n_ * 4,
-             k_tr_stage<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf));
+             k_tr_stage, c2(N, rows), BS1, *g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf);
   ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_hflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf));
+             k_tr_hflux, g2(E, rows), BS, *g, Lm, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf);
   ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
                              mixed * r_ * n_ * 4,
-             k_tr_stage<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf));
+             k_tr_stage, c2(N, rows), BS1, *g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
   ISW_LAUNCH("tr_aflux", 2 * r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_aflux<<<g2(E, rows), BS, 0, st>>>(*g, Lm, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A));
+             k_tr_aflux, g2(E, rows), BS, *g, Lm, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
   ISW_LAUNCH("tr_ratios", 2 * r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 6 * n_ * I + cellE + 2 * r_ * n_ * D,
-             k_tr_ratios<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm));
+             k_tr_ratios, c2(N, rows), BS1, *g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm);
   ISW_LAUNCH("tr_apply", r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 2 * r_ * n_ * D + cellE + r_ * n_ * D,
-             k_tr_apply<<<c2(N, rows), BS1, 0, st>>>(*g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi));
+             k_tr_apply, c2(N, rows), BS1, *g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi);
   return (int)hipGetLastError();
 }
 
@@ -1397,20 +1397,20 @@ int swe_step(const isw_grid* g, const isw_layers* s, double* h, double* u, doubl
   for (int k = 0; k < 3; k++) {
     const double c = cs[k];
     ISW_LAUNCH("swe_scale", l_ * n_ * D + l_ * e_ * D + 6 * n_ * I + 2 * e_ * I + e_ * D + n_ * D + l_ * n_ * D,
-               k_swe_scale<<<c2(N, L), BS1, 0, st>>>(*g, c, hsv[k], usv[k], s->scale));
+               k_swe_scale, c2(N, L), BS1, *g, c, hsv[k], usv[k], s->scale);
     ISW_LAUNCH("swe_flux", l_ * n_ * D + l_ * e_ * D + 2 * e_ * I + l_ * n_ * D + l_ * e_ * D +
                                (k == 2 && Fm ? l_ * e_ * D + e_ * D : 0),
-               k_swe_flux<<<g2(E, L), BS, 0, st>>>(*g, k == 2 ? dt : 0.0, hsv[k], usv[k], s->scale, s->F,
-                                                   k == 2 ? Fm : nullptr));
+               k_swe_flux, g2(E, L), BS, *g, k == 2 ? dt : 0.0, hsv[k], usv[k], s->scale, s->F,
+                                                   k == 2 ? Fm : nullptr);
     ISW_LAUNCH("swe_cell", 2 * l_ * n_ * D + n_ * D + 2 * l_ * e_ * D + 6 * n_ * (I + C) + 2 * e_ * D + n_ * D +
                                2 * l_ * n_ * D,
-               k_swe_cell<<<c2(N, L), BS1, 0, st>>>(*g, s->grav, c, hsv[k], s->b, s->F, usv[k], h, hnv[k], s->B));
+               k_swe_cell, c2(N, L), BS1, *g, s->grav, c, hsv[k], s->b, s->F, usv[k], h, hnv[k], s->B);
     ISW_LAUNCH("swe_vertex", l_ * n_ * D + l_ * e_ * D + 3 * v_ * (2 * I + 2 * C) + 6 * n_ * D + e_ * D +
                                  2 * v_ * D + l_ * v_ * D,
-               k_swe_vertex<<<c2(NV, L), BS1, 0, st>>>(*g, dv, hsv[k], usv[k], s->fV, s->q));
+               k_swe_vertex, c2(NV, L), BS1, *g, dv, hsv[k], usv[k], s->fV, s->q);
     ISW_LAUNCH("swe_edge", l_ * e_ * D + l_ * v_ * D + l_ * n_ * D + l_ * e_ * D + g->n_eoe * (D + I) +
                                (e_ + 1) * I + 4 * e_ * I + e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + 2 * l_ * e_ * D,
-               k_swe_edge_lp<<<g2(E, L), BS, 0, st>>>(*g, p, c, s->F, s->q, s->B, usv[k], s->L2, u, unv[k]));
+               k_swe_edge_lp, g2(E, L), BS, *g, p, c, s->F, s->q, s->B, usv[k], s->L2, u, unv[k]);
   }
   return (int)hipGetLastError();
 }
@@ -1421,8 +1421,8 @@ int couple_stress(const isw_grid* g, int La, const double* ha, double* ua, const
   const double n_ = g->N, e_ = g->E;
   ISW_LAUNCH("couple_stress", 2 * n_ * D + 4 * e_ * D + 2 * e_ * I + e_ * D + 2 * e_ * D + (wall ? e_ * D : 0) +
                                   (cd ? n_ * D : 0),
-             k_couple_stress<<<g1(g->E), BS, 0, st>>>(*g, La, c, rho_w, dt, land_drag, ha, ua, ho, uo, diss, tau_acc,
-                                                      wall, cd));
+             k_couple_stress, g1(g->E), BS, *g, La, c, rho_w, dt, land_drag, ha, ua, ho, uo, diss, tau_acc,
+                                                      wall, cd);
   return (int)hipGetLastError();
 }
 
@@ -1430,7 +1430,7 @@ int land_column(const isw_grid* g, int K, const double* land, double* T, const d
                 const double* dz, double c_soil, double k_soil, double dt, hipStream_t st) {
   const double n_ = g->N;
   ISW_LAUNCH("land_column", n_ * D + 2 * n_ * D + n_ * D + 2.0 * K * n_ * D + K * D,
-             k_land_column<<<c1(g->N), BS1, 0, st>>>(g->N, K, c_soil, k_soil, dt, land, T, cap, Tg, dz));
+             k_land_column, c1(g->N), BS1, g->N, K, c_soil, k_soil, dt, land, T, cap, Tg, dz);
   return (int)hipGetLastError();
 }
 
@@ -1438,12 +1438,12 @@ int ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* cap, cons
             hipStream_t st) {
   const double n_ = g->N;
   ISW_LAUNCH("ocn_cap", n_ * D + 2 * n_ * D + (land ? n_ * D : 0),
-             k_ocn_cap<<<c1(g->N), BS1, 0, st>>>(g->N, rho_cw, c_land, h, cap, land));
+             k_ocn_cap, c1(g->N), BS1, g->N, rho_cw, c_land, h, cap, land);
   return (int)hipGetLastError();
 }
 
 int fill(double* x, long n, double v, hipStream_t st) {
-  if (n > 0) ISW_LAUNCH("fill", 8.0 * n, k_fill<<<g1(n), BS, 0, st>>>(n, v, x));
+  if (n > 0) ISW_LAUNCH("fill", 8.0 * n, k_fill, g1(n), BS, n, v, x);
   return (int)hipGetLastError();
 }
 
