@@ -33,7 +33,11 @@ def parse():
     ap.add_argument('--level', type=int, default=6)
     ap.add_argument('--no-fp64', action='store_true', help='skip the plain FP64 form')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline')
-    ap.add_argument('--cpu-hours', type=float, default=1.0, help='model hours of the CPU sample')
+    ap.add_argument('--cpu-hours', type=float, default=48.0, help='model hours of the CPU sample')
+    ap.add_argument('--ensemble', type=int, default=0, help='run the batched-ensemble workload with this many worlds')
+    ap.add_argument('--ens-days', type=int, default=1, help='model days the ensemble advances')
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--batched', action='store_true', help='ensemble: batch the worlds of each level on one mesh')
     return ap.parse_args()
 
 
@@ -64,8 +68,50 @@ def kernel_profile(w, hours=1):
     return _hip.timing_stop()
 
 
+def ensemble_main(a):
+    """The batched-ensemble workload: `a.ensemble` coupled worlds with seeded levels (4/5/6), planet radius, rotation
+    and gravity, mixed form, all advanced together by `a.ens_days` model days; one JSON line with the aggregate
+    model-hours/s and the p50/p99 wall time per model day over the members' days."""
+    import numpy as np
+    import torch
+    from insilica.ensemble import Ensemble, draw_planets
+    from insilica.world import World
+    torch.cuda.set_device(0)
+    specs = draw_planets(a.ensemble, seed=a.seed)
+    t0 = time.perf_counter()
+    worlds = [World(lev, 'coupled_stack', precision='mixed', params=p) for lev, p in specs]
+    print(f'built {len(worlds)} worlds in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
+    ens = Ensemble(worlds, batch=a.batched)
+    ens.advance(1)                                   # warm-up: the first tick of each kind runs directly, then capture
+    torch.cuda.synchronize()
+    hours = 24 * a.ens_days
+    t0 = time.perf_counter()
+    ens.advance(hours, day_marks=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    lat = np.concatenate([np.asarray(x) for x in ens.day_latencies()])
+    levels = [lev for lev, _ in specs]
+    line = dict(metric='ensemble model-hours/s and wall seconds per model day (p50/p99), 1 GPU',
+                value=round(len(worlds) * hours / el, 2), unit='model-hours/s', n_gpus=1, steps=hours, warmup=1,
+                ms_per_step=round(el / hours * 1e3, 3), higher_is_better=True, scaling='weak', vs_baseline=None,
+                dtype='f64 state, f32 stencils (mixed)', data='synthetic',
+                config=dict(workload=f'{len(worlds)} coupled_stack worlds, levels {sorted(set(levels))} '
+                                     f'({levels.count(4)}/{levels.count(5)}/{levels.count(6)} at 4/5/6), seeded planets '
+                                     f'(radius 0.5-1.5 Earth, rotation 0.5-2x, gravity 3.7-11), {a.ens_days} model days',
+                            parallelism='1 GPU, ' + ('one batch (union mesh) per level, one launch per phase'
+                                                     if a.batched else 'one stream and graph per world')),
+                day_p50_s=round(float(np.percentile(lat, 50)), 4), day_p99_s=round(float(np.percentile(lat, 99)), 4),
+                days_measured=int(lat.size))
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
+    if a.ensemble:
+        if a.gpus != 1:
+            print('bench.py runs on one GPU only', file=sys.stderr, flush=True)
+            sys.exit(2)
+        return ensemble_main(a)
     if a.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for --gpus {a.gpus}): this build has no multi-GPU path; '
               f'run with --gpus 1', file=sys.stderr, flush=True)

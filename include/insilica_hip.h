@@ -48,6 +48,20 @@ typedef struct isw_grid {
   const float *wf, *d2wtf;
 } isw_grid;
 
+/* A batch of worlds stepped together on one mesh: the disjoint union of the worlds' meshes (all of one level, so of
+ * one step), cell c of world w at c + the world's cell offset, likewise edges, vertices and every index table. The
+ * per-world values that differ between planets live in small device tables; everything else (settings, scratch) is
+ * the layer stack's. Each launch then covers every world at once. Host struct, device pointers:
+ *   wc (N), we (E): the world of each cell and edge of the union; grav, nu4 (nworlds): gravity and the del^4
+ *   coefficient; nsub (nworlds x 3): fast sub-steps per stage; tau, inv_n (nworlds x 3): their length c dt / n and
+ *   1 / n (computed as the reference's driver computes them, layers.py:276-277); nmax (3, host): the largest nsub per
+ *   stage. No counterpart in the reference, whose Ensemble runs one graph per world (ensemble.py:17-23). */
+typedef struct isw_batch {
+  int32_t nworlds, nmax[3];
+  const int32_t *wc, *we, *nsub;
+  const double *grav, *nu4, *tau, *inv_n;
+} isw_batch;
+
 /* One stack of layers and its preallocated scratch (reference: src/insilica/layers.py:161-218 Layers.__init__).
  * air != 0: air layers (h is the mass per area, theta per layer); else Boussinesq layers (h in m).
  * gp (L-1), drag (L), stress (L) are values in the struct. fV (NV): the Coriolis parameter with the rotation applied.
@@ -55,7 +69,9 @@ typedef struct isw_grid {
  * Scratch sizes -- (L,N): B, scale, h1, h2, D1, hold, pibar, S, muf, mup, phi; (L,NV): q, Z1;
  * (L,E): Tp, F, u1, u2, L1, L2, hedge, uf, accL; (E): G, He, ubs, ub0, ub, acc, diss; (N): eta0, eta, eta_prev;
  * ufb (L,E) and ubb (E): the second buffers of the fast sub-steps' winds (the sub-steps alternate between two);
- * pk: 3*L*E doubles (plain form); pkf: 3*L*E floats (mixed form; NULL otherwise). */
+ * pk: 3*L*E doubles (plain form); pkf: 3*L*E floats (mixed form; NULL otherwise).
+ * batch: NULL for one world; else the batch the union mesh holds (then nu4 != 0 marks that some world has del^4, and
+ * isw_layers_step's nfast is ignored in favour of the batch's per-world sub-steps). */
 typedef struct isw_layers {
   int32_t L, air, prec, pad_;
   double grav, cp, kappa, p0, rd, nu4, floor_, div_damp, thin, shelf, vmix, vmix_cd, thin_wind;
@@ -65,6 +81,7 @@ typedef struct isw_layers {
   double *B, *q, *Tp, *F, *scale, *G, *He, *ubs, *ub0, *ub, *acc, *eta0, *eta, *eta_prev, *h1, *h2, *u1, *u2;
   double *D1, *Z1, *L1, *L2, *hold, *hedge, *diss, *pibar, *S, *muf, *mup, *phi, *uf, *accL, *pk, *ufb, *ubb;
   float *pkf;
+  const isw_batch* batch;
 } isw_layers;
 
 /* FCT transport scratch for up to rows_max rows (reference: src/insilica/transport.py:19-30 Transport.__init__).

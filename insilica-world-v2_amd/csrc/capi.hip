@@ -49,6 +49,10 @@ bool layers_ok(const isw_grid* g, const isw_layers* s) {
   if (s->prec != ISW_FP64 && s->prec != ISW_MIXED) return false;
   if (s->prec == ISW_MIXED && (!s->pkf || !g->wf || !g->rA || !g->rde || !g->rAV)) return false;
   if (s->prec == ISW_FP64 && !s->pk) return false;
+  const isw_batch* b = s->batch;
+  if (b && (b->nworlds < 1 || !b->wc || !b->we || !b->nsub || !b->grav || !b->nu4 || !b->tau || !b->inv_n ||
+            b->nmax[0] < 1 || b->nmax[1] < 1 || b->nmax[2] < 1))
+    return false;
   return s->fV && s->b && s->Gc && s->B && s->q && s->Tp && s->G && s->He && s->ubs && s->scale && s->h1 && s->h2 &&
          s->u1 && s->u2 && s->D1 && s->Z1 && s->L1 && s->L2 && s->hedge && s->diss && s->hold &&
          (!s->air || (s->pibar && s->S && s->muf && s->mup && s->uf && s->ufb && s->accL)) &&

@@ -68,6 +68,10 @@ struct LP {
   int L, air;
   double grav, cp, kappa, p0, floor_, div_damp, thin, shelf, nu4;
   double gp[ISW_LMAX], drag[ISW_LMAX], stress[ISW_LMAX];
+  // a batch of worlds (all NULL for one world): the world of each cell and edge, per-world gravity, del^4 coefficient,
+  // fast sub-steps, their length and 1/n per stage (isw_batch)
+  const int32_t *wc, *we, *wn;
+  const double *wgrav, *wnu4, *wtau, *winv;
 };
 
 static LP make_lp(const isw_layers* s) {
@@ -76,7 +80,30 @@ static LP make_lp(const isw_layers* s) {
   p.grav = s->grav; p.cp = s->cp; p.kappa = s->kappa; p.p0 = s->p0; p.floor_ = s->floor_; p.div_damp = s->div_damp;
   p.thin = s->thin; p.shelf = s->shelf; p.nu4 = s->nu4;
   for (int k = 0; k < ISW_LMAX; k++) { p.gp[k] = s->gp[k]; p.drag[k] = s->drag[k]; p.stress[k] = s->stress[k]; }
+  const isw_batch* b = s->batch;
+  p.wc = b ? b->wc : nullptr; p.we = b ? b->we : nullptr; p.wn = b ? b->nsub : nullptr;
+  p.wgrav = b ? b->grav : nullptr; p.wnu4 = b ? b->nu4 : nullptr; p.wtau = b ? b->tau : nullptr;
+  p.winv = b ? b->inv_n : nullptr;
   return p;
+}
+
+// Gravity at a cell and the del^4 coefficient at an edge: the world's in a batch, else the stack's.
+__device__ __forceinline__ double grav_at(const LP& p, int i) { return p.wc ? p.wgrav[p.wc[i]] : p.grav; }
+__device__ __forceinline__ double nu4_at(const LP& p, int e) { return p.we ? p.wnu4[p.we[e]] : p.nu4; }
+
+// A fast sub-step's element: in a batch its world's sub-step count and length for this stage (a world whose
+// sub-steps are done skips), else the stack's.
+struct Sub {
+  bool active, last;
+  double tau;
+};
+__device__ __forceinline__ Sub sub_at(const LP& p, const int32_t* wid, int x, int stage, int it, int n, double tau) {
+  if (wid) {
+    const int w = wid[x];
+    n = p.wn[w * 3 + stage];
+    tau = p.wtau[w * 3 + stage];
+  }
+  return Sub{it < n, it == n - 1, tau};
 }
 
 // What the DIVBY divisions take: the quantity (plain form) or its stored reciprocal (mixed form).
@@ -175,12 +202,13 @@ __global__ __launch_bounds__(BS) void k_air_stiff(int N, LP p, const double* __r
                                                   const double* __restrict__ th, double* __restrict__ S) {
   const int i = XTID;
   if (i >= N) return;
+  const double gr = grav_at(p, i);
   double pr = 0;
   for (int k = 0; k < p.L; k++) {
-    pr += p.grav * h[(size_t)k * N + i];
+    pr += gr * h[(size_t)k * N + i];
     const double dth = th[(size_t)k * N + i] - (k + 1 < p.L ? th[(size_t)(k + 1) * N + i] : 0.0);
     const double dpi = p.kappa * p.cp * pow(pr / p.p0, p.kappa) / pr;
-    S[(size_t)k * N + i] = dth > 0 ? dth * p.grav * dpi : 0.0;
+    S[(size_t)k * N + i] = dth > 0 ? dth * gr * dpi : 0.0;
   }
 }
 
@@ -193,13 +221,14 @@ __global__ __launch_bounds__(BS) void k_air_gfast(int N, LP p, double rgas, cons
   for (int k = 0; k < L; k++) mu += h[(size_t)k * N + i];
   int kb = L - 1;
   while (kb > 0 && h[(size_t)kb * N + i] < 2 * p.floor_) kb--;
-  const double Ts = th[(size_t)kb * N + i] * pow(p.grav * mu / p.p0, p.kappa);
+  const double gr = grav_at(p, i);
+  const double Ts = th[(size_t)kb * N + i] * pow(gr * mu / p.p0, p.kappa);
   double gc = rgas * Ts / mu;
   double above = 0;
   for (int j = 0; j < L - 1; j++) {
     above += h[(size_t)j * N + i];
     const double dth = th[(size_t)j * N + i] - th[(size_t)(j + 1) * N + i];
-    if (dth > 0) gc += dth * rgas * pow(p.grav * above / p.p0, p.kappa) * above / (mu * mu);
+    if (dth > 0) gc += dth * rgas * pow(gr * above / p.p0, p.kappa) * above / (mu * mu);
   }
   Gc[i] = gc;
 }
@@ -250,10 +279,11 @@ __global__ __launch_bounds__(512) void k_bern_pv(isw_grid g, LP p, Div dv, int n
     ring6(g.cellEdge, ic, e);
     const double ke = cell_ke(g, u + (size_t)k * E, e);
     if (p.air) {
+      const double gr = grav_at(p, ic);
       double pr = 0, pt = 0;
       for (int m = 0; m <= k; m++) {
         pt = pr;
-        pr = pr + p.grav * h[(size_t)m * N + ic];
+        pr = pr + gr * h[(size_t)m * N + ic];
       }
       const double x = pow(pr / p.p0, p.kappa);
       sx[k][j] = x;
@@ -261,7 +291,7 @@ __global__ __launch_bounds__(512) void k_bern_pv(isw_grid g, LP p, Div dv, int n
       if (!live) return;
       const double xt = k ? sx[k - 1][j] : 0.0;
       pibar[(size_t)k * N + i] = p.cp * (pr * x - pt * xt) / ((p.kappa + 1) * (pr - pt));
-      double M = p.grav * b[i] + th[(size_t)(L - 1) * N + i] * p.cp * sx[L - 1][j];
+      double M = gr * b[i] + th[(size_t)(L - 1) * N + i] * p.cp * sx[L - 1][j];
       for (int m = L - 2; m >= k; m--) M = M + (th[(size_t)m * N + i] - th[(size_t)(m + 1) * N + i]) * p.cp * sx[m][j];
       double P[ISW_LMAX];
       double a = 0;
@@ -386,7 +416,7 @@ __global__ __launch_bounds__(512) void k_slow(isw_grid g, LP p, Div dv, double d
     const double* pl = pibar + (size_t)k * N;
     d = d + DIVBY(0.5 * (pl[eI] + pl[eJ]) * (tl[eJ] - tl[eI]), dv.de[ec]);
   }
-  if (p.nu4 != 0) d = d - p.nu4 * lap4[o];
+  if (lap4) d = d - nu4_at(p, ec) * lap4[o];
   sT[k][j] = d;
   sh[k][j] = he;
   su[k][j] = ue;
@@ -433,7 +463,8 @@ __global__ __launch_bounds__(512) void k_slow(isw_grid g, LP p, Div dv, double d
 // The mass sub-step: (64, L) blocks, thread (x, y) updates layer y's mass of one of 64 cells (its six edges gathered
 // at once); the column's forward-weighted masses and the linear potential Phi* are then summed through LDS in the
 // reference's order (P from the top, Phi from the bottom) by every thread for its own layer.
-__global__ __launch_bounds__(512) void k_af_muphi(isw_grid g, LP p, Div dv, double tau, const double* __restrict__ hedge,
+__global__ __launch_bounds__(512) void k_af_muphi(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
+                                                  const double* __restrict__ hedge,
                                                   const double* __restrict__ uf, const double* mu_src, double* mu,
                                                   const double* __restrict__ S, double* __restrict__ phi) {
   __shared__ double sa[ISW_LMAX][BS1], sS[ISW_LMAX][BS1];
@@ -442,6 +473,7 @@ __global__ __launch_bounds__(512) void k_af_muphi(isw_grid g, LP p, Div dv, doub
   const int i = xblock() * BS1 + j;
   const bool live = i < N;
   const int ic = live ? i : 0;
+  const Sub sb = sub_at(p, p.wc, ic, stage, it, n, tau);
   int e[6];
   ring6(g.cellEdge, ic, e);
   const double* hl = hedge + (size_t)k * E;
@@ -458,11 +490,11 @@ __global__ __launch_bounds__(512) void k_af_muphi(isw_grid g, LP p, Div dv, doub
     if (e[m] >= 0) div += t[m];
   const size_t o = (size_t)k * N + ic;
   const double m0 = mu_src[o];
-  const double m1 = m0 - DIVBY(tau * div, dv.A[ic]);
+  const double m1 = m0 - DIVBY(sb.tau * div, dv.A[ic]);
   sa[k][j] = m1 + p.div_damp * (m1 - m0);
   sS[k][j] = S[o];
   __syncthreads();
-  if (!live) return;
+  if (!live || !sb.active) return;
   mu[o] = m1;
   double P[ISW_LMAX];
   double a = 0;
@@ -472,27 +504,31 @@ __global__ __launch_bounds__(512) void k_af_muphi(isw_grid g, LP p, Div dv, doub
   phi[o] = ph;
 }
 
-__global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, double tau, int first,
+__global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, LP p, int stage, int it, int n, double tau,
                                              const double* __restrict__ phi, const double* __restrict__ Tp,
-                                             const double* __restrict__ G, const double* uin, double* acc,
-                                             double* uout) {
+                                             const double* __restrict__ G, const double* uin, double* acc, double* uf,
+                                             double* un) {
   const int t = XTID;
   if (t >= g.E * L) return;
   const int l = t / g.E, e = t - l * g.E;
+  const Sub sb = sub_at(p, p.we, e, stage, it, n, tau);
+  if (!sb.active) return;
   const double v = uin[t];
-  acc[t] = first ? 0.0 + v : acc[t] + v;
+  acc[t] = it == 0 ? 0.0 + v : acc[t] + v;
   const double* pl = phi + (size_t)l * g.N;
-  uout[t] = v + tau * (DIVBY(-(pl[g.eJ[e]] - pl[g.eI[e]]), dv.de[e]) + Tp[t] + G[e]);
+  (sb.last ? un : uf)[t] = v + sb.tau * (DIVBY(-(pl[g.eJ[e]] - pl[g.eI[e]]), dv.de[e]) + Tp[t] + G[e]);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
 // Ocean barotropic sub-steps (layers.cu:366-388) with the wall mask of the reference's lay_mask calls fused in.
 // ---------------------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(BS) void k_bt_eta(isw_grid g, Div dv, double tau, const double* __restrict__ He,
-                                               const double* __restrict__ ub, const double* eta_src, double* eta,
-                                               double* __restrict__ eta_prev) {
+__global__ __launch_bounds__(BS) void k_bt_eta(isw_grid g, Div dv, LP p, int stage, int it, int n, double tau,
+                                               const double* __restrict__ He, const double* __restrict__ ub,
+                                               const double* eta_src, double* eta, double* __restrict__ eta_prev) {
   const int i = XTID;
   if (i >= g.N) return;
+  const Sub sb = sub_at(p, p.wc, i, stage, it, n, tau);
+  if (!sb.active) return;
   const double e0 = eta_src[i];
   eta_prev[i] = e0;
   int e[6];
@@ -507,22 +543,24 @@ __global__ __launch_bounds__(BS) void k_bt_eta(isw_grid g, Div dv, double tau, c
 #pragma unroll
   for (int m = 0; m < 6; m++)
     if (e[m] >= 0) div += t[m];
-  eta[i] = e0 - DIVBY(tau * div, dv.A[i]);
+  eta[i] = e0 - DIVBY(sb.tau * div, dv.A[i]);
 }
 
-__global__ __launch_bounds__(BS) void k_bt_u(isw_grid g, Div dv, double tau, double gamma, int first,
-                                             const double* __restrict__ eta, const double* __restrict__ eta_prev,
-                                             const double* __restrict__ G, const double* __restrict__ Gc,
-                                             const double* __restrict__ wall, const double* ub_src, double* ub,
-                                             double* acc) {
+__global__ __launch_bounds__(BS) void k_bt_u(isw_grid g, Div dv, LP p, int stage, int it, int n, double tau,
+                                             double gamma, const double* __restrict__ eta,
+                                             const double* __restrict__ eta_prev, const double* __restrict__ G,
+                                             const double* __restrict__ Gc, const double* __restrict__ wall,
+                                             const double* ub_src, double* ub, double* acc) {
   const int e = XTID;
   if (e >= g.E) return;
+  const Sub sb = sub_at(p, p.we, e, stage, it, n, tau);
+  if (!sb.active) return;
   const double v = ub_src[e];
-  double a = first ? 0.0 + v : acc[e] + v;
+  double a = it == 0 ? 0.0 + v : acc[e] + v;
   const int i = g.eI[e], j = g.eJ[e];
   const double ej = Gc[j] * (eta[j] + gamma * (eta[j] - eta_prev[j]));
   const double ei = Gc[i] * (eta[i] + gamma * (eta[i] - eta_prev[i]));
-  double un = v + tau * (DIVBY(-(ej - ei), dv.de[e]) + G[e]);
+  double un = v + sb.tau * (DIVBY(-(ej - ei), dv.de[e]) + G[e]);
   if (wall && wall[e] == 0) { un = 0; a = 0; }
   ub[e] = un;
   acc[e] = a;
@@ -533,18 +571,21 @@ __global__ __launch_bounds__(BS) void k_bt_u(isw_grid g, Div dv, double tau, dou
 // air F = h_e acc / n (af_flux); ocean F = h_e (u - ubar*) + h_e ubar_avg (lay_flux), 0 on walls.
 // ---------------------------------------------------------------------------------------------------------------
 struct FluxSrc {
-  int air;
+  int air, stage;
   double inv_n, shelf;
+  const int32_t* we;    // a batch's world per edge and per-world 1/n (else NULL: inv_n)
+  const double* winv;
   const double *hedge, *acc;                  // air
   const double *hs, *us, *ubs, *bacc, *wall;  // ocean
 };
 
 __device__ __forceinline__ double stage_flux(const FluxSrc& f, const isw_grid& g, int l, int e) {
   const size_t o = (size_t)l * g.E + e;
-  if (f.air) return f.hedge[o] * f.acc[o] * f.inv_n;
+  const double inv_n = f.we ? f.winv[f.we[e] * 3 + f.stage] : f.inv_n;
+  if (f.air) return f.hedge[o] * f.acc[o] * inv_n;
   if (f.wall && f.wall[e] == 0) return 0.0;
   const double he = edge_h(f.hs[(size_t)l * g.N + g.eI[e]], f.hs[(size_t)l * g.N + g.eJ[e]], f.shelf);
-  return he * (f.us[o] - f.ubs[e]) + he * (f.bacc[e] * f.inv_n);
+  return he * (f.us[o] - f.ubs[e]) + he * (f.bacc[e] * inv_n);
 }
 
 __global__ __launch_bounds__(BS) void k_scale(isw_grid g, FluxSrc fs, double dts, double floor_,
@@ -663,10 +704,11 @@ __global__ __launch_bounds__(BS) void k_vmix_heat(isw_grid g, LP p, const double
   q /= g.areaCell[i];
   double mu = 0;
   for (int k = 0; k < L; k++) mu += h[(size_t)k * N + i];
+  const double gr = grav_at(p, i);
   double pr = 0, xt = 0;
   for (int k = 0; k < L; k++) {
     const double pt = pr;
-    pr = pr + p.grav * h[(size_t)k * N + i];
+    pr = pr + gr * h[(size_t)k * N + i];
     const double x = pow(pr / p.p0, p.kappa);
     const double pib = p.cp * (pr * x - pt * xt) / ((p.kappa + 1) * (pr - pt));
     th[(size_t)k * N + i] += q / (mu * pib);
@@ -1181,7 +1223,7 @@ static void lap(const isw_grid* g, int L, const double* u, double* D1, double* Z
 }
 
 // Layers._stage (layers.py:255-309).
-static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div& dv, const double* hs,
+static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div& dv, int si, const double* hs,
                   const double* us, const double* h0, const double* u0, double c, double dt, int nfast, double* hn,
                   double* un, double* Fm, const double* theta, hipStream_t st) {
   const int N = g->N, E = g->E, L = s->L;
@@ -1212,16 +1254,22 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   FluxSrc fs{};
   fs.inv_n = 1.0 / n;
   fs.shelf = s->shelf;
+  fs.stage = si;
+  if (s->batch) {                                  // every world's own sub-steps: the loop runs the largest count
+    n = s->batch->nmax[si];
+    fs.we = s->batch->we;
+    fs.winv = s->batch->inv_n;
+  }
   if (s->air) {
     for (int it = 0; it < n; it++) {
-      const bool first = it == 0, last = it == n - 1;
+      const bool first = it == 0;
       ISW_LAUNCH("af_muphi", 2 * l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 4 * l_ * n_ * D,
-                 k_af_muphi, c1(N), dim3(BS1, L), *g, p, dv, tau, s->hedge, first ? u0 : s->uf,
+                 k_af_muphi, c1(N), dim3(BS1, L), *g, p, dv, si, it, n, tau, s->hedge, first ? u0 : s->uf,
                                                              first ? h0 : s->muf, s->muf, s->S, s->phi);
       ISW_LAUNCH("af_u", l_ * n_ * D + l_ * e_ * D + e_ * D + 2 * e_ * I + e_ * D + 2 * l_ * e_ * D +
                              (first ? 1 : 2) * l_ * e_ * D,
-                 k_af_u, g1((long)E * L), BS, *g, L, dv, tau, first ? 1 : 0, s->phi, s->Tp, s->G,
-                                                        first ? u0 : s->uf, s->accL, last ? un : s->uf);
+                 k_af_u, g1((long)E * L), BS, *g, L, dv, p, si, it, n, tau, s->phi, s->Tp, s->G,
+                 first ? u0 : s->uf, s->accL, s->uf, un);
     }
     fs.air = 1;
     fs.hedge = s->hedge;
@@ -1230,11 +1278,11 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     for (int it = 0; it < n; it++) {
       const bool first = it == 0;
       ISW_LAUNCH("bt_eta", 2 * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 3 * n_ * D,
-                 k_bt_eta, c1(N), BS1, *g, dv, tau, s->He, first ? s->ub0 : s->ub,
+                 k_bt_eta, c1(N), BS1, *g, dv, p, si, it, n, tau, s->He, first ? s->ub0 : s->ub,
                                                  first ? s->eta0 : s->eta, s->eta, s->eta_prev);
       ISW_LAUNCH("bt_u", 3 * n_ * D + e_ * D + 2 * e_ * I + e_ * D + (s->wall ? e_ * D : 0) + 2 * e_ * D +
                              (first ? 1 : 2) * e_ * D,
-                 k_bt_u, g1(E), BS, *g, dv, tau, s->div_damp, first ? 1 : 0, s->eta, s->eta_prev, s->G,
+                 k_bt_u, g1(E), BS, *g, dv, p, si, it, n, tau, s->div_damp, s->eta, s->eta_prev, s->G,
                                               s->Gc, s->wall, first ? s->ub0 : s->ub, s->ub, s->acc);
     }
     fs.air = 0;
@@ -1272,9 +1320,9 @@ int layers_step(const isw_grid* g, const isw_layers* s, double* h, double* u, do
     lap(g, L, u, s->D1, s->Z1, s->L1, st);
     lap(g, L, s->L1, s->D1, s->Z1, s->L2, st);
   }
-  stage(g, s, p, dv, h, u, h, u, 1.0 / 3.0, dt, nfast, s->h1, s->u1, nullptr, theta, st);
-  stage(g, s, p, dv, s->h1, s->u1, h, u, 1.0 / 2.0, dt, nfast, s->h2, s->u2, nullptr, theta, st);
-  stage(g, s, p, dv, s->h2, s->u2, h, u, 1.0, dt, nfast, h, u, Fm, theta, st);
+  stage(g, s, p, dv, 0, h, u, h, u, 1.0 / 3.0, dt, nfast, s->h1, s->u1, nullptr, theta, st);
+  stage(g, s, p, dv, 1, s->h1, s->u1, h, u, 1.0 / 2.0, dt, nfast, s->h2, s->u2, nullptr, theta, st);
+  stage(g, s, p, dv, 2, s->h2, s->u2, h, u, 1.0, dt, nfast, h, u, Fm, theta, st);
   if ((s->vmix != 0 || s->vmix_cd != 0) && L > 1) {
     ISW_LAUNCH("lay_vmix", l_ * n_ * D + 2 * e_ * I + 2 * l_ * e_ * D + e_ * D + (s->wall ? e_ * D : 0),
                k_lay_vmix, g1(g->E), BS, *g, L, s->vmix, dt, s->vmix_cd, h, s->wall, u, s->diss);

@@ -27,6 +27,12 @@ LAYER_SCRATCH = ('B', 'q', 'Tp', 'F', 'scale', 'G', 'He', 'ubs', 'ub0', 'ub', 'a
                  'uf', 'accL', 'pk', 'ufb', 'ubb')
 
 
+class IswBatch(ctypes.Structure):
+    _fields_ = ([('nworlds', ctypes.c_int32), ('nmax', ctypes.c_int32 * 3)]
+                + [(k, _i32p) for k in ('wc', 'we', 'nsub')]
+                + [(k, _f64p) for k in ('grav', 'nu4', 'tau', 'inv_n')])
+
+
 class IswLayers(ctypes.Structure):
     _fields_ = ([(k, ctypes.c_int32) for k in ('L', 'air', 'prec', 'pad_')]
                 + [(k, ctypes.c_double) for k in ('grav', 'cp', 'kappa', 'p0', 'rd', 'nu4', 'floor_', 'div_damp', 'thin',
@@ -34,7 +40,7 @@ class IswLayers(ctypes.Structure):
                 + [(k, ctypes.c_double * LMAX) for k in ('gp', 'drag', 'stress')]
                 + [(k, _f64p) for k in ('fV', 'b', 'wall', 'Gc')]
                 + [(k, _f64p) for k in LAYER_SCRATCH]
-                + [('pkf', _f32p)])
+                + [('pkf', _f32p), ('batch', ctypes.POINTER(IswBatch))])
 
 
 class IswTransport(ctypes.Structure):

@@ -149,21 +149,40 @@ def aqua_air(g):
     return h, np.zeros(g.N), np.zeros((3, g.E)), dict(theta=theta, Ts=Ts, q=q)
 
 
-def harmonic_field(g, rng, lmax=20, pos=None):
-    """A random-phase field of spherical-harmonic degree <= lmax at unit positions pos (default: the cells), scaled
-    to a maximum of 1: sum over l, m of a_lm P_l^m(sin lat) cos(m lon + phase), with Gaussian a_lm ~ 1/(l+1)."""
-    from scipy.special import sph_harm_y
-    pos = g.pos if pos is None else pos
-    lat = np.arcsin(np.clip(pos[:, 2], -1, 1))
-    lon = np.arctan2(pos[:, 1], pos[:, 0])
-    theta_c = math.pi / 2 - lat
-    f = np.zeros(len(pos))
+def harmonic_fields(rng, pos, count, lmax=20):
+    """`count` random-phase fields of spherical-harmonic degree <= lmax at unit positions pos, each scaled to a
+    maximum of 1: sum over l, m of a_lm P_l^m(sin lat) cos(m lon + phase_lm), Gaussian a_lm ~ 1/(l+1), phases uniform.
+    Each (l, m) basis function is evaluated once for all the fields."""
+    P, lon = _legendre(pos, lmax)
+    A = np.zeros((lmax + 1, count, len(pos)))                       # per order m: sum_l a cos(phase) P_lm
+    B = np.zeros((lmax + 1, count, len(pos)))                       # per order m: sum_l a sin(phase) P_lm
     for l in range(1, lmax + 1):
         for m in range(0, l + 1):
-            a, ph = rng.standard_normal() / (l + 1), rng.uniform(0, 2 * math.pi)
-            Y = sph_harm_y(l, m, theta_c, lon)
-            f += a * np.real(Y * np.exp(1j * ph))
-    return f / np.max(np.abs(f))
+            a = rng.standard_normal(count) / (l + 1)
+            ph = rng.uniform(0, 2 * math.pi, count)
+            A[m] += (a * np.cos(ph))[:, None] * P[(l, m)][None, :]
+            B[m] += (a * np.sin(ph))[:, None] * P[(l, m)][None, :]
+    f = np.zeros((count, len(pos)))
+    for m in range(lmax + 1):                                       # cos(m lon + ph) = cos m lon cos ph - sin m lon sin ph
+        f += A[m] * np.cos(m * lon)[None, :] - B[m] * np.sin(m * lon)[None, :]
+    return f / np.max(np.abs(f), axis=1, keepdims=True)
+
+
+_LEGENDRE = {}
+
+
+def _legendre(pos, lmax):
+    """The normalised associated Legendre functions P_lm(sin lat) at unit positions (cached per position set)."""
+    key = (len(pos), lmax, pos[0].tobytes(), pos[-1].tobytes())
+    if key not in _LEGENDRE:
+        from scipy.special import sph_harm_y
+        lat = np.arcsin(np.clip(pos[:, 2], -1, 1))
+        theta_c = math.pi / 2 - lat
+        P = {(l, m): np.real(sph_harm_y(l, m, theta_c, 0.0)) for l in range(1, lmax + 1) for m in range(0, l + 1)}
+        if len(_LEGENDRE) > 8:
+            _LEGENDRE.clear()
+        _LEGENDRE[key] = (P, np.arctan2(pos[:, 1], pos[:, 0]))
+    return _LEGENDRE[key]
 
 
 def land_mask(g, rng, frac=0.3, smooth=2):
@@ -183,10 +202,9 @@ def coupled_stack(g, seed=0, lmax=20, dtheta=2.0, du=5.0, land_frac=0.3):
     layer's theta and +-du m/s on every air layer's wind (sampled at the edge midpoints along the normals)."""
     rng = np.random.default_rng(seed)
     h, b, u, x = aqua_air(g)
-    theta = x['theta'] + dtheta * np.stack([harmonic_field(g, rng, lmax) for _ in range(3)])
+    theta = x['theta'] + dtheta * harmonic_fields(rng, g.pos, 3, lmax)
     lat_e, _, en, nn = edge_east_north(g)
-    ue = np.stack([harmonic_field(g, rng, lmax, pos=g.xm) for _ in range(3)])
-    vn = np.stack([harmonic_field(g, rng, lmax, pos=g.xm) for _ in range(3)])
-    u = du * (ue * en + vn * nn) / math.sqrt(2.0)
+    w = harmonic_fields(rng, g.xm, 6, lmax)
+    u = du * (w[:3] * en + w[3:] * nn) / math.sqrt(2.0)
     land = land_mask(g, rng, land_frac)
     return dict(h=h, b=b, u=u, theta=theta, q=x['q'], Ts=x['Ts'], land=land)

@@ -54,7 +54,7 @@ class DeviceGrid:
             setattr(self, k, put(getattr(grid, k), np.int8))
         for k in self.REALS:
             setattr(self, k, put(getattr(grid, k), np.float64))
-        idx, wt = d2_stencil(grid)
+        idx, wt = grid.d2 if hasattr(grid, 'd2') else d2_stencil(grid)
         self.d2idx, self.d2wt = put(idx, np.int32), put(wt, np.float64)
         self.rA, self.rde, self.rAV = 1.0 / self.areaCell, 1.0 / self.de, 1.0 / self.areaV
         self.wf, self.d2wtf = self.w.float(), self.d2wt.float()

@@ -25,7 +25,7 @@ DIV_DAMP = 0.1              # the fast mode's divergence damping
 class Layers:
     def __init__(self, dg, layers=1, b=None, grav=GRAV, gprime=(), drag=0.0, stress=0.0, nu4=0.0, floor=1.0,
                  rotation=1.0, div_damp=DIV_DAMP, air=None, vmix=0.0, thin=2.0, wall=None, thin_wind=0.0, shelf=0.0,
-                 vmix_cd=0.0, shelf_bed=False):
+                 vmix_cd=0.0, shelf_bed=False, batch=None):
         """Same arguments as the reference's Layers (layers.py:161-218). air: None for Boussinesq layers, or a dict
         (cp, kappa, rd) for air layers. wall: None or an (E,) mask, 0 on edges no water crosses. shelf_bed is not
         supported (the reference's squeezed-layer flags, off by default)."""
@@ -74,8 +74,29 @@ class Layers:
         s.pkf = ctypes.cast(self.pkf.data_ptr(), _hip._f32p) if self.pkf is not None else None
         self.s = s
         self.ref = ctypes.byref(s)
+        self.batch = None
+        if batch is not None:
+            self._set_batch(batch)
         self.dmin = float(np.min(dg.host.de))
         self.nu4dt = 0.0
+
+    def _set_batch(self, b):
+        """A batch of worlds on a union mesh (insilica.batch): per-world gravity, del^4 coefficient and fast
+        sub-steps through the isw_batch table; each cell's fast-mode coefficient starts at its world's gravity."""
+        self.batch = b
+        t = _hip.IswBatch()
+        t.nworlds = b['nworlds']
+        for k in range(3):
+            t.nmax[k] = int(b['nmax'][k])
+        for k in ('wc', 'we', 'nsub'):
+            setattr(t, k, ctypes.cast(b[k].data_ptr(), _hip._i32p))
+        for k in ('grav', 'nu4', 'tau', 'inv_n'):
+            setattr(t, k, ctypes.cast(b[k].data_ptr(), _hip._f64p))
+        self._batch_struct = t
+        self.s.batch = ctypes.pointer(t)
+        self.s.nu4 = float(torch.max(b['nu4']))
+        self.nu4 = self.s.nu4
+        self.Gc.copy_(b['grav'][b['wc'].long()])
 
     def mask(self, a, rows):
         """Zero the wall edges of `rows` rows of the edge field a (no-op without walls)."""

@@ -109,6 +109,9 @@ class World:
         self.tick = 0
         self._Fm, self._M1 = dg.zeros(self.L, dg.E), dg.zeros(self.L, dg.N)
         self._graphs, self._tick0 = {}, 0
+        # the world's own stream (default priority): its ticks queue there, so an ensemble's members overlap
+        self._stream = torch.cuda.Stream()
+        self._stream.wait_stream(torch.cuda.current_stream())
         self.STATE = ('h', 'u', 'M', 'rows') + (('sea.h', 'sea.u', 'sea.M', 'sea.rows', 'sea.Tg')
                                                  if self.sea is not None else ())
 
@@ -133,26 +136,36 @@ class World:
                 self.sea.step(mix=False)
 
     def launch_tick(self, graph=True):
-        """Queue one tick on the current stream without waiting for it."""
+        """Queue one tick on the world's stream without waiting for it (step() and Ensemble.step() call this)."""
         kind = self._kind(self.tick)
         gr = self._graphs.get(kind)
-        if graph and gr is None and self.tick > self._tick0:
-            gr = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(gr):
+        with torch.cuda.stream(self._stream):
+            if graph and gr is None and self.tick > self._tick0:
+                gr = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(gr):
+                    self._tick(kind)
+                self._graphs[kind] = gr
+            if graph and gr is not None:
+                gr.replay()
+            else:
                 self._tick(kind)
-            self._graphs[kind] = gr
-        if graph and gr is not None:
-            gr.replay()
-        else:
-            self._tick(kind)
         self.tick += 1
 
     def step(self, k=1, graph=True):
         """k ticks; the first runs directly, later ones replay a captured graph per tick kind."""
         for _ in range(k):
             self.launch_tick(graph)
-        torch.cuda.synchronize()
+        self.sync()
         return self
+
+    def sync(self):
+        """Wait for the world's queued ticks; later work on the current stream sees their results."""
+        self._stream.synchronize()
+        torch.cuda.current_stream().wait_stream(self._stream)
+
+    @property
+    def ticks_per_hour(self):
+        return 3600.0 / self.dt
 
     # ---- diagnostics ----
     def totals(self):
