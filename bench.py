@@ -38,6 +38,7 @@ def parse():
     ap.add_argument('--ens-days', type=int, default=1, help='model days the ensemble advances')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--batched', action='store_true', help='ensemble: batch the worlds of each level on one mesh')
+    ap.add_argument('--serve', type=int, default=0, help='run the serving workload with this many requests')
     return ap.parse_args()
 
 
@@ -105,8 +106,36 @@ def ensemble_main(a):
     print(json.dumps(line), flush=True)
 
 
+def serve_main(a):
+    """The serving workload: `a.serve` requests with seeded Poisson arrivals (mean 2/s) at levels 5-6, 1-30 model
+    days each, seeded planets; one JSON line with the aggregate model-hours/s and p50/p99 latency per model day."""
+    import torch
+    from insilica.serve import Server, draw_requests
+    torch.cuda.set_device(0)
+    from insilica import grid as gm
+    for n in (5, 6):
+        gm.get(n)                                    # the meshes are built (or loaded) before the clock starts
+    reqs = draw_requests(a.serve, seed=a.seed, rate=2.0, levels=(5, 6))
+    done, st = Server(precision='mixed').run(reqs)
+    line = dict(metric='serving: model-hours/s and latency per model day (p50/p99), 1 GPU',
+                value=round(st['model_hours_per_s'], 2), unit='model-hours/s', n_gpus=1, steps=len(done), warmup=0,
+                ms_per_step=round(st['seconds'] / len(done) * 1e3, 3), higher_is_better=True, scaling='weak',
+                vs_baseline=None, dtype='f64 state, f32 stencils (mixed)', data='synthetic',
+                config=dict(workload=f'{len(done)} coupled_stack requests, Poisson arrivals 2/s (seed {a.seed}), levels '
+                                     f'5-6, 1-30 model days, seeded planets; world construction inside the timed loop',
+                            parallelism='1 GPU, continuous admission, one stream and graph per world'),
+                day_p50_s=round(st['day_p50_s'], 4), day_p99_s=round(st['day_p99_s'], 4),
+                model_hours=st['model_hours'], seconds=round(st['seconds'], 3))
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
+    if a.serve:
+        if a.gpus != 1:
+            print('bench.py runs on one GPU only', file=sys.stderr, flush=True)
+            sys.exit(2)
+        return serve_main(a)
     if a.ensemble:
         if a.gpus != 1:
             print('bench.py runs on one GPU only', file=sys.stderr, flush=True)

@@ -43,3 +43,11 @@ def test_bad_arguments_rejected_on_host():
     tr = _hip.IswTransport()
     tr.L, tr.rows_max = 2, 4
     assert lib.isw_transport_step(ctypes.byref(g), ctypes.byref(tr), None, None, None, None, 3, 3, None) == E
+
+
+def test_request_draws_are_seeded():
+    from insilica.serve import draw_requests
+    a, b = draw_requests(20, seed=3), draw_requests(20, seed=3)
+    assert [(r.arrival, r.level, r.days) for r in a] == [(r.arrival, r.level, r.days) for r in b]
+    assert all(5 <= r.level <= 6 and 1 <= r.days <= 30 for r in a)
+    assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
