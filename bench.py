@@ -39,6 +39,8 @@ def parse():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--batched', action='store_true', help='ensemble: batch the worlds of each level on one mesh')
     ap.add_argument('--serve', type=int, default=0, help='run the serving workload with this many requests')
+    ap.add_argument('--radiation', action='store_true', help='run the column-radiation workload (passes/s)')
+    ap.add_argument('--cpu-cols', type=int, default=0, help='radiation: columns in the CPU oracle sample (0: all)')
     return ap.parse_args()
 
 
@@ -129,8 +131,128 @@ def serve_main(a):
     print(json.dumps(line), flush=True)
 
 
+# Algorithmic operations per (column, g-point, sub-level), clear sky, counted from the kernels' formulas: adds and
+# multiplies 1 each, and exp, division and square root at their instruction cost (FP64: 20, 10, 10; FP32: 4, 4, 4 --
+# the transcendental unit's quarter rate). Longwave: 6 bilinear ln k lookups (12 each) and exps, the optical depth (12),
+# two Planck lookups per pass (6 each), the two-stream step down (11, exp, div) and up (clear and all-sky: 22, 3 exps,
+# 2 divs, 2 products). Shortwave: 7 lookups and exps, the optical depth (14), scattering terms (10, 2 divs),
+# Delta-Eddington (60, 2 exps, sqrt, 3 divs), the clear and all-sky albedos (20, 2 divs), the downward pass (15, div).
+def rad_ops(form):
+    ex, dv = (20, 10) if form == 'double' else (4, 4)
+    lw = 6 * 12 + 12 + 2 * 12 + 11 + 22 + 2 + ex * (6 + 1 + 3) + dv * (1 + 2)
+    sw = 7 * 12 + 14 + 10 + 60 + 20 + 15 + ex * (7 + 2) + dv * (2 + 3 + 1 + 2 + 1)
+    return lw, sw
+
+
+def radiation_main(a):
+    """The column-radiation workload: 40,962 columns x 7 layers (the level-6 mesh's cells), synthetic k-tables with
+    16 x 16 longwave and 14 x 16 shortwave g-points; one step = one radiation pass (lw_setup, lw_column, sw_column)
+    over every column, the state resident in HBM. Mixed form as `value`, the FP64 form beside it."""
+    import numpy as np
+    import torch
+    from insilica import _hip
+    from insilica import grid as gm
+    from insilica.atmosphere import par_array
+    from insilica.ktables import column_state, synthetic_lw, synthetic_sw
+    from insilica.radiation import Longwave, Shortwave
+    torch.cuda.set_device(0)
+    g = gm.get(6)
+    L = 7
+    lwk, swk = synthetic_lw(), synthetic_sw()
+    pos = np.ascontiguousarray(g.pos, dtype=np.float64)
+    c = column_state(pos, layers=L, seed=a.seed)
+    par = par_array(L, dict(edge_k=2.0, theta_stars=[500.0, 380.0, 330.0, 305.0, 204.0, 200.0]))
+    dev = lambda x: torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device='cuda')
+    mu, th, q, Ts, alb, cl = (dev(c[k]) for k in ('mu', 'theta', 'q', 'Ts', 'alb', 'cloudo'))
+    p = dev(pos)
+    t = dev([2.0e6])
+    N = g.N
+
+    def build(form):
+        lw = Longwave(N, L, par, tables=lwk, precision=form)
+        sw = Shortwave(N, L, par, lw, tables=swk, ice_re=30e-6, precision=form, cloud_ssa=swk['cloud_ssa'],
+                       band_top=swk['band_top'])
+        return lw, sw
+
+    def one(lw, sw):
+        lw(mu, th, q, cl, Ts)
+        sw(0, t, 3600.0, p, alb, cl)
+
+    def timed(form):
+        lw, sw = build(form)
+        for _ in range(a.warmup):
+            one(lw, sw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            one(lw, sw)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        _hip.timing_start()
+        one(lw, sw)
+        torch.cuda.synchronize()
+        return el, _hip.timing_stop()
+
+    el_m, prof_m = timed('mixed')
+    print(f'mixed: {a.steps} passes in {el_m:.3f} s', file=sys.stderr, flush=True)
+    el_d, prof_d = (None, None) if a.no_fp64 else timed('double')
+    if el_d:
+        print(f'fp64: {a.steps} passes in {el_d:.3f} s', file=sys.stderr, flush=True)
+    NGl, NGs, NS = lwk['logk'].shape[3], swk['logk'].shape[3], 4 * L
+
+    def roof(prof, form):
+        lw_ops, sw_ops = rad_ops(form)
+        ops = dict(lw_column=lw_ops * NGl * NS * N, sw_column=sw_ops * NGs * NS * N)
+        name = max(ops, key=lambda k: prof[k]['ms'])
+        avg = prof[name]['ms'] / prof[name]['launches'] / 1e3
+        ach = ops[name] / avg / 1e12
+        peak = 78.6 if form == 'double' else 157.3
+        return dict(bound='valu', kernel=name, achieved=round(ach, 2), peak=peak, unit='TFLOP/s',
+                    frac=round(ach / peak, 4), traffic=None, ops_per_launch=ops[name], avg_us=round(avg * 1e6, 2),
+                    kernels_us={k: round(v['ms'] / v['launches'] * 1e3, 2) for k, v in prof.items()})
+    cpu = None
+    if not a.no_cpu:
+        from oracle import radiation as orad
+        cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+        cols = np.arange(N) if a.cpu_cols <= 0 else np.arange(0, N, max(1, N // a.cpu_cols))
+        n = len(cols)
+        lwt = orad.LwTables(lwk)
+        from insilica.radiation import ice_optics
+        ng = np.asarray(swk['ng'], int)
+        io = ice_optics(30.0, swk['band_top'])
+        swt = orad.SwTables(swk, cssa_i=np.repeat([x[0] for x in io], ng), g_i=np.repeat([x[1] for x in io], ng))
+        sub_c = {k: np.ascontiguousarray(v[..., cols]) for k, v in c.items()}
+        t0 = time.perf_counter()
+        sub = orad.lw_setup(n, L, par, lwt, sub_c['mu'], sub_c['theta'], sub_c['q'], sub_c['cloudo'])
+        orad.lw_column(n, L, par, lwt, sub, sub_c['Ts'])
+        orad.sw_column(n, L, par, swt, sub, sub_c['alb'], mode=0, tnow=2.0e6, interval=3600.0,
+                       pos=np.ascontiguousarray(pos[cols]), cloudo=sub_c['cloudo'])
+        el = time.perf_counter() - t0
+        cpu = dict(value=round(n / N / el, 5), unit='passes/s', cores=cores, kind='port',
+                   sample=(f'{n} of the {N} columns (every {N // n}th), ' if n < N else f'all {N} columns, ') +
+                          f'one FP64 oracle pass (C, OpenMP), {el:.1f} s' + (', scaled to the whole pass' if n < N else ''))
+    line = dict(metric='radiation passes/s: 40962 columns x 7 layers, LW 256 + SW 224 g-points (FP64 vs mixed, 1 GPU)',
+                value=round(a.steps / el_m, 3), unit='passes/s', n_gpus=1, steps=a.steps, warmup=a.warmup,
+                ms_per_step=round(el_m / a.steps * 1e3, 3), higher_is_better=True, scaling='weak', vs_baseline=None,
+                dtype='f32 arithmetic, f64 g-sums (mixed)', data='synthetic',
+                config=dict(workload='column radiation: level-6 cells (40962) x 7 layers, 4 sub-levels each; synthetic '
+                                     'k-tables LW 16x16 / SW 14x16 g-points, k log-uniform 1e-6..1e2 m^2/kg; lapse-rate '
+                                     'T 180-305 K, vapor exp(-z/2 km), clouds in ~1/3 of layers; step = lw_setup + '
+                                     'lw_column + sw_column', parallelism='1 GPU'),
+                fp64_value=round(a.steps / el_d, 3) if el_d else None,
+                mixed_over_fp64=round(el_d / el_m, 3) if el_d else None,
+                roofline=roof(prof_m, 'mixed'), roofline_fp64=roof(prof_d, 'double') if prof_d else None,
+                cpu_baseline=cpu)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
+    if a.radiation:
+        if a.gpus != 1:
+            print('bench.py runs on one GPU only', file=sys.stderr, flush=True)
+            sys.exit(2)
+        return radiation_main(a)
     if a.serve:
         if a.gpus != 1:
             print('bench.py runs on one GPU only', file=sys.stderr, flush=True)

@@ -162,6 +162,76 @@ int32_t isw_csum(const double* x, const double* y, int64_t n, double* scratch, d
 /* Fill n doubles with v (a kernel, not hipMemsetAsync, so that captured graphs replay it safely). */
 int32_t isw_fill(double* x, int64_t n, double v, void* stream);
 
+/* ---- k-table radiation (reference: src/insilica/radiation.py, kernels/air.cu:2111-2511) ----
+ *
+ * The column's parameters arrive as the reference's `par` array (atmosphere.PAR_ORDER, 132 values, then the theta*
+ * of the interfaces), in HOST memory: the entry points read eps_s [4], grav [13], cp [14], kappa [15], p0 [16],
+ * tilt [17], year_s [18], day_s [19], cloud_g [59], bl_q [89], edge_k [101], ice_air [102], lw_refl [124], ecc [126]
+ * and lam_peri [127] and pass them to the kernels by value. Layers k = 0 (top) .. L-1 (bottom), L <= ISW_LMAX; each
+ * is split into 4 sub-levels, s = 4k + j. The sub-levels' properties live in `sub`, float32, sub[(s*15 + v)*N + i]
+ * (v: 0/1 edge temperatures top/bottom, 2/3 table positions in log p and T, 4-8 H2O/CO2/O3/N2O/CH4 molecules per
+ * cm^2, 9 H2O's place between its two tables, 10 the layer's cloud longwave optical depth in its cloud sub-level,
+ * 11 cloud cover, 12 air molecules per cm^2, 13 the haze's share, 14 collision-induced absorption's weight). */
+
+/* The air's radiative constants (reference: the AIR_* compile-time defines of kernels/air.cu:75-116 that
+ * constants.air_props sets for another air; Earth air: m_air 28.9647, m_air_kg 28.9647e-3, o2 0.2095, ray = broad =
+ * o3 = 1, eps 0.622 (water's molar mass over the air's, in the saturation humidity), cia 0). cia != 0: lw_setup writes
+ * the collision-induced absorption weights (v = 14). */
+typedef struct isw_air {
+  double m_air, m_air_kg, o2, ray, broad, o3, eps;
+  int32_t cia, pad_;
+} isw_air;
+
+/* Longwave tables (reference: Longwave.__init__, radiation.py:26-45): logk (NC = 6, NP, NT, NG) ln k in
+ * cm^2/molecule, component-major; planck (NTP, NG) per-g Planck radiance; o3 (NO3) ozone mole fraction on the
+ * pressure axis; lp0 = ln P[0], dlp the log-pressure step, t0, dtt the temperature axis, tp0 the Planck table's first
+ * temperature, o3lp0/o3dlp the ozone axis; cia (NT, NG) or NULL. Device pointers. NG <= 1024. */
+typedef struct isw_lw_tables {
+  int32_t NC, NP, NT, NG, NTP, NO3;
+  double lp0, dlp, t0, dtt, tp0, o3lp0, o3dlp;
+  const float *logk, *planck, *o3, *cia;
+} isw_lw_tables;
+
+/* Shortwave tables (reference: Shortwave.__init__, radiation.py:89-106): logk (NC = 7, NP, NT, NG); sun (NG) the
+ * solar weights summing to 1; ray (NG) Rayleigh cross sections per air molecule; hzf (NG) the haze's spectral factor;
+ * cssa (NG) the cloud drops' single-scattering albedo; cssa_i, g_i (NG) ice clouds' optics or NULL; cia (NT, NG) or
+ * NULL; yscale maps lw_setup's temperature position to this table's; tau550, hssa, hg the haze's optical depth, albedo
+ * and asymmetry; inhom the cloud inhomogeneity factor. Device pointers. NG <= 1024. */
+typedef struct isw_sw_tables {
+  int32_t NC, NP, NT, NG;
+  double yscale, tau550, hssa, hg, inhom;
+  const float *logk, *sun, *ray, *hzf, *cssa, *cssa_i, *g_i, *cia;
+} isw_sw_tables;
+
+/* Longwave.__call__'s first kernel (reference: kernels/air.cu:2145-2229 lw_setup): the sub-levels of every
+ * (column, layer) from the layer masses mu (L, N), theta (L, N), vapor qv (L, N; NULL dry), the clouds' cloudo
+ * ((4L + 3), N; NULL clear: row k cover, row L + 3 + k in-cloud longwave optical depth), the gas tracers gx
+ * (3, L, N; NULL: the planet's co2, n2o, ch4), the haze scale height hs (m) and the temperature mode tmode (0
+ * isothermal, 1 uniform theta, 2 standard lapse) -> sub. prec: ISW_FP64 or ISW_MIXED. */
+int32_t isw_lw_setup(int32_t N, int32_t L, const double* par, const isw_lw_tables* t, const isw_air* air, double co2,
+                     double n2o, double ch4, const double* gx, const double* mu, const double* theta, const double* qv,
+                     const double* cloudo, double hs, int32_t tmode, float* sub, int32_t prec, void* stream);
+
+/* Longwave.__call__'s second kernel (reference: kernels/air.cu:2231-2348 lw_column): per column, per g-point, the
+ * optical depths from the tables, two-stream down and up through the 4L sub-levels (clear and cloudy streams mixed per
+ * layer), the surface at Ts (N). Writes radf ((3(L+1) + 3 + 6L), N): rows 0..L up (air's emission) and L+1..2L+1
+ * down at the layer edges, 2(L+1)..3(L+1)-1 the surface emission's share reaching each edge, 3(L+1) the clear sky's
+ * top up flux, +1 its transmission share, +2 the surface's Planck sum, then 6 rows per layer at its edge sub-levels.
+ * Sums over g-points in FP64 of per-g values rounded to float32, as the reference's. */
+int32_t isw_lw_column(int32_t N, int32_t L, const double* par, const isw_lw_tables* t, const float* sub,
+                      const double* Ts, double* radf, int32_t prec, void* stream);
+
+/* Shortwave.__call__ (reference: kernels/air.cu:2350-2511 sw_column, sw_layer): per column, per g-point, gases,
+ * Rayleigh, haze and clouds per sub-level, Delta-Eddington, adding over the surface albedo alb (N). The sun: mode 0 the
+ * mean of its cosine over the interval's daylight from the device clock t (1 double, s) and the unit cell positions
+ * pos (N, 3); mode 1 a fixed 0.5. Writes swf ((3L + 2), N) per unit of sunlight on the horizontal: rows 0..L-1 each
+ * layer's absorbed share, L the surface's, L + 1 the clear sky's reflected share, then each layer's top and bottom
+ * sub-level's absorption. cloudo as lw_setup's (rows k cover, 2L + 3 + k shortwave optical depth, 3L + 3 + k the
+ * liquid share) or NULL. */
+int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tables* t, const isw_air* air,
+                      const float* sub, int32_t mode, const double* tnow, double interval, const double* pos,
+                      const double* alb, const double* cloudo, double* swf, int32_t prec, void* stream);
+
 /* Measurement mode (no counterpart in the reference, whose tools/profile.py wraps its kernels in CUDA event pairs
  * from Python; this is that instrument inside the library). isw_timing_start() switches it on: from then every kernel
  * launched directly (not under stream capture) by the entry points above is bracketed by two HIP events on its stream

@@ -35,6 +35,17 @@
 ISW_DECLARE(isw_f64)
 ISW_DECLARE(isw_mix)
 
+#define ISW_DECLARE_RAD(NS)                                                                                          \
+  namespace NS {                                                                                                     \
+  int lw_setup(int, int, const double*, const isw_lw_tables*, const isw_air*, double, double, double, const double*, \
+               const double*, const double*, const double*, const double*, double, int, float*, hipStream_t);         \
+  int lw_column(int, int, const double*, const isw_lw_tables*, const float*, const double*, double*, hipStream_t);   \
+  int sw_column(int, int, const double*, const isw_sw_tables*, const isw_air*, const float*, int, const double*,     \
+                double, const double*, const double*, const double*, double*, hipStream_t);                         \
+  }
+ISW_DECLARE_RAD(isw_f64)
+ISW_DECLARE_RAD(isw_mix)
+
 namespace {
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -58,6 +69,17 @@ bool layers_ok(const isw_grid* g, const isw_layers* s) {
          (!s->air || (s->pibar && s->S && s->muf && s->mup && s->uf && s->ufb && s->accL)) &&
          (s->air || (s->ub0 && s->ub && s->ubb && s->acc && s->eta0 && s->eta && s->eta_prev));
 }
+
+// The radiation's tables: shapes within the kernels' limits (g-points one block, table axes at least two points).
+bool lw_tables_ok(const isw_lw_tables* t) {
+  return t && t->NC == 6 && t->NP >= 2 && t->NT >= 2 && t->NG >= 1 && t->NG <= 256 && t->NTP >= 2 && t->NO3 >= 2 &&
+         t->logk && t->planck && t->o3 && t->dlp != 0 && t->dtt != 0 && t->o3dlp != 0;
+}
+bool sw_tables_ok(const isw_sw_tables* t) {
+  return t && t->NC == 7 && t->NP >= 2 && t->NT >= 2 && t->NG >= 1 && t->NG <= 256 && t->logk && t->sun && t->ray &&
+         t->hzf && t->cssa && (!t->cssa_i) == (!t->g_i);
+}
+bool prec_ok(int32_t p) { return p == ISW_FP64 || p == ISW_MIXED; }
 
 }  // namespace
 
@@ -143,6 +165,32 @@ int32_t isw_ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* c
 int32_t isw_fill(double* x, int64_t n, double v, void* stream) {
   if (n < 0 || (n > 0 && !x)) return ISW_EBADARG;
   return isw_f64::fill(x, (long)n, v, S(stream));
+}
+
+int32_t isw_lw_setup(int32_t N, int32_t L, const double* par, const isw_lw_tables* t, const isw_air* air, double co2,
+                     double n2o, double ch4, const double* gx, const double* mu, const double* theta, const double* qv,
+                     const double* cloudo, double hs, int32_t tmode, float* sub, int32_t prec, void* stream) {
+  if (N < 1 || L < 1 || L > ISW_LMAX || !par || !lw_tables_ok(t) || !air || !mu || !theta || !sub || !(hs > 0) ||
+      tmode < 0 || tmode > 2 || !prec_ok(prec) || (air->cia && !t->cia))
+    return ISW_EBADARG;
+  return DISPATCH(prec, lw_setup(N, L, par, t, air, co2, n2o, ch4, gx, mu, theta, qv, cloudo, hs, tmode, sub,
+                                 S(stream)));
+}
+
+int32_t isw_lw_column(int32_t N, int32_t L, const double* par, const isw_lw_tables* t, const float* sub,
+                      const double* Ts, double* radf, int32_t prec, void* stream) {
+  if (N < 1 || L < 1 || L > ISW_LMAX || !par || !lw_tables_ok(t) || !sub || !Ts || !radf || !prec_ok(prec))
+    return ISW_EBADARG;
+  return DISPATCH(prec, lw_column(N, L, par, t, sub, Ts, radf, S(stream)));
+}
+
+int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tables* t, const isw_air* air,
+                      const float* sub, int32_t mode, const double* tnow, double interval, const double* pos,
+                      const double* alb, const double* cloudo, double* swf, int32_t prec, void* stream) {
+  if (N < 1 || L < 1 || L > ISW_LMAX || !par || !sw_tables_ok(t) || !air || !sub || !alb || !swf ||
+      (mode != 0 && mode != 1) || (mode == 0 && (!tnow || !pos || !(interval > 0))) || !prec_ok(prec))
+    return ISW_EBADARG;
+  return DISPATCH(prec, sw_column(N, L, par, t, air, sub, mode, tnow, interval, pos, alb, cloudo, swf, S(stream)));
 }
 
 const char* isw_version(void) { return "insilica_hip 0.1 gfx950"; }

@@ -22,3 +22,13 @@ bool begin(const char* name, double bytes, hipStream_t st, hipEvent_t* a, hipEve
     else                                                                                      \
       kern<<<(grid), (block), 0, st>>>(__VA_ARGS__);                                          \
   } while (0)
+
+// The same with `shm` bytes of dynamic LDS.
+#define ISW_LAUNCH_SHM(name, bytes, kern, grid, block, shm, ...)                              \
+  do {                                                                                        \
+    hipEvent_t isw_a_ = nullptr, isw_b_ = nullptr;                                            \
+    if (isw_diag::enabled() && isw_diag::begin((name), (bytes), st, &isw_a_, &isw_b_))        \
+      hipExtLaunchKernelGGL(kern, (grid), (block), (shm), st, isw_a_, isw_b_, 0, __VA_ARGS__); \
+    else                                                                                      \
+      kern<<<(grid), (block), (shm), st>>>(__VA_ARGS__);                                      \
+  } while (0)

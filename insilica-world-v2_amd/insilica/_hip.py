@@ -50,6 +50,23 @@ class IswTransport(ctypes.Structure):
                 + [('pf', _f32p)])
 
 
+class IswAir(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_double) for k in ('m_air', 'm_air_kg', 'o2', 'ray', 'broad', 'o3', 'eps')]
+                + [('cia', ctypes.c_int32), ('pad_', ctypes.c_int32)])
+
+
+class IswLwTables(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int32) for k in ('NC', 'NP', 'NT', 'NG', 'NTP', 'NO3')]
+                + [(k, ctypes.c_double) for k in ('lp0', 'dlp', 't0', 'dtt', 'tp0', 'o3lp0', 'o3dlp')]
+                + [(k, _f32p) for k in ('logk', 'planck', 'o3', 'cia')])
+
+
+class IswSwTables(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int32) for k in ('NC', 'NP', 'NT', 'NG')]
+                + [(k, ctypes.c_double) for k in ('yscale', 'tau550', 'hssa', 'hg', 'inhom')]
+                + [(k, _f32p) for k in ('logk', 'sun', 'ray', 'hzf', 'cssa', 'cssa_i', 'g_i', 'cia')])
+
+
 class HipError(RuntimeError):
     pass
 
@@ -81,6 +98,9 @@ def lib():
             'isw_dsum': [vp, vp, i64, vp, vp, vp],
             'isw_csum': [vp, vp, i64, vp, vp, vp],
             'isw_fill': [vp, i64, d, vp],
+            'isw_lw_setup': [i32, i32, vp, vp, vp, d, d, d, vp, vp, vp, vp, vp, d, i32, vp, i32, vp],
+            'isw_lw_column': [i32, i32, vp, vp, vp, vp, vp, i32, vp],
+            'isw_sw_column': [i32, i32, vp, vp, vp, vp, i32, vp, d, vp, vp, vp, vp, i32, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -94,7 +114,8 @@ def lib():
 
 EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_exchange', 'isw_vmix_heat',
            'isw_mass_update', 'isw_transport_step', 'isw_swe_step', 'isw_couple_stress', 'isw_land_column',
-           'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_timing_start', 'isw_timing_stop', 'isw_version')
+           'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_lw_setup', 'isw_lw_column', 'isw_sw_column',
+           'isw_timing_start', 'isw_timing_stop', 'isw_version')
 
 
 def check(rc, what):

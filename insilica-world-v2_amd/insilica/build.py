@@ -1,7 +1,8 @@
 """Build the native libraries in-tree: libinsilica_hip.so (hipcc, gfx950) and libinsilica_mesh.so (g++).
 
-core.hip is compiled twice: the plain FP64 form with -ffp-contract=off (every product rounds on its own) and the
-mixed form with -ffp-contract=fast. All HIP code is compiled with packed-FP32 instructions disabled.
+core.hip and radiation.hip are compiled twice each: the plain FP64 form with -ffp-contract=off (every product rounds on its own) and the
+mixed form with -ffp-contract=fast (radiation.hip: fast-honor-pragmas, so that its set-up kernel's own
+`fp contract(off)` holds). All HIP code is compiled with packed-FP32 instructions disabled.
 """
 import os
 import subprocess
@@ -34,6 +35,8 @@ def build_hip(verbose=False):
     objs = []
     units = [('core.hip', 'core_f64.o', ['-DISW_FORM_MIXED=0', '-ffp-contract=off']),
              ('core.hip', 'core_mix.o', ['-DISW_FORM_MIXED=1', '-ffp-contract=fast']),
+             ('radiation.hip', 'radiation_f64.o', ['-DISW_FORM_MIXED=0', '-ffp-contract=off']),
+             ('radiation.hip', 'radiation_mix.o', ['-DISW_FORM_MIXED=1', '-ffp-contract=fast-honor-pragmas']),
              ('capi.hip', 'capi.o', ['-ffp-contract=off'])]
     for src, obj, extra in units:
         out = OBJDIR / obj

@@ -1,7 +1,8 @@
 """The CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
 ctypes bindings of liboracle.so (oracle.c: a scalar C restatement of the reference's double-precision kernels and of
-the Python drivers that sequence them: layers.py, transport.py, swe.py). Only the tests, __graft_entry__.smoke() and
+the Python drivers that sequence them: layers.py, transport.py, swe.py; radiation.c: the k-table radiation, wrapped
+in oracle/radiation.py). Only the tests, __graft_entry__.smoke() and
 bench.py's CPU-baseline leg use it, as the checker of the HIP library and as the timed CPU path; the product never
 imports it. `build()` compiles it with gcc (-O2 -ffp-contract=off -fopenmp).
 """
@@ -21,9 +22,9 @@ _i32p, _i8p, _f64p = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_in
 
 
 def build():
-    """Compile liboracle.so from oracle.c (seconds)."""
+    """Compile liboracle.so from oracle.c and radiation.c (seconds)."""
     cmd = ['gcc', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(LIB) + '.tmp',
-           str(HERE / 'oracle.c'), '-lm']
+           str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), '-lm']
     subprocess.run(cmd, check=True)
     os.replace(str(LIB) + '.tmp', LIB)
 
@@ -68,6 +69,10 @@ def lib():
             'og_ocn_cap': (None, [vp, vp, d, vp, vp, d]),
             'og_vmix_heat': (None, [vp, i, vp, vp, d, d, d, d, vp]),
             'og_dsum': (d, [vp, vp, ctypes.c_int64]),
+            'og_lw_setup': (None, [i, i, vp, vp, vp, d, d, d, vp, vp, vp, vp, vp, d, i, vp]),
+            'og_lw_column': (None, [i, i, i, vp, vp, vp, vp, vp]),
+            'og_sw_column': (None, [i, i, i, vp, vp, vp, vp, i, d, d, vp, vp, vp, vp]),
+            'og_sw_layer': (None, [d, d, d, d, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

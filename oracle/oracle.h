@@ -68,4 +68,37 @@ void og_vmix_heat(const og_grid* g, int L, const double* h, const double* diss, 
 /* gpu.dsum's fixed-order tree (reference: kernels/trisk.cu:61-81, gpu.py:111-126). */
 double og_dsum(const double* x, const double* y, int64_t n);
 
+/* ---- k-table radiation (reference: src/insilica/radiation.py, kernels/air.cu:2111-2511); radiation.c ----
+ * The double-precision path, with the reference's float32 storage points kept: the tables, the sub-levels' 15
+ * properties (sub, float32), the clamped table positions (rounded through fminf/fmaxf), and every g-point's
+ * contribution, rounded to float32 before the FP64 sum over g-points. par: the reference's PAR_ORDER array. */
+typedef struct og_air {
+  double m_air, m_air_kg, o2, ray, broad, o3, eps;
+  int32_t cia, pad_;
+} og_air;
+
+typedef struct og_lwt {
+  int32_t NC, NP, NT, NG, NTP, NO3;
+  double lp0, dlp, t0, dtt, tp0, o3lp0, o3dlp;
+  const float *logk, *planck, *o3, *cia;
+} og_lwt;
+
+typedef struct og_swt {
+  int32_t NC, NP, NT, NG;
+  double yscale, tau550, hssa, hg, inhom;
+  const float *logk, *sun, *ray, *hzf, *cssa, *cssa_i, *g_i, *cia;
+} og_swt;
+
+void og_lw_setup(int N, int L, const double* par, const og_lwt* t, const og_air* air, double co2, double n2o,
+                 double ch4, const double* gx, const double* mu, const double* theta, const double* qv,
+                 const double* cloudo, double hs, int tmode, float* sub);
+/* nsub sub-levels per layer (the model's 4; the line-by-line columns' test uses 1: each level its own layer). */
+void og_lw_column(int N, int L, int nsub, const double* par, const og_lwt* t, const float* sub, const double* Ts,
+                  double* radf);
+void og_sw_column(int N, int L, int nsub, const double* par, const og_swt* t, const og_air* air, const float* sub,
+                  int mode, double tnow, double interval, const double* pos, const double* alb, const double* cloudo,
+                  double* swf);
+/* The Delta-Eddington layer alone (air.cu:2358-2382), for the closed-form checks. out: Rd, Td, Rs, Ts, Tdd. */
+void og_sw_layer(double tau, double ssa, double g, double mu0, double* out);
+
 #endif

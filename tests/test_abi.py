@@ -43,6 +43,24 @@ def test_bad_arguments_rejected_on_host():
     tr = _hip.IswTransport()
     tr.L, tr.rows_max = 2, 4
     assert lib.isw_transport_step(ctypes.byref(g), ctypes.byref(tr), None, None, None, None, 3, 3, None) == E
+    # radiation: missing tables, a layer count beyond the maximum, more g-points than one block holds
+    air = _hip.IswAir()
+    lwt, swt = _hip.IswLwTables(), _hip.IswSwTables()
+    assert lib.isw_lw_setup(10, 3, None, None, None, 0, 0, 0, None, None, None, None, None, 1500.0, 2, None, 0,
+                            None) == E
+    assert lib.isw_lw_column(10, 3, None, ctypes.byref(lwt), None, None, None, 0, None) == E
+    par = (ctypes.c_double * 140)()
+    lwt.NC, lwt.NP, lwt.NT, lwt.NG, lwt.NTP, lwt.NO3 = 6, 31, 20, 300, 211, 31
+    lwt.dlp = lwt.dtt = lwt.o3dlp = 1.0
+    junk = (ctypes.c_float * 8)()
+    lwt.logk = lwt.planck = lwt.o3 = ctypes.cast(junk, _hip._f32p)
+    buf = ctypes.c_void_p(1)
+    assert lib.isw_lw_column(10, 3, par, ctypes.byref(lwt), buf, buf, buf, 0, None) == E          # NG 300 > 256
+    lwt.NG = 136
+    assert lib.isw_lw_column(10, 9, par, ctypes.byref(lwt), buf, buf, buf, 0, None) == E          # L 9 > 8
+    assert lib.isw_lw_column(10, 3, par, ctypes.byref(lwt), buf, buf, buf, 7, None) == E          # unknown form
+    assert lib.isw_sw_column(10, 3, par, ctypes.byref(swt), ctypes.byref(air), buf, 0, None, 3600.0, None, buf,
+                             None, buf, 0, None) == E
 
 
 def test_request_draws_are_seeded():
