@@ -175,10 +175,11 @@ int32_t isw_fill(double* x, int64_t n, double v, void* stream);
 
 /* The air's radiative constants (reference: the AIR_* compile-time defines of kernels/air.cu:75-116 that
  * constants.air_props sets for another air; Earth air: m_air 28.9647, m_air_kg 28.9647e-3, o2 0.2095, ray = broad =
- * o3 = 1, eps 0.622 (water's molar mass over the air's, in the saturation humidity), cia 0). cia != 0: lw_setup writes
- * the collision-induced absorption weights (v = 14). */
+ * o3 = 1, eps 0.622 (water's molar mass over the air's, in the saturation humidity), rd 287.0 (the column physics' gas
+ * constant), vf 0.61 (the virtual-temperature factor), cia 0). cia != 0: lw_setup writes the collision-induced
+ * absorption weights (v = 14). */
 typedef struct isw_air {
-  double m_air, m_air_kg, o2, ray, broad, o3, eps;
+  double m_air, m_air_kg, o2, ray, broad, o3, eps, rd, vf;
   int32_t cia, pad_;
 } isw_air;
 
@@ -231,6 +232,41 @@ int32_t isw_lw_column(int32_t N, int32_t L, const double* par, const isw_lw_tabl
 int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tables* t, const isw_air* air,
                       const float* sub, int32_t mode, const double* tnow, double interval, const double* pos,
                       const double* alb, const double* cloudo, double* swf, int32_t prec, void* stream);
+
+/* ---- the air column's Held-Suarez path and the surface exchange (reference: kernels/air.cu; FP64 in both forms) ---- */
+
+/* AirPhysics.column with forcing='held_suarez' (reference: atmosphere.py:142-146; kernels/air.cu:2065-2109 hs_column
+ * with the interface crossing 744-850): each of L >= 2 layers relaxed toward Held & Suarez's equilibrium theta at its
+ * mass-mean pressure (rates hs[0] = k_a, hs[1] = k_s, 1/s; HOST array of 2), lat (N) the cells' latitudes; then the
+ * air crossing each isentropic interface theta* when heated or cooled across it. theta (L, N) updated in place;
+ * wx (L-1, N) the mass to move up across each interface (for isw_layers_exchange); qnet (N) the heating (W/m^2);
+ * stored (N) the column's enthalpy change (J/m^2). mu (L, N) is read only. par as the radiation's (host), with the
+ * interfaces' theta* at par[132..]; bl_mix > 0 (the mixed bottom layer) needs L >= 3. */
+int32_t isw_hs_column(int32_t N, int32_t L, const double* par, const double* hs, const double* lat, const double* mu,
+                      double* theta, double dt, double* wx, double* qnet, double* stored, void* stream);
+
+/* AirPhysics.surface (reference: atmosphere.py:125-132; air.cu:2015-2061 surface_layer): the surface layer's exchange
+ * coefficients cd, ch (N) = rho C_D U and rho C_H U (kg/m^2/s) by Louis's bulk form from the bottom layer's wind speed
+ * uabs (N), mu, theta (L, N), vapor qv (L, N; NULL dry), the surface temperature Ts (N), sea ice (N; NULL none), the
+ * land mask (N; NULL all sea) and the land cover's roughness lc (N; NULL: par's z0_land). air: rd, vf, eps. */
+int32_t isw_surface_layer(int32_t N, int32_t L, const double* par, const isw_air* air, const double* uabs,
+                          const double* mu, const double* theta, const double* qv, const double* Ts, const double* ice,
+                          const double* land, double* cd, double* ch, const double* lc, void* stream);
+
+/* AirPhysics._speed (reference: atmosphere.py:109-115): the bottom layer's wind speed at cells from its edge winds
+ * u (E): cell_speed from the edges' kinetic energy (air.cu:1978-1989), cell_speed_perot from Perot's reconstructed
+ * wind with op (N, 6, 2) east/north coefficients per edge slot (air.cu:1994-2006; op from orodrag.operators). */
+int32_t isw_cell_speed(const isw_grid* g, const double* u, double* out, void* stream);
+int32_t isw_cell_speed_perot(const isw_grid* g, const double* u, const double* op, double* out, void* stream);
+
+/* World._tick's gas sources (reference: world.py:343-346; air.cu:2133-2137 gas_src): x (G*L, N) gas mole fractions,
+ * each gas's surface source src (G, N) (mole fraction x kg/m^2/s) into the bottom layer of mass h (L, N) over dt. */
+int32_t isw_gas_src(int32_t N, int32_t L, int32_t G, const double* src, const double* h, double dt, double* x,
+                    void* stream);
+
+/* The device clock (reference: air.cu:1975 advance_time): t[0] += dt on the stream (a kernel, so captured ticks
+ * advance it on replay). */
+int32_t isw_advance_time(double* t, double dt, void* stream);
 
 /* Measurement mode (no counterpart in the reference, whose tools/profile.py wraps its kernels in CUDA event pairs
  * from Python; this is that instrument inside the library). isw_timing_start() switches it on: from then every kernel

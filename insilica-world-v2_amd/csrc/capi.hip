@@ -35,6 +35,17 @@
 ISW_DECLARE(isw_f64)
 ISW_DECLARE(isw_mix)
 
+namespace isw_col {
+int hs_column(int, int, const double*, const double*, const double*, const double*, double*, double, double*, double*,
+              double*, hipStream_t);
+int surface_layer(int, int, const double*, const isw_air*, const double*, const double*, const double*, const double*,
+                  const double*, const double*, const double*, double*, double*, const double*, hipStream_t);
+int cell_speed(int, int, const double*, const int*, const double*, const double*, const double*, double*, hipStream_t);
+int cell_speed_perot(int, int, const double*, const int*, const double*, double*, hipStream_t);
+int gas_src(int, int, int, const double*, const double*, double, double*, hipStream_t);
+int advance_time(double*, double, hipStream_t);
+}  // namespace isw_col
+
 #define ISW_DECLARE_RAD(NS)                                                                                          \
   namespace NS {                                                                                                     \
   int lw_setup(int, int, const double*, const isw_lw_tables*, const isw_air*, double, double, double, const double*, \
@@ -191,6 +202,42 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
       (mode != 0 && mode != 1) || (mode == 0 && (!tnow || !pos || !(interval > 0))) || !prec_ok(prec))
     return ISW_EBADARG;
   return DISPATCH(prec, sw_column(N, L, par, t, air, sub, mode, tnow, interval, pos, alb, cloudo, swf, S(stream)));
+}
+
+int32_t isw_hs_column(int32_t N, int32_t L, const double* par, const double* hs, const double* lat, const double* mu,
+                      double* theta, double dt, double* wx, double* qnet, double* stored, void* stream) {
+  if (N < 1 || L < 2 || L > ISW_LMAX || !par || !hs || !lat || !mu || !theta || !wx || !qnet || !stored ||
+      !(dt > 0) || (par[70] > 0 && L < 3))
+    return ISW_EBADARG;
+  return isw_col::hs_column(N, L, par, hs, lat, mu, theta, dt, wx, qnet, stored, S(stream));
+}
+
+int32_t isw_surface_layer(int32_t N, int32_t L, const double* par, const isw_air* air, const double* uabs,
+                          const double* mu, const double* theta, const double* qv, const double* Ts, const double* ice,
+                          const double* land, double* cd, double* ch, const double* lc, void* stream) {
+  if (N < 1 || L < 1 || L > ISW_LMAX || !par || !air || !uabs || !mu || !theta || !Ts || !cd || !ch) return ISW_EBADARG;
+  return isw_col::surface_layer(N, L, par, air, uabs, mu, theta, qv, Ts, ice, land, cd, ch, lc, S(stream));
+}
+
+int32_t isw_cell_speed(const isw_grid* g, const double* u, double* out, void* stream) {
+  if (!grid_ok(g) || !u || !out) return ISW_EBADARG;
+  return isw_col::cell_speed(g->N, g->E, u, g->cellEdge, g->le, g->de, g->areaCell, out, S(stream));
+}
+
+int32_t isw_cell_speed_perot(const isw_grid* g, const double* u, const double* op, double* out, void* stream) {
+  if (!grid_ok(g) || !u || !op || !out) return ISW_EBADARG;
+  return isw_col::cell_speed_perot(g->N, g->E, u, g->cellEdge, op, out, S(stream));
+}
+
+int32_t isw_gas_src(int32_t N, int32_t L, int32_t G, const double* src, const double* h, double dt, double* x,
+                    void* stream) {
+  if (N < 1 || L < 1 || L > ISW_LMAX || G < 0 || (G > 0 && (!src || !h || !x))) return ISW_EBADARG;
+  return G ? isw_col::gas_src(N, L, G, src, h, dt, x, S(stream)) : 0;
+}
+
+int32_t isw_advance_time(double* t, double dt, void* stream) {
+  if (!t) return ISW_EBADARG;
+  return isw_col::advance_time(t, dt, S(stream));
 }
 
 const char* isw_version(void) { return "insilica_hip 0.1 gfx950"; }

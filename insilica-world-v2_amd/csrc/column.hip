@@ -1,0 +1,502 @@
+// The air column's Held-Suarez path and the surface-exchange kernels as gfx950 HIP kernels (reference:
+// src/insilica/kernels/air.cu: hs_column 2065-2109 with the crossing of the isentropic interfaces 744-850,
+// surface_layer 2015-2061, cell_speed 1978-1989, cell_speed_perot 1994-2006, gas_src 2133-2137, advance_time 1975).
+//
+// FP64 in both forms (compiled once, -ffp-contract=off): these are N-sized passes that cost microseconds per tick,
+// and the crossing's mass moves re-divide the column with an energy residual that must stay at rounding.
+//
+// Design (MI355X): one thread per column, 64-thread blocks (641 blocks at level 6, over all 256 CUs), blocks in XCD
+// order; the column's layers (L <= 8, a compile-time instantiation) live in registers; layer-major loads are coalesced
+// across the wave.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "diag.hpp"
+#include "insilica_hip.h"
+
+namespace isw_col {
+
+constexpr int BS = 64;
+
+__device__ __forceinline__ int xblock() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+#define XTID ((int)(xblock() * BS + threadIdx.x))
+static inline dim3 grid1(long n) { return dim3((unsigned)((n + BS - 1) / BS)); }
+
+// The parameters the column kernels read, from the reference's PAR_ORDER array (air.cu:118-127), by value.
+struct CP {
+  double grav, cp, kappa, p0, floor_, theta_star, Lv, moist, mu_bl, bl_mix, bl_cap, rho_ice, ice_cover, bulk_gust,
+      q_sea, z0_land;
+  double ths[ISW_LMAX];
+};
+
+static CP make_cp(const double* a) {
+  CP p;
+  p.grav = a[13]; p.cp = a[14]; p.kappa = a[15]; p.p0 = a[16]; p.floor_ = a[11]; p.theta_star = a[10]; p.Lv = a[23];
+  p.moist = a[22]; p.mu_bl = a[9]; p.bl_mix = a[70]; p.bl_cap = a[73]; p.rho_ice = a[29]; p.ice_cover = a[30];
+  p.bulk_gust = a[75]; p.q_sea = a[97]; p.z0_land = a[91];
+  for (int j = 0; j < ISW_LMAX; j++) p.ths[j] = a[132 + j];
+  return p;
+}
+
+template <int L>
+__device__ __forceinline__ void layer_exner(const double* mu, const CP& P, double* pb, double* pib) {
+  double p = 0, xt = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    const double pt = p;
+    p = p + P.grav * mu[k];
+    pb[k] = p;
+    const double x = pow(p / P.p0, P.kappa);
+    pib[k] = P.cp * (p * x - pt * xt) / ((P.kappa + 1) * (p - pt));
+    xt = x;
+  }
+}
+
+// theta* of interface j: the lowest is theta_star, the ones above from the list (from the top).
+template <int L>
+__device__ __forceinline__ double tstar(int j, const CP& P) { return j == L - 2 ? P.theta_star : P.ths[j]; }
+
+// The shape s of a layer's truncated-exponential theta density between edges a < b with mean m (air.cu:245-270):
+// r(s) = 1 - 1/s + 1/(e^s - 1) = (m - a) / (b - a) by Newton's method kept inside a bracket.
+__device__ double edge_shape(double a, double b, double m) {
+  const double W = b - a;
+  double x = (m - a) / W;
+  if (x < 1e-6) x = 1e-6;
+  if (x > 1 - 1e-6) x = 1 - 1e-6;
+  double lo = -700, hi = 700, s = 12 * (x - 0.5);
+  for (int it = 0; it < 40; it++) {
+    double r, dr;
+    if (fabs(s) < 1e-4) {
+      r = 0.5 + s / 12;
+      dr = 1.0 / 12;
+    } else {
+      const double e1 = expm1(s);
+      r = 1 - 1 / s + 1 / e1;
+      dr = 1 / (s * s) - (e1 + 1) / (e1 * e1);
+    }
+    if (r < x) lo = s; else hi = s;
+    double sn = s - (r - x) / dr;
+    if (r == x) break;
+    if (!(sn >= lo && sn <= hi) || !(dr > 0)) sn = 0.5 * (lo + hi);
+    if (fabs(sn - s) < 1e-12 * (1 + fabs(s))) {
+      s = sn;
+      break;
+    }
+    s = sn;
+  }
+  return s;
+}
+
+// The layer's mass per kelvin at its lower (fa) and upper (fb) edge (air.cu:525-539).
+__device__ void edge_density(double a, double b, double m, double& fa, double& fb) {
+  if (!(b > a)) {
+    fa = 0;
+    fb = 0;
+    return;
+  }
+  const double W = b - a, s = edge_shape(a, b, m);
+  if (fabs(s) < 1e-6) {
+    fa = (1 - s / 2) / W;
+    fb = (1 + s / 2) / W;
+    return;
+  }
+  const double e1 = expm1(s);
+  fa = s / e1 / W;
+  fb = s * (e1 + 1) / e1 / W;
+}
+
+template <int L>
+__device__ __forceinline__ double column_energy(const double* mu, const double* pib, const double* th, const CP& P) {
+  double E = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) E += mu[k] * pib[k] * th[k];   // (dry: the vapor's latent term is zero)
+  return E;
+}
+
+// Move m (> 0) kg/m^2 of dry air from layer `from` to `to` (air.cu:495-520): the layers' masses, Exner functions and
+// theta re-divided, the column's energy residual booked into the receiver.
+template <int L>
+__device__ void move_mass(const CP& P, double* mu, double* th, double* pib, int from, int to, double m,
+                          double th_leave, double th_arrive) {
+  if (m == 0) return;
+  double pb2[L], pib2[L], mu2[L], th2[L];
+  const double Eb = column_energy<L>(mu, pib, th, P);
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    mu2[k] = mu[k];
+    th2[k] = th[k];
+  }
+  // (from and to are runtime indices; the arrays stay in registers through compile-time selects)
+  double mf = 0, mt = 0, tf = 0, tt = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    if (k == from) { mf = mu[k]; tf = th[k]; mu2[k] -= m; }
+    if (k == to) { mt = mu[k]; tt = th[k]; mu2[k] += m; }
+  }
+  layer_exner<L>(mu2, P, pb2, pib2);
+  double muf2 = 0, mut2 = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    if (k == from) muf2 = mu2[k];
+    if (k == to) mut2 = mu2[k];
+  }
+  const double thf = (mf * tf - m * th_leave) / muf2, tht = (mt * tt + m * th_arrive) / mut2;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    if (k == from) th2[k] = thf;
+    if (k == to) th2[k] = tht;
+  }
+  const double Ea = column_energy<L>(mu2, pib2, th2, P);
+#pragma unroll
+  for (int k = 0; k < L; k++)
+    if (k == to) th2[k] += (Eb - Ea) / (mu2[k] * pib2[k]);
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    th[k] = th2[k];
+    mu[k] = mu2[k];
+    pib[k] = pib2[k];
+  }
+}
+
+// The dry crossing of the isentropic interfaces by the heating dth (air.cu:744-850); wxo[j] the mass moved up
+// across interface j. Dry air: the edge-vapor and freeze-drying options move the same mass without vapor.
+template <int L>
+__device__ void crossing(const CP& P, double* mu, double* th, const double* dth, double dth_up, double* pib,
+                         double* wxo) {
+#pragma unroll
+  for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
+  if (P.theta_star <= 0) return;
+  double dmj[L], thcj[L], efa[L], efb[L];
+  bool got[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) got[k] = false;
+  // a layer's edge densities, from its theta before the heating, once per layer (only layers below the top one: the
+  // top layer has no upper interface, and no branch asks for it)
+  auto dens = [&](int k, double& fa, double& fb) {
+#pragma unroll
+    for (int kk = 1; kk < L; kk++)
+      if (kk == k) {
+        if (!got[kk]) {
+          edge_density(tstar<L>(kk, P), tstar<L>(kk - 1, P), th[kk] - dth[kk], efa[kk], efb[kk]);
+          got[kk] = true;
+        }
+        fa = efa[kk];
+        fb = efb[kk];
+      }
+  };
+  const bool blm = P.bl_mix > 0;
+#pragma unroll
+  for (int j = 0; j + 1 < L; j++) {
+    const int a = j, b = j + 1;
+    const double ts_ = tstar<L>(j, P);
+    double dm = 0, thc = ts_, thd = ts_;
+    if (dth[b] > 0 && blm && b == L - 1) {
+      if (th[b] >= ts_) {
+        double fa, fb;
+        dens(a, fa, fb);
+        dm = -fmin(fmin(mu[a] * dth[b] * fa, fmax(mu[a] - P.mu_bl, 0.0) * P.bl_cap), 1e30);
+        thd = ts_ - 0.5 * dth[b];
+      }
+    } else if (blm && b < L - 1 && th[b] >= ts_) {
+      dm = mu[b] - P.floor_;
+      thc = th[b];
+    } else if (dth[b] > 0) {
+      const double eh = dth[b];
+      if (th[b] >= ts_) {
+        dm = mu[b] - P.floor_;
+        thc = th[b];
+      } else if (b == L - 1) {
+        dm = mu[b] * eh / (2 * (ts_ - (th[b] - dth[b])));
+      } else {
+        double fa, fb;
+        dens(b, fa, fb);
+        dm = mu[b] * eh * fb;
+      }
+    }
+    const double da = a == 0 ? dth_up : dth[a];
+    if (blm && a >= 1 && th[a] <= ts_ && dm <= 0) {
+      const double down = mu[a] - P.floor_;
+      thd = th[a];
+      dm -= down;
+    } else if (da < 0) {
+      double down;
+      if (th[a] <= ts_) {
+        down = mu[a] - P.floor_;
+        thd = th[a];
+      } else if (a == 0) {
+        down = mu[a] * (-da) / (2 * ((th[a] - dth[a]) - ts_));
+      } else {
+        double fa, fb;
+        dens(a, fa, fb);
+        down = mu[a] * (-da) * fa;
+      }
+      dm -= down;
+    }
+    dmj[j] = dm;
+    thcj[j] = dm > 0 ? thc : thd;
+  }
+#pragma unroll
+  for (int j = L - 2; j >= 0; j--) {
+    const int a = j, b = j + 1;
+    double dm = dmj[j];
+    const double maxup = mu[b] - P.floor_, maxdn = mu[a] - P.floor_;
+    if (dm > maxup) dm = maxup > 0 ? maxup : 0.0;
+    if (-dm > maxdn) dm = maxdn > 0 ? -maxdn : 0.0;
+    if (dm > 0)
+      move_mass<L>(P, mu, th, pib, b, a, dm, thcj[j], thcj[j]);
+    else if (dm < 0)
+      move_mass<L>(P, mu, th, pib, a, b, -dm, thcj[j], thcj[j]);
+    wxo[j] = dm;
+  }
+  const double tiny = P.floor_ * 1.000001;
+#pragma unroll
+  for (int j = L - 2; j >= 0; j--) {
+    const int a = j, b = j + 1;
+    const double ts_ = tstar<L>(j, P);
+    if (mu[b] <= tiny && th[b] > ts_) {
+      th[a] += mu[b] * pib[b] * (th[b] - ts_) / (mu[a] * pib[a]);
+      th[b] = ts_;
+    }
+    if (mu[a] <= tiny && th[a] < ts_) {
+      th[b] -= mu[a] * pib[a] * (ts_ - th[a]) / (mu[b] * pib[b]);
+      th[a] = ts_;
+    }
+  }
+}
+
+// hs_column (air.cu:2065-2109): each layer relaxed toward Held & Suarez's equilibrium theta at its mass-mean pressure,
+// then the interfaces crossed; theta updated in place, wx (L-1, N) the mass to move up, qnet the heating (W/m^2),
+// stored the column's enthalpy change (J/m^2). The layer masses are left to the exchange that moves wx.
+template <int L>
+__global__ void __launch_bounds__(BS) k_hs_column(int N, CP P, double ka, double ks, const double* __restrict__ lat,
+                                                  const double* __restrict__ mu_in, double* __restrict__ theta,
+                                                  double dt, double* __restrict__ wx, double* __restrict__ qnet,
+                                                  double* __restrict__ stored) {
+  const int i = XTID;
+  if (i >= N) return;
+  double mu[L], th[L], pb[L], pib[L], dth[L], wxo[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    mu[k] = mu_in[(size_t)k * N + i];
+    th[k] = theta[(size_t)k * N + i];
+  }
+  layer_exner<L>(mu, P, pb, pib);
+  double E0 = 0, Qs = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) E0 += mu[k] * pib[k] * th[k];
+  const double s2 = sin(lat[i]) * sin(lat[i]), c2 = 1 - s2;
+  const double ps = pb[L - 1];
+  double pt = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    const double pbar = pow(pib[k] / P.cp, 1 / P.kappa) * P.p0;
+    double Teq = (315 - 60 * s2 - 10 * log(pbar / P.p0) * c2) * pow(pbar / P.p0, P.kappa);
+    if (Teq < 200) Teq = 200;
+    const double theq = Teq * P.cp / pib[k];
+    const double st = pt / ps, sb = pb[k] / ps;
+    const double lo = st > 0.7 ? st : 0.7;
+    const double f = sb > 0.7 ? ((sb - 0.7) * (sb - 0.7) - (lo - 0.7) * (lo - 0.7)) / (2 * 0.3 * (sb - st)) : 0.0;
+    const double kt = ka + (ks - ka) * f * c2 * c2;
+    const double d = (theq - th[k]) * (1 - exp(-kt * dt));
+    dth[k] = d;
+    th[k] += d;
+    Qs += mu[k] * pib[k] * d / dt;
+    pt = pb[k];
+  }
+  double dth_up = dth[0];
+  if (L == 2) {
+    const double pi_ = pb[0], si = pi_ / ps;
+    double Ti = (315 - 60 * s2 - 10 * log(pi_ / P.p0) * c2) * pow(pi_ / P.p0, P.kappa);
+    if (Ti < 200) Ti = 200;
+    const double thi = Ti * pow(P.p0 / pi_, P.kappa);
+    const double ki = ka + (ks - ka) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
+    dth_up = (thi - P.theta_star) * (1 - exp(-ki * dt));
+  }
+  crossing<L>(P, mu, th, dth, dth_up, pib, wxo);
+#pragma unroll
+  for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
+  double E1 = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    theta[(size_t)k * N + i] = th[k];
+    E1 += mu[k] * pib[k] * th[k];
+  }
+  qnet[i] = Qs;
+  stored[i] = E1 - E0;
+}
+
+// surface_layer (air.cu:2015-2061): rho C_D U and rho C_H U by Louis's bulk form over the sea (Charnock), sea ice and
+// land, four fixed-point passes with the free convection's gusts.
+__global__ void __launch_bounds__(BS) k_surface_layer(int N, int L, CP P, double rd, double vf, double eps,
+                                                      const double* __restrict__ uabs, const double* __restrict__ mu_in,
+                                                      const double* __restrict__ theta, const double* __restrict__ qv,
+                                                      const double* __restrict__ Ts, const double* __restrict__ ice,
+                                                      const double* __restrict__ land, double* __restrict__ cd,
+                                                      double* __restrict__ ch, const double* __restrict__ lc) {
+  const int i = XTID;
+  if (i >= N) return;
+  const int bot = L - 1;
+  double ps = 0;
+  for (int k = 0; k < L; k++) ps += P.grav * mu_in[(size_t)k * N + i];
+  const double mb = mu_in[(size_t)bot * N + i], th = theta[(size_t)bot * N + i];
+  const double Ta = th * pow(ps / P.p0, P.kappa), T0 = Ts[i];
+  const double rho = ps / (rd * Ta);
+  const double qa = qv ? qv[(size_t)bot * N + i] : 0.0;
+  const bool isl = land && land[i] > 0.5;
+  const double q0 = qv && P.moist > 0 && !isl
+                        ? (P.q_sea > 0 ? P.q_sea : 1.0) * (eps * 611.0 * exp(17.27 * (T0 - 273.15) / (T0 - 35.85)) / ps)
+                        : qa;
+  const double fice = qv && P.moist > 0 && ice && !isl ? fmin(1.0, ice[i] / P.rho_ice / P.ice_cover) : 0.0;
+  const double depth = rd * Ta / P.grav * log(ps / fmax(ps - P.grav * mb, 1.0));
+  const double zr = fmax(10.0, 0.1 * depth);
+  const double dthv = (Ta - T0) + vf * Ta * (qa - q0);
+  const double K = 0.4, nu = 1.5e-5, b = 5, c = 5;
+  const double u2 = uabs[i] * uabs[i];
+  double U = sqrt(u2 + P.bulk_gust * P.bulk_gust), Cd = 1e-3, Ch = 1e-3;
+  for (int it = 0; it < 4; it++) {
+    const double us = fmax(sqrt(Cd) * U, 1e-3);
+    double z0, zh;
+    if (isl) {
+      z0 = lc ? lc[i] : P.z0_land;
+      zh = 0.1 * z0;
+    } else {
+      const double zs = 0.011 * us * us / P.grav + 0.11 * nu / us, zsh = 0.40 * nu / us;
+      z0 = (1 - fice) * zs + fice * 1e-3;
+      zh = (1 - fice) * zsh + fice * 1e-4;
+    }
+    const double lm = log(zr / z0), lh = log(zr / zh);
+    const double CdN = K * K / (lm * lm), ChN = K * K / (lm * lh);
+    const double Ri = P.grav * zr * dthv / (Ta * U * U);
+    double fm, fh;
+    if (Ri > 0) {
+      const double s = sqrt(1 + c * Ri);
+      fm = 1 / (1 + 2 * b * Ri / s);
+      fh = 1 / (1 + 3 * b * Ri * s);
+    } else {
+      const double d = 1 + 3 * b * c * CdN * sqrt(zr / z0 * -Ri);
+      fm = 1 - 2 * b * Ri / d;
+      fh = 1 - 3 * b * Ri / d;
+    }
+    Cd = CdN * fm;
+    Ch = ChN * fh;
+    const double B = -Ch * U * dthv;
+    const double ws = B > 0 ? cbrt(P.grav / Ta * B * depth) : 0.0;
+    U = sqrt(u2 + P.bulk_gust * P.bulk_gust + 1.44 * ws * ws);
+  }
+  cd[i] = rho * Cd * U;
+  ch[i] = rho * Ch * U;
+}
+
+// cell_speed (air.cu:1978-1989): the bottom layer's wind speed at cells from its edges' kinetic energy.
+__global__ void __launch_bounds__(BS) k_cell_speed(int N, const double* __restrict__ u, const int* __restrict__ cellEdge,
+                                                   const double* __restrict__ le, const double* __restrict__ de,
+                                                   const double* __restrict__ areaCell, double* __restrict__ out) {
+  const int i = XTID;
+  if (i >= N) return;
+  const int2* ce = reinterpret_cast<const int2*>(cellEdge + (size_t)i * 6);
+  const int2 p0 = ce[0], p1 = ce[1], p2 = ce[2];
+  const int e[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
+  double ke = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++)
+    if (e[m] >= 0) ke += le[e[m]] * de[e[m]] * 0.25 * u[e[m]] * u[e[m]];
+  out[i] = sqrt(2 * ke / areaCell[i]);
+}
+
+// cell_speed_perot (air.cu:1994-2006): the speed of Perot's reconstructed cell wind, op (N, 6, 2) east and north
+// coefficients per edge slot.
+__global__ void __launch_bounds__(BS) k_cell_speed_perot(int N, const double* __restrict__ u,
+                                                         const int* __restrict__ cellEdge,
+                                                         const double* __restrict__ op, double* __restrict__ out) {
+  const int i = XTID;
+  if (i >= N) return;
+  const int2* ce = reinterpret_cast<const int2*>(cellEdge + (size_t)i * 6);
+  const int2 p0 = ce[0], p1 = ce[1], p2 = ce[2];
+  const int e[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
+  double ue = 0, un = 0;
+#pragma unroll
+  for (int m = 0; m < 6; m++)
+    if (e[m] >= 0) {
+      ue += op[((size_t)i * 6 + m) * 2] * u[e[m]];
+      un += op[((size_t)i * 6 + m) * 2 + 1] * u[e[m]];
+    }
+  out[i] = sqrt(ue * ue + un * un);
+}
+
+// gas_src (air.cu:2133-2137): each gas's surface source (mole fraction x kg/m^2/s) into the bottom layer over dt.
+__global__ void __launch_bounds__(BS) k_gas_src(int N, int L, int G, const double* __restrict__ src,
+                                                const double* __restrict__ h, double dt, double* __restrict__ x) {
+  const int i = XTID;
+  if (i >= N) return;
+  const double hb = h[(size_t)(L - 1) * N + i];
+  for (int g = 0; g < G; g++) x[((size_t)g * L + L - 1) * N + i] += src[(size_t)g * N + i] * dt / hb;
+}
+
+// advance_time (air.cu:1975): the device clock, advanced inside captured ticks.
+__global__ void k_advance_time(double* t, double dt) { t[0] += dt; }
+
+#define ISW_BY_L(CALL)       \
+  switch (L) {               \
+    case 2: CALL(2); break;  \
+    case 3: CALL(3); break;  \
+    case 4: CALL(4); break;  \
+    case 5: CALL(5); break;  \
+    case 6: CALL(6); break;  \
+    case 7: CALL(7); break;  \
+    case 8: CALL(8); break;  \
+    default: return ISW_EBADARG; \
+  }
+
+int hs_column(int N, int L, const double* par, const double* hs, const double* lat, const double* mu,
+              double* theta, double dt, double* wx, double* qnet, double* stored, hipStream_t st) {
+  const CP P = make_cp(par);
+  const double D = 8;
+  const double bytes = N * D * (1 + L + 2.0 * L + (L - 1) + 2);
+#define CALL(LL) \
+  ISW_LAUNCH("hs_column", bytes, k_hs_column<LL>, grid1(N), dim3(BS), N, P, hs[0], hs[1], lat, mu, theta, dt, wx, qnet, stored)
+  ISW_BY_L(CALL)
+#undef CALL
+  return (int)hipGetLastError();
+}
+
+int surface_layer(int N, int L, const double* par, const isw_air* air, const double* uabs, const double* mu,
+                  const double* theta, const double* qv, const double* Ts, const double* ice, const double* land,
+                  double* cd, double* ch, const double* lc, hipStream_t st) {
+  const CP P = make_cp(par);
+  const double bytes = 8.0 * N * (1 + L + 1 + (qv ? 1 : 0) + 1 + (ice ? 1 : 0) + (land ? 1 : 0) + (lc ? 1 : 0) + 2);
+  ISW_LAUNCH("surface_layer", bytes, k_surface_layer, grid1(N), dim3(BS), N, L, P, air->rd, air->vf, air->eps, uabs,
+             mu, theta, qv, Ts, ice, land, cd, ch, lc);
+  return (int)hipGetLastError();
+}
+
+int cell_speed(int N, int E, const double* u, const int* cellEdge, const double* le, const double* de,
+               const double* areaCell, double* out, hipStream_t st) {
+  const double bytes = 8.0 * E * 3 + 4.0 * 6 * N + 8.0 * N * 2;
+  ISW_LAUNCH("cell_speed", bytes, k_cell_speed, grid1(N), dim3(BS), N, u, cellEdge, le, de, areaCell, out);
+  return (int)hipGetLastError();
+}
+
+int cell_speed_perot(int N, int E, const double* u, const int* cellEdge, const double* op, double* out,
+                     hipStream_t st) {
+  const double bytes = 8.0 * E + 4.0 * 6 * N + 8.0 * 12 * N + 8.0 * N;
+  ISW_LAUNCH("cell_speed_perot", bytes, k_cell_speed_perot, grid1(N), dim3(BS), N, u, cellEdge, op, out);
+  return (int)hipGetLastError();
+}
+
+int gas_src(int N, int L, int G, const double* src, const double* h, double dt, double* x, hipStream_t st) {
+  const double bytes = 8.0 * N * (G + 1 + 2.0 * G);
+  ISW_LAUNCH("gas_src", bytes, k_gas_src, grid1(N), dim3(BS), N, L, G, src, h, dt, x);
+  return (int)hipGetLastError();
+}
+
+int advance_time(double* t, double dt, hipStream_t st) {
+  ISW_LAUNCH("advance_time", 16.0, k_advance_time, dim3(1), dim3(1), t, dt);
+  return (int)hipGetLastError();
+}
+
+}  // namespace isw_col

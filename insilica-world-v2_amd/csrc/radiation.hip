@@ -52,11 +52,21 @@ __device__ __forceinline__ double FPOWd(double x, double y) {
   return pow(x, y);
 #endif
 }
+// The column passes' exp and division: FP64 in the plain form (the reference's operations); in the mixed form the
+// hardware's: exp as 2^(x log2 e) (v_exp_f32) and a / b as a times the hardware reciprocal, each within a few float
+// steps (the mixed form's bar against FP64 is 1e-4 of the fluxes).
 __device__ __forceinline__ fast fexp(fast x) {
 #if ISW_FORM_MIXED
-  return expf(x);
+  return __expf(x);
 #else
   return exp(x);
+#endif
+}
+__device__ __forceinline__ fast fdiv(fast a, fast b) {
+#if ISW_FORM_MIXED
+  return __fdividef(a, b);
+#else
+  return a / b;
 #endif
 }
 
@@ -214,14 +224,14 @@ __global__ void __launch_bounds__(64) k_lw_setup(int N, RadP P, isw_lw_tables t,
 // ln k bilinear in (log p, T) on one table cell, then exp (air.cu:2266-2271).
 __device__ __forceinline__ fast kbil(const float* __restrict__ K, int NT, int NG, fast fp, fast ft) {
   const fast v00 = K[0], v01 = K[NG], v10 = K[(size_t)NT * NG], v11 = K[(size_t)NT * NG + NG];
-  return (fast)FEXPd(v00 + (v10 - v00) * fp + (v01 - v00) * ft + (v00 - v01 - v10 + v11) * fp * ft);
+  return fexp(v00 + (v10 - v00) * fp + (v01 - v00) * ft + (v00 - v01 - v10 + v11) * fp * ft);
 }
 
 // Two-stream through one sub-level, far side's Planck radiance Bf to the near side's Bn, linear in optical depth,
 // diffusivity 1.66 (air.cu:2123-2127).
 __device__ __forceinline__ fast lw_through(fast I, fast tau, fast Bn, fast Bf) {
-  const fast x = tau * (fast)1.66, tr = (fast)FEXPd(-x);
-  const fast lin = x > (fast)1e-3 ? 1 - (1 - tr) / x : x * ((fast)0.5 - x / 6);
+  const fast x = tau * (fast)1.66, tr = fexp(-x);
+  const fast lin = x > (fast)1e-3 ? 1 - fdiv(1 - tr, x) : x * ((fast)0.5 - x / 6);
   return I * tr + Bf * (1 - tr) + (Bn - Bf) * lin;
 }
 
@@ -328,14 +338,14 @@ __global__ void __launch_bounds__(NGMAX) k_lw_column(int N, RadP P, isw_lw_table
 #pragma unroll
       for (int s = (k + 1) * NSUB - 1; s >= k * NSUB; s--) {
         const fast Bt = Bof(sTt[s]), Bb = Bof(sTb[s]);
-        const fast t1 = (fast)FEXPd(-tau[s] * (fast)1.66);
+        const fast t1 = fexp(-tau[s] * (fast)1.66);
         Iuc = lw_through(Iuc, tau[s], Bt, Bb);
         trc *= t1;
         Ic = lw_through(Ic, tau[s], Bt, Bb);
         tc *= t1;
         if (f > 0) {
           Id = lw_through(Id, tau[s] + stc[s] / (fast)1.66, Bt, Bb);
-          td *= t1 * (fast)FEXPd(-stc[s]);
+          td *= t1 * fexp(-stc[s]);
         }
         const int e = s == (k + 1) * NSUB - 1 ? 3 : (s == k * NSUB + 1 ? 0 : -1);
         if (e >= 0) {
@@ -373,21 +383,21 @@ __device__ __forceinline__ void sw_layer(fast tau, fast ssa, fast g, fast mu0, f
                                          fast& Tdd) {
   const fast f = g * g;
   tau = (1 - ssa * f) * tau;
-  ssa = (1 - f) * ssa / (1 - ssa * f);
-  g = (g - f) / (1 - f);
+  ssa = fdiv((1 - f) * ssa, 1 - ssa * f);
+  g = fdiv(g - f, 1 - f);
   const fast g1 = (7 - ssa * (4 + 3 * g)) / 4, g2 = -(1 - ssa * (4 - 3 * g)) / 4, g3 = (2 - 3 * g * mu0) / 4;
   const fast g4 = 1 - g3;
   const fast a1 = g1 * g4 + g2 * g3, a2 = g1 * g3 + g2 * g4;
   const fast k = sqrt(fmax((g1 - g2) * (g1 + g2), (fast)1e-12));
-  const fast e = (fast)FEXPd(-k * tau), e2 = e * e, k2e = 2 * k * e;
-  fast rf = 1 / (k + g1 + (k - g1) * e2);
+  const fast e = fexp(-k * tau), e2 = e * e, k2e = 2 * k * e;
+  fast rf = fdiv(1, k + g1 + (k - g1) * e2);
   Rd = g2 * (1 - e2) * rf;
   Td = k2e * rf;
-  Tdd = (fast)FEXPd(-tau / mu0);
+  Tdd = fexp(fdiv(-tau, mu0));
   const fast km = k * mu0;
   fast om = 1 - km * km;
   if (fabs(om) < (fast)1e-3) om = om < 0 ? (fast)-1e-3 : (fast)1e-3;
-  rf = ssa * rf / om;
+  rf = fdiv(ssa * rf, om);
   Rs = rf * ((1 - km) * (a2 + k * g3) - (1 + km) * (a2 - k * g3) * e2 - k2e * (g3 - a2 * mu0) * Tdd);
   Ts = rf * (k2e * (g4 + a1 * mu0) - Tdd * ((1 + km) * (a1 + k * g4) - (1 - km) * (a1 - k * g4) * e2));
   Rs = fmin(fmax(Rs, (fast)0), 1 - Tdd);
@@ -472,9 +482,41 @@ __global__ void __launch_bounds__(NGMAX) k_sw_column(int N, RadP P, isw_sw_table
     const fast mu0 = smu, a = (fast)alb[i];
     const size_t cs = (size_t)t.NP * NT * NG;
     const fast cssa = t.cssa[g], ray = t.ray[g], hzf = t.hzf[g];
-    // bottom up: each sub-level's optics, the clear sky's albedos (nothing kept) and the all-sky's (kept with what
-    // the downward pass reads)
-    fast Rd[NSS], Td[NSS], Ts[NSS], Tdd[NSS], Ad[NSS + 1], As[NSS + 1];
+    const fast cssa_i = t.cssa_i ? (fast)t.cssa_i[g] : (fast)0, g_i = t.g_i ? (fast)t.g_i[g] : (fast)0;
+    // Sub-level s's optics from its gas optical depth tg: the clear sky's (c*) and the all-sky's (the cloud's share
+    // mixed in), as the reference computes them (air.cu:2440-2458). Evaluated twice per sub-level, in the albedos'
+    // pass up and the fluxes' pass down, so that only tg and the albedos are kept between the passes (a third of the
+    // registers that keeping every reflectance and transmittance would need; the table lookups are not repeated).
+    auto optics = [&](int s, fast tg, fast& cRd, fast& cTd, fast& cRs, fast& cTs, fast& cTdd, fast& rd, fast& td,
+                      fast& rs, fast& ts, fast& tdd) {
+      const fast trs = ray * sr[s] * (fast)air.ray, ths = (fast)t.tau550 * hzf * sh[s];
+      const fast tt = tg + trs + ths, sc = trs + (fast)t.hssa * ths;
+      const fast ssa = tt > 0 ? fmin(fdiv(sc, tt), (fast)(1 - 1e-6)) : (fast)0;
+      const fast ga = sc > 0 ? fdiv((fast)t.hg * (fast)t.hssa * ths, sc) : (fast)0;
+      sw_layer(tt, ssa, ga, mu0, cRd, cTd, cRs, cTs, cTdd);
+      rd = cRd; td = cTd; rs = cRs; ts = cTs; tdd = cTdd;
+      const fast f = sfc[s];
+      if (f > 0 && stc[s] > 0) {
+        const fast tc = stc[s], t2 = tt + tc;
+        fast cw = cssa, cg = (fast)P.cloud_g;
+        if (t.cssa_i) {
+          const fast sl = sli[s], wl = sl * cssa, wi = (1 - sl) * cssa_i;
+          cw = wl + wi;
+          cg = cw > 0 ? fdiv(wl * (fast)P.cloud_g + wi * g_i, cw) : cg;
+        }
+        const fast sc2 = sc + cw * tc;
+        const fast g2 = fdiv(ga * sc + cg * cw * tc, sc2);
+        fast r1, t1, r2, t3, d1;
+        sw_layer(t2, fmin(fdiv(sc2, t2), (fast)(1 - 1e-6)), g2, mu0, r1, t1, r2, t3, d1);
+        rd = (1 - f) * cRd + f * r1;
+        td = (1 - f) * cTd + f * t1;
+        rs = (1 - f) * cRs + f * r2;
+        ts = (1 - f) * cTs + f * t3;
+        tdd = (1 - f) * cTdd + f * d1;
+      }
+    };
+    // bottom up: the gas optical depths (the table lookups), the clear sky's albedos (nothing kept) and the all-sky's
+    fast tgs[NSS], Ad[NSS + 1], As[NSS + 1];
     Ad[NSS] = a;
     As[NSS] = a;
     fast cAd = a, cAs = a;
@@ -491,54 +533,29 @@ __global__ void __launch_bounds__(NGMAX) k_sw_column(int N, RadP P, isw_sw_table
         const float* C = t.cia + (size_t)sit[s] * NG + g;
         tg += scia[s] * (C[0] + (C[NG] - C[0]) * ft);
       }
-      const fast trs = ray * sr[s] * (fast)air.ray, ths = (fast)t.tau550 * hzf * sh[s];
-      const fast tt = tg + trs + ths, sc = trs + (fast)t.hssa * ths;
-      const fast ssa = tt > 0 ? fmin(sc / tt, (fast)(1 - 1e-6)) : (fast)0;
-      const fast ga = sc > 0 ? (fast)t.hg * (fast)t.hssa * ths / sc : (fast)0;
-      fast cRd, cTd, cRs, cTs, cTdd;
-      sw_layer(tt, ssa, ga, mu0, cRd, cTd, cRs, cTs, cTdd);
+      tgs[s] = tg;
+      fast cRd, cTd, cRs, cTs, cTdd, rd, td, rs, ts, tdd;
+      optics(s, tg, cRd, cTd, cRs, cTs, cTdd, rd, td, rs, ts, tdd);
       {
-        const fast den = 1 / (1 - cAd * cRd);
+        const fast den = fdiv(1, 1 - cAd * cRd);
         const fast nAd = cRd + cTd * cTd * cAd * den;
         cAs = cRs + (cTdd * cAs + cTs * cAd) * cTd * den;
         cAd = nAd;
       }
-      fast rd = cRd, td = cTd, rs = cRs, ts = cTs, tdd = cTdd;
-      const fast f = sfc[s];
-      if (f > 0 && stc[s] > 0) {
-        const fast tc = stc[s], t2 = tt + tc;
-        fast cw = cssa, cg = (fast)P.cloud_g;
-        if (t.cssa_i) {
-          const fast sl = sli[s], wl = sl * cssa, wi = (1 - sl) * t.cssa_i[g];
-          cw = wl + wi;
-          cg = cw > 0 ? (wl * (fast)P.cloud_g + wi * t.g_i[g]) / cw : cg;
-        }
-        const fast sc2 = sc + cw * tc;
-        const fast g2 = (ga * sc + cg * cw * tc) / sc2;
-        fast r1, t1, r2, t3, d1;
-        sw_layer(t2, fmin(sc2 / t2, (fast)(1 - 1e-6)), g2, mu0, r1, t1, r2, t3, d1);
-        rd = (1 - f) * cRd + f * r1;
-        td = (1 - f) * cTd + f * t1;
-        rs = (1 - f) * cRs + f * r2;
-        ts = (1 - f) * cTs + f * t3;
-        tdd = (1 - f) * cTdd + f * d1;
-      }
-      const fast den = 1 / (1 - Ad[s + 1] * rd);
+      const fast den = fdiv(1, 1 - Ad[s + 1] * rd);
       Ad[s] = rd + td * td * Ad[s + 1] * den;
       As[s] = rs + (tdd * As[s + 1] + ts * Ad[s + 1]) * td * den;
-      Rd[s] = rd;
-      Td[s] = td;
-      Ts[s] = ts;
-      Tdd[s] = tdd;
     }
-    // the fluxes down from the top: each layer's absorption, its top and bottom sub-level's, the surface's
+    // top down: the fluxes, each layer's absorption, its top and bottom sub-level's, the surface's
     const fast wg = t.sun[g];
     fast Fdir = 1, Fdif = 0, net = 1 - As[0], absk = 0;
 #pragma unroll
     for (int s = 0; s < NSS; s++) {
-      const fast den = 1 / (1 - Ad[s + 1] * Rd[s]);
-      const fast Fdn = Fdir * Tdd[s];
-      Fdif = (Td[s] * Fdif + Rd[s] * As[s + 1] * Fdn + Ts[s] * Fdir) * den;
+      fast cRd, cTd, cRs, cTs, cTdd, rd, td, rs, ts, tdd;
+      optics(s, tgs[s], cRd, cTd, cRs, cTs, cTdd, rd, td, rs, ts, tdd);
+      const fast den = fdiv(1, 1 - Ad[s + 1] * rd);
+      const fast Fdn = Fdir * tdd;
+      Fdif = (td * Fdif + rd * As[s + 1] * Fdn + ts * Fdir) * den;
       Fdir = Fdn;
       const fast n2 = Fdir + Fdif - (Ad[s + 1] * Fdif + As[s + 1] * Fdir);
       const int k = s / NSUB;

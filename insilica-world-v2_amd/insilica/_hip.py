@@ -51,7 +51,8 @@ class IswTransport(ctypes.Structure):
 
 
 class IswAir(ctypes.Structure):
-    _fields_ = ([(k, ctypes.c_double) for k in ('m_air', 'm_air_kg', 'o2', 'ray', 'broad', 'o3', 'eps')]
+    _fields_ = ([(k, ctypes.c_double) for k in ('m_air', 'm_air_kg', 'o2', 'ray', 'broad', 'o3', 'eps', 'rd',
+                                                    'vf')]
                 + [('cia', ctypes.c_int32), ('pad_', ctypes.c_int32)])
 
 
@@ -101,6 +102,12 @@ def lib():
             'isw_lw_setup': [i32, i32, vp, vp, vp, d, d, d, vp, vp, vp, vp, vp, d, i32, vp, i32, vp],
             'isw_lw_column': [i32, i32, vp, vp, vp, vp, vp, i32, vp],
             'isw_sw_column': [i32, i32, vp, vp, vp, vp, i32, vp, d, vp, vp, vp, vp, i32, vp],
+            'isw_hs_column': [i32, i32, vp, vp, vp, vp, vp, d, vp, vp, vp, vp],
+            'isw_surface_layer': [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            'isw_cell_speed': [vp, vp, vp, vp],
+            'isw_cell_speed_perot': [vp, vp, vp, vp, vp],
+            'isw_gas_src': [i32, i32, i32, vp, vp, d, vp, vp],
+            'isw_advance_time': [vp, d, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -115,7 +122,8 @@ def lib():
 EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_exchange', 'isw_vmix_heat',
            'isw_mass_update', 'isw_transport_step', 'isw_swe_step', 'isw_couple_stress', 'isw_land_column',
            'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_lw_setup', 'isw_lw_column', 'isw_sw_column',
-           'isw_timing_start', 'isw_timing_stop', 'isw_version')
+           'isw_hs_column', 'isw_surface_layer', 'isw_cell_speed', 'isw_cell_speed_perot', 'isw_gas_src',
+           'isw_advance_time', 'isw_timing_start', 'isw_timing_stop', 'isw_version')
 
 
 def check(rc, what):

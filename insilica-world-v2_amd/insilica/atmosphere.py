@@ -1,10 +1,12 @@
-"""The air column's parameters and the radiation pass of the air physics (reference: src/insilica/atmosphere.py).
+"""The air physics' parameters and drivers (reference: src/insilica/atmosphere.py).
 
 `PAR_ORDER` and `DEFAULTS` keep the reference's names, order and default values (atmosphere.py:26-43): the kernels
 read the parameters by position, so the array `par_array` builds is the reference's `AirPhysics.par` value for value
-(132 parameters, then the interfaces' theta* from the top, padded to LMAX). `AirPhysics.radiate` is the reference's
-k-table radiation call (atmosphere.py:117-123): the longwave's set-up and column pass, then the shortwave.
+(132 parameters, then the interfaces' theta* from the top, padded to LMAX). `AirPhysics` drives the column kernels
+through the C ABI: the surface layer and the wind speed, the Held-Suarez column with its interface crossing, the
+k-table radiation (the longwave's set-up and column pass, then the shortwave) and the device clock.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -50,7 +52,7 @@ DEFAULTS = dict(
     revap_area=0.05, root_q=0.0, ent_det=0.0, ent_ifs=0.0, shallow=0.0, cu_dth=0.0, cu_dp=0.0, buoy_v=0.0,
     cu_sub=0.0, bare_fx=0.0, det_all=0.0, cl_float=0.0, ml_clear=0.0, cu_cloud=0.0, firn=0.0, k_firn=0.34,
     rho_firn=450.0, c_ice=2090.0, lw_refl=0.0, ent_loc=0.0, ecc=0.0, lam_peri=0.0, liq_cold=20.0, cw_qs=0.0,
-    ent_lcl=0.0, sub_mix=0.0)
+    ent_lcl=0.0, sub_mix=0.0, cover=0.0, sfc_wind=0.0)
 assert len(PAR_ORDER) == 132 and set(PAR_ORDER) <= set(DEFAULTS)
 
 
@@ -65,32 +67,121 @@ def par_array(layers, params=None):
     return np.array([float(p[k]) for k in PAR_ORDER] + [float(x) for x in stars] + [0.0] * (LMAX - len(stars)))
 
 
-class AirPhysics:
-    """The radiation of the air physics (reference: AirPhysics with longwave='ktables', atmosphere.py:56-123):
-    `lw`, `sw` (insilica.radiation.Longwave/Shortwave), `cloudo` ((4L + 3), N: per layer cover, total cover, the
-    clouds' effect at the top, per layer longwave and shortwave in-cloud optical depths and liquid share), `pos`
-    (unit cell positions), `swalb` (the surface albedo the shortwave sees)."""
+def perot_operators(g):
+    """Perot's reconstruction of a cell's wind (east, north) from its edges' normal winds: (N, 6, 2) coefficients per
+    edge slot (reference: orodrag.operators, orodrag.py:20-28), u_cell = (1/A) sum_e sign_e le_e u_e (x_e - x_c)."""
+    from .constants import R_PLANET
+    pos = np.asarray(g.pos, float)
+    lon, lat = np.arctan2(pos[:, 1], pos[:, 0]), np.arcsin(np.clip(pos[:, 2], -1, 1))
+    east = np.stack([-np.sin(lon), np.cos(lon), np.zeros_like(lon)], 1)
+    north = np.stack([-np.sin(lat) * np.cos(lon), -np.sin(lat) * np.sin(lon), np.cos(lat)], 1)
+    ce = np.where(g.cellEdge >= 0, g.cellEdge, 0)
+    c = np.where(g.cellEdge >= 0, g.cellSign * g.le[ce] * getattr(g, 'radius', R_PLANET), 0.0) / g.areaCell[:, None]
+    d = g.xm[ce] - pos[:, None, :]
+    return np.stack([c * np.sum(d * east[:, None, :], 2), c * np.sum(d * north[:, None, :], 2)], 2)
 
-    def __init__(self, N, layers, params=None, mean_sun=False, lw_tables=None, sw_tables=None, pos=None,
-                 precision='mixed', device=None):
-        from .radiation import Longwave, Shortwave
-        self.N, self.L = N, layers
+
+def held_suarez_theta(p, lat):
+    """Held & Suarez's equilibrium potential temperature at pressure p (Pa) and latitude lat (radians) (reference:
+    atmosphere.py:161-165)."""
+    s2 = np.sin(lat) ** 2
+    T = np.maximum(200.0, (315.0 - 60.0 * s2 - 10.0 * np.log(p / P0) * (1 - s2)) * (p / P0) ** KAPPA)
+    return T * (P0 / p) ** KAPPA
+
+
+class AirPhysics:
+    """The air physics on a DeviceGrid (reference: atmosphere.py:56-158): `surface` (the surface layer's exchange
+    coefficients), `column` (the Held-Suarez forcing with the interfaces' crossing), `radiate` (the k-tables'
+    longwave and shortwave), and the device clock. Parameters in the reference's PAR_ORDER array (`par`, host).
+
+    forcing: 'held_suarez' steps the Held & Suarez column (air.cu hs_column); 'radiation' is the reference's moist
+    column (air.cu air_column), not built here: its `column` raises. longwave='ktables' builds `lw`, `sw` from
+    lw_tables, sw_tables (npz paths or dicts). `cloudo` ((4L + 3), N): per layer cover, total cover, the clouds'
+    effect at the top, per layer longwave and shortwave in-cloud optical depths and liquid share."""
+
+    def __init__(self, dg, layers, params=None, mean_sun=False, forcing='radiation', longwave='gray', lw_tables=None,
+                 sw_tables=None, air=None):
+        self.dg, self.L, self.forcing = dg, layers, forcing
         self.p = dict(DEFAULTS, **(params or {}))
         self.par = par_array(layers, params)
+        self._par = self.par.ctypes.data_as(ctypes.c_void_p)
         self.mode = 1 if mean_sun else 0
-        dev = device or torch.device('cuda', torch.cuda.current_device())
-        self.pos = None if pos is None else torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64), device=dev)
-        self.cloudo = torch.zeros(4 * layers + 3, N, dtype=torch.float64, device=dev)
-        self.lw = Longwave(N, layers, self.par, tables=lw_tables, precision=precision, device=dev,
-                           **{k: self.p[k] for k in ('co2', 'n2o', 'ch4', 'tmode') if k in self.p})
-        ice_re = self.p['re_ice'] if self.p.get('ice_sw', 0) > 0 else None
-        self.sw = Shortwave(N, layers, self.par, self.lw, tables=sw_tables, ice_re=ice_re, precision=precision,
-                            device=dev, **({'inhom': self.p['cloud_inhom']} if 'cloud_inhom' in self.p else {}))
-        self.swalb = torch.full((N,), float(self.p['alb']), dtype=torch.float64, device=dev)
+        from .radiation import _air
+        self.air = _air(air)
+        N, dev = dg.N, dg.dev
+        z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)
+        self.pos = torch.as_tensor(np.ascontiguousarray(dg.host.pos, dtype=np.float64), device=dev)
+        self.lat = torch.as_tensor(np.arcsin(np.clip(np.asarray(dg.host.pos)[:, 2], -1, 1)), device=dev)
+        self.uabs, self.wx, self.qnet, self.stored = z(N), z(max(layers - 1, 1), N), z(N), z(N)
+        self.hs = np.array([1.0 / (40 * DAY_S), 1.0 / (4 * DAY_S)])          # k_a, k_s (1/s)
+        self.perot = None
+        if self.p.get('sfc_wind', 0.0) > 0:
+            self.perot = torch.as_tensor(np.ascontiguousarray(perot_operators(dg.host)).reshape(-1), device=dev)
+        self.sfc = self.p['sfc'] > 0
+        self.cd, self.ch = (z(N), z(N)) if self.sfc else (None, None)
+        self.cloudo = z(4 * layers + 3, N)
+        self.lw = self.sw = None
+        if longwave == 'ktables':
+            from .radiation import Longwave, Shortwave
+            self.lw = Longwave(dg, layers, self.par, tables=lw_tables, air=air, precision=dg.precision, device=dev,
+                               **{k: self.p[k] for k in ('co2', 'n2o', 'ch4', 'tmode') if k in self.p})
+            ice_re = self.p['re_ice'] if self.p.get('ice_sw', 0) > 0 else None
+            self.sw = Shortwave(dg, layers, self.par, self.lw, tables=sw_tables, ice_re=ice_re, air=air,
+                                precision=dg.precision, device=dev,
+                                **({'inhom': self.p['cloud_inhom']} if 'cloud_inhom' in self.p else {}))
+            self.swalb = torch.full((N,), float(self.p['alb']), dtype=torch.float64, device=dev)
+
+    def _speed(self, u):
+        """The surface's wind speed at cells from the bottom layer's edge winds u (L, E), into self.uabs (reference:
+        atmosphere.py:109-115)."""
+        from . import _hip
+        ub = u[self.L - 1]
+        if self.perot is not None:
+            _hip.call('isw_cell_speed_perot', self.dg.ref, _hip.ptr(ub), _hip.ptr(self.perot), _hip.ptr(self.uabs),
+                      _hip.stream())
+        else:
+            _hip.call('isw_cell_speed', self.dg.ref, _hip.ptr(ub), _hip.ptr(self.uabs), _hip.stream())
 
     def radiate(self, mu, theta, Ts, t, interval, q=None, cw=None, gx=None):
         """The k-tables' longwave and shortwave from the present state, the sun over the coming `interval` (s) from
         the device clock t (1 double); the clouds' cloudo used when cw (the cloud water) is given."""
-        co = self.cloudo if cw is not None else None
-        self.lw(mu, theta, q, co, Ts, gx)
-        self.sw(self.mode, t, interval, self.pos, self.swalb, co)
+        if self.lw is not None:
+            co = self.cloudo if cw is not None else None
+            self.lw(mu, theta, q, co, Ts, gx)
+            self.sw(self.mode, t, interval, self.pos, self.swalb, co)
+
+    def surface(self, mu, theta, Ts, u, q=None, ice=None, land=None, lc=None):
+        """The surface layer's rho C_D U and rho C_H U into self.cd, self.ch (reference: atmosphere.py:125-132)."""
+        from . import _hip
+        if not self.sfc:
+            raise ValueError("surface(): the surface layer is off (params sfc = 0)")
+        self._speed(u)
+        _hip.call('isw_surface_layer', self.dg.N, self.L, self._par, ctypes.byref(self.air), _hip.ptr(self.uabs),
+                  _hip.ptr(mu), _hip.ptr(theta), _hip.ptr(q), _hip.ptr(Ts), _hip.ptr(ice), _hip.ptr(land),
+                  _hip.ptr(self.cd), _hip.ptr(self.ch), _hip.ptr(lc), _hip.stream())
+
+    def column(self, mu, theta, Ts, u, t, dt, **kw):
+        """One column step (reference: atmosphere.py:134-158): with forcing 'held_suarez', the relaxation and the
+        crossing into theta (in place) and self.wx (the mass to move up, for Layers.exchange), self.qnet,
+        self.stored; then the device clock t advances by dt."""
+        from . import _hip
+        if self.forcing != 'held_suarez':
+            raise NotImplementedError("the moist air column (forcing='radiation', air.cu air_column) is not built")
+        _hip.call('isw_hs_column', self.dg.N, self.L, self._par, self.hs.ctypes.data_as(ctypes.c_void_p),
+                  _hip.ptr(self.lat), _hip.ptr(mu), _hip.ptr(theta), float(dt), _hip.ptr(self.wx),
+                  _hip.ptr(self.qnet), _hip.ptr(self.stored), _hip.stream())
+        self.advance_time(t, dt)
+
+    @staticmethod
+    def advance_time(t, dt):
+        from . import _hip
+        _hip.call('isw_advance_time', _hip.ptr(t), float(dt), _hip.stream())
+
+
+def gas_sources(src, h, dt, x, layers):
+    """World._tick's gas sources (reference: world.py:343-346): src (G, N) into the bottom layer of x (G*L, N)."""
+    from . import _hip
+    G, N = src.shape
+    _hip.call('isw_gas_src', N, layers, G, _hip.ptr(src), _hip.ptr(h), float(dt), _hip.ptr(x), _hip.stream())
+
+

@@ -37,6 +37,7 @@ def build_hip(verbose=False):
              ('core.hip', 'core_mix.o', ['-DISW_FORM_MIXED=1', '-ffp-contract=fast']),
              ('radiation.hip', 'radiation_f64.o', ['-DISW_FORM_MIXED=0', '-ffp-contract=off']),
              ('radiation.hip', 'radiation_mix.o', ['-DISW_FORM_MIXED=1', '-ffp-contract=fast-honor-pragmas']),
+             ('column.hip', 'column.o', ['-ffp-contract=off']),
              ('capi.hip', 'capi.o', ['-ffp-contract=off'])]
     for src, obj, extra in units:
         out = OBJDIR / obj

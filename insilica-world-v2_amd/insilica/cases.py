@@ -208,3 +208,27 @@ def coupled_stack(g, seed=0, lmax=20, dtheta=2.0, du=5.0, land_frac=0.3):
     u = du * (w[:3] * en + w[3:] * nn) / math.sqrt(2.0)
     land = land_mask(g, rng, land_frac)
     return dict(h=h, b=b, u=u, theta=theta, q=x['q'], Ts=x['Ts'], land=land)
+
+
+def held_suarez(g, seed=1):
+    """Held & Suarez (1994) in two air layers, started as their protocol starts (reference: cases.py:657-670): at
+    rest and horizontally uniform, each layer at the area-mean equilibrium theta of its half's mass-mean pressure
+    (250 and 750 hPa), the lower capped 1 K below theta*, with 0.1 K of seeded noise. Returns h (2, N), b, u (2, E),
+    theta (2, N)."""
+    from .atmosphere import held_suarez_theta
+    h = np.full((2, g.N), MU_REF / 2)
+    A = g.areaCell / np.sum(g.areaCell)
+    lat = np.arcsin(np.clip(g.pos[:, 2], -1, 1))
+    rng = np.random.default_rng(seed)
+    means = [float(np.sum(A * held_suarez_theta(p, lat))) for p in (0.25e5, 0.75e5)]
+    means[1] = min(means[1], held_suarez_theta_star(g) - 1.0)
+    theta = np.stack([np.full(g.N, m) for m in means]) + 0.1 * rng.standard_normal((2, g.N))
+    return h, np.zeros(g.N), np.zeros((2, g.E)), theta
+
+
+def held_suarez_theta_star(g):
+    """The Held-Suarez interface's theta*: the area mean of the equilibrium theta at 500 hPa (reference:
+    cases.py:673-675)."""
+    from .atmosphere import held_suarez_theta
+    lat = np.arcsin(np.clip(g.pos[:, 2], -1, 1))
+    return float(np.sum(g.areaCell * held_suarez_theta(0.5e5, lat)) / np.sum(g.areaCell))

@@ -30,7 +30,9 @@ EC92_E = (7.661e-01, 7.730e-01, 7.865e-01, 8.172e-01, 9.595e-01)
 EC92_F = (5.851e-04, 5.665e-04, 7.204e-04, 7.463e-04, 1.076e-04)
 BAND_TOP = (2600, 4000, 5250, 7700, 9350, 12850, 16000, 29000, 38000, 50000)
 
-EARTH_AIR = dict(m_air=28.9647, m_air_kg=28.9647e-3, o2=0.2095, ray=1.0, broad=1.0, o3=1.0, eps=0.622)
+# The air's constants the column kernels read (isw_air): Earth air's (reference: kernels/air.cu:75-116 defaults).
+EARTH_AIR = dict(m_air=28.9647, m_air_kg=28.9647e-3, o2=0.2095, ray=1.0, broad=1.0, o3=1.0, eps=0.622, rd=287.0,
+                 vf=0.61)
 
 
 def ice_optics(re_um, band_top=BAND_TOP):

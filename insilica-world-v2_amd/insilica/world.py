@@ -31,7 +31,7 @@ from .transport import BETA, Transport
 
 THIN_WIND = 100.0
 GAS_TRACERS = (('co2', 400e-6), ('n2o', 0.32e-6), ('ch4', 1.8e-6))
-PRESETS = ('w2', 'w5', 'jet', 'coupled_stack')
+PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez')
 
 
 class World:
@@ -45,9 +45,32 @@ class World:
         self.grid = gridmod.for_planet(gridmod.get(n), **self.planet)
         self.dg = dg = DeviceGrid(self.grid, precision)
         g = self.grid
-        self.air = preset == 'coupled_stack'
-        self.sea = None
-        if self.air:
+        self.air = preset in ('coupled_stack', 'held_suarez')
+        self.sea = self.physics = None
+        if preset == 'held_suarez':
+            # two air layers forced by Held & Suarez's relaxation with the interface crossing (reference preset
+            # 'held_suarez', cases.py:680-682): the column step then the exchange of the mass it moves
+            from .atmosphere import AirPhysics
+            h, b, u, theta = cases.held_suarez(g, seed=P.get('seed', 1))
+            h = h * (GRAV / self.grav)
+            self.L = 2
+            self.dt = float(P.get('dt', 900.0 * 2.0 ** (5 - n)))
+            nu4 = P.get('nu4', visc4(g, P.get('visc4_U', 15.0)))
+            floor = P.get('floor', 10.0)
+            self.core = Layers(dg, 2, b=b, grav=self.grav, gprime=[0.0], drag=P.get('drag', 0.0),
+                               stress=P.get('stress', [0.0, cases.SURFACE_DRAG]), nu4=nu4, floor=floor,
+                               rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP),
+                               air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY), vmix=P.get('vmix', 0.0),
+                               thin_wind=P.get('thin_wind', THIN_WIND))
+            self.rows = dg.asarray(theta)
+            self.theta = self.rows
+            self.psi = dg.asarray(np.tile(cases.cosine_bell(g), (2, 1)))      # moved by the exchange only
+            self.tr = Transport(dg, layers=2, beta=P.get('beta', BETA), tracers=1)
+            phys = dict(theta_star=cases.held_suarez_theta_star(g), floor_=floor, grav=self.grav,
+                        **P.get('physics', {}))
+            self.physics = AirPhysics(dg, 2, phys, forcing='held_suarez')
+            self.time = dg.zeros(1)
+        elif self.air:
             built = cases.coupled_stack(g, seed=P.get('seed', 0), land_frac=P.get('land_frac', 0.3))
             h, b, u = built['h'] * (GRAV / self.grav), built['b'], built['u']
             self.L = 3
@@ -113,7 +136,8 @@ class World:
         self._stream = torch.cuda.Stream()
         self._stream.wait_stream(torch.cuda.current_stream())
         self.STATE = ('h', 'u', 'M', 'rows') + (('sea.h', 'sea.u', 'sea.M', 'sea.rows', 'sea.Tg')
-                                                 if self.sea is not None else ())
+                                                 if self.sea is not None else ()) + \
+            (('psi', 'time') if self.physics is not None else ())
 
     # ---- the tick ----
     def _kind(self, tick):
@@ -134,6 +158,11 @@ class World:
             self.sea.ground(self.dt)
             if kind:
                 self.sea.step(mix=False)
+        if self.physics is not None:
+            # the column (Held & Suarez's relaxation, the crossing into wx, the clock), then the exchange of that mass
+            # with its theta, momentum and tracer between the layers (reference: world.py:319-344)
+            self.physics.column(self.h, self.theta, None, self.u, self.time, self.dt)
+            self.core.exchange(self.physics.wx, self.h, self.u, self.M, self.psi, theta=self.theta)
 
     def launch_tick(self, graph=True):
         """Queue one tick on the world's stream without waiting for it (step() and Ensemble.step() call this)."""

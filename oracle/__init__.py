@@ -22,9 +22,9 @@ _i32p, _i8p, _f64p = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_in
 
 
 def build():
-    """Compile liboracle.so from oracle.c and radiation.c (seconds)."""
+    """Compile liboracle.so from oracle.c, radiation.c and column.c (seconds)."""
     cmd = ['gcc', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(LIB) + '.tmp',
-           str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), '-lm']
+           str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), str(HERE / 'column.c'), '-lm']
     subprocess.run(cmd, check=True)
     os.replace(str(LIB) + '.tmp', LIB)
 
@@ -73,6 +73,11 @@ def lib():
             'og_lw_column': (None, [i, i, i, vp, vp, vp, vp, vp]),
             'og_sw_column': (None, [i, i, i, vp, vp, vp, vp, i, d, d, vp, vp, vp, vp]),
             'og_sw_layer': (None, [d, d, d, d, vp]),
+            'og_hs_column': (None, [i, i, vp, vp, vp, vp, vp, d, vp, vp, vp]),
+            'og_surface_layer': (None, [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            'og_cell_speed': (None, [i, vp, vp, vp, vp, vp, vp]),
+            'og_cell_speed_perot': (None, [i, vp, vp, vp, vp]),
+            'og_gas_src': (None, [i, i, i, vp, vp, d, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

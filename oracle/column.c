@@ -1,0 +1,363 @@
+/* CPU oracle of the air column's Held-Suarez path and the surface-exchange kernels -- TEST INFRASTRUCTURE ONLY
+ * (parity checker and CPU baseline; never linked into the product).
+ *
+ * A scalar restatement of the reference's double path (src/insilica/kernels/air.cu): hs_column 2065-2109 with the
+ * interface crossing it calls (crossing 744-850, move_mass 495-520, column_energy 484-488, edge_density 525-539,
+ * edge_shape 245-270 (the double build's), layer_exner 129-140, p_of 186), surface_layer 2015-2061, cell_speed
+ * 1978-1989, cell_speed_perot 1994-2006, gas_src 2133-2137, advance_time 1975; in the reference's operation order,
+ * built with gcc -O2 -ffp-contract=off.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { P_GRAV = 13, P_CP = 14, P_KAPPA = 15, P_P0 = 16, P_FLOOR = 11, P_THETA_STAR = 10, P_LV = 23, P_MOIST = 22,
+       P_MU_BL = 9, P_BL_MIX = 70, P_BL_CAP = 73,
+       P_RHO_ICE = 29, P_ICE_COVER = 30, P_BULK_GUST = 75, P_Q_SEA = 97, P_Z0_LAND = 91, P_THS = 132 };
+
+static void layer_exner(int L, const double* mu, const double* par, double* pb, double* pib) {
+  const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP];
+  double p = 0, xt = 0;
+  for (int k = 0; k < L; k++) {
+    const double pt = p;
+    p = p + grav * mu[k];
+    pb[k] = p;
+    const double x = pow(p / p0, kappa);
+    pib[k] = cp * (p * x - pt * xt) / ((kappa + 1) * (p - pt));
+    xt = x;
+  }
+}
+
+/* theta* of interface j (between layers j and j+1): the lowest is theta_star, the others par's list from the top. */
+static double tstar(int j, int L, const double* par) { return j == L - 2 ? par[P_THETA_STAR] : par[P_THS + j]; }
+
+/* The truncated-exponential shape s of a layer's theta between its edges a < b with mean m: r(s) = (m - a) / (b - a),
+ * r(s) = 1 - 1/s + 1/(e^s - 1), by safeguarded Newton in a bracket (air.cu:245-270). */
+static double edge_shape(double a, double b, double m) {
+  const double W = b - a;
+  double x = (m - a) / W;
+  if (x < 1e-6) x = 1e-6;
+  if (x > 1 - 1e-6) x = 1 - 1e-6;
+  double lo = -700, hi = 700, s = 12 * (x - 0.5);
+  for (int it = 0; it < 40; it++) {
+    double r, dr;
+    if (fabs(s) < 1e-4) {
+      r = 0.5 + s / 12;
+      dr = 1.0 / 12;
+    } else {
+      const double e1 = expm1(s);
+      r = 1 - 1 / s + 1 / e1;
+      dr = 1 / (s * s) - (e1 + 1) / (e1 * e1);
+    }
+    if (r < x) lo = s; else hi = s;
+    double sn = s - (r - x) / dr;
+    if (r == x) break;
+    if (!(sn >= lo && sn <= hi) || !(dr > 0)) sn = 0.5 * (lo + hi);
+    if (fabs(sn - s) < 1e-12 * (1 + fabs(s))) {
+      s = sn;
+      break;
+    }
+    s = sn;
+  }
+  return s;
+}
+
+/* The layer's share of mass per kelvin at its lower (fa) and upper (fb) edge (air.cu:525-539). */
+static void edge_density(double a, double b, double m, double* fa, double* fb) {
+  if (!(b > a)) {
+    *fa = 0;
+    *fb = 0;
+    return;
+  }
+  const double W = b - a, s = edge_shape(a, b, m);
+  if (fabs(s) < 1e-6) {
+    *fa = (1 - s / 2) / W;
+    *fb = (1 + s / 2) / W;
+    return;
+  }
+  const double e1 = expm1(s);
+  *fa = s / e1 / W;
+  *fb = s * (e1 + 1) / e1 / W;
+}
+
+static double column_energy(int L, const double* mu, const double* pib, const double* th, const double* q,
+                            const double* par) {
+  double E = 0;
+  for (int k = 0; k < L; k++) E += mu[k] * pib[k] * th[k] + par[P_LV] * mu[k] * q[k];
+  return E;
+}
+
+/* Move m (> 0) kg/m^2 from layer `from` to `to`, re-dividing the column; the energy residual to the receiver. */
+static void move_mass(int L, const double* par, double* mu, double* th, double* q, double* pib, int from, int to,
+                      double m, double th_leave, double th_arrive, double q_leave, double q_arrive) {
+  if (m == 0) return;
+  double pb2[OG_LMAX], pib2[OG_LMAX], mu2[OG_LMAX], th2[OG_LMAX], q2[OG_LMAX];
+  const double Eb = column_energy(L, mu, pib, th, q, par);
+  for (int k = 0; k < L; k++) {
+    mu2[k] = mu[k];
+    th2[k] = th[k];
+    q2[k] = q[k];
+  }
+  mu2[from] -= m;
+  mu2[to] += m;
+  layer_exner(L, mu2, par, pb2, pib2);
+  th2[from] = (mu[from] * th[from] - m * th_leave) / mu2[from];
+  th2[to] = (mu[to] * th[to] + m * th_arrive) / mu2[to];
+  q2[from] = (mu[from] * q[from] - m * q_leave) / mu2[from];
+  q2[to] = (mu[to] * q[to] + m * q_arrive) / mu2[to];
+  const double Ea = column_energy(L, mu2, pib2, th2, q2, par);
+  th2[to] += (Eb - Ea) / (mu2[to] * pib2[to]);
+  for (int k = 0; k < L; k++) {
+    th[k] = th2[k];
+    mu[k] = mu2[k];
+    pib[k] = pib2[k];
+    q[k] = q2[k];
+  }
+}
+
+/* The dry crossing of every interface theta*_j by the heating dth (air.cu:744-850): wxo[j] the mass moved up across
+ * interface j. Dry air (q = 0): the edge-vapor and freeze-drying options carry no vapor and move the same mass. The
+ * mixed bottom layer (bl_mix) needs L >= 3 (its top edge is then an interface). */
+static void crossing(int L, const double* par, double* mu, double* th, double* q, const double* dth, double dth_up,
+                     double* pib, double* wxo) {
+  const double floor_ = par[P_FLOOR];
+  for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
+  if (par[P_THETA_STAR] <= 0) return;
+  double dmj[OG_LMAX], thcj[OG_LMAX], efa[OG_LMAX], efb[OG_LMAX];
+  int got[OG_LMAX] = {0};
+#define DENS(k, fa, fb)                                                                         \
+  do {                                                                                          \
+    if (!got[k]) {                                                                              \
+      edge_density(tstar((k), L, par), tstar((k) - 1, L, par), th[k] - dth[k], &efa[k], &efb[k]); \
+      got[k] = 1;                                                                               \
+    }                                                                                           \
+    fa = efa[k];                                                                                \
+    fb = efb[k];                                                                                \
+  } while (0)
+  for (int j = 0; j + 1 < L; j++) {
+    const int a = j, b = j + 1;
+    const double ts_ = tstar(j, L, par);
+    double dm = 0, thc = ts_, thd = ts_;
+    const int blm = par[P_BL_MIX] > 0;
+    if (dth[b] > 0 && blm && b == L - 1) {
+      if (th[b] >= ts_) {                              /* the mixed bottom layer takes in the air above */
+        double fa, fb;
+        DENS(a, fa, fb);
+        dm = -fmin(fmin(mu[a] * dth[b] * fa, fmax(mu[a] - par[P_MU_BL], 0.0) * par[P_BL_CAP]), 1e30);
+        thd = ts_ - 0.5 * dth[b];
+      }
+    } else if (blm && b < L - 1 && th[b] >= ts_) {
+      dm = mu[b] - floor_;
+      thc = th[b];
+    } else if (dth[b] > 0) {
+      const double eh = dth[b];
+      if (th[b] >= ts_) {
+        dm = mu[b] - floor_;
+        thc = th[b];
+      } else if (b == L - 1) {
+        dm = mu[b] * eh / (2 * (ts_ - (th[b] - dth[b])));
+      } else {
+        double fa, fb;
+        DENS(b, fa, fb);
+        dm = mu[b] * eh * fb;
+      }
+    }
+    const double da = a == 0 ? dth_up : dth[a];
+    if (blm && a >= 1 && th[a] <= ts_ && dm <= 0) {
+      const double down = mu[a] - floor_;
+      thd = th[a];
+      dm -= down;
+    } else if (da < 0) {
+      double down;
+      if (th[a] <= ts_) {
+        down = mu[a] - floor_;
+        thd = th[a];
+      } else if (a == 0) {
+        down = mu[a] * (-da) / (2 * ((th[a] - dth[a]) - ts_));
+      } else {
+        double fa, fb;
+        DENS(a, fa, fb);
+        down = mu[a] * (-da) * fa;
+      }
+      dm -= down;
+    }
+    dmj[j] = dm;
+    thcj[j] = dm > 0 ? thc : thd;
+  }
+#undef DENS
+  for (int j = L - 2; j >= 0; j--) {
+    const int a = j, b = j + 1;
+    double dm = dmj[j];
+    const double maxup = mu[b] - floor_, maxdn = mu[a] - floor_;
+    if (dm > maxup) dm = maxup > 0 ? maxup : 0.0;
+    if (-dm > maxdn) dm = maxdn > 0 ? -maxdn : 0.0;
+    if (dm > 0)
+      move_mass(L, par, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], q[b], q[b]);
+    else if (dm < 0)
+      move_mass(L, par, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], q[a], q[a]);
+    wxo[j] = dm;
+  }
+  const double tiny = floor_ * 1.000001;
+  for (int j = L - 2; j >= 0; j--) {
+    const int a = j, b = j + 1;
+    const double ts_ = tstar(j, L, par);
+    if (mu[b] <= tiny && th[b] > ts_) {
+      th[a] += mu[b] * pib[b] * (th[b] - ts_) / (mu[a] * pib[a]);
+      th[b] = ts_;
+    }
+    if (mu[a] <= tiny && th[a] < ts_) {
+      th[b] -= mu[a] * pib[a] * (ts_ - th[a]) / (mu[b] * pib[b]);
+      th[a] = ts_;
+    }
+  }
+}
+
+/* hs_column (air.cu:2065-2109): Newtonian relaxation toward Held & Suarez's equilibrium, then the crossing. */
+void og_hs_column(int N, int L, const double* par, const double* hs, const double* lat, const double* mu_in,
+                  double* theta, double dt, double* wx, double* qnet, double* stored) {
+  const double cp = par[P_CP], kappa = par[P_KAPPA], p0 = par[P_P0];
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double mu[OG_LMAX], th[OG_LMAX], q[OG_LMAX], pb[OG_LMAX], pib[OG_LMAX], dth[OG_LMAX], wxo[OG_LMAX];
+    for (int k = 0; k < L; k++) {
+      mu[k] = mu_in[(size_t)k * N + i];
+      th[k] = theta[(size_t)k * N + i];
+      q[k] = 0;
+    }
+    layer_exner(L, mu, par, pb, pib);
+    double E0 = 0, Qs = 0;
+    for (int k = 0; k < L; k++) E0 += mu[k] * pib[k] * th[k];
+    const double s2 = sin(lat[i]) * sin(lat[i]), c2 = 1 - s2;
+    const double ps = pb[L - 1];
+    double pt = 0;
+    for (int k = 0; k < L; k++) {
+      const double pbar = pow(pib[k] / cp, 1 / kappa) * p0;
+      double Teq = (315 - 60 * s2 - 10 * log(pbar / p0) * c2) * pow(pbar / p0, kappa);
+      if (Teq < 200) Teq = 200;
+      const double theq = Teq * cp / pib[k];
+      const double st = pt / ps, sb = pb[k] / ps;
+      const double lo = st > 0.7 ? st : 0.7;
+      const double f = sb > 0.7 ? ((sb - 0.7) * (sb - 0.7) - (lo - 0.7) * (lo - 0.7)) / (2 * 0.3 * (sb - st)) : 0.0;
+      const double kt = hs[0] + (hs[1] - hs[0]) * f * c2 * c2;
+      const double d = (theq - th[k]) * (1 - exp(-kt * dt));
+      dth[k] = d;
+      th[k] += d;
+      Qs += mu[k] * pib[k] * d / dt;
+      pt = pb[k];
+    }
+    double dth_up = dth[0];
+    if (L == 2) {
+      const double pi_ = pb[0], si = pi_ / ps;
+      double Ti = (315 - 60 * s2 - 10 * log(pi_ / p0) * c2) * pow(pi_ / p0, kappa);
+      if (Ti < 200) Ti = 200;
+      const double thi = Ti * pow(p0 / pi_, kappa);
+      const double ki = hs[0] + (hs[1] - hs[0]) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
+      dth_up = (thi - par[P_THETA_STAR]) * (1 - exp(-ki * dt));
+    }
+    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo);
+    for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
+    double E1 = 0;
+    for (int k = 0; k < L; k++) {
+      theta[(size_t)k * N + i] = th[k];
+      E1 += mu[k] * pib[k] * th[k];
+    }
+    qnet[i] = Qs;
+    stored[i] = E1 - E0;
+  }
+}
+
+/* Tetens over water (air.cu:157-159), with the air's eps. */
+static double qsat(double T, double p, double eps) { return eps * 611.0 * exp(17.27 * (T - 273.15) / (T - 35.85)) / p; }
+
+/* surface_layer (air.cu:2015-2061): Louis's bulk exchange with Charnock's sea roughness, four fixed-point passes. */
+void og_surface_layer(int N, int L, const double* par, const og_air* air, const double* uabs, const double* mu_in,
+                      const double* theta, const double* qv, const double* Ts, const double* ice, const double* land,
+                      double* cd, double* ch, const double* lc) {
+  const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], rd = air->rd, vf = air->vf;
+  const double moist = par[P_MOIST], q_sea = par[P_Q_SEA], gust = par[P_BULK_GUST];
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    const int bot = L - 1;
+    double ps = 0;
+    for (int k = 0; k < L; k++) ps += grav * mu_in[(size_t)k * N + i];
+    const double mb = mu_in[(size_t)bot * N + i], th = theta[(size_t)bot * N + i];
+    const double Ta = th * pow(ps / p0, kappa), T0 = Ts[i];
+    const double rho = ps / (rd * Ta);
+    const double qa = qv ? qv[(size_t)bot * N + i] : 0.0;
+    const int isl = land && land[i] > 0.5;
+    const double q0 = qv && moist > 0 && !isl ? (q_sea > 0 ? q_sea : 1.0) * qsat(T0, ps, air->eps) : qa;
+    const double fice = qv && moist > 0 && ice && !isl ? fmin(1.0, ice[i] / par[P_RHO_ICE] / par[P_ICE_COVER]) : 0.0;
+    const double depth = rd * Ta / grav * log(ps / fmax(ps - grav * mb, 1.0));
+    const double zr = fmax(10.0, 0.1 * depth);
+    const double dthv = (Ta - T0) + vf * Ta * (qa - q0);
+    const double K = 0.4, nu = 1.5e-5, b = 5, c = 5;
+    double U = sqrt(uabs[i] * uabs[i] + gust * gust), Cd = 1e-3, Ch = 1e-3;
+    for (int it = 0; it < 4; it++) {
+      const double us = fmax(sqrt(Cd) * U, 1e-3);
+      double z0, zh;
+      if (isl) {
+        z0 = lc ? lc[i] : par[P_Z0_LAND];
+        zh = 0.1 * z0;
+      } else {
+        const double zs = 0.011 * us * us / grav + 0.11 * nu / us, zsh = 0.40 * nu / us;
+        z0 = (1 - fice) * zs + fice * 1e-3;
+        zh = (1 - fice) * zsh + fice * 1e-4;
+      }
+      const double lm = log(zr / z0), lh = log(zr / zh);
+      const double CdN = K * K / (lm * lm), ChN = K * K / (lm * lh);
+      const double Ri = grav * zr * dthv / (Ta * U * U);
+      double fm, fh;
+      if (Ri > 0) {
+        const double s = sqrt(1 + c * Ri);
+        fm = 1 / (1 + 2 * b * Ri / s);
+        fh = 1 / (1 + 3 * b * Ri * s);
+      } else {
+        const double d = 1 + 3 * b * c * CdN * sqrt(zr / z0 * -Ri);
+        fm = 1 - 2 * b * Ri / d;
+        fh = 1 - 3 * b * Ri / d;
+      }
+      Cd = CdN * fm;
+      Ch = ChN * fh;
+      const double B = -Ch * U * dthv;
+      const double ws = B > 0 ? cbrt(grav / Ta * B * depth) : 0.0;
+      U = sqrt(uabs[i] * uabs[i] + gust * gust + 1.44 * ws * ws);
+    }
+    cd[i] = rho * Cd * U;
+    ch[i] = rho * Ch * U;
+  }
+}
+
+void og_cell_speed(int N, const double* u, const int32_t* cellEdge, const double* le, const double* de,
+                   const double* areaCell, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double ke = 0;
+    for (int m = 0; m < 6; m++) {
+      const int e = cellEdge[i * 6 + m];
+      if (e < 0) break;
+      ke += le[e] * de[e] * 0.25 * u[e] * u[e];
+    }
+    out[i] = sqrt(2 * ke / areaCell[i]);
+  }
+}
+
+void og_cell_speed_perot(int N, const double* u, const int32_t* cellEdge, const double* op, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double ue = 0, un = 0;
+    for (int m = 0; m < 6; m++) {
+      const int e = cellEdge[i * 6 + m];
+      if (e < 0) break;
+      ue += op[(i * 6 + m) * 2] * u[e];
+      un += op[(i * 6 + m) * 2 + 1] * u[e];
+    }
+    out[i] = sqrt(ue * ue + un * un);
+  }
+}
+
+void og_gas_src(int N, int L, int G, const double* src, const double* h, double dt, double* x) {
+  for (int g = 0; g < G; g++)
+    for (int i = 0; i < N; i++) x[((size_t)g * L + L - 1) * N + i] += src[(size_t)g * N + i] * dt / h[(size_t)(L - 1) * N + i];
+}

@@ -1,0 +1,57 @@
+"""The CPU oracle of the Held-Suarez column and the surface exchange -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrappers of column.c (the reference's double path of air.cu hs_column with its crossing, surface_layer,
+cell_speed, cell_speed_perot, gas_src) on host numpy arrays. Used by the tests as the checker of the HIP kernels.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _p, lib
+from .radiation import air_struct
+
+
+def _d(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def hs_column(par, hs, lat, mu, theta, dt):
+    """air.cu hs_column: returns (theta, wx, qnet, stored); mu is read only."""
+    par, hs, lat, mu = _d(par), _d(hs), _d(lat), _d(mu)
+    theta = _d(theta).copy()
+    L, N = mu.shape
+    wx, qnet, stored = np.zeros((max(L - 1, 1), N)), np.zeros(N), np.zeros(N)
+    lib().og_hs_column(N, L, _p(par), _p(hs), _p(lat), _p(mu), _p(theta), float(dt), _p(wx), _p(qnet), _p(stored))
+    return theta, wx, qnet, stored
+
+
+def surface_layer(par, uabs, mu, theta, Ts, qv=None, ice=None, land=None, lc=None, air=None):
+    """air.cu surface_layer: returns (cd, ch)."""
+    par, uabs, mu, theta, Ts, qv, ice, land, lc = (_d(x) for x in (par, uabs, mu, theta, Ts, qv, ice, land, lc))
+    L, N = mu.shape
+    cd, ch = np.zeros(N), np.zeros(N)
+    lib().og_surface_layer(N, L, _p(par), ctypes.byref(air_struct(air)), _p(uabs), _p(mu), _p(theta), _p(qv), _p(Ts),
+                           _p(ice), _p(land), _p(cd), _p(ch), _p(lc))
+    return cd, ch
+
+
+def cell_speed(g, u):
+    out = np.zeros(g.N)
+    ce = np.ascontiguousarray(g.cellEdge, dtype=np.int32)
+    lib().og_cell_speed(g.N, _p(_d(u)), _p(ce), _p(_d(g.le)), _p(_d(g.de)), _p(_d(g.areaCell)), _p(out))
+    return out
+
+
+def cell_speed_perot(g, u, op):
+    out = np.zeros(g.N)
+    ce = np.ascontiguousarray(g.cellEdge, dtype=np.int32)
+    lib().og_cell_speed_perot(g.N, _p(_d(u)), _p(ce), _p(_d(op)), _p(out))
+    return out
+
+
+def gas_src(src, h, dt, x, L):
+    src, h = _d(src), _d(h)
+    x = _d(x).copy()
+    G, N = src.shape
+    lib().og_gas_src(N, L, G, _p(src), _p(h), float(dt), _p(x))
+    return x

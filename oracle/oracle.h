@@ -73,7 +73,7 @@ double og_dsum(const double* x, const double* y, int64_t n);
  * properties (sub, float32), the clamped table positions (rounded through fminf/fmaxf), and every g-point's
  * contribution, rounded to float32 before the FP64 sum over g-points. par: the reference's PAR_ORDER array. */
 typedef struct og_air {
-  double m_air, m_air_kg, o2, ray, broad, o3, eps;
+  double m_air, m_air_kg, o2, ray, broad, o3, eps, rd, vf;
   int32_t cia, pad_;
 } og_air;
 
@@ -100,5 +100,17 @@ void og_sw_column(int N, int L, int nsub, const double* par, const og_swt* t, co
                   double* swf);
 /* The Delta-Eddington layer alone (air.cu:2358-2382), for the closed-form checks. out: Rd, Td, Rs, Ts, Tdd. */
 void og_sw_layer(double tau, double ssa, double g, double mu0, double* out);
+
+
+/* ---- the air column's Held-Suarez path and the surface exchange (reference: kernels/air.cu); column.c ---- */
+void og_hs_column(int N, int L, const double* par, const double* hs, const double* lat, const double* mu,
+                  double* theta, double dt, double* wx, double* qnet, double* stored);
+void og_surface_layer(int N, int L, const double* par, const og_air* air, const double* uabs, const double* mu,
+                      const double* theta, const double* qv, const double* Ts, const double* ice, const double* land,
+                      double* cd, double* ch, const double* lc);
+void og_cell_speed(int N, const double* u, const int32_t* cellEdge, const double* le, const double* de,
+                   const double* areaCell, double* out);
+void og_cell_speed_perot(int N, const double* u, const int32_t* cellEdge, const double* op, double* out);
+void og_gas_src(int N, int L, int G, const double* src, const double* h, double dt, double* x);
 
 #endif

@@ -11,12 +11,14 @@ import numpy as np
 from . import _p, lib
 
 _f32p = ctypes.POINTER(ctypes.c_float)
-EARTH_AIR = dict(m_air=28.9647, m_air_kg=28.9647e-3, o2=0.2095, ray=1.0, broad=1.0, o3=1.0, eps=0.622)
+EARTH_AIR = dict(m_air=28.9647, m_air_kg=28.9647e-3, o2=0.2095, ray=1.0, broad=1.0, o3=1.0, eps=0.622, rd=287.0,
+                 vf=0.61)
 CLOUD_SSA = (0.60, 0.75, 0.98, 0.994, 0.9995, 0.9998, 0.99999, 0.99999, 0.99999, 0.99999)
 
 
 class OGAir(ctypes.Structure):
-    _fields_ = ([(k, ctypes.c_double) for k in ('m_air', 'm_air_kg', 'o2', 'ray', 'broad', 'o3', 'eps')]
+    _fields_ = ([(k, ctypes.c_double) for k in ('m_air', 'm_air_kg', 'o2', 'ray', 'broad', 'o3', 'eps', 'rd',
+                                                    'vf')]
                 + [('cia', ctypes.c_int32), ('pad_', ctypes.c_int32)])
 
 

@@ -33,6 +33,11 @@ class OracleWorld:
         self.tick = w.tick
         self.Fm, self.M1 = np.zeros((self.L, og.E)), np.zeros((self.L, og.N))
         self.sea = None
+        self.physics = None
+        if getattr(w, 'physics', None) is not None:                 # the Held-Suarez column
+            p = w.physics
+            self.physics = dict(par=np.array(p.par), hs=np.array(p.hs), lat=_np(p.lat))
+            self.psi, self.time = _np(w.psi), float(w.time[0])
         if w.sea is not None:
             s = w.sea
             self.surface_drag = w.surface_drag
@@ -85,9 +90,18 @@ class OracleWorld:
                 transport(og, s['L'], s['M'], s['M1'], s['Fm'], s['rows'], order=3)
                 s['M'][...] = s['M1']
                 ocn_cap(og, s['h'], self.rho * self.cw, s['cap'], s['land'], self.c_land)
+        if self.physics is not None:
+            from .column import hs_column
+            ph = self.physics
+            th, wx, _, _ = hs_column(ph['par'], ph['hs'], ph['lat'], self.h, self.rows[0:L], self.dt)
+            self.rows[0:L] = th
+            self.time += self.dt
+            self.core.exchange(wx[:L - 1], self.h, self.u, self.M, self.psi, theta=self.rows[0:L])
 
     def state(self):
         out = dict(h=self.h, u=self.u, M=self.M, rows=self.rows)
+        if self.physics is not None:
+            out.update(psi=self.psi, time=np.array([self.time]))
         if self.sea is not None:
             for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap'):
                 if self.sea[k] is not None:

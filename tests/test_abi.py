@@ -61,6 +61,16 @@ def test_bad_arguments_rejected_on_host():
     assert lib.isw_lw_column(10, 3, par, ctypes.byref(lwt), buf, buf, buf, 7, None) == E          # unknown form
     assert lib.isw_sw_column(10, 3, par, ctypes.byref(swt), ctypes.byref(air), buf, 0, None, 3600.0, None, buf,
                              None, buf, 0, None) == E
+    # the column: one layer has no interface; the mixed bottom layer needs three; no clock pointer
+    hs = (ctypes.c_double * 2)(1e-6, 1e-5)
+    assert lib.isw_hs_column(10, 1, par, hs, buf, buf, buf, 600.0, buf, buf, buf, None) == E
+    par[70] = 1.0
+    assert lib.isw_hs_column(10, 2, par, hs, buf, buf, buf, 600.0, buf, buf, buf, None) == E
+    assert lib.isw_surface_layer(10, 2, par, None, buf, buf, buf, None, buf, None, None, buf, buf, None, None) == E
+    assert lib.isw_cell_speed(None, buf, buf, None) == E
+    assert lib.isw_gas_src(10, 3, -1, None, None, 1.0, None, None) == E
+    assert lib.isw_gas_src(10, 3, 0, None, None, 1.0, None, None) == 0          # no gases: nothing to do
+    assert lib.isw_advance_time(None, 1.0, None) == E
 
 
 def test_request_draws_are_seeded():
