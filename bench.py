@@ -39,6 +39,7 @@ def parse():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--batched', action='store_true', help='ensemble: batch the worlds of each level on one mesh')
     ap.add_argument('--serve', type=int, default=0, help='run the serving workload with this many requests')
+    ap.add_argument('--serve-levels', default='5,6,7', help='serving: the request levels')
     ap.add_argument('--radiation', action='store_true', help='run the column-radiation workload (passes/s)')
     ap.add_argument('--cpu-cols', type=int, default=0, help='radiation: columns in the CPU oracle sample (0: all)')
     return ap.parse_args()
@@ -109,22 +110,27 @@ def ensemble_main(a):
 
 
 def serve_main(a):
-    """The serving workload: `a.serve` requests with seeded Poisson arrivals (mean 2/s) at levels 5-6, 1-30 model
-    days each, seeded planets; one JSON line with the aggregate model-hours/s and p50/p99 latency per model day."""
+    """The serving workload: `a.serve` requests with seeded Poisson arrivals (mean 2/s) at levels `a.serve_levels`
+    (default 5-7), 1-30 model days each, seeded planets; one JSON line with the aggregate model-hours/s and p50/p99
+    latency per model day."""
     import torch
     from insilica.serve import Server, draw_requests
     torch.cuda.set_device(0)
     from insilica import grid as gm
-    for n in (5, 6):
+    levels = tuple(int(x) for x in a.serve_levels.split(','))
+    for n in levels:
+        t0 = time.perf_counter()
         gm.get(n)                                    # the meshes are built (or loaded) before the clock starts
-    reqs = draw_requests(a.serve, seed=a.seed, rate=2.0, levels=(5, 6))
+        print(f'mesh level {n} ready in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
+    reqs = draw_requests(a.serve, seed=a.seed, rate=2.0, levels=levels)
     done, st = Server(precision='mixed').run(reqs)
     line = dict(metric='serving: model-hours/s and latency per model day (p50/p99), 1 GPU',
                 value=round(st['model_hours_per_s'], 2), unit='model-hours/s', n_gpus=1, steps=len(done), warmup=0,
                 ms_per_step=round(st['seconds'] / len(done) * 1e3, 3), higher_is_better=True, scaling='weak',
                 vs_baseline=None, dtype='f64 state, f32 stencils (mixed)', data='synthetic',
                 config=dict(workload=f'{len(done)} coupled_stack requests, Poisson arrivals 2/s (seed {a.seed}), levels '
-                                     f'5-6, 1-30 model days, seeded planets; world construction inside the timed loop',
+                                     f'{a.serve_levels}, 1-30 model days, seeded planets; world construction inside the '
+                                     f'timed loop',
                             parallelism='1 GPU, continuous admission, one stream and graph per world'),
                 day_p50_s=round(st['day_p50_s'], 4), day_p99_s=round(st['day_p99_s'], 4),
                 model_hours=st['model_hours'], seconds=round(st['seconds'], 3))
