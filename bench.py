@@ -40,9 +40,27 @@ def parse():
     ap.add_argument('--batched', action='store_true', help='ensemble: batch the worlds of each level on one mesh')
     ap.add_argument('--serve', type=int, default=0, help='run the serving workload with this many requests')
     ap.add_argument('--serve-levels', default='5,6,7', help='serving: the request levels')
+    ap.add_argument('--calibrate-copy', type=int, default=0,
+                    help='MiB: first run one plain copy of this size (the reference dispatch for HBM counters)')
     ap.add_argument('--radiation', action='store_true', help='run the column-radiation workload (passes/s)')
     ap.add_argument('--cpu-cols', type=int, default=0, help='radiation: columns in the CPU oracle sample (0: all)')
     return ap.parse_args()
+
+
+TRAFFIC_PROFILE = ROOT / 'profiles' / 'coupled_stack_level6_mixed_hbm_traffic.md'
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch of kernel `name` at level 6, mixed form, from the committed FETCH_SIZE / WRITE_SIZE
+    profile (copy-calibrated counters; the table's last column, MB per launch), or None if it has no row for it."""
+    import re
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        m = re.match(r'\|\s*`k_(\w+)`\s*\|.*\|\s*([0-9.]+)\s*\|\s*$', line)
+        if m and m.group(1) == name:
+            return float(m.group(2)) * 1e6
+    return None
 
 
 def make_world(level, precision):
@@ -275,6 +293,13 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if a.calibrate_copy:
+        # a plain copy of known size (reads and writes calibrate_copy MiB once each), for the HBM counters' scale
+        src = torch.ones(a.calibrate_copy * (1 << 20) // 8, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     print(f'building level-{a.level} worlds', file=sys.stderr, flush=True)
     wm = make_world(a.level, 'mixed')
     el_m = time_world(wm, a.steps, a.warmup)
@@ -294,8 +319,10 @@ def main():
     achieved = bytes_per / avg_s / 1e9
     total_ms = sum(v['ms'] for v in prof.values())
     total_bytes = sum(v['bytes'] for v in prof.values())
+    traffic = pmc_traffic(name) if a.level == 6 else None
     roofline = dict(bound='hbm', kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=bytes_per,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, bytes_per_launch=bytes_per,
+                    traffic_source=TRAFFIC_PROFILE.name if traffic else None,
                     avg_us=round(avg_s * 1e6, 3), launches_per_hour=k['launches'],
                     all_kernels_GBs=round(total_bytes / (total_ms / 1e3) / 1e9, 1),
                     kernel_ms_per_hour=round(total_ms, 3))
