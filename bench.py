@@ -113,6 +113,24 @@ def ensemble_main(a):
     el = time.perf_counter() - t0
     lat = np.concatenate([np.asarray(x) for x in ens.day_latencies()])
     levels = [lev for lev, _ in specs]
+    roofline = None
+    if a.batched:
+        # the largest batch's dominant kernel over one model hour of direct launches (the library's event timing)
+        from insilica import _hip
+        unit = max(ens.units, key=lambda u: u.grid.N if hasattr(u, 'grid') else 0)
+        per_hour = int(round(unit.ticks_per_hour))
+        torch.cuda.synchronize()
+        _hip.timing_start()
+        unit.step(per_hour, graph=False)
+        torch.cuda.synchronize()
+        prof = _hip.timing_stop()
+        name, k = max(prof.items(), key=lambda kv: kv[1]['ms'])
+        avg_s = k['ms'] / k['launches'] / 1e3
+        ach = k['bytes'] / k['launches'] / avg_s / 1e9
+        roofline = dict(bound='hbm', kernel=name, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                        frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                        bytes_per_launch=k['bytes'] / k['launches'], avg_us=round(avg_s * 1e6, 2),
+                        batch=f'{len(unit)} worlds, {unit.grid.N} cells')
     line = dict(metric='ensemble model-hours/s and wall seconds per model day (p50/p99), 1 GPU',
                 value=round(len(worlds) * hours / el, 2), unit='model-hours/s', n_gpus=1, steps=hours, warmup=1,
                 ms_per_step=round(el / hours * 1e3, 3), higher_is_better=True, scaling='weak', vs_baseline=None,
@@ -123,7 +141,7 @@ def ensemble_main(a):
                             parallelism='1 GPU, ' + ('one batch (union mesh) per level, one launch per phase'
                                                      if a.batched else 'one stream and graph per world')),
                 day_p50_s=round(float(np.percentile(lat, 50)), 4), day_p99_s=round(float(np.percentile(lat, 99)), 4),
-                days_measured=int(lat.size))
+                days_measured=int(lat.size), roofline=roofline)
     print(json.dumps(line), flush=True)
 
 
