@@ -159,7 +159,7 @@ def serve_main(a):
         gm.get(n)                                    # the meshes are built (or loaded) before the clock starts
         print(f'mesh level {n} ready in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
     reqs = draw_requests(a.serve, seed=a.seed, rate=2.0, levels=levels)
-    done, st = Server(precision='mixed').run(reqs)
+    done, st = Server(precision='mixed').run(reqs, log=lambda m: print(m, file=sys.stderr, flush=True))
     line = dict(metric='serving: model-hours/s and latency per model day (p50/p99), 1 GPU',
                 value=round(st['model_hours_per_s'], 2), unit='model-hours/s', n_gpus=1, steps=len(done), warmup=0,
                 ms_per_step=round(st['seconds'] / len(done) * 1e3, 3), higher_is_better=True, scaling='weak',

@@ -44,6 +44,10 @@ int cell_speed(int, int, const double*, const int*, const double*, const double*
 int cell_speed_perot(int, int, const double*, const int*, const double*, double*, hipStream_t);
 int gas_src(int, int, int, const double*, const double*, double, double*, hipStream_t);
 int advance_time(double*, double, hipStream_t);
+int air_column(int, int, const double*, const isw_air*, int, const double*, const double*, const double*,
+               const double*, const double*, double*, double*, double, double*, double*, double*, double*, double*,
+               double*, double*, double*, double*, const double*, double*, double*, const double*, double*,
+               const double*, const double*, double*, const double*, double*, int, hipStream_t);
 }  // namespace isw_col
 
 #define ISW_DECLARE_RAD(NS)                                                                                          \
@@ -210,6 +214,28 @@ int32_t isw_hs_column(int32_t N, int32_t L, const double* par, const double* hs,
       !(dt > 0) || (par[70] > 0 && L < 3))
     return ISW_EBADARG;
   return isw_col::hs_column(N, L, par, hs, lat, mu, theta, dt, wx, qnet, stored, S(stream));
+}
+
+int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* air, int32_t mode, const double* t,
+                       const double* insol, const double* pos, const double* uabs, const double* mu, double* theta,
+                       double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,
+                       double* stored, double* rain, double* evap, double* conv, double* tsurf, double* rainp,
+                       double* dgo, const double* cap, double* qocean, double* qsw, const double* land,
+                       const double* soilcap, double* soil, double* pond, double* snow, double* cloudw, double* cloudo,
+                       const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
+                       const double* ch, const double* lc, double* firn, double* xfer, int32_t prec, void* stream) {
+  (void)qv; (void)ice; (void)land; (void)soilcap; (void)soil; (void)pond; (void)snow; (void)cloudw; (void)cloudo;
+  (void)lc;   // (read only by the moist column: with moist = 0 the reference never touches them)
+  if (N < 1 || L < 2 || L > ISW_LMAX || !par || !air || (mode != 0 && mode != 1) || !uabs || !mu || !theta ||
+      !Ts || !wx || !olr || !sw_abs || !stored || !rain || !evap || !conv || !tsurf || !(dt > 0) || !prec_ok(prec))
+    return ISW_EBADARG;
+  if (mode == 0 ? (!t || !pos) : !insol) return ISW_EBADARG;
+  // the options this library's column does not carry: the moist column, the mixed bottom layer, the cumulus
+  // layer's floating edges, the diagnostics rows and the gas tracers moved with the crossing
+  if (par[22] > 0 || par[70] > 0 || (par[111] > 0 && L > 3) || dgo || xfer) return ISW_EBADARG;
+  return isw_col::air_column(N, L, par, air, mode, t, insol, pos, uabs, mu, theta, Ts, dt, wx, olr, sw_abs, stored,
+                             rain, evap, conv, tsurf, rainp, cap, qocean, qsw, elev, flash, radf, swf, swalb, ch, firn,
+                             prec == ISW_MIXED, S(stream));
 }
 
 int32_t isw_surface_layer(int32_t N, int32_t L, const double* par, const isw_air* air, const double* uabs,

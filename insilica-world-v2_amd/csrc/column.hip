@@ -165,9 +165,11 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* pib, int 
 
 // The dry crossing of the isentropic interfaces by the heating dth (air.cu:744-850); wxo[j] the mass moved up
 // across interface j. Dry air: the edge-vapor and freeze-drying options move the same mass without vapor.
+// et[k], eb[k]: the heating at layer k's top and bottom edge (the reference's dtop[k] or dth[k], and dbot[k] or dth[k],
+// the top layer's bottom-edge default its dth_up): always given, so every array stays in registers.
 template <int L>
-__device__ void crossing(const CP& P, double* mu, double* th, const double* dth, double dth_up, double* pib,
-                         double* wxo) {
+__device__ void crossing(const CP& P, double* mu, double* th, const double* dth, const double* et, const double* eb,
+                         double* pib, double* wxo) {
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (P.theta_star <= 0) return;
@@ -205,8 +207,8 @@ __device__ void crossing(const CP& P, double* mu, double* th, const double* dth,
     } else if (blm && b < L - 1 && th[b] >= ts_) {
       dm = mu[b] - P.floor_;
       thc = th[b];
-    } else if (dth[b] > 0) {
-      const double eh = dth[b];
+    } else if (et[b] > 0) {
+      const double eh = et[b];
       if (th[b] >= ts_) {
         dm = mu[b] - P.floor_;
         thc = th[b];
@@ -218,7 +220,7 @@ __device__ void crossing(const CP& P, double* mu, double* th, const double* dth,
         dm = mu[b] * eh * fb;
       }
     }
-    const double da = a == 0 ? dth_up : dth[a];
+    const double da = eb[a];
     if (blm && a >= 1 && th[a] <= ts_ && dm <= 0) {
       const double down = mu[a] - P.floor_;
       thd = th[a];
@@ -308,6 +310,9 @@ __global__ void __launch_bounds__(BS) k_hs_column(int N, CP P, double ka, double
     Qs += mu[k] * pib[k] * d / dt;
     pt = pb[k];
   }
+  double eb[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) eb[k] = dth[k];
   double dth_up = dth[0];
   if (L == 2) {
     const double pi_ = pb[0], si = pi_ / ps;
@@ -317,7 +322,8 @@ __global__ void __launch_bounds__(BS) k_hs_column(int N, CP P, double ka, double
     const double ki = ka + (ks - ka) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
     dth_up = (thi - P.theta_star) * (1 - exp(-ki * dt));
   }
-  crossing<L>(P, mu, th, dth, dth_up, pib, wxo);
+  eb[0] = dth_up;
+  crossing<L>(P, mu, th, dth, dth, eb, pib, wxo);
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
   double E1 = 0;
@@ -440,6 +446,311 @@ __global__ void __launch_bounds__(BS) k_gas_src(int N, int L, int G, const doubl
 // advance_time (air.cu:1975): the device clock, advanced inside captured ticks.
 __global__ void k_advance_time(double* t, double dt) { t[0] += dt; }
 
+// ---------------------------------------------------------------------------------------------------------------
+// air_column, the dry path (air.cu:876-1972 with moist = 0): the sun, the shortwave's split, the gray longwave or
+// the k-tables' fluxes, the slab surface under the bulk or surface-layer exchange, the heating as theta, and the
+// crossing with the k-tables' edge heating. MIX selects the reference's mixed build, whose 1 - e^-x and edge Exner
+// powers are float32 (air.cu:24-36); everything else is FP64 in both forms.
+// ---------------------------------------------------------------------------------------------------------------
+
+// The air column's own parameters (PAR_ORDER positions), by value beside CP.
+struct AP {
+  double S0, asw, tau0, mu_ref, eps_s, h_sens, u0, alb, c_slab, sigma, tilt, year_s, day_s, tau_lin, Lf, Tf, sw_tau,
+      bl_bot, theta_rise, bulk_c, bulk_z, bulk_w, edge_heat, sfc, edge_k, firn, rho_firn, c_ice, ecc, lam_peri, rd;
+};
+
+static AP make_ap(const double* a, double rd) {
+  AP p;
+  p.S0 = a[0]; p.asw = a[1]; p.tau0 = a[2]; p.mu_ref = a[3]; p.eps_s = a[4]; p.h_sens = a[5]; p.u0 = a[6];
+  p.alb = a[7]; p.c_slab = a[8]; p.sigma = a[12]; p.tilt = a[17]; p.year_s = a[18]; p.day_s = a[19];
+  p.tau_lin = a[20]; p.Lf = a[24]; p.Tf = a[25]; p.sw_tau = a[34]; p.bl_bot = a[54]; p.theta_rise = a[69];
+  p.bulk_c = a[74]; p.bulk_z = a[76]; p.bulk_w = a[77]; p.edge_heat = a[84]; p.sfc = a[90]; p.edge_k = a[101];
+  p.firn = a[120]; p.rho_firn = a[122]; p.c_ice = a[123]; p.ecc = a[126]; p.lam_peri = a[127]; p.rd = rd;
+  return p;
+}
+
+template <bool MIX>
+__device__ __forceinline__ double omexp(double x) {
+  if constexpr (MIX) return (double)-expm1f(-(float)x);
+  else return 1 - exp(-(x));
+}
+template <bool MIX>
+__device__ __forceinline__ double fpow_(double x, double y) {
+  if constexpr (MIX) return (double)powf((float)x, (float)y);
+  else return pow(x, y);
+}
+
+// The gray optical depth down to pressure p of a column with surface pressure ps (air.cu:853-856).
+__device__ __forceinline__ double tau_p(double p, double ps, const AP& A) {
+  const double s = p / ps, s2 = s * s;
+  return A.tau0 * (A.tau_lin * s + (1 - A.tau_lin) * s2 * s2);
+}
+
+// The sun's ecliptic longitude at t and its distance factor (air.cu:174-185): Kepler's equation, five Newton steps.
+__device__ double sun_lon_fd(double t, const AP& A, double* fd) {
+  constexpr double PI_ = 3.14159265358979323846;
+  if (!(A.ecc > 0)) {
+    *fd = 1;
+    return 2 * PI_ * t / A.year_s;
+  }
+  const double e = A.ecc, nu0 = -A.lam_peri;
+  const double E0 = 2 * atan(sqrt((1 - e) / (1 + e)) * tan(nu0 / 2));
+  const double M = E0 - e * sin(E0) + 2 * PI_ * t / A.year_s;
+  double E = M;
+  for (int it = 0; it < 5; it++) E -= (E - e * sin(E) - M) / (1 - e * cos(E));
+  const double nu = 2 * atan2(sqrt(1 + e) * sin(E / 2), sqrt(1 - e) * cos(E / 2)), r = 1 - e * cos(E);
+  *fd = 1 / (r * r);
+  return nu + A.lam_peri;
+}
+
+struct AirOut {
+  double *wx, *olr, *sw_abs, *stored, *rain, *evap, *conv, *tsurf, *rainp, *qocean, *qsw, *flash, *swalb, *firn;
+};
+
+template <int L, bool MIX>
+__global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, const double* __restrict__ t,
+                                                   const double* __restrict__ insol, const double* __restrict__ pos,
+                                                   const double* __restrict__ uabs, const double* __restrict__ mu_in,
+                                                   double* __restrict__ theta, double* __restrict__ Ts, double dt,
+                                                   const double* __restrict__ cap, const double* __restrict__ elev,
+                                                   const double* __restrict__ radf, const double* __restrict__ swf,
+                                                   const double* __restrict__ ch, AirOut o) {
+  const int i = XTID;
+  if (i >= N) return;
+  if (elev && A.theta_rise > 0 && elev[i] > 0) P.theta_star += A.theta_rise * elev[i];
+  const int top = 0, bot = L - 1;
+  double mu[L], th[L], pb[L], pib[L], Q[L], T[L], eps[L];
+  double mt = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    mu[k] = mu_in[(size_t)k * N + i];
+    th[k] = theta[(size_t)k * N + i];
+    mt += mu[k];
+    Q[k] = 0;
+  }
+  layer_exner<L>(mu, P, pb, pib);
+  const double exb = pow(pb[bot] / P.p0, P.kappa);
+  const double m_ice = 0;
+  const double Cw = cap ? cap[i] : A.c_slab;
+  const double Cf = o.firn && A.firn > 0 ? A.c_ice * A.rho_firn * A.firn : 0.0;
+  double Tfn = Cf > 0 ? o.firn[i] : A.Tf;
+  const double E0 = column_energy<L>(mu, pib, th, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
+#pragma unroll
+  for (int k = 0; k < L; k++) T[k] = pib[k] * th[k] / P.cp;
+  // the air touching the ground: the bottom mu_bl of the column (bl_bot: the thin bottom layer's share of it); the
+  // layers above give their air at their lower edge's theta* where warmer
+  double wbl[L], thb = 0, mb = 0;
+  {
+    double need = A.bl_bot > 0 && mu[bot] < P.mu_bl ? fmax(mu[bot], A.bl_bot * P.mu_bl) : P.mu_bl;
+#pragma unroll
+    for (int k = bot; k >= 0; k--) {
+      const double m = mu[k] < need ? mu[k] : need;
+      wbl[k] = m;
+      need -= m;
+      thb += m * (A.bl_bot > 0 && k < bot && th[k] > tstar<L>(k, P) ? tstar<L>(k, P) : th[k]);
+      mb += m;
+    }
+  }
+  thb /= mb;
+  const double ps = pb[bot];
+  const double Tsa = thb * exb;
+  const bool sfc = A.sfc > 0 && ch;
+  const double rough = 1;
+  double stab = 1;
+  if (A.bulk_c > 0 && A.bulk_z > 0) {
+    const double Ub = uabs[i] + P.bulk_gust, Ri = P.grav * A.bulk_z * (Tsa - Ts[i]) / (Tsa * Ub * Ub);
+    stab = Ri > 0 ? 1 / (1 + 10 * Ri * (1 + 5 * Ri)) : 1 - 10 * Ri / (1 + 10 * sqrt(-Ri));
+  }
+  double bulk = sfc ? ch[i] : A.bulk_c > 0 ? ps / (A.rd * Tsa) * A.bulk_c * stab * (uabs[i] + P.bulk_gust) : 0.0;
+  if (!sfc && A.bulk_c > 0 && A.bulk_w > 0) {
+    const double gt = ps / (A.rd * Tsa) * A.bulk_w * (Ts[i] > Tsa ? 3.0 : 1.0);
+    bulk = bulk * gt / (bulk + gt);
+  }
+  if (sfc) bulk = bulk / (1 + bulk * dt / mb);
+  const double hs = rough * (A.bulk_c > 0 || sfc ? P.cp * bulk : A.h_sens * (1 + uabs[i] / A.u0));
+
+  // ---- shortwave ----
+  double qrad[L], qet[L], qeb[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) qrad[k] = qet[k] = qeb[k] = 0;
+  double I;
+  if (mode == 0) {
+    double fd;
+    const double ang = sun_lon_fd(t[0], A, &fd);
+    const double dec = asin(sin(A.tilt) * sin(ang));
+    const double rot = 2 * 3.14159265358979323846 * fmod(t[0], A.day_s) / A.day_s;
+    const double sx = cos(dec) * cos(-rot), sy = cos(dec) * sin(-rot), sz = sin(dec);
+    const double cz = pos[3 * i] * sx + pos[3 * i + 1] * sy + pos[3 * i + 2] * sz;
+    I = A.S0 * fd * (cz > 0 ? cz : 0.0);
+  } else {
+    I = insol[i];
+  }
+  const double alb = A.alb;
+  if (o.swalb) o.swalb[i] = alb;
+  double toAir;
+  if (swf) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++) s += swf[(size_t)k * N + i];
+    toAir = s * I;
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double f = swf[(size_t)k * N + i];
+      Q[k] += I * f;
+      qrad[k] += I * f;
+      qet[k] += I * swf[(size_t)(L + 2 + 2 * k) * N + i];
+      qeb[k] += I * swf[(size_t)(L + 3 + 2 * k) * N + i];
+    }
+  } else {
+    toAir = A.asw * (mt / A.mu_ref) * I;
+    if (A.sw_tau > 1) {
+      double st = 0;
+#pragma unroll
+      for (int k = 0; k < L; k++) {
+        const double sb = pb[k] / pb[bot], sb4 = sb * sb * sb * sb;
+        Q[k] += toAir * (sb4 - st);
+        st = sb4;
+      }
+    } else if (A.sw_tau > 0) {
+#pragma unroll
+      for (int k = 0; k < L; k++)
+        Q[k] += toAir * (tau_p(pb[k], pb[bot], A) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], A)) / A.tau0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < L; k++) Q[k] += toAir * mu[k] / mt;
+    }
+  }
+  const double Tc = 1;
+  const double swS = swf ? I * swf[(size_t)L * N + i] : (I - toAir) * Tc * (1 - alb) / (1 - alb * (1 - Tc));
+
+  // ---- longwave ----
+#pragma unroll
+  for (int k = 0; k < L; k++) eps[k] = omexp<MIX>(tau_p(pb[k], pb[bot], A) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], A));
+  double ldown = 0;
+  constexpr int RW = L + 1;
+  if (radf) {
+    ldown = radf[(size_t)(2 * RW - 1) * N + i] - radf[(size_t)L * N + i];
+  } else {
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      double dn = eps[k] * A.sigma * T[k] * T[k] * T[k] * T[k];
+#pragma unroll
+      for (int j = k + 1; j < L; j++) dn -= eps[j] * dn;
+      ldown += dn;
+    }
+  }
+  // a firn layer's heat goes to the slab (a dry column has no ice sheet to hold it)
+  if (Cf > 0 && Tfn != A.Tf) {
+    Ts[i] += Cf * (Tfn - A.Tf) / Cw;
+    Tfn = A.Tf;
+  }
+  const double ts = Ts[i];
+  const double tsk = ts;
+  double up = A.eps_s * A.sigma * tsk * tsk * tsk * tsk;
+  if (radf) {
+    const double Es = up;
+    double n0 = radf[i] - radf[(size_t)RW * N + i], tr0 = radf[(size_t)(2 * RW) * N + i];   // edge 0's net, transmission
+    const double up0 = radf[i];
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double n1e = radf[(size_t)(k + 1) * N + i] - radf[(size_t)(RW + k + 1) * N + i];
+      const double tr1 = radf[(size_t)(2 * RW + k + 1) * N + i];
+      const double ql = n1e - n0 + (tr1 - tr0) * Es;
+      Q[k] += ql;
+      const double* re = radf + (size_t)(3 * RW + 3 + 6 * k) * N + i;
+      const double s1 = re[0] - re[N] + re[2 * (size_t)N] * Es, s2 = re[3 * (size_t)N] - re[4 * (size_t)N] + re[5 * (size_t)N] * Es;
+      qrad[k] += ql;
+      qet[k] += s1 - (n0 + tr0 * Es);
+      qeb[k] += (n1e + tr1 * Es) - s2;
+      n0 = n1e;
+      tr0 = tr1;
+    }
+    up = up0 + radf[(size_t)(2 * RW) * N + i] * Es;
+  } else {
+#pragma unroll
+    for (int k = bot; k >= 0; k--) {
+      const double a = eps[k] * up;
+      Q[k] += a;
+      const double em = eps[k] * A.sigma * T[k] * T[k] * T[k] * T[k];
+      Q[k] -= 2 * em;
+      up = up - a + em;
+    }
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      double dn = eps[k] * A.sigma * T[k] * T[k] * T[k] * T[k];
+#pragma unroll
+      for (int j = k + 1; j < L; j++) {
+        const double a = eps[j] * dn;
+        Q[j] += a;
+        dn -= a;
+      }
+    }
+  }
+  o.olr[i] = up;
+  // ---- sensible heat into the bottom air; the heating as theta; the slab ----
+  const double H = hs * (tsk - Tsa);
+#pragma unroll
+  for (int k = 0; k < L; k++) Q[k] += H * wbl[k] / mb;
+  const double E = 0;
+  o.evap[i] = E;
+  o.tsurf[i] = tsk;
+  double dth[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    dth[k] = Q[k] * dt / (mu[k] * pib[k]);
+    th[k] += dth[k];
+  }
+  const double qS = swS + ldown - A.eps_s * A.sigma * tsk * tsk * tsk * tsk - H - P.Lv * E;
+  const double Tn = ts + qS * dt / Cw;
+  if (o.qocean) o.qocean[i] = qS;
+  if (o.qsw) o.qsw[i] = swS;
+  Ts[i] = Tn;
+  o.sw_abs[i] = toAir + swS;
+  // ---- the crossing: the k-tables' edge sub-levels heat the layers' edges ----
+  double et[L], eb[L], wxo[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    et[k] = dth[k];
+    eb[k] = dth[k];
+  }
+  eb[0] = dth[top];
+  if (radf) {
+    const bool edges = A.edge_heat > 0;
+    double pt_ = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double pbk = pt_ + P.grav * mu[k], dp = pbk - pt_;
+      const double ft = k == 0 ? 0.05 : 0.25, fb = k == 0 ? 0.5 : 0.25;
+      const double pit = P.cp * fpow_<MIX>((pt_ + 0.5 * ft * dp) / P.p0, P.kappa);
+      const double pibt = P.cp * fpow_<MIX>((pbk - 0.5 * fb * dp) / P.p0, P.kappa);
+      const double mean = qrad[k] * dt / (mu[k] * pib[k]);
+      const double dtop = dth[k] - mean + qet[k] * dt / (ft * mu[k] * pit);
+      const double dbot = dth[k] - mean + qeb[k] * dt / (fb * mu[k] * pibt);
+      if (edges) {
+        et[k] = dtop;
+        eb[k] = dbot;
+      } else if (k >= 1 && k <= A.edge_k) {
+        et[k] = dtop;
+      }
+      pt_ = pbk;
+    }
+  }
+  crossing<L>(P, mu, th, dth, et, eb, pib, wxo);
+  if (o.flash) o.flash[i] = 0;
+  if (o.conv) o.conv[i] = 0;
+#pragma unroll
+  for (int j = 0; j + 1 < L; j++) o.wx[(size_t)j * N + i] = wxo[j];
+  o.rain[i] = 0 / dt;
+  if (o.rainp)
+#pragma unroll
+    for (int r = 0; r < 4; r++) o.rainp[(size_t)r * N + i] = 0;
+  const double E1 = column_energy<L>(mu, pib, th, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
+  if (Cf > 0) o.firn[i] = Tfn;
+#pragma unroll
+  for (int k = 0; k < L; k++) theta[(size_t)k * N + i] = th[k];
+  o.stored[i] = E1 - E0;
+}
+
 #define ISW_BY_L(CALL)       \
   switch (L) {               \
     case 2: CALL(2); break;  \
@@ -471,6 +782,33 @@ int surface_layer(int N, int L, const double* par, const isw_air* air, const dou
   const double bytes = 8.0 * N * (1 + L + 1 + (qv ? 1 : 0) + 1 + (ice ? 1 : 0) + (land ? 1 : 0) + (lc ? 1 : 0) + 2);
   ISW_LAUNCH("surface_layer", bytes, k_surface_layer, grid1(N), dim3(BS), N, L, P, air->rd, air->vf, air->eps, uabs,
              mu, theta, qv, Ts, ice, land, cd, ch, lc);
+  return (int)hipGetLastError();
+}
+
+int air_column(int N, int L, const double* par, const isw_air* air, int mode, const double* t, const double* insol,
+               const double* pos, const double* uabs, const double* mu, double* theta, double* Ts, double dt,
+               double* wx, double* olr, double* sw_abs, double* stored, double* rain, double* evap, double* conv,
+               double* tsurf, double* rainp, const double* cap, double* qocean, double* qsw, const double* elev,
+               double* flash, const double* radf, const double* swf, double* swalb, const double* ch, double* firn,
+               int mixed, hipStream_t st) {
+  const CP P = make_cp(par);
+  const AP A = make_ap(par, air->rd);
+  const AirOut o{wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, qocean, qsw, flash, swalb, firn};
+  const int RW = L + 1;
+  const double rows_in = 1 + 2.0 * L + 1 + (mode == 0 ? 3 : 1) + (cap ? 1 : 0) + (elev ? 1 : 0) + (ch ? 1 : 0) +
+                         (radf ? 3.0 * RW + 6 * L : 0) + (swf ? 3.0 * L + 1 : 0) + (firn ? 1 : 0);
+  const double rows_out = L + 1 + (L - 1) + 6 + (rainp ? 4 : 0) + (qocean ? 1 : 0) + (qsw ? 1 : 0) + (flash ? 1 : 0) +
+                          (swalb ? 1 : 0) + (firn ? 1 : 0);
+  const double bytes = 8.0 * N * (rows_in + rows_out);
+#define CALL(LL)                                                                                                   \
+  if (mixed)                                                                                                       \
+    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, true>), grid1(N), dim3(BS), N, P, A, mode, t, insol, pos,     \
+               uabs, mu, theta, Ts, dt, cap, elev, radf, swf, ch, o);                                              \
+  else                                                                                                             \
+    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, false>), grid1(N), dim3(BS), N, P, A, mode, t, insol, pos,    \
+               uabs, mu, theta, Ts, dt, cap, elev, radf, swf, ch, o);
+  ISW_BY_L(CALL)
+#undef CALL
   return (int)hipGetLastError();
 }
 

@@ -108,6 +108,8 @@ def lib():
             'isw_cell_speed_perot': [vp, vp, vp, vp, vp],
             'isw_gas_src': [i32, i32, i32, vp, vp, d, vp, vp],
             'isw_advance_time': [vp, d, vp],
+            # N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, then 29 arrays, prec, stream
+            'isw_air_column': [i32, i32, vp, vp, i32] + [vp] * 9 + [d] + [vp] * 29 + [i32, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -123,7 +125,7 @@ EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_
            'isw_mass_update', 'isw_transport_step', 'isw_swe_step', 'isw_couple_stress', 'isw_land_column',
            'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_lw_setup', 'isw_lw_column', 'isw_sw_column',
            'isw_hs_column', 'isw_surface_layer', 'isw_cell_speed', 'isw_cell_speed_perot', 'isw_gas_src',
-           'isw_advance_time', 'isw_timing_start', 'isw_timing_stop', 'isw_version')
+           'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version')
 
 
 def check(rc, what):

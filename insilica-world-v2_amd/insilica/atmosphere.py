@@ -56,6 +56,17 @@ DEFAULTS = dict(
 assert len(PAR_ORDER) == 132 and set(PAR_ORDER) <= set(DEFAULTS)
 
 
+def annual_mean_insolation(lat, S0=1361.0, tilt=TILT, days=365):
+    """The annual mean of the daily-mean insolation (W/m^2) at latitudes lat (radians) on a circular orbit, averaged
+    over `days` days at mid-day declinations (reference: atmosphere.py:48-57): S0 / pi (h0 sin(lat) sin(dec) +
+    cos(lat) cos(dec) sin(h0)), h0 the sunset hour angle, arccos(-tan(lat) tan(dec)) clipped to [0, pi]."""
+    lat = np.asarray(lat, dtype=float)
+    dec = np.arcsin(math.sin(tilt) * np.sin(2 * math.pi * (np.arange(days) + 0.5) / days))[:, None]
+    h0 = np.arccos(np.clip(-np.tan(lat)[None, :] * np.tan(dec), -1, 1))
+    day = S0 / math.pi * (h0 * np.sin(lat)[None, :] * np.sin(dec) + np.cos(lat)[None, :] * np.cos(dec) * np.sin(h0))
+    return day.mean(0).reshape(lat.shape)
+
+
 def par_array(layers, params=None):
     """The reference's AirPhysics.par (atmosphere.py:84-90) as a host float64 array: PAR_ORDER's values, then the
     interfaces' theta* from the top (`theta_stars`, or theta_star2/theta_star), zero-padded to LMAX."""
@@ -94,8 +105,9 @@ class AirPhysics:
     coefficients), `column` (the Held-Suarez forcing with the interfaces' crossing), `radiate` (the k-tables'
     longwave and shortwave), and the device clock. Parameters in the reference's PAR_ORDER array (`par`, host).
 
-    forcing: 'held_suarez' steps the Held & Suarez column (air.cu hs_column); 'radiation' is the reference's moist
-    column (air.cu air_column), not built here: its `column` raises. longwave='ktables' builds `lw`, `sw` from
+    forcing: 'held_suarez' steps the Held & Suarez column (air.cu hs_column); 'radiation' the reference's air column
+    (air.cu air_column) for dry air (moist = 0; the moist column is not built here and raises). mean_sun: the
+    annual-mean insolation instead of the diurnal and seasonal sun. longwave='ktables' builds `lw`, `sw` from
     lw_tables, sw_tables (npz paths or dicts). `cloudo` ((4L + 3), N): per layer cover, total cover, the clouds'
     effect at the top, per layer longwave and shortwave in-cloud optical depths and liquid share."""
 
@@ -113,6 +125,13 @@ class AirPhysics:
         self.pos = torch.as_tensor(np.ascontiguousarray(dg.host.pos, dtype=np.float64), device=dev)
         self.lat = torch.as_tensor(np.arcsin(np.clip(np.asarray(dg.host.pos)[:, 2], -1, 1)), device=dev)
         self.uabs, self.wx, self.qnet, self.stored = z(N), z(max(layers - 1, 1), N), z(N), z(N)
+        self.olr, self.sw_abs, self.rain, self.evap, self.conv, self.tsurf = (z(N) for _ in range(6))
+        self.rainp, self.flash = z(4, N), z(N)
+        self.insol = None
+        if mean_sun:
+            lat = np.arcsin(np.clip(np.asarray(dg.host.pos)[:, 2], -1, 1))
+            self.insol = torch.as_tensor(annual_mean_insolation(lat, self.p['S0'], self.p['tilt']), device=dev)
+        self.moist = bool(self.p['moist'])
         self.hs = np.array([1.0 / (40 * DAY_S), 1.0 / (4 * DAY_S)])          # k_a, k_s (1/s)
         self.perot = None
         if self.p.get('sfc_wind', 0.0) > 0:
@@ -130,6 +149,7 @@ class AirPhysics:
                                 precision=dg.precision, device=dev,
                                 **({'inhom': self.p['cloud_inhom']} if 'cloud_inhom' in self.p else {}))
             self.swalb = torch.full((N,), float(self.p['alb']), dtype=torch.float64, device=dev)
+        self._mixed = 1 if dg.precision == 'mixed' else 0
 
     def _speed(self, u):
         """The surface's wind speed at cells from the bottom layer's edge winds u (L, E), into self.uabs (reference:
@@ -160,16 +180,37 @@ class AirPhysics:
                   _hip.ptr(mu), _hip.ptr(theta), _hip.ptr(q), _hip.ptr(Ts), _hip.ptr(ice), _hip.ptr(land),
                   _hip.ptr(self.cd), _hip.ptr(self.ch), _hip.ptr(lc), _hip.stream())
 
-    def column(self, mu, theta, Ts, u, t, dt, **kw):
-        """One column step (reference: atmosphere.py:134-158): with forcing 'held_suarez', the relaxation and the
-        crossing into theta (in place) and self.wx (the mass to move up, for Layers.exchange), self.qnet,
-        self.stored; then the device clock t advances by dt."""
+    def column(self, mu, theta, Ts, u, t, dt, q=None, ice=None, cap=None, qocean=None, qsw=None, land=None,
+               soilcap=None, soil=None, pond=None, snow=None, cw=None, elev=None, firn=None, gx=None):
+        """One column step (reference: atmosphere.py:134-158), then the device clock t advances by dt. With forcing
+        'held_suarez': the relaxation and the crossing into theta (in place) and self.wx (the mass to move up, for
+        Layers.exchange), self.qnet, self.stored. With forcing 'radiation', the dry column (moist = 0): the sun,
+        the gray or k-table radiation (radf/swf of the last radiate()), the slab surface at Ts (cap its heat capacity
+        per cell when given; qocean, qsw the ocean's forcing), the surface's sensible heat (the surface layer's ch
+        with sfc > 0) and the crossing, into theta, Ts, self.wx, self.olr, self.sw_abs, self.stored, self.rain,
+        self.evap, self.conv, self.tsurf, self.rainp, self.flash (the last five zero for dry air). The moist column
+        (moist > 0) is not in this library and raises."""
         from . import _hip
-        if self.forcing != 'held_suarez':
-            raise NotImplementedError("the moist air column (forcing='radiation', air.cu air_column) is not built")
-        _hip.call('isw_hs_column', self.dg.N, self.L, self._par, self.hs.ctypes.data_as(ctypes.c_void_p),
-                  _hip.ptr(self.lat), _hip.ptr(mu), _hip.ptr(theta), float(dt), _hip.ptr(self.wx),
-                  _hip.ptr(self.qnet), _hip.ptr(self.stored), _hip.stream())
+        if self.forcing == 'held_suarez':
+            _hip.call('isw_hs_column', self.dg.N, self.L, self._par, self.hs.ctypes.data_as(ctypes.c_void_p),
+                      _hip.ptr(self.lat), _hip.ptr(mu), _hip.ptr(theta), float(dt), _hip.ptr(self.wx),
+                      _hip.ptr(self.qnet), _hip.ptr(self.stored), _hip.stream())
+            self.advance_time(t, dt)
+            return
+        if self.moist:
+            raise NotImplementedError('the moist air column (moist > 0) is not in this library')
+        if gx is not None:
+            raise NotImplementedError('the gas tracers carried by the column\'s crossing are not in this library')
+        self._speed(u)
+        P = _hip.ptr
+        lw, sw = self.lw, self.sw
+        _hip.call('isw_air_column', self.dg.N, self.L, self._par, ctypes.byref(self.air), self.mode, P(t),
+                  P(self.insol), P(self.pos), P(self.uabs), P(mu), P(theta), None, P(Ts), None, float(dt),
+                  P(self.wx), P(self.olr), P(self.sw_abs), P(self.stored), P(self.rain), P(self.evap), P(self.conv),
+                  P(self.tsurf), P(self.rainp), None, P(cap), P(qocean), P(qsw), None, None, None, None, None, None,
+                  None, P(elev), P(self.flash), P(lw.radf) if lw is not None else None,
+                  P(sw.swf) if sw is not None else None, P(self.swalb) if sw is not None else None,
+                  P(self.ch) if self.sfc else None, None, P(firn), None, self._mixed, _hip.stream())
         self.advance_time(t, dt)
 
     @staticmethod

@@ -149,6 +149,13 @@ def aqua_air(g):
     return h, np.zeros(g.N), np.zeros((3, g.E)), dict(theta=theta, Ts=Ts, q=q)
 
 
+def aqua_dry(g):
+    """The dry aquaplanet's start (reference: cases.py:244-247): the three air layers of aqua_air at rest, the surface
+    at 288 - 40 (sin^2(lat) - 1/3) K, no vapor. Returns h (3, N), b, u (3, E), dict(theta, Ts)."""
+    h, b, u, x = aqua_air(g)
+    return h, b, u, dict(theta=x['theta'], Ts=288.0 - 40.0 * (np.sin(g.lat) ** 2 - 1.0 / 3.0))
+
+
 def harmonic_fields(rng, pos, count, lmax=20):
     """`count` random-phase fields of spherical-harmonic degree <= lmax at unit positions pos, each scaled to a
     maximum of 1: sum over l, m of a_lm P_l^m(sin lat) cos(m lon + phase_lm), Gaussian a_lm ~ 1/(l+1), phases uniform.

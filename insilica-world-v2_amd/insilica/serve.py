@@ -64,8 +64,9 @@ class Server:
             self._bytes_per_cell = 1.1 * (torch.cuda.memory_allocated() - before) / w.grid.N
         return w
 
-    def run(self, requests, clock=time.perf_counter):
-        """Serve `requests` (arrival times relative to now); returns them with their times filled in, and stats."""
+    def run(self, requests, clock=time.perf_counter, log=None):
+        """Serve `requests` (arrival times relative to now); returns them with their times filled in, and stats.
+        log(text), when given, is called as each request is admitted and as it finishes."""
         pending = sorted(requests, key=lambda r: r.arrival)
         active, used, hours_done = [], 0, 0.0
         t0 = clock()
@@ -83,6 +84,9 @@ class Server:
                 active.append(dict(r=r, w=w, need=need, left=int(round(r.days * 24 * w.ticks_per_hour)),
                                    per_hour=int(round(w.ticks_per_hour)), marks=[]))
                 used += need
+                if log:
+                    log(f'{r.admitted:8.2f} s  admitted request {r.rid} (level {r.level}, {r.days} days), '
+                        f'{len(active)} active')
             if not active:
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
                 continue
@@ -108,6 +112,9 @@ class Server:
                         a['w'].sync()
                         r.hash = a['w'].state_hash()
                     used -= a['need']
+                    if log:
+                        log(f'{r.finished:8.2f} s  finished request {r.rid} '
+                            f'({(r.finished - r.arrival) / r.days:.3f} s per model day)')
                 else:
                     still.append(a)
             active = still

@@ -31,7 +31,7 @@ from .transport import BETA, Transport
 
 THIN_WIND = 100.0
 GAS_TRACERS = (('co2', 400e-6), ('n2o', 0.32e-6), ('ch4', 1.8e-6))
-PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez')
+PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez', 'aqua_dry')
 
 
 class World:
@@ -45,9 +45,42 @@ class World:
         self.grid = gridmod.for_planet(gridmod.get(n), **self.planet)
         self.dg = dg = DeviceGrid(self.grid, precision)
         g = self.grid
-        self.air = preset in ('coupled_stack', 'held_suarez')
+        self.air = preset in ('coupled_stack', 'held_suarez', 'aqua_dry')
         self.sea = self.physics = None
-        if preset == 'held_suarez':
+        self.Ts = None
+        self.rad_every = 0
+        if preset == 'aqua_dry':
+            # three dry air layers at rest over a slab aquaplanet under the diurnal and seasonal sun, gray radiation
+            # (or the k-tables with longwave='ktables'), the bottom layer dragged by the surface (reference preset
+            # 'aqua_dry', cases.py:244-247, 277): the column step then the exchange of the mass it moves
+            from .atmosphere import AirPhysics
+            h, b, u, extra = cases.aqua_dry(g)
+            h = h * (GRAV / self.grav)
+            self.L = 3
+            self.dt = float(P.get('dt', 900.0 * 2.0 ** (5 - n)))
+            nu4 = P.get('nu4', visc4(g, P.get('visc4_U', 15.0)))
+            floor = P.get('floor', 10.0)
+            self.core = Layers(dg, 3, b=b, grav=self.grav, gprime=[0.0, 0.0], drag=P.get('drag', 0.0),
+                               stress=P.get('stress', [0.0, 0.0, cases.SURFACE_DRAG]), nu4=nu4, floor=floor,
+                               rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP),
+                               air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY), vmix=P.get('vmix', 0.0),
+                               thin_wind=P.get('thin_wind', THIN_WIND))
+            self.rows = dg.asarray(extra['theta'])
+            self.theta = self.rows
+            self.Ts = dg.asarray(extra['Ts'])
+            self.psi = dg.asarray(np.tile(cases.cosine_bell(g), (3, 1)))      # moved by the exchange only
+            self.tr = Transport(dg, layers=3, beta=P.get('beta', BETA), tracers=1)
+            phys = dict(theta_star=305.0, theta_star2=380.0, floor_=floor, grav=self.grav)
+            if self.grav != GRAV:
+                phys.update(mu_ref=1.0e5 / self.grav, mu_bl=1.0e4 / self.grav)
+            phys.update(P.get('physics', {}))
+            longwave = P.get('longwave', 'gray')
+            self.physics = AirPhysics(dg, 3, phys, mean_sun=P.get('mean_sun', False), forcing='radiation',
+                                      longwave=longwave, lw_tables=P.get('lw_tables'), sw_tables=P.get('sw_tables'))
+            if self.physics.lw is not None:
+                self.rad_every = max(1, int(round(P.get('rad_dt', 3600.0) / self.dt)))
+            self.time = dg.zeros(1)
+        elif preset == 'held_suarez':
             # two air layers forced by Held & Suarez's relaxation with the interface crossing (reference preset
             # 'held_suarez', cases.py:680-682): the column step then the exchange of the mass it moves
             from .atmosphere import AirPhysics
@@ -137,13 +170,16 @@ class World:
         self._stream.wait_stream(torch.cuda.current_stream())
         self.STATE = ('h', 'u', 'M', 'rows') + (('sea.h', 'sea.u', 'sea.M', 'sea.rows', 'sea.Tg')
                                                  if self.sea is not None else ()) + \
-            (('psi', 'time') if self.physics is not None else ())
+            (('psi', 'time') if self.physics is not None else ()) + (('Ts',) if self.Ts is not None else ())
 
     # ---- the tick ----
     def _kind(self, tick):
-        if self.sea is None or (tick + 1) % self.ocean_every:
-            return 0
-        return 1
+        """The tick's kind: bit 0 the ocean's step, bit 1 the k-tables' radiation (every rad_every ticks from the
+        first; reference: world.py:283-284)."""
+        k = 0 if self.sea is None or (tick + 1) % self.ocean_every else 1
+        if self.rad_every and tick % self.rad_every == 0:
+            k |= 2
+        return k
 
     def _tick(self, kind=0):
         """One tick's work on fixed buffers (nothing is allocated), so the sequence can be captured as a graph."""
@@ -156,12 +192,15 @@ class World:
             self.sea.couple(self.h, self.u, self.surface_drag, self.dt)
             self.core.vmix_heat(self.h, self.sea.diss, self.theta)
             self.sea.ground(self.dt)
-            if kind:
+            if kind & 1:
                 self.sea.step(mix=False)
         if self.physics is not None:
-            # the column (Held & Suarez's relaxation, the crossing into wx, the clock), then the exchange of that mass
-            # with its theta, momentum and tracer between the layers (reference: world.py:319-344)
-            self.physics.column(self.h, self.theta, None, self.u, self.time, self.dt)
+            # the radiation step (the k-tables, saved for the column steps until the next), the column (the
+            # radiation or Held & Suarez's relaxation, the crossing into wx, the clock), then the exchange of that
+            # mass with its theta, momentum and tracer between the layers (reference: world.py:319-344)
+            if kind & 2:
+                self.physics.radiate(self.h, self.theta, self.Ts, self.time, self.rad_every * self.dt)
+            self.physics.column(self.h, self.theta, self.Ts, self.u, self.time, self.dt, elev=self.b)
             self.core.exchange(self.physics.wx, self.h, self.u, self.M, self.psi, theta=self.theta)
 
     def launch_tick(self, graph=True):
@@ -217,7 +256,12 @@ class World:
         return out
 
     def energy(self):
-        return self.core.energy(self.h, self.u, self.theta if self.air else None)
+        """Total energy (J): kinetic + potential (+ for air the enthalpy; over a slab its heat c_slab Ts; reference:
+        world.py:424-441)."""
+        e = self.core.energy(self.h, self.u, self.theta if self.air else None)
+        if self.Ts is not None and self.sea is None:
+            e += self.physics.p['c_slab'] * csum(self.dg, self.Ts, self.dg.areaCell)
+        return e
 
     def state_arrays(self):
         out = {}
@@ -247,6 +291,8 @@ class World:
             return self.theta[layer]
         if name == 'q' and self.air:
             return self.q[layer] * 1000.0
+        if name == 'Ts' and self.Ts is not None:
+            return self.Ts
         if name == 'ps' and self.air:
             return self.grav * torch.sum(self.h, dim=0) / 100.0
         if name == 'sst' and self.sea is not None:

@@ -119,9 +119,10 @@ static void move_mass(int L, const double* par, double* mu, double* th, double* 
 
 /* The dry crossing of every interface theta*_j by the heating dth (air.cu:744-850): wxo[j] the mass moved up across
  * interface j. Dry air (q = 0): the edge-vapor and freeze-drying options carry no vapor and move the same mass. The
- * mixed bottom layer (bl_mix) needs L >= 3 (its top edge is then an interface). */
+ * mixed bottom layer (bl_mix) needs L >= 3 (its top edge is then an interface). dtop/dbot (optional): the heating at
+ * each layer's top and bottom edge when it differs from the layer's mean (the k-tables' edge sub-levels). */
 static void crossing(int L, const double* par, double* mu, double* th, double* q, const double* dth, double dth_up,
-                     double* pib, double* wxo) {
+                     double* pib, double* wxo, const double* dtop, const double* dbot) {
   const double floor_ = par[P_FLOOR];
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (par[P_THETA_STAR] <= 0) return;
@@ -151,8 +152,8 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
     } else if (blm && b < L - 1 && th[b] >= ts_) {
       dm = mu[b] - floor_;
       thc = th[b];
-    } else if (dth[b] > 0) {
-      const double eh = dth[b];
+    } else if ((dtop ? dtop[b] : dth[b]) > 0) {
+      const double eh = dtop ? dtop[b] : dth[b];     /* the heating at b's top edge */
       if (th[b] >= ts_) {
         dm = mu[b] - floor_;
         thc = th[b];
@@ -164,7 +165,7 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
         dm = mu[b] * eh * fb;
       }
     }
-    const double da = a == 0 ? dth_up : dth[a];
+    const double da = dbot ? dbot[a] : (a == 0 ? dth_up : dth[a]);
     if (blm && a >= 1 && th[a] <= ts_ && dm <= 0) {
       const double down = mu[a] - floor_;
       thd = th[a];
@@ -256,7 +257,7 @@ void og_hs_column(int N, int L, const double* par, const double* hs, const doubl
       const double ki = hs[0] + (hs[1] - hs[0]) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
       dth_up = (thi - par[P_THETA_STAR]) * (1 - exp(-ki * dt));
     }
-    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo);
+    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL);
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
     double E1 = 0;
     for (int k = 0; k < L; k++) {
@@ -360,4 +361,242 @@ void og_cell_speed_perot(int N, const double* u, const int32_t* cellEdge, const 
 void og_gas_src(int N, int L, int G, const double* src, const double* h, double dt, double* x) {
   for (int g = 0; g < G; g++)
     for (int i = 0; i < N; i++) x[((size_t)g * L + L - 1) * N + i] += src[(size_t)g * N + i] * dt / h[(size_t)(L - 1) * N + i];
+}
+
+/* ---- air_column, the dry path (air.cu:876-1972 with moist = 0) ----
+ * The sun (mode 0: the diurnal and seasonal sun at t; mode 1: insol), the shortwave split (the k-tables' swf, or by
+ * the s^4 or tau profile, or by mass), the gray longwave (or the k-tables' radf), the slab surface under the bulk or
+ * surface-layer exchange, the heating as theta, and the crossing with the k-tables' edge heating. `mix` selects the
+ * reference's mixed build: 1 - e^-x and the edge sub-levels' Exner powers in float32 (air.cu:24-36). */
+enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5, P_U0 = 6, P_ALB = 7, P_C_SLAB = 8,
+       P_SIGMA = 12, P_TILT = 17, P_DAY = 19, P_TAU_LIN = 20, P_LF = 24, P_TF = 25, P_SW_TAU = 34, P_BL_BOT = 54,
+       P_THETA_RISE = 69, P_BULK_C = 74, P_BULK_Z = 76, P_BULK_W = 77, P_EDGE_HEAT = 84, P_SFC = 90, P_EDGE_K = 101,
+       P_FIRN = 120, P_RHO_FIRN = 122, P_C_ICE = 123 };
+
+static double omexp(double x, int mix) { return mix ? (double)-expm1f(-(float)x) : 1 - exp(-(x)); }
+static double fpow_(double x, double y, int mix) { return mix ? (double)powf((float)x, (float)y) : pow(x, y); }
+
+/* tau_p (air.cu:853-856): the gray optical depth down to pressure p of a column with surface pressure ps. */
+static double tau_p(double p, double ps, const double* par) {
+  const double s = p / ps, s2 = s * s;
+  return par[P_TAU0] * (par[P_TAU_LIN] * s + (1 - par[P_TAU_LIN]) * s2 * s2);
+}
+
+void og_air_column(int N, int L, const double* par_in, const og_air* air, int mode, const double* t,
+                   const double* insol, const double* pos, const double* uabs, const double* mu_in, double* theta,
+                   double* Ts, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
+                   double* evap, double* conv, double* tsurf, double* rainp, const double* cap, double* qocean,
+                   double* qsw, const double* elev, double* flash, const double* radf, const double* swf,
+                   double* swalb, const double* ch, double* firn, int mix) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double par[132 + OG_LMAX];
+    memcpy(par, par_in, sizeof(par));
+    if (elev && par[P_THETA_RISE] > 0 && elev[i] > 0) par[P_THETA_STAR] += par[P_THETA_RISE] * elev[i];
+    const double grav = par[P_GRAV], cp = par[P_CP], kappa = par[P_KAPPA], p0 = par[P_P0], sigma = par[P_SIGMA];
+    const double eps_s = par[P_EPS_S], Tf = par[P_TF], Lf = par[P_LF];
+    double mu[OG_LMAX], th[OG_LMAX], q[OG_LMAX], pb[OG_LMAX], pib[OG_LMAX], Q[OG_LMAX], T[OG_LMAX], eps[OG_LMAX];
+    double mt = 0;
+    for (int k = 0; k < L; k++) {
+      mu[k] = mu_in[(size_t)k * N + i];
+      th[k] = theta[(size_t)k * N + i];
+      q[k] = 0;
+      mt += mu[k];
+      Q[k] = 0;
+    }
+    layer_exner(L, mu, par, pb, pib);
+    const int top = 0, bot = L - 1;
+    const double exb = pow(pb[bot] / p0, kappa);     /* (p_bot / p0)^kappa at the column's lower edge */
+    const double m_ice = 0;
+    const double Cw = cap ? cap[i] : par[P_C_SLAB];
+    const double Cf = firn && par[P_FIRN] > 0 ? par[P_C_ICE] * par[P_RHO_FIRN] * par[P_FIRN] : 0.0;
+    double Tfn = Cf > 0 ? firn[i] : Tf;
+    const double E0 = column_energy(L, mu, pib, th, q, par) + Cw * Ts[i] - Lf * m_ice + Cf * (Tfn - Tf);
+    for (int k = 0; k < L; k++) T[k] = pib[k] * th[k] / cp;
+    /* the air touching the ground: the bottom mu_bl of the column (or bl_bot's share of it, the thin bottom layer's
+     * own mass at least); the layers above give their air at their lower edge's theta* where warmer */
+    double wbl[OG_LMAX], thb = 0, mb = 0;
+    {
+      const double mu_bl = par[P_MU_BL], bl_bot = par[P_BL_BOT];
+      double need = bl_bot > 0 && mu[bot] < mu_bl ? fmax(mu[bot], bl_bot * mu_bl) : mu_bl;
+      for (int k = bot; k >= 0; k--) {
+        const double m = mu[k] < need ? mu[k] : need;
+        wbl[k] = m;
+        need -= m;
+        thb += m * (bl_bot > 0 && k < bot && th[k] > tstar(k, L, par) ? tstar(k, L, par) : th[k]);
+        mb += m;
+      }
+    }
+    thb /= mb;
+    const double ps = pb[bot];
+    const double Tsa = thb * exb;
+    const int sfc = par[P_SFC] > 0 && ch;
+    const double rough = 1;
+    const double bulk_c = par[P_BULK_C], gust = par[P_BULK_GUST], rd = air->rd;
+    double stab = 1;
+    if (bulk_c > 0 && par[P_BULK_Z] > 0) {
+      const double Ub = uabs[i] + gust, Ri = grav * par[P_BULK_Z] * (Tsa - Ts[i]) / (Tsa * Ub * Ub);
+      stab = Ri > 0 ? 1 / (1 + 10 * Ri * (1 + 5 * Ri)) : 1 - 10 * Ri / (1 + 10 * sqrt(-Ri));
+    }
+    double bulk = sfc ? ch[i] : bulk_c > 0 ? ps / (rd * Tsa) * bulk_c * stab * (uabs[i] + gust) : 0.0;
+    if (!sfc && bulk_c > 0 && par[P_BULK_W] > 0) {
+      const double gt = ps / (rd * Tsa) * par[P_BULK_W] * (Ts[i] > Tsa ? 3.0 : 1.0);
+      bulk = bulk * gt / (bulk + gt);
+    }
+    if (sfc) bulk = bulk / (1 + bulk * dt / mb);
+    const double hs = rough * (bulk_c > 0 || sfc ? cp * bulk : par[P_H_SENS] * (1 + uabs[i] / par[P_U0]));
+
+    /* shortwave */
+    double qrad[OG_LMAX], qet[OG_LMAX], qeb[OG_LMAX];
+    for (int k = 0; k < L; k++) qrad[k] = qet[k] = qeb[k] = 0;
+    double I;
+    if (mode == 0) {
+      double fd;
+      const double ang = og_sun_lon(t[0], par, &fd);
+      const double dec = asin(sin(par[P_TILT]) * sin(ang));
+      const double rot = 2 * 3.14159265358979323846 * fmod(t[0], par[P_DAY]) / par[P_DAY];
+      const double sx = cos(dec) * cos(-rot), sy = cos(dec) * sin(-rot), sz = sin(dec);
+      const double cz = pos[3 * i] * sx + pos[3 * i + 1] * sy + pos[3 * i + 2] * sz;
+      I = par[P_S0] * fd * (cz > 0 ? cz : 0.0);
+    } else {
+      I = insol[i];
+    }
+    const double alb = par[P_ALB];                    /* (dry: no ice, no land) */
+    if (swalb) swalb[i] = alb;
+    double toAir;
+    if (swf) {
+      double s = 0;
+      for (int k = 0; k < L; k++) s += swf[(size_t)k * N + i];
+      toAir = s * I;
+    } else {
+      toAir = par[P_ASW] * (mt / par[P_MU_REF]) * I;
+    }
+    if (swf) {
+      for (int k = 0; k < L; k++) {
+        Q[k] += I * swf[(size_t)k * N + i];
+        qrad[k] += I * swf[(size_t)k * N + i];
+        qet[k] += I * swf[(size_t)(L + 2 + 2 * k) * N + i];
+        qeb[k] += I * swf[(size_t)(L + 3 + 2 * k) * N + i];
+      }
+    } else if (par[P_SW_TAU] > 1) {
+      double st = 0;
+      for (int k = 0; k < L; k++) {
+        const double sb = pb[k] / pb[bot], sb4 = sb * sb * sb * sb;
+        Q[k] += toAir * (sb4 - st);
+        st = sb4;
+      }
+    } else if (par[P_SW_TAU] > 0) {
+      for (int k = 0; k < L; k++)
+        Q[k] += toAir * (tau_p(pb[k], pb[bot], par) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], par)) / par[P_TAU0];
+    } else {
+      for (int k = 0; k < L; k++) Q[k] += toAir * mu[k] / mt;
+    }
+    const double Tc = 1;
+    const double swS = swf ? I * swf[(size_t)L * N + i] : (I - toAir) * Tc * (1 - alb) / (1 - alb * (1 - Tc));
+
+    /* gray longwave: each layer's emissivity from its share of the optical depth */
+    for (int k = 0; k < L; k++)
+      eps[k] = omexp(tau_p(pb[k], pb[bot], par) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], par), mix);
+    double ldown = 0;
+    const int RW = L + 1;
+    if (radf) {
+      ldown = radf[(size_t)(2 * RW - 1) * N + i] - radf[(size_t)L * N + i];
+    } else {
+      for (int k = 0; k < L; k++) {
+        double dn = eps[k] * sigma * T[k] * T[k] * T[k] * T[k];
+        for (int j = k + 1; j < L; j++) dn -= eps[j] * dn;
+        ldown += dn;
+      }
+    }
+    /* the slab (a firn layer's heat, with no ice sheet under a dry column, goes to the slab) */
+    if (Cf > 0 && Tfn != Tf) {
+      Ts[i] += Cf * (Tfn - Tf) / Cw;
+      Tfn = Tf;
+    }
+    const double ts = Ts[i];
+    const double tsk = ts;
+    double up = eps_s * sigma * tsk * tsk * tsk * tsk;
+    if (radf) {
+      const double Es = up;
+#define RF(r, k) radf[(size_t)((r) * RW + (k)) * N + i]
+      for (int k = 0; k < L; k++) {
+        const double ql = (RF(0, k + 1) - RF(1, k + 1)) - (RF(0, k) - RF(1, k)) + (RF(2, k + 1) - RF(2, k)) * Es;
+        Q[k] += ql;
+#define RE(r) radf[(size_t)(3 * RW + 3 + 6 * k + (r)) * N + i]
+        const double n1 = RE(0) - RE(1) + RE(2) * Es, n2 = RE(3) - RE(4) + RE(5) * Es;
+#undef RE
+        qrad[k] += ql;
+        qet[k] += n1 - ((RF(0, k) - RF(1, k)) + RF(2, k) * Es);
+        qeb[k] += ((RF(0, k + 1) - RF(1, k + 1)) + RF(2, k + 1) * Es) - n2;
+      }
+      up = RF(0, 0) + RF(2, 0) * Es;
+#undef RF
+    } else {
+      for (int k = bot; k >= 0; k--) {
+        const double a = eps[k] * up;
+        Q[k] += a;
+        const double em = eps[k] * sigma * T[k] * T[k] * T[k] * T[k];
+        Q[k] -= 2 * em;
+        up = up - a + em;
+      }
+      for (int k = 0; k < L; k++) {
+        double dn = eps[k] * sigma * T[k] * T[k] * T[k] * T[k];
+        for (int j = k + 1; j < L; j++) {
+          const double a = eps[j] * dn;
+          Q[j] += a;
+          dn -= a;
+        }
+      }
+    }
+    olr[i] = up;
+    /* sensible heat into the bottom air */
+    const double H = hs * (tsk - Tsa);
+    for (int k = 0; k < L; k++) Q[k] += H * wbl[k] / mb;
+    const double E = 0;
+    evap[i] = E;
+    tsurf[i] = tsk;
+    double dth[OG_LMAX];
+    for (int k = 0; k < L; k++) {
+      dth[k] = Q[k] * dt / (mu[k] * pib[k]);
+      th[k] += dth[k];
+    }
+    const double qS = swS + ldown - eps_s * sigma * tsk * tsk * tsk * tsk - H - par[P_LV] * E;
+    const double Tn = ts + qS * dt / Cw;
+    if (qocean) qocean[i] = qS;
+    if (qsw) qsw[i] = swS;
+    Ts[i] = Tn;
+    sw_abs[i] = toAir + swS;
+    /* the crossing, with the k-tables' edge sub-levels' own heating at the layers' edges */
+    double dtop[OG_LMAX], dbot[OG_LMAX], dtop1[OG_LMAX], wxo[OG_LMAX];
+    if (radf) {
+      double pt_ = 0;
+      for (int k = 0; k < L; k++) {
+        const double pbk = pt_ + grav * mu[k], dp = pbk - pt_;
+        const double ft = k == 0 ? 0.05 : 0.25, fb = k == 0 ? 0.5 : 0.25;
+        const double pit = cp * fpow_((pt_ + 0.5 * ft * dp) / p0, kappa, mix);
+        const double pibt = cp * fpow_((pbk - 0.5 * fb * dp) / p0, kappa, mix);
+        const double mean = qrad[k] * dt / (mu[k] * pib[k]);
+        dtop[k] = dth[k] - mean + qet[k] * dt / (ft * mu[k] * pit);
+        dbot[k] = dth[k] - mean + qeb[k] * dt / (fb * mu[k] * pibt);
+        pt_ = pbk;
+      }
+    }
+    const int edges = par[P_EDGE_HEAT] > 0;
+    if (radf && !edges && L > 1) {
+      for (int k = 0; k < L; k++) dtop1[k] = dth[k];
+      for (int k = 1; k <= par[P_EDGE_K] && k < L; k++) dtop1[k] = dtop[k];
+    }
+    crossing(L, par, mu, th, q, dth, dth[top], pib, wxo,
+             radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL);
+    if (flash) flash[i] = 0;
+    if (conv) conv[i] = 0;
+    for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
+    rain[i] = 0 / dt;
+    if (rainp)
+      for (int r = 0; r < 4; r++) rainp[(size_t)r * N + i] = 0;
+    const double E1 = column_energy(L, mu, pib, th, q, par) + Cw * Ts[i] - Lf * m_ice + Cf * (Tfn - Tf);
+    if (Cf > 0) firn[i] = Tfn;
+    for (int k = 0; k < L; k++) theta[(size_t)k * N + i] = th[k];
+    stored[i] = E1 - E0;
+  }
 }

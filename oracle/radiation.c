@@ -48,7 +48,7 @@ static double qsat_a(double T, double p, double ice, double eps) {
 }
 
 /* The sun's longitude at time t and its distance factor (air.cu:174-185): Kepler's equation by five Newton steps. */
-static double sun_lon(double t, const double* par, double* fd) {
+double og_sun_lon(double t, const double* par, double* fd) {
   const double ecc = par[P_ECC], year = par[P_YEAR];
   if (!(ecc > 0)) {
     *fd = 1;
@@ -410,7 +410,7 @@ void og_sw_column(int N, int L, int nsub, const double* par, const og_swt* t, co
         for (int j = 0; j < 4; j++) {
           const double tt = tnow + (j + 0.5) * interval / 4;
           double fd;
-          const double ang = sun_lon(tt, par, &fd), dec = asin(sin(tilt) * sin(ang));
+          const double ang = og_sun_lon(tt, par, &fd), dec = asin(sin(tilt) * sin(ang));
           const double rot = 2 * PI_ * fmod(tt, day_s) / day_s;
           const double cz = pos[3 * i] * cos(dec) * cos(-rot) + pos[3 * i + 1] * cos(dec) * sin(-rot) +
                             pos[3 * i + 2] * sin(dec);

@@ -2,7 +2,7 @@
 
 Restates the tick order of the reference's World._tick (src/insilica/world.py:286-358) for the parts this build
 covers (the layered core, FCT transport, the air-sea stress with its heat, the land column, the hourly ocean step
-without its column mixing) and the Ocean.step order (src/insilica/ocean.py:238-252), on the CPU oracle. Built from a
+without its column mixing, the Held-Suarez and the dry gray air column) and the Ocean.step order (src/insilica/ocean.py:238-252), on the CPU oracle. Built from a
 product World's settings and starting state, so that the two can be stepped side by side.
 """
 import numpy as np
@@ -34,9 +34,17 @@ class OracleWorld:
         self.Fm, self.M1 = np.zeros((self.L, og.E)), np.zeros((self.L, og.N))
         self.sea = None
         self.physics = None
-        if getattr(w, 'physics', None) is not None:                 # the Held-Suarez column
+        self.Ts = None
+        if getattr(w, 'physics', None) is not None:                 # the Held-Suarez or the dry radiative column
             p = w.physics
-            self.physics = dict(par=np.array(p.par), hs=np.array(p.hs), lat=_np(p.lat))
+            if p.forcing == 'held_suarez':
+                self.physics = dict(kind='hs', par=np.array(p.par), hs=np.array(p.hs), lat=_np(p.lat))
+            else:
+                if p.lw is not None or p.perot is not None or p.sfc:
+                    raise NotImplementedError('the oracle world steps the gray dry column with the edge-wind speed')
+                self.physics = dict(kind='air', par=np.array(p.par), mode=p.mode, pos=_np(p.pos), grid=w.grid,
+                                    insol=_np(p.insol) if p.insol is not None else None, b=_np(w.b))
+                self.Ts = _np(w.Ts)
             self.psi, self.time = _np(w.psi), float(w.time[0])
         if w.sea is not None:
             s = w.sea
@@ -91,9 +99,15 @@ class OracleWorld:
                 s['M'][...] = s['M1']
                 ocn_cap(og, s['h'], self.rho * self.cw, s['cap'], s['land'], self.c_land)
         if self.physics is not None:
-            from .column import hs_column
+            from .column import air_column, cell_speed, hs_column
             ph = self.physics
-            th, wx, _, _ = hs_column(ph['par'], ph['hs'], ph['lat'], self.h, self.rows[0:L], self.dt)
+            if ph['kind'] == 'hs':
+                th, wx, _, _ = hs_column(ph['par'], ph['hs'], ph['lat'], self.h, self.rows[0:L], self.dt)
+            else:
+                uabs = cell_speed(ph['grid'], self.u[L - 1])
+                o = air_column(ph['par'], ph['mode'], np.array([self.time]), ph['pos'], uabs, self.h, self.rows[0:L],
+                               self.Ts, self.dt, insol=ph['insol'], elev=ph['b'])
+                th, wx, self.Ts = o['theta'], o['wx'], o['Ts']
             self.rows[0:L] = th
             self.time += self.dt
             self.core.exchange(wx[:L - 1], self.h, self.u, self.M, self.psi, theta=self.rows[0:L])
@@ -102,6 +116,8 @@ class OracleWorld:
         out = dict(h=self.h, u=self.u, M=self.M, rows=self.rows)
         if self.physics is not None:
             out.update(psi=self.psi, time=np.array([self.time]))
+        if self.Ts is not None:
+            out['Ts'] = self.Ts
         if self.sea is not None:
             for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap'):
                 if self.sea[k] is not None:

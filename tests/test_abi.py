@@ -71,6 +71,21 @@ def test_bad_arguments_rejected_on_host():
     assert lib.isw_gas_src(10, 3, -1, None, None, 1.0, None, None) == E
     assert lib.isw_gas_src(10, 3, 0, None, None, 1.0, None, None) == 0          # no gases: nothing to do
     assert lib.isw_advance_time(None, 1.0, None) == E
+    # the air column: every option outside the dry column, a missing sun, an unknown form, bad sizes
+    par[70] = 0.0
+
+    def air_column(L=3, mode=0, t=buf, insol=None, dgo=None, xfer=None, prec=0, dt=600.0, a=ctypes.byref(air)):
+        # N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt; wx ... tsurf; rainp, dgo; cap ...
+        # cloudo; elev, flash, radf, swf, swalb, ch, lc, firn; xfer, prec, stream
+        return lib.isw_air_column(10, L, par, a, mode, t, insol, buf, buf, buf, buf, None, buf, None, dt,
+                                  *([buf] * 8), None, dgo, *([None] * 10), *([None] * 8), xfer, prec, None)
+    for k, v in ((22, 1.0), (70, 1.0), (111, 1.0)):          # moist, bl_mix, cu_dth (with L > 3)
+        par[k] = v
+        assert air_column(L=4) == E
+        par[k] = 0.0
+    assert air_column(dgo=buf) == E and air_column(xfer=buf) == E and air_column(a=None) == E
+    assert air_column(t=None) == E and air_column(mode=1) == E and air_column(mode=2, insol=buf) == E
+    assert air_column(prec=5) == E and air_column(L=1) == E and air_column(L=9) == E and air_column(dt=0.0) == E
 
 
 def test_request_draws_are_seeded():

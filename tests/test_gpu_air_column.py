@@ -1,0 +1,189 @@
+"""The dry air column on the GPU against the CPU oracle, through the C ABI (isw_air_column).
+
+Bars: FP64 <= 1e-12 relative (max |diff| / max |oracle|) on theta, Ts, the crossing's masses, olr, sw_abs and the
+surface outputs (the device's pow/exp/sin/cos/asin/fmod against glibc's are the only difference); the stored energy,
+a difference of two ~1e11 J/m^2 totals, to 1e-9 and to its closure sw_abs - olr within 2e-3 J/m^2. Mixed form
+(float32 1 - e^-x and edge Exner powers, the reference's mixed build) against the oracle's mixed build: <= 1e-6.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import relmax
+
+pytestmark = pytest.mark.gpu
+
+GRAV = 9.80616
+
+
+def _dev(a):
+    return None if a is None else torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device='cuda')
+
+
+def _np(t):
+    return t.detach().cpu().numpy().copy()
+
+
+def par(L, **kw):
+    from insilica.atmosphere import par_array
+    stars = list(np.linspace(380.0, 305.0, L - 1)) if L > 2 else [305.0]
+    return par_array(L, dict(kw, theta_stars=stars))
+
+
+def columns(N, L, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(size=(N, 3))
+    pos /= np.linalg.norm(pos, axis=1)[:, None]
+    share = np.concatenate([[0.0], np.linspace(0.2, 1.0, L)])
+    edges = share[:, None] * 1.0e5 * rng.uniform(0.97, 1.03, (1, N))
+    mu = np.diff(edges, axis=0) / GRAV
+    th = np.linspace(420.0, 285.0, L)[:, None] + rng.normal(0, 2.0, (L, N))
+    Ts = 288.0 - 40.0 * (pos[:, 2] ** 2 - 1.0 / 3.0) + rng.normal(0, 2.0, N)
+    return dict(pos=pos, mu=mu, theta=th, Ts=Ts, uabs=rng.uniform(0, 15, N))
+
+
+def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None, swf=None, ch=None, firn=None,
+                   mixed=False, ocean=False):
+    from insilica import _hip
+    from insilica.radiation import _air
+    L, N = c['mu'].shape
+    z = lambda *s: torch.zeros(*s, dtype=torch.float64, device='cuda')
+    o = dict(theta=_dev(c['theta']), Ts=_dev(c['Ts']), wx=z(L - 1, N), olr=z(N), sw_abs=z(N), stored=z(N),
+             rain=z(N), evap=z(N), conv=z(N), tsurf=z(N), rainp=z(4, N), flash=torch.full((N,), -1.0,
+             dtype=torch.float64, device='cuda'), swalb=z(N))
+    if ocean:
+        o['qocean'], o['qsw'] = z(N), z(N)
+    if firn is not None:
+        o['firn'] = _dev(firn)
+    ins = [_dev(x) for x in (np.array([t]), insol, c['pos'], c['uabs'], c['mu'], cap, elev, radf, swf, ch)]
+    tt, ti, tp, tu, tm, tc, te, tr, ts, tch = ins
+    P = _hip.ptr
+    air = _air(None)
+    _hip.call('isw_air_column', N, L, p.ctypes.data_as(ctypes.c_void_p), ctypes.byref(air), mode, P(tt), P(ti),
+              P(tp), P(tu), P(tm), P(o['theta']), None, P(o['Ts']), None, float(dt), P(o['wx']), P(o['olr']),
+              P(o['sw_abs']), P(o['stored']), P(o['rain']), P(o['evap']), P(o['conv']), P(o['tsurf']),
+              P(o['rainp']), None, P(tc), P(o.get('qocean')), P(o.get('qsw')), None, None, None, None, None, None,
+              None, P(te), P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), None, P(o.get('firn')), None,
+              1 if mixed else 0, _hip.stream())
+    torch.cuda.synchronize()
+    return {k: _np(v) for k, v in o.items()}
+
+
+def compare(g, o, tol, dt, label):
+    keys = [k for k in ('theta', 'Ts', 'wx', 'olr', 'sw_abs', 'tsurf', 'qocean', 'qsw', 'swalb', 'firn') if k in o]
+    e = {k: relmax(g[k], o[k]) for k in keys}
+    e['stored'] = relmax(g['stored'], o['stored'])
+    print(label, {k: float(f'{v:.3g}') for k, v in e.items()}, 'moved', int(np.count_nonzero(o['wx'])), flush=True)
+    for k in keys:
+        assert e[k] <= tol, k
+    assert e['stored'] <= max(tol, 1e-9)
+    np.testing.assert_allclose(g['stored'], (g['sw_abs'] - g['olr']) * dt, rtol=0, atol=2e-3)
+    for k in ('rain', 'evap', 'conv', 'rainp', 'flash'):
+        assert np.all(g[k] == 0), k
+    assert np.count_nonzero(o['wx']) > 0
+
+
+@pytest.mark.parametrize('L,mixed', [(2, False), (3, False), (5, False), (3, True), (7, True)])
+def test_gray_sun(L, mixed):
+    from oracle import column as ocol
+    N, dt, t = 40962, 900.0, 0.37 * 86400.0 + 40 * 86400.0
+    print(f'case: air_column gray mode 0 L={L} mixed={mixed} N={N}', flush=True)
+    c = columns(N, L, seed=L)
+    p = par(L, ecc=0.0167, lam_peri=4.94)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed)
+    compare(g, o, 1e-6 if mixed else 1e-12, dt, f'gray L={L} mixed={mixed}')
+
+
+@pytest.mark.parametrize('opts', [dict(sfc=1.0, sw_tau=0.5), dict(bulk_c=1.2e-3, bulk_z=50.0, bulk_w=0.05, sw_tau=0.0),
+                                  dict(bl_bot=0.5, mu_bl=4000.0, theta_rise=0.01, firn=10.0)])
+def test_options(opts):
+    from oracle import column as ocol
+    L, N, dt = 3, 40962, 600.0
+    print(f'case: air_column options {opts} mode 1 N={N}', flush=True)
+    c = columns(N, L, seed=11)
+    rng = np.random.default_rng(12)
+    insol = rng.uniform(0, 450, N)
+    cap = rng.uniform(1e7, 4e8, N)
+    ch = rng.uniform(1e-3, 3e-2, N)
+    elev = rng.uniform(-100, 3000, N)
+    firn = rng.uniform(240, 273.15, N)
+    p = par(L, **opts)
+    kw = dict(insol=insol, cap=cap, ch=ch, elev=elev, firn=firn if 'firn' in opts else None, ocean=True)
+    g = gpu_air_column(p, 1, 0.0, c, dt, **kw)
+    o = ocol.air_column(p, 1, np.array([0.0]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, **kw)
+    compare(g, o, 1e-12, dt, f'options {opts}')
+
+
+@pytest.mark.parametrize('edge_heat,mixed', [(0.0, False), (1.0, False), (1.0, True)])
+def test_ktables(edge_heat, mixed):
+    from oracle import column as ocol
+    from oracle import radiation as orad
+    from insilica.ktables import synthetic_lw, synthetic_sw
+    L, N, dt = 5, 8192, 900.0
+    print(f'case: air_column k-tables edge_heat={edge_heat} mixed={mixed} N={N}', flush=True)
+    c = columns(N, L, seed=21)
+    p = par(L, edge_k=2.0, edge_heat=edge_heat)
+    lwt, swt = orad.LwTables(synthetic_lw()), orad.SwTables(synthetic_sw())
+    q = np.full((L, N), 1e-3)
+    sub = orad.lw_setup(N, L, p, lwt, c['mu'], c['theta'], q)
+    radf = orad.lw_column(N, L, p, lwt, sub, c['Ts'])
+    swf = orad.sw_column(N, L, p, swt, sub, np.full(N, 0.06), mode=1)
+    t = 0.61 * 86400.0
+    g = gpu_air_column(p, 0, t, c, dt, radf=radf, swf=swf, mixed=mixed)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, radf=radf,
+                        swf=swf, mix=mixed)
+    compare(g, o, 1e-6 if mixed else 1e-12, dt, f'k-tables edge_heat={edge_heat} mixed={mixed}')
+
+
+def test_aqua_dry_world():
+    """The 'aqua_dry' world (three dry air layers over the slab, the sun, gray radiation, the column and the exchange
+    every tick) against the oracle's tick, 12 ticks at level 4 in the FP64 form (<= 1e-10, as the core's own
+    multi-step bar); graph replay bit-identical to direct launches; the column moved air across an interface. The
+    annual-mean sun (mean_sun) likewise."""
+    from insilica.world import World
+    from oracle.world import OracleWorld
+    for mean_sun in (False, True):
+        print(f'case: aqua_dry world level 4, 12 ticks, mean_sun={mean_sun}', flush=True)
+        w = World(4, 'aqua_dry', precision='double', params=dict(mean_sun=mean_sun))
+        o = OracleWorld(w)
+        w.step(12, graph=False)
+        o.step(12)
+        got = {k: _np(v) for k, v in w.state_arrays().items()}
+        ref = o.state()
+        errs = {k: relmax(got[k], ref[k]) for k in ref}
+        print('aqua_dry world errors', {k: float(f'{v:.3g}') for k, v in errs.items()}, flush=True)
+        assert max(errs.values()) <= 1e-10
+        assert np.count_nonzero(_np(w.physics.wx)) > 0
+        w2 = World(4, 'aqua_dry', precision='double', params=dict(mean_sun=mean_sun))
+        w2.step(12, graph=True)
+        assert w2.state_hash() == w.state_hash()
+
+
+def test_aqua_dry_world_ktables_and_mixed():
+    """The aqua_dry world with the k-tables' radiation every hour (radiation ticks captured as their own graph kind):
+    graph replay bit-identical to direct launches, the column's energy closes per cell on its last tick. The mixed
+    form of the gray world stays within 1e-5 (relative L2 in h) of the FP64 form over a model day at level 5."""
+    from insilica.ktables import synthetic_lw, synthetic_sw
+    from insilica.world import World
+    print('case: aqua_dry world level 4 k-tables, 10 ticks', flush=True)
+    P = dict(longwave='ktables', lw_tables=synthetic_lw(), sw_tables=synthetic_sw())
+    w = World(4, 'aqua_dry', precision='double', params=P)
+    w.step(10, graph=False)
+    w2 = World(4, 'aqua_dry', precision='double', params=P)
+    w2.step(10, graph=True)
+    assert w2.state_hash() == w.state_hash()
+    ph = w.physics
+    np.testing.assert_allclose(_np(ph.stored), (_np(ph.sw_abs) - _np(ph.olr)) * w.dt, rtol=0, atol=2e-3)
+    assert float(ph.olr.min()) > 50.0
+    print('case: aqua_dry world level 5 mixed vs FP64, one model day', flush=True)
+    a, b = World(5, 'aqua_dry', precision='double'), World(5, 'aqua_dry', precision='mixed')
+    k = int(round(86400.0 / a.dt))
+    a.step(k)
+    b.step(k)
+    ha, hb = _np(a.h), _np(b.h)
+    l2 = float(np.linalg.norm(hb - ha) / np.linalg.norm(ha))
+    print('aqua_dry mixed vs FP64 after a day: rel L2 h', l2, 'theta', relmax(_np(b.theta), _np(a.theta)), flush=True)
+    assert l2 <= 1e-5

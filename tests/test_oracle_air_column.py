@@ -1,0 +1,268 @@
+"""The dry air column's oracle (oracle/column.c og_air_column; reference air.cu:876-1972 with moist = 0) pinned on the
+CPU by what the physics fixes, independently of the reference's loops:
+
+- Energy: the column's stored energy (air enthalpy + slab heat) changes by exactly the absorbed sunlight less the
+  outgoing longwave, sw_abs - olr, times dt (the sensible heat and the crossing's mass moves only move energy inside
+  the column), with the gray scheme, the k-tables' fluxes, the bulk and surface-layer exchange and the ocean's cap.
+- An isothermal gray column over a black surface at its temperature: the outgoing longwave is sigma T^4; each layer
+  cools by sigma T^4 times the difference of its top's and bottom's transmission to space, exp(-tau(p)) with
+  tau(p) = tau0 (f s + (1 - f) s^4); the surface loses sigma T^4 exp(-tau0).
+- Sunlight: mode 1 absorbs asw (m / m_ref) I in the air (split by s^4 of pressure) and (I - that)(1 - alb) at the
+  surface; mode 0's sun at the equinox stands over (1, 0, 0) at t = 0 and over (0, -cos tilt, sin tilt) at a quarter
+  year, its mean over the sphere is S0 / 4, and Kepler's orbit gives the distance factor 1 / (1 -+ e)^2 at perihelion
+  and aphelion.
+- Sensible heat: h_sens (1 + |U| / u0)(Ts - theta_b (ps / p0)^kappa) out of the slab into the air touching the
+  ground; the surface layer's implicit exchange brings that air at most to the surface's temperature in one step.
+- Options with exact equivalents: theta_rise over elev is the raised theta*, the ocean's cap equal to c_slab is the
+  slab, a firn layer's heat lands in the slab.
+The k-tables' edge heating in the crossing (dtop/dbot) is pinned by the energy identity only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import column as ocol
+from oracle import lib
+
+CP, KAPPA, P0, GRAV, SIGMA = 1004.6, 287.04 / 1004.6, 1.0e5, 9.80616, 5.67e-8
+
+
+def par3(**kw):
+    from insilica.atmosphere import par_array
+    return par_array(3, dict(dict(theta_star=305.0, theta_star2=380.0), **kw))
+
+
+def aqua(N=96, seed=0):
+    """The dry aquaplanet's start (three layers with interfaces near 220 and 600 hPa), with seeded noise."""
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(size=(N, 3))
+    pos /= np.linalg.norm(pos, axis=1)[:, None]
+    s2 = pos[:, 2] ** 2
+    edges = np.array([0.0, 0.22e5, 0.60e5, 1.0e5])[:, None] * rng.uniform(0.97, 1.03, (1, N))
+    mu = np.diff(edges, axis=0) / GRAV
+    theta = np.stack([np.full(N, 415.0), np.full(N, 342.5), 290.0 - 30.0 * s2]) + rng.normal(0, 1.5, (3, N))
+    Ts = 288.0 - 40.0 * (s2 - 1.0 / 3.0) + rng.normal(0, 2.0, N)
+    uabs = rng.uniform(0.0, 15.0, N)
+    return dict(pos=pos, mu=mu, theta=theta, Ts=Ts, uabs=uabs)
+
+
+def pib_of(mu):
+    """Each layer's mass-mean Exner function cp (p / p0)^kappa over its pressure span."""
+    pb = np.cumsum(GRAV * mu, 0)
+    pt = np.concatenate([np.zeros((1,) + mu.shape[1:]), pb[:-1]])
+    return pt, pb, CP * (pb ** (KAPPA + 1) - pt ** (KAPPA + 1)) / ((KAPPA + 1) * P0 ** KAPPA * (pb - pt))
+
+
+def run(c, p, mode=0, t=0.3 * 86400.0, dt=900.0, **kw):
+    return ocol.air_column(p, mode, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, **kw)
+
+
+def stored_matches(o, dt):
+    np.testing.assert_allclose(o['stored'], (o['sw_abs'] - o['olr']) * dt, rtol=0, atol=2e-3)
+
+
+@pytest.mark.parametrize('opts', [dict(), dict(sw_tau=0.5), dict(sw_tau=0.0), dict(bulk_c=1.2e-3, bulk_z=50.0),
+                                  dict(bulk_c=1.2e-3, bulk_w=0.05), dict(bl_bot=0.5, mu_bl=3000.0),
+                                  dict(theta_star=330.0, edge_heat=1.0)])
+def test_energy_closes(opts):
+    c = aqua()
+    dt = 900.0
+    o = run(c, par3(**opts), dt=dt)
+    stored_matches(o, dt)
+    assert np.all(o['sw_abs'] >= 0) and np.all(o['olr'] > 100)
+    assert np.any(np.abs(o['wx']) > 0)                     # the crossing moved air
+    assert np.all(o['rain'] == 0) and np.all(o['evap'] == 0) and np.all(o['conv'] == 0)
+    assert np.all(o['rainp'] == 0) and np.all(o['flash'] == 0)
+    np.testing.assert_array_equal(o['tsurf'], c['Ts'])
+
+
+def test_energy_closes_mean_sun_ocean_cap_surface_layer():
+    c = aqua(seed=4)
+    N, dt = c['Ts'].size, 600.0
+    rng = np.random.default_rng(5)
+    insol = rng.uniform(0, 450, N)
+    cap = rng.uniform(1e7, 4e8, N)
+    ch = rng.uniform(1e-3, 3e-2, N)
+    o = run(c, par3(sfc=1.0), mode=1, dt=dt, insol=insol, cap=cap, ch=ch, ocean=True)
+    stored_matches(o, dt)
+    # the ocean's forcing is the slab's net gain
+    np.testing.assert_allclose((o['Ts'] - c['Ts']) * cap / dt, o['qocean'], rtol=1e-9, atol=1e-9)
+    assert np.all(o['qsw'] >= 0)
+
+
+def test_energy_closes_with_ktables():
+    from oracle import radiation as orad
+    from insilica.ktables import synthetic_lw, synthetic_sw
+    c = aqua(N=48, seed=7)
+    N, L, dt = 48, 3, 900.0
+    p = par3(edge_k=1.0)
+    lwt = orad.LwTables(synthetic_lw())
+    swk = synthetic_sw()
+    swt = orad.SwTables(swk)
+    q = np.full((L, N), 1e-3)
+    sub = orad.lw_setup(N, L, p, lwt, c['mu'], c['theta'], q)
+    radf = orad.lw_column(N, L, p, lwt, sub, c['Ts'])
+    swf = orad.sw_column(N, L, p, swt, sub, np.full(N, 0.06), mode=1)
+    for opts in (dict(), dict(edge_heat=1.0)):
+        o = run(c, par3(edge_k=1.0, **opts), dt=dt, radf=radf, swf=swf)
+        stored_matches(o, dt)
+        # the outgoing longwave is the k-tables' top-of-air up flux with the surface's emission carried up
+        Es = 0.95 * SIGMA * c['Ts'] ** 4
+        np.testing.assert_allclose(o['olr'], radf[0] + radf[2 * (L + 1)] * Es, rtol=1e-14)
+
+
+def test_isothermal_gray_column():
+    N, T0, dt, tau0, f = 5, 262.0, 1800.0, 1.7, 0.35
+    mu = np.array([[2100.0], [3900.0], [4100.0]]) * np.linspace(0.9, 1.1, N)[None, :]
+    pt, pb, pib = pib_of(mu)
+    theta = T0 * CP / pib
+    c = dict(pos=np.tile([0.0, 0.0, 1.0], (N, 1)), mu=mu, theta=theta, Ts=np.full(N, T0), uabs=np.full(N, 5.0))
+    p = par3(eps_s=1.0, h_sens=0.0, tau0=tau0, tau_lin=f, theta_star=-1.0, theta_star2=-1.0)
+    o = run(c, p, mode=1, dt=dt, insol=np.zeros(N))
+    B = SIGMA * T0 ** 4
+    np.testing.assert_allclose(o['olr'], B, rtol=1e-13)
+    tau = lambda pp: tau0 * (f * pp / pb[-1] + (1 - f) * (pp / pb[-1]) ** 4)
+    Q = -B * (np.exp(-tau(pt)) - np.exp(-tau(pb)))
+    np.testing.assert_allclose((o['theta'] - theta) * mu * pib / dt, Q, rtol=1e-10)
+    cw = p[8]
+    np.testing.assert_allclose((o['Ts'] - T0) * cw / dt, -B * math.exp(-tau0), rtol=1e-9)
+    assert np.all(o['wx'] == 0)
+
+
+def test_transparent_air_and_black_surface():
+    c = aqua(N=16, seed=2)
+    o = run(c, par3(tau0=0.0, eps_s=1.0, S0=0.0, h_sens=0.0, theta_star=-1.0))
+    np.testing.assert_allclose(o['olr'], SIGMA * c['Ts'] ** 4, rtol=1e-14)
+    np.testing.assert_array_equal(o['theta'], c['theta'])
+
+
+def test_sunlight_split():
+    c = aqua(N=24, seed=3)
+    N, dt, asw, alb = 24, 900.0, 0.23, 0.11
+    I = np.linspace(0.0, 500.0, N)
+    o = run(c, par3(asw=asw, alb=alb, tau0=0.0, eps_s=0.0, h_sens=0.0, theta_star=-1.0, sw_tau=2.0), mode=1,
+            dt=dt, insol=I)
+    m = c['mu'].sum(0)
+    air = asw * m / (P0 / GRAV) * I
+    sfc = (I - air) * (1 - alb)
+    np.testing.assert_allclose(o['sw_abs'], air + sfc, rtol=1e-13, atol=1e-12)
+    np.testing.assert_allclose((o['Ts'] - c['Ts']) * (1000.0 * 4186.0 * 100.0) / dt, sfc, rtol=1e-8, atol=1e-8)
+    pt, pb, pib = pib_of(c['mu'])
+    share = (pb / pb[-1]) ** 4 - (pt / pb[-1]) ** 4
+    np.testing.assert_allclose((o['theta'] - c['theta']) * c['mu'] * pib / dt, air * share, rtol=1e-9, atol=1e-9)
+
+
+def sun_I(g, t, **kw):
+    N = len(g.pos)
+    z = np.zeros((3, N))
+    c = dict(pos=np.asarray(g.pos, float), mu=z + 3000.0, theta=z + 300.0, Ts=np.full(N, 280.0), uabs=np.zeros(N))
+    return run(c, par3(asw=0.0, alb=0.0, theta_star=-1.0, **kw), mode=0, t=t)['sw_abs']
+
+
+def test_sun_geometry():
+    from insilica import grid as gridmod
+    g = gridmod.get(4)
+    pos, A = np.asarray(g.pos, float), np.asarray(g.areaCell, float)
+    S0, tilt, year, day = 1361.0, math.radians(23.44), 365 * 86400.0, 86400.0
+    I0 = sun_I(g, 0.0)
+    np.testing.assert_allclose(I0, S0 * np.maximum(pos[:, 0], 0.0), rtol=1e-12, atol=1e-9)
+    assert (A * I0).sum() / A.sum() == pytest.approx(S0 / 4, rel=2e-3)
+    Iq = sun_I(g, year / 4)
+    s = np.array([0.0, -math.cos(tilt), math.sin(tilt)])
+    np.testing.assert_allclose(Iq, S0 * np.maximum(pos @ s, 0.0), rtol=1e-9, atol=1e-6)
+    # Kepler's orbit: the distance factor at perihelion and aphelion, the longitude of perihelion
+    e, lp = 0.0167, math.radians(282.9)
+    p = par3(ecc=e, lam_peri=lp)
+    E0 = 2 * math.atan(math.sqrt((1 - e) / (1 + e)) * math.tan(-lp / 2))
+    tp = -(E0 - e * math.sin(E0)) * year / (2 * math.pi)
+    fd = np.zeros(1)
+    lon = lib().og_sun_lon(tp, p.ctypes.data, fd.ctypes.data)
+    assert fd[0] == pytest.approx(1 / (1 - e) ** 2, rel=1e-12)
+    assert math.remainder(lon - lp, 2 * math.pi) == pytest.approx(0.0, abs=1e-9)
+    lib().og_sun_lon(tp + year / 2, p.ctypes.data, fd.ctypes.data)
+    assert fd[0] == pytest.approx(1 / (1 + e) ** 2, rel=1e-12)
+
+
+def test_sensible_heat():
+    c = aqua(N=32, seed=6)
+    N, dt, hsn, u0 = 32, 900.0, 12.0, 10.0
+    p = par3(tau0=0.0, eps_s=0.0, h_sens=hsn, u0=u0, theta_star=-1.0)
+    o = run(c, p, mode=1, dt=dt, insol=np.zeros(N))
+    ps = GRAV * c['mu'].sum(0)
+    Tsa = c['theta'][2] * (ps / P0) ** KAPPA                # the bottom layer is thicker than mu_bl
+    H = hsn * (1 + c['uabs'] / u0) * (c['Ts'] - Tsa)
+    np.testing.assert_allclose((c['Ts'] - o['Ts']) * p[8] / dt, H, rtol=1e-8, atol=1e-9)
+    _, _, pib = pib_of(c['mu'])
+    np.testing.assert_allclose((o['theta'][2] - c['theta'][2]) * c['mu'][2] * pib[2] / dt, H, rtol=1e-8, atol=1e-9)
+    np.testing.assert_array_equal(o['theta'][:2], c['theta'][:2])
+    # the surface layer's implicit exchange: a huge rho C_H U brings the air touching the ground (mu_bl) to the
+    # surface's temperature within the step, never past it
+    ch = np.full(N, 1e9)
+    o2 = run(c, par3(tau0=0.0, eps_s=0.0, sfc=1.0, theta_star=-1.0), mode=1, dt=dt, insol=np.zeros(N), ch=ch)
+    mb = P0 / GRAV * 0.1
+    heat = (o2['theta'][2] - c['theta'][2]) * c['mu'][2] * pib[2]
+    np.testing.assert_allclose(heat / (CP * mb * (c['Ts'] - Tsa)), 1.0, rtol=1e-6)
+
+
+def test_bulk_stability():
+    """Louis's stability: air colder than the surface takes more heat per kelvin than air warmer than it."""
+    c = aqua(N=2, seed=1)
+    c['uabs'][:] = 3.0
+    ps = GRAV * c['mu'].sum(0)
+    Tsa = c['theta'][2] * (ps / P0) ** KAPPA
+    p = par3(tau0=0.0, eps_s=0.0, bulk_c=1.2e-3, bulk_z=50.0, theta_star=-1.0)
+    rate = []
+    for d in (4.0, -4.0):
+        c['Ts'] = Tsa + d
+        o = run(c, p, mode=1, insol=np.zeros(2))
+        rate.append((c['Ts'] - o['Ts']) / d)
+    assert np.all(rate[0] > 1.2 * rate[1])
+
+
+def test_bottom_air_share():
+    """bl_bot: a bottom layer thinner than bl_bot mu_bl shares the surface's heat with the air above it."""
+    c = aqua(N=8, seed=9)
+    c['mu'][1] += c['mu'][2] - 600.0
+    c['mu'][2] = 600.0
+    p = par3(tau0=0.0, eps_s=0.0, bl_bot=0.5, theta_star=-1.0)
+    o = run(c, p, mode=1, insol=np.zeros(8))
+    _, _, pib = pib_of(c['mu'])
+    heat = (o['theta'] - c['theta']) * c['mu'] * pib
+    mb = max(600.0, 0.5 * P0 / GRAV * 0.1)
+    np.testing.assert_allclose(heat[1] / heat[2], (mb - 600.0) / 600.0, rtol=1e-9)
+
+
+def test_exact_equivalents():
+    c = aqua(N=40, seed=11)
+    N = 40
+    elev = np.where(np.arange(N) % 2 == 0, 300.0, -50.0)
+    a = run(c, par3(theta_rise=0.01), elev=elev)
+    for i in range(N):
+        ci = {k: (v[..., i:i + 1] if k != 'pos' else v[i:i + 1]) for k, v in c.items()}
+        ts = 305.0 + (0.01 * elev[i] if elev[i] > 0 else 0.0)
+        b = run(ci, par3(theta_star=ts))
+        np.testing.assert_array_equal(a['theta'][:, i], b['theta'][:, 0])
+        np.testing.assert_array_equal(a['wx'][:, i], b['wx'][:, 0])
+    s = run(c, par3())
+    cs = run(c, par3(), cap=np.full(N, 1000.0 * 4186.0 * 100.0))
+    for k in ('theta', 'Ts', 'wx', 'stored'):
+        np.testing.assert_array_equal(s[k], cs[k])
+    # a firn layer's heat lands in the slab before the surface's fluxes: the same step from the slab warmed by it
+    Tfn = np.full(N, 265.0)
+    f = run(c, par3(firn=10.0), firn=Tfn)
+    Cf = 2090.0 * 450.0 * 10.0
+    cw = dict(c, Ts=c['Ts'] + Cf * (265.0 - 273.15) / (1000.0 * 4186.0 * 100.0))
+    w = run(cw, par3())
+    for k in ('theta', 'Ts', 'wx', 'olr'):
+        np.testing.assert_array_equal(f[k], w[k])
+    np.testing.assert_array_equal(f['firn'], 273.15)
+    stored_matches(f, 900.0)
+
+
+def test_mixed_build_close():
+    c = aqua(N=64, seed=12)
+    a, b = run(c, par3()), run(c, par3(), mix=True)
+    for k in ('theta', 'Ts', 'olr'):
+        np.testing.assert_allclose(b[k], a[k], rtol=2e-6)
+    assert np.any(b['olr'] != a['olr'])
