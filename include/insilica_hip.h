@@ -235,20 +235,23 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
 
 /* ---- the air column's Held-Suarez path and the surface exchange (reference: kernels/air.cu; FP64 in both forms) ---- */
 
-/* AirPhysics.column with the default forcing (reference: atmosphere.py:134-158; kernels/air.cu:876-1972 air_column),
- * the dry column (moist = 0): the sun (mode 0 the diurnal and seasonal sun from the device clock t (1 double, s) at the
- * unit cell positions pos (N, 3); mode 1 the mean insolation insol (N)); the shortwave absorbed in the air and at the
- * surface (the k-tables' swf ((3L + 2), N) when given, else the gray split); the gray longwave (or the k-tables' radf
- * ((9L + 6), N)); the slab surface at Ts (N; heat capacity cap (N) when given, else c_slab) under the bulk exchange or
- * the surface layer's rho C_H U (ch (N), with sfc > 0); the heating as theta and the crossing of the interfaces
- * (the k-tables' edge heating). Updates theta (L, N) and Ts in place and writes wx (L-1, N), olr, sw_abs, stored
- * (the column's energy change, J/m^2), rain, evap, conv (zero for dry air), tsurf; rainp (4, N), qocean, qsw, flash,
- * swalb and firn (the firn's temperature; its heat goes to the slab) when given. The argument list is the
- * reference's, with qv, ice, land, soilcap, soil, pond, snow, cloudw, cloudo and lc unread (the reference reads them
- * only for moist air); elev (N) with theta_rise > 0 raises the lowest interface over high ground. Returns ISW_EBADARG
- * for moist > 0, bl_mix > 0, cu_dth > 0 with L > 3, or a non-NULL dgo or xfer: those parts of the reference's column
- * are not in this library. prec: ISW_FP64, or ISW_MIXED (the reference's mixed build: 1 - e^-x and the edge
- * sub-levels' Exner powers in float32). */
+/* AirPhysics.column with the default forcing (reference: atmosphere.py:134-158; kernels/air.cu:876-1972 air_column):
+ * the sun (mode 0 the diurnal and seasonal sun from the device clock t (1 double, s) at the unit cell positions pos
+ * (N, 3); mode 1 the mean insolation insol (N)); the shortwave absorbed in the air and at the surface (the k-tables'
+ * swf ((3L + 2), N) when given, else the gray split); the gray longwave (or the k-tables' radf ((9L + 6), N)); the
+ * slab surface at Ts (N; heat capacity cap (N) when given, else c_slab) under the bulk exchange or the surface
+ * layer's rho C_H U (ch (N), with sfc > 0); the heating as theta and the crossing of the interfaces (the k-tables'
+ * edge heating). With moist > 0 also the vapor qv (L, N) and sea ice ice (N, kg/m^2): evaporation, the ice's skin
+ * and albedo, freezing and melting, condensation per layer, the parcel plume of moist convection (plume_deep = 0),
+ * and the vapor's longwave part with lw_w0. Updates theta, Ts (and qv, ice) in place and writes wx (L-1, N), olr,
+ * sw_abs, stored (the column's energy change, J/m^2), rain, evap, conv (kg/m^2/s), tsurf; rainp (4, N), qocean, qsw,
+ * flash, swalb and firn (the firn's temperature; its heat goes to the slab) when given. The argument list is the
+ * reference's; soilcap, soil, pond, snow, cloudo and lc are unread (the reference reads them only over land or with
+ * clouds); elev (N) with theta_rise > 0 raises the lowest interface over high ground. Returns ISW_EBADARG for bl_mix
+ * > 0, cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or ice, with clouds (cloud > 0 and
+ * cloudw given), land, plume_deep, plume_root, q_shape, sat_prof, sat_base, edge_vap or freeze_dry: those parts of
+ * the reference's column are not in this library. prec: ISW_FP64, or ISW_MIXED (the reference's mixed build:
+ * 1 - e^-x, the edge sub-levels' Exner powers and the saturation and condensation arithmetic in float32). */
 int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* air, int32_t mode, const double* t,
                        const double* insol, const double* pos, const double* uabs, const double* mu, double* theta,
                        double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,

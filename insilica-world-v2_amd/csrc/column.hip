@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "diag.hpp"
 #include "insilica_hip.h"
@@ -110,33 +111,40 @@ __device__ void edge_density(double a, double b, double m, double& fa, double& f
   fb = s * (e1 + 1) / e1 / W;
 }
 
-template <int L>
-__device__ __forceinline__ double column_energy(const double* mu, const double* pib, const double* th, const CP& P) {
+// The column's energy (air.cu:484-488): enthalpy, and for moist air the vapor's latent energy.
+template <int L, bool MOIST = false>
+__device__ __forceinline__ double column_energy(const double* mu, const double* pib, const double* th, const double* q,
+                                                const CP& P) {
   double E = 0;
 #pragma unroll
-  for (int k = 0; k < L; k++) E += mu[k] * pib[k] * th[k];   // (dry: the vapor's latent term is zero)
+  for (int k = 0; k < L; k++) {
+    if constexpr (MOIST) E += mu[k] * pib[k] * th[k] + P.Lv * mu[k] * q[k];
+    else E += mu[k] * pib[k] * th[k];   // (dry: the vapor's latent term is zero)
+  }
   return E;
 }
 
-// Move m (> 0) kg/m^2 of dry air from layer `from` to `to` (air.cu:495-520): the layers' masses, Exner functions and
-// theta re-divided, the column's energy residual booked into the receiver.
-template <int L>
-__device__ void move_mass(const CP& P, double* mu, double* th, double* pib, int from, int to, double m,
-                          double th_leave, double th_arrive) {
+// Move m (> 0) kg/m^2 of air from layer `from` to `to` (air.cu:495-520): the layers' masses, Exner functions, theta
+// (and vapor: it leaves with q_leave per kg and arrives with q_arrive) re-divided, the column's energy residual booked
+// into the receiver.
+template <int L, bool MOIST = false>
+__device__ void move_mass(const CP& P, double* mu, double* th, double* q, double* pib, int from, int to, double m,
+                          double th_leave, double th_arrive, double q_leave = 0, double q_arrive = 0) {
   if (m == 0) return;
-  double pb2[L], pib2[L], mu2[L], th2[L];
-  const double Eb = column_energy<L>(mu, pib, th, P);
+  double pb2[L], pib2[L], mu2[L], th2[L], q2[L];
+  const double Eb = column_energy<L, MOIST>(mu, pib, th, q, P);
 #pragma unroll
   for (int k = 0; k < L; k++) {
     mu2[k] = mu[k];
     th2[k] = th[k];
+    if constexpr (MOIST) q2[k] = q[k];
   }
   // (from and to are runtime indices; the arrays stay in registers through compile-time selects)
-  double mf = 0, mt = 0, tf = 0, tt = 0;
+  double mf = 0, mt = 0, tf = 0, tt = 0, qf = 0, qt = 0;
 #pragma unroll
   for (int k = 0; k < L; k++) {
-    if (k == from) { mf = mu[k]; tf = th[k]; mu2[k] -= m; }
-    if (k == to) { mt = mu[k]; tt = th[k]; mu2[k] += m; }
+    if (k == from) { mf = mu[k]; tf = th[k]; mu2[k] -= m; if constexpr (MOIST) qf = q[k]; }
+    if (k == to) { mt = mu[k]; tt = th[k]; mu2[k] += m; if constexpr (MOIST) qt = q[k]; }
   }
   layer_exner<L>(mu2, P, pb2, pib2);
   double muf2 = 0, mut2 = 0;
@@ -146,12 +154,17 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* pib, int 
     if (k == to) mut2 = mu2[k];
   }
   const double thf = (mf * tf - m * th_leave) / muf2, tht = (mt * tt + m * th_arrive) / mut2;
+  double qfn = 0, qtn = 0;
+  if constexpr (MOIST) {
+    qfn = (mf * qf - m * q_leave) / muf2;
+    qtn = (mt * qt + m * q_arrive) / mut2;
+  }
 #pragma unroll
   for (int k = 0; k < L; k++) {
-    if (k == from) th2[k] = thf;
-    if (k == to) th2[k] = tht;
+    if (k == from) { th2[k] = thf; if constexpr (MOIST) q2[k] = qfn; }
+    if (k == to) { th2[k] = tht; if constexpr (MOIST) q2[k] = qtn; }
   }
-  const double Ea = column_energy<L>(mu2, pib2, th2, P);
+  const double Ea = column_energy<L, MOIST>(mu2, pib2, th2, q2, P);
 #pragma unroll
   for (int k = 0; k < L; k++)
     if (k == to) th2[k] += (Eb - Ea) / (mu2[k] * pib2[k]);
@@ -160,6 +173,7 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* pib, int 
     th[k] = th2[k];
     mu[k] = mu2[k];
     pib[k] = pib2[k];
+    if constexpr (MOIST) q[k] = q2[k];
   }
 }
 
@@ -167,9 +181,9 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* pib, int 
 // across interface j. Dry air: the edge-vapor and freeze-drying options move the same mass without vapor.
 // et[k], eb[k]: the heating at layer k's top and bottom edge (the reference's dtop[k] or dth[k], and dbot[k] or dth[k],
 // the top layer's bottom-edge default its dth_up): always given, so every array stays in registers.
-template <int L>
-__device__ void crossing(const CP& P, double* mu, double* th, const double* dth, const double* et, const double* eb,
-                         double* pib, double* wxo) {
+template <int L, bool MOIST = false>
+__device__ void crossing(const CP& P, double* mu, double* th, double* q, const double* dth, const double* et,
+                         const double* eb, double* pib, double* wxo) {
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (P.theta_star <= 0) return;
@@ -249,10 +263,19 @@ __device__ void crossing(const CP& P, double* mu, double* th, const double* dth,
     const double maxup = mu[b] - P.floor_, maxdn = mu[a] - P.floor_;
     if (dm > maxup) dm = maxup > 0 ? maxup : 0.0;
     if (-dm > maxdn) dm = maxdn > 0 ? -maxdn : 0.0;
+    // (the crossing air carries its layer's vapor)
+    double qb_ = 0, qa_ = 0;
+    if constexpr (MOIST) {
+#pragma unroll
+      for (int k = 0; k < L; k++) {
+        if (k == b) qb_ = q[k];
+        if (k == a) qa_ = q[k];
+      }
+    }
     if (dm > 0)
-      move_mass<L>(P, mu, th, pib, b, a, dm, thcj[j], thcj[j]);
+      move_mass<L, MOIST>(P, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qb_, qb_);
     else if (dm < 0)
-      move_mass<L>(P, mu, th, pib, a, b, -dm, thcj[j], thcj[j]);
+      move_mass<L, MOIST>(P, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], qa_, qa_);
     wxo[j] = dm;
   }
   const double tiny = P.floor_ * 1.000001;
@@ -323,7 +346,7 @@ __global__ void __launch_bounds__(BS) k_hs_column(int N, CP P, double ka, double
     dth_up = (thi - P.theta_star) * (1 - exp(-ki * dt));
   }
   eb[0] = dth_up;
-  crossing<L>(P, mu, th, dth, dth, eb, pib, wxo);
+  crossing<L>(P, mu, th, nullptr, dth, dth, eb, pib, wxo);
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
   double E1 = 0;
@@ -447,25 +470,31 @@ __global__ void __launch_bounds__(BS) k_gas_src(int N, int L, int G, const doubl
 __global__ void k_advance_time(double* t, double dt) { t[0] += dt; }
 
 // ---------------------------------------------------------------------------------------------------------------
-// air_column, the dry path (air.cu:876-1972 with moist = 0): the sun, the shortwave's split, the gray longwave or
-// the k-tables' fluxes, the slab surface under the bulk or surface-layer exchange, the heating as theta, and the
-// crossing with the k-tables' edge heating. MIX selects the reference's mixed build, whose 1 - e^-x and edge Exner
-// powers are float32 (air.cu:24-36); everything else is FP64 in both forms.
+// air_column (air.cu:876-1972): the dry column, and the moist one without clouds, land or the mixed boundary layer:
+// the sun, the shortwave's split, the gray longwave (the vapor's part from the column's vapor with lw_w0) or the
+// k-tables' fluxes, the slab surface or the sea-ice skin under the bulk or surface-layer exchange, the evaporation,
+// the heating as theta, the condensation, the crossing with the k-tables' edge heating, and the moist convection's
+// parcel plume. MIX selects the reference's mixed build (air.cu:24-36): 1 - e^-x, the edge Exner powers and the
+// saturation and condensation arithmetic (its `fast` type) in float32; the rest is FP64 in both forms.
 // ---------------------------------------------------------------------------------------------------------------
 
-// The air column's own parameters (PAR_ORDER positions), by value beside CP.
+// The air column's own parameters (PAR_ORDER positions) and the air's constants, by value beside CP.
 struct AP {
   double S0, asw, tau0, mu_ref, eps_s, h_sens, u0, alb, c_slab, sigma, tilt, year_s, day_s, tau_lin, Lf, Tf, sw_tau,
-      bl_bot, theta_rise, bulk_c, bulk_z, bulk_w, edge_heat, sfc, edge_k, firn, rho_firn, c_ice, ecc, lam_peri, rd;
+      bl_bot, theta_rise, bulk_c, bulk_z, bulk_w, edge_heat, sfc, edge_k, firn, rho_firn, c_ice, ecc, lam_peri, rd,
+      k_evap, tau_conv, alb_ice, k_ice, Rv, sat_layer, alb_ice_melt, alb_ramp, lw_w0, ice_air, dry_edge, eps;
 };
 
-static AP make_ap(const double* a, double rd) {
+static AP make_ap(const double* a, const isw_air* air) {
   AP p;
   p.S0 = a[0]; p.asw = a[1]; p.tau0 = a[2]; p.mu_ref = a[3]; p.eps_s = a[4]; p.h_sens = a[5]; p.u0 = a[6];
   p.alb = a[7]; p.c_slab = a[8]; p.sigma = a[12]; p.tilt = a[17]; p.year_s = a[18]; p.day_s = a[19];
   p.tau_lin = a[20]; p.Lf = a[24]; p.Tf = a[25]; p.sw_tau = a[34]; p.bl_bot = a[54]; p.theta_rise = a[69];
   p.bulk_c = a[74]; p.bulk_z = a[76]; p.bulk_w = a[77]; p.edge_heat = a[84]; p.sfc = a[90]; p.edge_k = a[101];
-  p.firn = a[120]; p.rho_firn = a[122]; p.c_ice = a[123]; p.ecc = a[126]; p.lam_peri = a[127]; p.rd = rd;
+  p.firn = a[120]; p.rho_firn = a[122]; p.c_ice = a[123]; p.ecc = a[126]; p.lam_peri = a[127]; p.rd = air->rd;
+  p.k_evap = a[26]; p.tau_conv = a[27]; p.alb_ice = a[28]; p.k_ice = a[31]; p.Rv = a[32]; p.sat_layer = a[33];
+  p.alb_ice_melt = a[41]; p.alb_ramp = a[45]; p.lw_w0 = a[48]; p.ice_air = a[102]; p.dry_edge = a[99];
+  p.eps = air->eps;
   return p;
 }
 
@@ -503,43 +532,123 @@ __device__ double sun_lon_fd(double t, const AP& A, double* fd) {
   return nu + A.lam_peri;
 }
 
-struct AirOut {
+// Saturation specific humidity (air.cu:156-171): Tetens over water; with ice_air, below freezing the blend with
+// Tetens over ice by the liquid share. F is the build's `fast` type.
+template <typename F>
+__device__ __forceinline__ F qsat_a(F T, F p, F eps, double ice) {
+  const F w = eps * (F)611.0 * (F)exp(((F)17.27 * (T - (F)273.15) / (T - (F)35.85))) / p;
+  if (!(ice > 0) || T >= (F)273.15) return w;
+  const F l = fmin((F)1, fmax((F)0, (T - (F)253.15) / (F)20));
+  const F si = eps * (F)611.0 * (F)exp(((F)21.875 * (T - (F)273.16) / (T - (F)7.66))) / p;
+  return l * w + (1 - l) * si;
+}
+
+// Saturation adjustment at Exner pib and pressure p in one Newton step (air.cu:190-198); returns the condensate.
+template <bool MIX>
+__device__ double condense(double& th, double& q, double pib, double p, const CP& P, const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  const F T = (F)pib * (F)th / (F)P.cp, qs = qsat_a<F>(T, (F)p, (F)A.eps, A.ice_air);
+  if (!(q > qs)) return 0;
+  const double dq = (F)(q - qs) / ((F)1 + (F)P.Lv * (F)P.Lv * qs / ((F)P.cp * (F)A.Rv * T * T));
+  q -= dq;
+  th += P.Lv * dq / pib;
+  return dq;
+}
+
+// The layer's capacity over its pressure span [pt, pb] (4-point Gauss-Legendre; nodes colder than 160 K hold
+// nothing), one Newton step with Clausius-Clapeyron's slope at each node (air.cu:275-291).
+template <bool MIX>
+__device__ double condense_layer(double& th, double& q, double pib, double pt, double pb, const CP& P, const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  constexpr double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  constexpr double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  F qs = 0, dqs = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
+    const F ex = (F)fpow_<MIX>(p / P.p0, P.kappa), T = (F)th * ex;
+    if (T < 160) continue;
+    const F s = qsat_a<F>(T, (F)p, (F)A.eps, A.ice_air);
+    qs += (F)0.5 * (F)wq[j] * s;
+    dqs += (F)0.5 * (F)wq[j] * s * (F)P.Lv / ((F)A.Rv * T * T) * ex;
+  }
+  if (!(q > qs)) return 0;
+  const double dq = (F)(q - qs) / ((F)1 + (F)P.Lv / (F)pib * dqs);
+  q -= dq;
+  th += P.Lv * dq / pib;
+  return dq;
+}
+
+// Condensation in layer k (air.cu:319-331; sat_prof = sat_base = 0): the layer's capacity, else v1's rule at the
+// pressure whose Exner function is the layer's. k is a runtime index: the layer's values go through selects.
+template <int L, bool MIX>
+__device__ double condense_k(int k, const double* mu, double* th, double* q, const double* pib, const CP& P,
+                             const AP& A) {
+  double pt = 0, mk = 0, tk = 0, qk = 0, pk = 0;
+#pragma unroll
+  for (int j = 0; j < L; j++) {
+    if (j < k) pt += P.grav * mu[j];
+    if (j == k) { mk = mu[j]; tk = th[j]; qk = q[j]; pk = pib[j]; }
+  }
+  double dq;
+  if (A.sat_layer > 0) dq = condense_layer<MIX>(tk, qk, pk, pt, pt + P.grav * mk, P, A);
+  else dq = condense<MIX>(tk, qk, pk, fpow_<MIX>(pk / P.cp, 1 / P.kappa) * P.p0, P, A);
+#pragma unroll
+  for (int j = 0; j < L; j++)
+    if (j == k) { th[j] = tk; q[j] = qk; }
+  return dq;
+}
+
+// A parcel (th, q) lifted to pressure p, condensing what it cannot hold (air.cu:468-473).
+template <bool MIX>
+__device__ __forceinline__ void parcel(double th, double q, double p, const CP& P, const AP& A, double& thp,
+                                       double& qp) {
+  thp = th;
+  qp = q;
+  const double ex = fpow_<MIX>(p / P.p0, P.kappa);
+  condense<MIX>(thp, qp, P.cp * ex, p, P, A);
+}
+
+struct AirIO {
+  double *theta, *qv, *Ts, *ice;
   double *wx, *olr, *sw_abs, *stored, *rain, *evap, *conv, *tsurf, *rainp, *qocean, *qsw, *flash, *swalb, *firn;
 };
 
-template <int L, bool MIX>
+template <int L, bool MIX, bool MOIST>
 __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, const double* __restrict__ t,
                                                    const double* __restrict__ insol, const double* __restrict__ pos,
                                                    const double* __restrict__ uabs, const double* __restrict__ mu_in,
-                                                   double* __restrict__ theta, double* __restrict__ Ts, double dt,
-                                                   const double* __restrict__ cap, const double* __restrict__ elev,
-                                                   const double* __restrict__ radf, const double* __restrict__ swf,
-                                                   const double* __restrict__ ch, AirOut o) {
+                                                   double dt, const double* __restrict__ cap,
+                                                   const double* __restrict__ elev, const double* __restrict__ radf,
+                                                   const double* __restrict__ swf, const double* __restrict__ ch,
+                                                   AirIO o) {
   const int i = XTID;
   if (i >= N) return;
   if (elev && A.theta_rise > 0 && elev[i] > 0) P.theta_star += A.theta_rise * elev[i];
   const int top = 0, bot = L - 1;
-  double mu[L], th[L], pb[L], pib[L], Q[L], T[L], eps[L];
+  double mu[L], th[L], q[L], pb[L], pib[L], Q[L], T[L], eps[L];
   double mt = 0;
 #pragma unroll
   for (int k = 0; k < L; k++) {
     mu[k] = mu_in[(size_t)k * N + i];
-    th[k] = theta[(size_t)k * N + i];
+    th[k] = o.theta[(size_t)k * N + i];
+    q[k] = MOIST ? o.qv[(size_t)k * N + i] : 0.0;
     mt += mu[k];
     Q[k] = 0;
   }
   layer_exner<L>(mu, P, pb, pib);
   const double exb = pow(pb[bot] / P.p0, P.kappa);
-  const double m_ice = 0;
+  double m_ice = MOIST ? o.ice[i] : 0.0;
   const double Cw = cap ? cap[i] : A.c_slab;
   const double Cf = o.firn && A.firn > 0 ? A.c_ice * A.rho_firn * A.firn : 0.0;
   double Tfn = Cf > 0 ? o.firn[i] : A.Tf;
-  const double E0 = column_energy<L>(mu, pib, th, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
+  double* Ts = o.Ts;
+  const double E0 = column_energy<L, MOIST>(mu, pib, th, q, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
 #pragma unroll
   for (int k = 0; k < L; k++) T[k] = pib[k] * th[k] / P.cp;
   // the air touching the ground: the bottom mu_bl of the column (bl_bot: the thin bottom layer's share of it); the
   // layers above give their air at their lower edge's theta* where warmer
-  double wbl[L], thb = 0, mb = 0;
+  double wbl[L], thb = 0, qb = 0, mb = 0;
   {
     double need = A.bl_bot > 0 && mu[bot] < P.mu_bl ? fmax(mu[bot], A.bl_bot * P.mu_bl) : P.mu_bl;
 #pragma unroll
@@ -548,10 +657,12 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       wbl[k] = m;
       need -= m;
       thb += m * (A.bl_bot > 0 && k < bot && th[k] > tstar<L>(k, P) ? tstar<L>(k, P) : th[k]);
+      if constexpr (MOIST) qb += m * q[k];
       mb += m;
     }
   }
   thb /= mb;
+  qb /= mb;
   const double ps = pb[bot];
   const double Tsa = thb * exb;
   const bool sfc = A.sfc > 0 && ch;
@@ -569,7 +680,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   if (sfc) bulk = bulk / (1 + bulk * dt / mb);
   const double hs = rough * (A.bulk_c > 0 || sfc ? P.cp * bulk : A.h_sens * (1 + uabs[i] / A.u0));
 
-  // ---- shortwave ----
+  // ---- shortwave; the sea's albedo with the ice's cover ----
   double qrad[L], qet[L], qeb[L];
 #pragma unroll
   for (int k = 0; k < L; k++) qrad[k] = qet[k] = qeb[k] = 0;
@@ -585,7 +696,14 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   } else {
     I = insol[i];
   }
-  const double alb = A.alb;
+  const double dice = m_ice / P.rho_ice;
+  const double fice = MOIST ? fmin(1.0, dice / P.ice_cover) : 0.0;
+  double alb = A.alb;
+  if constexpr (MOIST) {
+    const double warm = fmax(0.0, fmin(1.0, (Tsa - A.Tf + A.alb_ramp) / A.alb_ramp));
+    const double alb_i = A.alb_ice + (A.alb_ice_melt - A.alb_ice) * warm;
+    alb = fice * alb_i + (1 - fice) * A.alb;
+  }
   if (o.swalb) o.swalb[i] = alb;
   double toAir;
   if (swf) {
@@ -624,8 +742,21 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   const double swS = swf ? I * swf[(size_t)L * N + i] : (I - toAir) * Tc * (1 - alb) / (1 - alb * (1 - Tc));
 
   // ---- longwave ----
+  if (MOIST && A.lw_w0 > 0) {
+    double W = 0;
 #pragma unroll
-  for (int k = 0; k < L; k++) eps[k] = omexp<MIX>(tau_p(pb[k], pb[bot], A) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], A));
+    for (int k = 0; k < L; k++) W += mu[k] * (q[k] > 0 ? q[k] : 0.0);
+    const double tv = A.tau0 * (1 - A.tau_lin) * sqrt(W / A.lw_w0);
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double sb = pb[k] / pb[bot], st = k ? pb[k - 1] / pb[bot] : 0.0;
+      eps[k] = omexp<MIX>((A.tau0 * A.tau_lin * (sb - st) + (W > 0 ? tv * mu[k] * (q[k] > 0 ? q[k] : 0.0) / W : 0.0)));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < L; k++)
+      eps[k] = omexp<MIX>(tau_p(pb[k], pb[bot], A) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], A));
+  }
   double ldown = 0;
   constexpr int RW = L + 1;
   if (radf) {
@@ -639,13 +770,32 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       ldown += dn;
     }
   }
-  // a firn layer's heat goes to the slab (a dry column has no ice sheet to hold it)
-  if (Cf > 0 && Tfn != A.Tf) {
+  // ---- the surface: open water at Ts, or under ice thicker than ice_cover a skin balancing conduction through the
+  // ice against the surface's fluxes, capped at freezing ----
+  const double kex = rough * (A.bulk_c > 0 || sfc ? bulk : A.k_evap * (1 + uabs[i] / A.u0));
+  const double qa = qb;
+  const double qfac = P.q_sea > 0 ? P.q_sea : 1.0;
+  const bool skin = MOIST && dice > P.ice_cover;
+  if (Cf > 0 && Tfn != A.Tf) {                       // (a firn layer's heat to the slab: no ice sheet on the sea)
     Ts[i] += Cf * (Tfn - A.Tf) / Cw;
     Tfn = A.Tf;
   }
   const double ts = Ts[i];
-  const double tsk = ts;
+  double tsk = ts, melt_top = 0, cond = 0;
+  if (skin) {
+    const double kc = A.k_ice / dice, tb = ts;
+    double x = fmin(tb, A.Tf);
+    for (int it = 0; it < 8; it++) {
+      const double x3 = x * x * x;
+      const double f = kc * (tb - x) - A.eps_s * A.sigma * x3 * x + swS + ldown - hs * (x - Tsa);
+      x -= f / (-kc - 4 * A.eps_s * A.sigma * x3 - hs);
+    }
+    if (x > A.Tf) x = A.Tf;
+    const double x4 = x * x * x * x;
+    melt_top = kc * (tb - x) - A.eps_s * A.sigma * x4 + swS + ldown - hs * (x - Tsa);
+    tsk = x;
+    cond = kc * (tb - tsk);
+  }
   double up = A.eps_s * A.sigma * tsk * tsk * tsk * tsk;
   if (radf) {
     const double Es = up;
@@ -658,7 +808,8 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double ql = n1e - n0 + (tr1 - tr0) * Es;
       Q[k] += ql;
       const double* re = radf + (size_t)(3 * RW + 3 + 6 * k) * N + i;
-      const double s1 = re[0] - re[N] + re[2 * (size_t)N] * Es, s2 = re[3 * (size_t)N] - re[4 * (size_t)N] + re[5 * (size_t)N] * Es;
+      const double s1 = re[0] - re[N] + re[2 * (size_t)N] * Es;
+      const double s2 = re[3 * (size_t)N] - re[4 * (size_t)N] + re[5 * (size_t)N] * Es;
       qrad[k] += ql;
       qet[k] += s1 - (n0 + tr0 * Es);
       qeb[k] += (n1e + tr1 * Es) - s2;
@@ -687,25 +838,77 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     }
   }
   o.olr[i] = up;
-  // ---- sensible heat into the bottom air; the heating as theta; the slab ----
+  // ---- sensible heat and evaporation into the bottom air; the heating as theta ----
   const double H = hs * (tsk - Tsa);
 #pragma unroll
   for (int k = 0; k < L; k++) Q[k] += H * wbl[k] / mb;
-  const double E = 0;
+  double E = 0;
+  if constexpr (MOIST) {
+    using F = std::conditional_t<MIX, float, double>;
+    const double def = qfac * qsat_a<F>((F)tsk, (F)ps, (F)A.eps, 0.0) - qa;
+    E = def > 0 ? kex * def * (1 - fice) : 0.0;
+#pragma unroll
+    for (int k = 0; k < L; k++) q[k] += E * dt * wbl[k] / mb / mu[k];
+  }
   o.evap[i] = E;
   o.tsurf[i] = tsk;
-  double dth[L];
+  double dth[L], dlat[L];
 #pragma unroll
   for (int k = 0; k < L; k++) {
     dth[k] = Q[k] * dt / (mu[k] * pib[k]);
     th[k] += dth[k];
+    dlat[k] = 0;
   }
-  const double qS = swS + ldown - A.eps_s * A.sigma * tsk * tsk * tsk * tsk - H - P.Lv * E;
-  const double Tn = ts + qS * dt / Cw;
+  // ---- the slab: open water takes the surface's fluxes (evaporation costs it Lv per kg), under ice the conduction
+  // up into the skin; below Tf its heat freezes into ice, ice melts with slab heat above Tf and from the top ----
+  const double qS = skin ? -cond : swS + ldown - A.eps_s * A.sigma * tsk * tsk * tsk * tsk - H - P.Lv * E;
+  double Tn = ts + qS * dt / Cw;
   if (o.qocean) o.qocean[i] = qS;
-  if (o.qsw) o.qsw[i] = swS;
+  if (o.qsw) o.qsw[i] = skin ? 0.0 : swS;
+  if constexpr (MOIST) {
+    m_ice -= melt_top * dt / A.Lf;
+    if (m_ice < 0) {
+      Tn += -m_ice * A.Lf / Cw;
+      m_ice = 0;
+    }
+    if (Tn < A.Tf) {
+      m_ice += (A.Tf - Tn) * Cw / A.Lf;
+      Tn = A.Tf;
+    } else if (m_ice > 0) {
+      const double m = (Tn - A.Tf) * Cw / A.Lf;
+      if (m <= m_ice) {
+        m_ice -= m;
+        Tn = A.Tf;
+      } else {
+        Tn = A.Tf + (m - m_ice) * A.Lf / Cw;
+        m_ice = 0;
+      }
+    }
+    o.ice[i] = m_ice;
+  }
   Ts[i] = Tn;
   o.sw_abs[i] = toAir + swS;
+  // ---- condensation: one-step saturation adjustment per layer, its latent heat heating for the crossing ----
+  double R = 0, Rk0 = 0, Rk1 = 0, Rc = 0;
+  if constexpr (MOIST) {
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      double tk = th[k], qk = q[k];
+      double pt = 0;
+#pragma unroll
+      for (int j = 0; j < k; j++) pt += P.grav * mu[j];
+      const double dq = A.sat_layer > 0
+                            ? condense_layer<MIX>(tk, qk, pib[k], pt, pt + P.grav * mu[k], P, A)
+                            : condense<MIX>(tk, qk, pib[k], fpow_<MIX>(pib[k] / P.cp, 1 / P.kappa) * P.p0, P, A);
+      th[k] = tk;
+      q[k] = qk;
+      dth[k] += P.Lv * dq / pib[k];
+      dlat[k] += P.Lv * dq / pib[k];
+      R += mu[k] * dq;
+      if (k == bot) Rk1 += mu[k] * dq;
+      else Rk0 += mu[k] * dq;
+    }
+  }
   // ---- the crossing: the k-tables' edge sub-levels heat the layers' edges ----
   double et[L], eb[L], wxo[L];
 #pragma unroll
@@ -716,6 +919,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   eb[0] = dth[top];
   if (radf) {
     const bool edges = A.edge_heat > 0;
+    const bool de = A.dry_edge > 0 && L > 2;
     double pt_ = 0;
 #pragma unroll
     for (int k = 0; k < L; k++) {
@@ -724,8 +928,9 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double pit = P.cp * fpow_<MIX>((pt_ + 0.5 * ft * dp) / P.p0, P.kappa);
       const double pibt = P.cp * fpow_<MIX>((pbk - 0.5 * fb * dp) / P.p0, P.kappa);
       const double mean = qrad[k] * dt / (mu[k] * pib[k]);
-      const double dtop = dth[k] - mean + qet[k] * dt / (ft * mu[k] * pit);
+      double dtop = dth[k] - mean + qet[k] * dt / (ft * mu[k] * pit);
       const double dbot = dth[k] - mean + qeb[k] * dt / (fb * mu[k] * pibt);
+      if (de && k == (int)A.edge_k) dtop -= dlat[k];
       if (edges) {
         et[k] = dtop;
         eb[k] = dbot;
@@ -735,19 +940,80 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       pt_ = pbk;
     }
   }
-  crossing<L>(P, mu, th, dth, et, eb, pib, wxo);
+  crossing<L, MOIST>(P, mu, th, q, dth, et, eb, pib, wxo);
+  // ---- moist convection: a parcel of bottom air lifted to the interface above it; buoyant there, it rises through
+  // every interface it stays buoyant at and detrains above the last, its condensate raining; the mass that would
+  // remove the buoyancy at the first interface relaxes over tau_conv ----
+  double mc = 0;
   if (o.flash) o.flash[i] = 0;
-  if (o.conv) o.conv[i] = 0;
+  if (MOIST && P.theta_star > 0) {
+    double pi_ = 0;
+#pragma unroll
+    for (int k = 0; k < bot; k++) pi_ += P.grav * mu[k];
+    const double th0 = th[bot], q0 = q[bot];
+    double thp, qc, thp2, qc2;
+    parcel<MIX>(th0, q0, pi_, P, A, thp, qc);
+    const double d = thp - tstar<L>(bot - 1, P);
+    if (d > 0) {
+      const double dp = 500;
+      parcel<MIX>(th0, q0, pi_ + dp, P, A, thp2, qc2);
+      const double slope = (thp - thp2) / dp;
+      const double maxup = mu[bot] - P.floor_;
+      const double want = slope > 0 ? d / slope / P.grav : maxup;
+      mc = fmin(want, maxup) * omexp<MIX>(dt / A.tau_conv);
+      if (mc > 0) {
+        int kd = bot - 1;
+        double pj = pi_;
+        while (kd > 0) {
+          double mk = 0;
+#pragma unroll
+          for (int j = 0; j < L; j++)
+            if (j == kd) mk = mu[j];
+          pj -= P.grav * mk;
+          double t2, q2;
+          parcel<MIX>(th0, q0, pj, P, A, t2, q2);
+          if (!(t2 > tstar<L>(kd - 1, P))) break;
+          thp = t2;
+          qc = q2;
+          kd--;
+        }
+        R += mc * (q0 - qc);
+        Rc = mc * (q0 - qc);
+        move_mass<L, true>(P, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qc);
+#pragma unroll
+        for (int j = 0; j < L - 1; j++)
+          if (j >= kd) wxo[j] += mc;
+        const double rt = [&] {
+          double r = 0;
+#pragma unroll
+          for (int k = 0; k < L - 1; k++)
+            if (k == kd) r = mu[k] * condense_k<L, MIX>(k, mu, th, q, pib, P, A);
+          return r;
+        }();
+        R += rt;
+        Rc += rt;
+      } else {
+        mc = 0;
+      }
+    }
+  }
+  if (o.conv) o.conv[i] = mc;
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) o.wx[(size_t)j * N + i] = wxo[j];
-  o.rain[i] = 0 / dt;
-  if (o.rainp)
-#pragma unroll
-    for (int r = 0; r < 4; r++) o.rainp[(size_t)r * N + i] = 0;
-  const double E1 = column_energy<L>(mu, pib, th, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
+  o.rain[i] = R / dt;
+  if (o.rainp) {
+    o.rainp[i] = Rk0 / dt;
+    o.rainp[(size_t)N + i] = Rk1 / dt;
+    o.rainp[(size_t)2 * N + i] = Rc / dt;
+    o.rainp[(size_t)3 * N + i] = 0 / dt;
+  }
+  const double E1 = column_energy<L, MOIST>(mu, pib, th, q, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
   if (Cf > 0) o.firn[i] = Tfn;
 #pragma unroll
-  for (int k = 0; k < L; k++) theta[(size_t)k * N + i] = th[k];
+  for (int k = 0; k < L; k++) {
+    o.theta[(size_t)k * N + i] = th[k];
+    if constexpr (MOIST) o.qv[(size_t)k * N + i] = q[k];
+  }
   o.stored[i] = E1 - E0;
 }
 
@@ -786,29 +1052,37 @@ int surface_layer(int N, int L, const double* par, const isw_air* air, const dou
 }
 
 int air_column(int N, int L, const double* par, const isw_air* air, int mode, const double* t, const double* insol,
-               const double* pos, const double* uabs, const double* mu, double* theta, double* Ts, double dt,
-               double* wx, double* olr, double* sw_abs, double* stored, double* rain, double* evap, double* conv,
-               double* tsurf, double* rainp, const double* cap, double* qocean, double* qsw, const double* elev,
-               double* flash, const double* radf, const double* swf, double* swalb, const double* ch, double* firn,
-               int mixed, hipStream_t st) {
+               const double* pos, const double* uabs, const double* mu, double* theta, double* qv, double* Ts,
+               double* ice, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
+               double* evap, double* conv, double* tsurf, double* rainp, const double* cap, double* qocean,
+               double* qsw, const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
+               const double* ch, double* firn, int mixed, hipStream_t st) {
   const CP P = make_cp(par);
-  const AP A = make_ap(par, air->rd);
-  const AirOut o{wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, qocean, qsw, flash, swalb, firn};
+  const AP A = make_ap(par, air);
+  const bool moist = P.moist > 0;
+  const AirIO o{theta, qv, Ts, ice, wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, qocean, qsw, flash,
+                swalb, firn};
   const int RW = L + 1;
   const double rows_in = 1 + 2.0 * L + 1 + (mode == 0 ? 3 : 1) + (cap ? 1 : 0) + (elev ? 1 : 0) + (ch ? 1 : 0) +
-                         (radf ? 3.0 * RW + 6 * L : 0) + (swf ? 3.0 * L + 1 : 0) + (firn ? 1 : 0);
+                         (radf ? 3.0 * RW + 6 * L : 0) + (swf ? 3.0 * L + 1 : 0) + (firn ? 1 : 0) +
+                         (moist ? L + 1.0 : 0);
   const double rows_out = L + 1 + (L - 1) + 6 + (rainp ? 4 : 0) + (qocean ? 1 : 0) + (qsw ? 1 : 0) + (flash ? 1 : 0) +
-                          (swalb ? 1 : 0) + (firn ? 1 : 0);
+                          (swalb ? 1 : 0) + (firn ? 1 : 0) + (moist ? L + 1.0 : 0);
   const double bytes = 8.0 * N * (rows_in + rows_out);
-#define CALL(LL)                                                                                                   \
-  if (mixed)                                                                                                       \
-    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, true>), grid1(N), dim3(BS), N, P, A, mode, t, insol, pos,     \
-               uabs, mu, theta, Ts, dt, cap, elev, radf, swf, ch, o);                                              \
-  else                                                                                                             \
-    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, false>), grid1(N), dim3(BS), N, P, A, mode, t, insol, pos,    \
-               uabs, mu, theta, Ts, dt, cap, elev, radf, swf, ch, o);
+#define LAUNCH(LL, MX, MO)                                                                                         \
+  ISW_LAUNCH("air_column", bytes, (k_air_column<LL, MX, MO>), grid1(N), dim3(BS), N, P, A, mode, t, insol, pos,     \
+             uabs, mu, dt, cap, elev, radf, swf, ch, o)
+#define CALL(LL)                           \
+  if (mixed) {                             \
+    if (moist) LAUNCH(LL, true, true);     \
+    else LAUNCH(LL, true, false);          \
+  } else {                                 \
+    if (moist) LAUNCH(LL, false, true);    \
+    else LAUNCH(LL, false, false);         \
+  }
   ISW_BY_L(CALL)
 #undef CALL
+#undef LAUNCH
   return (int)hipGetLastError();
 }
 

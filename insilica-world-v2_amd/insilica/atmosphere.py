@@ -4,7 +4,8 @@
 read the parameters by position, so the array `par_array` builds is the reference's `AirPhysics.par` value for value
 (132 parameters, then the interfaces' theta* from the top, padded to LMAX). `AirPhysics` drives the column kernels
 through the C ABI: the surface layer and the wind speed, the Held-Suarez column with its interface crossing, the
-k-table radiation (the longwave's set-up and column pass, then the shortwave) and the device clock.
+radiative air column (dry, or moist over the sea), the k-table radiation (the longwave's set-up and column pass,
+then the shortwave) and the device clock.
 """
 import ctypes
 import math
@@ -106,7 +107,7 @@ class AirPhysics:
     longwave and shortwave), and the device clock. Parameters in the reference's PAR_ORDER array (`par`, host).
 
     forcing: 'held_suarez' steps the Held & Suarez column (air.cu hs_column); 'radiation' the reference's air column
-    (air.cu air_column) for dry air (moist = 0; the moist column is not built here and raises). mean_sun: the
+    (air.cu air_column), dry or moist over the sea without clouds (clouds and land are not built here). mean_sun: the
     annual-mean insolation instead of the diurnal and seasonal sun. longwave='ktables' builds `lw`, `sw` from
     lw_tables, sw_tables (npz paths or dicts). `cloudo` ((4L + 3), N): per layer cover, total cover, the clouds'
     effect at the top, per layer longwave and shortwave in-cloud optical depths and liquid share."""
@@ -184,12 +185,13 @@ class AirPhysics:
                soilcap=None, soil=None, pond=None, snow=None, cw=None, elev=None, firn=None, gx=None):
         """One column step (reference: atmosphere.py:134-158), then the device clock t advances by dt. With forcing
         'held_suarez': the relaxation and the crossing into theta (in place) and self.wx (the mass to move up, for
-        Layers.exchange), self.qnet, self.stored. With forcing 'radiation', the dry column (moist = 0): the sun,
-        the gray or k-table radiation (radf/swf of the last radiate()), the slab surface at Ts (cap its heat capacity
-        per cell when given; qocean, qsw the ocean's forcing), the surface's sensible heat (the surface layer's ch
-        with sfc > 0) and the crossing, into theta, Ts, self.wx, self.olr, self.sw_abs, self.stored, self.rain,
-        self.evap, self.conv, self.tsurf, self.rainp, self.flash (the last five zero for dry air). The moist column
-        (moist > 0) is not in this library and raises."""
+        Layers.exchange), self.qnet, self.stored. With forcing 'radiation': the sun, the gray or k-table radiation
+        (radf/swf of the last radiate()), the slab surface at Ts (cap its heat capacity per cell when given; qocean,
+        qsw the ocean's forcing), the surface's sensible heat (the surface layer's ch with sfc > 0) and the crossing;
+        for moist air (moist > 0; q (L, N) and ice (N) updated in place) evaporation, sea ice, condensation and the
+        parcel plume. Into theta, Ts, self.wx, self.olr, self.sw_abs, self.stored, self.rain, self.evap, self.conv,
+        self.tsurf, self.rainp, self.flash. Clouds (cw), land and the column's gas tracers (gx) are not in this
+        library and raise."""
         from . import _hip
         if self.forcing == 'held_suarez':
             _hip.call('isw_hs_column', self.dg.N, self.L, self._par, self.hs.ctypes.data_as(ctypes.c_void_p),
@@ -197,15 +199,18 @@ class AirPhysics:
                       _hip.ptr(self.qnet), _hip.ptr(self.stored), _hip.stream())
             self.advance_time(t, dt)
             return
-        if self.moist:
-            raise NotImplementedError('the moist air column (moist > 0) is not in this library')
+        if self.moist and (q is None or ice is None):
+            raise ValueError('the moist column needs the vapor q and the sea ice')
+        if (cw is not None and self.p['cloud'] > 0) or land is not None:
+            raise NotImplementedError('the column over land and with clouds is not in this library')
         if gx is not None:
             raise NotImplementedError('the gas tracers carried by the column\'s crossing are not in this library')
         self._speed(u)
         P = _hip.ptr
         lw, sw = self.lw, self.sw
         _hip.call('isw_air_column', self.dg.N, self.L, self._par, ctypes.byref(self.air), self.mode, P(t),
-                  P(self.insol), P(self.pos), P(self.uabs), P(mu), P(theta), None, P(Ts), None, float(dt),
+                  P(self.insol), P(self.pos), P(self.uabs), P(mu), P(theta), P(q) if self.moist else None, P(Ts),
+                  P(ice) if self.moist else None, float(dt),
                   P(self.wx), P(self.olr), P(self.sw_abs), P(self.stored), P(self.rain), P(self.evap), P(self.conv),
                   P(self.tsurf), P(self.rainp), None, P(cap), P(qocean), P(qsw), None, None, None, None, None, None,
                   None, P(elev), P(self.flash), P(lw.radf) if lw is not None else None,

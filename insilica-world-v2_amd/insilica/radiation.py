@@ -140,7 +140,7 @@ class Shortwave:
                  cloud_ssa=None, band_top=None, precision='mixed', device=None):
         """lw: the Longwave whose sub-levels this reads; ice_re: ice clouds' effective radius (m) for their own optics
         (None: the drops'); cloud_ssa, band_top: per band, the drops' albedo and the band's upper edge (cm-1)
-        (default: the reference's ten bands)."""
+        (default: the tables' own, else the reference's ten bands)."""
         t = _load(tables)
         dev = _dev(device)
         self.N, self.L, self.lwm, self.inhom = _n(dg), int(layers), lw, float(inhom)
@@ -153,10 +153,14 @@ class Shortwave:
         sun = np.asarray(t['sun'], float)
         self.sun, self.ray, self.hzf = _f32(sun / sun.sum(), dev), _f32(t['ray'], dev), _f32(t['haze'], dev)
         ng = np.asarray(t['ng'], int)
-        self.cssa = _f32(np.repeat(cloud_ssa or CLOUD_SSA, ng), dev)
+        if cloud_ssa is None:                            # the tables' own bands, else the reference's ten
+            cloud_ssa = t['cloud_ssa'] if 'cloud_ssa' in t else CLOUD_SSA
+        if band_top is None:
+            band_top = t['band_top'] if 'band_top' in t else BAND_TOP
+        self.cssa = _f32(np.repeat(cloud_ssa, ng), dev)
         self.cssa_i = self.g_i = None
         if ice_re is not None:
-            io = ice_optics(ice_re * 1e6, band_top or BAND_TOP)
+            io = ice_optics(ice_re * 1e6, band_top)
             self.cssa_i = _f32(np.repeat([x[0] for x in io], ng), dev)
             self.g_i = _f32(np.repeat([x[1] for x in io], ng), dev)
         self.cia = _f32(t['cia'], dev) if (air or {}).get('cia') else None

@@ -10,7 +10,9 @@ energy back as the air's heat), the land column, and on every `ocean_every`-th t
 
 Presets: 'w2', 'w5' (Williamson 2 and 5, one layer), 'jet' (one 8 km layer in a balanced 20 cos(lat) m/s jet with
 three seeded 100 m bumps, dt 600 s), 'coupled_stack' (three air layers over two ocean layers with a seeded 30% land
-mask and random-phase harmonic perturbations, dt 450 s, 15 fast sub-steps, an hourly ocean step).
+mask and random-phase harmonic perturbations, dt 450 s, 15 fast sub-steps, an hourly ocean step), 'held_suarez' (two
+air layers under Held & Suarez's forcing), 'aqua_dry' and 'aqua' (three air layers over a slab aquaplanet under the
+sun with the radiative column; 'aqua' with vapor, condensation, convection and sea ice).
 
 Ticks are captured as HIP graphs per tick kind (air only / with the ocean step) and replayed (torch.cuda.CUDAGraph);
 `step(graph=False)` launches every kernel directly. Both give bit-identical results.
@@ -31,7 +33,7 @@ from .transport import BETA, Transport
 
 THIN_WIND = 100.0
 GAS_TRACERS = (('co2', 400e-6), ('n2o', 0.32e-6), ('ch4', 1.8e-6))
-PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez', 'aqua_dry')
+PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez', 'aqua_dry', 'aqua')
 
 
 class World:
@@ -45,16 +47,18 @@ class World:
         self.grid = gridmod.for_planet(gridmod.get(n), **self.planet)
         self.dg = dg = DeviceGrid(self.grid, precision)
         g = self.grid
-        self.air = preset in ('coupled_stack', 'held_suarez', 'aqua_dry')
+        self.air = preset in ('coupled_stack', 'held_suarez', 'aqua_dry', 'aqua')
         self.sea = self.physics = None
-        self.Ts = None
+        self.Ts = self.ice = None
         self.rad_every = 0
-        if preset == 'aqua_dry':
-            # three dry air layers at rest over a slab aquaplanet under the diurnal and seasonal sun, gray radiation
-            # (or the k-tables with longwave='ktables'), the bottom layer dragged by the surface (reference preset
-            # 'aqua_dry', cases.py:244-247, 277): the column step then the exchange of the mass it moves
+        if preset in ('aqua_dry', 'aqua'):
+            # three air layers at rest over a slab aquaplanet under the diurnal and seasonal sun, gray radiation (or
+            # the k-tables with longwave='ktables'), the bottom layer dragged by the surface; 'aqua' with vapor and
+            # sea ice (reference presets 'aqua_dry' and 'aqua', cases.py:225-277): the column step then the exchange
+            # of the mass it moves
             from .atmosphere import AirPhysics
-            h, b, u, extra = cases.aqua_dry(g)
+            moist = preset == 'aqua'
+            h, b, u, extra = cases.aqua_air(g) if moist else cases.aqua_dry(g)
             h = h * (GRAV / self.grav)
             self.L = 3
             self.dt = float(P.get('dt', 900.0 * 2.0 ** (5 - n)))
@@ -65,12 +69,15 @@ class World:
                                rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP),
                                air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY), vmix=P.get('vmix', 0.0),
                                thin_wind=P.get('thin_wind', THIN_WIND))
-            self.rows = dg.asarray(extra['theta'])
-            self.theta = self.rows
+            self.rows = dg.asarray(np.concatenate([extra['theta'], extra['q']]) if moist else extra['theta'])
+            self.theta = self.rows[0:3]
+            if moist:
+                self.q = self.rows[3:6]
+                self.ice = dg.zeros(g.N)
             self.Ts = dg.asarray(extra['Ts'])
             self.psi = dg.asarray(np.tile(cases.cosine_bell(g), (3, 1)))      # moved by the exchange only
-            self.tr = Transport(dg, layers=3, beta=P.get('beta', BETA), tracers=1)
-            phys = dict(theta_star=305.0, theta_star2=380.0, floor_=floor, grav=self.grav)
+            self.tr = Transport(dg, layers=3, beta=P.get('beta', BETA), tracers=2 if moist else 1)
+            phys = dict(theta_star=305.0, theta_star2=380.0, floor_=floor, grav=self.grav, moist=1.0 if moist else 0.0)
             if self.grav != GRAV:
                 phys.update(mu_ref=1.0e5 / self.grav, mu_bl=1.0e4 / self.grav)
             phys.update(P.get('physics', {}))
@@ -170,7 +177,8 @@ class World:
         self._stream.wait_stream(torch.cuda.current_stream())
         self.STATE = ('h', 'u', 'M', 'rows') + (('sea.h', 'sea.u', 'sea.M', 'sea.rows', 'sea.Tg')
                                                  if self.sea is not None else ()) + \
-            (('psi', 'time') if self.physics is not None else ()) + (('Ts',) if self.Ts is not None else ())
+            (('psi', 'time') if self.physics is not None else ()) + (('Ts',) if self.Ts is not None else ()) + \
+            (('ice',) if self.ice is not None else ())
 
     # ---- the tick ----
     def _kind(self, tick):
@@ -200,7 +208,9 @@ class World:
             # mass with its theta, momentum and tracer between the layers (reference: world.py:319-344)
             if kind & 2:
                 self.physics.radiate(self.h, self.theta, self.Ts, self.time, self.rad_every * self.dt)
-            self.physics.column(self.h, self.theta, self.Ts, self.u, self.time, self.dt, elev=self.b)
+            moist = self.ice is not None
+            self.physics.column(self.h, self.theta, self.Ts, self.u, self.time, self.dt, elev=self.b,
+                                q=self.q if moist else None, ice=self.ice if moist else None)
             self.core.exchange(self.physics.wx, self.h, self.u, self.M, self.psi, theta=self.theta)
 
     def launch_tick(self, graph=True):
@@ -261,6 +271,10 @@ class World:
         e = self.core.energy(self.h, self.u, self.theta if self.air else None)
         if self.Ts is not None and self.sea is None:
             e += self.physics.p['c_slab'] * csum(self.dg, self.Ts, self.dg.areaCell)
+        if self.ice is not None:                         # the vapor's latent energy, the sea ice's -L_f m
+            A, p = self.dg.areaCell, self.physics.p
+            e += p['Lv'] * sum(csum(self.dg, self.h[k] * self.q[k], A) for k in range(self.L))
+            e -= p['Lf'] * csum(self.dg, self.ice, A)
         return e
 
     def state_arrays(self):
@@ -293,6 +307,8 @@ class World:
             return self.q[layer] * 1000.0
         if name == 'Ts' and self.Ts is not None:
             return self.Ts
+        if name == 'ice' and self.ice is not None:
+            return self.ice
         if name == 'ps' and self.air:
             return self.grav * torch.sum(self.h, dim=0) / 100.0
         if name == 'sst' and self.sea is not None:

@@ -363,15 +363,18 @@ void og_gas_src(int N, int L, int G, const double* src, const double* h, double 
     for (int i = 0; i < N; i++) x[((size_t)g * L + L - 1) * N + i] += src[(size_t)g * N + i] * dt / h[(size_t)(L - 1) * N + i];
 }
 
-/* ---- air_column, the dry path (air.cu:876-1972 with moist = 0) ----
+/* ---- air_column (air.cu:876-1972): the dry column, and the moist one without clouds, land or the mixed layer ----
  * The sun (mode 0: the diurnal and seasonal sun at t; mode 1: insol), the shortwave split (the k-tables' swf, or by
- * the s^4 or tau profile, or by mass), the gray longwave (or the k-tables' radf), the slab surface under the bulk or
- * surface-layer exchange, the heating as theta, and the crossing with the k-tables' edge heating. `mix` selects the
- * reference's mixed build: 1 - e^-x and the edge sub-levels' Exner powers in float32 (air.cu:24-36). */
+ * the s^4 or tau profile, or by mass), the gray longwave (the vapor's part from the column's vapor with lw_w0; or the
+ * k-tables' radf), the slab surface (or, moist, the sea-ice skin) under the bulk or surface-layer exchange, the
+ * evaporation, the heating as theta, the condensation, the crossing with the k-tables' edge heating, and the moist
+ * convection's parcel plume. `mix` selects the reference's mixed build (air.cu:24-36): 1 - e^-x, the exponentials and
+ * powers of the saturation and condensation arithmetic, and that arithmetic itself in float32 (`fast`). */
 enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5, P_U0 = 6, P_ALB = 7, P_C_SLAB = 8,
-       P_SIGMA = 12, P_TILT = 17, P_DAY = 19, P_TAU_LIN = 20, P_LF = 24, P_TF = 25, P_SW_TAU = 34, P_BL_BOT = 54,
-       P_THETA_RISE = 69, P_BULK_C = 74, P_BULK_Z = 76, P_BULK_W = 77, P_EDGE_HEAT = 84, P_SFC = 90, P_EDGE_K = 101,
-       P_FIRN = 120, P_RHO_FIRN = 122, P_C_ICE = 123 };
+       P_SIGMA = 12, P_TILT = 17, P_DAY = 19, P_TAU_LIN = 20, P_LF = 24, P_TF = 25, P_K_EVAP = 26, P_TAU_CONV = 27,
+       P_ALB_ICE = 28, P_K_ICE = 31, P_RV = 32, P_SAT_LAYER = 33, P_SW_TAU = 34, P_ALB_ICE_MELT = 41, P_ALB_RAMP = 45,
+       P_LW_W0 = 48, P_BL_BOT = 54, P_THETA_RISE = 69, P_BULK_C = 74, P_BULK_Z = 76, P_BULK_W = 77, P_EDGE_HEAT = 84,
+       P_SFC = 90, P_DRY_EDGE = 99, P_EDGE_K = 101, P_ICE_AIR = 102, P_FIRN = 120, P_RHO_FIRN = 122, P_C_ICE = 123 };
 
 static double omexp(double x, int mix) { return mix ? (double)-expm1f(-(float)x) : 1 - exp(-(x)); }
 static double fpow_(double x, double y, int mix) { return mix ? (double)powf((float)x, (float)y) : pow(x, y); }
@@ -382,32 +385,128 @@ static double tau_p(double p, double ps, const double* par) {
   return par[P_TAU0] * (par[P_TAU_LIN] * s + (1 - par[P_TAU_LIN]) * s2 * s2);
 }
 
+/* qsat and qsat_a (air.cu:156-171): Tetens over water; with ice_air, below freezing the blend with Tetens over ice
+ * (Murray 1967) by the liquid share (1 at -20 C ... 0). In the mixed build in float32 throughout. */
+static double qsat_x(double T, double p, double eps, double ice, int mix) {
+  if (mix) {
+    const float t = (float)T, pp = (float)p, e = (float)eps;
+    const float w = e * 611.0f * expf(17.27f * (t - 273.15f) / (t - 35.85f)) / pp;
+    if (!(ice > 0) || t >= 273.15f) return w;
+    const float l = fminf(1.0f, fmaxf(0.0f, (t - 253.15f) / 20.0f));
+    const float si = e * 611.0f * expf(21.875f * (t - 273.16f) / (t - 7.66f)) / pp;
+    return l * w + (1 - l) * si;
+  }
+  const double w = eps * 611.0 * exp(17.27 * (T - 273.15) / (T - 35.85)) / p;
+  if (!(ice > 0) || T >= 273.15) return w;
+  const double l = fmin(1.0, fmax(0.0, (T - 253.15) / 20.0));
+  const double si = eps * 611.0 * exp(21.875 * (T - 273.16) / (T - 7.66)) / p;
+  return l * w + (1 - l) * si;
+}
+
+/* condense (air.cu:190-198): saturation adjustment at Exner pib and pressure p in one Newton step. */
+static double condense(double* th, double* q, double pib, double p, const double* par, double eps, int mix) {
+  const double Lv = par[P_LV], cp = par[P_CP], Rv = par[P_RV], ice = par[P_ICE_AIR];
+  double dq;
+  if (mix) {
+    const float T = (float)pib * (float)*th / (float)cp, qs = (float)qsat_x(T, (float)p, eps, ice, 1);
+    if (!(*q > qs)) return 0;
+    dq = (float)(*q - qs) / (1.0f + (float)Lv * (float)Lv * qs / ((float)cp * (float)Rv * T * T));
+  } else {
+    const double T = pib * *th / cp, qs = qsat_x(T, p, eps, ice, 0);
+    if (!(*q > qs)) return 0;
+    dq = (*q - qs) / (1 + Lv * Lv * qs / (cp * Rv * T * T));
+  }
+  *q -= dq;
+  *th += Lv * dq / pib;
+  return dq;
+}
+
+/* condense_layer (air.cu:275-291): the capacity averaged over the layer's pressure span (4-point Gauss-Legendre,
+ * nodes colder than 160 K left out), one Newton step with Clausius-Clapeyron's slope at each node. */
+static double condense_layer(double* th, double* q, double pib, double pt, double pb, const double* par, double eps,
+                             int mix) {
+  static const double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  static const double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  const double Lv = par[P_LV], Rv = par[P_RV], ice = par[P_ICE_AIR], p0 = par[P_P0], kappa = par[P_KAPPA];
+  double dq;
+  if (mix) {
+    float qs = 0, dqs = 0;
+    for (int j = 0; j < 4; j++) {
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
+      const float ex = powf((float)(p / p0), (float)kappa), T = (float)*th * ex;
+      if (T < 160) continue;
+      const float s = (float)qsat_x(T, (float)p, eps, ice, 1);
+      qs += 0.5f * (float)wq[j] * s;
+      dqs += 0.5f * (float)wq[j] * s * (float)Lv / ((float)Rv * T * T) * ex;
+    }
+    if (!(*q > qs)) return 0;
+    dq = (float)(*q - qs) / (1.0f + (float)Lv / (float)pib * dqs);
+  } else {
+    double qs = 0, dqs = 0;
+    for (int j = 0; j < 4; j++) {
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
+      const double ex = pow(p / p0, kappa), T = *th * ex;
+      if (T < 160) continue;
+      const double s = qsat_x(T, p, eps, ice, 0);
+      qs += 0.5 * wq[j] * s;
+      dqs += 0.5 * wq[j] * s * Lv / (Rv * T * T) * ex;
+    }
+    if (!(*q > qs)) return 0;
+    dq = (*q - qs) / (1 + Lv / pib * dqs);
+  }
+  *q -= dq;
+  *th += Lv * dq / pib;
+  return dq;
+}
+
+/* condense_k (air.cu:319-331) with sat_prof = sat_base = 0: the layer's capacity (sat_layer), else v1's rule at the
+ * pressure whose Exner function is the layer's. */
+static double condense_k(int k, const double* mu, double* th, double* q, const double* pib, const double* par,
+                         double eps, int mix) {
+  if (par[P_SAT_LAYER] > 0) {
+    double pt = 0;
+    for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
+    return condense_layer(&th[k], &q[k], pib[k], pt, pt + par[P_GRAV] * mu[k], par, eps, mix);
+  }
+  const double p = fpow_(pib[k] / par[P_CP], 1 / par[P_KAPPA], mix) * par[P_P0];
+  return condense(&th[k], &q[k], pib[k], p, par, eps, mix);
+}
+
+/* parcel (air.cu:468-473): air (th, q) lifted to pressure p, condensing what it cannot hold. */
+static void parcel(double th, double q, double p, const double* par, double eps, int mix, double* thp, double* qp) {
+  *thp = th;
+  *qp = q;
+  const double ex = fpow_(p / par[P_P0], par[P_KAPPA], mix);
+  condense(thp, qp, par[P_CP] * ex, p, par, eps, mix);
+}
+
 void og_air_column(int N, int L, const double* par_in, const og_air* air, int mode, const double* t,
                    const double* insol, const double* pos, const double* uabs, const double* mu_in, double* theta,
-                   double* Ts, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
-                   double* evap, double* conv, double* tsurf, double* rainp, const double* cap, double* qocean,
-                   double* qsw, const double* elev, double* flash, const double* radf, const double* swf,
-                   double* swalb, const double* ch, double* firn, int mix) {
+                   double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,
+                   double* stored, double* rain, double* evap, double* conv, double* tsurf, double* rainp,
+                   const double* cap, double* qocean, double* qsw, const double* elev, double* flash,
+                   const double* radf, const double* swf, double* swalb, const double* ch, double* firn, int mix) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++) {
     double par[132 + OG_LMAX];
     memcpy(par, par_in, sizeof(par));
     if (elev && par[P_THETA_RISE] > 0 && elev[i] > 0) par[P_THETA_STAR] += par[P_THETA_RISE] * elev[i];
     const double grav = par[P_GRAV], cp = par[P_CP], kappa = par[P_KAPPA], p0 = par[P_P0], sigma = par[P_SIGMA];
-    const double eps_s = par[P_EPS_S], Tf = par[P_TF], Lf = par[P_LF];
+    const double eps_s = par[P_EPS_S], Tf = par[P_TF], Lf = par[P_LF], Lv = par[P_LV], aeps = air->eps;
+    const int moist = par[P_MOIST] > 0;
     double mu[OG_LMAX], th[OG_LMAX], q[OG_LMAX], pb[OG_LMAX], pib[OG_LMAX], Q[OG_LMAX], T[OG_LMAX], eps[OG_LMAX];
     double mt = 0;
     for (int k = 0; k < L; k++) {
       mu[k] = mu_in[(size_t)k * N + i];
       th[k] = theta[(size_t)k * N + i];
-      q[k] = 0;
+      q[k] = moist ? qv[(size_t)k * N + i] : 0.0;
       mt += mu[k];
       Q[k] = 0;
     }
     layer_exner(L, mu, par, pb, pib);
     const int top = 0, bot = L - 1;
     const double exb = pow(pb[bot] / p0, kappa);     /* (p_bot / p0)^kappa at the column's lower edge */
-    const double m_ice = 0;
+    double m_ice = moist ? ice[i] : 0.0;
     const double Cw = cap ? cap[i] : par[P_C_SLAB];
     const double Cf = firn && par[P_FIRN] > 0 ? par[P_C_ICE] * par[P_RHO_FIRN] * par[P_FIRN] : 0.0;
     double Tfn = Cf > 0 ? firn[i] : Tf;
@@ -415,7 +514,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     for (int k = 0; k < L; k++) T[k] = pib[k] * th[k] / cp;
     /* the air touching the ground: the bottom mu_bl of the column (or bl_bot's share of it, the thin bottom layer's
      * own mass at least); the layers above give their air at their lower edge's theta* where warmer */
-    double wbl[OG_LMAX], thb = 0, mb = 0;
+    double wbl[OG_LMAX], thb = 0, qb = 0, mb = 0;
     {
       const double mu_bl = par[P_MU_BL], bl_bot = par[P_BL_BOT];
       double need = bl_bot > 0 && mu[bot] < mu_bl ? fmax(mu[bot], bl_bot * mu_bl) : mu_bl;
@@ -424,10 +523,12 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         wbl[k] = m;
         need -= m;
         thb += m * (bl_bot > 0 && k < bot && th[k] > tstar(k, L, par) ? tstar(k, L, par) : th[k]);
+        qb += m * q[k];
         mb += m;
       }
     }
     thb /= mb;
+    qb /= mb;
     const double ps = pb[bot];
     const double Tsa = thb * exb;
     const int sfc = par[P_SFC] > 0 && ch;
@@ -446,7 +547,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     if (sfc) bulk = bulk / (1 + bulk * dt / mb);
     const double hs = rough * (bulk_c > 0 || sfc ? cp * bulk : par[P_H_SENS] * (1 + uabs[i] / par[P_U0]));
 
-    /* shortwave */
+    /* shortwave; the surface's albedo with the sea ice's cover */
     double qrad[OG_LMAX], qet[OG_LMAX], qeb[OG_LMAX];
     for (int k = 0; k < L; k++) qrad[k] = qet[k] = qeb[k] = 0;
     double I;
@@ -461,7 +562,13 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     } else {
       I = insol[i];
     }
-    const double alb = par[P_ALB];                    /* (dry: no ice, no land) */
+    const double dice = m_ice / par[P_RHO_ICE];
+    const double fice = moist ? fmin(1.0, dice / par[P_ICE_COVER]) : 0.0;
+    const double alb0 = par[P_ALB];
+    const double warm = fmax(0.0, fmin(1.0, (Tsa - Tf + par[P_ALB_RAMP]) / par[P_ALB_RAMP]));
+    const double alb_i = par[P_ALB_ICE] + (par[P_ALB_ICE_MELT] - par[P_ALB_ICE]) * warm;
+    const double fsee = fice;
+    const double alb = fsee * alb_i + (1 - fsee) * alb0;
     if (swalb) swalb[i] = alb;
     double toAir;
     if (swf) {
@@ -494,9 +601,21 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     const double Tc = 1;
     const double swS = swf ? I * swf[(size_t)L * N + i] : (I - toAir) * Tc * (1 - alb) / (1 - alb * (1 - Tc));
 
-    /* gray longwave: each layer's emissivity from its share of the optical depth */
-    for (int k = 0; k < L; k++)
-      eps[k] = omexp(tau_p(pb[k], pb[bot], par) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], par), mix);
+    /* longwave: each layer's emissivity from its share of the optical depth (moist with lw_w0: the vapor's part from
+     * the column's own vapor, growing as the square root of its path) */
+    if (moist && par[P_LW_W0] > 0) {
+      const double tau0 = par[P_TAU0], tau_lin = par[P_TAU_LIN];
+      double W = 0;
+      for (int k = 0; k < L; k++) W += mu[k] * (q[k] > 0 ? q[k] : 0.0);
+      const double tv = tau0 * (1 - tau_lin) * sqrt(W / par[P_LW_W0]);
+      for (int k = 0; k < L; k++) {
+        const double sb = pb[k] / pb[bot], st = k ? pb[k - 1] / pb[bot] : 0.0;
+        eps[k] = omexp((tau0 * tau_lin * (sb - st) + (W > 0 ? tv * mu[k] * (q[k] > 0 ? q[k] : 0.0) / W : 0.0)), mix);
+      }
+    } else {
+      for (int k = 0; k < L; k++)
+        eps[k] = omexp(tau_p(pb[k], pb[bot], par) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], par), mix);
+    }
     double ldown = 0;
     const int RW = L + 1;
     if (radf) {
@@ -508,13 +627,32 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         ldown += dn;
       }
     }
-    /* the slab (a firn layer's heat, with no ice sheet under a dry column, goes to the slab) */
-    if (Cf > 0 && Tfn != Tf) {
+    /* the surface: open water at Ts, or under ice thicker than ice_cover a skin balancing conduction through the ice
+     * against sunlight, longwave and sensible heat, capped at freezing */
+    const double kex = rough * (bulk_c > 0 || sfc ? bulk : par[P_K_EVAP] * (1 + uabs[i] / par[P_U0]));
+    const double qa = qb;
+    const double qfac = par[P_Q_SEA] > 0 ? par[P_Q_SEA] : 1.0;
+    const int skin = moist && dice > par[P_ICE_COVER];
+    if (Cf > 0 && Tfn != Tf) {                       /* (a firn layer's heat to the slab: no ice sheet on the sea) */
       Ts[i] += Cf * (Tfn - Tf) / Cw;
       Tfn = Tf;
     }
     const double ts = Ts[i];
-    const double tsk = ts;
+    double tsk = ts, melt_top = 0, cond = 0;
+    if (skin) {
+      const double kc = par[P_K_ICE] / dice, tb = ts;
+      double x = fmin(tb, Tf);
+      for (int it = 0; it < 8; it++) {
+        const double x3 = x * x * x;
+        const double f = kc * (tb - x) - eps_s * sigma * x3 * x + swS + ldown - hs * (x - Tsa);
+        x -= f / (-kc - 4 * eps_s * sigma * x3 - hs);
+      }
+      if (x > Tf) x = Tf;
+      const double x4 = x * x * x * x;
+      melt_top = kc * (tb - x) - eps_s * sigma * x4 + swS + ldown - hs * (x - Tsa);
+      tsk = x;
+      cond = kc * (tb - tsk);
+    }
     double up = eps_s * sigma * tsk * tsk * tsk * tsk;
     if (radf) {
       const double Es = up;
@@ -549,23 +687,64 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       }
     }
     olr[i] = up;
-    /* sensible heat into the bottom air */
+    /* sensible heat and evaporation into the bottom air */
     const double H = hs * (tsk - Tsa);
     for (int k = 0; k < L; k++) Q[k] += H * wbl[k] / mb;
-    const double E = 0;
+    double E = 0;
+    if (moist) {
+      const double def = qfac * qsat_x(tsk, ps, aeps, 0.0, mix) - qa;
+      E = def > 0 ? kex * def * (1 - fice) : 0.0;
+      for (int k = 0; k < L; k++) q[k] += E * dt * wbl[k] / mb / mu[k];
+    }
     evap[i] = E;
     tsurf[i] = tsk;
-    double dth[OG_LMAX];
+    double dth[OG_LMAX], dlat[OG_LMAX];
     for (int k = 0; k < L; k++) {
       dth[k] = Q[k] * dt / (mu[k] * pib[k]);
       th[k] += dth[k];
+      dlat[k] = 0;
     }
-    const double qS = swS + ldown - eps_s * sigma * tsk * tsk * tsk * tsk - H - par[P_LV] * E;
-    const double Tn = ts + qS * dt / Cw;
+    /* the slab: open water takes the surface's fluxes (evaporation costs it Lv per kg); under ice the conduction up
+     * into the skin. Below Tf the slab's heat freezes into ice; ice melts with slab heat above Tf and from the top */
+    double qS;
+    if (skin) qS = -cond;
+    else qS = swS + ldown - eps_s * sigma * tsk * tsk * tsk * tsk - H - Lv * E;
+    double Tn = ts + qS * dt / Cw;
     if (qocean) qocean[i] = qS;
-    if (qsw) qsw[i] = swS;
+    if (qsw) qsw[i] = skin ? 0.0 : swS;
+    if (moist) {
+      m_ice -= melt_top * dt / Lf;
+      if (m_ice < 0) {
+        Tn += -m_ice * Lf / Cw;
+        m_ice = 0;
+      }
+      if (Tn < Tf) {
+        m_ice += (Tf - Tn) * Cw / Lf;
+        Tn = Tf;
+      } else if (m_ice > 0) {
+        const double m = (Tn - Tf) * Cw / Lf;
+        if (m <= m_ice) {
+          m_ice -= m;
+          Tn = Tf;
+        } else {
+          Tn = Tf + (m - m_ice) * Lf / Cw;
+          m_ice = 0;
+        }
+      }
+      ice[i] = m_ice;
+    }
     Ts[i] = Tn;
     sw_abs[i] = toAir + swS;
+    /* condensation: one-step saturation adjustment per layer; its latent heat counts as heating for the crossing */
+    double R = 0, Rk[2] = {0, 0}, Rc = 0;
+    if (moist)
+      for (int k = 0; k < L; k++) {
+        const double dq = condense_k(k, mu, th, q, pib, par, aeps, mix);
+        dth[k] += Lv * dq / pib[k];
+        dlat[k] += Lv * dq / pib[k];
+        R += mu[k] * dq;
+        Rk[k == bot ? 1 : 0] += mu[k] * dq;
+      }
     /* the crossing, with the k-tables' edge sub-levels' own heating at the layers' edges */
     double dtop[OG_LMAX], dbot[OG_LMAX], dtop1[OG_LMAX], wxo[OG_LMAX];
     if (radf) {
@@ -580,6 +759,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         dbot[k] = dth[k] - mean + qeb[k] * dt / (fb * mu[k] * pibt);
         pt_ = pbk;
       }
+      if (par[P_DRY_EDGE] > 0 && L > 2) dtop[(int)par[P_EDGE_K]] -= dlat[(int)par[P_EDGE_K]];
     }
     const int edges = par[P_EDGE_HEAT] > 0;
     if (radf && !edges && L > 1) {
@@ -588,15 +768,64 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     }
     crossing(L, par, mu, th, q, dth, dth[top], pib, wxo,
              radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL);
+    /* moist convection: a parcel of bottom-layer air lifted to the interface above it; buoyant there (theta_p above
+     * theta*), it rises through every interface it stays buoyant at and detrains above the last, its condensate
+     * raining; the mass that would remove the buoyancy at the first interface relaxes over tau_conv */
+    double mc = 0;
     if (flash) flash[i] = 0;
-    if (conv) conv[i] = 0;
+    if (moist && par[P_THETA_STAR] > 0) {
+      double pi_ = 0;
+      for (int k = 0; k < bot; k++) pi_ += grav * mu[k];
+      const double th0 = th[bot], q0 = q[bot];
+      double thp, qc, thp2, qc2;
+      parcel(th0, q0, pi_, par, aeps, mix, &thp, &qc);
+      const double d = thp - tstar(bot - 1, L, par);
+      if (d > 0) {
+        const double dp = 500;
+        parcel(th0, q0, pi_ + dp, par, aeps, mix, &thp2, &qc2);
+        const double slope = (thp - thp2) / dp;
+        const double maxup = mu[bot] - par[P_FLOOR];
+        const double want = slope > 0 ? d / slope / grav : maxup;
+        mc = fmin(want, maxup) * omexp(dt / par[P_TAU_CONV], mix);
+        if (mc > 0) {
+          int kd = bot - 1;
+          double pj = pi_;
+          while (kd > 0) {
+            pj -= grav * mu[kd];
+            double t2, q2;
+            parcel(th0, q0, pj, par, aeps, mix, &t2, &q2);
+            if (!(t2 > tstar(kd - 1, L, par))) break;
+            thp = t2;
+            qc = q2;
+            kd--;
+          }
+          R += mc * (q0 - qc);
+          Rc = mc * (q0 - qc);
+          move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qc);
+          for (int j = kd; j < bot; j++) wxo[j] += mc;
+          const double rt = mu[kd] * condense_k(kd, mu, th, q, pib, par, aeps, mix);
+          R += rt;
+          Rc += rt;
+        } else {
+          mc = 0;
+        }
+      }
+    }
+    if (conv) conv[i] = mc;
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
-    rain[i] = 0 / dt;
-    if (rainp)
-      for (int r = 0; r < 4; r++) rainp[(size_t)r * N + i] = 0;
+    rain[i] = R / dt;
+    if (rainp) {
+      rainp[i] = Rk[0] / dt;
+      rainp[(size_t)N + i] = Rk[1] / dt;
+      rainp[(size_t)2 * N + i] = Rc / dt;
+      rainp[(size_t)3 * N + i] = 0 / dt;
+    }
     const double E1 = column_energy(L, mu, pib, th, q, par) + Cw * Ts[i] - Lf * m_ice + Cf * (Tfn - Tf);
     if (Cf > 0) firn[i] = Tfn;
-    for (int k = 0; k < L; k++) theta[(size_t)k * N + i] = th[k];
+    for (int k = 0; k < L; k++) {
+      theta[(size_t)k * N + i] = th[k];
+      if (moist) qv[(size_t)k * N + i] = q[k];
+    }
     stored[i] = E1 - E0;
   }
 }

@@ -2,7 +2,8 @@
 
 Restates the tick order of the reference's World._tick (src/insilica/world.py:286-358) for the parts this build
 covers (the layered core, FCT transport, the air-sea stress with its heat, the land column, the hourly ocean step
-without its column mixing, the Held-Suarez and the dry gray air column) and the Ocean.step order (src/insilica/ocean.py:238-252), on the CPU oracle. Built from a
+without its column mixing, the Held-Suarez column and the gray radiative air column, dry or moist) and the
+Ocean.step order (src/insilica/ocean.py:238-252), on the CPU oracle. Built from a
 product World's settings and starting state, so that the two can be stepped side by side.
 """
 import numpy as np
@@ -34,7 +35,7 @@ class OracleWorld:
         self.Fm, self.M1 = np.zeros((self.L, og.E)), np.zeros((self.L, og.N))
         self.sea = None
         self.physics = None
-        self.Ts = None
+        self.Ts = self.ice = None
         if getattr(w, 'physics', None) is not None:                 # the Held-Suarez or the dry radiative column
             p = w.physics
             if p.forcing == 'held_suarez':
@@ -45,6 +46,7 @@ class OracleWorld:
                 self.physics = dict(kind='air', par=np.array(p.par), mode=p.mode, pos=_np(p.pos), grid=w.grid,
                                     insol=_np(p.insol) if p.insol is not None else None, b=_np(w.b))
                 self.Ts = _np(w.Ts)
+                self.ice = _np(w.ice) if w.ice is not None else None
             self.psi, self.time = _np(w.psi), float(w.time[0])
         if w.sea is not None:
             s = w.sea
@@ -105,9 +107,13 @@ class OracleWorld:
                 th, wx, _, _ = hs_column(ph['par'], ph['hs'], ph['lat'], self.h, self.rows[0:L], self.dt)
             else:
                 uabs = cell_speed(ph['grid'], self.u[L - 1])
+                moist = self.ice is not None
                 o = air_column(ph['par'], ph['mode'], np.array([self.time]), ph['pos'], uabs, self.h, self.rows[0:L],
-                               self.Ts, self.dt, insol=ph['insol'], elev=ph['b'])
+                               self.Ts, self.dt, insol=ph['insol'], elev=ph['b'],
+                               q=self.rows[L:2 * L] if moist else None, ice=self.ice)
                 th, wx, self.Ts = o['theta'], o['wx'], o['Ts']
+                if moist:
+                    self.rows[L:2 * L], self.ice = o['q'], o['ice']
             self.rows[0:L] = th
             self.time += self.dt
             self.core.exchange(wx[:L - 1], self.h, self.u, self.M, self.psi, theta=self.rows[0:L])
@@ -118,6 +124,8 @@ class OracleWorld:
             out.update(psi=self.psi, time=np.array([self.time]))
         if self.Ts is not None:
             out['Ts'] = self.Ts
+        if getattr(self, 'ice', None) is not None:
+            out['ice'] = self.ice
         if self.sea is not None:
             for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap'):
                 if self.sea[k] is not None:

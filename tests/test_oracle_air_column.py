@@ -266,3 +266,156 @@ def test_mixed_build_close():
     for k in ('theta', 'Ts', 'olr'):
         np.testing.assert_allclose(b[k], a[k], rtol=2e-6)
     assert np.any(b['olr'] != a['olr'])
+
+
+# ---- the moist column (moist = 1; no clouds, no land) ----
+
+def tetens(T, p, eps=0.622):
+    """Saturation specific humidity over water (Tetens, v1's formula)."""
+    return eps * 611.0 * np.exp(17.27 * (T - 273.15) / (T - 35.85)) / p
+
+
+def moist_aqua(N=96, seed=0, rh=(0.1, 0.5, 1.05), warm=0.0, ice_share=0.3):
+    c = aqua(N, seed)
+    c['theta'][2] += warm
+    pt, pb, _ = pib_of(c['mu'])
+    pm = 0.5 * (pt + pb)
+    c['q'] = np.array(rh)[:, None] * tetens(c['theta'] * (pm / P0) ** KAPPA, pm)
+    rng = np.random.default_rng(seed + 100)
+    c['ice'] = np.where(rng.uniform(size=N) < ice_share, rng.uniform(0, 200, N), 0.0)
+    return c
+
+
+def mrun(c, p, **kw):
+    return run(c, p, q=c['q'], ice=c['ice'], **kw)
+
+
+def new_masses(c, wx):
+    mu = c['mu'].copy()
+    mu[:-1] += wx
+    mu[1:] -= wx
+    return mu
+
+
+@pytest.mark.parametrize('opts', [dict(), dict(lw_w0=5.0), dict(sat_layer=0.0), dict(q_sea=0.98, sfc=1.0),
+                                  dict(bulk_c=1.2e-3, bulk_z=50.0)])
+@pytest.mark.parametrize('mix', [False, True])
+def test_moist_energy_and_water_close(opts, mix):
+    """Energy: latent heat moves between the air's vapor and its enthalpy, the slab's heat into ice, nothing leaves
+    but sunlight in and longwave out. Water: the column gains evaporation and loses rain."""
+    c = moist_aqua(rh=(0.1, 0.8, 1.6), warm=18.0)
+    dt = 900.0
+    kw = dict(ch=np.full(96, 0.02)) if opts.get('sfc') else {}
+    o = mrun(c, par3(moist=1.0, **opts), dt=dt, mix=mix, **kw)
+    stored_matches(o, dt)
+    W0 = (c['mu'] * c['q']).sum(0)
+    W1 = (new_masses(c, o['wx']) * o['q']).sum(0)
+    np.testing.assert_allclose(W1 - W0, (o['evap'] - o['rain']) * dt, rtol=0, atol=1e-12 * W0.max())
+    assert o['rain'].max() > 0 and o['conv'].max() > 0 and o['evap'].max() > 0
+    np.testing.assert_allclose(o['rainp'][:3].sum(0) - o['rainp'][2], o['rain'] - o['rainp'][2], rtol=1e-12,
+                               atol=1e-18)
+    assert np.all(o['q'] >= 0) and np.all(o['ice'] >= 0)
+
+
+def test_condensation_saturates_the_layer():
+    """A supersaturated layer (no sun, no longwave, no crossing or plumes) condenses in one Newton step to within a
+    few per mille of its capacity, the layer's saturation averaged over its pressure span (adaptive quadrature here),
+    and its latent heat warms it exactly as its vapor fell: L_v dq = pib dtheta."""
+    from scipy import integrate
+    c = moist_aqua(N=8, seed=3, ice_share=0.0)
+    pt, pb, pib = pib_of(c['mu'])
+
+    def capacity(th, k, i):
+        return integrate.quad(lambda pp: tetens(th * (pp / P0) ** KAPPA, pp), pt[k, i], pb[k, i],
+                              epsrel=1e-12)[0] / (pb[k, i] - pt[k, i])
+    for k in (1, 2):                                     # 12% beyond the layer's capacity
+        c['q'][k] = [1.12 * capacity(c['theta'][k, i], k, i) for i in range(8)]
+    p = par3(moist=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, k_evap=0.0, theta_star=-1.0, theta_star2=-1.0)
+    o = mrun(c, p, mode=1, insol=np.zeros(8))
+    for k in (1, 2):
+        dq = c['q'][k] - o['q'][k]
+        assert np.all(dq > 0)
+        np.testing.assert_allclose(2.5e6 * dq, pib[k] * (o['theta'][k] - c['theta'][k]), rtol=1e-12)
+        for i in range(8):
+            cap = capacity(o['theta'][k, i], k, i)
+            assert o['q'][k, i] == pytest.approx(cap, rel=5e-3)
+            # the Newton step removed nearly all of the excess: what is left is second order in it
+            assert abs(o['q'][k, i] - cap) < 0.05 * (c['q'][k, i] - capacity(c['theta'][k, i], k, i))
+    assert o['rain'] == pytest.approx((c['mu'][1:] * (c['q'][1:] - o['q'][1:])).sum(0) / 900.0, rel=1e-12)
+
+
+def test_evaporation():
+    """Over open water the air touching the sea takes k_evap (1 + |U| / u0) (q_s(Ts, ps) - q_b) kg/m^2/s, the slab
+    pays Lv for it; under ice cover f only the open share 1 - f evaporates."""
+    c = moist_aqua(N=16, seed=5, rh=(0.1, 0.3, 0.5), ice_share=0.0)
+    c['ice'][:8] = np.linspace(0.0, 60.0, 8) * 917.0 / 1000.0          # 0..6 cm of ice: cover 0..1
+    p = par3(moist=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, theta_star=-1.0, theta_star2=-1.0)
+    dt = 600.0
+    o = mrun(c, p, mode=1, insol=np.zeros(16), dt=dt)
+    ps = GRAV * c['mu'].sum(0)
+    fice = np.minimum(1.0, c['ice'] / 917.0 / 0.05)
+    E = 4e-3 * (1 + c['uabs'] / 10.0) * (tetens(c['Ts'], ps) - c['q'][2]) * (1 - fice)
+    thick = c['ice'] / 917.0 > 0.05
+    np.testing.assert_allclose(o['evap'][~thick], E[~thick], rtol=1e-12)
+    np.testing.assert_allclose((o['q'][2] - c['q'][2]) * c['mu'][2], o['evap'] * dt, rtol=1e-10)
+    open_ = (c['ice'] == 0)
+    np.testing.assert_allclose((o['Ts'] - c['Ts'])[open_ & (o['Ts'] > 273.15)] * p[8] / dt,
+                               -2.5e6 * o['evap'][open_ & (o['Ts'] > 273.15)], rtol=1e-8)
+
+
+def test_sea_ice():
+    """A slab below freezing turns its heat below Tf into ice (Cw (Tf - T) / Lf kg/m^2) and stays at Tf; under thick
+    ice the skin's temperature balances conduction k_ice (T_slab - x) / d against the surface's fluxes at x (solved here
+    by Brent's method; transparent air, so no longwave comes down), capped at Tf; the conducted heat freezes ice under
+    it; the ice's albedo covers the sea by the ice's cover."""
+    from scipy import optimize
+    N = 6
+    c = moist_aqua(N=N, seed=8, ice_share=0.0)
+    c['Ts'] = np.array([268.0, 270.0, 272.0, 275.0, 271.5, 273.15])
+    c['ice'] = np.array([0.0, 0.0, 0.0, 0.0, 500.0, 300.0])
+    c['theta'][2] = 250.0
+    p = par3(moist=1.0, tau0=0.0, theta_star=-1.0, theta_star2=-1.0)
+    dt, Cw, Lf, Tf = 900.0, p[8], 3.34e5, 273.15
+    o = mrun(c, p, mode=1, insol=np.full(N, 300.0), dt=dt)
+    stored_matches(o, dt)
+    for i in range(3):                                   # open water below freezing: its heat below Tf is ice
+        assert o['Ts'][i] == Tf
+        assert o['ice'][i] == pytest.approx((Tf - o['tsurf'][i]) * Cw / Lf
+                                            + (o['Ts'][i] - Tf) * 0, rel=0.02)
+    assert o['Ts'][3] > Tf and o['ice'][3] == 0
+    for i in (4, 5):
+        d = c['ice'][i] / 917.0
+        kc, tb = 2.2 / d, c['Ts'][i]
+        ps = GRAV * c['mu'][:, i].sum()
+        Tsa = c['theta'][2, i] * (ps / P0) ** KAPPA
+        alb = 0.7 + (0.5 - 0.7) * min(1.0, max(0.0, (Tsa - Tf + 10.0) / 10.0))
+        toAir = 0.2 * (c['mu'][:, i].sum() / (P0 / GRAV)) * 300.0
+        swS = (300.0 - toAir) * (1 - alb)
+        assert o['sw_abs'][i] == pytest.approx(toAir + swS, rel=1e-12)
+        hs = 10.0 * (1 + c['uabs'][i] / 10.0)
+        f = lambda xx: kc * (tb - xx) - 0.95 * SIGMA * xx ** 4 + swS - hs * (xx - Tsa)
+        xb = min(optimize.brentq(f, 150.0, 400.0, xtol=1e-13), Tf)
+        assert o['tsurf'][i] == pytest.approx(xb, abs=1e-8)
+        assert o['Ts'][i] == Tf
+        # the slab gave kc (tb - x) dt up through the ice and came back to Tf by freezing; the top melted what the
+        # balance left over at Tf
+        melt = f(xb) * dt / Lf
+        Tn = tb - kc * (tb - xb) * dt / Cw
+        assert o['ice'][i] == pytest.approx(c['ice'][i] - melt + (Tf - Tn) * Cw / Lf, rel=1e-9)
+
+
+def test_plume_relaxes_over_tau_conv():
+    """A conditionally unstable column (a warm, moist bottom layer whose lifted parcel is warmer than theta* at its top)
+    sends up mc = m_max (1 - exp(-dt / tau_conv)): halving dt at a long tau_conv halves it, its rain is the parcel's
+    condensate, it detrains where the parcel stops being buoyant."""
+    c = moist_aqua(N=32, seed=9, rh=(0.1, 0.4, 1.0), warm=22.0, ice_share=0.0)
+    p = par3(moist=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, k_evap=0.0, tau_conv=1e7)
+    a = mrun(c, p, mode=1, insol=np.zeros(32), dt=900.0)
+    b = mrun(c, p, mode=1, insol=np.zeros(32), dt=450.0)
+    act = a['conv'] > 0
+    assert act.sum() >= 8
+    ratio = -math.expm1(-900.0 / 1e7) / -math.expm1(-450.0 / 1e7)
+    np.testing.assert_allclose(a['conv'][act], ratio * b['conv'][act], rtol=1e-6)
+    # the plume's mass crossed the interface above the bottom layer
+    assert np.all(a['wx'][1][act] >= a['conv'][act] * (1 - 1e-12))
+    assert np.all(a['rainp'][2][act] > 0)
