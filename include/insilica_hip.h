@@ -46,6 +46,13 @@ typedef struct isw_grid {
   const double *le, *de, *areaCell, *areaV, *fV, *kite, *w, *d2wt;
   const double *rA, *rde, *rAV;
   const float *wf, *d2wtf;
+  /* Patches for the air's fast sub-steps in one launch each (optional: ph_off NULL keeps two launches per sub-step;
+   * no counterpart in the reference). Patch b (of npatch) holds the cells [b pcells, (b + 1) pcells); its halo cells
+   * (the second cells of its edges outside it) ph_cell[ph_off[b] .. ph_off[b + 1]); its edges (those whose first cell
+   * eI is in it) pe_edge[pe_off[b] .. pe_off[b + 1]), each with its cells' slots pe_slot = sI | sJ << 16 (a cell of
+   * the patch at its offset in it, a halo cell at pcells + its index in the halo); pslots: the most slots a patch uses. */
+  int32_t npatch, pcells, pslots, pad_;
+  const int32_t *ph_off, *ph_cell, *pe_off, *pe_edge, *pe_slot;
 } isw_grid;
 
 /* A batch of worlds stepped together on one mesh: the disjoint union of the worlds' meshes (all of one level, so of

@@ -66,8 +66,13 @@ namespace {
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 bool grid_ok(const isw_grid* g) {
-  return g && g->N > 0 && g->E > 0 && g->NV > 0 && g->eI && g->eJ && g->cellEdge && g->cellEdgeAsc && g->cellSign &&
-         g->le && g->de && g->areaCell;
+  if (!(g && g->N > 0 && g->E > 0 && g->NV > 0 && g->eI && g->eJ && g->cellEdge && g->cellEdgeAsc && g->cellSign &&
+        g->le && g->de && g->areaCell))
+    return false;
+  // the fast sub-steps' patches, when given: every table, a patch count that covers the cells, slots for a patch
+  return !g->ph_off || (g->ph_cell && g->pe_off && g->pe_edge && g->pe_slot && g->pcells > 0 &&
+                        g->npatch == (g->N + g->pcells - 1) / g->pcells && g->pslots >= g->pcells &&
+                        g->pslots < (1 << 15));
 }
 
 bool layers_ok(const isw_grid* g, const isw_layers* s) {

@@ -519,6 +519,125 @@ __global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, LP p, in
   (sb.last ? un : uf)[t] = v + sb.tau * (DIVBY(-(pl[g.eJ[e]] - pl[g.eI[e]]), dv.de[e]) + Tp[t] + G[e]);
 }
 
+// One air fast sub-step in one launch, on patches of consecutive (Hilbert-ordered, so compact) cells: each block
+// takes the mass step and Phi* of its patch's cells and of the halo cells across its edges (the halo's recomputed
+// from the same inputs, so every copy is the same to the bit), keeps Phi* in LDS, then steps the wind on the edges
+// whose first cell is in the patch. The arithmetic is k_af_muphi's and k_af_u's in their order. Masses and winds
+// alternate between two buffers (a neighbour block may still read this sub-step's inputs), so in/out never alias.
+// Patch tables (isw_grid): patch b holds cells [b pcells, (b + 1) pcells), its halo cells ph_cell[ph_off[b] ..],
+// its edges pe_edge[pe_off[b] ..] with their cells' LDS slots pe_slot (first | second << 16; halo slots from pcells).
+constexpr int FBS = 256;
+template <int L>
+__global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
+                                                  const double* __restrict__ hedge, const double* __restrict__ uin,
+                                                  const double* __restrict__ mu_src, double* __restrict__ mu_out,
+                                                  const double* __restrict__ S, const double* __restrict__ Tp,
+                                                  const double* __restrict__ G, double* __restrict__ acc,
+                                                  double* __restrict__ uout, double* __restrict__ un) {
+  extern __shared__ double sphi[];                 // [L][pslots]
+  const int N = g.N, E = g.E, PC = g.pcells, PS = g.pslots;
+  const int b = xblock();
+  const int c0 = b * PC, nown = min(PC, N - c0);
+  const int h0 = g.ph_off[b], nh = g.ph_off[b + 1] - h0;
+  const int e0 = g.pe_off[b], ne = g.pe_off[b + 1] - e0;
+  // the wind step's inputs for this thread's first edge, loaded before the mass step so their latency overlaps it
+  // (they do not depend on it); further edges, if a patch has more than FBS, load in the loop below
+  const int q0 = threadIdx.x;
+  int e_ = 0, sl_ = 0;
+  Sub sbe{false, false, 0.0};
+  double v_[L], a_[L], tp_[L], rde_ = 0, G_ = 0;
+  if (q0 < ne) {
+    e_ = g.pe_edge[e0 + q0];
+    sl_ = g.pe_slot[e0 + q0];
+    sbe = sub_at(p, p.we, e_, stage, it, n, tau);
+    if (sbe.active) {
+      rde_ = dv.de[e_];
+      G_ = G[e_];
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        const size_t t = (size_t)l * E + e_;
+        v_[l] = uin[t];
+        a_[l] = it == 0 ? 0.0 : acc[t];
+        tp_[l] = Tp[t];
+      }
+    }
+  }
+  for (int s = threadIdx.x; s < nown + nh; s += FBS) {
+    const int i = s < nown ? c0 + s : g.ph_cell[h0 + s - nown];
+    const int slot = s < nown ? s : PC + (s - nown);
+    const Sub sb = sub_at(p, p.wc, i, stage, it, n, tau);
+    if (!sb.active) continue;
+    int e[6];
+    ring6(g.cellEdge, i, e);
+    double sl[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[i * 6 + m] * g.le[ok0(e[m])];
+    double m1k[L], sak[L];
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double* hl = hedge + (size_t)k * E;
+      const double* ul = uin + (size_t)k * E;
+      double t[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const int ek = ok0(e[m]);
+        t[m] = sl[m] * hl[ek] * ul[ek];
+      }
+      double div = 0;
+#pragma unroll
+      for (int m = 0; m < 6; m++)
+        if (e[m] >= 0) div += t[m];
+      const double m0 = mu_src[(size_t)k * N + i];
+      const double m1 = m0 - DIVBY(sb.tau * div, dv.A[i]);
+      m1k[k] = m1;
+      sak[k] = m1 + p.div_damp * (m1 - m0);
+    }
+    double P[L], Sk[L];
+    double a = 0;
+#pragma unroll
+    for (int m = 0; m < L; m++) {
+      a += sak[m];
+      P[m] = a;
+      Sk[m] = S[(size_t)m * N + i];
+    }
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      double ph = 0;
+#pragma unroll
+      for (int m = L - 1; m >= k; m--) ph += Sk[m] * P[m];
+      sphi[k * PS + slot] = ph;
+      if (s < nown) mu_out[(size_t)k * N + i] = m1k[k];
+    }
+  }
+  __syncthreads();
+  if (q0 < ne && sbe.active) {
+    const int sI = sl_ & 0xffff, sJ = sl_ >> 16;
+    double* out = sbe.last ? un : uout;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const size_t t = (size_t)l * E + e_;
+      acc[t] = it == 0 ? 0.0 + v_[l] : a_[l] + v_[l];
+      out[t] = v_[l] + sbe.tau * (DIVBY(-(sphi[l * PS + sJ] - sphi[l * PS + sI]), rde_) + tp_[l] + G_);
+    }
+  }
+  for (int q = q0 + FBS; q < ne; q += FBS) {
+    const int e = g.pe_edge[e0 + q];
+    const Sub sb = sub_at(p, p.we, e, stage, it, n, tau);
+    if (!sb.active) continue;
+    const int sl = g.pe_slot[e0 + q];
+    const int sI = sl & 0xffff, sJ = sl >> 16;
+    const double rde = dv.de[e], Ge = G[e];
+    double* out = sb.last ? un : uout;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const size_t t = (size_t)l * E + e;
+      const double v = uin[t];
+      acc[t] = it == 0 ? 0.0 + v : acc[t] + v;
+      out[t] = v + sb.tau * (DIVBY(-(sphi[l * PS + sJ] - sphi[l * PS + sI]), rde) + Tp[t] + Ge);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Ocean barotropic sub-steps (layers.cu:366-388) with the wall mask of the reference's lay_mask calls fused in.
 // ---------------------------------------------------------------------------------------------------------------
@@ -1260,7 +1379,40 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     fs.we = s->batch->we;
     fs.winv = s->batch->inv_n;
   }
-  if (s->air) {
+  const size_t patch_lds = g->ph_off ? (size_t)L * g->pslots * sizeof(double) : 0;
+  if (s->air && g->ph_off && patch_lds <= 65536) {
+    // one launch per sub-step on the grid's patches; masses and winds alternate between (muf, mup) and (uf, ufb)
+    const double* uin = u0;
+    const double* mus = h0;
+    for (int it = 0; it < n; it++) {
+      double* uo = (it & 1) ? s->ufb : s->uf;
+      double* mo = (it & 1) ? s->mup : s->muf;
+      // (algorithmic bytes: hedge, uin, mu in and out, S, Tp, G, acc, the wind out, the cell ring and signs, le,
+      // 1/A, de, the patch's edge list and slots; the halo's re-reads and Phi* in LDS are free)
+      const double bytes = 2 * l_ * e_ * D + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
+                           (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D +
+                           e_ * D + 2 * e_ * I;
+#define PATCH(LL)                                                                                                 \
+  ISW_LAUNCH_SHM("af_patch", bytes, k_af_patch<LL>, dim3((unsigned)g->npatch), dim3(FBS), patch_lds, *g, p, dv, si, \
+                 it, n, tau, s->hedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
+      switch (L) {
+        case 1: PATCH(1); break;
+        case 2: PATCH(2); break;
+        case 3: PATCH(3); break;
+        case 4: PATCH(4); break;
+        case 5: PATCH(5); break;
+        case 6: PATCH(6); break;
+        case 7: PATCH(7); break;
+        default: PATCH(8); break;
+      }
+#undef PATCH
+      uin = uo;
+      mus = mo;
+    }
+    fs.air = 1;
+    fs.hedge = s->hedge;
+    fs.acc = s->accL;
+  } else if (s->air) {
     for (int it = 0; it < n; it++) {
       const bool first = it == 0;
       ISW_LAUNCH("af_muphi", 2 * l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 4 * l_ * n_ * D,

@@ -39,12 +39,19 @@ def build_hip(verbose=False):
              ('radiation.hip', 'radiation_mix.o', ['-DISW_FORM_MIXED=1', '-ffp-contract=fast-honor-pragmas']),
              ('column.hip', 'column.o', ['-ffp-contract=off']),
              ('capi.hip', 'capi.o', ['-ffp-contract=off'])]
-    for src, obj, extra in units:
+    # the units compile in parallel (each hipcc is one process; at most ISW_BUILD_JOBS at a time, default 4)
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, int(os.environ.get('ISW_BUILD_JOBS', '4')))
+
+    def one(unit):
+        src, obj, extra = unit
         out = OBJDIR / obj
         _run([HIPCC, '-c', *HIP_FLAGS, *extra, str(CSRC / src), '-o', str(out)])
-        objs.append(str(out))
         if verbose:
             print('compiled', src, '->', obj, flush=True)
+        return str(out)
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(one, units))
     tmp = LIBDIR / 'libinsilica_hip.so.tmp'
     _run([HIPCC, '-shared', f'--offload-arch={ARCH}', '-o', str(tmp), *objs])
     os.replace(tmp, LIBDIR / 'libinsilica_hip.so')

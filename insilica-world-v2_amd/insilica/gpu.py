@@ -30,6 +30,44 @@ def device():
     return torch.device('cuda', torch.cuda.current_device())
 
 
+PATCH_CELLS = 64
+
+
+def fast_patches(grid, pcells=64):
+    """The air fast sub-step's patches (isw_grid's ph_* and pe_* tables): patch b holds cells [b pcells, (b + 1)
+    pcells), its edges are those whose first cell is in it, its halo the second cells of those edges outside it. Each
+    edge's two cells get LDS slots (in the patch: their offset; in the halo: pcells + their index in the sorted halo).
+    Returns None for a mesh with an edge lacking a cell. Every invariant the kernel relies on is checked here."""
+    N, E = int(grid.N), int(grid.E)
+    eI, eJ = np.asarray(grid.eI, np.int64), np.asarray(grid.eJ, np.int64)
+    if E == 0 or eI.min() < 0 or eJ.min() < 0 or eI.max() >= N or eJ.max() >= N:
+        return None
+    nb = (N + pcells - 1) // pcells
+    bI, bJ = eI // pcells, eJ // pcells
+    order = np.lexsort((np.arange(E), bI))
+    pe_off = np.searchsorted(bI[order], np.arange(nb + 1)).astype(np.int64)
+    ob, oJ = bI[order], eJ[order]
+    out = bJ[order] != ob
+    keys = np.unique(ob[out] * N + oJ[out])                          # (patch, halo cell), sorted by patch then cell
+    kb = keys // N
+    ph_off = np.searchsorted(kb, np.arange(nb + 1)).astype(np.int64)
+    ph_cell = keys % N
+    sI = eI[order] - ob * pcells
+    sJ = np.where(out, 0, oJ - ob * pcells)
+    hidx = np.searchsorted(keys, ob * N + oJ)
+    sJ = np.where(out, pcells + hidx - ph_off[ob], sJ)
+    pslots = int(pcells + np.max(np.diff(ph_off)))
+    # the kernel's assumptions: every edge once; its first cell in its patch; its slots in range and naming its cells
+    assert np.array_equal(np.sort(order), np.arange(E))
+    nown = np.minimum(pcells, N - ob * pcells)
+    assert np.all((sI >= 0) & (sI < nown))
+    assert np.all(np.where(out, (sJ >= pcells) & (sJ < pslots), (sJ >= 0) & (sJ < nown)))
+    assert np.all(np.where(out, ph_cell[np.clip(hidx, 0, len(ph_cell) - 1)] == oJ, ob * pcells + sJ == oJ))
+    assert pslots < 1 << 15 and np.all(np.diff(ph_off) >= 0) and ph_off[-1] == len(ph_cell)
+    return dict(npatch=nb, pcells=pcells, pslots=pslots, ph_off=ph_off, ph_cell=ph_cell, pe_off=pe_off,
+                pe_edge=order, pe_slot=sI | (sJ << 16))
+
+
 class DeviceGrid:
     """A Grid's arrays on the GPU: indices int32/int8, geometry float64, reciprocals and FP32 weight copies."""
 
@@ -68,6 +106,12 @@ class DeviceGrid:
             setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._f64p))
         for k in ('wf', 'd2wtf'):
             setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._f32p))
+        pt = fast_patches(grid, PATCH_CELLS)
+        if pt is not None:
+            s.npatch, s.pcells, s.pslots = pt['npatch'], pt['pcells'], pt['pslots']
+            for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):
+                setattr(self, k, put(pt[k], np.int32))
+                setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._i32p))
         self.s = s
         self.ref = ctypes.byref(s)
         self._sum_scratch = None
