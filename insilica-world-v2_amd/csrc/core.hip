@@ -44,6 +44,9 @@ static inline dim3 g2(long n, int y) { return dim3((unsigned)((n + BS - 1) / BS)
 constexpr int BS1 = 64;
 static inline dim3 c1(long n) { return dim3((unsigned)((n + BS1 - 1) / BS1)); }
 static inline dim3 c2(long n, int y) { return dim3((unsigned)((n + BS1 - 1) / BS1), (unsigned)y); }
+// (64, 4) blocks over cells (or edges) x rows: four rows of the same 64 elements per block, their index tables
+// and stencil weights shared in L1
+static inline dim3 c4(long n, int y) { return dim3((unsigned)((n + BS1 - 1) / BS1), (unsigned)((y + 3) / 4)); }
 
 // XCD-aware block order: the dispatcher deals consecutive blocks to the 8 XCDs in turn; renumbered so that each XCD
 // takes one contiguous range of blocks, neighbouring cells (Hilbert-ordered) then gather through the same L2.
@@ -979,13 +982,13 @@ __device__ __forceinline__ double hoflux(const double* Fml, const double* ps, co
   return f * (0.5 * (ps[g.eI[e]] + ps[g.eJ[e]]) - (double)(a + b)) + fabs(f) * ((double)(b - a) * beta);
 }
 
-__global__ __launch_bounds__(BS) void k_tr_td(isw_grid g, int Lm, const double* __restrict__ M0,
+__global__ __launch_bounds__(BS) void k_tr_td(isw_grid g, int Lm, int rows, const double* __restrict__ M0,
                                               const double* __restrict__ psi, const double* __restrict__ Fm,
                                               const double* __restrict__ M1, double* __restrict__ td,
                                               float* __restrict__ pf) {
   const int i = XTID;
-  if (i >= g.N) return;
-  const int y = blockIdx.y, l = y % Lm;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y, l = y % Lm;
+  if (i >= g.N || y >= rows) return;
   const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
   const double* p = psi + (size_t)y * g.N;
   const double* Fml = Fm + (size_t)l * g.E;
@@ -1005,24 +1008,24 @@ __global__ __launch_bounds__(BS) void k_tr_td(isw_grid g, int Lm, const double* 
   if (pf) pf[o] = (float)psi[o];
 }
 
-__global__ __launch_bounds__(BS) void k_tr_hflux(isw_grid g, int Lm, double frac, double beta,
+__global__ __launch_bounds__(BS) void k_tr_hflux(isw_grid g, int Lm, int rows, double frac, double beta,
                                                  const double* __restrict__ src, const fast* __restrict__ srcf,
                                                  const fast* __restrict__ wt, const double* __restrict__ Fm,
                                                  double* __restrict__ Hf) {
   const int e = XTID;
-  if (e >= g.E) return;
-  const int y = blockIdx.y, l = y % Lm;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y, l = y % Lm;
+  if (e >= g.E || y >= rows) return;
   Hf[(size_t)y * g.E + e] =
       frac * hoflux(Fm + (size_t)l * g.E, src + (size_t)y * g.N, srcf + (size_t)y * g.N, wt, g, e, beta);
 }
 
-__global__ __launch_bounds__(BS) void k_tr_stage(isw_grid g, int Lm, double frac, const double* __restrict__ M0,
+__global__ __launch_bounds__(BS) void k_tr_stage(isw_grid g, int Lm, int rows, double frac, const double* __restrict__ M0,
                                                  const double* __restrict__ psi, const double* __restrict__ Hf,
                                                  const double* __restrict__ Fm, double* __restrict__ out,
                                                  float* __restrict__ pf) {
   const int i = XTID;
-  if (i >= g.N) return;
-  const int y = blockIdx.y, l = y % Lm;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y, l = y % Lm;
+  if (i >= g.N || y >= rows) return;
   const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
   const double* Hl = Hf + (size_t)y * g.E;
   const double* Fml = Fm + (size_t)l * g.E;
@@ -1049,25 +1052,25 @@ __global__ __launch_bounds__(BS) void k_tr_stage(isw_grid g, int Lm, double frac
   if (pf) pf[o] = (float)r;
 }
 
-__global__ __launch_bounds__(BS) void k_tr_aflux(isw_grid g, int Lm, double beta, const double* __restrict__ psi,
+__global__ __launch_bounds__(BS) void k_tr_aflux(isw_grid g, int Lm, int rows, double beta, const double* __restrict__ psi,
                                                  const double* __restrict__ p2, const fast* __restrict__ p2f,
                                                  const fast* __restrict__ wt, const double* __restrict__ Fm,
                                                  double* __restrict__ A) {
   const int e = XTID;
-  if (e >= g.E) return;
-  const int y = blockIdx.y, l = y % Lm;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y, l = y % Lm;
+  if (e >= g.E || y >= rows) return;
   const double* Fml = Fm + (size_t)l * g.E;
   A[(size_t)y * g.E + e] = hoflux(Fml, p2 + (size_t)y * g.N, p2f + (size_t)y * g.N, wt, g, e, beta) -
                            upwind(Fml, psi + (size_t)y * g.N, g, e);
 }
 
-__global__ __launch_bounds__(BS) void k_tr_ratios(isw_grid g, int Lm, const double* __restrict__ psi,
+__global__ __launch_bounds__(BS) void k_tr_ratios(isw_grid g, int Lm, int rows, const double* __restrict__ psi,
                                                   const double* __restrict__ td, const double* __restrict__ M1,
                                                   const double* __restrict__ A, double* __restrict__ Rp,
                                                   double* __restrict__ Rm) {
   const int i = XTID;
-  if (i >= g.N) return;
-  const int y = blockIdx.y, l = y % Lm;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y, l = y % Lm;
+  if (i >= g.N || y >= rows) return;
   const double* p = psi + (size_t)y * g.N;
   const double* t = td + (size_t)y * g.N;
   const double* Al = A + (size_t)y * g.E;
@@ -1104,13 +1107,13 @@ __global__ __launch_bounds__(BS) void k_tr_ratios(isw_grid g, int Lm, const doub
   Rm[(size_t)y * g.N + i] = Pm > 0 ? fmin(1.0, qmin / Pm) : 0.0;
 }
 
-__global__ __launch_bounds__(BS) void k_tr_apply(isw_grid g, int Lm, const double* __restrict__ td,
+__global__ __launch_bounds__(BS) void k_tr_apply(isw_grid g, int Lm, int rows, const double* __restrict__ td,
                                                  const double* __restrict__ M1, const double* __restrict__ A,
                                                  const double* __restrict__ Rp, const double* __restrict__ Rm,
                                                  double* __restrict__ out) {
   const int i = XTID;
-  if (i >= g.N) return;
-  const int y = blockIdx.y, l = y % Lm;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y, l = y % Lm;
+  if (i >= g.N || y >= rows) return;
   const size_t o = (size_t)y * g.N + i, om = (size_t)l * g.N + i;
   const double* Al = A + (size_t)y * g.E;
   const double* rp = Rp + (size_t)y * g.N;
@@ -1446,10 +1449,10 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   }
   const double fb = flux_bytes(fs, n_, e_, l_);
   ISW_LAUNCH("scale", fb + 6 * n_ * I + 2 * e_ * I + e_ * D + l_ * n_ * D + n_ * D + l_ * n_ * D,
-             k_scale, c1((long)N * L), BS1, *g, fs, c * dt, s->floor_, h0, s->scale, L);
+             k_scale, g1((long)N * L), BS, *g, fs, c * dt, s->floor_, h0, s->scale, L);
   ISW_LAUNCH("limit_h", fb + 6 * n_ * (I + C) + 2 * e_ * I + e_ * D + 3 * l_ * n_ * D + n_ * D +
                             (Fm ? l_ * e_ * D : 0),
-             k_limit_h, c1((long)N * L), BS1, *g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L);
+             k_limit_h, g1((long)N * L), BS, *g, dv, fs, c * dt, dt, s->scale, h0, hn, Fm, L);
   if (!s->air)
     ISW_LAUNCH("lay_u", 3 * l_ * e_ * D + 2 * e_ * D + (s->wall ? e_ * D : 0),
                k_lay_u, g1((long)E * L), BS, E, L, c * dt, u0, s->Tp, s->ub, s->ub0, s->wall, un);
@@ -1547,7 +1550,7 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
   const double cellE = 6 * n_ * I + 2 * e_ * I;  // cellEdgeAsc, eI/eJ
   const double sten = 8 * e_ * (R + I) + l_ * e_ * D + 2 * e_ * I;
   ISW_LAUNCH("tr_td", 2 * l_ * n_ * D + r_ * n_ * D + l_ * e_ * D + cellE + r_ * n_ * D + mixed * r_ * n_ * 4,
-             k_tr_td, c2(N, rows), BS1, *g, Lm, M0, psi, Fm, M1, tr->td, pf);
+             k_tr_td, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, M0, psi, Fm, M1, tr->td, pf);
   if (order == 1) {
     ISW_LAUNCH("copy", 2 * r_ * n_ * D, k_copy, g1((long)N * rows), BS, (long)N * rows, tr->td, psi);
     return (int)hipGetLastError();
@@ -1562,21 +1565,21 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
   const fast* p2f = tr->p2;
 #endif
   ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_hflux, g2(E, rows), BS, *g, Lm, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf);
+             k_tr_hflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf);
   ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
                              mixed * r_ * n_ * 4,
-             k_tr_stage, c2(N, rows), BS1, *g, Lm, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf);
+             k_tr_stage, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf);
   ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_hflux, g2(E, rows), BS, *g, Lm, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf);
+             k_tr_hflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf);
   ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
                              mixed * r_ * n_ * 4,
-             k_tr_stage, c2(N, rows), BS1, *g, Lm, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
+             k_tr_stage, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
   ISW_LAUNCH("tr_aflux", 2 * r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_aflux, g2(E, rows), BS, *g, Lm, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
+             k_tr_aflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
   ISW_LAUNCH("tr_ratios", 2 * r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 6 * n_ * I + cellE + 2 * r_ * n_ * D,
-             k_tr_ratios, c2(N, rows), BS1, *g, Lm, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm);
+             k_tr_ratios, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm);
   ISW_LAUNCH("tr_apply", r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 2 * r_ * n_ * D + cellE + r_ * n_ * D,
-             k_tr_apply, c2(N, rows), BS1, *g, Lm, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi);
+             k_tr_apply, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, tr->td, M1, tr->A, tr->Rp, tr->Rm, psi);
   return (int)hipGetLastError();
 }
 
