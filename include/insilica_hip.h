@@ -76,6 +76,7 @@ typedef struct isw_batch {
  * Scratch sizes -- (L,N): B, scale, h1, h2, D1, hold, pibar, S, muf, mup, phi; (L,NV): q, Z1;
  * (L,E): Tp, F, u1, u2, L1, L2, hedge, uf, accL; (E): G, He, ubs, ub0, ub, acc, diss; (N): eta0, eta, eta_prev;
  * ufb (L,E) and ubb (E): the second buffers of the fast sub-steps' winds (the sub-steps alternate between two);
+ * ubc (E, nullable): a third barotropic wind buffer, which lets the ocean's sub-steps run in one launch each;
  * pk: 3*L*E doubles (plain form); pkf: 3*L*E floats (mixed form; NULL otherwise).
  * batch: NULL for one world; else the batch the union mesh holds (then nu4 != 0 marks that some world has del^4, and
  * isw_layers_step's nfast is ignored in favour of the batch's per-world sub-steps). */
@@ -86,7 +87,7 @@ typedef struct isw_layers {
   const double *fV, *b, *wall;
   double *Gc;
   double *B, *q, *Tp, *F, *scale, *G, *He, *ubs, *ub0, *ub, *acc, *eta0, *eta, *eta_prev, *h1, *h2, *u1, *u2;
-  double *D1, *Z1, *L1, *L2, *hold, *hedge, *diss, *pibar, *S, *muf, *mup, *phi, *uf, *accL, *pk, *ufb, *ubb;
+  double *D1, *Z1, *L1, *L2, *hold, *hedge, *diss, *pibar, *S, *muf, *mup, *phi, *uf, *accL, *pk, *ufb, *ubb, *ubc;
   float *pkf;
   const isw_batch* batch;
 } isw_layers;

@@ -644,6 +644,86 @@ __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int 
 // ---------------------------------------------------------------------------------------------------------------
 // Ocean barotropic sub-steps (layers.cu:366-388) with the wall mask of the reference's lay_mask calls fused in.
 // ---------------------------------------------------------------------------------------------------------------
+// One barotropic sub-step in one launch on the grid's patches (as k_af_patch for the air): the surface height of
+// the patch's and halo cells, its potential Gc (eta + gamma (eta - eta_prev)) in LDS, then the barotropic wind on
+// the patch's edges. Heights alternate between eta and eta_prev, winds between ubb and ubc; an edge's last sub-step
+// writes ub (what the stage's update reads). Arithmetic as k_bt_eta's and k_bt_u's.
+__global__ __launch_bounds__(FBS) void k_bt_patch(isw_grid g, Div dv, LP p, int stage, int it, int n, double tau,
+                                                  double gamma, const double* __restrict__ He,
+                                                  const double* __restrict__ ub_src,
+                                                  const double* __restrict__ eta_src, double* __restrict__ eta_out,
+                                                  const double* __restrict__ G, const double* __restrict__ Gc,
+                                                  const double* __restrict__ wall, double* __restrict__ acc,
+                                                  double* __restrict__ ub_out, double* __restrict__ ub_fin) {
+  extern __shared__ double sx[];                   // [pslots]
+  const int N = g.N, PC = g.pcells;
+  const int b = xblock();
+  const int c0 = b * PC, nown = min(PC, N - c0);
+  const int h0 = g.ph_off[b], nh = g.ph_off[b + 1] - h0;
+  const int e0 = g.pe_off[b], ne = g.pe_off[b + 1] - e0;
+  const int q0 = threadIdx.x;
+  int e_ = 0, sl_ = 0;
+  Sub sbe{false, false, 0.0};
+  double v_ = 0, a_ = 0, rde_ = 0, G_ = 0, w_ = 1;
+  if (q0 < ne) {
+    e_ = g.pe_edge[e0 + q0];
+    sl_ = g.pe_slot[e0 + q0];
+    sbe = sub_at(p, p.we, e_, stage, it, n, tau);
+    if (sbe.active) {
+      v_ = ub_src[e_];
+      a_ = it == 0 ? 0.0 : acc[e_];
+      rde_ = dv.de[e_];
+      G_ = G[e_];
+      if (wall) w_ = wall[e_];
+    }
+  }
+  for (int s = threadIdx.x; s < nown + nh; s += FBS) {
+    const int i = s < nown ? c0 + s : g.ph_cell[h0 + s - nown];
+    const int slot = s < nown ? s : PC + (s - nown);
+    const Sub sb = sub_at(p, p.wc, i, stage, it, n, tau);
+    if (!sb.active) continue;
+    const double et0 = eta_src[i];
+    int e[6];
+    ring6(g.cellEdge, i, e);
+    double t[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const int ek = ok0(e[m]);
+      t[m] = (double)g.cellSign[i * 6 + m] * g.le[ek] * He[ek] * ub_src[ek];
+    }
+    double div = 0;
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+      if (e[m] >= 0) div += t[m];
+    const double et = et0 - DIVBY(sb.tau * div, dv.A[i]);
+    sx[slot] = Gc[i] * (et + gamma * (et - et0));
+    if (s < nown) eta_out[i] = et;
+  }
+  __syncthreads();
+  for (int q = q0; q < ne; q += FBS) {
+    int e = e_, sl = sl_;
+    Sub sb = sbe;
+    double v = v_, a = a_, rde = rde_, Ge = G_, wl = w_;
+    if (q != q0) {                                 // (a patch with more edges than threads)
+      e = g.pe_edge[e0 + q];
+      sl = g.pe_slot[e0 + q];
+      sb = sub_at(p, p.we, e, stage, it, n, tau);
+      if (!sb.active) continue;
+      v = ub_src[e];
+      a = it == 0 ? 0.0 : acc[e];
+      rde = dv.de[e];
+      Ge = G[e];
+      wl = wall ? wall[e] : 1.0;
+    }
+    if (!sb.active) continue;
+    double an = it == 0 ? 0.0 + v : a + v;
+    double un = v + sb.tau * (DIVBY(-(sx[sl >> 16] - sx[sl & 0xffff]), rde) + Ge);
+    if (wall && wl == 0) { un = 0; an = 0; }
+    (sb.last ? ub_fin : ub_out)[e] = un;
+    acc[e] = an;
+  }
+}
+
 __global__ __launch_bounds__(BS) void k_bt_eta(isw_grid g, Div dv, LP p, int stage, int it, int n, double tau,
                                                const double* __restrict__ He, const double* __restrict__ ub,
                                                const double* eta_src, double* eta, double* __restrict__ eta_prev) {
@@ -1430,6 +1510,24 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     fs.hedge = s->hedge;
     fs.acc = s->accL;
   } else {
+    if (g->ph_off && s->ubc) {
+      // one launch per sub-step on the grid's patches: heights alternate between eta and eta_prev, winds between
+      // ubb and ubc, each edge's last sub-step writes ub
+      const double* ubs_ = s->ub0;
+      const double* ets = s->eta0;
+      for (int it = 0; it < n; it++) {
+        double* uo = (it & 1) ? s->ubc : s->ubb;
+        double* eo = (it & 1) ? s->eta_prev : s->eta;
+        ISW_LAUNCH_SHM("bt_patch", 2 * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 2 * n_ * D + e_ * D + e_ * D +
+                                       n_ * D + (s->wall ? e_ * D : 0) + e_ * D + (it == 0 ? 1 : 2) * e_ * D +
+                                       2 * e_ * I,
+                       k_bt_patch, dim3((unsigned)g->npatch), dim3(FBS), (size_t)g->pslots * sizeof(double), *g, dv,
+                       p, si, it, n, tau, s->div_damp, s->He, ubs_, ets, eo, s->G, s->Gc, s->wall, s->acc, uo,
+                       s->ub);
+        ubs_ = uo;
+        ets = eo;
+      }
+    } else
     for (int it = 0; it < n; it++) {
       const bool first = it == 0;
       ISW_LAUNCH("bt_eta", 2 * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D + 3 * n_ * D,

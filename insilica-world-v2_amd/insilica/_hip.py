@@ -26,7 +26,7 @@ class IswGrid(ctypes.Structure):
 
 LAYER_SCRATCH = ('B', 'q', 'Tp', 'F', 'scale', 'G', 'He', 'ubs', 'ub0', 'ub', 'acc', 'eta0', 'eta', 'eta_prev', 'h1',
                  'h2', 'u1', 'u2', 'D1', 'Z1', 'L1', 'L2', 'hold', 'hedge', 'diss', 'pibar', 'S', 'muf', 'mup', 'phi',
-                 'uf', 'accL', 'pk', 'ufb', 'ubb')
+                 'uf', 'accL', 'pk', 'ufb', 'ubb', 'ubc')
 
 
 class IswBatch(ctypes.Structure):

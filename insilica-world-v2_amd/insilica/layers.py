@@ -49,7 +49,8 @@ class Layers:
         shapes = dict(B=(L, N), q=(L, NV), Tp=(L, E), F=(L, E), scale=(L, N), G=(E,), He=(E,), ubs=(E,), ub0=(E,),
                       ub=(E,), acc=(E,), eta0=(N,), eta=(N,), eta_prev=(N,), h1=(L, N), h2=(L, N), u1=(L, E),
                       u2=(L, E), D1=(L, N), Z1=(L, NV), L1=(L, E), L2=(L, E), hold=(L, N), hedge=(L, E), diss=(E,),
-                      pibar=(L, N), S=(L, N), muf=(L, N), mup=(L, N), phi=(L, N), uf=(L, E), accL=(L, E), ufb=(L, E), ubb=(E,))
+                      pibar=(L, N), S=(L, N), muf=(L, N), mup=(L, N), phi=(L, N), uf=(L, E), accL=(L, E), ufb=(L, E), ubb=(E,),
+                      ubc=(E,))
         for k, shp in shapes.items():
             setattr(self, k, z(*shp))
         self.pk = z(3 * L * E) if not dg.mixed else z(1)
