@@ -53,6 +53,13 @@ typedef struct isw_grid {
    * the patch at its offset in it, a halo cell at pcells + its index in the halo); pslots: the most slots a patch uses. */
   int32_t npatch, pcells, pslots, pad_;
   const int32_t *ph_off, *ph_cell, *pe_off, *pe_edge, *pe_slot;
+  /* Two air fast sub-steps per launch on the same patches (optional: ph2_off NULL keeps one per launch). The first
+   * sub-step's cells: patch, halo and the halo's halo ph2_cell[ph2_off[b] ..] (slots after the halo's); its winds on
+   * every edge of patch and halo: the patch's edges (slots 0 .. ne - 1) and px_edge[px_off[b] ..] (slots from ne),
+   * with their cells' slots px_slot (as pe_slot); pc_eslot[3 pc_off[b] ..]: each patch-then-halo cell's six edge
+   * slots, two per int32 (0xffff: no edge); pslots2, pedges2: the most cell and edge slots a patch uses. */
+  int32_t pslots2, pedges2;
+  const int32_t *ph2_off, *ph2_cell, *px_off, *px_edge, *px_slot, *pc_off, *pc_eslot;
 } isw_grid;
 
 /* A batch of worlds stepped together on one mesh: the disjoint union of the worlds' meshes (all of one level, so of

@@ -70,9 +70,13 @@ bool grid_ok(const isw_grid* g) {
         g->le && g->de && g->areaCell))
     return false;
   // the fast sub-steps' patches, when given: every table, a patch count that covers the cells, slots for a patch
-  return !g->ph_off || (g->ph_cell && g->pe_off && g->pe_edge && g->pe_slot && g->pcells > 0 &&
-                        g->npatch == (g->N + g->pcells - 1) / g->pcells && g->pslots >= g->pcells &&
-                        g->pslots < (1 << 15));
+  if (g->ph_off && !(g->ph_cell && g->pe_off && g->pe_edge && g->pe_slot && g->pcells > 0 &&
+                     g->npatch == (g->N + g->pcells - 1) / g->pcells && g->pslots >= g->pcells &&
+                     g->pslots < (1 << 15)))
+    return false;
+  return !g->ph2_off || (g->ph_off && g->ph2_cell && g->px_off && g->px_edge && g->px_slot && g->pc_off &&
+                         g->pc_eslot && g->pslots2 >= g->pslots && g->pslots2 < (1 << 15) && g->pedges2 > 0 &&
+                         g->pedges2 < 0xffff);
 }
 
 bool layers_ok(const isw_grid* g, const isw_layers* s) {

@@ -644,6 +644,200 @@ __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int 
 // ---------------------------------------------------------------------------------------------------------------
 // Ocean barotropic sub-steps (layers.cu:366-388) with the wall mask of the reference's lay_mask calls fused in.
 // ---------------------------------------------------------------------------------------------------------------
+// Two air fast sub-steps (it, it + 1) in one launch on the grid's patches (tables ph2_*, px_*, pc_*): A) the mass
+// step and Phi* of every cell of the patch's edges' cells' edges (patch, halo, the halo's halo); B) the wind of
+// sub-step it on every edge of patch and halo, in LDS; C) the mass step and Phi* of sub-step it + 1 on patch and
+// halo from B's winds; D) the wind of sub-step it + 1 on the patch's edges. Every value is computed as k_af_muphi and
+// k_af_u compute it, so the result is the same to the bit as two sub-steps of one launch each. The patch's own
+// edges' inputs stay in registers from B to D (one thread per own edge: at most FBS of them).
+template <int L>
+__global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
+                                                   const double* __restrict__ hedge, const double* __restrict__ uin,
+                                                   const double* __restrict__ mu_src, double* __restrict__ mu_out,
+                                                   const double* __restrict__ S, const double* __restrict__ Tp,
+                                                   const double* __restrict__ G, double* __restrict__ acc,
+                                                   double* __restrict__ uout, double* __restrict__ un) {
+  extern __shared__ double sm[];
+  const int N = g.N, E = g.E, PC = g.pcells, PS = g.pslots2, PE = g.pedges2;
+  double* phiA = sm;                               // [L][PS] Phi* of sub-step it
+  double* muA = phiA + L * PS;                     // [L][PS] the masses after it (patch and halo)
+  double* uB = muA + L * PS;                       // [L][PE] the winds after it
+  double* phiC = uB + L * PE;                      // [L][PS] Phi* of sub-step it + 1
+  const int b = xblock();
+  const int c0 = b * PC, nown = min(PC, N - c0);
+  const int h0 = g.ph_off[b], nh = g.ph_off[b + 1] - h0;
+  const int k0 = g.ph2_off[b], nk = g.ph2_off[b + 1] - k0;
+  const int e0 = g.pe_off[b], ne = g.pe_off[b + 1] - e0;
+  const int x0 = g.px_off[b], nx = g.px_off[b + 1] - x0;
+  // this thread's own edge (if any): its inputs for B and D, loaded first
+  const int q0 = threadIdx.x;
+  int eo = 0, slo = 0;
+  Sub s1{false, false, 0.0}, s2{false, false, 0.0};
+  double v_[L], a_[L], tp_[L], rde_ = 0, G_ = 0;
+  if (q0 < ne) {
+    eo = g.pe_edge[e0 + q0];
+    slo = g.pe_slot[e0 + q0];
+    s1 = sub_at(p, p.we, eo, stage, it, n, tau);
+    s2 = sub_at(p, p.we, eo, stage, it + 1, n, tau);
+    if (s1.active) {
+      rde_ = dv.de[eo];
+      G_ = G[eo];
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        const size_t t = (size_t)l * E + eo;
+        v_[l] = uin[t];
+        a_[l] = it == 0 ? 0.0 : acc[t];
+        tp_[l] = Tp[t];
+      }
+    }
+  }
+  // A: masses and Phi* of sub-step it on patch, halo and second halo
+  for (int s = threadIdx.x; s < nown + nh + nk; s += FBS) {
+    const int i = s < nown ? c0 + s : s < nown + nh ? g.ph_cell[h0 + s - nown] : g.ph2_cell[k0 + s - nown - nh];
+    const int slot = s < nown ? s : PC + (s - nown);
+    const Sub sb = sub_at(p, p.wc, i, stage, it, n, tau);
+    if (!sb.active) continue;
+    int e[6];
+    ring6(g.cellEdge, i, e);
+    double sl[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[i * 6 + m] * g.le[ok0(e[m])];
+    double sak[L];
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double* hl = hedge + (size_t)k * E;
+      const double* ul = uin + (size_t)k * E;
+      double t[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const int ek = ok0(e[m]);
+        t[m] = sl[m] * hl[ek] * ul[ek];
+      }
+      double div = 0;
+#pragma unroll
+      for (int m = 0; m < 6; m++)
+        if (e[m] >= 0) div += t[m];
+      const double m0 = mu_src[(size_t)k * N + i];
+      const double m1 = m0 - DIVBY(sb.tau * div, dv.A[i]);
+      muA[k * PS + slot] = m1;
+      sak[k] = m1 + p.div_damp * (m1 - m0);
+    }
+    double P[L], Sk[L];
+    double a = 0;
+#pragma unroll
+    for (int m = 0; m < L; m++) {
+      a += sak[m];
+      P[m] = a;
+      Sk[m] = S[(size_t)m * N + i];
+    }
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      double ph = 0;
+#pragma unroll
+      for (int m = L - 1; m >= k; m--) ph += Sk[m] * P[m];
+      phiA[k * PS + slot] = ph;
+    }
+  }
+  __syncthreads();
+  // B: the winds of sub-step it on the patch's edges (registers) and the halo's (loaded here)
+  double u1_[L], acc1_[L];
+  if (q0 < ne && s1.active) {
+    const int sI = slo & 0xffff, sJ = slo >> 16;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      acc1_[l] = it == 0 ? 0.0 + v_[l] : a_[l] + v_[l];
+      u1_[l] = v_[l] + s1.tau * (DIVBY(-(phiA[l * PS + sJ] - phiA[l * PS + sI]), rde_) + tp_[l] + G_);
+      uB[l * PE + q0] = u1_[l];
+    }
+    if (s1.last) {                                 // (a world whose last sub-step this is: done after B)
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        const size_t t = (size_t)l * E + eo;
+        un[t] = u1_[l];
+        acc[t] = acc1_[l];
+      }
+    }
+  }
+  for (int w = threadIdx.x; w < nx; w += FBS) {
+    const int e = g.px_edge[x0 + w];
+    const Sub sb = sub_at(p, p.we, e, stage, it, n, tau);
+    if (!sb.active) continue;
+    const int sl = g.px_slot[x0 + w];
+    const int sI = sl & 0xffff, sJ = sl >> 16;
+    const double rde = dv.de[e], Ge = G[e];
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const size_t t = (size_t)l * E + e;
+      uB[l * PE + ne + w] =
+          uin[t] + sb.tau * (DIVBY(-(phiA[l * PS + sJ] - phiA[l * PS + sI]), rde) + Tp[t] + Ge);
+    }
+  }
+  __syncthreads();
+  // C: masses and Phi* of sub-step it + 1 on patch and halo, from B's winds
+  const int cb = 3 * g.pc_off[b];
+  for (int s = threadIdx.x; s < nown + nh; s += FBS) {
+    const int i = s < nown ? c0 + s : g.ph_cell[h0 + s - nown];
+    const int slot = s < nown ? s : PC + (s - nown);
+    const Sub sb = sub_at(p, p.wc, i, stage, it + 1, n, tau);
+    if (!sb.active) continue;
+    int e[6];
+    ring6(g.cellEdge, i, e);
+    const int* ps_ = g.pc_eslot + cb + 3 * s;
+    const int pa = ps_[0], pb2 = ps_[1], pcc = ps_[2];
+    const int es[6] = {pa & 0xffff, pa >> 16, pb2 & 0xffff, pb2 >> 16, pcc & 0xffff, pcc >> 16};
+    double sl[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[i * 6 + m] * g.le[ok0(e[m])];
+    double m1k[L], sak[L];
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double* hl = hedge + (size_t)k * E;
+      double t[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) {
+        const int ek = ok0(e[m]);
+        t[m] = sl[m] * hl[ek] * uB[k * PE + (e[m] >= 0 ? es[m] : 0)];
+      }
+      double div = 0;
+#pragma unroll
+      for (int m = 0; m < 6; m++)
+        if (e[m] >= 0) div += t[m];
+      const double m0 = muA[k * PS + slot];
+      const double m1 = m0 - DIVBY(sb.tau * div, dv.A[i]);
+      m1k[k] = m1;
+      sak[k] = m1 + p.div_damp * (m1 - m0);
+    }
+    double P[L], Sk[L];
+    double a = 0;
+#pragma unroll
+    for (int m = 0; m < L; m++) {
+      a += sak[m];
+      P[m] = a;
+      Sk[m] = S[(size_t)m * N + i];
+    }
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      double ph = 0;
+#pragma unroll
+      for (int m = L - 1; m >= k; m--) ph += Sk[m] * P[m];
+      phiC[k * PS + slot] = ph;
+      if (s < nown) mu_out[(size_t)k * N + i] = m1k[k];
+    }
+  }
+  __syncthreads();
+  // D: the winds of sub-step it + 1 on the patch's edges
+  if (q0 < ne && s1.active && s2.active) {
+    const int sI = slo & 0xffff, sJ = slo >> 16;
+    double* out = s2.last ? un : uout;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const size_t t = (size_t)l * E + eo;
+      acc[t] = acc1_[l] + u1_[l];
+      out[t] = u1_[l] + s2.tau * (DIVBY(-(phiC[l * PS + sJ] - phiC[l * PS + sI]), rde_) + tp_[l] + G_);
+    }
+  }
+}
+
 // One barotropic sub-step in one launch on the grid's patches (as k_af_patch for the air): the surface height of
 // the patch's and halo cells, its potential Gc (eta + gamma (eta - eta_prev)) in LDS, then the barotropic wind on
 // the patch's edges. Heights alternate between eta and eta_prev, winds between ubb and ubc; an edge's last sub-step
@@ -1465,11 +1659,46 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   const size_t patch_lds = g->ph_off ? (size_t)L * g->pslots * sizeof(double) : 0;
   if (s->air && g->ph_off && patch_lds <= 65536) {
     // one launch per sub-step on the grid's patches; masses and winds alternate between (muf, mup) and (uf, ufb)
+    // the last stage steps the wind in place (un is u0): where a launch might both read u0 (its first sub-step's
+    // halo) and write un (an element's last sub-step), the sub-steps read a copy of u0 instead (s->F is free here)
     const double* uin = u0;
+    if (un == u0 && (s->batch || n <= 2)) {
+      ISW_LAUNCH("copy", 2 * l_ * e_ * D, k_copy, g1((long)E * L), BS, (long)E * L, u0, s->F);
+      uin = s->F;
+    }
     const double* mus = h0;
-    for (int it = 0; it < n; it++) {
-      double* uo = (it & 1) ? s->ufb : s->uf;
-      double* mo = (it & 1) ? s->mup : s->muf;
+    const size_t pair_lds = g->ph2_off ? (size_t)L * (3 * g->pslots2 + g->pedges2) * sizeof(double) : 0;
+    int it = 0;
+    int flip = 0;
+    if (g->ph2_off && pair_lds <= 65536) {
+      // two sub-steps per launch while two remain; an odd one left runs alone below
+      for (; it + 1 < n; it += 2, flip ^= 1) {
+        double* uo = flip ? s->ufb : s->uf;
+        double* mo = flip ? s->mup : s->muf;
+        const double bytes = 2 * l_ * e_ * D + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
+                             (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D +
+                             e_ * D + 2 * e_ * I;
+#define PAIR(LL)                                                                                                  \
+  ISW_LAUNCH_SHM("af_patch2", bytes, k_af_patch2<LL>, dim3((unsigned)g->npatch), dim3(FBS), pair_lds, *g, p, dv,  \
+                 si, it, n, tau, s->hedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
+        switch (L) {
+          case 1: PAIR(1); break;
+          case 2: PAIR(2); break;
+          case 3: PAIR(3); break;
+          case 4: PAIR(4); break;
+          case 5: PAIR(5); break;
+          case 6: PAIR(6); break;
+          case 7: PAIR(7); break;
+          default: PAIR(8); break;
+        }
+#undef PAIR
+        uin = uo;
+        mus = mo;
+      }
+    }
+    for (; it < n; it++, flip ^= 1) {
+      double* uo = flip ? s->ufb : s->uf;
+      double* mo = flip ? s->mup : s->muf;
       // (algorithmic bytes: hedge, uin, mu in and out, S, Tp, G, acc, the wind out, the cell ring and signs, le,
       // 1/A, de, the patch's edge list and slots; the halo's re-reads and Phi* in LDS are free)
       const double bytes = 2 * l_ * e_ * D + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +

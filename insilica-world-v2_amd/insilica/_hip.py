@@ -21,7 +21,9 @@ class IswGrid(ctypes.Structure):
                 + [(k, _f64p) for k in ('le', 'de', 'areaCell', 'areaV', 'fV', 'kite', 'w', 'd2wt', 'rA', 'rde', 'rAV')]
                 + [(k, _f32p) for k in ('wf', 'd2wtf')]
                 + [(k, ctypes.c_int32) for k in ('npatch', 'pcells', 'pslots', 'pad_')]
-                + [(k, _i32p) for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot')])
+                + [(k, _i32p) for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot')]
+                + [(k, ctypes.c_int32) for k in ('pslots2', 'pedges2')]
+                + [(k, _i32p) for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot')])
 
 
 LAYER_SCRATCH = ('B', 'q', 'Tp', 'F', 'scale', 'G', 'He', 'ubs', 'ub0', 'ub', 'acc', 'eta0', 'eta', 'eta_prev', 'h1',

@@ -31,6 +31,7 @@ def device():
 
 
 PATCH_CELLS = 64
+PATCH_PAIRS = True          # the air's fast sub-steps two per launch where the patches' LDS allows
 
 
 def fast_patches(grid, pcells=64):
@@ -66,6 +67,61 @@ def fast_patches(grid, pcells=64):
     assert pslots < 1 << 15 and np.all(np.diff(ph_off) >= 0) and ph_off[-1] == len(ph_cell)
     return dict(npatch=nb, pcells=pcells, pslots=pslots, ph_off=ph_off, ph_cell=ph_cell, pe_off=pe_off,
                 pe_edge=order, pe_slot=sI | (sJ << 16))
+
+
+def fast_patches2(grid, pt, max_owned=256):
+    """Two air fast sub-steps per launch on the same patches (isw_grid's ph2_*, px_*, pc_* tables): the cells of the
+    first sub-step are the patch, its halo (pt's) and the halo's own halo, the cells of the edges of patch and halo;
+    the winds of the first sub-step are needed on every edge of patch and halo (the patch's own edges first, in pt's
+    order, then the others); the second sub-step's cells are patch and halo again. Cell slots: the patch at its
+    offsets, the halo from pcells, the second halo after it; edge slots: the patch's edges 0 .. ne - 1, the others
+    from ne. pc_eslot: each patch-and-halo cell's six edge slots, two per int32 (0xffff for a missing edge). None
+    when a patch owns more than max_owned edges (one thread per owned edge in the kernel)."""
+    N, PC, nb = int(grid.N), pt['pcells'], pt['npatch']
+    eI, eJ = np.asarray(grid.eI, np.int64), np.asarray(grid.eJ, np.int64)
+    ce = np.asarray(grid.cellEdge, np.int64)
+    ph_off, ph_cell, pe_off, pe_edge = pt['ph_off'], pt['ph_cell'], pt['pe_off'], pt['pe_edge']
+    if np.max(np.diff(pe_off)) > max_owned:
+        return None
+    h2_off, h2_cell, x_off, x_edge, x_slot, c_off, c_slot = [0], [], [0], [], [], [0], []
+    ps2 = pe2 = 0
+    for b in range(nb):
+        c0 = b * PC
+        own = np.arange(c0, min(c0 + PC, N))
+        h1 = ph_cell[ph_off[b]:ph_off[b + 1]]
+        c1 = np.concatenate([own, h1])
+        ep = pe_edge[pe_off[b]:pe_off[b + 1]]
+        ec = np.unique(ce[c1][ce[c1] >= 0])
+        ex = np.setdiff1d(ec, ep, assume_unique=True)
+        assert np.all(np.isin(ep, ec))
+        ca = np.unique(np.concatenate([eI[ec], eJ[ec]]))
+        h2 = np.setdiff1d(ca, c1)
+        # cell slots in the first sub-step's space
+        cells = np.concatenate([own, h1, h2])
+        slots = np.concatenate([own - c0, PC + np.arange(len(h1)), PC + len(h1) + np.arange(len(h2))])
+        srt = np.argsort(cells)
+        cs, ss = cells[srt], slots[srt]
+        slot_of = lambda c: ss[np.searchsorted(cs, c)]
+        assert np.all(cs[np.searchsorted(cs, eI[ex])] == eI[ex]) and np.all(cs[np.searchsorted(cs, eJ[ex])] == eJ[ex])
+        xs = slot_of(eI[ex]) | (slot_of(eJ[ex]) << 16)
+        # edge slots: the patch's edges in order, then the others
+        edges = np.concatenate([ep, ex])
+        esrt = np.argsort(edges)
+        es_e, es_s = edges[esrt], np.arange(len(edges))[esrt]
+        cc = ce[c1]
+        k = np.clip(np.searchsorted(es_e, np.where(cc >= 0, cc, 0)), 0, len(es_e) - 1)
+        assert np.all((cc < 0) | (es_e[k] == cc))
+        sl6 = np.where(cc >= 0, es_s[k], 0xffff)
+        c_slot.append((sl6[:, 0::2] | (sl6[:, 1::2] << 16)).reshape(-1))
+        h2_cell.append(h2); x_edge.append(ex); x_slot.append(xs)
+        h2_off.append(h2_off[-1] + len(h2)); x_off.append(x_off[-1] + len(ex)); c_off.append(c_off[-1] + len(c1))
+        ps2 = max(ps2, PC + len(h1) + len(h2))
+        pe2 = max(pe2, len(edges))
+    cat = lambda v: np.concatenate(v) if v else np.zeros(0, np.int64)
+    out = dict(pslots2=ps2, pedges2=pe2, ph2_off=np.array(h2_off), ph2_cell=cat(h2_cell), px_off=np.array(x_off),
+               px_edge=cat(x_edge), px_slot=cat(x_slot), pc_off=np.array(c_off), pc_eslot=cat(c_slot))
+    assert ps2 < 1 << 15 and pe2 < 0xffff and len(out['pc_eslot']) == 3 * out['pc_off'][-1]
+    return out
 
 
 class DeviceGrid:
@@ -112,6 +168,12 @@ class DeviceGrid:
             for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):
                 setattr(self, k, put(pt[k], np.int32))
                 setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._i32p))
+            p2 = fast_patches2(grid, pt) if PATCH_PAIRS else None
+            if p2 is not None:
+                s.pslots2, s.pedges2 = p2['pslots2'], p2['pedges2']
+                for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot'):
+                    setattr(self, k, put(p2[k], np.int32))
+                    setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._i32p))
         self.s = s
         self.ref = ctypes.byref(s)
         self._sum_scratch = None
