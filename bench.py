@@ -186,17 +186,33 @@ def rad_ops(form):
     return lw, sw
 
 
+def col_line(pm, pd, em, ed, steps):
+    """The moist air column's pass beside the radiation: steps/s in both forms and its algorithmic bytes over its
+    measured duration (an N-sized pass bound by its per-column arithmetic, reported against HBM for scale)."""
+    out = dict(what='air_column, moist over the sea, k-table fluxes, 40962 x 7', steps_per_s=round(steps / em, 1),
+               fp64_steps_per_s=round(steps / ed, 1) if ed else None)
+    for form, pr in (('mixed', pm), ('fp64', pd)):
+        if pr and 'air_column' in pr:
+            v = pr['air_column']
+            us = v['ms'] / v['launches'] * 1e3
+            out[form + '_us'] = round(us, 2)
+            out[form + '_GBs'] = round(v['bytes'] / v['launches'] / (us * 1e-6) / 1e9, 1)
+    return out
+
+
 def radiation_main(a):
     """The column-radiation workload: 40,962 columns x 7 layers (the level-6 mesh's cells), synthetic k-tables with
     16 x 16 longwave and 14 x 16 shortwave g-points; one step = one radiation pass (lw_setup, lw_column, sw_column)
-    over every column, the state resident in HBM. Mixed form as `value`, the FP64 form beside it."""
+    over every column, the state resident in HBM. Mixed form as `value`, the FP64 form beside it. The moist air
+    column's step (column physics with the pass's k-table fluxes) is timed beside it (`column`)."""
+    import ctypes
     import numpy as np
     import torch
     from insilica import _hip
     from insilica import grid as gm
     from insilica.atmosphere import par_array
     from insilica.ktables import column_state, synthetic_lw, synthetic_sw
-    from insilica.radiation import Longwave, Shortwave
+    from insilica.radiation import Longwave, Shortwave, _air
     torch.cuda.set_device(0)
     g = gm.get(6)
     L = 7
@@ -210,36 +226,60 @@ def radiation_main(a):
     t = dev([2.0e6])
     N = g.N
 
+    # the column physics beside it: the moist air column over the sea (the k-tables' fluxes of the pass, the sun,
+    # evaporation, condensation, plumes, the crossing of interfaces between the layers' median theta)
+    col_par = par_array(L, dict(moist=1.0, edge_k=2.0, theta_stars=[620.0, 485.0, 410.0, 360.0, 334.0, 312.0]))
+    rng = np.random.default_rng(a.seed + 7)
+    uabs = dev(rng.uniform(0.0, 15.0, N))
+    air = _air(None)
+
     def build(form):
         lw = Longwave(N, L, par, tables=lwk, precision=form)
         sw = Shortwave(N, L, par, lw, tables=swk, ice_re=30e-6, precision=form, cloud_ssa=swk['cloud_ssa'],
                        band_top=swk['band_top'])
-        return lw, sw
+        z = lambda *sh: torch.zeros(*sh, dtype=torch.float64, device='cuda')
+        cs = dict(theta=th.clone(), q=q.clone(), Ts=Ts.clone(), ice=z(N), wx=z(L - 1, N), rainp=z(4, N),
+                  **{k: z(N) for k in ('olr', 'sw_abs', 'stored', 'rain', 'evap', 'conv', 'tsurf', 'flash')})
+        return lw, sw, cs
 
-    def one(lw, sw):
+    def one(lw, sw, cs=None):
         lw(mu, th, q, cl, Ts)
         sw(0, t, 3600.0, p, alb, cl)
 
-    def timed(form):
-        lw, sw = build(form)
-        for _ in range(a.warmup):
-            one(lw, sw)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            one(lw, sw)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        _hip.timing_start()
-        one(lw, sw)
-        torch.cuda.synchronize()
-        return el, _hip.timing_stop()
+    def column(lw, sw, cs, form):
+        P = _hip.ptr
+        _hip.call('isw_air_column', N, L, col_par.ctypes.data_as(ctypes.c_void_p), ctypes.byref(air), 0, P(t), None,
+                  P(p), P(uabs), P(mu), P(cs['theta']), P(cs['q']), P(cs['Ts']), P(cs['ice']), 450.0, P(cs['wx']),
+                  P(cs['olr']), P(cs['sw_abs']), P(cs['stored']), P(cs['rain']), P(cs['evap']), P(cs['conv']),
+                  P(cs['tsurf']), P(cs['rainp']), None, None, None, None, None, None, None, None, None, None, None,
+                  None, P(cs['flash']), P(lw.radf), P(sw.swf), None, None, None, None, None,
+                  1 if form == 'mixed' else 0, _hip.stream())
 
-    el_m, prof_m = timed('mixed')
-    print(f'mixed: {a.steps} passes in {el_m:.3f} s', file=sys.stderr, flush=True)
-    el_d, prof_d = (None, None) if a.no_fp64 else timed('double')
+    def timed(form):
+        lw, sw, cs = build(form)
+        out = []
+        for f in (one, lambda *x: column(*x, form)):
+            for _ in range(a.warmup):
+                f(lw, sw, cs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                f(lw, sw, cs)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            _hip.timing_start()
+            f(lw, sw, cs)
+            torch.cuda.synchronize()
+            out += [el, _hip.timing_stop()]
+        return out
+
+    el_m, prof_m, elc_m, profc_m = timed('mixed')
+    print(f'mixed: {a.steps} passes in {el_m:.3f} s, {a.steps} column steps in {elc_m:.3f} s', file=sys.stderr,
+          flush=True)
+    el_d, prof_d, elc_d, profc_d = (None,) * 4 if a.no_fp64 else timed('double')
     if el_d:
-        print(f'fp64: {a.steps} passes in {el_d:.3f} s', file=sys.stderr, flush=True)
+        print(f'fp64: {a.steps} passes in {el_d:.3f} s, {a.steps} column steps in {elc_d:.3f} s', file=sys.stderr,
+              flush=True)
     NGl, NGs, NS = lwk['logk'].shape[3], swk['logk'].shape[3], 4 * L
 
     def roof(prof, form):
@@ -273,6 +313,19 @@ def radiation_main(a):
         cpu = dict(value=round(n / N / el, 5), unit='passes/s', cores=cores, kind='port',
                    sample=(f'{n} of the {N} columns (every {N // n}th), ' if n < N else f'all {N} columns, ') +
                           f'one FP64 oracle pass (C, OpenMP), {el:.1f} s' + (', scaled to the whole pass' if n < N else ''))
+        if n == N:
+            from oracle import column as ocol
+            radf = orad.lw_column(n, L, par, lwt, sub, sub_c['Ts'])
+            swf = orad.sw_column(n, L, par, swt, sub, sub_c['alb'], mode=0, tnow=2.0e6, interval=3600.0,
+                                 pos=np.ascontiguousarray(pos), cloudo=sub_c['cloudo'])
+            u_h = uabs.cpu().numpy()
+            t0 = time.perf_counter()
+            reps = 0
+            while reps < 3 or time.perf_counter() - t0 < 2.0:
+                ocol.air_column(col_par, 0, np.array([2.0e6]), pos, u_h, c['mu'], c['theta'], c['Ts'], 450.0,
+                                radf=radf, swf=swf, q=c['q'], ice=np.zeros(N))
+                reps += 1
+            cpu['column_steps_per_s'] = round(reps / (time.perf_counter() - t0), 3)
     line = dict(metric='radiation passes/s: 40962 columns x 7 layers, LW 256 + SW 224 g-points (FP64 vs mixed, 1 GPU)',
                 value=round(a.steps / el_m, 3), unit='passes/s', n_gpus=1, steps=a.steps, warmup=a.warmup,
                 ms_per_step=round(el_m / a.steps * 1e3, 3), higher_is_better=True, scaling='weak', vs_baseline=None,
@@ -284,7 +337,7 @@ def radiation_main(a):
                 fp64_value=round(a.steps / el_d, 3) if el_d else None,
                 mixed_over_fp64=round(el_d / el_m, 3) if el_d else None,
                 roofline=roof(prof_m, 'mixed'), roofline_fp64=roof(prof_d, 'double') if prof_d else None,
-                cpu_baseline=cpu)
+                column=col_line(profc_m, profc_d, elc_m, elc_d, a.steps), cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 
 
