@@ -350,29 +350,24 @@ __global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf,
 
 // Per stage, pass 3 (lay_slow, layers.cu:269-359): the slow momentum tendency of every layer of an edge (W-weight PV
 // flux with the thin-layer cap, -grad B, drag, exact-exponential stress, pibar grad theta for air, del^4), its
-// thickness-weighted column mean G, T' = T - G, H_e and the column wind. (64, L) blocks: thread (x, y) is layer y of
-// one of 64 edges; the column's sums run through LDS in the reference's layer order. The ~10 eoe neighbours' indices
-// and weights are loaded before the sum so that their packed records are gathered together.
-__global__ __launch_bounds__(512) void k_slow(isw_grid g, LP p, Div dv, double dts, const fast* __restrict__ pk,
-                                              const fast* __restrict__ wf, const double* __restrict__ hedge,
-                                              const double* __restrict__ u, const double* __restrict__ B,
-                                              const double* __restrict__ lap4, const double* __restrict__ th,
-                                              const double* __restrict__ pibar, double* __restrict__ Tp,
-                                              double* __restrict__ G, double* __restrict__ He,
-                                              double* __restrict__ ubs) {
-  __shared__ double sT[ISW_LMAX][BS1], sh[ISW_LMAX][BS1], su[ISW_LMAX][BS1], sq[ISW_LMAX][BS1];
-  const int N = g.N, E = g.E, L = p.L;
-  const int j = threadIdx.x, k = threadIdx.y;
-  const int e = xblock() * BS1 + j;
-  const bool live = e < E;
-  const int ec = live ? e : 0;
-  const int eI = g.eI[ec], eJ = g.eJ[ec];
-  const size_t o = (size_t)k * E + ec;
-  const double he = hedge[o];
-  const double ue = u[o];
-  const fast cap = (fast)(p.thin * he), qf = pk[((size_t)ec * L + k) * 3 + 2];
-  fast Q = 0;
-  const int m0 = g.eoeStart[ec], m1 = g.eoeStart[ec + 1];
+// thickness-weighted column mean G, T' = T - G, H_e and the column wind. The ~10 eoe neighbours' indices and
+// weights are loaded before the sum so that their packed records are gathered together.
+// One thread per edge with the layers in registers (L a compile-time count): the edge's neighbour list and weights
+// load once for every layer, each neighbour's L packed records are one contiguous run, and the column's blend and
+// mean need no LDS; the column sums run in the reference's layer order.
+template <int L>
+__global__ __launch_bounds__(BS) void k_slow_e(isw_grid g, LP p, Div dv, double dts, const fast* __restrict__ pk,
+                                               const fast* __restrict__ wf, const double* __restrict__ hedge,
+                                               const double* __restrict__ u, const double* __restrict__ B,
+                                               const double* __restrict__ lap4, const double* __restrict__ th,
+                                               const double* __restrict__ pibar, double* __restrict__ Tp,
+                                               double* __restrict__ G, double* __restrict__ He,
+                                               double* __restrict__ ubs) {
+  const int e = XTID;
+  if (e >= g.E) return;
+  const int N = g.N, E = g.E;
+  const int eI = g.eI[e], eJ = g.eJ[e];
+  const int m0 = g.eoeStart[e], m1 = g.eoeStart[e + 1];
   constexpr int NB = 10;
   int nb[NB];
   fast w[NB];
@@ -382,81 +377,84 @@ __global__ __launch_bounds__(512) void k_slow(isw_grid g, LP p, Div dv, double d
     nb[m] = g.eoe[mm];
     w[m] = wf[mm];
   }
-  fast h2v[NB], u2v[NB], q2v[NB];
+  const double rde = dv.de[e];
+  const double nu4 = lap4 ? nu4_at(p, e) : 0.0;
+  double dk[L], hk[L], uk[L], sqk[L];
 #pragma unroll
-  for (int m = 0; m < NB; m++) {
-    const fast* pp = pk + ((size_t)nb[m] * L + k) * 3;
-    h2v[m] = pp[0]; u2v[m] = pp[1]; q2v[m] = pp[2];
-  }
+  for (int k = 0; k < L; k++) {
+    const size_t o = (size_t)k * E + e;
+    const double he = hedge[o];
+    const double ue = u[o];
+    const fast cap = (fast)(p.thin * he), qf = pk[((size_t)e * L + k) * 3 + 2];
+    fast Q = 0;
 #pragma unroll
-  for (int m = 0; m < NB; m++) {
-    if (m0 + m >= m1) break;
-    fast h2 = h2v[m];
-    if (h2 > cap) h2 = cap;
-    const fast F2 = h2 * u2v[m];
-    Q += w[m] * F2 * (fast)0.5 * (qf + q2v[m]);
-  }
-  for (int m = m0 + NB; m < m1; m++) {             // (meshes with more neighbours per edge than 10)
-    const fast* pp = pk + ((size_t)g.eoe[m] * L + k) * 3;
-    fast h2 = pp[0];
-    if (h2 > cap) h2 = cap;
-    const fast F2 = h2 * pp[1];
-    Q += wf[m] * F2 * (fast)0.5 * (qf + pp[2]);
-  }
-  const double shk = p.shelf;
-  const double sq0 =
-      shk > 0 && he < shk ? (he - 2 * p.floor_ > 0 ? (he - 2 * p.floor_) / (shk - 2 * p.floor_) : 0.0) : 1.0;
-  const double sqv = th ? 1.0 : sq0;
-  const double* Bl = B + (size_t)k * N;
-  double d = Q - DIVBY(sqv * (Bl[eJ] - Bl[eI]), dv.de[ec]) - (p.drag[k] + (1 - sqv) / 21600.0) * ue;
-  const double ck = p.stress[k];
-  if (ck != 0) {
-    const double r = ck / (he > p.floor_ ? he : p.floor_);
-    d = d - ((r * dts > 1e-8) ? (1 - exp(-r * dts)) / dts : r) * ue;
-  }
-  if (th) {
-    const double* tl = th + (size_t)k * N;
-    const double* pl = pibar + (size_t)k * N;
-    d = d + DIVBY(0.5 * (pl[eI] + pl[eJ]) * (tl[eJ] - tl[eI]), dv.de[ec]);
-  }
-  if (lap4) d = d - nu4_at(p, ec) * lap4[o];
-  sT[k][j] = d;
-  sh[k][j] = he;
-  su[k][j] = ue;
-  sq[k][j] = sq0;
-  __syncthreads();
-  if (!live) return;
-  double T = d;
-  if (th && p.shelf > 0 && sq0 < 1) {
-    double wgt = 0, s = 0;
-    for (int m = k - 1; m >= 0; m--) if (sq[m][j] >= 1) { wgt += sh[m][j]; s += sh[m][j] * sT[m][j]; break; }
-    for (int m = k + 1; m < L; m++) if (sq[m][j] >= 1) { wgt += sh[m][j]; s += sh[m][j] * sT[m][j]; break; }
-    if (wgt > 0) T = sq0 * d + (1 - sq0) * s / wgt;
-  }
-  // the column mean takes every layer's (blended) tendency: a blend reads only unblended neighbours, so each
-  // layer's value is recomputed here from LDS in the same way
-  double H = 0, U = 0, GT = 0;
-  for (int m = 0; m < L; m++) {
-    H += sh[m][j];
-    U += sh[m][j] * su[m][j];
-  }
-  for (int m = 0; m < L; m++) {
-    double Tm = sT[m][j];
-    if (th && p.shelf > 0 && sq[m][j] < 1) {
-      double wgt = 0, s = 0;
-      for (int q2 = m - 1; q2 >= 0; q2--) if (sq[q2][j] >= 1) { wgt += sh[q2][j]; s += sh[q2][j] * sT[q2][j]; break; }
-      for (int q2 = m + 1; q2 < L; q2++) if (sq[q2][j] >= 1) { wgt += sh[q2][j]; s += sh[q2][j] * sT[q2][j]; break; }
-      if (wgt > 0) Tm = sq[m][j] * sT[m][j] + (1 - sq[m][j]) * s / wgt;
+    for (int m = 0; m < NB; m++) {
+      if (m0 + m >= m1) break;
+      const fast* pp = pk + ((size_t)nb[m] * L + k) * 3;
+      fast h2 = pp[0];
+      if (h2 > cap) h2 = cap;
+      const fast F2 = h2 * pp[1];
+      Q += w[m] * F2 * (fast)0.5 * (qf + pp[2]);
     }
-    GT += sh[m][j] * Tm;
+    for (int m = m0 + NB; m < m1; m++) {           // (meshes with more neighbours per edge than 10)
+      const fast* pp = pk + ((size_t)g.eoe[m] * L + k) * 3;
+      fast h2 = pp[0];
+      if (h2 > cap) h2 = cap;
+      const fast F2 = h2 * pp[1];
+      Q += wf[m] * F2 * (fast)0.5 * (qf + pp[2]);
+    }
+    const double shk = p.shelf;
+    const double sq0 =
+        shk > 0 && he < shk ? (he - 2 * p.floor_ > 0 ? (he - 2 * p.floor_) / (shk - 2 * p.floor_) : 0.0) : 1.0;
+    const double sqv = th ? 1.0 : sq0;
+    const double* Bl = B + (size_t)k * N;
+    double d = Q - DIVBY(sqv * (Bl[eJ] - Bl[eI]), rde) - (p.drag[k] + (1 - sqv) / 21600.0) * ue;
+    const double ck = p.stress[k];
+    if (ck != 0) {
+      const double r = ck / (he > p.floor_ ? he : p.floor_);
+      d = d - ((r * dts > 1e-8) ? (1 - exp(-r * dts)) / dts : r) * ue;
+    }
+    if (th) {
+      const double* tl = th + (size_t)k * N;
+      const double* pl = pibar + (size_t)k * N;
+      d = d + DIVBY(0.5 * (pl[eI] + pl[eJ]) * (tl[eJ] - tl[eI]), rde);
+    }
+    if (lap4) d = d - nu4 * lap4[o];
+    dk[k] = d;
+    hk[k] = he;
+    uk[k] = ue;
+    sqk[k] = sq0;
+  }
+  // the blend of a thin layer's tendency with its nearest thick neighbours' (shelf), then the column mean
+  const bool blend = th && p.shelf > 0;
+  auto blended = [&](int m) {
+    double Tm = dk[m];
+    if (blend && sqk[m] < 1) {
+      double wgt = 0, s = 0;
+      for (int q2 = m - 1; q2 >= 0; q2--) if (sqk[q2] >= 1) { wgt += hk[q2]; s += hk[q2] * dk[q2]; break; }
+      for (int q2 = m + 1; q2 < L; q2++) if (sqk[q2] >= 1) { wgt += hk[q2]; s += hk[q2] * dk[q2]; break; }
+      if (wgt > 0) Tm = sqk[m] * dk[m] + (1 - sqk[m]) * s / wgt;
+    }
+    return Tm;
+  };
+  double H = 0, U = 0, GT = 0;
+#pragma unroll
+  for (int m = 0; m < L; m++) {
+    H += hk[m];
+    U += hk[m] * uk[m];
+  }
+  double Tb[L];
+#pragma unroll
+  for (int m = 0; m < L; m++) {
+    Tb[m] = blended(m);
+    GT += hk[m] * Tb[m];
   }
   const double gg = GT / H;
-  Tp[o] = T - gg;
-  if (k == 0) {
-    G[e] = gg;
-    He[e] = H;
-    ubs[e] = U / H;
-  }
+#pragma unroll
+  for (int k = 0; k < L; k++) Tp[(size_t)k * E + e] = Tb[k] - gg;
+  G[e] = gg;
+  He[e] = H;
+  ubs[e] = U / H;
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1639,11 +1637,24 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
   ISW_LAUNCH("edge_hq", l_ * n_ * D + l_ * v_ * D + l_ * e_ * D + 4 * e_ * I + l_ * e_ * D + 3 * l_ * e_ * R,
              k_edge_hq, g1((long)E * L), BS, *g, L, s->shelf, hs, s->q, us, s->hedge, pk);
   const double neoe = g->n_eoe;
-  ISW_LAUNCH("slow", 3 * l_ * e_ * R + neoe * (R + I) + (e_ + 1) * I + 2 * l_ * e_ * D + l_ * n_ * D + 2 * e_ * I +
-                         e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + air * 2 * l_ * n_ * D + l_ * e_ * D + 3 * e_ * D,
-             k_slow, c1(E), dim3(BS1, L), *g, p, dv, c * dt, pk, wf, s->hedge, us, s->B, s->nu4 != 0 ? s->L2 : nullptr,
-                                          s->air ? theta : nullptr, s->air ? s->pibar : nullptr, s->Tp, s->G, s->He,
-                                          s->ubs);
+  const double slow_bytes = 3 * l_ * e_ * R + neoe * (R + I) + (e_ + 1) * I + 2 * l_ * e_ * D + l_ * n_ * D +
+                            2 * e_ * I + e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + air * 2 * l_ * n_ * D + l_ * e_ * D +
+                            3 * e_ * D;
+#define SLOW(LL)                                                                                                   \
+  ISW_LAUNCH("slow", slow_bytes, k_slow_e<LL>, g1(E), BS, *g, p, dv, c * dt, pk, wf, s->hedge, us, s->B,             \
+             s->nu4 != 0 ? s->L2 : nullptr, s->air ? theta : nullptr, s->air ? s->pibar : nullptr, s->Tp, s->G, s->He, \
+             s->ubs)
+  switch (L) {
+    case 1: SLOW(1); break;
+    case 2: SLOW(2); break;
+    case 3: SLOW(3); break;
+    case 4: SLOW(4); break;
+    case 5: SLOW(5); break;
+    case 6: SLOW(6); break;
+    case 7: SLOW(7); break;
+    default: SLOW(8); break;
+  }
+#undef SLOW
   int n = (int)std::ceil(c * nfast);
   if (n < 1) n = 1;
   const double tau = c * dt / n;
