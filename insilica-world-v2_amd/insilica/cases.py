@@ -149,6 +149,30 @@ def aqua_air(g):
     return h, np.zeros(g.N), np.zeros((3, g.E)), dict(theta=theta, Ts=Ts, q=q)
 
 
+def aqua_layers(g, stars, edges, q_top=0.1, q_bot=0.7):
+    """The aquaplanet's start in L = len(stars) + 1 layers (reference: cases.py:257-272): stars the interfaces'
+    theta* from the top, edges their starting pressures (Pa); each layer's theta midway between its edges, the top
+    layer's 35 K above its edge and the bottom's 290 - 30 sin^2(lat); vapor from q_top to q_bot of saturation at each
+    layer's mid-pressure down the column; the surface 288 - 40 (sin^2(lat) - 1/3) K, no colder than 274 K."""
+    L = len(stars) + 1
+    s2 = np.sin(g.lat) ** 2
+    e = np.array([0.0, *edges, MU_REF * GRAV])
+    h = np.repeat((np.diff(e) / GRAV)[:, None], g.N, axis=1)
+    th = [np.full(g.N, stars[0] + 35.0)] + [np.full(g.N, 0.5 * (stars[k - 1] + stars[k])) for k in range(1, L - 1)]
+    theta = np.stack(th + [290.0 - 30.0 * s2])
+    Ts = np.maximum(288.0 - 40.0 * (s2 - 1.0 / 3.0), 274.0)
+    pm = (0.5 * (e[:-1] + e[1:]))[:, None]
+    T = theta * (pm / P0) ** KAPPA
+    q = np.linspace(q_top, q_bot, L)[:, None] * (0.622 * 611.0 * np.exp(17.27 * (T - 273.15) / (T - 35.85)) / pm)
+    return h, np.zeros(g.N), np.zeros((L, g.E)), dict(theta=theta, Ts=Ts, q=q)
+
+
+# The aquaplanet in five and seven layers (reference presets 'aqua5', 'aqua7', cases.py:278-279): theta* of the
+# interfaces from the top and their starting pressures
+AQUA_N = {'aqua5': ([400.0, 350.0, 325.0, 305.0], [1.0e4, 2.5e4, 4.5e4, 8.0e4]),
+          'aqua7': ([420.0, 370.0, 345.0, 330.0, 317.0, 305.0], [0.8e4, 1.8e4, 3.0e4, 4.5e4, 6.2e4, 8.0e4])}
+
+
 def aqua_dry(g):
     """The dry aquaplanet's start (reference: cases.py:244-247): the three air layers of aqua_air at rest, the surface
     at 288 - 40 (sin^2(lat) - 1/3) K, no vapor. Returns h (3, N), b, u (3, E), dict(theta, Ts)."""

@@ -12,7 +12,8 @@ Presets: 'w2', 'w5' (Williamson 2 and 5, one layer), 'jet' (one 8 km layer in a 
 three seeded 100 m bumps, dt 600 s), 'coupled_stack' (three air layers over two ocean layers with a seeded 30% land
 mask and random-phase harmonic perturbations, dt 450 s, 15 fast sub-steps, an hourly ocean step), 'held_suarez' (two
 air layers under Held & Suarez's forcing), 'aqua_dry' and 'aqua' (three air layers over a slab aquaplanet under the
-sun with the radiative column; 'aqua' with vapor, condensation, convection and sea ice).
+sun with the radiative column; 'aqua' with vapor, condensation, convection and sea ice; 'aqua5', 'aqua7' the moist
+aquaplanet in five and seven layers).
 
 Ticks are captured as HIP graphs per tick kind (air only / with the ocean step) and replayed (torch.cuda.CUDAGraph);
 `step(graph=False)` launches every kernel directly. Both give bit-identical results.
@@ -33,7 +34,7 @@ from .transport import BETA, Transport
 
 THIN_WIND = 100.0
 GAS_TRACERS = (('co2', 400e-6), ('n2o', 0.32e-6), ('ch4', 1.8e-6))
-PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez', 'aqua_dry', 'aqua')
+PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez', 'aqua_dry', 'aqua', 'aqua5', 'aqua7')
 
 
 class World:
@@ -47,42 +48,46 @@ class World:
         self.grid = gridmod.for_planet(gridmod.get(n), **self.planet)
         self.dg = dg = DeviceGrid(self.grid, precision)
         g = self.grid
-        self.air = preset in ('coupled_stack', 'held_suarez', 'aqua_dry', 'aqua')
+        self.air = preset in ('coupled_stack', 'held_suarez', 'aqua_dry', 'aqua', 'aqua5', 'aqua7')
         self.sea = self.physics = None
         self.Ts = self.ice = None
         self.rad_every = 0
-        if preset in ('aqua_dry', 'aqua'):
+        if preset in ('aqua_dry', 'aqua', 'aqua5', 'aqua7'):
             # three air layers at rest over a slab aquaplanet under the diurnal and seasonal sun, gray radiation (or
             # the k-tables with longwave='ktables'), the bottom layer dragged by the surface; 'aqua' with vapor and
             # sea ice (reference presets 'aqua_dry' and 'aqua', cases.py:225-277): the column step then the exchange
             # of the mass it moves
             from .atmosphere import AirPhysics
-            moist = preset == 'aqua'
-            h, b, u, extra = cases.aqua_air(g) if moist else cases.aqua_dry(g)
+            moist = preset != 'aqua_dry'
+            stars = cases.AQUA_N[preset][0] if preset in cases.AQUA_N else [380.0, 305.0]
+            if preset in cases.AQUA_N:
+                h, b, u, extra = cases.aqua_layers(g, *cases.AQUA_N[preset])
+            else:
+                h, b, u, extra = cases.aqua_air(g) if moist else cases.aqua_dry(g)
             h = h * (GRAV / self.grav)
-            self.L = 3
+            L = self.L = len(stars) + 1
             self.dt = float(P.get('dt', 900.0 * 2.0 ** (5 - n)))
             nu4 = P.get('nu4', visc4(g, P.get('visc4_U', 15.0)))
             floor = P.get('floor', 10.0)
-            self.core = Layers(dg, 3, b=b, grav=self.grav, gprime=[0.0, 0.0], drag=P.get('drag', 0.0),
-                               stress=P.get('stress', [0.0, 0.0, cases.SURFACE_DRAG]), nu4=nu4, floor=floor,
+            self.core = Layers(dg, L, b=b, grav=self.grav, gprime=[0.0] * (L - 1), drag=P.get('drag', 0.0),
+                               stress=P.get('stress', [0.0] * (L - 1) + [cases.SURFACE_DRAG]), nu4=nu4, floor=floor,
                                rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP),
                                air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY), vmix=P.get('vmix', 0.0),
                                thin_wind=P.get('thin_wind', THIN_WIND))
             self.rows = dg.asarray(np.concatenate([extra['theta'], extra['q']]) if moist else extra['theta'])
-            self.theta = self.rows[0:3]
+            self.theta = self.rows[0:L]
             if moist:
-                self.q = self.rows[3:6]
+                self.q = self.rows[L:2 * L]
                 self.ice = dg.zeros(g.N)
             self.Ts = dg.asarray(extra['Ts'])
-            self.psi = dg.asarray(np.tile(cases.cosine_bell(g), (3, 1)))      # moved by the exchange only
-            self.tr = Transport(dg, layers=3, beta=P.get('beta', BETA), tracers=2 if moist else 1)
-            phys = dict(theta_star=305.0, theta_star2=380.0, floor_=floor, grav=self.grav, moist=1.0 if moist else 0.0)
+            self.psi = dg.asarray(np.tile(cases.cosine_bell(g), (L, 1)))      # moved by the exchange only
+            self.tr = Transport(dg, layers=L, beta=P.get('beta', BETA), tracers=2 if moist else 1)
+            phys = dict(theta_stars=list(stars), floor_=floor, grav=self.grav, moist=1.0 if moist else 0.0)
             if self.grav != GRAV:
                 phys.update(mu_ref=1.0e5 / self.grav, mu_bl=1.0e4 / self.grav)
             phys.update(P.get('physics', {}))
             longwave = P.get('longwave', 'gray')
-            self.physics = AirPhysics(dg, 3, phys, mean_sun=P.get('mean_sun', False), forcing='radiation',
+            self.physics = AirPhysics(dg, L, phys, mean_sun=P.get('mean_sun', False), forcing='radiation',
                                       longwave=longwave, lw_tables=P.get('lw_tables'), sw_tables=P.get('sw_tables'))
             if self.physics.lw is not None:
                 self.rad_every = max(1, int(round(P.get('rad_dt', 3600.0) / self.dt)))

@@ -206,7 +206,8 @@ def test_aqua_dry_world():
     annual-mean sun (mean_sun) likewise."""
     from insilica.world import World
     from oracle.world import OracleWorld
-    for preset, mean_sun in (('aqua_dry', False), ('aqua_dry', True), ('aqua', False)):
+    for preset, mean_sun in (('aqua_dry', False), ('aqua_dry', True), ('aqua', False), ('aqua5', False),
+                             ('aqua7', False)):
         print(f'case: {preset} world level 4, 12 ticks, mean_sun={mean_sun}', flush=True)
         w = World(4, preset, precision='double', params=dict(mean_sun=mean_sun))
         o = OracleWorld(w)
