@@ -91,7 +91,7 @@ static double column_energy(int L, const double* mu, const double* pib, const do
 
 /* Move m (> 0) kg/m^2 from layer `from` to `to`, re-dividing the column; the energy residual to the receiver. */
 static void move_mass(int L, const double* par, double* mu, double* th, double* q, double* pib, int from, int to,
-                      double m, double th_leave, double th_arrive, double q_leave, double q_arrive) {
+                      double m, double th_leave, double th_arrive, double q_leave, double q_arrive, double* cl) {
   if (m == 0) return;
   double pb2[OG_LMAX], pib2[OG_LMAX], mu2[OG_LMAX], th2[OG_LMAX], q2[OG_LMAX];
   const double Eb = column_energy(L, mu, pib, th, q, par);
@@ -107,6 +107,7 @@ static void move_mass(int L, const double* par, double* mu, double* th, double* 
   th2[to] = (mu[to] * th[to] + m * th_arrive) / mu2[to];
   q2[from] = (mu[from] * q[from] - m * q_leave) / mu2[from];
   q2[to] = (mu[to] * q[to] + m * q_arrive) / mu2[to];
+  if (cl) cl[to] = (mu[to] * cl[to] + m * cl[from]) / mu2[to];   /* (the air takes its cloud water along) */
   const double Ea = column_energy(L, mu2, pib2, th2, q2, par);
   th2[to] += (Eb - Ea) / (mu2[to] * pib2[to]);
   for (int k = 0; k < L; k++) {
@@ -122,7 +123,7 @@ static void move_mass(int L, const double* par, double* mu, double* th, double* 
  * mixed bottom layer (bl_mix) needs L >= 3 (its top edge is then an interface). dtop/dbot (optional): the heating at
  * each layer's top and bottom edge when it differs from the layer's mean (the k-tables' edge sub-levels). */
 static void crossing(int L, const double* par, double* mu, double* th, double* q, const double* dth, double dth_up,
-                     double* pib, double* wxo, const double* dtop, const double* dbot) {
+                     double* pib, double* wxo, const double* dtop, const double* dbot, double* cl) {
   const double floor_ = par[P_FLOOR];
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (par[P_THETA_STAR] <= 0) return;
@@ -195,9 +196,9 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
     if (dm > maxup) dm = maxup > 0 ? maxup : 0.0;
     if (-dm > maxdn) dm = maxdn > 0 ? -maxdn : 0.0;
     if (dm > 0)
-      move_mass(L, par, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], q[b], q[b]);
+      move_mass(L, par, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], q[b], q[b], cl);
     else if (dm < 0)
-      move_mass(L, par, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], q[a], q[a]);
+      move_mass(L, par, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], q[a], q[a], cl);
     wxo[j] = dm;
   }
   const double tiny = floor_ * 1.000001;
@@ -257,7 +258,7 @@ void og_hs_column(int N, int L, const double* par, const double* hs, const doubl
       const double ki = hs[0] + (hs[1] - hs[0]) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
       dth_up = (thi - par[P_THETA_STAR]) * (1 - exp(-ki * dt));
     }
-    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL);
+    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL, NULL);
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
     double E1 = 0;
     for (int k = 0; k < L; k++) {
@@ -374,7 +375,11 @@ enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5,
        P_SIGMA = 12, P_TILT = 17, P_DAY = 19, P_TAU_LIN = 20, P_LF = 24, P_TF = 25, P_K_EVAP = 26, P_TAU_CONV = 27,
        P_ALB_ICE = 28, P_K_ICE = 31, P_RV = 32, P_SAT_LAYER = 33, P_SW_TAU = 34, P_ALB_ICE_MELT = 41, P_ALB_RAMP = 45,
        P_LW_W0 = 48, P_BL_BOT = 54, P_THETA_RISE = 69, P_BULK_C = 74, P_BULK_Z = 76, P_BULK_W = 77, P_EDGE_HEAT = 84,
-       P_SFC = 90, P_DRY_EDGE = 99, P_EDGE_K = 101, P_ICE_AIR = 102, P_FIRN = 120, P_RHO_FIRN = 122, P_C_ICE = 123 };
+       P_SFC = 90, P_DRY_EDGE = 99, P_EDGE_K = 101, P_ICE_AIR = 102, P_FIRN = 120, P_RHO_FIRN = 122, P_C_ICE = 123,
+       P_CLOUD = 55, P_RH_CRIT = 56, P_RE_LIQ = 57, P_RE_ICE = 58, P_CLOUD_G = 59, P_K_LW_LIQ = 60, P_K_LW_ICE = 61,
+       P_C_AUTO = 62, P_Q_AUTO = 63, P_V_ICE = 64, P_TAU_ERODE = 66, P_K_REVAP = 67, P_MU_CLOUD = 68, P_L_ICE = 79,
+       P_CONV_REVAP = 87, P_DOWNDRAFT = 92, P_REVAP = 103, P_REVAP_RHS = 104, P_REVAP_AREA = 106, P_LIQ_COLD = 128,
+       P_CW_QS = 129 };
 
 static double omexp(double x, int mix) { return mix ? (double)-expm1f(-(float)x) : 1 - exp(-(x)); }
 static double fpow_(double x, double y, int mix) { return mix ? (double)powf((float)x, (float)y) : pow(x, y); }
@@ -480,12 +485,96 @@ static void parcel(double th, double q, double p, const double* par, double eps,
   condense(thp, qp, par[P_CP] * ex, p, par, eps, mix);
 }
 
+/* layer_cap (air.cu:334-359, the layer's theta uniform): the layer's vapor capacity over its pressure span and its
+ * change with theta, as condense_layer takes them. */
+static void layer_cap(int k, const double* mu, double th, const double* par, double eps, int mix, double* qs,
+                      double* dqs) {
+  static const double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  static const double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  const double Lv = par[P_LV], Rv = par[P_RV], ice = par[P_ICE_AIR], p0 = par[P_P0], kappa = par[P_KAPPA];
+  double pt = 0;
+  for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
+  const double pb = pt + par[P_GRAV] * mu[k];
+  if (mix) {
+    float qf = 0, dqf = 0;
+    for (int j = 0; j < 4; j++) {
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
+      const float ex = powf((float)(p / p0), (float)kappa), T = (float)th * ex;
+      if (T < 160) continue;
+      const float v = (float)qsat_x(T, (float)p, eps, ice, 1);
+      qf += 0.5f * (float)wq[j] * v;
+      dqf += 0.5f * (float)wq[j] * v * (float)Lv / ((float)Rv * T * T) * ex;
+    }
+    *qs = qf;
+    *dqs = dqf;
+  } else {
+    double qf = 0, dqf = 0;
+    for (int j = 0; j < 4; j++) {
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
+      const double ex = pow(p / p0, kappa), T = th * ex;
+      if (T < 160) continue;
+      const double v = qsat_x(T, p, eps, ice, 0);
+      qf += 0.5 * wq[j] * v;
+      dqf += 0.5 * wq[j] * v * Lv / (Rv * T * T) * ex;
+    }
+    *qs = qf;
+    *dqs = dqf;
+  }
+}
+
+/* cloud_pdf (air.cu:383-388): the water spread evenly over +-(1 - rh_crit) qs about its mean qt; its saturated share
+ * f and the mean condensate l there. */
+static void cloud_pdf(double qt, double qs, const double* par, double* l, double* f) {
+  const double b = (1 - par[P_RH_CRIT]) * qs, s = qt - qs;
+  if (s <= -b) {
+    *l = 0;
+    *f = 0;
+  } else if (s >= b) {
+    *l = s;
+    *f = 1;
+  } else {
+    *f = (s + b) / (2 * b);
+    *l = (s + b) * *f / 2;
+  }
+}
+
+/* cover_of (air.cu:393-396): the cover of cloud water cw spread at a cloud's typical water content by phase. */
+static double cover_of(double cw, double mu, double liq, const double* par, double qs) {
+  const double s = par[P_CW_QS] > 0 && qs > 0 ? fmin(1.0, par[P_CW_QS] * qs / par[P_Q_AUTO]) : 1.0;
+  return fmin(1.0, cw / (s * (liq * par[P_Q_AUTO] + (1 - liq) * par[P_L_ICE])) * fmax(1.0, mu / par[P_MU_CLOUD]));
+}
+
+/* cloud_adjust (air.cu:438-450): condense up to the statistics' condensate with the latent heat's feedback on the
+ * capacity; a cloud holding more erodes over tau_erode. */
+static double cloud_adjust(int k, const double* mu, double* th, double* q, double* cw, const double* pib, double dt,
+                           const double* par, double eps, int mix) {
+  double qs, dqs, l, f;
+  layer_cap(k, mu, th[k], par, eps, mix, &qs, &dqs);
+  if (!(qs > 0)) return 0;
+  cloud_pdf(q[k] + cw[k], qs, par, &l, &f);
+  const double dl = l > cw[k] ? (l - cw[k]) / (1 + par[P_LV] / pib[k] * dqs * f)
+                              : -(cw[k] - l) * omexp(dt / par[P_TAU_ERODE], mix);
+  cw[k] += dl;
+  q[k] -= dl;
+  th[k] += par[P_LV] * dl / pib[k];
+  return dl;
+}
+
+/* base_T (air.cu:476-482): the temperature at layer k's lower edge (its theta there, sat_prof = 0). */
+static double base_T(int k, const double* mu, const double* th, const double* par, int mix) {
+  double pt = 0;
+  for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
+  const double pb = pt + par[P_GRAV] * mu[k];
+  return th[k] * fpow_(pb / par[P_P0], par[P_KAPPA], mix);
+}
+
 void og_air_column(int N, int L, const double* par_in, const og_air* air, int mode, const double* t,
                    const double* insol, const double* pos, const double* uabs, const double* mu_in, double* theta,
                    double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,
                    double* stored, double* rain, double* evap, double* conv, double* tsurf, double* rainp,
                    const double* cap, double* qocean, double* qsw, const double* elev, double* flash,
-                   const double* radf, const double* swf, double* swalb, const double* ch, double* firn, int mix) {
+                   const double* radf, const double* swf, double* swalb, const double* ch, double* firn,
+                   double* cloudw, double* cloudo, int mix) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++) {
     double par[132 + OG_LMAX];
@@ -494,7 +583,9 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     const double grav = par[P_GRAV], cp = par[P_CP], kappa = par[P_KAPPA], p0 = par[P_P0], sigma = par[P_SIGMA];
     const double eps_s = par[P_EPS_S], Tf = par[P_TF], Lf = par[P_LF], Lv = par[P_LV], aeps = air->eps;
     const int moist = par[P_MOIST] > 0;
+    const int cloudy = moist && par[P_CLOUD] > 0 && cloudw;
     double mu[OG_LMAX], th[OG_LMAX], q[OG_LMAX], pb[OG_LMAX], pib[OG_LMAX], Q[OG_LMAX], T[OG_LMAX], eps[OG_LMAX];
+    double cl[OG_LMAX], fc[OG_LMAX], liq[OG_LMAX], cwp[OG_LMAX], eps0[OG_LMAX];
     double mt = 0;
     for (int k = 0; k < L; k++) {
       mu[k] = mu_in[(size_t)k * N + i];
@@ -502,6 +593,8 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       q[k] = moist ? qv[(size_t)k * N + i] : 0.0;
       mt += mu[k];
       Q[k] = 0;
+      cl[k] = cloudy ? cloudw[(size_t)k * N + i] : 0.0;
+      fc[k] = liq[k] = cwp[k] = 0;
     }
     layer_exner(L, mu, par, pb, pib);
     const int top = 0, bot = L - 1;
@@ -512,6 +605,19 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     double Tfn = Cf > 0 ? firn[i] : Tf;
     const double E0 = column_energy(L, mu, pib, th, q, par) + Cw * Ts[i] - Lf * m_ice + Cf * (Tfn - Tf);
     for (int k = 0; k < L; k++) T[k] = pib[k] * th[k] / cp;
+    /* each layer's cover, in-cloud water path (kg/m^2) and liquid share (all ice below -liq_cold C); a middle layer's
+     * clouds sit high in it, so their phase is by its upper quarter's pressure */
+    if (cloudy)
+      for (int k = 0; k < L; k++) {
+        const double ptk = k ? pb[k - 1] : 0.0;
+        const double pc = k == bot ? 0.5 * (pb[k] + ptk) : ptk + 0.25 * (pb[k] - ptk);
+        liq[k] = fmax(0.0, fmin(1.0, (th[k] * fpow_(pc / p0, kappa, mix) - Tf + par[P_LIQ_COLD]) / par[P_LIQ_COLD]));
+        double qs, dqs, l, f;
+        layer_cap(k, mu, th[k], par, aeps, mix, &qs, &dqs);
+        cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
+        fc[k] = fmax(f, cover_of(cl[k], mu[k], liq[k], par, qs));
+        cwp[k] = fc[k] > 1e-6 ? mu[k] * cl[k] / fc[k] : 0.0;
+      }
     /* the air touching the ground: the bottom mu_bl of the column (or bl_bot's share of it, the thin bottom layer's
      * own mass at least); the layers above give their air at their lower edge's theta* where warmer */
     double wbl[OG_LMAX], thb = 0, qb = 0, mb = 0;
@@ -598,7 +704,15 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     } else {
       for (int k = 0; k < L; k++) Q[k] += toAir * mu[k] / mt;
     }
-    const double Tc = 1;
+    /* clouds reflect: each layer's by conservative two-stream scattering of its optical depth 3 W / (2 rho r), the
+     * layers at random overlap, with the reflections between them and the ground */
+    double Tc = 1;
+    if (cloudy)
+      for (int k = 0; k < L; k++) {
+        const double tau = 1.5 * cwp[k] * (liq[k] / (1000 * par[P_RE_LIQ]) + (1 - liq[k]) / (par[P_RHO_ICE] * par[P_RE_ICE]));
+        const double g1 = (1 - par[P_CLOUD_G]) * tau;
+        Tc *= 1 - fc[k] * g1 / (2 + g1);
+      }
     const double swS = swf ? I * swf[(size_t)L * N + i] : (I - toAir) * Tc * (1 - alb) / (1 - alb * (1 - Tc));
 
     /* longwave: each layer's emissivity from its share of the optical depth (moist with lw_w0: the vapor's part from
@@ -616,6 +730,12 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       for (int k = 0; k < L; k++)
         eps[k] = omexp(tau_p(pb[k], pb[bot], par) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], par), mix);
     }
+    if (cloudy)                                      /* clouds absorb and emit by their water path */
+      for (int k = 0; k < L; k++) {
+        eps0[k] = eps[k];
+        const double kl = liq[k] * par[P_K_LW_LIQ] + (1 - liq[k]) * par[P_K_LW_ICE];
+        eps[k] = 1 - (1 - eps[k]) * (1 - fc[k] * omexp(kl * cwp[k], mix));
+      }
     double ldown = 0;
     const int RW = L + 1;
     if (radf) {
@@ -668,6 +788,26 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         qeb[k] += ((RF(0, k + 1) - RF(1, k + 1)) + RF(2, k + 1) * Es) - n2;
       }
       up = RF(0, 0) + RF(2, 0) * Es;
+      if (cloudy && cloudo) {                        /* cover, the clouds' effects at the top, optical depths */
+        double clear = 1;
+        for (int k = 0; k < L; k++) {
+          cloudo[(size_t)k * N + i] = fc[k];
+          clear *= 1 - fc[k];
+        }
+        cloudo[(size_t)L * N + i] = 1 - clear;
+        cloudo[(size_t)(L + 1) * N + i] =
+            swf ? I * (swf[(size_t)(L + 1) * N + i] - (1 - (toAir / fmax(I, 1e-30) + swf[(size_t)L * N + i])))
+                : swS - (I - toAir) * (1 - alb);
+        cloudo[(size_t)(L + 2) * N + i] = radf[(size_t)(3 * RW) * N + i] + radf[(size_t)(3 * RW + 1) * N + i] * Es - up;
+        for (int k = 0; k < L; k++) {
+          const double re_l = 1000 * par[P_RE_LIQ], re_i = par[P_RHO_ICE] * par[P_RE_ICE];
+          cloudo[(size_t)(L + 3 + k) * N + i] =
+              fc[k] > 1e-6 ? (liq[k] * par[P_K_LW_LIQ] + (1 - liq[k]) * par[P_K_LW_ICE]) * cwp[k] : 0.0;
+          cloudo[(size_t)(2 * L + 3 + k) * N + i] = 1.5 * cwp[k] * (liq[k] / re_l + (1 - liq[k]) / re_i);
+          const double tl_ = liq[k] / re_l, ti_ = (1 - liq[k]) / re_i;
+          cloudo[(size_t)(3 * L + 3 + k) * N + i] = tl_ + ti_ > 0 ? tl_ / (tl_ + ti_) : 1.0;
+        }
+      }
 #undef RF
     } else {
       for (int k = bot; k >= 0; k--) {
@@ -676,6 +816,17 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         const double em = eps[k] * sigma * T[k] * T[k] * T[k] * T[k];
         Q[k] -= 2 * em;
         up = up - a + em;
+      }
+      if (cloudy && cloudo) {                        /* cover and the clouds' effects at the top */
+        double up0 = eps_s * sigma * tsk * tsk * tsk * tsk, clear = 1;
+        for (int k = bot; k >= 0; k--) up0 = up0 - eps0[k] * up0 + eps0[k] * sigma * T[k] * T[k] * T[k] * T[k];
+        for (int k = 0; k < L; k++) {
+          cloudo[(size_t)k * N + i] = fc[k];
+          clear *= 1 - fc[k];
+        }
+        cloudo[(size_t)L * N + i] = 1 - clear;
+        cloudo[(size_t)(L + 1) * N + i] = swS - (I - toAir) * (1 - alb);
+        cloudo[(size_t)(L + 2) * N + i] = up0 - up;
       }
       for (int k = 0; k < L; k++) {
         double dn = eps[k] * sigma * T[k] * T[k] * T[k] * T[k];
@@ -736,8 +887,70 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     Ts[i] = Tn;
     sw_abs[i] = toAir + swS;
     /* condensation: one-step saturation adjustment per layer; its latent heat counts as heating for the crossing */
-    double R = 0, Rk[2] = {0, 0}, Rc = 0;
-    if (moist)
+    double R = 0, Rk[2] = {0, 0}, Rc = 0, S = 0;
+    const int icy = par[P_ICE_AIR] > 0;
+    if (cloudy) {
+      /* cloud water forms and erodes; liquid turns to rain (Sundqvist 1989), ice falls out; rain evaporates in the
+       * clear air it falls through. With ice_air the falling ice freezes in its layer (Lf), melts in a layer whose
+       * lower edge is above freezing, sublimates with the rain's evaporation; what is left reaches the ground as
+       * snow. */
+      double Fr = 0, Fs = 0, fa = 0, psf = 0;
+      for (int k = 0; k < L; k++) psf += grav * mu[k];
+      for (int k = 0; k < L; k++) {
+        const double dl = cloud_adjust(k, mu, th, q, cl, pib, dt, par, aeps, mix);
+        dth[k] += Lv * dl / pib[k];
+        dlat[k] += Lv * dl / pib[k];
+        double qs, dqs, l, f;
+        layer_cap(k, mu, th[k], par, aeps, mix, &qs, &dqs);
+        cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
+        f = fmax(f, cover_of(cl[k], mu[k], liq[k], par, qs));
+        const double inc = f > 1e-6 ? cl[k] / f * fmax(1.0, mu[k] / par[P_MU_CLOUD]) : 0.0, r = inc / par[P_Q_AUTO];
+        double pt_ = 0;
+        for (int j = 0; j < k; j++) pt_ += grav * mu[j];
+        const double pm = pt_ + 0.5 * grav * mu[k], Tk = pib[k] * th[k] / cp, depth = mu[k] / (pm / (air->rd * Tk));
+        const double rate = liq[k] * par[P_C_AUTO] * omexp(r * r, mix) + (1 - liq[k]) * par[P_V_ICE] / depth;
+        const double loss = cl[k] * omexp(rate * dt, mix);
+        cl[k] -= loss;
+        Fr += mu[k] * loss / dt;
+        if (icy) {
+          const double fz = rate > 0 ? fmin(1.0, (1 - liq[k]) * par[P_V_ICE] / depth / rate) : 0.0, dz = loss * fz;
+          Fs += mu[k] * dz / dt;
+          th[k] += Lf * dz / pib[k];
+          dth[k] += Lf * dz / pib[k];
+          dlat[k] += Lf * dz / pib[k];
+          if (Fs > 0 && base_T(k, mu, th, par, mix) > Tf) {
+            const double dm = Fs * dt / (mu[k] * pib[k]) * Lf;
+            th[k] -= dm;
+            dth[k] -= dm;
+            dlat[k] -= dm;
+            Fs = 0;
+          }
+        }
+        double ev = 0;
+        if (loss > 0) fa = fmax(fa, f);
+        const double rc_ = par[P_REVAP] > 0 ? par[P_REVAP_RHS] : 1.0;
+        if (Fr > 0 && rc_ * qs > q[k]) {
+          if (par[P_REVAP] > 0) {
+            const double a_ = fmax(fa, par[P_REVAP_AREA]), ac = fmax(0.0, a_ - f);
+            ev = ac * 5.44e-4 * (rc_ * qs - q[k]) * pow(sqrt(pm / psf) / 5.09e-3 * Fr / a_, 0.577) * mu[k];
+          } else {
+            ev = par[P_K_REVAP] * (1 - f) * (1 - q[k] / qs) * sqrt(Fr) * mu[k];
+          }
+          ev = fmin(ev, fmin(Fr, (rc_ * qs - q[k]) * mu[k] / dt / (1 + Lv / pib[k] * dqs)));
+          const double es = Fr > 0 ? ev * Fs / Fr : 0.0;
+          const double Le = Lv * ev + Lf * es;
+          Fr -= ev;
+          Fs -= es;
+          q[k] += ev * dt / mu[k];
+          th[k] -= Le * dt / (mu[k] * pib[k]);
+          dth[k] -= Le * dt / (mu[k] * pib[k]);
+          dlat[k] -= Le * dt / (mu[k] * pib[k]);
+        }
+        Rk[k == bot ? 1 : 0] += mu[k] * loss - ev * dt;
+      }
+      R = Fr * dt;
+      S = fmax(0.0, fmin(Fs * dt, R));
+    } else if (moist)
       for (int k = 0; k < L; k++) {
         const double dq = condense_k(k, mu, th, q, pib, par, aeps, mix);
         dth[k] += Lv * dq / pib[k];
@@ -767,7 +980,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       for (int k = 1; k <= par[P_EDGE_K] && k < L; k++) dtop1[k] = dtop[k];
     }
     crossing(L, par, mu, th, q, dth, dth[top], pib, wxo,
-             radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL);
+             radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL, cloudy ? cl : NULL);
     /* moist convection: a parcel of bottom-layer air lifted to the interface above it; buoyant there (theta_p above
      * theta*), it rises through every interface it stays buoyant at and detrains above the last, its condensate
      * raining; the mass that would remove the buoyancy at the first interface relaxes over tau_conv */
@@ -776,7 +989,13 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     if (moist && par[P_THETA_STAR] > 0) {
       double pi_ = 0;
       for (int k = 0; k < bot; k++) pi_ += grav * mu[k];
-      const double th0 = th[bot], q0 = q[bot];
+      const double th0 = th[bot];
+      double q0 = q[bot];
+      if (cloudy) {                                  /* plumes root in the cell's moist air: its moistest part */
+        double qs, dqs;
+        layer_cap(bot, mu, th0, par, aeps, mix, &qs, &dqs);
+        q0 = fmax(q0, fmin(qs, q0 + (1 - par[P_RH_CRIT]) * qs));
+      }
       double thp, qc, thp2, qc2;
       parcel(th0, q0, pi_, par, aeps, mix, &thp, &qc);
       const double d = thp - tstar(bot - 1, L, par);
@@ -801,7 +1020,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           }
           R += mc * (q0 - qc);
           Rc = mc * (q0 - qc);
-          move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qc);
+          move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qc, NULL);
           for (int j = kd; j < bot; j++) wxo[j] += mc;
           const double rt = mu[kd] * condense_k(kd, mu, th, q, pib, par, aeps, mix);
           R += rt;
@@ -811,20 +1030,52 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         }
       }
     }
+    /* the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the
+     * convection's area revap_area; else on conv_revap of the cell) */
+    if (cloudy && (par[P_REVAP] > 0 ? Rc > 0 : par[P_CONV_REVAP] > 0 && !(par[P_DOWNDRAFT] > 0) && Rc > 0) && L > 1) {
+      double qs, dqs;
+      layer_cap(bot, mu, th[bot], par, aeps, mix, &qs, &dqs);
+      const double rc_ = par[P_REVAP] > 0 ? par[P_REVAP_RHS] : 1.0;
+      if (rc_ * qs > q[bot]) {
+        double ev;
+        if (par[P_REVAP] > 0) {
+          const double Fc = Rc / dt;
+          double pt_ = 0;
+          for (int k = 0; k < bot; k++) pt_ += grav * mu[k];
+          const double pm = pt_ + 0.5 * grav * mu[bot], psf = pt_ + grav * mu[bot];
+          ev = par[P_REVAP_AREA] * 5.44e-4 * (rc_ * qs - q[bot]) *
+               pow(sqrt(pm / psf) / 5.09e-3 * Fc / par[P_REVAP_AREA], 0.577) * mu[bot];
+          ev = fmin(ev, fmin(Fc, (rc_ * qs - q[bot]) * mu[bot] / dt / (1 + Lv / pib[bot] * dqs)));
+        } else {
+          const double Fc = Rc / dt;
+          ev = par[P_K_REVAP] * (1 - q[bot] / qs) * sqrt(par[P_CONV_REVAP] * Fc) * mu[bot];
+          ev = fmin(ev, fmin(Fc, (qs - q[bot]) * mu[bot] / dt / (1 + Lv / pib[bot] * dqs)));
+        }
+        q[bot] += ev * dt / mu[bot];
+        th[bot] -= Lv * ev * dt / (mu[bot] * pib[bot]);
+        R -= ev * dt;
+        Rc -= ev * dt;
+      }
+    }
     if (conv) conv[i] = mc;
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
     rain[i] = R / dt;
+    if (S > 0) {                                     /* snow into the sea melts with the slab's heat */
+      Ts[i] -= Lf * S / Cw;
+      if (qocean) qocean[i] -= Lf * S / dt;
+    }
     if (rainp) {
       rainp[i] = Rk[0] / dt;
       rainp[(size_t)N + i] = Rk[1] / dt;
       rainp[(size_t)2 * N + i] = Rc / dt;
-      rainp[(size_t)3 * N + i] = 0 / dt;
+      rainp[(size_t)3 * N + i] = S / dt;
     }
     const double E1 = column_energy(L, mu, pib, th, q, par) + Cw * Ts[i] - Lf * m_ice + Cf * (Tfn - Tf);
     if (Cf > 0) firn[i] = Tfn;
     for (int k = 0; k < L; k++) {
       theta[(size_t)k * N + i] = th[k];
       if (moist) qv[(size_t)k * N + i] = q[k];
+      if (cloudy) cloudw[(size_t)k * N + i] = cl[k];
     }
     stored[i] = E1 - E0;
   }

@@ -58,15 +58,17 @@ def gas_src(src, h, dt, x, L):
 
 
 def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None, elev=None, radf=None, swf=None,
-               ch=None, firn=None, air=None, mix=False, ocean=False, q=None, ice=None):
-    """air.cu air_column (the dry column; the moist one, par's moist > 0, with q (L, N) and ice (N)): returns a dict
-    of theta, Ts, wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, flash, swalb (and qocean, qsw with
-    ocean=True; firn when given; q and ice when moist). mu is read only."""
+               ch=None, firn=None, air=None, mix=False, ocean=False, q=None, ice=None, cw=None, cloudo=False):
+    """air.cu air_column (the dry column; the moist one, par's moist > 0, with q (L, N) and ice (N); clouds with cloud
+    > 0 and the cloud water cw (L, N)): returns a dict of theta, Ts, wx, olr, sw_abs, stored, rain, evap, conv, tsurf,
+    rainp, flash, swalb (and qocean, qsw with ocean=True; firn when given; q and ice when moist; cw, and cloudo
+    ((4L + 3), N) with cloudo=True, with clouds). mu is read only."""
     par, t, pos, uabs, mu, insol, cap, elev, radf, swf, ch = (_d(x) for x in (par, t, pos, uabs, mu, insol, cap, elev,
                                                                              radf, swf, ch))
     theta, Ts = _d(theta).copy(), _d(Ts).copy()
     q = None if q is None else _d(q).copy()
     ice = None if ice is None else _d(ice).copy()
+    cw = None if cw is None else _d(cw).copy()
     firn = None if firn is None else _d(firn).copy()
     L, N = mu.shape
     z = lambda *s: np.zeros(s)
@@ -74,13 +76,17 @@ def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None,
              conv=z(N), tsurf=z(N), rainp=z(4, N), flash=np.full(N, -1.0), swalb=z(N))
     if ocean:
         o['qocean'], o['qsw'] = z(N), z(N)
+    if cw is not None and cloudo:
+        o['cloudo'] = z(4 * L + 3, N)
     lib().og_air_column(N, L, _p(par), ctypes.byref(air_struct(air)), int(mode), _p(t), _p(insol), _p(pos), _p(uabs),
                         _p(mu), _p(theta), _p(q), _p(Ts), _p(ice), float(dt), _p(o['wx']), _p(o['olr']), _p(o['sw_abs']),
                         _p(o['stored']), _p(o['rain']), _p(o['evap']), _p(o['conv']), _p(o['tsurf']), _p(o['rainp']),
                         _p(cap), _p(o.get('qocean')), _p(o.get('qsw')), _p(elev), _p(o['flash']), _p(radf), _p(swf),
-                        _p(o['swalb']), _p(ch), _p(firn), int(bool(mix)))
+                        _p(o['swalb']), _p(ch), _p(firn), _p(cw), _p(o.get('cloudo')), int(bool(mix)))
     if firn is not None:
         o['firn'] = firn
     if q is not None:
         o['q'], o['ice'] = q, ice
+    if cw is not None:
+        o['cw'] = cw
     return o

@@ -112,14 +112,15 @@ void og_cell_speed(int N, const double* u, const int32_t* cellEdge, const double
                    const double* areaCell, double* out);
 void og_cell_speed_perot(int N, const double* u, const int32_t* cellEdge, const double* op, double* out);
 void og_gas_src(int N, int L, int G, const double* src, const double* h, double dt, double* x);
-/* air_column (air.cu:876-1972): the dry column, and the moist one without clouds, land or the mixed layer; qv (L, N)
- * and ice (N) read and written when moist. mix: the reference's mixed build's float32 arithmetic. */
+/* air_column (air.cu:876-1972): the dry column, and the moist one over the sea without land or the mixed layer; qv
+ * (L, N) and ice (N) read and written when moist, the cloud water cloudw (L, N) with cloud > 0 (cloudo ((4L + 3), N)
+ * the clouds' cover, effects and optical depths, when given). mix: the reference's mixed build's float32 arithmetic. */
 void og_air_column(int N, int L, const double* par, const og_air* air, int mode, const double* t, const double* insol,
                    const double* pos, const double* uabs, const double* mu, double* theta, double* qv, double* Ts,
                    double* ice, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
                    double* evap, double* conv, double* tsurf, double* rainp, const double* cap, double* qocean,
                    double* qsw, const double* elev, double* flash, const double* radf, const double* swf,
-                   double* swalb, const double* ch, double* firn, int mix);
+                   double* swalb, const double* ch, double* firn, double* cloudw, double* cloudo, int mix);
 double og_sun_lon(double t, const double* par, double* fd);
 
 #endif

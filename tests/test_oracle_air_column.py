@@ -419,3 +419,123 @@ def test_plume_relaxes_over_tau_conv():
     # the plume's mass crossed the interface above the bottom layer
     assert np.all(a['wx'][1][act] >= a['conv'][act] * (1 - 1e-12))
     assert np.all(a['rainp'][2][act] > 0)
+
+
+# ---- clouds (cloud > 0, with the cloud water cw) ----
+
+def cloudy(seed=21, N=64, cw=2e-5, **kw):
+    c = moist_aqua(N=N, seed=seed, **kw)
+    c['cw'] = np.full_like(c['q'], cw)
+    return c
+
+
+def crun(c, p, **kw):
+    return run(c, p, q=c['q'], ice=c['ice'], cw=c['cw'], cloudo=True, **kw)
+
+
+@pytest.mark.parametrize('opts', [dict(), dict(ice_air=1.0), dict(revap=1.0), dict(conv_revap=0.05),
+                                  dict(cw_qs=1.0), dict(sat_layer=0.0, lw_w0=5.0)])
+@pytest.mark.parametrize('mix', [False, True])
+def test_clouds_energy_and_water_close(opts, mix):
+    """With clouds the energy still closes (cloud water holds no energy of its own: its latent heat went into theta
+    when it formed); vapor plus cloud water change by evaporation less rain (the plumes held off by a long tau_conv:
+    a plume moves its base layer's air without its cloud water, as the reference's does)."""
+    c = cloudy(rh=(0.3, 1.0, 1.3), warm=18.0)
+    dt = 900.0
+    o = crun(c, par3(moist=1.0, cloud=1.0, **opts), dt=dt, mix=mix)
+    stored_matches(o, dt)
+    assert o['rain'].max() > 0 and np.all(o['cw'] >= 0) and np.all(o['q'] >= 0)
+    cover = o['cloudo'][:3]
+    assert np.all((cover >= 0) & (cover <= 1)) and np.all(o['cloudo'][3] >= cover.max(0) - 1e-15)
+    q = crun(c, par3(moist=1.0, cloud=1.0, tau_conv=1e12, **opts), dt=dt, mix=mix)
+    W0 = (c['mu'] * (c['q'] + c['cw'])).sum(0)
+    W1 = (new_masses(c, q['wx']) * (q['q'] + q['cw'])).sum(0)
+    np.testing.assert_allclose(W1 - W0, (q['evap'] - q['rain']) * dt, rtol=0, atol=1e-11 * W0.max())
+
+
+def test_cloud_statistics():
+    """A layer's cloud water goes to the mean condensate of its total water spread evenly over +-(1 - rh_crit) q_s
+    (the mean of max(qt + x - q_s, 0), integrated here), q_s the layer's capacity, with one Newton step for the latent
+    heat's feedback: what is left is within a tenth of the initial departure. Rain, erosion, sunlight, longwave, the
+    surface and the plumes are off."""
+    from scipy import integrate
+    c = cloudy(seed=5, N=6, cw=0.0, ice_share=0.0)
+    pt, pb, pib = pib_of(c['mu'])
+    k = 2
+    capf = lambda th, i: integrate.quad(lambda pp: tetens(th * (pp / P0) ** KAPPA, pp), pt[k, i], pb[k, i],
+                                        epsrel=1e-12)[0] / (pb[k, i] - pt[k, i])
+    for i in range(6):
+        c['q'][k, i] = capf(c['theta'][k, i], i) * (0.9 + 0.04 * i)      # around saturation: a partial cover
+    p = par3(moist=1.0, cloud=1.0, c_auto=0.0, v_ice=0.0, tau_erode=1e30, tau0=0.0, eps_s=0.0, h_sens=0.0,
+             k_evap=0.0, theta_star=-1.0, theta_star2=-1.0, rh_crit=0.8, liq_cold=1e-9)
+    o = crun(c, p, mode=1, insol=np.zeros(6))
+    for i in range(6):
+        qs = capf(o['theta'][k, i], i)
+        b = 0.2 * qs
+        qt = o['q'][k, i] + o['cw'][k, i]
+        mean = integrate.quad(lambda x: max(qt + x - qs, 0.0), -b, b, points=[qs - qt])[0] / (2 * b)
+        qs0 = capf(c['theta'][k, i], i)
+        mean0 = integrate.quad(lambda x: max(c['q'][k, i] + x - qs0, 0.0), -0.2 * qs0, 0.2 * qs0,
+                               points=[qs0 - c['q'][k, i]])[0] / (0.4 * qs0)
+        assert mean0 > 0 and abs(o['cw'][k, i] - mean) < 0.1 * mean0       # (without the feedback: ~2x off)
+    # the condensate's latent heat: L_v dl = pib dtheta in the layer
+    np.testing.assert_allclose(2.5e6 * o['cw'][k], pib[k] * (o['theta'][k] - c['theta'][k]), rtol=1e-10)
+
+
+def test_cloud_radiation():
+    """A thick cloud filling the top layer over transparent air is a black body at that layer's temperature for the
+    outgoing longwave; the sunlight reaching a reflecting surface under a conservatively scattering cloud layer of
+    transmission T is the series of reflections between them: (1 - alb) T / (1 - alb (1 - T)) of what enters."""
+    c = cloudy(seed=8, N=8, cw=0.0, ice_share=0.0)
+    c['q'][:] = 0.0
+    c['cw'][0] = 5e-3                                    # ~11 kg/m^2 of cloud water in the top layer
+    p = par3(moist=1.0, cloud=1.0, tau0=0.0, eps_s=1.0, h_sens=0.0, k_evap=0.0, theta_star=-1.0, theta_star2=-1.0,
+             c_auto=0.0, v_ice=0.0, tau_erode=1e30, asw=0.0, alb=0.3)
+    I = np.full(8, 400.0)
+    o = crun(c, p, mode=1, insol=I)
+    _, _, pib = pib_of(c['mu'])
+    T0 = pib[0] * c['theta'][0] / CP
+    np.testing.assert_allclose(o['olr'], SIGMA * T0 ** 4, rtol=1e-9)
+    assert np.all(o['cloudo'][0] == 1.0)
+    # the cloud's transmission T from a black surface (alb 0); over alb 0.3 the reflections between the surface and
+    # the conservatively scattering cloud (reflectance 1 - T) sum as a series
+    T = crun(c, par3(moist=1.0, cloud=1.0, tau0=0.0, eps_s=1.0, h_sens=0.0, k_evap=0.0, theta_star=-1.0,
+                     theta_star2=-1.0, c_auto=0.0, v_ice=0.0, tau_erode=1e30, asw=0.0, alb=0.0),
+             mode=1, insol=I)['sw_abs'] / I
+    assert np.all((T > 0.005) & (T < 0.5))
+    series = sum((1 - 0.3) * T * (0.3 * (1 - T)) ** n for n in range(400))
+    np.testing.assert_allclose(o['sw_abs'], I * series, rtol=1e-12)
+
+
+def test_autoconversion_limit():
+    """Warm cloud far above the autoconversion threshold rains at Sundqvist's rate c_auto: a share
+    1 - exp(-c_auto dt) of its water per step (evaporation of the rain held off: k_revap 0). The bottom layer is
+    warmed above freezing: with liq_cold tiny its cloud is then all liquid (below freezing it would all be ice and
+    fall at v_ice / depth instead)."""
+    c = cloudy(seed=9, N=4, cw=0.0, ice_share=0.0, warm=30.0)
+    c['q'][:] = 0.0
+    c['cw'][2] = 5e-2                                    # 50 g/kg: (l / q_auto)^2 ~ 2.8e4
+    p = par3(moist=1.0, cloud=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, k_evap=0.0, theta_star=-1.0, theta_star2=-1.0,
+             tau_erode=1e30, k_revap=0.0, liq_cold=1e-9)
+    dt = 600.0
+    o = crun(c, p, mode=1, insol=np.zeros(4), dt=dt)
+    cw1 = o['cw'][2] + o['rain'] * dt / c['mu'][2]      # the cloud water before it rained
+    np.testing.assert_allclose(o['rain'] * dt / c['mu'][2], cw1 * -math.expm1(-1e-3 * dt), rtol=1e-9)
+
+
+def test_ice_fall_rate():
+    """All-ice cloud (bottom layer below freezing, liq_cold tiny) falls out at v_ice over the layer's depth: a share
+    1 - exp(-v_ice dt / depth) per step, the depth mu R_d T / p at its mid pressure (hydrostatic, ideal gas; the air's R_d 287.0)."""
+    c = cloudy(seed=9, N=4, cw=0.0, ice_share=0.0)
+    c['q'][:] = 0.0
+    c['cw'][2] = 5e-3
+    p = par3(moist=1.0, cloud=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, k_evap=0.0, theta_star=-1.0, theta_star2=-1.0,
+             tau_erode=1e30, k_revap=0.0, liq_cold=1e-9, v_ice=0.5, k_lw_liq=0.0, k_lw_ice=0.0)
+    dt = 600.0
+    pt, pb, pib = pib_of(c['mu'])
+    T = pib[2] * c['theta'][2] / CP
+    assert np.all(T < 272.0)
+    depth = c['mu'][2] * 287.0 * T / (0.5 * (pt[2] + pb[2]))
+    o = crun(c, p, mode=1, insol=np.zeros(4), dt=dt)
+    cw1 = o['cw'][2] + o['rain'] * dt / c['mu'][2]
+    np.testing.assert_allclose(o['rain'] * dt / c['mu'][2], cw1 * -np.expm1(-0.5 * dt / depth), rtol=1e-9)
