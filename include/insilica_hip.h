@@ -258,14 +258,20 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * layer's rho C_H U (ch (N), with sfc > 0); the heating as theta and the crossing of the interfaces (the k-tables'
  * edge heating). With moist > 0 also the vapor qv (L, N) and sea ice ice (N, kg/m^2): evaporation, the ice's skin
  * and albedo, freezing and melting, condensation per layer, the parcel plume of moist convection (plume_deep = 0),
- * and the vapor's longwave part with lw_w0. Updates theta, Ts (and qv, ice) in place and writes wx (L-1, N), olr,
- * sw_abs, stored (the column's energy change, J/m^2), rain, evap, conv (kg/m^2/s), tsurf; rainp (4, N), qocean, qsw,
- * flash, swalb and firn (the firn's temperature; its heat goes to the slab) when given. The argument list is the
- * reference's; soilcap, soil, pond, snow, cloudo and lc are unread (the reference reads them only over land or with
- * clouds); elev (N) with theta_rise > 0 raises the lowest interface over high ground. Returns ISW_EBADARG for bl_mix
- * > 0, cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or ice, with clouds (cloud > 0 and
- * cloudw given), land, plume_deep, plume_root, q_shape, sat_prof, sat_base, edge_vap or freeze_dry: those parts of
- * the reference's column are not in this library. prec: ISW_FP64, or ISW_MIXED (the reference's mixed build:
+ * and the vapor's longwave part with lw_w0. With clouds (moist, cloud > 0 and the cloud water cloudw (L, N) given,
+ * updated in place): the cloud statistics and erosion, Sundqvist's autoconversion, the ice's fall (ice_air: its
+ * freezing, melting and snow), the rain's evaporation (k_revap, or Kessler's with revap; conv_revap below the plumes),
+ * the clouds' reflection and emission, the cloud water carried by the crossing; cloudo (L + 3 rows, or 4L + 3 with
+ * radf, N) when given: each layer's cover, the total cover, the clouds' shortwave and longwave effects at the top
+ * (and with radf the in-cloud longwave and shortwave optical depths and liquid share), as the radiation's set-up reads
+ * them. Updates theta, Ts (and qv, ice) in place and writes wx (L-1, N), olr, sw_abs, stored (the column's energy
+ * change, J/m^2), rain, evap, conv (kg/m^2/s), tsurf; rainp (4, N: rain from the layers above the bottom one, from
+ * the bottom one, the plumes', snow), qocean, qsw, flash, swalb and firn (the firn's temperature; its heat goes to the
+ * slab) when given. The argument list is the reference's; soilcap, soil, pond, snow and lc are unread (the reference
+ * reads them only over land); elev (N) with theta_rise > 0 raises the lowest interface over high ground. Returns
+ * ISW_EBADARG for bl_mix > 0, cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or ice,
+ * with land, plume_deep, plume_root, q_shape, sat_prof, sat_base, edge_vap or freeze_dry, and with clouds and bl_q:
+ * those parts of the reference's column are not in this library. prec: ISW_FP64, or ISW_MIXED (the reference's mixed build:
  * 1 - e^-x, the edge sub-levels' Exner powers and the saturation and condensation arithmetic in float32). */
 int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* air, int32_t mode, const double* t,
                        const double* insol, const double* pos, const double* uabs, const double* mu, double* theta,

@@ -126,10 +126,11 @@ __device__ __forceinline__ double column_energy(const double* mu, const double* 
 
 // Move m (> 0) kg/m^2 of air from layer `from` to `to` (air.cu:495-520): the layers' masses, Exner functions, theta
 // (and vapor: it leaves with q_leave per kg and arrives with q_arrive) re-divided, the column's energy residual booked
-// into the receiver.
-template <int L, bool MOIST = false>
+// into the receiver. CL: the air takes its cloud water cl along (the receiver's share re-divided).
+template <int L, bool MOIST = false, bool CL = false>
 __device__ void move_mass(const CP& P, double* mu, double* th, double* q, double* pib, int from, int to, double m,
-                          double th_leave, double th_arrive, double q_leave = 0, double q_arrive = 0) {
+                          double th_leave, double th_arrive, double q_leave = 0, double q_arrive = 0,
+                          double* cl = nullptr) {
   if (m == 0) return;
   double pb2[L], pib2[L], mu2[L], th2[L], q2[L];
   const double Eb = column_energy<L, MOIST>(mu, pib, th, q, P);
@@ -164,6 +165,18 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* q, double
     if (k == from) { th2[k] = thf; if constexpr (MOIST) q2[k] = qfn; }
     if (k == to) { th2[k] = tht; if constexpr (MOIST) q2[k] = qtn; }
   }
+  if constexpr (CL) {
+    double clf = 0, clt = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      if (k == from) clf = cl[k];
+      if (k == to) clt = cl[k];
+    }
+    const double cln = (mt * clt + m * clf) / mut2;
+#pragma unroll
+    for (int k = 0; k < L; k++)
+      if (k == to) cl[k] = cln;
+  }
   const double Ea = column_energy<L, MOIST>(mu2, pib2, th2, q2, P);
 #pragma unroll
   for (int k = 0; k < L; k++)
@@ -180,10 +193,11 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* q, double
 // The dry crossing of the isentropic interfaces by the heating dth (air.cu:744-850); wxo[j] the mass moved up
 // across interface j. Dry air: the edge-vapor and freeze-drying options move the same mass without vapor.
 // et[k], eb[k]: the heating at layer k's top and bottom edge (the reference's dtop[k] or dth[k], and dbot[k] or dth[k],
-// the top layer's bottom-edge default its dth_up): always given, so every array stays in registers.
-template <int L, bool MOIST = false>
+// the top layer's bottom-edge default its dth_up): always given, so every array stays in registers. CL: the crossing
+// air takes its cloud water cl along.
+template <int L, bool MOIST = false, bool CL = false>
 __device__ void crossing(const CP& P, double* mu, double* th, double* q, const double* dth, const double* et,
-                         const double* eb, double* pib, double* wxo) {
+                         const double* eb, double* pib, double* wxo, double* cl = nullptr) {
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (P.theta_star <= 0) return;
@@ -273,9 +287,9 @@ __device__ void crossing(const CP& P, double* mu, double* th, double* q, const d
       }
     }
     if (dm > 0)
-      move_mass<L, MOIST>(P, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qb_, qb_);
+      move_mass<L, MOIST, CL>(P, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qb_, qb_, cl);
     else if (dm < 0)
-      move_mass<L, MOIST>(P, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], qa_, qa_);
+      move_mass<L, MOIST, CL>(P, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], qa_, qa_, cl);
     wxo[j] = dm;
   }
   const double tiny = P.floor_ * 1.000001;
@@ -359,6 +373,7 @@ __global__ void __launch_bounds__(BS) k_hs_column(int N, CP P, double ka, double
   stored[i] = E1 - E0;
 }
 
+#if !ISW_COLUMN_CLOUDS
 // surface_layer (air.cu:2015-2061): rho C_D U and rho C_H U by Louis's bulk form over the sea (Charnock), sea ice and
 // land, four fixed-point passes with the free convection's gusts.
 __global__ void __launch_bounds__(BS) k_surface_layer(int N, int L, CP P, double rd, double vf, double eps,
@@ -468,6 +483,7 @@ __global__ void __launch_bounds__(BS) k_gas_src(int N, int L, int G, const doubl
 
 // advance_time (air.cu:1975): the device clock, advanced inside captured ticks.
 __global__ void k_advance_time(double* t, double dt) { t[0] += dt; }
+#endif  // !ISW_COLUMN_CLOUDS
 
 // ---------------------------------------------------------------------------------------------------------------
 // air_column (air.cu:876-1972): the dry column, and the moist one without clouds, land or the mixed boundary layer:
@@ -483,6 +499,9 @@ struct AP {
   double S0, asw, tau0, mu_ref, eps_s, h_sens, u0, alb, c_slab, sigma, tilt, year_s, day_s, tau_lin, Lf, Tf, sw_tau,
       bl_bot, theta_rise, bulk_c, bulk_z, bulk_w, edge_heat, sfc, edge_k, firn, rho_firn, c_ice, ecc, lam_peri, rd,
       k_evap, tau_conv, alb_ice, k_ice, Rv, sat_layer, alb_ice_melt, alb_ramp, lw_w0, ice_air, dry_edge, eps;
+  // the clouds'
+  double rh_crit, re_liq, re_ice, cloud_g, k_lw_liq, k_lw_ice, c_auto, q_auto, v_ice, tau_erode, k_revap, mu_cloud,
+      l_ice, conv_revap, downdraft, revap, revap_rhs, revap_area, liq_cold, cw_qs;
 };
 
 static AP make_ap(const double* a, const isw_air* air) {
@@ -495,6 +514,10 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.k_evap = a[26]; p.tau_conv = a[27]; p.alb_ice = a[28]; p.k_ice = a[31]; p.Rv = a[32]; p.sat_layer = a[33];
   p.alb_ice_melt = a[41]; p.alb_ramp = a[45]; p.lw_w0 = a[48]; p.ice_air = a[102]; p.dry_edge = a[99];
   p.eps = air->eps;
+  p.rh_crit = a[56]; p.re_liq = a[57]; p.re_ice = a[58]; p.cloud_g = a[59]; p.k_lw_liq = a[60]; p.k_lw_ice = a[61];
+  p.c_auto = a[62]; p.q_auto = a[63]; p.v_ice = a[64]; p.tau_erode = a[66]; p.k_revap = a[67]; p.mu_cloud = a[68];
+  p.l_ice = a[79]; p.conv_revap = a[87]; p.downdraft = a[92]; p.revap = a[103]; p.revap_rhs = a[104];
+  p.revap_area = a[106]; p.liq_cold = a[128]; p.cw_qs = a[129];
   return p;
 }
 
@@ -555,14 +578,18 @@ __device__ double condense(double& th, double& q, double pib, double p, const CP
   return dq;
 }
 
-// The layer's capacity over its pressure span [pt, pb] (4-point Gauss-Legendre; nodes colder than 160 K hold
-// nothing), one Newton step with Clausius-Clapeyron's slope at each node (air.cu:275-291).
+// The layer's vapor capacity qs over its pressure span [pt, pb] at a uniform theta, and its change with theta dqs
+// (air.cu:334-359): 4-point Gauss-Legendre, nodes colder than 160 K holding nothing, Clausius-Clapeyron's slope at
+// each node. In the build's `fast` type, returned as such.
 template <bool MIX>
-__device__ double condense_layer(double& th, double& q, double pib, double pt, double pb, const CP& P, const AP& A) {
+__device__ __forceinline__ void layer_cap(double th, double pt, double pb, const CP& P, const AP& A,
+                                          std::conditional_t<MIX, float, double>& qs,
+                                          std::conditional_t<MIX, float, double>& dqs) {
   using F = std::conditional_t<MIX, float, double>;
   constexpr double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
   constexpr double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
-  F qs = 0, dqs = 0;
+  qs = 0;
+  dqs = 0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
@@ -572,6 +599,14 @@ __device__ double condense_layer(double& th, double& q, double pib, double pt, d
     qs += (F)0.5 * (F)wq[j] * s;
     dqs += (F)0.5 * (F)wq[j] * s * (F)P.Lv / ((F)A.Rv * T * T) * ex;
   }
+}
+
+// Saturation adjustment against the layer's capacity over [pt, pb] (air.cu:275-291): one Newton step.
+template <bool MIX>
+__device__ double condense_layer(double& th, double& q, double pib, double pt, double pb, const CP& P, const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  F qs, dqs;
+  layer_cap<MIX>(th, pt, pb, P, A, qs, dqs);
   if (!(q > qs)) return 0;
   const double dq = (F)(q - qs) / ((F)1 + (F)P.Lv / (F)pib * dqs);
   q -= dq;
@@ -609,12 +644,35 @@ __device__ __forceinline__ void parcel(double th, double q, double p, const CP& 
   condense<MIX>(thp, qp, P.cp * ex, p, P, A);
 }
 
+// The cloud statistics (air.cu:383-388): total water qt spread evenly over +-(1 - rh_crit) qs about its mean; the
+// saturated share f and the mean condensate l there.
+__device__ __forceinline__ void cloud_pdf(double qt, double qs, const AP& A, double& l, double& f) {
+  const double b = (1 - A.rh_crit) * qs, s = qt - qs;
+  if (s <= -b) {
+    l = 0;
+    f = 0;
+  } else if (s >= b) {
+    l = s;
+    f = 1;
+  } else {
+    f = (s + b) / (2 * b);
+    l = (s + b) * f / 2;
+  }
+}
+
+// The cover of cloud water cw spread at a cloud's typical water content by phase (air.cu:393-396).
+__device__ __forceinline__ double cover_of(double cw, double mu, double liq, const AP& A, double qs) {
+  const double s = A.cw_qs > 0 && qs > 0 ? fmin(1.0, A.cw_qs * qs / A.q_auto) : 1.0;
+  return fmin(1.0, cw / (s * (liq * A.q_auto + (1 - liq) * A.l_ice)) * fmax(1.0, mu / A.mu_cloud));
+}
+
 struct AirIO {
   double *theta, *qv, *Ts, *ice;
   double *wx, *olr, *sw_abs, *stored, *rain, *evap, *conv, *tsurf, *rainp, *qocean, *qsw, *flash, *swalb, *firn;
+  double *cloudw, *cloudo;
 };
 
-template <int L, bool MIX, bool MOIST>
+template <int L, bool MIX, bool MOIST, bool CLOUD>
 __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, const double* __restrict__ t,
                                                    const double* __restrict__ insol, const double* __restrict__ pos,
                                                    const double* __restrict__ uabs, const double* __restrict__ mu_in,
@@ -646,6 +704,25 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   const double E0 = column_energy<L, MOIST>(mu, pib, th, q, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
 #pragma unroll
   for (int k = 0; k < L; k++) T[k] = pib[k] * th[k] / P.cp;
+  using F = std::conditional_t<MIX, float, double>;
+  // clouds: each layer's cover, in-cloud water path (kg/m^2) and liquid share (all ice below -liq_cold C); a middle
+  // layer's clouds sit high in it, so their phase is by its upper quarter's pressure
+  double cl[L], fc[L], liq[L], cwp[L];
+  if constexpr (CLOUD) {
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      cl[k] = o.cloudw[(size_t)k * N + i];
+      const double ptk = k ? pb[k - 1] : 0.0;
+      const double pc = k == bot ? 0.5 * (pb[k] + ptk) : ptk + 0.25 * (pb[k] - ptk);
+      liq[k] = fmax(0.0, fmin(1.0, (th[k] * fpow_<MIX>(pc / P.p0, P.kappa) - A.Tf + A.liq_cold) / A.liq_cold));
+      F qs, dqs;
+      layer_cap<MIX>(th[k], ptk, ptk + P.grav * mu[k], P, A, qs, dqs);
+      double l, f;
+      cloud_pdf(q[k] + cl[k], qs, A, l, f);
+      fc[k] = fmax(f, cover_of(cl[k], mu[k], liq[k], A, qs));
+      cwp[k] = fc[k] > 1e-6 ? mu[k] * cl[k] / fc[k] : 0.0;
+    }
+  }
   // the air touching the ground: the bottom mu_bl of the column (bl_bot: the thin bottom layer's share of it); the
   // layers above give their air at their lower edge's theta* where warmer
   double wbl[L], thb = 0, qb = 0, mb = 0;
@@ -738,7 +815,17 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       for (int k = 0; k < L; k++) Q[k] += toAir * mu[k] / mt;
     }
   }
-  const double Tc = 1;
+  // clouds reflect: each layer's by conservative two-stream scattering of its optical depth 3 W / (2 rho r), the
+  // layers at random overlap, with the reflections between them and the ground
+  double Tc = 1;
+  if constexpr (CLOUD) {
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      const double tau = 1.5 * cwp[k] * (liq[k] / (1000 * A.re_liq) + (1 - liq[k]) / (P.rho_ice * A.re_ice));
+      const double g1 = (1 - A.cloud_g) * tau;
+      Tc *= 1 - fc[k] * g1 / (2 + g1);
+    }
+  }
   const double swS = swf ? I * swf[(size_t)L * N + i] : (I - toAir) * Tc * (1 - alb) / (1 - alb * (1 - Tc));
 
   // ---- longwave ----
@@ -756,6 +843,15 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 #pragma unroll
     for (int k = 0; k < L; k++)
       eps[k] = omexp<MIX>(tau_p(pb[k], pb[bot], A) - tau_p(k ? pb[k - 1] : 0.0, pb[bot], A));
+  }
+  double eps0[L];
+  if constexpr (CLOUD) {                             // clouds absorb and emit by their water path
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      eps0[k] = eps[k];
+      const double kl = liq[k] * A.k_lw_liq + (1 - liq[k]) * A.k_lw_ice;
+      eps[k] = 1 - (1 - eps[k]) * (1 - fc[k] * omexp<MIX>(kl * cwp[k]));
+    }
   }
   double ldown = 0;
   constexpr int RW = L + 1;
@@ -817,6 +913,30 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       tr0 = tr1;
     }
     up = up0 + radf[(size_t)(2 * RW) * N + i] * Es;
+    if constexpr (CLOUD) {
+      if (o.cloudo) {                                // cover, the clouds' effects at the top, optical depths
+        double* co = o.cloudo + i;
+        double clear = 1;
+#pragma unroll
+        for (int k = 0; k < L; k++) {
+          co[(size_t)k * N] = fc[k];
+          clear *= 1 - fc[k];
+        }
+        co[(size_t)L * N] = 1 - clear;
+        co[(size_t)(L + 1) * N] =
+            swf ? I * (swf[(size_t)(L + 1) * N + i] - (1 - (toAir / fmax(I, 1e-30) + swf[(size_t)L * N + i])))
+                : swS - (I - toAir) * (1 - alb);
+        co[(size_t)(L + 2) * N] = radf[(size_t)(3 * RW) * N + i] + radf[(size_t)(3 * RW + 1) * N + i] * Es - up;
+        const double re_l = 1000 * A.re_liq, re_i = P.rho_ice * A.re_ice;
+#pragma unroll
+        for (int k = 0; k < L; k++) {
+          co[(size_t)(L + 3 + k) * N] = fc[k] > 1e-6 ? (liq[k] * A.k_lw_liq + (1 - liq[k]) * A.k_lw_ice) * cwp[k] : 0.0;
+          co[(size_t)(2 * L + 3 + k) * N] = 1.5 * cwp[k] * (liq[k] / re_l + (1 - liq[k]) / re_i);
+          const double tl_ = liq[k] / re_l, ti_ = (1 - liq[k]) / re_i;
+          co[(size_t)(3 * L + 3 + k) * N] = tl_ + ti_ > 0 ? tl_ / (tl_ + ti_) : 1.0;
+        }
+      }
+    }
   } else {
 #pragma unroll
     for (int k = bot; k >= 0; k--) {
@@ -825,6 +945,23 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double em = eps[k] * A.sigma * T[k] * T[k] * T[k] * T[k];
       Q[k] -= 2 * em;
       up = up - a + em;
+    }
+    if constexpr (CLOUD) {
+      if (o.cloudo) {                                // cover and the clouds' effects at the top
+        double* co = o.cloudo + i;
+        double up0 = A.eps_s * A.sigma * tsk * tsk * tsk * tsk, clear = 1;
+#pragma unroll
+        for (int k = bot; k >= 0; k--)
+          up0 = up0 - eps0[k] * up0 + eps0[k] * A.sigma * T[k] * T[k] * T[k] * T[k];
+#pragma unroll
+        for (int k = 0; k < L; k++) {
+          co[(size_t)k * N] = fc[k];
+          clear *= 1 - fc[k];
+        }
+        co[(size_t)L * N] = 1 - clear;
+        co[(size_t)(L + 1) * N] = swS - (I - toAir) * (1 - alb);
+        co[(size_t)(L + 2) * N] = up0 - up;
+      }
     }
 #pragma unroll
     for (int k = 0; k < L; k++) {
@@ -889,8 +1026,84 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   Ts[i] = Tn;
   o.sw_abs[i] = toAir + swS;
   // ---- condensation: one-step saturation adjustment per layer, its latent heat heating for the crossing ----
-  double R = 0, Rk0 = 0, Rk1 = 0, Rc = 0;
-  if constexpr (MOIST) {
+  double R = 0, Rk0 = 0, Rk1 = 0, Rc = 0, S = 0;
+  if constexpr (CLOUD) {
+    // cloud water forms and erodes; liquid turns to rain (Sundqvist 1989), ice falls out; rain evaporates in the clear
+    // air it falls through. With ice_air the falling ice freezes in its layer (Lf), melts in a layer whose lower edge
+    // is above freezing, sublimates with the rain's evaporation; what is left reaches the ground as snow.
+    const bool icy = A.ice_air > 0;
+    double Fr = 0, Fs = 0, fa = 0, psf = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++) psf += P.grav * mu[k];
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+      double pt = 0;
+#pragma unroll
+      for (int j = 0; j < k; j++) pt += P.grav * mu[j];
+      const double pbk = pt + P.grav * mu[k];
+      // condense up to the statistics' condensate with the latent heat's feedback on the capacity; a cloud holding
+      // more erodes over tau_erode (air.cu:438-450)
+      F qs, dqs;
+      double l, f, dl = 0;
+      layer_cap<MIX>(th[k], pt, pbk, P, A, qs, dqs);
+      if (qs > 0) {
+        cloud_pdf(q[k] + cl[k], qs, A, l, f);
+        dl = l > cl[k] ? (l - cl[k]) / (1 + P.Lv / pib[k] * dqs * f) : -(cl[k] - l) * omexp<MIX>(dt / A.tau_erode);
+        cl[k] += dl;
+        q[k] -= dl;
+        th[k] += P.Lv * dl / pib[k];
+      }
+      dth[k] += P.Lv * dl / pib[k];
+      dlat[k] += P.Lv * dl / pib[k];
+      layer_cap<MIX>(th[k], pt, pbk, P, A, qs, dqs);
+      cloud_pdf(q[k] + cl[k], qs, A, l, f);
+      f = fmax(f, cover_of(cl[k], mu[k], liq[k], A, qs));
+      const double inc = f > 1e-6 ? cl[k] / f * fmax(1.0, mu[k] / A.mu_cloud) : 0.0, r = inc / A.q_auto;
+      const double pm = pt + 0.5 * P.grav * mu[k], Tk = pib[k] * th[k] / P.cp, depth = mu[k] / (pm / (A.rd * Tk));
+      const double rate = liq[k] * A.c_auto * omexp<MIX>(r * r) + (1 - liq[k]) * A.v_ice / depth;
+      const double loss = cl[k] * omexp<MIX>(rate * dt);
+      cl[k] -= loss;
+      Fr += mu[k] * loss / dt;
+      if (icy) {
+        const double fz = rate > 0 ? fmin(1.0, (1 - liq[k]) * A.v_ice / depth / rate) : 0.0, dz = loss * fz;
+        Fs += mu[k] * dz / dt;
+        th[k] += A.Lf * dz / pib[k];
+        dth[k] += A.Lf * dz / pib[k];
+        dlat[k] += A.Lf * dz / pib[k];
+        if (Fs > 0 && th[k] * fpow_<MIX>(pbk / P.p0, P.kappa) > A.Tf) {     // (the layer's lower edge above freezing)
+          const double dm = Fs * dt / (mu[k] * pib[k]) * A.Lf;
+          th[k] -= dm;
+          dth[k] -= dm;
+          dlat[k] -= dm;
+          Fs = 0;
+        }
+      }
+      double ev = 0;
+      if (loss > 0) fa = fmax(fa, f);
+      const double rc_ = A.revap > 0 ? A.revap_rhs : 1.0;
+      if (Fr > 0 && rc_ * qs > q[k]) {
+        if (A.revap > 0) {
+          const double a_ = fmax(fa, A.revap_area), ac = fmax(0.0, a_ - f);
+          ev = ac * 5.44e-4 * (rc_ * qs - q[k]) * pow(sqrt(pm / psf) / 5.09e-3 * Fr / a_, 0.577) * mu[k];
+        } else {
+          ev = A.k_revap * (1 - f) * (1 - q[k] / qs) * sqrt(Fr) * mu[k];
+        }
+        ev = fmin(ev, fmin(Fr, (rc_ * qs - q[k]) * mu[k] / dt / (1 + P.Lv / pib[k] * dqs)));
+        const double es = Fr > 0 ? ev * Fs / Fr : 0.0;
+        const double Le = P.Lv * ev + A.Lf * es;
+        Fr -= ev;
+        Fs -= es;
+        q[k] += ev * dt / mu[k];
+        th[k] -= Le * dt / (mu[k] * pib[k]);
+        dth[k] -= Le * dt / (mu[k] * pib[k]);
+        dlat[k] -= Le * dt / (mu[k] * pib[k]);
+      }
+      if (k == bot) Rk1 += mu[k] * loss - ev * dt;
+      else Rk0 += mu[k] * loss - ev * dt;
+    }
+    R = Fr * dt;
+    S = fmax(0.0, fmin(Fs * dt, R));
+  } else if constexpr (MOIST) {
 #pragma unroll
     for (int k = 0; k < L; k++) {
       double tk = th[k], qk = q[k];
@@ -940,7 +1153,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       pt_ = pbk;
     }
   }
-  crossing<L, MOIST>(P, mu, th, q, dth, et, eb, pib, wxo);
+  crossing<L, MOIST, CLOUD>(P, mu, th, q, dth, et, eb, pib, wxo, cl);
   // ---- moist convection: a parcel of bottom air lifted to the interface above it; buoyant there, it rises through
   // every interface it stays buoyant at and detrains above the last, its condensate raining; the mass that would
   // remove the buoyancy at the first interface relaxes over tau_conv ----
@@ -950,7 +1163,13 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     double pi_ = 0;
 #pragma unroll
     for (int k = 0; k < bot; k++) pi_ += P.grav * mu[k];
-    const double th0 = th[bot], q0 = q[bot];
+    const double th0 = th[bot];
+    double q0 = q[bot];
+    if constexpr (CLOUD) {                           // plumes root in the cell's moist air: its moistest part
+      F qs, dqs;
+      layer_cap<MIX>(th0, pi_, pi_ + P.grav * mu[bot], P, A, qs, dqs);
+      q0 = fmax(q0, fmin((double)qs, q0 + (1 - A.rh_crit) * qs));
+    }
     double thp, qc, thp2, qc2;
     parcel<MIX>(th0, q0, pi_, P, A, thp, qc);
     const double d = thp - tstar<L>(bot - 1, P);
@@ -997,15 +1216,48 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       }
     }
   }
+  // the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the
+  // convection's area revap_area; else on conv_revap of the cell)
+  if constexpr (CLOUD) {
+    if ((A.revap > 0 ? Rc > 0 : A.conv_revap > 0 && !(A.downdraft > 0) && Rc > 0) && L > 1) {
+      double pt_ = 0;
+#pragma unroll
+      for (int k = 0; k < bot; k++) pt_ += P.grav * mu[k];
+      F qs, dqs;
+      layer_cap<MIX>(th[bot], pt_, pt_ + P.grav * mu[bot], P, A, qs, dqs);
+      const double rc_ = A.revap > 0 ? A.revap_rhs : 1.0;
+      if (rc_ * qs > q[bot]) {
+        double ev;
+        const double Fc = Rc / dt;
+        if (A.revap > 0) {
+          const double pm = pt_ + 0.5 * P.grav * mu[bot], ps_ = pt_ + P.grav * mu[bot];
+          ev = A.revap_area * 5.44e-4 * (rc_ * qs - q[bot]) * pow(sqrt(pm / ps_) / 5.09e-3 * Fc / A.revap_area, 0.577) *
+               mu[bot];
+          ev = fmin(ev, fmin(Fc, (rc_ * qs - q[bot]) * mu[bot] / dt / (1 + P.Lv / pib[bot] * dqs)));
+        } else {
+          ev = A.k_revap * (1 - q[bot] / qs) * sqrt(A.conv_revap * Fc) * mu[bot];
+          ev = fmin(ev, fmin(Fc, (qs - q[bot]) * mu[bot] / dt / (1 + P.Lv / pib[bot] * dqs)));
+        }
+        q[bot] += ev * dt / mu[bot];
+        th[bot] -= P.Lv * ev * dt / (mu[bot] * pib[bot]);
+        R -= ev * dt;
+        Rc -= ev * dt;
+      }
+    }
+  }
   if (o.conv) o.conv[i] = mc;
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) o.wx[(size_t)j * N + i] = wxo[j];
   o.rain[i] = R / dt;
+  if (S > 0) {                                       // snow into the sea melts with the slab's heat
+    Ts[i] -= A.Lf * S / Cw;
+    if (o.qocean) o.qocean[i] -= A.Lf * S / dt;
+  }
   if (o.rainp) {
     o.rainp[i] = Rk0 / dt;
     o.rainp[(size_t)N + i] = Rk1 / dt;
     o.rainp[(size_t)2 * N + i] = Rc / dt;
-    o.rainp[(size_t)3 * N + i] = 0 / dt;
+    o.rainp[(size_t)3 * N + i] = S / dt;
   }
   const double E1 = column_energy<L, MOIST>(mu, pib, th, q, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
   if (Cf > 0) o.firn[i] = Tfn;
@@ -1013,6 +1265,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   for (int k = 0; k < L; k++) {
     o.theta[(size_t)k * N + i] = th[k];
     if constexpr (MOIST) o.qv[(size_t)k * N + i] = q[k];
+    if constexpr (CLOUD) o.cloudw[(size_t)k * N + i] = cl[k];
   }
   o.stored[i] = E1 - E0;
 }
@@ -1028,6 +1281,31 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     case 8: CALL(8); break;  \
     default: return ISW_EBADARG; \
   }
+
+#if ISW_COLUMN_CLOUDS
+// The cloudy air column's launches, compiled in a unit of their own (ISW_COLUMN_CLOUDS=1) so that its fourteen
+// kernels build in parallel with the rest of this file.
+int air_column_cloudy(int N, int L, int mixed, const CP& P, const AP& A, int mode, const double* t,
+                      const double* insol, const double* pos, const double* uabs, const double* mu, double dt,
+                      const double* cap, const double* elev, const double* radf, const double* swf, const double* ch,
+                      const AirIO& o, double bytes, hipStream_t st) {
+#define CALL(LL)                                                                                                     \
+  if (mixed)                                                                                                         \
+    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, true, true, true>), grid1(N), dim3(BS), N, P, A, mode, t,      \
+               insol, pos, uabs, mu, dt, cap, elev, radf, swf, ch, o);                                               \
+  else                                                                                                               \
+    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, false, true, true>), grid1(N), dim3(BS), N, P, A, mode, t,     \
+               insol, pos, uabs, mu, dt, cap, elev, radf, swf, ch, o);
+  ISW_BY_L(CALL)
+#undef CALL
+  return (int)hipGetLastError();
+}
+#else
+// (defined in the ISW_COLUMN_CLOUDS=1 unit)
+int air_column_cloudy(int N, int L, int mixed, const CP& P, const AP& A, int mode, const double* t,
+                      const double* insol, const double* pos, const double* uabs, const double* mu, double dt,
+                      const double* cap, const double* elev, const double* radf, const double* swf, const double* ch,
+                      const AirIO& o, double bytes, hipStream_t st);
 
 int hs_column(int N, int L, const double* par, const double* hs, const double* lat, const double* mu,
               double* theta, double dt, double* wx, double* qnet, double* stored, hipStream_t st) {
@@ -1056,22 +1334,26 @@ int air_column(int N, int L, const double* par, const isw_air* air, int mode, co
                double* ice, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
                double* evap, double* conv, double* tsurf, double* rainp, const double* cap, double* qocean,
                double* qsw, const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
-               const double* ch, double* firn, int mixed, hipStream_t st) {
+               const double* ch, double* firn, double* cloudw, double* cloudo, int mixed, hipStream_t st) {
   const CP P = make_cp(par);
   const AP A = make_ap(par, air);
-  const bool moist = P.moist > 0;
+  const bool moist = P.moist > 0, cloudy = moist && par[55] > 0 && cloudw;
   const AirIO o{theta, qv, Ts, ice, wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, qocean, qsw, flash,
-                swalb, firn};
+                swalb, firn, cloudw, cloudo};
   const int RW = L + 1;
   const double rows_in = 1 + 2.0 * L + 1 + (mode == 0 ? 3 : 1) + (cap ? 1 : 0) + (elev ? 1 : 0) + (ch ? 1 : 0) +
                          (radf ? 3.0 * RW + 6 * L : 0) + (swf ? 3.0 * L + 1 : 0) + (firn ? 1 : 0) +
-                         (moist ? L + 1.0 : 0);
+                         (moist ? L + 1.0 : 0) + (cloudy ? L : 0);
   const double rows_out = L + 1 + (L - 1) + 6 + (rainp ? 4 : 0) + (qocean ? 1 : 0) + (qsw ? 1 : 0) + (flash ? 1 : 0) +
-                          (swalb ? 1 : 0) + (firn ? 1 : 0) + (moist ? L + 1.0 : 0);
+                          (swalb ? 1 : 0) + (firn ? 1 : 0) + (moist ? L + 1.0 : 0) + (cloudy ? L : 0) +
+                          (cloudy && cloudo ? (radf ? 4.0 * L + 3 : L + 3.0) : 0);
   const double bytes = 8.0 * N * (rows_in + rows_out);
+  if (cloudy)
+    return air_column_cloudy(N, L, mixed, P, A, mode, t, insol, pos, uabs, mu, dt, cap, elev, radf, swf, ch, o, bytes,
+                             st);
 #define LAUNCH(LL, MX, MO)                                                                                         \
-  ISW_LAUNCH("air_column", bytes, (k_air_column<LL, MX, MO>), grid1(N), dim3(BS), N, P, A, mode, t, insol, pos,     \
-             uabs, mu, dt, cap, elev, radf, swf, ch, o)
+  ISW_LAUNCH("air_column", bytes, (k_air_column<LL, MX, MO, false>), grid1(N), dim3(BS), N, P, A, mode, t, insol,  \
+             pos, uabs, mu, dt, cap, elev, radf, swf, ch, o)
 #define CALL(LL)                           \
   if (mixed) {                             \
     if (moist) LAUNCH(LL, true, true);     \
@@ -1111,4 +1393,5 @@ int advance_time(double* t, double dt, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+#endif  // ISW_COLUMN_CLOUDS
 }  // namespace isw_col

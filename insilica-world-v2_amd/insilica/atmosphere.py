@@ -107,7 +107,7 @@ class AirPhysics:
     longwave and shortwave), and the device clock. Parameters in the reference's PAR_ORDER array (`par`, host).
 
     forcing: 'held_suarez' steps the Held & Suarez column (air.cu hs_column); 'radiation' the reference's air column
-    (air.cu air_column), dry or moist over the sea without clouds (clouds and land are not built here). mean_sun: the
+    (air.cu air_column), dry or moist over the sea, with or without clouds (land is not built here). mean_sun: the
     annual-mean insolation instead of the diurnal and seasonal sun. longwave='ktables' builds `lw`, `sw` from
     lw_tables, sw_tables (npz paths or dicts). `cloudo` ((4L + 3), N): per layer cover, total cover, the clouds'
     effect at the top, per layer longwave and shortwave in-cloud optical depths and liquid share."""
@@ -189,9 +189,11 @@ class AirPhysics:
         (radf/swf of the last radiate()), the slab surface at Ts (cap its heat capacity per cell when given; qocean,
         qsw the ocean's forcing), the surface's sensible heat (the surface layer's ch with sfc > 0) and the crossing;
         for moist air (moist > 0; q (L, N) and ice (N) updated in place) evaporation, sea ice, condensation and the
-        parcel plume. Into theta, Ts, self.wx, self.olr, self.sw_abs, self.stored, self.rain, self.evap, self.conv,
-        self.tsurf, self.rainp, self.flash. Clouds (cw), land and the column's gas tracers (gx) are not in this
-        library and raise."""
+        parcel plume; with clouds (cloud > 0 and the cloud water cw (L, N), updated in place) the cloud statistics,
+        autoconversion, ice fall and rain evaporation, the clouds' reflection and emission, and their cover and
+        optical depths into self.cloudo for the next radiate(). Into theta, Ts, self.wx, self.olr, self.sw_abs,
+        self.stored, self.rain, self.evap, self.conv, self.tsurf, self.rainp, self.flash. Land and the column's gas
+        tracers (gx) are not in this library and raise."""
         from . import _hip
         if self.forcing == 'held_suarez':
             _hip.call('isw_hs_column', self.dg.N, self.L, self._par, self.hs.ctypes.data_as(ctypes.c_void_p),
@@ -201,8 +203,8 @@ class AirPhysics:
             return
         if self.moist and (q is None or ice is None):
             raise ValueError('the moist column needs the vapor q and the sea ice')
-        if (cw is not None and self.p['cloud'] > 0) or land is not None:
-            raise NotImplementedError('the column over land and with clouds is not in this library')
+        if land is not None:
+            raise NotImplementedError('the column over land is not in this library')
         if gx is not None:
             raise NotImplementedError('the gas tracers carried by the column\'s crossing are not in this library')
         self._speed(u)
@@ -212,8 +214,9 @@ class AirPhysics:
                   P(self.insol), P(self.pos), P(self.uabs), P(mu), P(theta), P(q) if self.moist else None, P(Ts),
                   P(ice) if self.moist else None, float(dt),
                   P(self.wx), P(self.olr), P(self.sw_abs), P(self.stored), P(self.rain), P(self.evap), P(self.conv),
-                  P(self.tsurf), P(self.rainp), None, P(cap), P(qocean), P(qsw), None, None, None, None, None, None,
-                  None, P(elev), P(self.flash), P(lw.radf) if lw is not None else None,
+                  P(self.tsurf), P(self.rainp), None, P(cap), P(qocean), P(qsw), None, None, None, None, None, P(cw),
+                  P(self.cloudo) if cw is not None else None, P(elev), P(self.flash),
+                  P(lw.radf) if lw is not None else None,
                   P(sw.swf) if sw is not None else None, P(self.swalb) if sw is not None else None,
                   P(self.ch) if self.sfc else None, None, P(firn), None, self._mixed, _hip.stream())
         self.advance_time(t, dt)

@@ -2,7 +2,8 @@
 
 core.hip and radiation.hip are compiled twice each: the plain FP64 form with -ffp-contract=off (every product rounds on its own) and the
 mixed form with -ffp-contract=fast (radiation.hip: fast-honor-pragmas, so that its set-up kernel's own
-`fp contract(off)` holds). All HIP code is compiled with packed-FP32 instructions disabled.
+`fp contract(off)` holds). column.hip is compiled twice too, its cloudy air-column kernels (ISW_COLUMN_CLOUDS=1) apart
+from the rest, so that the two build in parallel. All HIP code is compiled with packed-FP32 instructions disabled.
 """
 import os
 import subprocess
@@ -38,10 +39,12 @@ def build_hip(verbose=False):
              ('radiation.hip', 'radiation_f64.o', ['-DISW_FORM_MIXED=0', '-ffp-contract=off']),
              ('radiation.hip', 'radiation_mix.o', ['-DISW_FORM_MIXED=1', '-ffp-contract=fast-honor-pragmas']),
              ('column.hip', 'column.o', ['-ffp-contract=off']),
+             ('column.hip', 'column_clouds.o', ['-DISW_COLUMN_CLOUDS=1', '-ffp-contract=off']),
              ('capi.hip', 'capi.o', ['-ffp-contract=off'])]
-    # the units compile in parallel (each hipcc is one process; at most ISW_BUILD_JOBS at a time, default 4)
+    # the units compile in parallel (each hipcc is one process; at most ISW_BUILD_JOBS at a time, default one per unit
+    # up to the CPU count)
     from concurrent.futures import ThreadPoolExecutor
-    jobs = max(1, int(os.environ.get('ISW_BUILD_JOBS', '4')))
+    jobs = max(1, int(os.environ.get('ISW_BUILD_JOBS', str(min(len(units), os.cpu_count() or 4)))))
 
     def one(unit):
         src, obj, extra = unit

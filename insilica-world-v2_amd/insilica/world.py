@@ -50,7 +50,7 @@ class World:
         g = self.grid
         self.air = preset in ('coupled_stack', 'held_suarez', 'aqua_dry', 'aqua', 'aqua5', 'aqua7')
         self.sea = self.physics = None
-        self.Ts = self.ice = None
+        self.Ts = self.ice = self.cw = None
         self.rad_every = 0
         if preset in ('aqua_dry', 'aqua', 'aqua5', 'aqua7'):
             # three air layers at rest over a slab aquaplanet under the diurnal and seasonal sun, gray radiation (or
@@ -59,6 +59,9 @@ class World:
             # of the mass it moves
             from .atmosphere import AirPhysics
             moist = preset != 'aqua_dry'
+            # with clouds (physics cloud > 0, moist) the cloud water per layer, starting at none and moved as a tracer
+            # (reference: world.py:145-148, 294-296)
+            cloudy = moist and P.get('physics', {}).get('cloud', 0.0) > 0
             stars = cases.AQUA_N[preset][0] if preset in cases.AQUA_N else [380.0, 305.0]
             if preset in cases.AQUA_N:
                 h, b, u, extra = cases.aqua_layers(g, *cases.AQUA_N[preset])
@@ -74,14 +77,17 @@ class World:
                                rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP),
                                air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY), vmix=P.get('vmix', 0.0),
                                thin_wind=P.get('thin_wind', THIN_WIND))
-            self.rows = dg.asarray(np.concatenate([extra['theta'], extra['q']]) if moist else extra['theta'])
+            rows = [extra['theta']] + ([extra['q']] if moist else []) + ([np.zeros_like(extra['q'])] if cloudy else [])
+            self.rows = dg.asarray(np.concatenate(rows))
             self.theta = self.rows[0:L]
             if moist:
                 self.q = self.rows[L:2 * L]
                 self.ice = dg.zeros(g.N)
+            if cloudy:
+                self.cw = self.rows[2 * L:3 * L]
             self.Ts = dg.asarray(extra['Ts'])
             self.psi = dg.asarray(np.tile(cases.cosine_bell(g), (L, 1)))      # moved by the exchange only
-            self.tr = Transport(dg, layers=L, beta=P.get('beta', BETA), tracers=2 if moist else 1)
+            self.tr = Transport(dg, layers=L, beta=P.get('beta', BETA), tracers=len(rows))
             phys = dict(theta_stars=list(stars), floor_=floor, grav=self.grav, moist=1.0 if moist else 0.0)
             if self.grav != GRAV:
                 phys.update(mu_ref=1.0e5 / self.grav, mu_bl=1.0e4 / self.grav)
@@ -211,11 +217,12 @@ class World:
             # the radiation step (the k-tables, saved for the column steps until the next), the column (the
             # radiation or Held & Suarez's relaxation, the crossing into wx, the clock), then the exchange of that
             # mass with its theta, momentum and tracer between the layers (reference: world.py:319-344)
-            if kind & 2:
-                self.physics.radiate(self.h, self.theta, self.Ts, self.time, self.rad_every * self.dt)
             moist = self.ice is not None
+            if kind & 2:
+                self.physics.radiate(self.h, self.theta, self.Ts, self.time, self.rad_every * self.dt,
+                                     q=self.q if moist else None, cw=self.cw)
             self.physics.column(self.h, self.theta, self.Ts, self.u, self.time, self.dt, elev=self.b,
-                                q=self.q if moist else None, ice=self.ice if moist else None)
+                                q=self.q if moist else None, ice=self.ice if moist else None, cw=self.cw)
             self.core.exchange(self.physics.wx, self.h, self.u, self.M, self.psi, theta=self.theta)
 
     def launch_tick(self, graph=True):

@@ -2,7 +2,7 @@
 
 Restates the tick order of the reference's World._tick (src/insilica/world.py:286-358) for the parts this build
 covers (the layered core, FCT transport, the air-sea stress with its heat, the land column, the hourly ocean step
-without its column mixing, the Held-Suarez column and the gray radiative air column, dry or moist) and the
+without its column mixing, the Held-Suarez column and the gray radiative air column, dry or moist, with or without clouds) and the
 Ocean.step order (src/insilica/ocean.py:238-252), on the CPU oracle. Built from a
 product World's settings and starting state, so that the two can be stepped side by side.
 """
@@ -47,6 +47,7 @@ class OracleWorld:
                                     insol=_np(p.insol) if p.insol is not None else None, b=_np(w.b))
                 self.Ts = _np(w.Ts)
                 self.ice = _np(w.ice) if w.ice is not None else None
+                self.cloudy = getattr(w, 'cw', None) is not None      # (the cloud water: rows 2L..3L)
             self.psi, self.time = _np(w.psi), float(w.time[0])
         if w.sea is not None:
             s = w.sea
@@ -108,12 +109,16 @@ class OracleWorld:
             else:
                 uabs = cell_speed(ph['grid'], self.u[L - 1])
                 moist = self.ice is not None
+                cloudy = getattr(self, 'cloudy', False)
                 o = air_column(ph['par'], ph['mode'], np.array([self.time]), ph['pos'], uabs, self.h, self.rows[0:L],
                                self.Ts, self.dt, insol=ph['insol'], elev=ph['b'],
-                               q=self.rows[L:2 * L] if moist else None, ice=self.ice)
+                               q=self.rows[L:2 * L] if moist else None, ice=self.ice,
+                               cw=self.rows[2 * L:3 * L] if cloudy else None)
                 th, wx, self.Ts = o['theta'], o['wx'], o['Ts']
                 if moist:
                     self.rows[L:2 * L], self.ice = o['q'], o['ice']
+                if cloudy:
+                    self.rows[2 * L:3 * L] = o['cw']
             self.rows[0:L] = th
             self.time += self.dt
             self.core.exchange(wx[:L - 1], self.h, self.u, self.M, self.psi, theta=self.rows[0:L])

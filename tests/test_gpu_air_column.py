@@ -1,5 +1,5 @@
-"""The air column (dry, and moist over the sea) on the GPU against the CPU oracle, through the C ABI (isw_air_column),
-and the aqua_dry and aqua worlds against the oracle's tick.
+"""The air column (dry, and moist over the sea with and without clouds) on the GPU against the CPU oracle, through the C
+ABI (isw_air_column), and the aqua_dry and aqua worlds against the oracle's tick.
 
 Bars: FP64 <= 1e-12 relative (max |diff| / max |oracle|) on theta, Ts, the crossing's masses, olr, sw_abs, the
 surface outputs and (moist) vapor, ice, rain, evaporation and the plumes' mass (the device's pow/exp/sin/cos/asin/fmod
@@ -50,7 +50,7 @@ def columns(N, L, seed):
 
 
 def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None, swf=None, ch=None, firn=None,
-                   mixed=False, ocean=False, q=None, ice=None):
+                   mixed=False, ocean=False, q=None, ice=None, cw=None, cloudo=False):
     from insilica import _hip
     from insilica.radiation import _air
     L, N = c['mu'].shape
@@ -64,6 +64,10 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
         o['firn'] = _dev(firn)
     if q is not None:
         o['q'], o['ice'] = _dev(q), _dev(ice)
+    if cw is not None:
+        o['cw'] = _dev(cw)
+        if cloudo:
+            o['cloudo'] = z(4 * L + 3, N)
     ins = [_dev(x) for x in (np.array([t]), insol, c['pos'], c['uabs'], c['mu'], cap, elev, radf, swf, ch)]
     tt, ti, tp, tu, tm, tc, te, tr, ts, tch = ins
     P = _hip.ptr
@@ -72,16 +76,16 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
               P(tp), P(tu), P(tm), P(o['theta']), P(o.get('q')), P(o['Ts']), P(o.get('ice')), float(dt), P(o['wx']),
               P(o['olr']),
               P(o['sw_abs']), P(o['stored']), P(o['rain']), P(o['evap']), P(o['conv']), P(o['tsurf']),
-              P(o['rainp']), None, P(tc), P(o.get('qocean')), P(o.get('qsw')), None, None, None, None, None, None,
-              None, P(te), P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), None, P(o.get('firn')), None,
+              P(o['rainp']), None, P(tc), P(o.get('qocean')), P(o.get('qsw')), None, None, None, None, None,
+              P(o.get('cw')), P(o.get('cloudo')), P(te), P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), None, P(o.get('firn')), None,
               1 if mixed else 0, _hip.stream())
     torch.cuda.synchronize()
     return {k: _np(v) for k, v in o.items()}
 
 
-def compare(g, o, tol, dt, label):
-    keys = [k for k in ('theta', 'Ts', 'wx', 'olr', 'sw_abs', 'tsurf', 'qocean', 'qsw', 'swalb', 'firn', 'q', 'ice')
-            if k in o] + (['rain', 'evap', 'conv', 'rainp'] if 'q' in o else [])
+def compare(g, o, tol, dt, label, bars=None):
+    keys = [k for k in ('theta', 'Ts', 'wx', 'olr', 'sw_abs', 'tsurf', 'qocean', 'qsw', 'swalb', 'firn', 'q', 'ice',
+                        'cw', 'cloudo') if k in o] + (['rain', 'evap', 'conv', 'rainp'] if 'q' in o else [])
     e = {k: relmax(g[k], o[k]) for k in keys}
     e['stored'] = relmax(g['stored'], o['stored'])
     print(label, {k: float(f'{v:.3g}') for k, v in e.items()}, 'moved', int(np.count_nonzero(o['wx'])), flush=True)
@@ -89,6 +93,7 @@ def compare(g, o, tol, dt, label):
     # a ~600x cancellation of one-ulp pow/exp differences (device against glibc): 1e-13..1e-12 in FP64, and in the
     # mixed form's float32 parcels ~1e-4; the masses it moves and its rain carry a share of that
     bar = {'conv': 1e-10, } if tol < 1e-9 else {'conv': 1e-3, 'wx': 1e-4, 'rain': 1e-4, 'rainp': 1e-4}
+    bar.update(bars or {})
     for k in keys:
         assert e[k] <= max(tol, bar.get(k, 0.0)), k
     assert e['stored'] <= max(tol, 1e-9)
@@ -152,10 +157,11 @@ def test_ktables(edge_heat, mixed):
     compare(g, o, 1e-6 if mixed else 1e-12, dt, f'k-tables edge_heat={edge_heat} mixed={mixed}')
 
 
-def moist_columns(N, L, seed):
+def moist_columns(N, L, seed, polar=0.0):
     c = columns(N, L, seed)
     rng = np.random.default_rng(seed + 50)
     c['theta'][-1] += 15.0                                  # a warm bottom layer: the plumes go
+    c['theta'][-1] -= polar * c['pos'][:, 2] ** 2            # (polar: that much colder at the poles)
     pe = np.concatenate([np.zeros((1, N)), np.cumsum(GRAV * c['mu'], 0)])
     pm = 0.5 * (pe[1:] + pe[:-1])
     T = c['theta'] * (pm / 1e5) ** (287.04 / 1004.6)
@@ -199,17 +205,73 @@ def test_moist_ktables_dry_edge():
     compare(g, o, 1e-12, dt, 'moist k-tables dry_edge')
 
 
+def cloudy_columns(N, L, seed):
+    c = moist_columns(N, L, seed, polar=45.0)                # (cold polar columns: ice clouds, snow)
+    rng = np.random.default_rng(seed + 70)
+    # cloud water in most cells, up to a few times the autoconversion threshold; none in the rest
+    c['cw'] = np.where(rng.uniform(size=(L, N)) < 0.7, rng.uniform(0, 1.2e-3, (L, N)), 0.0)
+    return c
+
+
+@pytest.mark.parametrize('L,mixed,opts', [(3, False, {}), (5, False, dict(ice_air=1.0, revap=1.0, lw_w0=5.0)),
+                                          (4, False, dict(conv_revap=0.05, cw_qs=1.0, sat_layer=0.0)),
+                                          (3, True, {}), (7, True, dict(ice_air=1.0, conv_revap=0.05))])
+def test_clouds(L, mixed, opts):
+    """The moist column with clouds: the cloud water, the cover and the clouds' effects at the top (cloudo's first
+    L + 3 rows) to the moist column's bars. The mixed form in seven layers: the masses crossing an interface to 3e-4
+    and the vapor to 3e-5 (measured 1.7e-4 and 1.3e-5 in the worst of 40962 cells, 22 and 2 cells above 1e-5; no
+    branch flips: a crossing's mass is the layer's net heating, radiative cooling less float32 latent heating, times
+    its edge density, so one-ulp device-against-glibc expm1f/powf differences in the autoconversion and condensation
+    reach it amplified)."""
+    from oracle import column as ocol
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column clouds L={L} mixed={mixed} {opts} N={N}', flush=True)
+    c = cloudy_columns(N, L, seed=60 + L)
+    p = par(L, moist=1.0, cloud=1.0, **opts)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, q=c['q'], ice=c['ice'], cw=c['cw'], cloudo=True)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed,
+                        q=c['q'], ice=c['ice'], cw=c['cw'], cloudo=True)
+    assert o['rain'].max() > 0 and o['cw'].max() > 0 and o['cloudo'][L].max() > 0.5
+    if 'ice_air' in opts:
+        assert o['rainp'][3].max() > 0                      # snow reached the ground
+    compare(g, o, 1e-5 if mixed else 1e-12, dt, f'clouds L={L} mixed={mixed}',
+            bars=dict(wx=3e-4, q=3e-5) if mixed else None)
+
+
+def test_clouds_ktables():
+    """Clouds under the k-tables' radiation: every one of cloudo's 4L + 3 rows (cover, the effects at the top, the
+    in-cloud longwave and shortwave optical depths and liquid share)."""
+    from oracle import column as ocol
+    from oracle import radiation as orad
+    from insilica.ktables import synthetic_lw, synthetic_sw
+    L, N, dt = 5, 8192, 900.0
+    print(f'case: air_column clouds k-tables N={N}', flush=True)
+    c = cloudy_columns(N, L, seed=81)
+    p = par(L, moist=1.0, cloud=1.0, edge_k=2.0, edge_heat=1.0, ice_air=1.0)
+    lwt, swt = orad.LwTables(synthetic_lw()), orad.SwTables(synthetic_sw())
+    sub = orad.lw_setup(N, L, p, lwt, c['mu'], c['theta'], c['q'])
+    radf = orad.lw_column(N, L, p, lwt, sub, c['Ts'])
+    swf = orad.sw_column(N, L, p, swt, sub, np.full(N, 0.06), mode=1)
+    kw = dict(insol=np.full(N, 340.0), radf=radf, swf=swf, q=c['q'], ice=c['ice'], cw=c['cw'], cloudo=True)
+    g = gpu_air_column(p, 1, 0.0, c, dt, **kw)
+    o = ocol.air_column(p, 1, np.array([0.0]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, **kw)
+    assert np.abs(o['cloudo'][2 * L + 3:3 * L + 3]).max() > 0
+    compare(g, o, 1e-12, dt, 'clouds k-tables')
+
+
 def test_aqua_dry_world():
     """The 'aqua_dry' world (three dry air layers over the slab, the sun, gray radiation, the column and the exchange
     every tick) against the oracle's tick, 12 ticks at level 4 in the FP64 form (<= 1e-10, as the core's own
     multi-step bar); graph replay bit-identical to direct launches; the column moved air across an interface. The
-    annual-mean sun (mean_sun) likewise."""
+    annual-mean sun (mean_sun), the moist worlds and the moist worlds with clouds likewise."""
     from insilica.world import World
     from oracle.world import OracleWorld
-    for preset, mean_sun in (('aqua_dry', False), ('aqua_dry', True), ('aqua', False), ('aqua5', False),
-                             ('aqua7', False)):
-        print(f'case: {preset} world level 4, 12 ticks, mean_sun={mean_sun}', flush=True)
-        w = World(4, preset, precision='double', params=dict(mean_sun=mean_sun))
+    for preset, mean_sun, phys in (('aqua_dry', False, {}), ('aqua_dry', True, {}), ('aqua', False, {}),
+                                   ('aqua5', False, {}), ('aqua7', False, {}),
+                                   ('aqua', False, dict(cloud=1.0, ice_air=1.0, rh_crit=0.6)),
+                                   ('aqua5', False, dict(cloud=1.0, rh_crit=0.6))):
+        print(f'case: {preset} world level 4, 12 ticks, mean_sun={mean_sun} {phys}', flush=True)
+        w = World(4, preset, precision='double', params=dict(mean_sun=mean_sun, physics=phys))
         o = OracleWorld(w)
         w.step(12, graph=False)
         o.step(12)
@@ -219,7 +281,9 @@ def test_aqua_dry_world():
         print(f'{preset} world errors', {k: float(f'{v:.3g}') for k, v in errs.items()}, flush=True)
         assert max(errs.values()) <= 1e-10
         assert np.count_nonzero(_np(w.physics.wx)) > 0
-        w2 = World(4, preset, precision='double', params=dict(mean_sun=mean_sun))
+        if phys:
+            assert _np(w.cw).max() > 0                    # clouds formed (the start's vapor at 70% of saturation below)
+        w2 = World(4, preset, precision='double', params=dict(mean_sun=mean_sun, physics=phys))
         w2.step(12, graph=True)
         assert w2.state_hash() == w.state_hash()
 
