@@ -189,7 +189,8 @@ def rad_ops(form):
 def col_line(pm, pd, em, ed, steps):
     """The moist air column's pass beside the radiation: steps/s in both forms and its algorithmic bytes over its
     measured duration (an N-sized pass bound by its per-column arithmetic, reported against HBM for scale)."""
-    out = dict(what='air_column, moist over the sea, k-table fluxes, 40962 x 7', steps_per_s=round(steps / em, 1),
+    out = dict(what='air_column, moist with clouds over the sea, k-table fluxes, 40962 x 7',
+               steps_per_s=round(steps / em, 1),
                fp64_steps_per_s=round(steps / ed, 1) if ed else None)
     for form, pr in (('mixed', pm), ('fp64', pd)):
         if pr and 'air_column' in pr:
@@ -204,7 +205,7 @@ def radiation_main(a):
     """The column-radiation workload: 40,962 columns x 7 layers (the level-6 mesh's cells), synthetic k-tables with
     16 x 16 longwave and 14 x 16 shortwave g-points; one step = one radiation pass (lw_setup, lw_column, sw_column)
     over every column, the state resident in HBM. Mixed form as `value`, the FP64 form beside it. The moist air
-    column's step (column physics with the pass's k-table fluxes) is timed beside it (`column`)."""
+    column's step with clouds (column physics with the pass's k-table fluxes) is timed beside it (`column`)."""
     import ctypes
     import numpy as np
     import torch
@@ -226,11 +227,14 @@ def radiation_main(a):
     t = dev([2.0e6])
     N = g.N
 
-    # the column physics beside it: the moist air column over the sea (the k-tables' fluxes of the pass, the sun,
-    # evaporation, condensation, plumes, the crossing of interfaces between the layers' median theta)
-    col_par = par_array(L, dict(moist=1.0, edge_k=2.0, theta_stars=[620.0, 485.0, 410.0, 360.0, 334.0, 312.0]))
+    # the column physics beside it: the moist air column over the sea with clouds (the k-tables' fluxes of the pass,
+    # the sun, evaporation, the cloud scheme with ice fall and snow, plumes, the crossing of interfaces between the
+    # layers' median theta); cloud water 0.3 g/kg in the synthetic clouds' cover
+    col_par = par_array(L, dict(moist=1.0, cloud=1.0, ice_air=1.0, edge_k=2.0,
+                                theta_stars=[620.0, 485.0, 410.0, 360.0, 334.0, 312.0]))
     rng = np.random.default_rng(a.seed + 7)
     uabs = dev(rng.uniform(0.0, 15.0, N))
+    cw0 = 3e-4 * np.asarray(c['cloudo'][:L])
     air = _air(None)
 
     def build(form):
@@ -239,6 +243,7 @@ def radiation_main(a):
                        band_top=swk['band_top'])
         z = lambda *sh: torch.zeros(*sh, dtype=torch.float64, device='cuda')
         cs = dict(theta=th.clone(), q=q.clone(), Ts=Ts.clone(), ice=z(N), wx=z(L - 1, N), rainp=z(4, N),
+                  cw=dev(cw0), co=z(4 * L + 3, N),
                   **{k: z(N) for k in ('olr', 'sw_abs', 'stored', 'rain', 'evap', 'conv', 'tsurf', 'flash')})
         return lw, sw, cs
 
@@ -251,8 +256,8 @@ def radiation_main(a):
         _hip.call('isw_air_column', N, L, col_par.ctypes.data_as(ctypes.c_void_p), ctypes.byref(air), 0, P(t), None,
                   P(p), P(uabs), P(mu), P(cs['theta']), P(cs['q']), P(cs['Ts']), P(cs['ice']), 450.0, P(cs['wx']),
                   P(cs['olr']), P(cs['sw_abs']), P(cs['stored']), P(cs['rain']), P(cs['evap']), P(cs['conv']),
-                  P(cs['tsurf']), P(cs['rainp']), None, None, None, None, None, None, None, None, None, None, None,
-                  None, P(cs['flash']), P(lw.radf), P(sw.swf), None, None, None, None, None,
+                  P(cs['tsurf']), P(cs['rainp']), None, None, None, None, None, None, None, None, None, P(cs['cw']),
+                  P(cs['co']), None, P(cs['flash']), P(lw.radf), P(sw.swf), None, None, None, None, None,
                   1 if form == 'mixed' else 0, _hip.stream())
 
     def timed(form):
@@ -323,7 +328,7 @@ def radiation_main(a):
             reps = 0
             while reps < 3 or time.perf_counter() - t0 < 2.0:
                 ocol.air_column(col_par, 0, np.array([2.0e6]), pos, u_h, c['mu'], c['theta'], c['Ts'], 450.0,
-                                radf=radf, swf=swf, q=c['q'], ice=np.zeros(N))
+                                radf=radf, swf=swf, q=c['q'], ice=np.zeros(N), cw=cw0, cloudo=True)
                 reps += 1
             cpu['column_steps_per_s'] = round(reps / (time.perf_counter() - t0), 3)
     line = dict(metric='radiation passes/s: 40962 columns x 7 layers, LW 256 + SW 224 g-points (FP64 vs mixed, 1 GPU)',
