@@ -364,11 +364,12 @@ void og_gas_src(int N, int L, int G, const double* src, const double* h, double 
     for (int i = 0; i < N; i++) x[((size_t)g * L + L - 1) * N + i] += src[(size_t)g * N + i] * dt / h[(size_t)(L - 1) * N + i];
 }
 
-/* ---- air_column (air.cu:876-1972): the dry column, and the moist one without clouds, land or the mixed layer ----
+/* ---- air_column (air.cu:876-1972): the dry column, and the moist one with clouds and land, without the mixed layer ----
  * The sun (mode 0: the diurnal and seasonal sun at t; mode 1: insol), the shortwave split (the k-tables' swf, or by
  * the s^4 or tau profile, or by mass), the gray longwave (the vapor's part from the column's vapor with lw_w0; or the
- * k-tables' radf), the slab surface (or, moist, the sea-ice skin) under the bulk or surface-layer exchange, the
- * evaporation, the heating as theta, the condensation, the crossing with the k-tables' edge heating, and the moist
+ * k-tables' radf), the slab surface (or, moist, the sea-ice skin; over land the ground with its soil bucket, ponds,
+ * snow, firn and skin) under the bulk or surface-layer exchange, the evaporation, the heating as theta, the
+ * condensation (with clouds the cloud scheme), the crossing with the k-tables' edge heating, and the moist
  * convection's parcel plume. `mix` selects the reference's mixed build (air.cu:24-36): 1 - e^-x, the exponentials and
  * powers of the saturation and condensation arithmetic, and that arithmetic itself in float32 (`fast`). */
 enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5, P_U0 = 6, P_ALB = 7, P_C_SLAB = 8,
@@ -379,7 +380,10 @@ enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5,
        P_CLOUD = 55, P_RH_CRIT = 56, P_RE_LIQ = 57, P_RE_ICE = 58, P_CLOUD_G = 59, P_K_LW_LIQ = 60, P_K_LW_ICE = 61,
        P_C_AUTO = 62, P_Q_AUTO = 63, P_V_ICE = 64, P_TAU_ERODE = 66, P_K_REVAP = 67, P_MU_CLOUD = 68, P_L_ICE = 79,
        P_CONV_REVAP = 87, P_DOWNDRAFT = 92, P_REVAP = 103, P_REVAP_RHS = 104, P_REVAP_AREA = 106, P_LIQ_COLD = 128,
-       P_CW_QS = 129 };
+       P_CW_QS = 129, P_ALB_DRY = 35, P_ALB_WET = 36, P_SOIL_BETA = 37, P_K_INF = 38, P_POND_COVER = 39,
+       P_ALB_SNOW = 40, P_ALB_SNOW_MELT = 42, P_ALB_GLACIER = 43, P_SNOW_COVER = 44, P_LAND_EXCH = 46, P_C_DRY = 49,
+       P_C_WET = 50, P_LAND_RS = 52, P_C_DEEP = 94, P_RS_MAX = 95, P_RS_SUN = 96, P_REVAP_RHL = 105, P_BARE_FX = 115,
+       P_K_FIRN = 121 };
 
 static double omexp(double x, int mix) { return mix ? (double)-expm1f(-(float)x) : 1 - exp(-(x)); }
 static double fpow_(double x, double y, int mix) { return mix ? (double)powf((float)x, (float)y) : pow(x, y); }
@@ -572,9 +576,10 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
                    const double* insol, const double* pos, const double* uabs, const double* mu_in, double* theta,
                    double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,
                    double* stored, double* rain, double* evap, double* conv, double* tsurf, double* rainp,
-                   const double* cap, double* qocean, double* qsw, const double* elev, double* flash,
+                   double* cap, double* qocean, double* qsw, const double* elev, double* flash,
                    const double* radf, const double* swf, double* swalb, const double* ch, double* firn,
-                   double* cloudw, double* cloudo, int mix) {
+                   double* cloudw, double* cloudo, const double* land, const double* soilcap, double* soil,
+                   double* pond, double* snow, const double* lc, int mix) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++) {
     double par[132 + OG_LMAX];
@@ -600,6 +605,12 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     const int top = 0, bot = L - 1;
     const double exb = pow(pb[bot] / p0, kappa);     /* (p_bot / p0)^kappa at the column's lower edge */
     double m_ice = moist ? ice[i] : 0.0;
+    const int isl = moist && land && land[i] > 0.5;  /* land: the soil bucket, ponds, snow and the ground's heat */
+    if (cap && par[P_C_WET] > 0 && isl) {            /* (c_wet: the ground's capacity by its water, its heat kept) */
+      const double r = fmin(1.0, soil[i] / soilcap[i]), Cn = par[P_C_DRY] + (par[P_C_WET] - par[P_C_DRY]) * r;
+      Ts[i] = Tf + (Ts[i] - Tf) * cap[i] / Cn;
+      cap[i] = Cn;
+    }
     const double Cw = cap ? cap[i] : par[P_C_SLAB];
     const double Cf = firn && par[P_FIRN] > 0 ? par[P_C_ICE] * par[P_RHO_FIRN] * par[P_FIRN] : 0.0;
     double Tfn = Cf > 0 ? firn[i] : Tf;
@@ -638,7 +649,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     const double ps = pb[bot];
     const double Tsa = thb * exb;
     const int sfc = par[P_SFC] > 0 && ch;
-    const double rough = 1;
+    const double rough = !sfc && isl ? par[P_LAND_EXCH] : 1.0;
     const double bulk_c = par[P_BULK_C], gust = par[P_BULK_GUST], rd = air->rd;
     double stab = 1;
     if (bulk_c > 0 && par[P_BULK_Z] > 0) {
@@ -670,10 +681,25 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     }
     const double dice = m_ice / par[P_RHO_ICE];
     const double fice = moist ? fmin(1.0, dice / par[P_ICE_COVER]) : 0.0;
-    const double alb0 = par[P_ALB];
+    /* the ground's albedo by its wetness (and the cover's lc rows: 1 dry, 2 wet, 3 the vegetated share, 4 the
+     * vegetation's), its ponds at the sea's; the ice's, on land by its snow; lc row 5 the share of ice that shows */
+    const double ice0 = m_ice;
+    double wet = isl ? soil[i] : 0.0, pw = isl ? pond[i] : 0.0;
+    const double pcov = isl && pw > 0 ? fmin(1.0, pw / par[P_POND_COVER]) : 0.0;
+    const size_t Ns = (size_t)N;
+    const double alb0 =
+        !isl ? par[P_ALB]
+        : lc ? pcov * par[P_ALB] + (1 - pcov) * (lc[3 * Ns + i] * (lc[Ns + i] + (lc[2 * Ns + i] - lc[Ns + i]) * wet /
+                                                                               soilcap[i]) +
+                                                 (1 - lc[3 * Ns + i]) * lc[4 * Ns + i])
+             : pcov * par[P_ALB] + (1 - pcov) * (par[P_ALB_DRY] + (par[P_ALB_WET] - par[P_ALB_DRY]) * wet / soilcap[i]);
     const double warm = fmax(0.0, fmin(1.0, (Tsa - Tf + par[P_ALB_RAMP]) / par[P_ALB_RAMP]));
-    const double alb_i = par[P_ALB_ICE] + (par[P_ALB_ICE_MELT] - par[P_ALB_ICE]) * warm;
-    const double fsee = fice;
+    double alb_i = par[P_ALB_ICE] + (par[P_ALB_ICE_MELT] - par[P_ALB_ICE]) * warm;
+    if (isl) {
+      const double fs = fmin(1.0, snow[i] / par[P_SNOW_COVER]);
+      alb_i = fs * (par[P_ALB_SNOW] + (par[P_ALB_SNOW_MELT] - par[P_ALB_SNOW]) * warm) + (1 - fs) * par[P_ALB_GLACIER];
+    }
+    const double fsee = isl && lc ? fice * lc[5 * Ns + i] : fice;
     const double alb = fsee * alb_i + (1 - fsee) * alb0;
     if (swalb) swalb[i] = alb;
     double toAir;
@@ -751,16 +777,41 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
      * against sunlight, longwave and sensible heat, capped at freezing */
     const double kex = rough * (bulk_c > 0 || sfc ? bulk : par[P_K_EVAP] * (1 + uabs[i] / par[P_U0]));
     const double qa = qb;
-    const double qfac = par[P_Q_SEA] > 0 ? par[P_Q_SEA] : 1.0;
+    /* soil and plants hold their water back: a surface resistance rs in series with the air's ra (land_rs; rs_max:
+     * the stomata open with the sunlight, rs = land_rs (1 + f) / (land_rs / rs_max + f), f the sunlight reaching the
+     * ground over rs_sun; lc rows 6, 7 the cover's own land_rs, rs_sun; its bare share, row 3's complement, has
+     * none); bare_fx: the bare soil's evaporation falls as (wet / full)^bare_fx, the plants' as wet / full */
+    double rfac = 1, fbare = 0, rveg = 1;
+    if (isl && par[P_LAND_RS] > 0 && kex > 0) {
+      const double r0 = lc ? lc[6 * Ns + i] : par[P_LAND_RS], rsun = lc ? lc[7 * Ns + i] : par[P_RS_SUN];
+      double rs = r0;
+      if (par[P_RS_MAX] > 0) {
+        const double f = swS / fmax(1 - alb, 0.05) / rsun;
+        rs = r0 * (1 + f) / (r0 / par[P_RS_MAX] + f);
+      }
+      const double ra = ps / (rd * Tsa) / kex;
+      rfac = ra / (ra + rs);
+      rveg = rfac;
+      if (lc) {
+        fbare = lc[3 * Ns + i];
+        rfac = fbare + (1 - fbare) * rfac;
+      }
+    }
+    const int bfx = isl && par[P_BARE_FX] > 0 && lc && par[P_LAND_RS] > 0;
+#define SOILF(w_) (fbare * pow(fmin(1.0, (w_)), par[P_BARE_FX]) + (1 - fbare) * rveg * fmin(1.0, (w_)))
+    const double qfac = !isl && par[P_Q_SEA] > 0 ? par[P_Q_SEA] : 1.0;
     const int skin = moist && dice > par[P_ICE_COVER];
-    if (Cf > 0 && Tfn != Tf) {                       /* (a firn layer's heat to the slab: no ice sheet on the sea) */
+    /* firn where the land's ice is thick enough; where it ends, its heat goes to the ground (or the slab) below */
+    const int fon = Cf > 0 && isl && skin && m_ice >= par[P_RHO_FIRN] * par[P_FIRN];
+    if (Cf > 0 && !fon && Tfn != Tf) {
       Ts[i] += Cf * (Tfn - Tf) / Cw;
       Tfn = Tf;
     }
     const double ts = Ts[i];
-    double tsk = ts, melt_top = 0, cond = 0;
+    double tsk = ts, melt_top = 0, cond = 0, condb = 0;
     if (skin) {
-      const double kc = par[P_K_ICE] / dice, tb = ts;
+      /* the skin conducts to what lies under it: the ice down to the ground (or the slab), or the firn's middle */
+      const double kc = fon ? 2 * par[P_K_FIRN] / par[P_FIRN] : par[P_K_ICE] / dice, tb = fon ? Tfn : ts;
       double x = fmin(tb, Tf);
       for (int it = 0; it < 8; it++) {
         const double x3 = x * x * x;
@@ -772,6 +823,26 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       melt_top = kc * (tb - x) - eps_s * sigma * x4 + swS + ldown - hs * (x - Tsa);
       tsk = x;
       cond = kc * (tb - tsk);
+      if (fon) {                                     /* and from the ground through the rest of the ice to the firn */
+        const double rest = fmax(dice - par[P_RHO_FIRN] * par[P_FIRN] / par[P_RHO_ICE], 0.0);
+        condb = (ts - Tfn) / (rest / par[P_K_ICE] + par[P_FIRN] / (2 * par[P_K_FIRN]));
+      }
+    }
+    /* the ground's skin (c_deep; its capacity Cw): stepped implicitly, C (x - ts) = dt F(x) with F the surface's net
+     * gain (sunlight, longwave, sensible heat, the evaporation before the water runs short), by Newton's method */
+    if (isl && !skin && par[P_C_DEEP] > 0) {
+      const double wfull = wet / fmax(par[P_SOIL_BETA] * soilcap[i], 1e-9);
+      const double beta = kex * (pcov + (1 - pcov) * (bfx ? SOILF(wfull) : rfac * fmin(1.0, wfull))) * (1 - fice);
+      double x = ts;
+      for (int it = 0; it < 6; it++) {
+        const double x3 = x * x * x, d = qsat_x(x, ps, aeps, 0.0, mix) - qa;
+        const double dqs = mix ? (double)((float)qsat_x(x + 0.01, ps, aeps, 0.0, 1) - (float)qsat_x(x - 0.01, ps, aeps, 0.0, 1))
+                               : qsat_x(x + 0.01, ps, aeps, 0.0, 0) - qsat_x(x - 0.01, ps, aeps, 0.0, 0);
+        const double e = d > 0 ? beta * d : 0.0, de = d > 0 ? beta * dqs * 50 : 0.0;
+        const double g = Cw * (x - ts) - dt * (swS + ldown - eps_s * sigma * x3 * x - hs * (x - Tsa) - Lv * e);
+        x -= g / (Cw + dt * (4 * eps_s * sigma * x3 + hs + Lv * de));
+      }
+      tsk = x;
     }
     double up = eps_s * sigma * tsk * tsk * tsk * tsk;
     if (radf) {
@@ -845,6 +916,21 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     if (moist) {
       const double def = qfac * qsat_x(tsk, ps, aeps, 0.0, mix) - qa;
       E = def > 0 ? kex * def * (1 - fice) : 0.0;
+      if (isl) {                                     /* land: from its ponds freely, from the soil held back */
+        const double full = par[P_SOIL_BETA] * soilcap[i];
+        double Es = (1 - pcov) * E, Ep = pcov * E;
+        if (bfx) {
+          Es *= SOILF(wet / full);
+        } else {
+          Es *= rfac;
+          if (wet < full) Es *= wet / full;
+        }
+        if (Es * dt > wet) Es = wet / dt;
+        if (Ep * dt > pw) Ep = pw / dt;
+        E = Es + Ep;
+        wet -= Es * dt;
+        pw -= Ep * dt;
+      }
       for (int k = 0; k < L; k++) q[k] += E * dt * wbl[k] / mb / mu[k];
     }
     evap[i] = E;
@@ -858,7 +944,8 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     /* the slab: open water takes the surface's fluxes (evaporation costs it Lv per kg); under ice the conduction up
      * into the skin. Below Tf the slab's heat freezes into ice; ice melts with slab heat above Tf and from the top */
     double qS;
-    if (skin) qS = -cond;
+    if (fon) Tfn += (condb - cond) * dt / Cf;       /* the firn between the ground's conduction and the skin's */
+    if (skin) qS = fon ? -condb : -cond;
     else qS = swS + ldown - eps_s * sigma * tsk * tsk * tsk * tsk - H - Lv * E;
     double Tn = ts + qS * dt / Cw;
     if (qocean) qocean[i] = qS;
@@ -870,8 +957,10 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         m_ice = 0;
       }
       if (Tn < Tf) {
-        m_ice += (Tf - Tn) * Cw / Lf;
-        Tn = Tf;
+        if (!isl) {                                  /* (the ground cools below freezing; the sea freezes) */
+          m_ice += (Tf - Tn) * Cw / Lf;
+          Tn = Tf;
+        }
       } else if (m_ice > 0) {
         const double m = (Tn - Tf) * Cw / Lf;
         if (m <= m_ice) {
@@ -928,7 +1017,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         }
         double ev = 0;
         if (loss > 0) fa = fmax(fa, f);
-        const double rc_ = par[P_REVAP] > 0 ? par[P_REVAP_RHS] : 1.0;
+        const double rc_ = par[P_REVAP] > 0 ? (isl ? par[P_REVAP_RHL] : par[P_REVAP_RHS]) : 1.0;
         if (Fr > 0 && rc_ * qs > q[k]) {
           if (par[P_REVAP] > 0) {
             const double a_ = fmax(fa, par[P_REVAP_AREA]), ac = fmax(0.0, a_ - f);
@@ -1035,7 +1124,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     if (cloudy && (par[P_REVAP] > 0 ? Rc > 0 : par[P_CONV_REVAP] > 0 && !(par[P_DOWNDRAFT] > 0) && Rc > 0) && L > 1) {
       double qs, dqs;
       layer_cap(bot, mu, th[bot], par, aeps, mix, &qs, &dqs);
-      const double rc_ = par[P_REVAP] > 0 ? par[P_REVAP_RHS] : 1.0;
+      const double rc_ = par[P_REVAP] > 0 ? (isl ? par[P_REVAP_RHL] : par[P_REVAP_RHS]) : 1.0;
       if (rc_ * qs > q[bot]) {
         double ev;
         if (par[P_REVAP] > 0) {
@@ -1060,9 +1149,70 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     if (conv) conv[i] = mc;
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
     rain[i] = R / dt;
-    if (S > 0) {                                     /* snow into the sea melts with the slab's heat */
+    if (!isl && S > 0) {                             /* snow into the sea melts with the slab's heat */
       Ts[i] -= Lf * S / Cw;
       if (qocean) qocean[i] -= Lf * S / dt;
+    }
+    if (isl) {
+      /* the ice's melt runs into the ponds, from the snow first. On ice the skin decides: below freezing rain and
+       * snow freeze as they land, a pond as fast as the surface can lose the heat (to the cold ice, the cold air, the
+       * sky; and into snow without firn at 10 mm/h), the freezing heat into the air touching the ground; at freezing
+       * rain runs on and the snow melts from that air. Bare ground: snow lies on frozen ground, its freezing (the
+       * rain's share) warming it; else rain runs to the ponds. A pond on frozen ground freezes up to its heat room;
+       * on thawed ground it soaks in at k_inf toward the bucket's capacity, and a full bucket springs. */
+      pw += ice0 - m_ice;
+      double sn = snow[i] - (ice0 - m_ice);
+      if (skin) {
+        if (tsk < Tf) {
+          const double room = (par[P_K_ICE] / dice * (Tf - tsk) + hs * fmax(0.0, Tf - Tsa) +
+                               fmax(0.0, eps_s * sigma * Tf * Tf * Tf * Tf - ldown - swS)) * dt / Lf +
+                              (sn > 0 && !fon ? 10.0 / 3600 * dt : 0.0);
+          const double mp = fmin(pw > 0 ? pw : 0.0, room), m = R + mp;
+          m_ice += m;
+          sn = (sn > 0 ? sn : 0.0) + R;
+          pw -= mp;
+          for (int k = 0; k <= bot; k++) th[k] += Lf * (m - S) * (wbl[k] / mb) / (mu[k] * pib[k]);
+        } else {
+          pw += R;
+          for (int k = 0; k <= bot; k++) th[k] -= Lf * S * (wbl[k] / mb) / (mu[k] * pib[k]);
+        }
+        if (fon && pw > 0 && Tfn < Tf) {             /* melt and rain soak into the cold firn and refreeze */
+          const double mr = fmin(pw, Cf * (Tf - Tfn) / Lf);
+          pw -= mr;
+          m_ice += mr;
+          sn = (sn > 0 ? sn : 0.0) + mr;
+          Tfn += Lf * mr / Cf;
+        }
+      } else if (Ts[i] < Tf) {
+        m_ice += R;
+        sn = (sn > 0 ? sn : 0.0) + R;
+        Ts[i] += Lf * (R - S) / Cw;
+      } else {
+        pw += R;
+        Ts[i] -= Lf * S / Cw;
+      }
+      if (!skin && Ts[i] < Tf && pw > 0) {
+        const double room = (Tf - Ts[i]) * Cw / Lf, m = pw < room ? pw : room;
+        pw -= m;
+        m_ice += m;
+        Ts[i] += Lf * m / Cw;
+      }
+      ice[i] = m_ice;
+      snow[i] = fmax(0.0, fmin(sn, m_ice));
+      if (!skin && Ts[i] >= Tf) {
+        const double room = par[P_K_INF] * dt * (1 - wet / soilcap[i]);
+        const double m = pw < room ? pw : room;
+        if (m > 0) {
+          pw -= m;
+          wet += m;
+        }
+      }
+      if (wet > soilcap[i]) {
+        pw += wet - soilcap[i];
+        wet = soilcap[i];
+      }
+      soil[i] = wet;
+      pond[i] = pw;
     }
     if (rainp) {
       rainp[i] = Rk[0] / dt;
@@ -1078,5 +1228,6 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       if (cloudy) cloudw[(size_t)k * N + i] = cl[k];
     }
     stored[i] = E1 - E0;
+#undef SOILF
   }
 }

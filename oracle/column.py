@@ -58,13 +58,16 @@ def gas_src(src, h, dt, x, L):
 
 
 def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None, elev=None, radf=None, swf=None,
-               ch=None, firn=None, air=None, mix=False, ocean=False, q=None, ice=None, cw=None, cloudo=False):
+               ch=None, firn=None, air=None, mix=False, ocean=False, q=None, ice=None, cw=None, cloudo=False,
+               land=None, soilcap=None, soil=None, pond=None, snow=None, lc=None):
     """air.cu air_column (the dry column; the moist one, par's moist > 0, with q (L, N) and ice (N); clouds with cloud
     > 0 and the cloud water cw (L, N)): returns a dict of theta, Ts, wx, olr, sw_abs, stored, rain, evap, conv, tsurf,
     rainp, flash, swalb (and qocean, qsw with ocean=True; firn when given; q and ice when moist; cw, and cloudo
-    ((4L + 3), N) with cloudo=True, with clouds). mu is read only."""
-    par, t, pos, uabs, mu, insol, cap, elev, radf, swf, ch = (_d(x) for x in (par, t, pos, uabs, mu, insol, cap, elev,
-                                                                             radf, swf, ch))
+    ((4L + 3), N) with cloudo=True, with clouds; over land (moist, the mask land (N) > 0.5, soilcap (N)) soil, pond and
+    snow (N), and cap when given (c_wet rewrites it); lc (8, N) the land cover's rows). mu is read only."""
+    par, t, pos, uabs, mu, insol, elev, radf, swf, ch, land, soilcap, lc = (
+        _d(x) for x in (par, t, pos, uabs, mu, insol, elev, radf, swf, ch, land, soilcap, lc))
+    cap, soil, pond, snow = (None if x is None else _d(x).copy() for x in (cap, soil, pond, snow))
     theta, Ts = _d(theta).copy(), _d(Ts).copy()
     q = None if q is None else _d(q).copy()
     ice = None if ice is None else _d(ice).copy()
@@ -82,11 +85,16 @@ def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None,
                         _p(mu), _p(theta), _p(q), _p(Ts), _p(ice), float(dt), _p(o['wx']), _p(o['olr']), _p(o['sw_abs']),
                         _p(o['stored']), _p(o['rain']), _p(o['evap']), _p(o['conv']), _p(o['tsurf']), _p(o['rainp']),
                         _p(cap), _p(o.get('qocean')), _p(o.get('qsw')), _p(elev), _p(o['flash']), _p(radf), _p(swf),
-                        _p(o['swalb']), _p(ch), _p(firn), _p(cw), _p(o.get('cloudo')), int(bool(mix)))
+                        _p(o['swalb']), _p(ch), _p(firn), _p(cw), _p(o.get('cloudo')), _p(land), _p(soilcap),
+                        _p(soil), _p(pond), _p(snow), _p(lc), int(bool(mix)))
     if firn is not None:
         o['firn'] = firn
     if q is not None:
         o['q'], o['ice'] = q, ice
     if cw is not None:
         o['cw'] = cw
+    if land is not None:
+        o['soil'], o['pond'], o['snow'] = soil, pond, snow
+        if cap is not None:
+            o['cap'] = cap
     return o

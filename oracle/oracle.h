@@ -118,9 +118,10 @@ void og_gas_src(int N, int L, int G, const double* src, const double* h, double 
 void og_air_column(int N, int L, const double* par, const og_air* air, int mode, const double* t, const double* insol,
                    const double* pos, const double* uabs, const double* mu, double* theta, double* qv, double* Ts,
                    double* ice, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
-                   double* evap, double* conv, double* tsurf, double* rainp, const double* cap, double* qocean,
+                   double* evap, double* conv, double* tsurf, double* rainp, double* cap, double* qocean,
                    double* qsw, const double* elev, double* flash, const double* radf, const double* swf,
-                   double* swalb, const double* ch, double* firn, double* cloudw, double* cloudo, int mix);
+                   double* swalb, const double* ch, double* firn, double* cloudw, double* cloudo, const double* land,
+                   const double* soilcap, double* soil, double* pond, double* snow, const double* lc, int mix);
 double og_sun_lon(double t, const double* par, double* fd);
 
 #endif

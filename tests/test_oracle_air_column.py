@@ -539,3 +539,145 @@ def test_ice_fall_rate():
     o = crun(c, p, mode=1, insol=np.zeros(4), dt=dt)
     cw1 = o['cw'][2] + o['rain'] * dt / c['mu'][2]
     np.testing.assert_allclose(o['rain'] * dt / c['mu'][2], cw1 * -np.expm1(-0.5 * dt / depth), rtol=1e-9)
+
+
+# ---- land (air.cu air_column's land branches): the soil bucket, ponds, snow, the ground's skin, firn ----
+
+def land_aqua(N=96, seed=3, warm=12.0, rh=(0.1, 0.8, 1.3), ice_share=0.3):
+    """moist_aqua's columns with two thirds of them land: a soil bucket of 50-300 kg/m^2 a fifth to fully wet, ponds,
+    snow on part of the land's ice, the ground's heat capacity c_land."""
+    c = moist_aqua(N=N, seed=seed, rh=rh, warm=warm, ice_share=ice_share)
+    rng = np.random.default_rng(seed + 200)
+    c['land'] = (rng.uniform(size=N) < 2 / 3).astype(float)
+    c['soilcap'] = rng.uniform(50.0, 300.0, N)
+    c['soil'] = c['soilcap'] * rng.uniform(0.2, 1.0, N)
+    c['pond'] = np.where(rng.uniform(size=N) < 0.4, rng.uniform(0.0, 60.0, N), 0.0)
+    c['snow'] = np.minimum(c['ice'], rng.uniform(0.0, 80.0, N))
+    c['cap'] = np.where(c['land'] > 0.5, 2.0e6, 4.186e8)
+    return c
+
+
+def lrun(c, p, **kw):
+    kw.setdefault('cap', c['cap'])
+    return run(c, p, q=c['q'], ice=c['ice'], land=c['land'], soilcap=c['soilcap'], soil=c['soil'], pond=c['pond'],
+               snow=c['snow'], ocean=True, **kw)
+
+
+def cover(N, seed):
+    """A land cover's eight rows: z0, the dry and wet soil's albedo, the vegetated share, the vegetation's albedo,
+    the share of ice that shows, the stomata's resistance and their sunlight scale."""
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(0.01, 1.0, N), rng.uniform(0.25, 0.35, N), rng.uniform(0.1, 0.2, N),
+                     rng.uniform(0.0, 1.0, N), rng.uniform(0.1, 0.2, N), rng.uniform(0.3, 1.0, N),
+                     rng.uniform(50.0, 150.0, N), rng.uniform(100.0, 250.0, N)])
+
+
+LAND_OPTS = [dict(), dict(land_rs=100.0, rs_max=5000.0), dict(c_deep=1.0), dict(c_wet=3.0e6, c_dry=1.3e6),
+             dict(land_exch=2.0, revap=1.0, cloud=1.0), dict(firn=2.0, ice_air=1.0, cloud=1.0),
+             dict(land_rs=100.0, bare_fx=2.0, lc=1)]
+
+
+@pytest.mark.parametrize('opts', LAND_OPTS)
+@pytest.mark.parametrize('mix', [False, True])
+def test_land_energy_and_water_close(opts, mix):
+    """Over land the water stays in the cell: the air's vapor (and cloud water) plus the soil's, the ponds' and the
+    ice's change by nothing; over the sea by evaporation less rain. The energy closes as over the sea (the ground's
+    heat C Ts, the firn's, the ice's -L_f m). The plumes are held off where clouds are on (a plume moves its base
+    layer's air without its cloud water, as the reference's does)."""
+    opts = dict(opts)
+    c = land_aqua()
+    N = c['land'].size
+    kw = {}
+    if opts.pop('lc', 0):
+        kw['lc'] = cover(N, 5)
+    if 'firn' in opts:
+        kw['firn'] = np.random.default_rng(9).uniform(240.0, 273.15, N)
+        c['ice'] = np.where(c['land'] > 0.5, np.maximum(c['ice'], 1500.0), c['ice'])
+    cloudy = opts.get('cloud', 0) > 0
+    if cloudy:
+        c['cw'] = np.full_like(c['q'], 1e-4)
+        kw['cw'] = c['cw']
+    dt = 900.0
+    p = par3(moist=1.0, **dict(opts, tau_conv=1e12) if cloudy else opts)
+    o = lrun(c, p, dt=dt, mix=mix, **kw)
+    stored_matches(o, dt)
+    land = c['land'] > 0.5
+    mu1 = new_masses(c, o['wx'])
+    W0 = (c['mu'] * (c['q'] + (c['cw'] if cloudy else 0))).sum(0)
+    W1 = (mu1 * (o['q'] + (o['cw'] if cloudy else 0))).sum(0)
+    G0, G1 = c['soil'] + c['pond'] + c['ice'], o['soil'] + o['pond'] + o['ice']
+    np.testing.assert_allclose((W1 - W0 + G1 - G0)[land], 0.0, atol=1e-11 * (W0 + G0).max())
+    np.testing.assert_allclose((W1 - W0)[~land], ((o['evap'] - o['rain']) * dt)[~land], atol=1e-11 * W0.max())
+    assert 'firn' in opts or o['evap'][land].max() > 0          # (with firn all the land is under ice)
+    assert np.all(o['soil'] >= 0) and np.all(o['pond'] >= -1e-12)
+    assert np.all(o['snow'][land] <= o['ice'][land] + 1e-12) and np.all(o['soil'] <= c['soilcap'] * (1 + 1e-15))
+    assert np.all(o['soil'][~land] == c['soil'][~land]) and np.all(o['pond'][~land] == c['pond'][~land])
+    if 'c_wet' in opts:                                   # the ground's capacity follows its wetness
+        r = np.minimum(1.0, c['soil'] / c['soilcap'])
+        np.testing.assert_allclose(o['cap'][land], (1.3e6 + 1.7e6 * r)[land], rtol=1e-15)
+
+
+def test_land_evaporation_closed_form():
+    """Bare wet ground below the bucket's soil_beta share, no ponds, no ice, no surface resistance: evaporation is the
+    open water's kex (q_s(Ts, p_s) - q_a) times land_exch times wet / (soil_beta capacity); the surface resistance
+    land_rs multiplies it by r_a / (r_a + r_s), r_a = rho / kex (the bottom layer holds the whole surface air here)."""
+    c = land_aqua(ice_share=0.0)
+    c['pond'][:] = 0.0
+    c['soil'] = 0.5 * c['soilcap']
+    N = c['land'].size
+    for rs in (0.0, 100.0):
+        p = par3(moist=1.0, land_exch=2.0, land_rs=rs, soil_beta=0.75, theta_star=-1.0, theta_star2=-1.0)
+        o = lrun(c, p, mode=1, insol=np.zeros(N), dt=600.0)
+        pt, pb, _ = pib_of(c['mu'])
+        ps = pb[2]
+        uabs = c['uabs']
+        kex = np.where(c['land'] > 0.5, 2.0, 1.0) * 4.0e-3 * (1 + uabs / 10.0)
+        qa = c['q'][2]
+        Tsa = c['theta'][2] * (ps / P0) ** KAPPA
+        E = kex * np.maximum(0.0, tetens(c['Ts'], ps) - qa)
+        land = c['land'] > 0.5
+        f = 0.5 / 0.75
+        if rs > 0:
+            ra = ps / (287.0 * Tsa) / kex
+            f = f * ra / (ra + rs)
+        np.testing.assert_allclose(o['evap'][land], (E * f)[land], rtol=1e-12)
+        np.testing.assert_allclose(o['evap'][~land], E[~land], rtol=1e-12)
+
+
+def test_land_infiltration_and_spring():
+    """A pond on thawed ground with nothing evaporating soaks in at k_inf dt (1 - wet / capacity), at most the pond;
+    a bucket filled past its capacity gives the rest back to the pond."""
+    c = land_aqua(ice_share=0.0)
+    c['q'][:] = 1e-9                                        # (no rain)
+    c['Ts'] = np.maximum(c['Ts'], 280.0)
+    N = c['land'].size
+    c['pond'] = np.random.default_rng(4).uniform(0.0, 40.0, N)
+    c['soil'] = np.random.default_rng(5).uniform(0.5, 1.0, N) * c['soilcap']
+    dt = 1800.0
+    p = par3(moist=1.0, k_evap=0.0, theta_star=-1.0, theta_star2=-1.0, k_inf=5.0 / 3600)
+    o = lrun(c, p, mode=1, insol=np.zeros(N), dt=dt)
+    land = c['land'] > 0.5
+    room = 5.0 / 3600 * dt * (1 - c['soil'] / c['soilcap'])
+    m = np.minimum(c['pond'], room)
+    np.testing.assert_allclose(o['soil'][land], (c['soil'] + m)[land], rtol=1e-15)
+    np.testing.assert_allclose(o['pond'][land], (c['pond'] - m)[land], rtol=1e-12, atol=1e-12)
+    c['soil'] = c['soilcap'] * 1.1                          # over the capacity: a spring
+    o = lrun(c, p, mode=1, insol=np.zeros(N), dt=dt)
+    np.testing.assert_allclose(o['soil'][land], c['soilcap'][land], rtol=1e-15)
+    np.testing.assert_allclose(o['pond'][land], (c['pond'] + 0.1 * c['soilcap'])[land], rtol=1e-12)
+
+
+def test_ground_skin_implicit():
+    """With c_deep the ground's skin is stepped implicitly: C (x - T_s) = dt F(x). Its energy then is the ground's
+    change, so the ground ends at the skin's temperature (Newton converged: to 1e-9 K) wherever neither ponds freeze
+    nor snow lands."""
+    c = land_aqua(ice_share=0.0)
+    c['Ts'] = np.maximum(c['Ts'], 285.0)
+    c['q'] = 0.3 * c['q']                                   # (no rain)
+    N = c['land'].size
+    c['cap'] = np.where(c['land'] > 0.5, 3.0e5, 4.186e8)
+    o = lrun(c, par3(moist=1.0, c_deep=1.0, theta_star=-1.0, theta_star2=-1.0), dt=900.0)
+    land = c['land'] > 0.5
+    assert o['rain'].max() == 0
+    np.testing.assert_allclose(o['Ts'][land], o['tsurf'][land], atol=1e-9)
+    assert np.abs(o['Ts'] - c['Ts'])[land].max() > 0.1      # (and it moved)
