@@ -264,20 +264,25 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * the clouds' reflection and emission, the cloud water carried by the crossing; cloudo (L + 3 rows, or 4L + 3 with
  * radf, N) when given: each layer's cover, the total cover, the clouds' shortwave and longwave effects at the top
  * (and with radf the in-cloud longwave and shortwave optical depths and liquid share), as the radiation's set-up reads
- * them. Updates theta, Ts (and qv, ice) in place and writes wx (L-1, N), olr, sw_abs, stored (the column's energy
- * change, J/m^2), rain, evap, conv (kg/m^2/s), tsurf; rainp (4, N: rain from the layers above the bottom one, from
- * the bottom one, the plumes', snow), qocean, qsw, flash, swalb and firn (the firn's temperature; its heat goes to the
- * slab) when given. The argument list is the reference's; soilcap, soil, pond, snow and lc are unread (the reference
- * reads them only over land); elev (N) with theta_rise > 0 raises the lowest interface over high ground. Returns
+ * them. Over land (moist; land (N) > 0.5 marks a land cell; soilcap (N) the soil bucket's capacity, kg/m^2): the
+ * soil water soil, the ponds pond and the snow snow (N, kg/m^2, updated in place), the ground's albedo by its
+ * wetness, the snow's albedo, the surface resistance (land_rs, rs_max, bare_fx), the ground's implicit skin (c_deep),
+ * the ground's capacity by its water (c_wet: cap rewritten), land_exch, firn on the land's ice sheets (firn), rain
+ * and snow landing, ponds freezing and soaking in (k_inf), a full bucket springing; lc (8, N) the land cover's rows
+ * when given (row 0 the roughness, read by isw_surface_layer). Updates theta, Ts (and qv, ice) in place and writes
+ * wx (L-1, N), olr, sw_abs, stored (the column's energy change, J/m^2), rain, evap, conv (kg/m^2/s), tsurf; rainp
+ * (4, N: rain from the layers above the bottom one, from the bottom one, the plumes', snow), qocean, qsw, flash,
+ * swalb and firn (the firn's temperature) when given; cap (N) the surface's heat capacity, else c_slab. The argument
+ * list is the reference's; elev (N) with theta_rise > 0 raises the lowest interface over high ground. Returns
  * ISW_EBADARG for bl_mix > 0, cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or ice,
- * with land, plume_deep, plume_root, q_shape, sat_prof, sat_base, edge_vap or freeze_dry, and with clouds and bl_q:
- * those parts of the reference's column are not in this library. prec: ISW_FP64, or ISW_MIXED (the reference's mixed build:
+ * with land but without soilcap, soil, pond or snow, with plume_deep, plume_root, q_shape, sat_prof, sat_base,
+ * edge_vap or freeze_dry, and with clouds and bl_q: those parts of the reference's column are not in this library. prec: ISW_FP64, or ISW_MIXED (the reference's mixed build:
  * 1 - e^-x, the edge sub-levels' Exner powers and the saturation and condensation arithmetic in float32). */
 int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* air, int32_t mode, const double* t,
                        const double* insol, const double* pos, const double* uabs, const double* mu, double* theta,
                        double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,
                        double* stored, double* rain, double* evap, double* conv, double* tsurf, double* rainp,
-                       double* dgo, const double* cap, double* qocean, double* qsw, const double* land,
+                       double* dgo, double* cap, double* qocean, double* qsw, const double* land,
                        const double* soilcap, double* soil, double* pond, double* snow, double* cloudw, double* cloudo,
                        const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
                        const double* ch, const double* lc, double* firn, double* xfer, int32_t prec, void* stream);

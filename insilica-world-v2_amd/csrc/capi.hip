@@ -46,9 +46,9 @@ int gas_src(int, int, int, const double*, const double*, double, double*, hipStr
 int advance_time(double*, double, hipStream_t);
 int air_column(int, int, const double*, const isw_air*, int, const double*, const double*, const double*,
                const double*, const double*, double*, double*, double*, double*, double, double*, double*, double*,
-               double*, double*, double*, double*, double*, double*, const double*, double*, double*, const double*,
-               double*, const double*, const double*, double*, const double*, double*, double*, double*, int,
-               hipStream_t);
+               double*, double*, double*, double*, double*, double*, double*, double*, double*, const double*,
+               const double*, double*, double*, double*, const double*, double*, const double*, const double*,
+               double*, const double*, const double*, double*, double*, double*, int, hipStream_t);
 }  // namespace isw_col
 
 #define ISW_DECLARE_RAD(NS)                                                                                          \
@@ -230,27 +230,27 @@ int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* a
                        const double* insol, const double* pos, const double* uabs, const double* mu, double* theta,
                        double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,
                        double* stored, double* rain, double* evap, double* conv, double* tsurf, double* rainp,
-                       double* dgo, const double* cap, double* qocean, double* qsw, const double* land,
+                       double* dgo, double* cap, double* qocean, double* qsw, const double* land,
                        const double* soilcap, double* soil, double* pond, double* snow, double* cloudw, double* cloudo,
                        const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
                        const double* ch, const double* lc, double* firn, double* xfer, int32_t prec, void* stream) {
-  (void)soilcap; (void)soil; (void)pond; (void)snow;
-  (void)lc;   // (read by the reference only over land, which this library's column does not take)
   if (N < 1 || L < 2 || L > ISW_LMAX || !par || !air || (mode != 0 && mode != 1) || !uabs || !mu || !theta ||
       !Ts || !wx || !olr || !sw_abs || !stored || !rain || !evap || !conv || !tsurf || !(dt > 0) || !prec_ok(prec))
     return ISW_EBADARG;
   if (mode == 0 ? (!t || !pos) : !insol) return ISW_EBADARG;
   // the options this library's column does not carry: the mixed bottom layer, the cumulus layer's floating edges,
-  // the diagnostics rows, the gas tracers moved with the crossing; for moist air land, the deep plume and its
-  // rooting, the surface air's vapor profile, the spread layers' capacity, the edge vapor and freeze-drying; with
-  // clouds the evenly mixed bottom layer's statistics (bl_q)
+  // the diagnostics rows, the gas tracers moved with the crossing; for moist air the deep plume and its rooting, the
+  // surface air's vapor profile, the spread layers' capacity, the edge vapor and freeze-drying; with clouds the
+  // evenly mixed bottom layer's statistics (bl_q). Land (moist) needs its bucket's capacity, soil, ponds and snow.
   if (par[70] > 0 || (par[111] > 0 && L > 3) || dgo || xfer) return ISW_EBADARG;
-  if (par[22] > 0 && (!qv || !ice || land || par[47] > 0 || par[88] > 0 || par[83] > 0 || par[51] > 0 ||
-                      par[53] > 0 || par[86] > 0 || par[85] > 0 || (par[55] > 0 && cloudw && par[89] > 0)))
+  if (par[22] > 0 && (!qv || !ice || par[47] > 0 || par[88] > 0 || par[83] > 0 || par[51] > 0 || par[53] > 0 ||
+                      par[86] > 0 || par[85] > 0 || (par[55] > 0 && cloudw && par[89] > 0) ||
+                      (land && (!soilcap || !soil || !pond || !snow))))
     return ISW_EBADARG;
   return isw_col::air_column(N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, wx, olr, sw_abs,
-                             stored, rain, evap, conv, tsurf, rainp, cap, qocean, qsw, elev, flash, radf, swf, swalb,
-                             ch, firn, cloudw, cloudo, prec == ISW_MIXED, S(stream));
+                             stored, rain, evap, conv, tsurf, rainp, cap, qocean, qsw, land, soilcap, soil, pond, snow,
+                             elev, flash, radf, swf, swalb, ch, lc, firn, cloudw, cloudo, prec == ISW_MIXED,
+                             S(stream));
 }
 
 int32_t isw_surface_layer(int32_t N, int32_t L, const double* par, const isw_air* air, const double* uabs,

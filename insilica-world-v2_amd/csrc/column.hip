@@ -502,6 +502,9 @@ struct AP {
   // the clouds'
   double rh_crit, re_liq, re_ice, cloud_g, k_lw_liq, k_lw_ice, c_auto, q_auto, v_ice, tau_erode, k_revap, mu_cloud,
       l_ice, conv_revap, downdraft, revap, revap_rhs, revap_area, liq_cold, cw_qs;
+  // the land's
+  double alb_dry, alb_wet, soil_beta, k_inf, pond_cover, alb_snow, alb_snow_melt, alb_glacier, snow_cover, land_exch,
+      c_dry, c_wet, land_rs, c_deep, rs_max, rs_sun, revap_rhl, bare_fx, k_firn;
 };
 
 static AP make_ap(const double* a, const isw_air* air) {
@@ -518,6 +521,10 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.c_auto = a[62]; p.q_auto = a[63]; p.v_ice = a[64]; p.tau_erode = a[66]; p.k_revap = a[67]; p.mu_cloud = a[68];
   p.l_ice = a[79]; p.conv_revap = a[87]; p.downdraft = a[92]; p.revap = a[103]; p.revap_rhs = a[104];
   p.revap_area = a[106]; p.liq_cold = a[128]; p.cw_qs = a[129];
+  p.alb_dry = a[35]; p.alb_wet = a[36]; p.soil_beta = a[37]; p.k_inf = a[38]; p.pond_cover = a[39]; p.alb_snow = a[40];
+  p.alb_snow_melt = a[42]; p.alb_glacier = a[43]; p.snow_cover = a[44]; p.land_exch = a[46]; p.c_dry = a[49];
+  p.c_wet = a[50]; p.land_rs = a[52]; p.c_deep = a[94]; p.rs_max = a[95]; p.rs_sun = a[96]; p.revap_rhl = a[105];
+  p.bare_fx = a[115]; p.k_firn = a[121];
   return p;
 }
 
@@ -670,13 +677,16 @@ struct AirIO {
   double *theta, *qv, *Ts, *ice;
   double *wx, *olr, *sw_abs, *stored, *rain, *evap, *conv, *tsurf, *rainp, *qocean, *qsw, *flash, *swalb, *firn;
   double *cloudw, *cloudo;
+  double* cap;                                         // (rewritten over land with c_wet)
+  const double *land, *soilcap, *lc;                   // the land mask (> 0.5 land), the bucket's capacity, the cover
+  double *soil, *pond, *snow;
 };
 
 template <int L, bool MIX, bool MOIST, bool CLOUD>
 __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, const double* __restrict__ t,
                                                    const double* __restrict__ insol, const double* __restrict__ pos,
                                                    const double* __restrict__ uabs, const double* __restrict__ mu_in,
-                                                   double dt, const double* __restrict__ cap,
+                                                   double dt,
                                                    const double* __restrict__ elev, const double* __restrict__ radf,
                                                    const double* __restrict__ swf, const double* __restrict__ ch,
                                                    AirIO o) {
@@ -697,10 +707,16 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   layer_exner<L>(mu, P, pb, pib);
   const double exb = pow(pb[bot] / P.p0, P.kappa);
   double m_ice = MOIST ? o.ice[i] : 0.0;
-  const double Cw = cap ? cap[i] : A.c_slab;
+  const bool isl = MOIST && o.land && o.land[i] > 0.5;   // land: the soil bucket, ponds, snow and the ground's heat
+  double* Ts = o.Ts;
+  if (o.cap && A.c_wet > 0 && isl) {                     // (c_wet: the ground's capacity by its water, its heat kept)
+    const double r = fmin(1.0, o.soil[i] / o.soilcap[i]), Cn = A.c_dry + (A.c_wet - A.c_dry) * r;
+    Ts[i] = A.Tf + (Ts[i] - A.Tf) * o.cap[i] / Cn;
+    o.cap[i] = Cn;
+  }
+  const double Cw = o.cap ? o.cap[i] : A.c_slab;
   const double Cf = o.firn && A.firn > 0 ? A.c_ice * A.rho_firn * A.firn : 0.0;
   double Tfn = Cf > 0 ? o.firn[i] : A.Tf;
-  double* Ts = o.Ts;
   const double E0 = column_energy<L, MOIST>(mu, pib, th, q, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
 #pragma unroll
   for (int k = 0; k < L; k++) T[k] = pib[k] * th[k] / P.cp;
@@ -743,7 +759,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   const double ps = pb[bot];
   const double Tsa = thb * exb;
   const bool sfc = A.sfc > 0 && ch;
-  const double rough = 1;
+  const double rough = !sfc && isl ? A.land_exch : 1.0;
   double stab = 1;
   if (A.bulk_c > 0 && A.bulk_z > 0) {
     const double Ub = uabs[i] + P.bulk_gust, Ri = P.grav * A.bulk_z * (Tsa - Ts[i]) / (Tsa * Ub * Ub);
@@ -775,11 +791,29 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   }
   const double dice = m_ice / P.rho_ice;
   const double fice = MOIST ? fmin(1.0, dice / P.ice_cover) : 0.0;
+  // the ground's albedo by its wetness (and the cover's lc rows: 1 dry, 2 wet, 3 the vegetated share, 4 the
+  // vegetation's), its ponds at the sea's; the ice's, on land by its snow; lc row 5 the share of ice that shows
   double alb = A.alb;
+  const double ice0 = m_ice;
+  double wet = isl ? o.soil[i] : 0.0, pw = isl ? o.pond[i] : 0.0;
+  const double pcov = isl && pw > 0 ? fmin(1.0, pw / A.pond_cover) : 0.0;
+  const double* lc = o.lc;
+  const size_t Ns = (size_t)N;
   if constexpr (MOIST) {
+    const double scap = isl ? o.soilcap[i] : 1.0;
+    const double alb0 = !isl ? A.alb
+                        : lc ? pcov * A.alb + (1 - pcov) * (lc[3 * Ns + i] * (lc[Ns + i] + (lc[2 * Ns + i] - lc[Ns + i]) *
+                                                                                           wet / scap) +
+                                                            (1 - lc[3 * Ns + i]) * lc[4 * Ns + i])
+                             : pcov * A.alb + (1 - pcov) * (A.alb_dry + (A.alb_wet - A.alb_dry) * wet / scap);
     const double warm = fmax(0.0, fmin(1.0, (Tsa - A.Tf + A.alb_ramp) / A.alb_ramp));
-    const double alb_i = A.alb_ice + (A.alb_ice_melt - A.alb_ice) * warm;
-    alb = fice * alb_i + (1 - fice) * A.alb;
+    double alb_i = A.alb_ice + (A.alb_ice_melt - A.alb_ice) * warm;
+    if (isl) {
+      const double fs = fmin(1.0, o.snow[i] / A.snow_cover);
+      alb_i = fs * (A.alb_snow + (A.alb_snow_melt - A.alb_snow) * warm) + (1 - fs) * A.alb_glacier;
+    }
+    const double fsee = isl && lc ? fice * lc[5 * Ns + i] : fice;
+    alb = fsee * alb_i + (1 - fsee) * alb0;
   }
   if (o.swalb) o.swalb[i] = alb;
   double toAir;
@@ -870,16 +904,44 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   // ice against the surface's fluxes, capped at freezing ----
   const double kex = rough * (A.bulk_c > 0 || sfc ? bulk : A.k_evap * (1 + uabs[i] / A.u0));
   const double qa = qb;
-  const double qfac = P.q_sea > 0 ? P.q_sea : 1.0;
+  // soil and plants hold their water back: a surface resistance rs in series with the air's ra (land_rs; rs_max: the
+  // stomata open with the sunlight, rs = land_rs (1 + f) / (land_rs / rs_max + f), f the sunlight reaching the ground
+  // over rs_sun; lc rows 6, 7 the cover's own land_rs, rs_sun; its bare share, row 3's complement, has none);
+  // bare_fx: the bare soil's evaporation falls as (wet / full)^bare_fx, the plants' as wet / full
+  double rfac = 1, fbare = 0, rveg = 1;
+  if (isl && A.land_rs > 0 && kex > 0) {
+    const double r0 = lc ? lc[6 * Ns + i] : A.land_rs, rsun = lc ? lc[7 * Ns + i] : A.rs_sun;
+    double rs = r0;
+    if (A.rs_max > 0) {
+      const double f = swS / fmax(1 - alb, 0.05) / rsun;
+      rs = r0 * (1 + f) / (r0 / A.rs_max + f);
+    }
+    const double ra = ps / (A.rd * Tsa) / kex;
+    rfac = ra / (ra + rs);
+    rveg = rfac;
+    if (lc) {
+      fbare = lc[3 * Ns + i];
+      rfac = fbare + (1 - fbare) * rfac;
+    }
+  }
+  const bool bfx = isl && A.bare_fx > 0 && lc && A.land_rs > 0;
+  auto soilf = [&](double w_) {
+    const double x_ = fmin(1.0, w_);
+    return fbare * pow(x_, A.bare_fx) + (1 - fbare) * rveg * x_;
+  };
+  const double qfac = !isl && P.q_sea > 0 ? P.q_sea : 1.0;
   const bool skin = MOIST && dice > P.ice_cover;
-  if (Cf > 0 && Tfn != A.Tf) {                       // (a firn layer's heat to the slab: no ice sheet on the sea)
+  // firn where the land's ice is thick enough; where it ends, its heat goes to the ground (or the slab) below
+  const bool fon = Cf > 0 && isl && skin && m_ice >= A.rho_firn * A.firn;
+  if (Cf > 0 && !fon && Tfn != A.Tf) {
     Ts[i] += Cf * (Tfn - A.Tf) / Cw;
     Tfn = A.Tf;
   }
   const double ts = Ts[i];
-  double tsk = ts, melt_top = 0, cond = 0;
+  double tsk = ts, melt_top = 0, cond = 0, condb = 0;
   if (skin) {
-    const double kc = A.k_ice / dice, tb = ts;
+    // the skin conducts to what lies under it: the ice down to the ground (or the slab), or the firn's middle
+    const double kc = fon ? 2 * A.k_firn / A.firn : A.k_ice / dice, tb = fon ? Tfn : ts;
     double x = fmin(tb, A.Tf);
     for (int it = 0; it < 8; it++) {
       const double x3 = x * x * x;
@@ -891,6 +953,26 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     melt_top = kc * (tb - x) - A.eps_s * A.sigma * x4 + swS + ldown - hs * (x - Tsa);
     tsk = x;
     cond = kc * (tb - tsk);
+    if (fon) {                                       // and from the ground through the rest of the ice to the firn
+      const double rest = fmax(dice - A.rho_firn * A.firn / P.rho_ice, 0.0);
+      condb = (ts - Tfn) / (rest / A.k_ice + A.firn / (2 * A.k_firn));
+    }
+  }
+  // the ground's skin (c_deep; its capacity Cw): stepped implicitly, C (x - ts) = dt F(x) with F the surface's net
+  // gain (sunlight, longwave, sensible heat, the evaporation before the water runs short), by Newton's method
+  if (isl && !skin && A.c_deep > 0) {
+    const double wfull = wet / fmax(A.soil_beta * o.soilcap[i], 1e-9);
+    const double beta = kex * (pcov + (1 - pcov) * (bfx ? soilf(wfull) : rfac * fmin(1.0, wfull))) * (1 - fice);
+    double x = ts;
+    for (int it = 0; it < 6; it++) {
+      const double x3 = x * x * x, d = (double)qsat_a<F>((F)x, (F)ps, (F)A.eps, 0.0) - qa;
+      const double dqs = (double)(qsat_a<F>((F)(x + 0.01), (F)ps, (F)A.eps, 0.0) -
+                                  qsat_a<F>((F)(x - 0.01), (F)ps, (F)A.eps, 0.0));
+      const double e = d > 0 ? beta * d : 0.0, de = d > 0 ? beta * dqs * 50 : 0.0;
+      const double g = Cw * (x - ts) - dt * (swS + ldown - A.eps_s * A.sigma * x3 * x - hs * (x - Tsa) - P.Lv * e);
+      x -= g / (Cw + dt * (4 * A.eps_s * A.sigma * x3 + hs + P.Lv * de));
+    }
+    tsk = x;
   }
   double up = A.eps_s * A.sigma * tsk * tsk * tsk * tsk;
   if (radf) {
@@ -984,6 +1066,21 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     using F = std::conditional_t<MIX, float, double>;
     const double def = qfac * qsat_a<F>((F)tsk, (F)ps, (F)A.eps, 0.0) - qa;
     E = def > 0 ? kex * def * (1 - fice) : 0.0;
+    if (isl) {                                       // land: from its ponds freely, from the soil held back
+      const double full = A.soil_beta * o.soilcap[i];
+      double Es = (1 - pcov) * E, Ep = pcov * E;
+      if (bfx) {
+        Es *= soilf(wet / full);
+      } else {
+        Es *= rfac;
+        if (wet < full) Es *= wet / full;
+      }
+      if (Es * dt > wet) Es = wet / dt;
+      if (Ep * dt > pw) Ep = pw / dt;
+      E = Es + Ep;
+      wet -= Es * dt;
+      pw -= Ep * dt;
+    }
 #pragma unroll
     for (int k = 0; k < L; k++) q[k] += E * dt * wbl[k] / mb / mu[k];
   }
@@ -998,7 +1095,8 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   }
   // ---- the slab: open water takes the surface's fluxes (evaporation costs it Lv per kg), under ice the conduction
   // up into the skin; below Tf its heat freezes into ice, ice melts with slab heat above Tf and from the top ----
-  const double qS = skin ? -cond : swS + ldown - A.eps_s * A.sigma * tsk * tsk * tsk * tsk - H - P.Lv * E;
+  if (fon) Tfn += (condb - cond) * dt / Cf;         // the firn between the ground's conduction and the skin's
+  const double qS = skin ? (fon ? -condb : -cond) : swS + ldown - A.eps_s * A.sigma * tsk * tsk * tsk * tsk - H - P.Lv * E;
   double Tn = ts + qS * dt / Cw;
   if (o.qocean) o.qocean[i] = qS;
   if (o.qsw) o.qsw[i] = skin ? 0.0 : swS;
@@ -1009,8 +1107,10 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       m_ice = 0;
     }
     if (Tn < A.Tf) {
-      m_ice += (A.Tf - Tn) * Cw / A.Lf;
-      Tn = A.Tf;
+      if (!isl) {                                    // (the ground cools below freezing; the sea freezes)
+        m_ice += (A.Tf - Tn) * Cw / A.Lf;
+        Tn = A.Tf;
+      }
     } else if (m_ice > 0) {
       const double m = (Tn - A.Tf) * Cw / A.Lf;
       if (m <= m_ice) {
@@ -1080,7 +1180,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       }
       double ev = 0;
       if (loss > 0) fa = fmax(fa, f);
-      const double rc_ = A.revap > 0 ? A.revap_rhs : 1.0;
+      const double rc_ = A.revap > 0 ? (isl ? A.revap_rhl : A.revap_rhs) : 1.0;
       if (Fr > 0 && rc_ * qs > q[k]) {
         if (A.revap > 0) {
           const double a_ = fmax(fa, A.revap_area), ac = fmax(0.0, a_ - f);
@@ -1225,7 +1325,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       for (int k = 0; k < bot; k++) pt_ += P.grav * mu[k];
       F qs, dqs;
       layer_cap<MIX>(th[bot], pt_, pt_ + P.grav * mu[bot], P, A, qs, dqs);
-      const double rc_ = A.revap > 0 ? A.revap_rhs : 1.0;
+      const double rc_ = A.revap > 0 ? (isl ? A.revap_rhl : A.revap_rhs) : 1.0;
       if (rc_ * qs > q[bot]) {
         double ev;
         const double Fc = Rc / dt;
@@ -1249,9 +1349,73 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) o.wx[(size_t)j * N + i] = wxo[j];
   o.rain[i] = R / dt;
-  if (S > 0) {                                       // snow into the sea melts with the slab's heat
+  if (!isl && S > 0) {                               // snow into the sea melts with the slab's heat
     Ts[i] -= A.Lf * S / Cw;
     if (o.qocean) o.qocean[i] -= A.Lf * S / dt;
+  }
+  if (isl) {
+    // the ice's melt runs into the ponds, from the snow first. On ice the skin decides: below freezing rain and snow
+    // freeze as they land, a pond as fast as the surface can lose the heat (to the cold ice, the cold air, the sky;
+    // and into snow without firn at 10 mm/h), the freezing heat into the air touching the ground; at freezing rain
+    // runs on and the snow melts from that air. Bare ground: snow lies on frozen ground, its freezing (the rain's
+    // share) warming it; else rain runs to the ponds. A pond on frozen ground freezes up to its heat room; on thawed
+    // ground it soaks in at k_inf toward the bucket's capacity, and a full bucket springs.
+    const double scap = o.soilcap[i];
+    pw += ice0 - m_ice;
+    double sn = o.snow[i] - (ice0 - m_ice);
+    if (skin) {
+      if (tsk < A.Tf) {
+        const double room = (A.k_ice / dice * (A.Tf - tsk) + hs * fmax(0.0, A.Tf - Tsa) +
+                             fmax(0.0, A.eps_s * A.sigma * A.Tf * A.Tf * A.Tf * A.Tf - ldown - swS)) * dt / A.Lf +
+                            (sn > 0 && !fon ? 10.0 / 3600 * dt : 0.0);
+        const double mp = fmin(pw > 0 ? pw : 0.0, room), m = R + mp;
+        m_ice += m;
+        sn = (sn > 0 ? sn : 0.0) + R;
+        pw -= mp;
+#pragma unroll
+        for (int k = 0; k <= bot; k++) th[k] += A.Lf * (m - S) * (wbl[k] / mb) / (mu[k] * pib[k]);
+      } else {
+        pw += R;
+#pragma unroll
+        for (int k = 0; k <= bot; k++) th[k] -= A.Lf * S * (wbl[k] / mb) / (mu[k] * pib[k]);
+      }
+      if (fon && pw > 0 && Tfn < A.Tf) {             // melt and rain soak into the cold firn and refreeze
+        const double mr = fmin(pw, Cf * (A.Tf - Tfn) / A.Lf);
+        pw -= mr;
+        m_ice += mr;
+        sn = (sn > 0 ? sn : 0.0) + mr;
+        Tfn += A.Lf * mr / Cf;
+      }
+    } else if (Ts[i] < A.Tf) {
+      m_ice += R;
+      sn = (sn > 0 ? sn : 0.0) + R;
+      Ts[i] += A.Lf * (R - S) / Cw;
+    } else {
+      pw += R;
+      Ts[i] -= A.Lf * S / Cw;
+    }
+    if (!skin && Ts[i] < A.Tf && pw > 0) {
+      const double room = (A.Tf - Ts[i]) * Cw / A.Lf, m = pw < room ? pw : room;
+      pw -= m;
+      m_ice += m;
+      Ts[i] += A.Lf * m / Cw;
+    }
+    o.ice[i] = m_ice;
+    o.snow[i] = fmax(0.0, fmin(sn, m_ice));
+    if (!skin && Ts[i] >= A.Tf) {
+      const double room = A.k_inf * dt * (1 - wet / scap);
+      const double m = pw < room ? pw : room;
+      if (m > 0) {
+        pw -= m;
+        wet += m;
+      }
+    }
+    if (wet > scap) {
+      pw += wet - scap;
+      wet = scap;
+    }
+    o.soil[i] = wet;
+    o.pond[i] = pw;
   }
   if (o.rainp) {
     o.rainp[i] = Rk0 / dt;
@@ -1287,15 +1451,15 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 // kernels build in parallel with the rest of this file.
 int air_column_cloudy(int N, int L, int mixed, const CP& P, const AP& A, int mode, const double* t,
                       const double* insol, const double* pos, const double* uabs, const double* mu, double dt,
-                      const double* cap, const double* elev, const double* radf, const double* swf, const double* ch,
-                      const AirIO& o, double bytes, hipStream_t st) {
+                      const double* elev, const double* radf, const double* swf, const double* ch, const AirIO& o,
+                      double bytes, hipStream_t st) {
 #define CALL(LL)                                                                                                     \
   if (mixed)                                                                                                         \
     ISW_LAUNCH("air_column", bytes, (k_air_column<LL, true, true, true>), grid1(N), dim3(BS), N, P, A, mode, t,      \
-               insol, pos, uabs, mu, dt, cap, elev, radf, swf, ch, o);                                               \
+               insol, pos, uabs, mu, dt, elev, radf, swf, ch, o);                                               \
   else                                                                                                               \
     ISW_LAUNCH("air_column", bytes, (k_air_column<LL, false, true, true>), grid1(N), dim3(BS), N, P, A, mode, t,     \
-               insol, pos, uabs, mu, dt, cap, elev, radf, swf, ch, o);
+               insol, pos, uabs, mu, dt, elev, radf, swf, ch, o);
   ISW_BY_L(CALL)
 #undef CALL
   return (int)hipGetLastError();
@@ -1304,8 +1468,8 @@ int air_column_cloudy(int N, int L, int mixed, const CP& P, const AP& A, int mod
 // (defined in the ISW_COLUMN_CLOUDS=1 unit)
 int air_column_cloudy(int N, int L, int mixed, const CP& P, const AP& A, int mode, const double* t,
                       const double* insol, const double* pos, const double* uabs, const double* mu, double dt,
-                      const double* cap, const double* elev, const double* radf, const double* swf, const double* ch,
-                      const AirIO& o, double bytes, hipStream_t st);
+                      const double* elev, const double* radf, const double* swf, const double* ch, const AirIO& o,
+                      double bytes, hipStream_t st);
 
 int hs_column(int N, int L, const double* par, const double* hs, const double* lat, const double* mu,
               double* theta, double dt, double* wx, double* qnet, double* stored, hipStream_t st) {
@@ -1332,28 +1496,31 @@ int surface_layer(int N, int L, const double* par, const isw_air* air, const dou
 int air_column(int N, int L, const double* par, const isw_air* air, int mode, const double* t, const double* insol,
                const double* pos, const double* uabs, const double* mu, double* theta, double* qv, double* Ts,
                double* ice, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
-               double* evap, double* conv, double* tsurf, double* rainp, const double* cap, double* qocean,
-               double* qsw, const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
-               const double* ch, double* firn, double* cloudw, double* cloudo, int mixed, hipStream_t st) {
+               double* evap, double* conv, double* tsurf, double* rainp, double* cap, double* qocean,
+               double* qsw, const double* land, const double* soilcap, double* soil, double* pond, double* snow,
+               const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
+               const double* ch, const double* lc, double* firn, double* cloudw, double* cloudo, int mixed,
+               hipStream_t st) {
   const CP P = make_cp(par);
   const AP A = make_ap(par, air);
   const bool moist = P.moist > 0, cloudy = moist && par[55] > 0 && cloudw;
+  if (!moist) land = nullptr;                          // (the dry column has no land)
   const AirIO o{theta, qv, Ts, ice, wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, qocean, qsw, flash,
-                swalb, firn, cloudw, cloudo};
+                swalb, firn, cloudw, cloudo, cap, land, soilcap, lc, soil, pond, snow};
   const int RW = L + 1;
   const double rows_in = 1 + 2.0 * L + 1 + (mode == 0 ? 3 : 1) + (cap ? 1 : 0) + (elev ? 1 : 0) + (ch ? 1 : 0) +
                          (radf ? 3.0 * RW + 6 * L : 0) + (swf ? 3.0 * L + 1 : 0) + (firn ? 1 : 0) +
-                         (moist ? L + 1.0 : 0) + (cloudy ? L : 0);
+                         (moist ? L + 1.0 : 0) + (cloudy ? L : 0) + (land ? 5 : 0) + (land && lc ? 8 : 0);
   const double rows_out = L + 1 + (L - 1) + 6 + (rainp ? 4 : 0) + (qocean ? 1 : 0) + (qsw ? 1 : 0) + (flash ? 1 : 0) +
                           (swalb ? 1 : 0) + (firn ? 1 : 0) + (moist ? L + 1.0 : 0) + (cloudy ? L : 0) +
-                          (cloudy && cloudo ? (radf ? 4.0 * L + 3 : L + 3.0) : 0);
+                          (cloudy && cloudo ? (radf ? 4.0 * L + 3 : L + 3.0) : 0) + (land ? 3 : 0) +
+                          (land && cap ? 1 : 0);
   const double bytes = 8.0 * N * (rows_in + rows_out);
   if (cloudy)
-    return air_column_cloudy(N, L, mixed, P, A, mode, t, insol, pos, uabs, mu, dt, cap, elev, radf, swf, ch, o, bytes,
-                             st);
+    return air_column_cloudy(N, L, mixed, P, A, mode, t, insol, pos, uabs, mu, dt, elev, radf, swf, ch, o, bytes, st);
 #define LAUNCH(LL, MX, MO)                                                                                         \
   ISW_LAUNCH("air_column", bytes, (k_air_column<LL, MX, MO, false>), grid1(N), dim3(BS), N, P, A, mode, t, insol,  \
-             pos, uabs, mu, dt, cap, elev, radf, swf, ch, o)
+             pos, uabs, mu, dt, elev, radf, swf, ch, o)
 #define CALL(LL)                           \
   if (mixed) {                             \
     if (moist) LAUNCH(LL, true, true);     \

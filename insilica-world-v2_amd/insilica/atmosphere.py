@@ -107,7 +107,7 @@ class AirPhysics:
     longwave and shortwave), and the device clock. Parameters in the reference's PAR_ORDER array (`par`, host).
 
     forcing: 'held_suarez' steps the Held & Suarez column (air.cu hs_column); 'radiation' the reference's air column
-    (air.cu air_column), dry or moist over the sea, with or without clouds (land is not built here). mean_sun: the
+    (air.cu air_column), dry, or moist over the sea and land, with or without clouds. mean_sun: the
     annual-mean insolation instead of the diurnal and seasonal sun. longwave='ktables' builds `lw`, `sw` from
     lw_tables, sw_tables (npz paths or dicts). `cloudo` ((4L + 3), N): per layer cover, total cover, the clouds'
     effect at the top, per layer longwave and shortwave in-cloud optical depths and liquid share."""
@@ -140,6 +140,7 @@ class AirPhysics:
         self.sfc = self.p['sfc'] > 0
         self.cd, self.ch = (z(N), z(N)) if self.sfc else (None, None)
         self.cloudo = z(4 * layers + 3, N)
+        self.cover = None                                  # the land cover's rows (8, N), when a world gives them
         self.lw = self.sw = None
         if longwave == 'ktables':
             from .radiation import Longwave, Shortwave
@@ -179,7 +180,7 @@ class AirPhysics:
         self._speed(u)
         _hip.call('isw_surface_layer', self.dg.N, self.L, self._par, ctypes.byref(self.air), _hip.ptr(self.uabs),
                   _hip.ptr(mu), _hip.ptr(theta), _hip.ptr(q), _hip.ptr(Ts), _hip.ptr(ice), _hip.ptr(land),
-                  _hip.ptr(self.cd), _hip.ptr(self.ch), _hip.ptr(lc), _hip.stream())
+                  _hip.ptr(self.cd), _hip.ptr(self.ch), _hip.ptr(lc if lc is not None else self.cover), _hip.stream())
 
     def column(self, mu, theta, Ts, u, t, dt, q=None, ice=None, cap=None, qocean=None, qsw=None, land=None,
                soilcap=None, soil=None, pond=None, snow=None, cw=None, elev=None, firn=None, gx=None):
@@ -191,9 +192,11 @@ class AirPhysics:
         for moist air (moist > 0; q (L, N) and ice (N) updated in place) evaporation, sea ice, condensation and the
         parcel plume; with clouds (cloud > 0 and the cloud water cw (L, N), updated in place) the cloud statistics,
         autoconversion, ice fall and rain evaporation, the clouds' reflection and emission, and their cover and
-        optical depths into self.cloudo for the next radiate(). Into theta, Ts, self.wx, self.olr, self.sw_abs,
-        self.stored, self.rain, self.evap, self.conv, self.tsurf, self.rainp, self.flash. Land and the column's gas
-        tracers (gx) are not in this library and raise."""
+        optical depths into self.cloudo for the next radiate(); over land (the mask land (N), the bucket's capacity
+        soilcap; soil, pond and snow updated in place; self.cover the land cover when set) the ground's water,
+        snow and heat. Into theta, Ts, self.wx, self.olr, self.sw_abs, self.stored, self.rain, self.evap,
+        self.conv, self.tsurf, self.rainp, self.flash. The column's gas tracers (gx) are not in this library and
+        raise."""
         from . import _hip
         if self.forcing == 'held_suarez':
             _hip.call('isw_hs_column', self.dg.N, self.L, self._par, self.hs.ctypes.data_as(ctypes.c_void_p),
@@ -203,8 +206,6 @@ class AirPhysics:
             return
         if self.moist and (q is None or ice is None):
             raise ValueError('the moist column needs the vapor q and the sea ice')
-        if land is not None:
-            raise NotImplementedError('the column over land is not in this library')
         if gx is not None:
             raise NotImplementedError('the gas tracers carried by the column\'s crossing are not in this library')
         self._speed(u)
@@ -214,11 +215,11 @@ class AirPhysics:
                   P(self.insol), P(self.pos), P(self.uabs), P(mu), P(theta), P(q) if self.moist else None, P(Ts),
                   P(ice) if self.moist else None, float(dt),
                   P(self.wx), P(self.olr), P(self.sw_abs), P(self.stored), P(self.rain), P(self.evap), P(self.conv),
-                  P(self.tsurf), P(self.rainp), None, P(cap), P(qocean), P(qsw), None, None, None, None, None, P(cw),
-                  P(self.cloudo) if cw is not None else None, P(elev), P(self.flash),
+                  P(self.tsurf), P(self.rainp), None, P(cap), P(qocean), P(qsw), P(land), P(soilcap), P(soil), P(pond),
+                  P(snow), P(cw), P(self.cloudo) if cw is not None else None, P(elev), P(self.flash),
                   P(lw.radf) if lw is not None else None,
                   P(sw.swf) if sw is not None else None, P(self.swalb) if sw is not None else None,
-                  P(self.ch) if self.sfc else None, None, P(firn), None, self._mixed, _hip.stream())
+                  P(self.ch) if self.sfc else None, P(self.cover), P(firn), None, self._mixed, _hip.stream())
         self.advance_time(t, dt)
 
     @staticmethod

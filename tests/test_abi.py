@@ -95,6 +95,10 @@ def test_bad_arguments_rejected_on_host():
                               *([buf] * 8), None, None, *([None] * 7), buf, buf, buf, *([None] * 8), None, 0,
                               None) == E
     par[55] = par[89] = 0.0
+    # land without its bucket's capacity, soil, ponds or snow
+    assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
+                              *([buf] * 8), None, None, None, None, None, buf, buf, buf, None, buf, None, None,
+                              *([None] * 8), None, 0, None) == E
     par[22] = 0.0
     assert air_column(dgo=buf) == E and air_column(xfer=buf) == E and air_column(a=None) == E
     assert air_column(t=None) == E and air_column(mode=1) == E and air_column(mode=2, insol=buf) == E

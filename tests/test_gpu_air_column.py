@@ -50,7 +50,8 @@ def columns(N, L, seed):
 
 
 def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None, swf=None, ch=None, firn=None,
-                   mixed=False, ocean=False, q=None, ice=None, cw=None, cloudo=False):
+                   mixed=False, ocean=False, q=None, ice=None, cw=None, cloudo=False, land=None, soilcap=None,
+                   soil=None, pond=None, snow=None, lc=None):
     from insilica import _hip
     from insilica.radiation import _air
     L, N = c['mu'].shape
@@ -68,6 +69,11 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
         o['cw'] = _dev(cw)
         if cloudo:
             o['cloudo'] = z(4 * L + 3, N)
+    if land is not None:
+        o['soil'], o['pond'], o['snow'] = _dev(soil), _dev(pond), _dev(snow)
+        if cap is not None:
+            o['cap'] = _dev(cap)
+    tl, tsc, tlc = _dev(land), _dev(soilcap), _dev(lc)
     ins = [_dev(x) for x in (np.array([t]), insol, c['pos'], c['uabs'], c['mu'], cap, elev, radf, swf, ch)]
     tt, ti, tp, tu, tm, tc, te, tr, ts, tch = ins
     P = _hip.ptr
@@ -76,8 +82,9 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
               P(tp), P(tu), P(tm), P(o['theta']), P(o.get('q')), P(o['Ts']), P(o.get('ice')), float(dt), P(o['wx']),
               P(o['olr']),
               P(o['sw_abs']), P(o['stored']), P(o['rain']), P(o['evap']), P(o['conv']), P(o['tsurf']),
-              P(o['rainp']), None, P(tc), P(o.get('qocean')), P(o.get('qsw')), None, None, None, None, None,
-              P(o.get('cw')), P(o.get('cloudo')), P(te), P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), None, P(o.get('firn')), None,
+              P(o['rainp']), None, P(o.get('cap', tc)), P(o.get('qocean')), P(o.get('qsw')), P(tl), P(tsc),
+              P(o.get('soil')), P(o.get('pond')), P(o.get('snow')), P(o.get('cw')), P(o.get('cloudo')), P(te),
+              P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), P(tlc), P(o.get('firn')), None,
               1 if mixed else 0, _hip.stream())
     torch.cuda.synchronize()
     return {k: _np(v) for k, v in o.items()}
@@ -85,7 +92,8 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
 
 def compare(g, o, tol, dt, label, bars=None):
     keys = [k for k in ('theta', 'Ts', 'wx', 'olr', 'sw_abs', 'tsurf', 'qocean', 'qsw', 'swalb', 'firn', 'q', 'ice',
-                        'cw', 'cloudo') if k in o] + (['rain', 'evap', 'conv', 'rainp'] if 'q' in o else [])
+                        'cw', 'cloudo', 'soil', 'pond', 'snow', 'cap') if k in o] + \
+        (['rain', 'evap', 'conv', 'rainp'] if 'q' in o else [])
     e = {k: relmax(g[k], o[k]) for k in keys}
     e['stored'] = relmax(g['stored'], o['stored'])
     print(label, {k: float(f'{v:.3g}') for k, v in e.items()}, 'moved', int(np.count_nonzero(o['wx'])), flush=True)
@@ -218,8 +226,9 @@ def cloudy_columns(N, L, seed):
                                           (3, True, {}), (7, True, dict(ice_air=1.0, conv_revap=0.05))])
 def test_clouds(L, mixed, opts):
     """The moist column with clouds: the cloud water, the cover and the clouds' effects at the top (cloudo's first
-    L + 3 rows) to the moist column's bars. The mixed form in seven layers: the masses crossing an interface to 3e-4
-    and the vapor to 3e-5 (measured 1.7e-4 and 1.3e-5 in the worst of 40962 cells, 22 and 2 cells above 1e-5; no
+    L + 3 rows) to the moist column's bars. The mixed form in seven layers: the masses crossing an interface to 3e-4,
+    vapor and theta to 3e-5 (measured 1.7e-4, 1.3e-5 and 5e-6 in the worst of 40962 cells, 22, 2 and 0 cells above
+    1e-5; with land beside it theta 1.3e-5; no
     branch flips: a crossing's mass is the layer's net heating, radiative cooling less float32 latent heating, times
     its edge density, so one-ulp device-against-glibc expm1f/powf differences in the autoconversion and condensation
     reach it amplified)."""
@@ -235,7 +244,7 @@ def test_clouds(L, mixed, opts):
     if 'ice_air' in opts:
         assert o['rainp'][3].max() > 0                      # snow reached the ground
     compare(g, o, 1e-5 if mixed else 1e-12, dt, f'clouds L={L} mixed={mixed}',
-            bars=dict(wx=3e-4, q=3e-5) if mixed else None)
+            bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed else None)
 
 
 def test_clouds_ktables():
@@ -257,6 +266,63 @@ def test_clouds_ktables():
     o = ocol.air_column(p, 1, np.array([0.0]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, **kw)
     assert np.abs(o['cloudo'][2 * L + 3:3 * L + 3]).max() > 0
     compare(g, o, 1e-12, dt, 'clouds k-tables')
+
+
+def land_fields(c, seed, firn=False):
+    """Two thirds of the cells land: a soil bucket of 50-300 kg/m^2 a fifth to fully wet, ponds on 40%, snow on the
+    land's ice (with firn: an ice sheet of at least 1.5 t/m^2 on the land's polar cells), the ground's capacity."""
+    N = c['Ts'].size
+    rng = np.random.default_rng(seed)
+    land = (rng.uniform(size=N) < 2 / 3).astype(float)
+    f = dict(land=land, soilcap=rng.uniform(50.0, 300.0, N))
+    f['soil'] = f['soilcap'] * rng.uniform(0.2, 1.0, N)
+    f['pond'] = np.where(rng.uniform(size=N) < 0.4, rng.uniform(0.0, 60.0, N), 0.0)
+    if firn:
+        polar = (land > 0.5) & (np.abs(c['pos'][:, 2]) > 0.8)
+        c['ice'] = np.where(polar, np.maximum(c['ice'], 1500.0), c['ice'])
+    f['snow'] = np.minimum(c['ice'], rng.uniform(0.0, 80.0, N))
+    f['cap'] = np.where(land > 0.5, 2.0e6, 4.186e8)
+    return f
+
+
+def land_cover(N, seed):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(0.01, 1.0, N), rng.uniform(0.25, 0.35, N), rng.uniform(0.1, 0.2, N),
+                     rng.uniform(0.0, 1.0, N), rng.uniform(0.1, 0.2, N), rng.uniform(0.3, 1.0, N),
+                     rng.uniform(50.0, 150.0, N), rng.uniform(100.0, 250.0, N)])
+
+
+@pytest.mark.parametrize('L,mixed,opts', [
+    (3, False, {}),
+    (5, False, dict(land_rs=100.0, rs_max=5000.0, c_deep=1.0, c_wet=3.0e6, c_dry=1.3e6, land_exch=2.0)),
+    (4, False, dict(firn=2.0, ice_air=1.0, cloud=1.0, revap=1.0, land_rs=100.0, bare_fx=2.0, lc=1)),
+    (3, True, {}),
+    (7, True, dict(c_deep=1.0, land_rs=100.0, cloud=1.0, ice_air=1.0))])
+def test_land(L, mixed, opts):
+    """The moist column over land and sea: the soil's, ponds' and snow's water and the ground's capacity to the moist
+    column's bars (the mixed form with clouds as the cloudy column's)."""
+    from oracle import column as ocol
+    opts = dict(opts)
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column land L={L} mixed={mixed} {opts} N={N}', flush=True)
+    cloudy = opts.get('cloud', 0) > 0
+    c = cloudy_columns(N, L, seed=90 + L) if cloudy else moist_columns(N, L, seed=90 + L, polar=45.0)
+    kw = land_fields(c, seed=95 + L, firn='firn' in opts)
+    if opts.pop('lc', 0):
+        kw['lc'] = land_cover(N, 7)
+    if 'firn' in opts:
+        kw['firn'] = np.random.default_rng(3).uniform(240.0, 273.15, N)
+    if cloudy:
+        kw.update(cw=c['cw'], cloudo=True)
+    p = par(L, moist=1.0, **opts)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, q=c['q'], ice=c['ice'], ocean=True, **kw)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed,
+                        q=c['q'], ice=c['ice'], ocean=True, **kw)
+    land = kw['land'] > 0.5
+    assert o['evap'][land].max() > 0 and o['rain'][land].max() > 0
+    assert np.abs(o['soil'] - kw['soil']).max() > 0 and np.abs(o['snow'] - kw['snow']).max() > 0
+    compare(g, o, 1e-5 if mixed else 1e-12, dt, f'land L={L} mixed={mixed}',
+            bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed and cloudy else None)
 
 
 def test_aqua_dry_world():
