@@ -33,7 +33,8 @@ def parse():
     ap.add_argument('--level', type=int, default=6)
     ap.add_argument('--no-fp64', action='store_true', help='skip the plain FP64 form')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU oracle baseline')
-    ap.add_argument('--cpu-hours', type=float, default=48.0, help='model hours of the CPU sample')
+    ap.add_argument('--cpu-hours', type=float, default=0.0,
+                    help='model hours of the CPU sample (0: 48 for coupled_stack, 150 for the aquaplanets)')
     ap.add_argument('--ensemble', type=int, default=0, help='run the batched-ensemble workload with this many worlds')
     ap.add_argument('--ens-days', type=int, default=1, help='model days the ensemble advances')
     ap.add_argument('--seed', type=int, default=0)
@@ -43,6 +44,8 @@ def parse():
     ap.add_argument('--calibrate-copy', type=int, default=0,
                     help='MiB: first run one plain copy of this size (the reference dispatch for HBM counters)')
     ap.add_argument('--radiation', action='store_true', help='run the column-radiation workload (passes/s)')
+    ap.add_argument('--preset', default='coupled_stack', choices=sorted(PRESET_LINES),
+                    help='the world the default workload times (aqua: the moist aquaplanet with its column physics)')
     ap.add_argument('--cpu-cols', type=int, default=0, help='radiation: columns in the CPU oracle sample (0: all)')
     return ap.parse_args()
 
@@ -63,14 +66,25 @@ def pmc_traffic(name):
     return None
 
 
-def make_world(level, precision):
+# The worlds the default workload can time: the description in the line's config, and the reference's own published
+# number for that world at level 6 in the mixed form where it has one (model-hours/s; its RTX 4070, SPEC.md:560).
+PRESET_LINES = {
+    'coupled_stack': ('3 air + 2 ocean layers, dt 450 s, nfast 15, 15 air + 4 ocean tracer rows, 30% land', None),
+    'aqua': ('3 moist air layers over a slab sea with sea ice, dt 450 s, the sun, gray radiation, the air column '
+             '(evaporation, condensation, plumes, the crossing) and the exchange every tick', 41.8),
+    'aqua5': ('the moist aquaplanet in 5 air layers, dt 450 s, the column every tick', None),
+    'aqua7': ('the moist aquaplanet in 7 air layers, dt 450 s, the column every tick', None),
+}
+
+
+def make_world(level, precision, preset='coupled_stack'):
     from insilica.world import World
-    return World(level, 'coupled_stack', precision=precision, params=dict(nfast=15))
+    return World(level, preset, precision=precision, params=dict(nfast=15) if preset == 'coupled_stack' else None)
 
 
 def time_world(w, steps, warmup):
     import torch
-    ticks = w.ocean_every
+    ticks = int(round(w.ticks_per_hour))
     w.step(warmup * ticks)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -85,7 +99,7 @@ def kernel_profile(w, hours=1):
     from insilica import _hip
     torch.cuda.synchronize()
     _hip.timing_start()
-    w.step(hours * w.ocean_every, graph=False)
+    w.step(hours * int(round(w.ticks_per_hour)), graph=False)
     torch.cuda.synchronize()
     return _hip.timing_stop()
 
@@ -377,13 +391,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     print(f'building level-{a.level} worlds', file=sys.stderr, flush=True)
-    wm = make_world(a.level, 'mixed')
+    wm = make_world(a.level, 'mixed', a.preset)
     el_m = time_world(wm, a.steps, a.warmup)
     print(f'mixed: {a.steps} model hours in {el_m:.3f} s', file=sys.stderr, flush=True)
     prof = kernel_profile(wm)
     fp64 = None
     if not a.no_fp64:
-        wd = make_world(a.level, 'double')
+        wd = make_world(a.level, 'double', a.preset)
         el_d = time_world(wd, a.steps, a.warmup)
         fp64 = a.steps / el_d
         print(f'fp64: {a.steps} model hours in {el_d:.3f} s', file=sys.stderr, flush=True)
@@ -395,7 +409,7 @@ def main():
     achieved = bytes_per / avg_s / 1e9
     total_ms = sum(v['ms'] for v in prof.values())
     total_bytes = sum(v['bytes'] for v in prof.values())
-    traffic = pmc_traffic(name) if a.level == 6 else None
+    traffic = pmc_traffic(name) if a.level == 6 and a.preset == 'coupled_stack' else None
     roofline = dict(bound='hbm', kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, bytes_per_launch=bytes_per,
                     traffic_source=TRAFFIC_PROFILE.name if traffic else None,
@@ -407,21 +421,29 @@ def main():
         from oracle.world import OracleWorld
         cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
         o = OracleWorld(wm)
-        ticks = max(1, int(round(a.cpu_hours * wm.ocean_every)))
+        tph = int(round(wm.ticks_per_hour))
+        hours = a.cpu_hours or (48.0 if a.preset == 'coupled_stack' else 150.0)     # (about 10-20 s of CPU work)
+        ticks = max(1, int(round(hours * tph)))
         t0 = time.perf_counter()
         o.step(ticks)
         el = time.perf_counter() - t0
-        cpu = dict(value=round(ticks / wm.ocean_every / el, 4), unit='model-hours/s', cores=cores, kind='port',
-                   sample=f'{ticks} ticks ({ticks / wm.ocean_every:g} model hours, one ocean step per hour) of the '
-                          f'same level-{a.level} coupled world, FP64 oracle (C, OpenMP), {el:.1f} s')
+        cpu = dict(value=round(ticks / tph / el, 4), unit='model-hours/s', cores=cores, kind='port',
+                   sample=f'{ticks} ticks ({ticks / tph:g} model hours) of the same level-{a.level} {a.preset} world, '
+                          f'FP64 oracle (C, OpenMP), {el:.1f} s')
     ms = el_m / a.steps * 1e3
-    line = dict(metric='model-hours/s at level 6 (FP64 vs mixed, 1 GPU)', value=round(a.steps / el_m, 3),
+    what, published = PRESET_LINES[a.preset]
+    value = a.steps / el_m
+    line = dict(metric=f'model-hours/s at level {a.level} (FP64 vs mixed, 1 GPU)' +
+                ('' if a.preset == 'coupled_stack' else f', {a.preset}'), value=round(value, 3),
                 unit='model-hours/s', n_gpus=1, steps=a.steps, warmup=a.warmup, ms_per_step=round(ms, 3),
-                higher_is_better=True, scaling='weak', vs_baseline=None, dtype='f64 state, f32 stencils (mixed)',
-                data='synthetic',
-                config=dict(workload=f'coupled_stack level {a.level}: 3 air + 2 ocean layers, dt 450 s, nfast 15, '
-                                     f'15 air + 4 ocean tracer rows, 30% land; step = 1 model hour (8 ticks)',
+                higher_is_better=True, scaling='weak',
+                vs_baseline=round(value / published, 3) if published and a.level == 6 else None,
+                dtype='f64 state, f32 stencils (mixed)', data='synthetic',
+                config=dict(workload=f'{a.preset} level {a.level}: {what}; step = 1 model hour '
+                                     f'({int(round(wm.ticks_per_hour))} ticks)',
                             cells=wm.grid.N, edges=wm.grid.E, parallelism='1 GPU'),
+                baseline=(dict(value=published, unit='model-hours/s', hardware='RTX 4070 (the reference, mixed)',
+                               source='SPEC.md:560') if published and a.level == 6 else None),
                 fp64_value=round(fp64, 3) if fp64 else None,
                 mixed_over_fp64=round(a.steps / el_m / fp64, 3) if fp64 else None,
                 roofline=roofline, cpu_baseline=cpu,
