@@ -123,7 +123,7 @@ static void move_mass(int L, const double* par, double* mu, double* th, double* 
  * mixed bottom layer (bl_mix) needs L >= 3 (its top edge is then an interface). dtop/dbot (optional): the heating at
  * each layer's top and bottom edge when it differs from the layer's mean (the k-tables' edge sub-levels). */
 static void crossing(int L, const double* par, double* mu, double* th, double* q, const double* dth, double dth_up,
-                     double* pib, double* wxo, const double* dtop, const double* dbot, double* cl) {
+                     double* pib, double* wxo, const double* dtop, const double* dbot, double* cl, double bl_room) {
   const double floor_ = par[P_FLOOR];
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (par[P_THETA_STAR] <= 0) return;
@@ -147,7 +147,7 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
       if (th[b] >= ts_) {                              /* the mixed bottom layer takes in the air above */
         double fa, fb;
         DENS(a, fa, fb);
-        dm = -fmin(fmin(mu[a] * dth[b] * fa, fmax(mu[a] - par[P_MU_BL], 0.0) * par[P_BL_CAP]), 1e30);
+        dm = -fmin(fmin(mu[a] * dth[b] * fa, fmax(mu[a] - par[P_MU_BL], 0.0) * par[P_BL_CAP]), bl_room);
         thd = ts_ - 0.5 * dth[b];
       }
     } else if (blm && b < L - 1 && th[b] >= ts_) {
@@ -258,7 +258,7 @@ void og_hs_column(int N, int L, const double* par, const double* hs, const doubl
       const double ki = hs[0] + (hs[1] - hs[0]) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
       dth_up = (thi - par[P_THETA_STAR]) * (1 - exp(-ki * dt));
     }
-    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL, NULL);
+    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL, NULL, 1e30);
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
     double E1 = 0;
     for (int k = 0; k < L; k++) {
@@ -383,7 +383,7 @@ enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5,
        P_CW_QS = 129, P_ALB_DRY = 35, P_ALB_WET = 36, P_SOIL_BETA = 37, P_K_INF = 38, P_POND_COVER = 39,
        P_ALB_SNOW = 40, P_ALB_SNOW_MELT = 42, P_ALB_GLACIER = 43, P_SNOW_COVER = 44, P_LAND_EXCH = 46, P_C_DRY = 49,
        P_C_WET = 50, P_LAND_RS = 52, P_C_DEEP = 94, P_RS_MAX = 95, P_RS_SUN = 96, P_REVAP_RHL = 105, P_BARE_FX = 115,
-       P_K_FIRN = 121 };
+       P_K_FIRN = 121, P_MU_NIGHT = 71, P_BL_ENT = 72, P_BL_REACH = 98, P_BL_LCL = 100, P_ENT_LCL = 130 };
 
 static double omexp(double x, int mix) { return mix ? (double)-expm1f(-(float)x) : 1 - exp(-(x)); }
 static double fpow_(double x, double y, int mix) { return mix ? (double)powf((float)x, (float)y) : pow(x, y); }
@@ -572,6 +572,27 @@ static double base_T(int k, const double* mu, const double* th, const double* pa
   return th[k] * fpow_(pb / par[P_P0], par[P_KAPPA], mix);
 }
 
+/* lcl_mass (air.cu:1413-1421): the mixed bottom layer's mass up to the level where its air, lifted
+ * dry-adiabatically, saturates (40 bisections in pressure); 0 if saturated at the ground, 1e30 if not even at a tenth
+ * of the surface pressure. */
+static double lcl_mass(int L, const double* mu, const double* th, const double* q, const double* par, double eps,
+                       int mix) {
+  const int bot = L - 1;
+  double ps_ = 0;
+  for (int k = 0; k < L; k++) ps_ += par[P_GRAV] * mu[k];
+#define SAT(p_) (qsat_x(th[bot] * fpow_((p_) / par[P_P0], par[P_KAPPA], mix), (p_), eps, par[P_ICE_AIR], mix) <= q[bot])
+  if (SAT(ps_)) return 0;
+  if (!SAT(0.1 * ps_)) return 1e30;
+  double lo = 0.1 * ps_, hi = ps_;
+  for (int it = 0; it < 40; it++) {
+    const double m_ = 0.5 * (lo + hi);
+    if (SAT(m_)) lo = m_;
+    else hi = m_;
+  }
+#undef SAT
+  return (ps_ - 0.5 * (lo + hi)) / par[P_GRAV];
+}
+
 void og_air_column(int N, int L, const double* par_in, const og_air* air, int mode, const double* t,
                    const double* insol, const double* pos, const double* uabs, const double* mu_in, double* theta,
                    double* qv, double* Ts, double* ice, double dt, double* wx, double* olr, double* sw_abs,
@@ -585,6 +606,9 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     double par[132 + OG_LMAX];
     memcpy(par, par_in, sizeof(par));
     if (elev && par[P_THETA_RISE] > 0 && elev[i] > 0) par[P_THETA_STAR] += par[P_THETA_RISE] * elev[i];
+    const double ts_base = par[P_THETA_STAR];
+    par[P_BL_CAP] = omexp(dt / 86400.0, mix);       /* (the mixed layer takes in at most a day's share per step) */
+    const int blm = par[P_BL_MIX] > 0;
     const double grav = par[P_GRAV], cp = par[P_CP], kappa = par[P_KAPPA], p0 = par[P_P0], sigma = par[P_SIGMA];
     const double eps_s = par[P_EPS_S], Tf = par[P_TF], Lf = par[P_LF], Lv = par[P_LV], aeps = air->eps;
     const int moist = par[P_MOIST] > 0;
@@ -604,6 +628,8 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     layer_exner(L, mu, par, pb, pib);
     const int top = 0, bot = L - 1;
     const double exb = pow(pb[bot] / p0, kappa);     /* (p_bot / p0)^kappa at the column's lower edge */
+    /* the mixed bottom layer: the lowest interface's theta* no colder than the layer, below the air above's mean */
+    if (blm && L > 2) par[P_THETA_STAR] = fmin(fmax(ts_base, th[L - 1]), th[L - 2] - 1);
     double m_ice = moist ? ice[i] : 0.0;
     const int isl = moist && land && land[i] > 0.5;  /* land: the soil bucket, ponds, snow and the ground's heat */
     if (cap && par[P_C_WET] > 0 && isl) {            /* (c_wet: the ground's capacity by its water, its heat kept) */
@@ -634,7 +660,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     double wbl[OG_LMAX], thb = 0, qb = 0, mb = 0;
     {
       const double mu_bl = par[P_MU_BL], bl_bot = par[P_BL_BOT];
-      double need = bl_bot > 0 && mu[bot] < mu_bl ? fmax(mu[bot], bl_bot * mu_bl) : mu_bl;
+      double need = blm ? mu[bot] : bl_bot > 0 && mu[bot] < mu_bl ? fmax(mu[bot], bl_bot * mu_bl) : mu_bl;
       for (int k = bot; k >= 0; k--) {
         const double m = mu[k] < need ? mu[k] : need;
         wbl[k] = m;
@@ -1048,6 +1074,14 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         Rk[k == bot ? 1 : 0] += mu[k] * dq;
       }
     /* the crossing, with the k-tables' edge sub-levels' own heating at the layers' edges */
+    if (blm && L > 2) par[P_THETA_STAR] = fmin(fmax(ts_base, th[bot]), th[bot - 1] - 1);
+    /* (bl_lcl: the mixed layer ends at its condensation level; lcl_mass: the layer's mass up to where its air, lifted
+     * dry-adiabatically, saturates, by bisection in pressure) */
+    double mu_lcl = 1e30, bl_room = 1e30;
+    if (par[P_BL_LCL] > 0 && blm && L > 2) {
+      mu_lcl = lcl_mass(L, mu, th, q, par, aeps, mix);
+      bl_room = fmax(0.0, mu_lcl - mu[bot]);
+    }
     double dtop[OG_LMAX], dbot[OG_LMAX], dtop1[OG_LMAX], wxo[OG_LMAX];
     if (radf) {
       double pt_ = 0;
@@ -1059,6 +1093,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         const double mean = qrad[k] * dt / (mu[k] * pib[k]);
         dtop[k] = dth[k] - mean + qet[k] * dt / (ft * mu[k] * pit);
         dbot[k] = dth[k] - mean + qeb[k] * dt / (fb * mu[k] * pibt);
+        if (k == L - 1 && blm) dtop[k] = dbot[k] = dth[k];   /* (the mixed bottom layer heats evenly) */
         pt_ = pbk;
       }
       if (par[P_DRY_EDGE] > 0 && L > 2) dtop[(int)par[P_EDGE_K]] -= dlat[(int)par[P_EDGE_K]];
@@ -1069,7 +1104,70 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       for (int k = 1; k <= par[P_EDGE_K] && k < L; k++) dtop1[k] = dtop[k];
     }
     crossing(L, par, mu, th, q, dth, dth[top], pib, wxo,
-             radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL, cloudy ? cl : NULL);
+             radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL, cloudy ? cl : NULL,
+             bl_room);
+    /* the mixed boundary layer: the surface's buoyancy flux sets its depth. Rising it entrains the air above at bl_ent
+     * of that flux across the jump to it (Tennekes 1973), a slice from the layer's bottom edge; falling, it gives its
+     * top back down to a thin night layer mu_night. bl_reach: dry convection first takes in the air above colder
+     * than it, layer by layer while they empty. bl_lcl: it ends at its condensation level, the air above that leaving
+     * into the layer above. */
+    if (blm && L > 2) {
+      int ka = bot - 1;
+#define REACH_EDGE(k_) ((k_) < bot - 1 ? fmax(tstar((k_), L, par), th[bot]) : tstar((k_), L, par))
+      if (par[P_BL_REACH] > 0) {
+        for (int k = bot - 1; k >= 1; k--) {
+          const double avail = mu[k] - par[P_FLOOR];
+          if (avail > 0) {
+            const double e = REACH_EDGE(k), D = th[k] - e;
+            const int spread = D > 0 && th[k] < tstar(k - 1, L, par);
+            const double x = spread ? (th[bot] > e ? mu[bot] / mu[k] * (sqrt(1 + mu[k] * (th[bot] - e) / (mu[bot] * D)) - 1) : 0.0)
+                                    : (th[k] < th[bot] ? 1.0 : 0.0);
+            const double want = fmin(1.0, x) * mu[k];
+            const double m = fmin(fmin(fmin(want, avail), par[P_BL_CAP] * fmax(mu[k] - par[P_MU_BL], par[P_MU_BL])),
+                                  fmax(0.0, mu_lcl - mu[bot]));
+            if (m > 0) {
+              const double fm = m / mu[k], thl = spread ? e + fm * D : th[k];
+              const double qx = q[k];
+              move_mass(L, par, mu, th, q, pib, k, bot, m, thl, thl, qx, qx, cloudy ? cl : NULL);
+              for (int j = k; j < bot; j++) wxo[j] -= m;
+            }
+            if (!(m >= avail * (1 - 1e-9))) break;
+          }
+        }
+        ka = bot - 1;
+        while (ka > 1 && !(mu[ka] > par[P_FLOOR] * 1.000001)) ka--;
+      }
+      const double eka = REACH_EDGE(ka);
+#undef REACH_EDGE
+      const double Bf = H + air->vf * cp * Tsa * E, jump = fmax(1.0, eka - th[bot]);
+      if (Bf > 0) {
+        const double m = fmin(fmin(par[P_BL_ENT] * Bf * dt / (cp * jump), mu[ka] - par[P_FLOOR]),
+                              par[P_ENT_LCL] > 0 ? 1e30 : fmax(0.0, mu_lcl - mu[bot]));
+        if (m > 0) {
+          const int spread = ka >= 1 && th[ka] < tstar(ka - 1, L, par) && th[ka] > eka;
+          const double fm = fmin(1.0, m / mu[ka]);
+          const double thc = spread ? eka + fm * (th[ka] - eka) : th[ka];
+          const double qx = q[ka];
+          move_mass(L, par, mu, th, q, pib, ka, bot, m, thc, thc, qx, qx, cloudy ? cl : NULL);
+          for (int j = ka; j < bot; j++) wxo[j] -= m;
+        }
+      }
+      if (!(Bf > 0) && mu[bot] > par[P_MU_NIGHT]) {
+        const double m = fmin(-Bf * dt / (cp * jump), mu[bot] - par[P_MU_NIGHT]);
+        const double qx = q[bot];
+        move_mass(L, par, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cloudy ? cl : NULL);
+        wxo[bot - 1] += m;
+      }
+      if (par[P_BL_LCL] > 0) {
+        const double m = mu[bot] - fmax(lcl_mass(L, mu, th, q, par, aeps, mix), par[P_MU_NIGHT]);
+        if (m > 0) {
+          const double qx = q[bot];
+          move_mass(L, par, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cloudy ? cl : NULL);
+          wxo[bot - 1] += m;
+        }
+      }
+    }
+
     /* moist convection: a parcel of bottom-layer air lifted to the interface above it; buoyant there (theta_p above
      * theta*), it rises through every interface it stays buoyant at and detrains above the last, its condensate
      * raining; the mass that would remove the buoyancy at the first interface relaxes over tau_conv */
@@ -1220,6 +1318,22 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       rainp[(size_t)2 * N + i] = Rc / dt;
       rainp[(size_t)3 * N + i] = S / dt;
     }
+    /* a layer emptied to its floor keeps a theta within its edges: its heat beyond them goes to the nearest layer
+     * with mass (below first), the column's energy unchanged */
+    if (blm)
+      for (int k = 1; k < bot; k++) {
+        if (mu[k] > 2 * par[P_FLOOR]) continue;
+        const double lo = tstar(k, L, par), hi = tstar(k - 1, L, par), tn = fmin(fmax(th[k], lo), hi);
+        if (tn == th[k]) continue;
+        int n = -1;
+        for (int j = k + 1; j < L && n < 0; j++)
+          if (mu[j] > 2 * par[P_FLOOR]) n = j;
+        for (int j = k - 1; j >= 0 && n < 0; j--)
+          if (mu[j] > 2 * par[P_FLOOR]) n = j;
+        if (n < 0) continue;
+        th[n] += mu[k] * pib[k] * (th[k] - tn) / (mu[n] * pib[n]);
+        th[k] = tn;
+      }
     const double E1 = column_energy(L, mu, pib, th, q, par) + Cw * Ts[i] - Lf * m_ice + Cf * (Tfn - Tf);
     if (Cf > 0) firn[i] = Tfn;
     for (int k = 0; k < L; k++) {

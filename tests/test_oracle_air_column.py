@@ -681,3 +681,91 @@ def test_ground_skin_implicit():
     assert o['rain'].max() == 0
     np.testing.assert_allclose(o['Ts'][land], o['tsurf'][land], atol=1e-9)
     assert np.abs(o['Ts'] - c['Ts'])[land].max() > 0.1      # (and it moved)
+
+
+# ---- the mixed boundary layer (bl_mix: air.cu air_column's mixed bottom layer, its entrainment, night layer,
+# dry convection and condensation level) ----
+
+BLM_OPTS = [dict(), dict(bl_reach=1.0), dict(bl_lcl=1.0), dict(bl_lcl=1.0, ent_lcl=1.0, bl_reach=1.0),
+            dict(moist=0.0), dict(cloud=1.0, bl_lcl=1.0), dict(land=1, bl_reach=1.0, mu_night=2000.0)]
+
+
+@pytest.mark.parametrize('opts', BLM_OPTS)
+@pytest.mark.parametrize('mix', [False, True])
+def test_mixed_layer_closes(opts, mix):
+    """With the mixed bottom layer the energy still closes, the water over the sea changes by evaporation less rain
+    (over land by nothing with the ground's), the column's mass is only moved between layers and every layer keeps
+    its floor."""
+    opts = dict(opts)
+    land = opts.pop('land', 0)
+    c = land_aqua() if land else moist_aqua(rh=(0.1, 0.8, 1.3), warm=12.0)
+    N = c['Ts'].size
+    moist = opts.pop('moist', 1.0)
+    cloudy = opts.get('cloud', 0) > 0
+    kw = {}
+    if cloudy:
+        c['cw'] = np.full_like(c['q'], 1e-4)
+        kw['cw'] = c['cw']
+    p = par3(moist=moist, bl_mix=1.0, **dict(opts, tau_conv=1e12) if cloudy else opts)
+    dt = 900.0
+    if land:
+        o = lrun(c, p, dt=dt, mix=mix, **kw)
+    elif moist:
+        o = run(c, p, q=c['q'], ice=c['ice'], dt=dt, mix=mix, **kw)
+    else:
+        o = run(c, p, dt=dt, mix=mix)
+    stored_matches(o, dt)
+    mu1 = new_masses(c, o['wx'])
+    np.testing.assert_allclose(mu1.sum(0), c['mu'].sum(0), rtol=1e-14)
+    assert np.all(mu1 >= 10.0 * (1 - 1e-9))
+    assert np.abs(o['wx'][-1]).max() > 0
+    if moist:
+        W0 = (c['mu'] * (c['q'] + (c['cw'] if cloudy else 0))).sum(0)
+        W1 = (mu1 * (o['q'] + (o['cw'] if cloudy else 0))).sum(0)
+        sea = c['land'] < 0.5 if land else np.ones(N, bool)
+        np.testing.assert_allclose((W1 - W0)[sea], ((o['evap'] - o['rain']) * dt)[sea], atol=1e-11 * W0.max())
+
+
+def test_mixed_layer_night():
+    """A mixed layer over a surface colder than its air, with no radiation and nothing else heating (so nothing
+    crosses an interface), gives its top back to the layer above down to the night layer mu_night when the surface's
+    buoyancy flux asks for more: the mass crossing the lowest interface is exactly mu_bot - mu_night."""
+    c = aqua()
+    N = c['Ts'].size
+    c['theta'][1] = np.minimum(c['theta'][1], 360.0)          # (the middle layer below its top edge's 380 K)
+    c['Ts'] = c['theta'][2] * (1.0e5 / P0) ** KAPPA - 30.0     # 30 K colder than the air above it
+    mu_night = float(np.median(c['mu'][2])) - 150.0
+    p = par3(bl_mix=1.0, mu_night=mu_night, tau0=0.0, eps_s=0.0, h_sens=200.0)
+    o = run(c, p, mode=1, insol=np.zeros(N), dt=3600.0)
+    capped = c['mu'][2] - mu_night
+    big = capped > 0
+    assert o['wx'][0].max() == 0 and big.sum() > 10
+    wx = o['wx'][1]
+    # where the flux's own ask exceeded the room, the layer ends at mu_night exactly
+    ended = np.isclose(wx, capped, rtol=1e-14)
+    assert ended[big].sum() > 0.3 * big.sum()
+    assert np.all(wx[big] <= capped[big] * (1 + 1e-14))
+
+
+def test_mixed_layer_condensation_level():
+    """bl_lcl: the mixed layer ends at its condensation level. Its air lifted dry-adiabatically saturates at the
+    pressure p_l where q_s(theta (p_l / p0)^kappa, p_l) = q (Tetens over water); the layer ends holding the mass
+    (p_s - p_l) / g (bisection in numpy against the oracle's, to 1e-6 kg/m^2) where it held more."""
+    c = moist_aqua(rh=(0.1, 0.5, 0.97), warm=6.0)
+    N = c['Ts'].size
+    p = par3(moist=1.0, bl_mix=1.0, bl_lcl=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, k_evap=0.0, mu_night=0.0,
+             tau_conv=1e30)                                 # (the plumes, after the mixed layer's steps, held off)
+    o = run(c, p, mode=1, insol=np.zeros(N), dt=900.0, q=c['q'], ice=np.zeros(N))
+    mu1 = new_masses(c, o['wx'])
+    ps = GRAV * c['mu'].sum(0)
+    th, q = o['theta'][2], o['q'][2]
+    sat = lambda pp: tetens(th * (pp / P0) ** KAPPA, pp) <= q
+    lo, hi = 0.1 * ps, ps.copy()
+    for _ in range(60):
+        m = 0.5 * (lo + hi)
+        s = sat(m)
+        lo, hi = np.where(s, m, lo), np.where(s, hi, m)
+    lcl = (ps - 0.5 * (lo + hi)) / GRAV
+    left = c['mu'][2] > lcl + 1.0
+    assert left.sum() > 10
+    np.testing.assert_allclose(mu1[2][left], lcl[left], atol=1e-6)
