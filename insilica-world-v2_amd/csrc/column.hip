@@ -32,6 +32,7 @@ static inline dim3 grid1(long n) { return dim3((unsigned)((n + BS - 1) / BS)); }
 struct CP {
   double grav, cp, kappa, p0, floor_, theta_star, Lv, moist, mu_bl, bl_mix, bl_cap, rho_ice, ice_cover, bulk_gust,
       q_sea, z0_land;
+  double bl_room;                                      // (the mixed layer's room up to its condensation level)
   double ths[ISW_LMAX];
 };
 
@@ -40,6 +41,7 @@ static CP make_cp(const double* a) {
   p.grav = a[13]; p.cp = a[14]; p.kappa = a[15]; p.p0 = a[16]; p.floor_ = a[11]; p.theta_star = a[10]; p.Lv = a[23];
   p.moist = a[22]; p.mu_bl = a[9]; p.bl_mix = a[70]; p.bl_cap = a[73]; p.rho_ice = a[29]; p.ice_cover = a[30];
   p.bulk_gust = a[75]; p.q_sea = a[97]; p.z0_land = a[91];
+  p.bl_room = 1e30;
   for (int j = 0; j < ISW_LMAX; j++) p.ths[j] = a[132 + j];
   return p;
 }
@@ -229,7 +231,7 @@ __device__ void crossing(const CP& P, double* mu, double* th, double* q, const d
       if (th[b] >= ts_) {
         double fa, fb;
         dens(a, fa, fb);
-        dm = -fmin(fmin(mu[a] * dth[b] * fa, fmax(mu[a] - P.mu_bl, 0.0) * P.bl_cap), 1e30);
+        dm = -fmin(fmin(mu[a] * dth[b] * fa, fmax(mu[a] - P.mu_bl, 0.0) * P.bl_cap), P.bl_room);
         thd = ts_ - 0.5 * dth[b];
       }
     } else if (blm && b < L - 1 && th[b] >= ts_) {
@@ -505,6 +507,8 @@ struct AP {
   // the land's
   double alb_dry, alb_wet, soil_beta, k_inf, pond_cover, alb_snow, alb_snow_melt, alb_glacier, snow_cover, land_exch,
       c_dry, c_wet, land_rs, c_deep, rs_max, rs_sun, revap_rhl, bare_fx, k_firn;
+  // the mixed boundary layer's
+  double mu_night, bl_ent, bl_reach, bl_lcl, ent_lcl, vf;
 };
 
 static AP make_ap(const double* a, const isw_air* air) {
@@ -525,6 +529,7 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.alb_snow_melt = a[42]; p.alb_glacier = a[43]; p.snow_cover = a[44]; p.land_exch = a[46]; p.c_dry = a[49];
   p.c_wet = a[50]; p.land_rs = a[52]; p.c_deep = a[94]; p.rs_max = a[95]; p.rs_sun = a[96]; p.revap_rhl = a[105];
   p.bare_fx = a[115]; p.k_firn = a[121];
+  p.mu_night = a[71]; p.bl_ent = a[72]; p.bl_reach = a[98]; p.bl_lcl = a[100]; p.ent_lcl = a[130]; p.vf = air->vf;
   return p;
 }
 
@@ -673,6 +678,29 @@ __device__ __forceinline__ double cover_of(double cw, double mu, double liq, con
   return fmin(1.0, cw / (s * (liq * A.q_auto + (1 - liq) * A.l_ice)) * fmax(1.0, mu / A.mu_cloud));
 }
 
+// The mixed bottom layer's mass up to the level where its air (theta th, vapor q), lifted dry-adiabatically,
+// saturates (air.cu:1413-1421): 40 bisections in pressure; 0 if saturated at the ground, 1e30 if not even at a tenth of
+// the surface pressure.
+template <int L, bool MIX>
+__device__ double lcl_mass(const double* mu, double th, double q, const CP& P, const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  double ps_ = 0;
+#pragma unroll
+  for (int k = 0; k < L; k++) ps_ += P.grav * mu[k];
+  auto sat = [&](double p_) {
+    return (double)qsat_a<F>((F)(th * fpow_<MIX>(p_ / P.p0, P.kappa)), (F)p_, (F)A.eps, A.ice_air) <= q;
+  };
+  if (sat(ps_)) return 0;
+  if (!sat(0.1 * ps_)) return 1e30;
+  double lo = 0.1 * ps_, hi = ps_;
+  for (int it = 0; it < 40; it++) {
+    const double m_ = 0.5 * (lo + hi);
+    if (sat(m_)) lo = m_;
+    else hi = m_;
+  }
+  return (ps_ - 0.5 * (lo + hi)) / P.grav;
+}
+
 struct AirIO {
   double *theta, *qv, *Ts, *ice;
   double *wx, *olr, *sw_abs, *stored, *rain, *evap, *conv, *tsurf, *rainp, *qocean, *qsw, *flash, *swalb, *firn;
@@ -693,6 +721,9 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   const int i = XTID;
   if (i >= N) return;
   if (elev && A.theta_rise > 0 && elev[i] > 0) P.theta_star += A.theta_rise * elev[i];
+  const double ts_base = P.theta_star;
+  P.bl_cap = omexp<MIX>(dt / 86400.0);                 // (the mixed layer takes in at most a day's share per step)
+  const bool blm = P.bl_mix > 0;
   const int top = 0, bot = L - 1;
   double mu[L], th[L], q[L], pb[L], pib[L], Q[L], T[L], eps[L];
   double mt = 0;
@@ -706,6 +737,8 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   }
   layer_exner<L>(mu, P, pb, pib);
   const double exb = pow(pb[bot] / P.p0, P.kappa);
+  // the mixed bottom layer: the lowest interface's theta* no colder than the layer, below the air above's mean
+  if (blm && L > 2) P.theta_star = fmin(fmax(ts_base, th[L - 1]), th[L - 2] - 1);
   double m_ice = MOIST ? o.ice[i] : 0.0;
   const bool isl = MOIST && o.land && o.land[i] > 0.5;   // land: the soil bucket, ponds, snow and the ground's heat
   double* Ts = o.Ts;
@@ -743,7 +776,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   // layers above give their air at their lower edge's theta* where warmer
   double wbl[L], thb = 0, qb = 0, mb = 0;
   {
-    double need = A.bl_bot > 0 && mu[bot] < P.mu_bl ? fmax(mu[bot], A.bl_bot * P.mu_bl) : P.mu_bl;
+    double need = blm ? mu[bot] : A.bl_bot > 0 && mu[bot] < P.mu_bl ? fmax(mu[bot], A.bl_bot * P.mu_bl) : P.mu_bl;
 #pragma unroll
     for (int k = bot; k >= 0; k--) {
       const double m = mu[k] < need ? mu[k] : need;
@@ -1242,7 +1275,8 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double pibt = P.cp * fpow_<MIX>((pbk - 0.5 * fb * dp) / P.p0, P.kappa);
       const double mean = qrad[k] * dt / (mu[k] * pib[k]);
       double dtop = dth[k] - mean + qet[k] * dt / (ft * mu[k] * pit);
-      const double dbot = dth[k] - mean + qeb[k] * dt / (fb * mu[k] * pibt);
+      double dbot = dth[k] - mean + qeb[k] * dt / (fb * mu[k] * pibt);
+      if (k == L - 1 && blm) dtop = dbot = dth[k];    // (the mixed bottom layer heats evenly)
       if (de && k == (int)A.edge_k) dtop -= dlat[k];
       if (edges) {
         et[k] = dtop;
@@ -1253,7 +1287,90 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       pt_ = pbk;
     }
   }
+  if (blm && L > 2) P.theta_star = fmin(fmax(ts_base, th[bot]), th[bot - 1] - 1);
+  double mu_lcl = 1e30;
+  if (A.bl_lcl > 0 && blm && L > 2) {                  // (bl_lcl: the mixed layer ends at its condensation level)
+    mu_lcl = lcl_mass<L, MIX>(mu, th[bot], q[bot], P, A);
+    P.bl_room = fmax(0.0, mu_lcl - mu[bot]);
+  }
   crossing<L, MOIST, CLOUD>(P, mu, th, q, dth, et, eb, pib, wxo, cl);
+  // the mixed boundary layer: the surface's buoyancy flux sets its depth. Rising it entrains the air above at bl_ent
+  // of that flux across the jump to it (Tennekes 1973), a slice from the layer's bottom edge; falling, it gives its
+  // top back down to a thin night layer mu_night. bl_reach: dry convection first takes in the air above colder than
+  // it, layer by layer while they empty. bl_lcl: it ends at its condensation level, the air above that leaving into
+  // the layer above.
+  if (blm && L > 2) {
+    auto reach_edge = [&](int k) { return k < bot - 1 ? fmax(tstar<L>(k, P), th[bot]) : tstar<L>(k, P); };
+    int ka = bot - 1;
+    if (A.bl_reach > 0) {
+      bool stop = false;
+#pragma unroll
+      for (int k = bot - 1; k >= 1; k--) {
+        if (stop) continue;
+        const double avail = mu[k] - P.floor_;
+        if (avail > 0) {
+          const double e = reach_edge(k), D = th[k] - e;
+          const bool spread = D > 0 && th[k] < tstar<L>(k - 1, P);
+          const double x = spread ? (th[bot] > e ? mu[bot] / mu[k] * (sqrt(1 + mu[k] * (th[bot] - e) / (mu[bot] * D)) - 1)
+                                                 : 0.0)
+                                  : (th[k] < th[bot] ? 1.0 : 0.0);
+          const double want = fmin(1.0, x) * mu[k];
+          const double m = fmin(fmin(fmin(want, avail), P.bl_cap * fmax(mu[k] - P.mu_bl, P.mu_bl)),
+                                fmax(0.0, mu_lcl - mu[bot]));
+          if (m > 0) {
+            const double fm = m / mu[k], thl = spread ? e + fm * D : th[k];
+            const double qx = q[k];
+            move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, k, bot, m, thl, thl, qx, qx, cl);
+#pragma unroll
+            for (int j = k; j < bot; j++) wxo[j] -= m;
+          }
+          if (!(m >= avail * (1 - 1e-9))) stop = true;
+        }
+      }
+      ka = bot - 1;
+#pragma unroll
+      for (int k = bot - 1; k >= 2; k--)                // (the lowest layer above with air, not below layer 1)
+        if (ka == k && !(mu[k] > P.floor_ * 1.000001)) ka = k - 1;
+    }
+    double mka = 0, thka = 0, qka = 0, eka = 0, tska = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++)
+      if (k == ka) {
+        mka = mu[k];
+        thka = th[k];
+        if constexpr (MOIST) qka = q[k];
+        eka = reach_edge(k);
+        tska = k >= 1 ? tstar<L>(k - 1, P) : 0.0;
+      }
+    const double Bf = H + A.vf * P.cp * Tsa * E, jump = fmax(1.0, eka - th[bot]);
+    if (Bf > 0) {
+      const double m = fmin(fmin(A.bl_ent * Bf * dt / (P.cp * jump), mka - P.floor_),
+                            A.ent_lcl > 0 ? 1e30 : fmax(0.0, mu_lcl - mu[bot]));
+      if (m > 0) {
+        const bool spread = ka >= 1 && thka < tska && thka > eka;
+        const double fm = fmin(1.0, m / mka);
+        const double thc = spread ? eka + fm * (thka - eka) : thka;
+        move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, ka, bot, m, thc, thc, qka, qka, cl);
+#pragma unroll
+        for (int j = 0; j < bot; j++)
+          if (j >= ka) wxo[j] -= m;
+      }
+    }
+    if (!(Bf > 0) && mu[bot] > A.mu_night) {
+      const double m = fmin(-Bf * dt / (P.cp * jump), mu[bot] - A.mu_night);
+      const double qx = q[bot];
+      move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cl);
+      wxo[bot - 1] += m;
+    }
+    if (A.bl_lcl > 0) {
+      const double m = mu[bot] - fmax(lcl_mass<L, MIX>(mu, th[bot], q[bot], P, A), A.mu_night);
+      if (m > 0) {
+        const double qx = q[bot];
+        move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cl);
+        wxo[bot - 1] += m;
+      }
+    }
+  }
   // ---- moist convection: a parcel of bottom air lifted to the interface above it; buoyant there, it rises through
   // every interface it stays buoyant at and detrains above the last, its condensate raining; the mass that would
   // remove the buoyancy at the first interface relaxes over tau_conv ----
@@ -1422,6 +1539,29 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     o.rainp[(size_t)N + i] = Rk1 / dt;
     o.rainp[(size_t)2 * N + i] = Rc / dt;
     o.rainp[(size_t)3 * N + i] = S / dt;
+  }
+  // a layer emptied to its floor keeps a theta within its edges: its heat beyond them goes to the nearest layer with
+  // mass (below first), the column's energy unchanged
+  if (blm) {
+#pragma unroll
+    for (int k = 1; k < bot; k++) {
+      if (mu[k] > 2 * P.floor_) continue;
+      const double lo = tstar<L>(k, P), hi = tstar<L>(k - 1, P), tn = fmin(fmax(th[k], lo), hi);
+      if (tn == th[k]) continue;
+      int n = -1;
+#pragma unroll
+      for (int j = k + 1; j < L; j++)
+        if (n < 0 && mu[j] > 2 * P.floor_) n = j;
+#pragma unroll
+      for (int j = k - 1; j >= 0; j--)
+        if (n < 0 && mu[j] > 2 * P.floor_) n = j;
+      if (n < 0) continue;
+      const double moved = mu[k] * pib[k] * (th[k] - tn);
+#pragma unroll
+      for (int j = 0; j < L; j++)
+        if (j == n) th[j] += moved / (mu[j] * pib[j]);
+      th[k] = tn;
+    }
   }
   const double E1 = column_energy<L, MOIST>(mu, pib, th, q, P) + Cw * Ts[i] - A.Lf * m_ice + Cf * (Tfn - A.Tf);
   if (Cf > 0) o.firn[i] = Tfn;

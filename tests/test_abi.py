@@ -79,10 +79,9 @@ def test_bad_arguments_rejected_on_host():
         # cloudo; elev, flash, radf, swf, swalb, ch, lc, firn; xfer, prec, stream
         return lib.isw_air_column(10, L, par, a, mode, t, insol, buf, buf, buf, buf, None, buf, None, dt,
                                   *([buf] * 8), None, dgo, *([None] * 10), *([None] * 8), xfer, prec, None)
-    for k, v in ((70, 1.0), (111, 1.0)):                     # bl_mix, cu_dth (with L > 3)
-        par[k] = v
-        assert air_column(L=4) == E
-        par[k] = 0.0
+    par[111] = 1.0                                           # cu_dth (with L > 3)
+    assert air_column(L=4) == E
+    par[111] = 0.0
     par[22] = 1.0                                            # moist air: no vapor or ice given; the moist options
     assert air_column() == E
     for k in (47, 88, 83, 51, 53, 86, 85):                   # plume_deep, plume_root, q_shape, sat_prof, sat_base,

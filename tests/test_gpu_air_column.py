@@ -90,21 +90,36 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
     return {k: _np(v) for k, v in o.items()}
 
 
-def compare(g, o, tol, dt, label, bars=None):
+def cell_errors(g, o):
+    """Per column: the largest |difference| over the field's rows, relative to the field's largest |value|."""
+    d = np.abs(g - o).reshape(-1, g.shape[-1]).max(0)
+    return d / max(np.abs(o).max(), 1e-300)
+
+
+def compare(g, o, tol, dt, label, bars=None, outliers=0.0):
+    """outliers: the share of columns allowed past the bars (columns on a knife edge of the reference's own
+    design, named by the caller), every other column within them."""
     keys = [k for k in ('theta', 'Ts', 'wx', 'olr', 'sw_abs', 'tsurf', 'qocean', 'qsw', 'swalb', 'firn', 'q', 'ice',
                         'cw', 'cloudo', 'soil', 'pond', 'snow', 'cap') if k in o] + \
         (['rain', 'evap', 'conv', 'rainp'] if 'q' in o else [])
-    e = {k: relmax(g[k], o[k]) for k in keys}
-    e['stored'] = relmax(g['stored'], o['stored'])
-    print(label, {k: float(f'{v:.3g}') for k, v in e.items()}, 'moved', int(np.count_nonzero(o['wx'])), flush=True)
+    keys.append('stored')
+    ce = {k: cell_errors(g[k], o[k]) for k in keys}
+    bar = {'conv': 1e-10, } if tol < 1e-9 else {'conv': 1e-3, 'wx': 1e-4, 'rain': 1e-4, 'rainp': 1e-4}
+    bar['stored'] = 1e-9
+    bar.update(bars or {})
+    past = np.zeros(o['stored'].size, bool)
+    for k in keys:
+        past |= ce[k] > max(tol, bar.get(k, 0.0))
+    keep = ~past if outliers > 0 else np.ones_like(past)
+    e = {k: float(ce[k][keep].max()) for k in keys}
+    print(label, {k: float(f'{v:.3g}') for k, v in e.items()}, 'moved', int(np.count_nonzero(o['wx'])),
+          'columns past the bars', int(past.sum()), flush=True)
     # the plume's mass divides by the parcel's buoyancy slope, a difference over 500 Pa of two lifted parcels' theta:
     # a ~600x cancellation of one-ulp pow/exp differences (device against glibc): 1e-13..1e-12 in FP64, and in the
     # mixed form's float32 parcels ~1e-4; the masses it moves and its rain carry a share of that
-    bar = {'conv': 1e-10, } if tol < 1e-9 else {'conv': 1e-3, 'wx': 1e-4, 'rain': 1e-4, 'rainp': 1e-4}
-    bar.update(bars or {})
+    assert past.sum() <= outliers * past.size
     for k in keys:
         assert e[k] <= max(tol, bar.get(k, 0.0)), k
-    assert e['stored'] <= max(tol, 1e-9)
     np.testing.assert_allclose(g['stored'], (g['sw_abs'] - g['olr']) * dt, rtol=0, atol=2e-3)
     zero = ('flash',) if 'q' in o else ('rain', 'evap', 'conv', 'rainp', 'flash')
     for k in zero:
@@ -323,6 +338,55 @@ def test_land(L, mixed, opts):
     assert np.abs(o['soil'] - kw['soil']).max() > 0 and np.abs(o['snow'] - kw['snow']).max() > 0
     compare(g, o, 1e-5 if mixed else 1e-12, dt, f'land L={L} mixed={mixed}',
             bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed and cloudy else None)
+
+
+@pytest.mark.parametrize('L,mixed,opts', [
+    (3, False, dict(moist=0.0, bl_reach=1.0)),
+    (3, False, dict(bl_lcl=1.0, bl_reach=1.0)),
+    (5, False, dict(bl_lcl=1.0, ent_lcl=1.0, mu_night=2000.0, cloud=1.0, land=1)),
+    (4, False, dict(bl_reach=1.0, ktables=1)),
+    (3, True, dict(bl_lcl=1.0)),
+    (7, True, dict(bl_lcl=1.0, bl_reach=1.0, cloud=1.0, ice_air=1.0))])
+def test_mixed_layer(L, mixed, opts):
+    """The mixed boundary layer (bl_mix) in the dry, moist, cloudy and land columns and under the k-tables' edge
+    heating, to the moist column's bars (the mixed form with clouds as the cloudy column's). A third of the columns
+    have the layer above the mixed one colder than it (dry convection, bl_reach)."""
+    from oracle import column as ocol
+    opts = dict(opts)
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column mixed layer L={L} mixed={mixed} {opts} N={N}', flush=True)
+    moist = opts.pop('moist', 1.0) > 0
+    land, kt = opts.pop('land', 0), opts.pop('ktables', 0)
+    cloudy = opts.get('cloud', 0) > 0
+    c = cloudy_columns(N, L, seed=110 + L) if cloudy else moist_columns(N, L, seed=110 + L, polar=30.0)
+    rng = np.random.default_rng(120 + L)
+    c['theta'][-2] -= np.where(rng.uniform(size=N) < 1 / 3, 25.0, 0.0)
+    kw = dict(ocean=True)
+    if moist:
+        kw.update(q=c['q'], ice=c['ice'])
+    if cloudy:
+        kw.update(cw=c['cw'], cloudo=True)
+    if land:
+        kw.update(land_fields(c, seed=125 + L))
+    p = par(L, moist=1.0 if moist else 0.0, bl_mix=1.0, **opts)
+    if kt:
+        from oracle import radiation as orad
+        from insilica.ktables import synthetic_lw, synthetic_sw
+        p = par(L, moist=1.0, bl_mix=1.0, edge_k=2.0, edge_heat=1.0, **opts)
+        lwt, swt = orad.LwTables(synthetic_lw()), orad.SwTables(synthetic_sw())
+        sub = orad.lw_setup(N, L, p, lwt, c['mu'], c['theta'], c['q'])
+        kw['radf'] = orad.lw_column(N, L, p, lwt, sub, c['Ts'])
+        kw['swf'] = orad.sw_column(N, L, p, swt, sub, np.full(N, 0.06), mode=1)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, **kw)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed, **kw)
+    assert np.abs(o['wx'][-1]).max() > 0
+    # bl_lcl ends the mixed layer where its air, lifted, saturates: the parcel the plume lifts to the layer's top is
+    # then saturated to the last bits, and one-ulp device-against-glibc exp/pow differences decide whether it
+    # condenses there, which switches the plume's closure (measured: 23-34 of 40962 columns in FP64, 400 in the mixed
+    # form, whose float32 saturation test is that much wider a knife edge); every other column is held to the bars
+    compare(g, o, (1e-5 if moist else 1e-6) if mixed else 1e-12, dt, f'mixed layer L={L} mixed={mixed}',
+            bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed and cloudy else None,
+            outliers=(2e-2 if mixed else 2e-3) if opts.get('bl_lcl', 0) > 0 and moist else 0.0)
 
 
 def test_aqua_dry_world():
