@@ -15,7 +15,9 @@
 
 enum { P_GRAV = 13, P_CP = 14, P_KAPPA = 15, P_P0 = 16, P_FLOOR = 11, P_THETA_STAR = 10, P_LV = 23, P_MOIST = 22,
        P_MU_BL = 9, P_BL_MIX = 70, P_BL_CAP = 73,
-       P_RHO_ICE = 29, P_ICE_COVER = 30, P_BULK_GUST = 75, P_Q_SEA = 97, P_Z0_LAND = 91, P_THS = 132 };
+       P_RHO_ICE = 29, P_ICE_COVER = 30, P_BULK_GUST = 75, P_Q_SEA = 97, P_Z0_LAND = 91, P_THS = 132,
+       P_EDGE_VAP = 86, P_FREEZE_DRY = 85, P_EDGE_K = 101, P_ICE_AIR = 102, P_SAT_PROF = 51, P_SAT_BASE = 53,
+       P_BL_Q = 89, P_Q_SHAPE = 83, P_PLUME_ROOT = 88, P_ROOT_Q = 107 };
 
 static void layer_exner(int L, const double* mu, const double* par, double* pb, double* pib) {
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP];
@@ -122,8 +124,14 @@ static void move_mass(int L, const double* par, double* mu, double* th, double* 
  * interface j. Dry air (q = 0): the edge-vapor and freeze-drying options carry no vapor and move the same mass. The
  * mixed bottom layer (bl_mix) needs L >= 3 (its top edge is then an interface). dtop/dbot (optional): the heating at
  * each layer's top and bottom edge when it differs from the layer's mean (the k-tables' edge sub-levels). */
+static double edge_q(int L, int k, int top, const double* mu, const double* th, const double* q, double m,
+                     const double* par, double eps, int mix);
+static double qsat_x(double T, double p, double eps, double ice, int mix);
+static double fpow_(double x, double y, int mix);
+
 static void crossing(int L, const double* par, double* mu, double* th, double* q, const double* dth, double dth_up,
-                     double* pib, double* wxo, const double* dtop, const double* dbot, double* cl, double bl_room) {
+                     double* pib, double* wxo, const double* dtop, const double* dbot, double* cl, double bl_room,
+                     double eps, int mix) {
   const double floor_ = par[P_FLOOR];
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (par[P_THETA_STAR] <= 0) return;
@@ -195,10 +203,28 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
     const double maxup = mu[b] - floor_, maxdn = mu[a] - floor_;
     if (dm > maxup) dm = maxup > 0 ? maxup : 0.0;
     if (-dm > maxdn) dm = maxdn > 0 ? -maxdn : 0.0;
-    if (dm > 0)
-      move_mass(L, par, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], q[b], q[b], cl);
-    else if (dm < 0)
-      move_mass(L, par, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], q[a], q[a], cl);
+    /* edge vapor: the crossing air carries the vapor at the edge it leaves through; freeze-drying: air rising into the
+     * stratosphere past the cold point at theta* holds no more than saturation there, the rest staying below */
+    const int ev = par[P_EDGE_VAP] > 0;
+    if (dm > 0 && a == (int)par[P_EDGE_K] - 1 && L >= 3 && par[P_FREEZE_DRY] > 0) {
+      double pi_ = 0;
+      for (int k = 0; k <= a; k++) pi_ += par[P_GRAV] * mu[k];
+      const double Ti = tstar(a, L, par) * fpow_(pi_ / par[P_P0], par[P_KAPPA], mix);
+      const double qx = fmin(ev ? edge_q(L, b, 1, mu, th, q, dm, par, eps, mix) : q[b],
+                             Ti < 160 ? 0.0 : qsat_x(Ti, pi_, eps, par[P_ICE_AIR], mix));
+      const double ma = mu[a], mb = mu[b], ca = cl ? cl[a] : 0.0, cb = cl ? cl[b] : 0.0;
+      move_mass(L, par, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qx, qx, cl);
+      if (cl) {
+        cl[a] = ma * ca / mu[a];
+        cl[b] = mb * cb / mu[b];
+      }
+    } else if (dm > 0) {
+      const double qx = ev ? edge_q(L, b, 1, mu, th, q, dm, par, eps, mix) : q[b];
+      move_mass(L, par, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qx, qx, cl);
+    } else if (dm < 0) {
+      const double qx = ev ? edge_q(L, a, 0, mu, th, q, -dm, par, eps, mix) : q[a];
+      move_mass(L, par, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], qx, qx, cl);
+    }
     wxo[j] = dm;
   }
   const double tiny = floor_ * 1.000001;
@@ -258,7 +284,7 @@ void og_hs_column(int N, int L, const double* par, const double* hs, const doubl
       const double ki = hs[0] + (hs[1] - hs[0]) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
       dth_up = (thi - par[P_THETA_STAR]) * (1 - exp(-ki * dt));
     }
-    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL, NULL, 1e30);
+    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL, NULL, 1e30, 0.622, 0);   /* (dry: no vapor moves) */
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
     double E1 = 0;
     for (int k = 0; k < L; k++) {
@@ -376,7 +402,7 @@ enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5,
        P_SIGMA = 12, P_TILT = 17, P_DAY = 19, P_TAU_LIN = 20, P_LF = 24, P_TF = 25, P_K_EVAP = 26, P_TAU_CONV = 27,
        P_ALB_ICE = 28, P_K_ICE = 31, P_RV = 32, P_SAT_LAYER = 33, P_SW_TAU = 34, P_ALB_ICE_MELT = 41, P_ALB_RAMP = 45,
        P_LW_W0 = 48, P_BL_BOT = 54, P_THETA_RISE = 69, P_BULK_C = 74, P_BULK_Z = 76, P_BULK_W = 77, P_EDGE_HEAT = 84,
-       P_SFC = 90, P_DRY_EDGE = 99, P_EDGE_K = 101, P_ICE_AIR = 102, P_FIRN = 120, P_RHO_FIRN = 122, P_C_ICE = 123,
+       P_SFC = 90, P_DRY_EDGE = 99, P_FIRN = 120, P_RHO_FIRN = 122, P_C_ICE = 123,
        P_CLOUD = 55, P_RH_CRIT = 56, P_RE_LIQ = 57, P_RE_ICE = 58, P_CLOUD_G = 59, P_K_LW_LIQ = 60, P_K_LW_ICE = 61,
        P_C_AUTO = 62, P_Q_AUTO = 63, P_V_ICE = 64, P_TAU_ERODE = 66, P_K_REVAP = 67, P_MU_CLOUD = 68, P_L_ICE = 79,
        P_CONV_REVAP = 87, P_DOWNDRAFT = 92, P_REVAP = 103, P_REVAP_RHS = 104, P_REVAP_AREA = 106, P_LIQ_COLD = 128,
@@ -387,6 +413,20 @@ enum { P_S0 = 0, P_ASW = 1, P_TAU0 = 2, P_MU_REF = 3, P_EPS_S = 4, P_H_SENS = 5,
 
 static double omexp(double x, int mix) { return mix ? (double)-expm1f(-(float)x) : 1 - exp(-(x)); }
 static double fpow_(double x, double y, int mix) { return mix ? (double)powf((float)x, (float)y) : pow(x, y); }
+
+/* The mass share m's theta in a layer of shape s as a fraction of the way between its edges, ln(1 + m (e^s - 1)) / s
+ * (air.cu:233-244; es = e^s - 1). The mixed build in float32 with asymptotes beyond |s| = 30; the shape s itself
+ * comes from the FP64 edge_shape in both forms. */
+static double spread_of(double m, double s, double es, int mix) {
+  if (mix) {
+    const float mf = (float)m, sf = (float)s;
+    if (fabsf(sf) < 1e-6f) return mf;
+    if (sf > 30.f) return 1.f + logf(mf) / sf;
+    if (sf < -30.f) return log1pf(-mf) / sf;
+    return log1pf(mf * expm1f(sf)) / sf;
+  }
+  return fabs(s) < 1e-6 ? m : log1p(m * es) / s;
+}
 
 /* tau_p (air.cu:853-856): the gray optical depth down to pressure p of a column with surface pressure ps. */
 static double tau_p(double p, double ps, const double* par) {
@@ -468,13 +508,62 @@ static double condense_layer(double* th, double* q, double pib, double pt, doubl
   return dq;
 }
 
-/* condense_k (air.cu:319-331) with sat_prof = sat_base = 0: the layer's capacity (sat_layer), else v1's rule at the
- * pressure whose Exner function is the layer's. */
-static double condense_k(int k, const double* mu, double* th, double* q, const double* pib, const double* par,
+/* condense_spread (air.cu:292-315): a middle layer's saturation adjustment with its theta spread between its edges a
+ * and b as edge_shape has it, the whole spread shifting with the mean; one Newton step. */
+static double condense_spread(double a, double b, double* th, double* q, double pib, double pt, double pb,
+                              const double* par, double eps, int mix) {
+  static const double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  static const double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  const double Lv = par[P_LV], Rv = par[P_RV], ice = par[P_ICE_AIR], p0 = par[P_P0], kappa = par[P_KAPPA];
+  const double s = edge_shape(a, b, *th), es = mix ? 0.0 : expm1(s);
+  double dq;
+  if (mix) {
+    float qs = 0, dqs = 0;
+    for (int j = 0; j < 4; j++) {
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j], m = (pb - p) / (pb - pt);
+      const double tn = a + (b - a) * spread_of(m, s, es, 1);
+      const float ex = powf((float)(p / p0), (float)kappa), T = (float)tn * ex;
+      if (T < 160) continue;
+      const float v = (float)qsat_x(T, (float)p, eps, ice, 1);
+      qs += 0.5f * (float)wq[j] * v;
+      dqs += 0.5f * (float)wq[j] * v * (float)Lv / ((float)Rv * T * T) * ex;
+    }
+    if (!(*q > qs)) return 0;
+    dq = (float)(*q - qs) / (1.0f + (float)Lv / (float)pib * dqs);
+  } else {
+    double qs = 0, dqs = 0;
+    for (int j = 0; j < 4; j++) {
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j], m = (pb - p) / (pb - pt);
+      const double tn = a + (b - a) * spread_of(m, s, es, 0);
+      const double ex = pow(p / p0, kappa), T = tn * ex;
+      if (T < 160) continue;
+      const double v = qsat_x(T, p, eps, ice, 0);
+      qs += 0.5 * wq[j] * v;
+      dqs += 0.5 * wq[j] * v * Lv / (Rv * T * T) * ex;
+    }
+    if (!(*q > qs)) return 0;
+    dq = (*q - qs) / (1 + Lv / pib * dqs);
+  }
+  *q -= dq;
+  *th += Lv * dq / pib;
+  return dq;
+}
+
+/* condense_k (air.cu:319-331): the layer's capacity (sat_layer; sat_prof: a middle layer below edge_k's tropopause
+ * with its theta spread between its edges; sat_base: the third layer at its base), else v1's rule at the pressure
+ * whose Exner function is the layer's. */
+static double condense_k(int L, int k, const double* mu, double* th, double* q, const double* pib, const double* par,
                          double eps, int mix) {
   if (par[P_SAT_LAYER] > 0) {
     double pt = 0;
     for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
+    if (par[P_SAT_PROF] > 0 && k >= (int)par[P_EDGE_K] && k < L - 1 && th[k] < tstar(k - 1, L, par))
+      return condense_spread(tstar(k, L, par), tstar(k - 1, L, par), &th[k], &q[k], pib[k], pt,
+                             pt + par[P_GRAV] * mu[k], par, eps, mix);
+    if (par[P_SAT_BASE] > 0 && k == 2) {
+      const double pb = pt + par[P_GRAV] * mu[k];
+      return condense(&th[k], &q[k], par[P_CP] * fpow_(pb / par[P_P0], par[P_KAPPA], mix), pb, par, eps, mix);
+    }
     return condense_layer(&th[k], &q[k], pib[k], pt, pt + par[P_GRAV] * mu[k], par, eps, mix);
   }
   const double p = fpow_(pib[k] / par[P_CP], 1 / par[P_KAPPA], mix) * par[P_P0];
@@ -491,7 +580,7 @@ static void parcel(double th, double q, double p, const double* par, double eps,
 
 /* layer_cap (air.cu:334-359, the layer's theta uniform): the layer's vapor capacity over its pressure span and its
  * change with theta, as condense_layer takes them. */
-static void layer_cap(int k, const double* mu, double th, const double* par, double eps, int mix, double* qs,
+static void layer_cap(int L, int k, const double* mu, double th, const double* par, double eps, int mix, double* qs,
                       double* dqs) {
   static const double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
   static const double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
@@ -499,11 +588,16 @@ static void layer_cap(int k, const double* mu, double th, const double* par, dou
   double pt = 0;
   for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
   const double pb = pt + par[P_GRAV] * mu[k];
+  /* (sat_prof: a middle layer below the tropopause with its theta spread between its edges) */
+  const int spread = par[P_SAT_PROF] > 0 && k >= (int)par[P_EDGE_K] && k < L - 1 && th < tstar(k - 1, L, par);
+  const double ea = spread ? tstar(k, L, par) : 0.0, eb = spread ? tstar(k - 1, L, par) : 0.0;
+  const double s = spread ? edge_shape(ea, eb, th) : 0.0, es = spread && !mix ? expm1(s) : 0.0;
   if (mix) {
     float qf = 0, dqf = 0;
     for (int j = 0; j < 4; j++) {
-      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
-      const float ex = powf((float)(p / p0), (float)kappa), T = (float)th * ex;
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j], m = (pb - p) / (pb - pt);
+      const double tn = !spread ? th : ea + (eb - ea) * spread_of(m, s, es, 1);
+      const float ex = powf((float)(p / p0), (float)kappa), T = (float)tn * ex;
       if (T < 160) continue;
       const float v = (float)qsat_x(T, (float)p, eps, ice, 1);
       qf += 0.5f * (float)wq[j] * v;
@@ -514,8 +608,9 @@ static void layer_cap(int k, const double* mu, double th, const double* par, dou
   } else {
     double qf = 0, dqf = 0;
     for (int j = 0; j < 4; j++) {
-      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
-      const double ex = pow(p / p0, kappa), T = th * ex;
+      const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j], m = (pb - p) / (pb - pt);
+      const double tn = !spread ? th : ea + (eb - ea) * spread_of(m, s, es, 0);
+      const double ex = pow(p / p0, kappa), T = tn * ex;
       if (T < 160) continue;
       const double v = qsat_x(T, p, eps, ice, 0);
       qf += 0.5 * wq[j] * v;
@@ -524,6 +619,28 @@ static void layer_cap(int k, const double* mu, double th, const double* par, dou
     *qs = qf;
     *dqs = dqf;
   }
+}
+
+/* edge_q (air.cu:361-378): the vapor of layer k's air at its top edge (top) or bottom edge, its relative humidity
+ * against the layer's capacity held uniform through the layer (a spread layer's edge at its theta*); layers above
+ * edge_k (with L >= 3) and, with bl_q, the mixed bottom layer keep their vapor even. Air of mass m leaving through
+ * that edge takes the edge's vapor plus m's share of the way to the mean, at most the layer's whole vapor. */
+static double edge_q(int L, int k, int top, const double* mu, const double* th, const double* q, double m,
+                     const double* par, double eps, int mix) {
+  if (!(q[k] > 0) || (k < (int)par[P_EDGE_K] && L >= 3) || (par[P_BL_Q] > 0 && k == L - 1 && k > 0)) return q[k];
+  double qs, dqs;
+  layer_cap(L, k, mu, th[k], par, eps, mix, &qs, &dqs);
+  if (!(qs > 0)) return q[k];
+  double pt = 0;
+  for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
+  const double p = top ? pt : pt + par[P_GRAV] * mu[k];
+  if (!(p > 0)) return q[k];
+  const int spread = par[P_SAT_PROF] > 0 && k >= (int)par[P_EDGE_K] && k < L - 1 && th[k] < tstar(k - 1, L, par);
+  const double te = spread ? (top ? tstar(k - 1, L, par) : tstar(k, L, par)) : th[k];
+  const double T = te * fpow_(p / par[P_P0], par[P_KAPPA], mix);
+  const double qe = T < 160 ? 0.0 : q[k] * qsat_x(T, p, eps, par[P_ICE_AIR], mix) / qs;
+  const double f = m > 0 ? fmin(1.0, m / mu[k]) : 0.0;
+  return m > 0 ? fmin(qe + (q[k] - qe) * f, mu[k] * q[k] / m) : qe;
 }
 
 /* cloud_pdf (air.cu:383-388): the water spread evenly over +-(1 - rh_crit) qs about its mean qt; its saturated share
@@ -550,10 +667,10 @@ static double cover_of(double cw, double mu, double liq, const double* par, doub
 
 /* cloud_adjust (air.cu:438-450): condense up to the statistics' condensate with the latent heat's feedback on the
  * capacity; a cloud holding more erodes over tau_erode. */
-static double cloud_adjust(int k, const double* mu, double* th, double* q, double* cw, const double* pib, double dt,
-                           const double* par, double eps, int mix) {
+static double cloud_adjust(int L, int k, const double* mu, double* th, double* q, double* cw, const double* pib,
+                           double dt, const double* par, double eps, int mix) {
   double qs, dqs, l, f;
-  layer_cap(k, mu, th[k], par, eps, mix, &qs, &dqs);
+  layer_cap(L, k, mu, th[k], par, eps, mix, &qs, &dqs);
   if (!(qs > 0)) return 0;
   cloud_pdf(q[k] + cw[k], qs, par, &l, &f);
   const double dl = l > cw[k] ? (l - cw[k]) / (1 + par[P_LV] / pib[k] * dqs * f)
@@ -650,7 +767,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         const double pc = k == bot ? 0.5 * (pb[k] + ptk) : ptk + 0.25 * (pb[k] - ptk);
         liq[k] = fmax(0.0, fmin(1.0, (th[k] * fpow_(pc / p0, kappa, mix) - Tf + par[P_LIQ_COLD]) / par[P_LIQ_COLD]));
         double qs, dqs, l, f;
-        layer_cap(k, mu, th[k], par, aeps, mix, &qs, &dqs);
+        layer_cap(L, k, mu, th[k], par, aeps, mix, &qs, &dqs);
         cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
         fc[k] = fmax(f, cover_of(cl[k], mu[k], liq[k], par, qs));
         cwp[k] = fc[k] > 1e-6 ? mu[k] * cl[k] / fc[k] : 0.0;
@@ -802,7 +919,15 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     /* the surface: open water at Ts, or under ice thicker than ice_cover a skin balancing conduction through the ice
      * against sunlight, longwave and sensible heat, capped at freezing */
     const double kex = rough * (bulk_c > 0 || sfc ? bulk : par[P_K_EVAP] * (1 + uabs[i] / par[P_U0]));
-    const double qa = qb;
+    /* q_shape: the air at the surface holds the bottom layer's vapor at its relative humidity there (its capacity
+     * layer_cap's), not the layer's mean; bl_q keeps the layer's vapor even */
+    double qa = qb;
+    if (moist && par[P_Q_SHAPE] > 0 && !(par[P_BL_Q] > 0)) {
+      double qsl, dqsl;
+      layer_cap(L, bot, mu, th[bot], par, aeps, mix, &qsl, &dqsl);
+      const double qss = qsat_x(Tsa, ps, aeps, 0.0, mix);
+      if (qsl > 0) qa = fmin(qb * qss / qsl, qss);
+    }
     /* soil and plants hold their water back: a surface resistance rs in series with the air's ra (land_rs; rs_max:
      * the stomata open with the sunlight, rs = land_rs (1 + f) / (land_rs / rs_max + f), f the sunlight reaching the
      * ground over rs_sun; lc rows 6, 7 the cover's own land_rs, rs_sun; its bare share, row 3's complement, has
@@ -1012,11 +1137,11 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       double Fr = 0, Fs = 0, fa = 0, psf = 0;
       for (int k = 0; k < L; k++) psf += grav * mu[k];
       for (int k = 0; k < L; k++) {
-        const double dl = cloud_adjust(k, mu, th, q, cl, pib, dt, par, aeps, mix);
+        const double dl = cloud_adjust(L, k, mu, th, q, cl, pib, dt, par, aeps, mix);
         dth[k] += Lv * dl / pib[k];
         dlat[k] += Lv * dl / pib[k];
         double qs, dqs, l, f;
-        layer_cap(k, mu, th[k], par, aeps, mix, &qs, &dqs);
+        layer_cap(L, k, mu, th[k], par, aeps, mix, &qs, &dqs);
         cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
         f = fmax(f, cover_of(cl[k], mu[k], liq[k], par, qs));
         const double inc = f > 1e-6 ? cl[k] / f * fmax(1.0, mu[k] / par[P_MU_CLOUD]) : 0.0, r = inc / par[P_Q_AUTO];
@@ -1067,7 +1192,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       S = fmax(0.0, fmin(Fs * dt, R));
     } else if (moist)
       for (int k = 0; k < L; k++) {
-        const double dq = condense_k(k, mu, th, q, pib, par, aeps, mix);
+        const double dq = condense_k(L, k, mu, th, q, pib, par, aeps, mix);
         dth[k] += Lv * dq / pib[k];
         dlat[k] += Lv * dq / pib[k];
         R += mu[k] * dq;
@@ -1105,7 +1230,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     }
     crossing(L, par, mu, th, q, dth, dth[top], pib, wxo,
              radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL, cloudy ? cl : NULL,
-             bl_room);
+             bl_room, aeps, mix);
     /* the mixed boundary layer: the surface's buoyancy flux sets its depth. Rising it entrains the air above at bl_ent
      * of that flux across the jump to it (Tennekes 1973), a slice from the layer's bottom edge; falling, it gives its
      * top back down to a thin night layer mu_night. bl_reach: dry convection first takes in the air above colder
@@ -1127,7 +1252,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
                                   fmax(0.0, mu_lcl - mu[bot]));
             if (m > 0) {
               const double fm = m / mu[k], thl = spread ? e + fm * D : th[k];
-              const double qx = q[k];
+              const double qx = par[P_EDGE_VAP] > 0 ? edge_q(L, k, 0, mu, th, q, m, par, aeps, mix) : q[k];
               move_mass(L, par, mu, th, q, pib, k, bot, m, thl, thl, qx, qx, cloudy ? cl : NULL);
               for (int j = k; j < bot; j++) wxo[j] -= m;
             }
@@ -1147,21 +1272,21 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           const int spread = ka >= 1 && th[ka] < tstar(ka - 1, L, par) && th[ka] > eka;
           const double fm = fmin(1.0, m / mu[ka]);
           const double thc = spread ? eka + fm * (th[ka] - eka) : th[ka];
-          const double qx = q[ka];
+          const double qx = par[P_EDGE_VAP] > 0 ? edge_q(L, ka, 0, mu, th, q, m, par, aeps, mix) : q[ka];
           move_mass(L, par, mu, th, q, pib, ka, bot, m, thc, thc, qx, qx, cloudy ? cl : NULL);
           for (int j = ka; j < bot; j++) wxo[j] -= m;
         }
       }
       if (!(Bf > 0) && mu[bot] > par[P_MU_NIGHT]) {
         const double m = fmin(-Bf * dt / (cp * jump), mu[bot] - par[P_MU_NIGHT]);
-        const double qx = q[bot];
+        const double qx = par[P_EDGE_VAP] > 0 ? edge_q(L, bot, 1, mu, th, q, m, par, aeps, mix) : q[bot];
         move_mass(L, par, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cloudy ? cl : NULL);
         wxo[bot - 1] += m;
       }
       if (par[P_BL_LCL] > 0) {
         const double m = mu[bot] - fmax(lcl_mass(L, mu, th, q, par, aeps, mix), par[P_MU_NIGHT]);
         if (m > 0) {
-          const double qx = q[bot];
+          const double qx = par[P_EDGE_VAP] > 0 ? edge_q(L, bot, 1, mu, th, q, m, par, aeps, mix) : q[bot];
           move_mass(L, par, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cloudy ? cl : NULL);
           wxo[bot - 1] += m;
         }
@@ -1178,9 +1303,28 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       for (int k = 0; k < bot; k++) pi_ += grav * mu[k];
       const double th0 = th[bot];
       double q0 = q[bot];
-      if (cloudy) {                                  /* plumes root in the cell's moist air: its moistest part */
+      if (par[P_PLUME_ROOT] > 0) {
+        /* plume_root: plumes root in the air at the surface, the layer's vapor at its relative humidity there (edge_q
+         * at its bottom edge); root_q: plus the thermals' excess b E / (rho w_m), b = 8.5, w_m = (u*^3 + 0.28
+         * w*^3)^(1/3) (Holtslag & Boville 1993), up to the surface air's saturation; else with clouds its moistest
+         * part by the statistics, against the surface air's capacity */
+        const double ps0 = pi_ + grav * mu[bot], Ts0 = th0 * fpow_(ps0 / p0, kappa, mix);
+        const double qss = Ts0 < 160 ? 0.0 : qsat_x(Ts0, ps0, aeps, par[P_ICE_AIR], mix);
+        q0 = edge_q(L, bot, 0, mu, th, q, 0, par, aeps, mix);
+        if (par[P_ROOT_Q] > 0) {
+          const double rho = ps / (rd * Tsa), Ub = uabs[i] + gust;
+          const double us3 = sfc ? fpow_(fmax(0.0, ch[i] / rho * Ub), 1.5, mix) : 0.0;
+          const double Bk = fmax(0.0, (H + air->vf * cp * Tsa * E) / (rho * cp));
+          const double hb = rd * Tsa / grav * log(ps / fmax(ps - grav * mu[bot], 1.0));
+          const double wm = cbrt(us3 + 0.28 * grav / Tsa * Bk * hb);
+          const double ex = wm > 1e-3 ? 8.5 * fmax(0.0, E) / (rho * wm) : 0.0;
+          q0 += fmin(ex, fmax(0.0, qss - q0));
+        } else if (cloudy) {
+          q0 = fmax(q0, fmin(qss, q0 + (1 - par[P_RH_CRIT]) * qss));
+        }
+      } else if (cloudy) {                           /* plumes root in the cell's moist air: its moistest part */
         double qs, dqs;
-        layer_cap(bot, mu, th0, par, aeps, mix, &qs, &dqs);
+        layer_cap(L, bot, mu, th0, par, aeps, mix, &qs, &dqs);
         q0 = fmax(q0, fmin(qs, q0 + (1 - par[P_RH_CRIT]) * qs));
       }
       double thp, qc, thp2, qc2;
@@ -1209,7 +1353,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           Rc = mc * (q0 - qc);
           move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qc, NULL);
           for (int j = kd; j < bot; j++) wxo[j] += mc;
-          const double rt = mu[kd] * condense_k(kd, mu, th, q, pib, par, aeps, mix);
+          const double rt = mu[kd] * condense_k(L, kd, mu, th, q, pib, par, aeps, mix);
           R += rt;
           Rc += rt;
         } else {
@@ -1221,7 +1365,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
      * convection's area revap_area; else on conv_revap of the cell) */
     if (cloudy && (par[P_REVAP] > 0 ? Rc > 0 : par[P_CONV_REVAP] > 0 && !(par[P_DOWNDRAFT] > 0) && Rc > 0) && L > 1) {
       double qs, dqs;
-      layer_cap(bot, mu, th[bot], par, aeps, mix, &qs, &dqs);
+      layer_cap(L, bot, mu, th[bot], par, aeps, mix, &qs, &dqs);
       const double rc_ = par[P_REVAP] > 0 ? (isl ? par[P_REVAP_RHL] : par[P_REVAP_RHS]) : 1.0;
       if (rc_ * qs > q[bot]) {
         double ev;

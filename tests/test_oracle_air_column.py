@@ -769,3 +769,135 @@ def test_mixed_layer_condensation_level():
     left = c['mu'][2] > lcl + 1.0
     assert left.sum() > 10
     np.testing.assert_allclose(mu1[2][left], lcl[left], atol=1e-6)
+
+
+# ---- the earth preset's moist options: sat_prof, sat_base, edge_vap, freeze_dry, q_shape, plume_root, root_q ----
+
+EARTH_OPTS = [dict(sat_prof=1.0), dict(sat_base=1.0), dict(edge_vap=1.0), dict(freeze_dry=1.0, edge_vap=1.0),
+              dict(q_shape=1.0), dict(plume_root=1.0), dict(plume_root=1.0, root_q=1.0, sfc=1.0),
+              dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, q_shape=1.0, plume_root=1.0, cloud=1.0),
+              dict(edge_vap=1.0, bl_mix=1.0, bl_reach=1.0)]
+
+
+@pytest.mark.parametrize('opts', EARTH_OPTS)
+@pytest.mark.parametrize('mix', [False, True])
+def test_earth_options_close(opts, mix):
+    """Each option only moves vapor between the layers (edge vapor, freeze-drying), changes a capacity (sat_prof,
+    sat_base, q_shape) or the plume's root: the energy still closes, the water changes by evaporation less rain."""
+    c = moist_aqua(rh=(0.1, 0.8, 1.3), warm=12.0)
+    N = c['Ts'].size
+    cloudy = opts.get('cloud', 0) > 0
+    kw = dict(ch=np.full(N, 0.02)) if opts.get('sfc') else {}
+    if cloudy:
+        c['cw'] = np.full_like(c['q'], 1e-4)
+        kw['cw'] = c['cw']
+    p = par3(moist=1.0, **dict(opts, tau_conv=1e12) if cloudy else opts)
+    dt = 900.0
+    o = run(c, p, q=c['q'], ice=c['ice'], dt=dt, mix=mix, **kw)
+    stored_matches(o, dt)
+    mu1 = new_masses(c, o['wx'])
+    W0 = (c['mu'] * (c['q'] + (c['cw'] if cloudy else 0))).sum(0)
+    W1 = (mu1 * (o['q'] + (o['cw'] if cloudy else 0))).sum(0)
+    np.testing.assert_allclose(W1 - W0, (o['evap'] - o['rain']) * dt, atol=1e-11 * W0.max())
+    base = run(c, par3(moist=1.0, **({'tau_conv': 1e12, 'cloud': 1.0} if cloudy else {}),
+                       **({'sfc': 1.0} if opts.get('sfc') else {})), q=c['q'], ice=c['ice'], dt=dt, mix=mix, **kw)
+    assert np.abs(o['q'] - base['q']).max() > 0                 # (the option changed something)
+
+
+QUIET = dict(moist=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, k_evap=0.0, theta_star=-1.0, theta_star2=-1.0)
+
+
+def newton_step(q, qs, dqs_dth, pib):
+    """One saturation-adjustment Newton step: the condensate (q - q_s) / (1 + L_v / pi dq_s/dtheta)."""
+    return (q - qs) / (1 + 2.5e6 / pib * dqs_dth)
+
+
+def test_sat_base_closed_form():
+    """sat_base: the third layer condenses against saturation at its base (its lower edge's pressure and Exner
+    function), one Newton step with Clausius-Clapeyron's slope."""
+    c = moist_aqua(rh=(0.1, 0.5, 1.3))
+    N = c['Ts'].size
+    o = run(c, par3(sat_base=1.0, **QUIET), mode=1, insol=np.zeros(N), q=c['q'], ice=np.zeros(N), dt=600.0)
+    pt, pb, _ = pib_of(c['mu'])
+    p, pi = pb[2], CP * (pb[2] / P0) ** KAPPA
+    T = pi * c['theta'][2] / CP
+    qs = tetens(T, p)
+    dq = (c['q'][2] - qs) / (1 + 2.5e6 * 2.5e6 * qs / (CP * 461.5 * T * T))
+    np.testing.assert_allclose(o['q'][2], c['q'][2] - np.maximum(dq, 0.0), rtol=1e-12)
+
+
+def spread_shape(a, b, m):
+    """The truncated-exponential shape s of a layer's theta between its edges a < b with mean m:
+    1 - 1/s + 1/(e^s - 1) = (m - a) / (b - a) (brentq)."""
+    from scipy.optimize import brentq
+    x = (m - a) / (b - a)
+    f = lambda s: (0.5 + s / 12 if abs(s) < 1e-4 else 1 - 1 / s + 1 / np.expm1(s)) - x
+    return brentq(f, -700, 700, xtol=1e-15, rtol=1e-15)
+
+
+def test_sat_prof_closed_form():
+    """sat_prof: the middle layer (below edge_k's tropopause) condenses against its capacity with its theta spread
+    between its edges by the truncated exponential of its mean (the mass share m from its base at theta
+    a + (b - a) ln(1 + m (e^s - 1)) / s), averaged by 4-point Gauss-Legendre over its pressure span."""
+    c = moist_aqua(N=8, rh=(0.1, 1.6, 0.5))
+    N = 8
+    # (theta* kept at 305 and 380 K, the spread's edges; the crossing then only lifts the middle layer's air, at its
+    # own vapor, into the top layer; the plumes held off)
+    p = par3(moist=1.0, sat_prof=1.0, tau0=0.0, eps_s=0.0, h_sens=0.0, k_evap=0.0, tau_conv=1e30)
+    o = run(c, p, mode=1, insol=np.zeros(N), q=c['q'], ice=np.zeros(N), dt=600.0)
+    pt, pb, pib = pib_of(c['mu'])
+    x, w = np.polynomial.legendre.leggauss(4)
+    a, b = 305.0, 380.0
+    for i in range(N):
+        s = spread_shape(a, b, c['theta'][1, i])
+        qs = dqs = 0.0
+        for xj, wj in zip(x, w):
+            p = 0.5 * (pt[1, i] + pb[1, i]) + 0.5 * (pb[1, i] - pt[1, i]) * xj
+            m = (pb[1, i] - p) / (pb[1, i] - pt[1, i])
+            tn = a + (b - a) * np.log1p(m * np.expm1(s)) / s
+            ex = (p / P0) ** KAPPA
+            v = tetens(tn * ex, p)
+            qs += 0.5 * wj * v
+            dqs += 0.5 * wj * v * 2.5e6 / (461.5 * (tn * ex) ** 2) * ex
+        dq = newton_step(c['q'][1, i], qs, dqs, pib[1, i])
+        np.testing.assert_allclose(o['q'][1, i], c['q'][1, i] - max(dq, 0.0), rtol=1e-10)
+
+
+def test_q_shape_closed_form():
+    """q_shape: the surface air holds the bottom layer's vapor at its relative humidity there, q_b q_s(T_sa, p_s) /
+    q_cap (q_cap the layer's 4-point Gauss-Legendre capacity), at most saturated; evaporation is kex (q_s(T_s) - q_a)."""
+    c = moist_aqua(rh=(0.1, 0.5, 0.6))
+    N = c['Ts'].size
+    p = par3(moist=1.0, q_shape=1.0, theta_star=-1.0, theta_star2=-1.0)
+    o = run(c, p, mode=1, insol=np.zeros(N), q=c['q'], ice=np.zeros(N), dt=600.0)
+    pt, pb, _ = pib_of(c['mu'])
+    x, w = np.polynomial.legendre.leggauss(4)
+    pp = 0.5 * (pt[2] + pb[2])[None] + 0.5 * (pb[2] - pt[2])[None] * x[:, None]
+    qcap = (0.5 * w[:, None] * tetens(c['theta'][2][None] * (pp / P0) ** KAPPA, pp)).sum(0)
+    ps = pb[2]
+    Tsa = c['theta'][2] * (ps / P0) ** KAPPA
+    qss = tetens(Tsa, ps)
+    qa = np.minimum(c['q'][2] * qss / qcap, qss)
+    kex = 4.0e-3 * (1 + c['uabs'] / 10.0)
+    np.testing.assert_allclose(o['evap'], kex * np.maximum(tetens(c['Ts'], ps) - qa, 0.0), rtol=1e-12)
+
+
+def test_freeze_dry_closed_form():
+    """freeze_dry: air rising into the stratosphere (across the interface below edge_k's layer) past the cold point
+    at theta* holds at most saturation there, q_s(theta* (p_i / p0)^kappa, p_i); the top layer's vapor is then its
+    own plus that, by mass. (The middle layer supersaturated: its condensation heats it across its top edge and the
+    cap binds; only the crossing moves air into the top.)"""
+    c = moist_aqua(rh=(0.01, 1.3, 0.5))
+    N = c['Ts'].size
+    c['theta'][1] = 381.0 + np.linspace(0, 5, N)                # above its top edge: it rises into the top layer
+    c['q'][1] = 0.15                                            # (past its capacity, down to its ~330 K base)
+    p = par3(moist=1.0, freeze_dry=1.0, edge_k=1.0, tau_conv=1e30)
+    o = run(c, p, q=c['q'], ice=c['ice'], dt=600.0)
+    wx = o['wx'][0]
+    up = wx > 0
+    assert up.sum() > N // 2
+    pi = GRAV * c['mu'][0]
+    qsi = tetens(380.0 * (pi / P0) ** KAPPA, pi)
+    assert np.all(qsi[up] < o['q'][1][up])
+    top = (c['mu'][0] * c['q'][0] + wx * qsi) / (c['mu'][0] + wx)
+    np.testing.assert_allclose(o['q'][0][up], top[up], rtol=1e-12)
