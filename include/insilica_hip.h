@@ -273,14 +273,19 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * layer: the surface's fluxes go into it alone, the lowest interface floats with it, it entrains the air above by
  * the surface's buoyancy flux (bl_ent; bl_reach: dry convection into the layers above colder than it) and gives its
  * top back at night down to mu_night, ends at its condensation level with bl_lcl (ent_lcl: its entrainment not
- * stopped there), and a layer emptied to its floor keeps its theta within its edges. Updates theta, Ts (and qv,
- * ice) in place and writes wx (L-1, N), olr, sw_abs, stored (the column's energy change, J/m^2), rain, evap, conv
+ * stopped there), and a layer emptied to its floor keeps its theta within its edges. The moist options sat_prof (a
+ * middle layer's capacity with its theta spread between its edges), sat_base (the third layer condensing at its
+ * base), edge_vap (air crossing an edge carries the vapor there), freeze_dry (air rising into the stratosphere holds
+ * at most the cold point's saturation), q_shape (the surface air's vapor at the bottom layer's relative humidity) and
+ * plume_root with root_q (plumes rooted in the surface air, with the thermals' excess) as the reference's. Updates
+ * theta, Ts (and qv, ice) in place and writes wx (L-1, N), olr, sw_abs, stored (the column's energy change, J/m^2),
+ * rain, evap, conv
  * (kg/m^2/s), tsurf; rainp (4, N: rain from the layers above the bottom one, from the bottom one, the plumes', snow),
  * qocean, qsw, flash, swalb and firn (the firn's temperature) when given; cap (N) the surface's heat capacity, else
  * c_slab. The argument list is the reference's; elev (N) with theta_rise > 0 raises the lowest interface over high
  * ground. Returns ISW_EBADARG for cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or
- * ice, with land but without soilcap, soil, pond or snow, with plume_deep, plume_root, q_shape, sat_prof, sat_base,
- * edge_vap or freeze_dry, and with clouds and bl_q: those parts of the reference's column are not in this library.
+ * ice, with land but without soilcap, soil, pond or snow, with plume_deep (the deep and shallow plume family), and
+ * with clouds and bl_q: those parts of the reference's column are not in this library.
  * prec: ISW_FP64, or ISW_MIXED (the reference's mixed build: 1 - e^-x, the edge sub-levels' Exner powers and the
  * saturation and condensation arithmetic in float32). */
 int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* air, int32_t mode, const double* t,

@@ -113,6 +113,257 @@ __device__ void edge_density(double a, double b, double m, double& fa, double& f
   fb = s * (e1 + 1) / e1 / W;
 }
 
+// The air column's own parameters (PAR_ORDER positions) and the air's constants, by value beside CP.
+struct AP {
+  double S0, asw, tau0, mu_ref, eps_s, h_sens, u0, alb, c_slab, sigma, tilt, year_s, day_s, tau_lin, Lf, Tf, sw_tau,
+      bl_bot, theta_rise, bulk_c, bulk_z, bulk_w, edge_heat, sfc, edge_k, firn, rho_firn, c_ice, ecc, lam_peri, rd,
+      k_evap, tau_conv, alb_ice, k_ice, Rv, sat_layer, alb_ice_melt, alb_ramp, lw_w0, ice_air, dry_edge, eps;
+  // the clouds'
+  double rh_crit, re_liq, re_ice, cloud_g, k_lw_liq, k_lw_ice, c_auto, q_auto, v_ice, tau_erode, k_revap, mu_cloud,
+      l_ice, conv_revap, downdraft, revap, revap_rhs, revap_area, liq_cold, cw_qs;
+  // the land's
+  double alb_dry, alb_wet, soil_beta, k_inf, pond_cover, alb_snow, alb_snow_melt, alb_glacier, snow_cover, land_exch,
+      c_dry, c_wet, land_rs, c_deep, rs_max, rs_sun, revap_rhl, bare_fx, k_firn;
+  // the mixed boundary layer's
+  double mu_night, bl_ent, bl_reach, bl_lcl, ent_lcl, vf;
+  // the spread layers' capacity, the base condensation, edge vapor, freeze-drying, the surface air's vapor, the
+  // plumes' root
+  double sat_prof, sat_base, edge_vap, freeze_dry, q_shape, plume_root, root_q, bl_q;
+};
+
+static AP make_ap(const double* a, const isw_air* air) {
+  AP p;
+  p.S0 = a[0]; p.asw = a[1]; p.tau0 = a[2]; p.mu_ref = a[3]; p.eps_s = a[4]; p.h_sens = a[5]; p.u0 = a[6];
+  p.alb = a[7]; p.c_slab = a[8]; p.sigma = a[12]; p.tilt = a[17]; p.year_s = a[18]; p.day_s = a[19];
+  p.tau_lin = a[20]; p.Lf = a[24]; p.Tf = a[25]; p.sw_tau = a[34]; p.bl_bot = a[54]; p.theta_rise = a[69];
+  p.bulk_c = a[74]; p.bulk_z = a[76]; p.bulk_w = a[77]; p.edge_heat = a[84]; p.sfc = a[90]; p.edge_k = a[101];
+  p.firn = a[120]; p.rho_firn = a[122]; p.c_ice = a[123]; p.ecc = a[126]; p.lam_peri = a[127]; p.rd = air->rd;
+  p.k_evap = a[26]; p.tau_conv = a[27]; p.alb_ice = a[28]; p.k_ice = a[31]; p.Rv = a[32]; p.sat_layer = a[33];
+  p.alb_ice_melt = a[41]; p.alb_ramp = a[45]; p.lw_w0 = a[48]; p.ice_air = a[102]; p.dry_edge = a[99];
+  p.eps = air->eps;
+  p.rh_crit = a[56]; p.re_liq = a[57]; p.re_ice = a[58]; p.cloud_g = a[59]; p.k_lw_liq = a[60]; p.k_lw_ice = a[61];
+  p.c_auto = a[62]; p.q_auto = a[63]; p.v_ice = a[64]; p.tau_erode = a[66]; p.k_revap = a[67]; p.mu_cloud = a[68];
+  p.l_ice = a[79]; p.conv_revap = a[87]; p.downdraft = a[92]; p.revap = a[103]; p.revap_rhs = a[104];
+  p.revap_area = a[106]; p.liq_cold = a[128]; p.cw_qs = a[129];
+  p.alb_dry = a[35]; p.alb_wet = a[36]; p.soil_beta = a[37]; p.k_inf = a[38]; p.pond_cover = a[39]; p.alb_snow = a[40];
+  p.alb_snow_melt = a[42]; p.alb_glacier = a[43]; p.snow_cover = a[44]; p.land_exch = a[46]; p.c_dry = a[49];
+  p.c_wet = a[50]; p.land_rs = a[52]; p.c_deep = a[94]; p.rs_max = a[95]; p.rs_sun = a[96]; p.revap_rhl = a[105];
+  p.bare_fx = a[115]; p.k_firn = a[121];
+  p.mu_night = a[71]; p.bl_ent = a[72]; p.bl_reach = a[98]; p.bl_lcl = a[100]; p.ent_lcl = a[130]; p.vf = air->vf;
+  p.sat_prof = a[51]; p.sat_base = a[53]; p.edge_vap = a[86]; p.freeze_dry = a[85]; p.q_shape = a[83];
+  p.plume_root = a[88]; p.root_q = a[107]; p.bl_q = a[89];
+  return p;
+}
+
+template <bool MIX>
+__device__ __forceinline__ double omexp(double x) {
+  if constexpr (MIX) return (double)-expm1f(-(float)x);
+  else return 1 - exp(-(x));
+}
+template <bool MIX>
+__device__ __forceinline__ double fpow_(double x, double y) {
+  if constexpr (MIX) return (double)powf((float)x, (float)y);
+  else return pow(x, y);
+}
+
+// The gray optical depth down to pressure p of a column with surface pressure ps (air.cu:853-856).
+__device__ __forceinline__ double tau_p(double p, double ps, const AP& A) {
+  const double s = p / ps, s2 = s * s;
+  return A.tau0 * (A.tau_lin * s + (1 - A.tau_lin) * s2 * s2);
+}
+
+// The sun's ecliptic longitude at t and its distance factor (air.cu:174-185): Kepler's equation, five Newton steps.
+__device__ double sun_lon_fd(double t, const AP& A, double* fd) {
+  constexpr double PI_ = 3.14159265358979323846;
+  if (!(A.ecc > 0)) {
+    *fd = 1;
+    return 2 * PI_ * t / A.year_s;
+  }
+  const double e = A.ecc, nu0 = -A.lam_peri;
+  const double E0 = 2 * atan(sqrt((1 - e) / (1 + e)) * tan(nu0 / 2));
+  const double M = E0 - e * sin(E0) + 2 * PI_ * t / A.year_s;
+  double E = M;
+  for (int it = 0; it < 5; it++) E -= (E - e * sin(E) - M) / (1 - e * cos(E));
+  const double nu = 2 * atan2(sqrt(1 + e) * sin(E / 2), sqrt(1 - e) * cos(E / 2)), r = 1 - e * cos(E);
+  *fd = 1 / (r * r);
+  return nu + A.lam_peri;
+}
+
+// Saturation specific humidity (air.cu:156-171): Tetens over water; with ice_air, below freezing the blend with
+// Tetens over ice by the liquid share. F is the build's `fast` type.
+template <typename F>
+__device__ __forceinline__ F qsat_a(F T, F p, F eps, double ice) {
+  const F w = eps * (F)611.0 * (F)exp(((F)17.27 * (T - (F)273.15) / (T - (F)35.85))) / p;
+  if (!(ice > 0) || T >= (F)273.15) return w;
+  const F l = fmin((F)1, fmax((F)0, (T - (F)253.15) / (F)20));
+  const F si = eps * (F)611.0 * (F)exp(((F)21.875 * (T - (F)273.16) / (T - (F)7.66))) / p;
+  return l * w + (1 - l) * si;
+}
+
+// Saturation adjustment at Exner pib and pressure p in one Newton step (air.cu:190-198); returns the condensate.
+template <bool MIX>
+__device__ double condense(double& th, double& q, double pib, double p, const CP& P, const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  const F T = (F)pib * (F)th / (F)P.cp, qs = qsat_a<F>(T, (F)p, (F)A.eps, A.ice_air);
+  if (!(q > qs)) return 0;
+  const double dq = (F)(q - qs) / ((F)1 + (F)P.Lv * (F)P.Lv * qs / ((F)P.cp * (F)A.Rv * T * T));
+  q -= dq;
+  th += P.Lv * dq / pib;
+  return dq;
+}
+
+// The mass share m's theta in a layer of shape s as a fraction of the way between its edges, ln(1 + m (e^s - 1)) / s
+// (air.cu:233-244; es = e^s - 1). The mixed build in float32 with asymptotes beyond |s| = 30.
+template <bool MIX>
+__device__ __forceinline__ double spread_of(double m, double s, double es) {
+  if constexpr (MIX) {
+    const float mf = (float)m, sf = (float)s;
+    if (fabsf(sf) < 1e-6f) return mf;
+    if (sf > 30.f) return 1.f + logf(mf) / sf;
+    if (sf < -30.f) return log1pf(-mf) / sf;
+    return log1pf(mf * expm1f(sf)) / sf;
+  } else {
+    return fabs(s) < 1e-6 ? m : log1p(m * es) / s;
+  }
+}
+
+// The layer's vapor capacity qs over its pressure span [pt, pb] and its change with theta dqs (air.cu:334-359):
+// 4-point Gauss-Legendre, nodes colder than 160 K holding nothing, Clausius-Clapeyron's slope at each node. Its theta
+// uniform, or (spread) spread between its edges ea < eb by the truncated exponential of its mean (the mass share m
+// from its base at ea + (eb - ea) spread_of(m)). In the build's `fast` type, returned as such.
+template <bool MIX>
+__device__ __forceinline__ void layer_cap(double th, double pt, double pb, const CP& P, const AP& A,
+                                          std::conditional_t<MIX, float, double>& qs,
+                                          std::conditional_t<MIX, float, double>& dqs, bool spread = false,
+                                          double ea = 0, double eb = 0) {
+  using F = std::conditional_t<MIX, float, double>;
+  constexpr double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  constexpr double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  const double s = spread ? edge_shape(ea, eb, th) : 0.0, es = spread && !MIX ? expm1(s) : 0.0;
+  qs = 0;
+  dqs = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j], m = (pb - p) / (pb - pt);
+    const double tn = !spread ? th : ea + (eb - ea) * spread_of<MIX>(m, s, es);
+    const F ex = (F)fpow_<MIX>(p / P.p0, P.kappa), T = (F)tn * ex;
+    if (T < 160) continue;
+    const F v = qsat_a<F>(T, (F)p, (F)A.eps, A.ice_air);
+    qs += (F)0.5 * (F)wq[j] * v;
+    dqs += (F)0.5 * (F)wq[j] * v * (F)P.Lv / ((F)A.Rv * T * T) * ex;
+  }
+}
+
+// Whether layer k's capacity has its theta spread between its edges (sat_prof: a middle layer below edge_k's
+// tropopause, colder than its top edge), and those edges.
+template <int L>
+__device__ __forceinline__ bool spread_layer(int k, double th, const CP& P, const AP& A, double& ea, double& eb) {
+  const bool sp = A.sat_prof > 0 && k >= (int)A.edge_k && k < L - 1 && th < tstar<L>(k - 1, P);
+  ea = sp ? tstar<L>(k, P) : 0.0;
+  eb = sp ? tstar<L>(k - 1, P) : 0.0;
+  return sp;
+}
+
+// Layer k's capacity (the spread one with sat_prof) over [pt, pb].
+template <int L, bool MIX>
+__device__ __forceinline__ void layer_cap_k(int k, double th, double pt, double pb, const CP& P, const AP& A,
+                                            std::conditional_t<MIX, float, double>& qs,
+                                            std::conditional_t<MIX, float, double>& dqs) {
+  double ea, eb;
+  const bool sp = spread_layer<L>(k, th, P, A, ea, eb);
+  layer_cap<MIX>(th, pt, pb, P, A, qs, dqs, sp, ea, eb);
+}
+
+// Saturation adjustment against the layer's capacity over [pt, pb] (air.cu:275-291; spread: condense_spread,
+// air.cu:292-315, the whole spread shifting with the mean): one Newton step.
+template <bool MIX>
+__device__ double condense_layer(double& th, double& q, double pib, double pt, double pb, const CP& P, const AP& A,
+                                 bool spread = false, double ea = 0, double eb = 0) {
+  using F = std::conditional_t<MIX, float, double>;
+  F qs, dqs;
+  layer_cap<MIX>(th, pt, pb, P, A, qs, dqs, spread, ea, eb);
+  if (!(q > qs)) return 0;
+  const double dq = (F)(q - qs) / ((F)1 + (F)P.Lv / (F)pib * dqs);
+  q -= dq;
+  th += P.Lv * dq / pib;
+  return dq;
+}
+
+// Condensation in layer k (air.cu:319-331): the layer's capacity (sat_layer; sat_prof: a middle layer below
+// edge_k's tropopause with its theta spread between its edges; sat_base: the third layer at its base), else v1's rule
+// at the pressure whose Exner function is the layer's. tk, qk: layer k's theta and vapor, updated.
+template <int L, bool MIX>
+__device__ double condense_at(int k, double& tk, double& qk, double pk, double pt, double mk, const CP& P,
+                              const AP& A) {
+  if (A.sat_layer > 0) {
+    double ea, eb;
+    if (spread_layer<L>(k, tk, P, A, ea, eb))
+      return condense_layer<MIX>(tk, qk, pk, pt, pt + P.grav * mk, P, A, true, ea, eb);
+    if (A.sat_base > 0 && k == 2) {
+      const double pb = pt + P.grav * mk;
+      return condense<MIX>(tk, qk, P.cp * fpow_<MIX>(pb / P.p0, P.kappa), pb, P, A);
+    }
+    return condense_layer<MIX>(tk, qk, pk, pt, pt + P.grav * mk, P, A);
+  }
+  return condense<MIX>(tk, qk, pk, fpow_<MIX>(pk / P.cp, 1 / P.kappa) * P.p0, P, A);
+}
+
+// condense_at for a runtime layer index: the layer's values go through selects.
+template <int L, bool MIX>
+__device__ double condense_k(int k, const double* mu, double* th, double* q, const double* pib, const CP& P,
+                             const AP& A) {
+  double pt = 0, mk = 0, tk = 0, qk = 0, pk = 0;
+#pragma unroll
+  for (int j = 0; j < L; j++) {
+    if (j < k) pt += P.grav * mu[j];
+    if (j == k) { mk = mu[j]; tk = th[j]; qk = q[j]; pk = pib[j]; }
+  }
+  const double dq = condense_at<L, MIX>(k, tk, qk, pk, pt, mk, P, A);
+#pragma unroll
+  for (int j = 0; j < L; j++)
+    if (j == k) { th[j] = tk; q[j] = qk; }
+  return dq;
+}
+
+// The vapor of layer k's air at its top edge (top) or bottom edge (air.cu:361-378): its relative humidity against
+// the layer's capacity held uniform through the layer (a spread layer's edge at its theta*); layers above edge_k (with
+// L >= 3) and, with bl_q, the mixed bottom layer keep their vapor even. Air of mass m leaving through that edge takes
+// the edge's vapor plus m's share of the way to the mean, at most the layer's whole vapor. k may be a runtime index.
+template <int L, bool MIX>
+__device__ double edge_q(int k, bool top, const double* mu, const double* th, const double* q, double m, const CP& P,
+                         const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  double mk = 0, tk = 0, qk = 0, pt = 0;
+#pragma unroll
+  for (int j = 0; j < L; j++) {
+    if (j < k) pt += P.grav * mu[j];
+    if (j == k) { mk = mu[j]; tk = th[j]; qk = q[j]; }
+  }
+  if (!(qk > 0) || (k < (int)A.edge_k && L >= 3) || (A.bl_q > 0 && k == L - 1 && k > 0)) return qk;
+  F qs, dqs;
+  layer_cap_k<L, MIX>(k, tk, pt, pt + P.grav * mk, P, A, qs, dqs);
+  if (!(qs > 0)) return qk;
+  const double p = top ? pt : pt + P.grav * mk;
+  if (!(p > 0)) return qk;
+  double ea, eb;
+  const bool sp = spread_layer<L>(k, tk, P, A, ea, eb);
+  const double te = sp ? (top ? eb : ea) : tk;
+  const double T = te * fpow_<MIX>(p / P.p0, P.kappa);
+  const double qe = T < 160 ? 0.0 : qk * (double)qsat_a<F>((F)T, (F)p, (F)A.eps, A.ice_air) / qs;
+  const double f = m > 0 ? fmin(1.0, m / mk) : 0.0;
+  return m > 0 ? fmin(qe + (qk - qe) * f, mk * qk / m) : qe;
+}
+
+// A parcel (th, q) lifted to pressure p, condensing what it cannot hold (air.cu:468-473).
+template <bool MIX>
+__device__ __forceinline__ void parcel(double th, double q, double p, const CP& P, const AP& A, double& thp,
+                                       double& qp) {
+  thp = th;
+  qp = q;
+  const double ex = fpow_<MIX>(p / P.p0, P.kappa);
+  condense<MIX>(thp, qp, P.cp * ex, p, P, A);
+}
+
 // The column's energy (air.cu:484-488): enthalpy, and for moist air the vapor's latent energy.
 template <int L, bool MOIST = false>
 __device__ __forceinline__ double column_energy(const double* mu, const double* pib, const double* th, const double* q,
@@ -197,9 +448,9 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* q, double
 // et[k], eb[k]: the heating at layer k's top and bottom edge (the reference's dtop[k] or dth[k], and dbot[k] or dth[k],
 // the top layer's bottom-edge default its dth_up): always given, so every array stays in registers. CL: the crossing
 // air takes its cloud water cl along.
-template <int L, bool MOIST = false, bool CL = false>
+template <int L, bool MOIST = false, bool CL = false, bool MIX = false>
 __device__ void crossing(const CP& P, double* mu, double* th, double* q, const double* dth, const double* et,
-                         const double* eb, double* pib, double* wxo, double* cl = nullptr) {
+                         const double* eb, double* pib, double* wxo, double* cl = nullptr, const AP* A = nullptr) {
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (P.theta_star <= 0) return;
@@ -288,10 +539,31 @@ __device__ void crossing(const CP& P, double* mu, double* th, double* q, const d
         if (k == a) qa_ = q[k];
       }
     }
-    if (dm > 0)
-      move_mass<L, MOIST, CL>(P, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qb_, qb_, cl);
-    else if (dm < 0)
-      move_mass<L, MOIST, CL>(P, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], qa_, qa_, cl);
+    // edge vapor: the crossing air carries the vapor at the edge it leaves through; freeze-drying: air rising into
+    // the stratosphere past the cold point at theta* holds no more than saturation there, the rest staying below with
+    // the layers' cloud water
+    const bool ev = MOIST && A && A->edge_vap > 0;
+    if (MOIST && A && dm > 0 && a == (int)A->edge_k - 1 && L >= 3 && A->freeze_dry > 0) {
+      using F = std::conditional_t<MIX, float, double>;
+      double pi_ = 0;
+#pragma unroll
+      for (int k = 0; k <= a; k++) pi_ += P.grav * mu[k];
+      const double Ti = tstar<L>(a, P) * fpow_<MIX>(pi_ / P.p0, P.kappa);
+      const double qx = fmin(ev ? edge_q<L, MIX>(b, true, mu, th, q, dm, P, *A) : qb_,
+                             Ti < 160 ? 0.0 : (double)qsat_a<F>((F)Ti, (F)pi_, (F)A->eps, A->ice_air));
+      const double ma = mu[a], mb = mu[b], ca = CL ? cl[a] : 0.0, cb = CL ? cl[b] : 0.0;
+      move_mass<L, MOIST, CL>(P, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qx, qx, cl);
+      if constexpr (CL) {
+        cl[a] = ma * ca / mu[a];
+        cl[b] = mb * cb / mu[b];
+      }
+    } else if (dm > 0) {
+      const double qx = ev ? edge_q<L, MIX>(b, true, mu, th, q, dm, P, *A) : qb_;
+      move_mass<L, MOIST, CL>(P, mu, th, q, pib, b, a, dm, thcj[j], thcj[j], qx, qx, cl);
+    } else if (dm < 0) {
+      const double qx = ev ? edge_q<L, MIX>(a, false, mu, th, q, -dm, P, *A) : qa_;
+      move_mass<L, MOIST, CL>(P, mu, th, q, pib, a, b, -dm, thcj[j], thcj[j], qx, qx, cl);
+    }
     wxo[j] = dm;
   }
   const double tiny = P.floor_ * 1.000001;
@@ -488,173 +760,13 @@ __global__ void k_advance_time(double* t, double dt) { t[0] += dt; }
 #endif  // !ISW_COLUMN_CLOUDS
 
 // ---------------------------------------------------------------------------------------------------------------
-// air_column (air.cu:876-1972): the dry column, and the moist one without clouds, land or the mixed boundary layer:
-// the sun, the shortwave's split, the gray longwave (the vapor's part from the column's vapor with lw_w0) or the
-// k-tables' fluxes, the slab surface or the sea-ice skin under the bulk or surface-layer exchange, the evaporation,
-// the heating as theta, the condensation, the crossing with the k-tables' edge heating, and the moist convection's
-// parcel plume. MIX selects the reference's mixed build (air.cu:24-36): 1 - e^-x, the edge Exner powers and the
+// air_column (air.cu:876-1972): the dry column, and the moist one over the sea and land, with clouds and the mixed
+// boundary layer: the sun, the shortwave's split, the gray longwave (the vapor's part from the column's vapor with
+// lw_w0) or the k-tables' fluxes, the slab surface, the sea-ice skin or the ground under the bulk or surface-layer
+// exchange, the evaporation, the heating as theta, the condensation (the cloud scheme), the crossing with the
+// k-tables' edge heating, the mixed layer's moves, and the moist convection's parcel plume. MIX selects the reference's mixed build (air.cu:24-36): 1 - e^-x, the edge Exner powers and the
 // saturation and condensation arithmetic (its `fast` type) in float32; the rest is FP64 in both forms.
 // ---------------------------------------------------------------------------------------------------------------
-
-// The air column's own parameters (PAR_ORDER positions) and the air's constants, by value beside CP.
-struct AP {
-  double S0, asw, tau0, mu_ref, eps_s, h_sens, u0, alb, c_slab, sigma, tilt, year_s, day_s, tau_lin, Lf, Tf, sw_tau,
-      bl_bot, theta_rise, bulk_c, bulk_z, bulk_w, edge_heat, sfc, edge_k, firn, rho_firn, c_ice, ecc, lam_peri, rd,
-      k_evap, tau_conv, alb_ice, k_ice, Rv, sat_layer, alb_ice_melt, alb_ramp, lw_w0, ice_air, dry_edge, eps;
-  // the clouds'
-  double rh_crit, re_liq, re_ice, cloud_g, k_lw_liq, k_lw_ice, c_auto, q_auto, v_ice, tau_erode, k_revap, mu_cloud,
-      l_ice, conv_revap, downdraft, revap, revap_rhs, revap_area, liq_cold, cw_qs;
-  // the land's
-  double alb_dry, alb_wet, soil_beta, k_inf, pond_cover, alb_snow, alb_snow_melt, alb_glacier, snow_cover, land_exch,
-      c_dry, c_wet, land_rs, c_deep, rs_max, rs_sun, revap_rhl, bare_fx, k_firn;
-  // the mixed boundary layer's
-  double mu_night, bl_ent, bl_reach, bl_lcl, ent_lcl, vf;
-};
-
-static AP make_ap(const double* a, const isw_air* air) {
-  AP p;
-  p.S0 = a[0]; p.asw = a[1]; p.tau0 = a[2]; p.mu_ref = a[3]; p.eps_s = a[4]; p.h_sens = a[5]; p.u0 = a[6];
-  p.alb = a[7]; p.c_slab = a[8]; p.sigma = a[12]; p.tilt = a[17]; p.year_s = a[18]; p.day_s = a[19];
-  p.tau_lin = a[20]; p.Lf = a[24]; p.Tf = a[25]; p.sw_tau = a[34]; p.bl_bot = a[54]; p.theta_rise = a[69];
-  p.bulk_c = a[74]; p.bulk_z = a[76]; p.bulk_w = a[77]; p.edge_heat = a[84]; p.sfc = a[90]; p.edge_k = a[101];
-  p.firn = a[120]; p.rho_firn = a[122]; p.c_ice = a[123]; p.ecc = a[126]; p.lam_peri = a[127]; p.rd = air->rd;
-  p.k_evap = a[26]; p.tau_conv = a[27]; p.alb_ice = a[28]; p.k_ice = a[31]; p.Rv = a[32]; p.sat_layer = a[33];
-  p.alb_ice_melt = a[41]; p.alb_ramp = a[45]; p.lw_w0 = a[48]; p.ice_air = a[102]; p.dry_edge = a[99];
-  p.eps = air->eps;
-  p.rh_crit = a[56]; p.re_liq = a[57]; p.re_ice = a[58]; p.cloud_g = a[59]; p.k_lw_liq = a[60]; p.k_lw_ice = a[61];
-  p.c_auto = a[62]; p.q_auto = a[63]; p.v_ice = a[64]; p.tau_erode = a[66]; p.k_revap = a[67]; p.mu_cloud = a[68];
-  p.l_ice = a[79]; p.conv_revap = a[87]; p.downdraft = a[92]; p.revap = a[103]; p.revap_rhs = a[104];
-  p.revap_area = a[106]; p.liq_cold = a[128]; p.cw_qs = a[129];
-  p.alb_dry = a[35]; p.alb_wet = a[36]; p.soil_beta = a[37]; p.k_inf = a[38]; p.pond_cover = a[39]; p.alb_snow = a[40];
-  p.alb_snow_melt = a[42]; p.alb_glacier = a[43]; p.snow_cover = a[44]; p.land_exch = a[46]; p.c_dry = a[49];
-  p.c_wet = a[50]; p.land_rs = a[52]; p.c_deep = a[94]; p.rs_max = a[95]; p.rs_sun = a[96]; p.revap_rhl = a[105];
-  p.bare_fx = a[115]; p.k_firn = a[121];
-  p.mu_night = a[71]; p.bl_ent = a[72]; p.bl_reach = a[98]; p.bl_lcl = a[100]; p.ent_lcl = a[130]; p.vf = air->vf;
-  return p;
-}
-
-template <bool MIX>
-__device__ __forceinline__ double omexp(double x) {
-  if constexpr (MIX) return (double)-expm1f(-(float)x);
-  else return 1 - exp(-(x));
-}
-template <bool MIX>
-__device__ __forceinline__ double fpow_(double x, double y) {
-  if constexpr (MIX) return (double)powf((float)x, (float)y);
-  else return pow(x, y);
-}
-
-// The gray optical depth down to pressure p of a column with surface pressure ps (air.cu:853-856).
-__device__ __forceinline__ double tau_p(double p, double ps, const AP& A) {
-  const double s = p / ps, s2 = s * s;
-  return A.tau0 * (A.tau_lin * s + (1 - A.tau_lin) * s2 * s2);
-}
-
-// The sun's ecliptic longitude at t and its distance factor (air.cu:174-185): Kepler's equation, five Newton steps.
-__device__ double sun_lon_fd(double t, const AP& A, double* fd) {
-  constexpr double PI_ = 3.14159265358979323846;
-  if (!(A.ecc > 0)) {
-    *fd = 1;
-    return 2 * PI_ * t / A.year_s;
-  }
-  const double e = A.ecc, nu0 = -A.lam_peri;
-  const double E0 = 2 * atan(sqrt((1 - e) / (1 + e)) * tan(nu0 / 2));
-  const double M = E0 - e * sin(E0) + 2 * PI_ * t / A.year_s;
-  double E = M;
-  for (int it = 0; it < 5; it++) E -= (E - e * sin(E) - M) / (1 - e * cos(E));
-  const double nu = 2 * atan2(sqrt(1 + e) * sin(E / 2), sqrt(1 - e) * cos(E / 2)), r = 1 - e * cos(E);
-  *fd = 1 / (r * r);
-  return nu + A.lam_peri;
-}
-
-// Saturation specific humidity (air.cu:156-171): Tetens over water; with ice_air, below freezing the blend with
-// Tetens over ice by the liquid share. F is the build's `fast` type.
-template <typename F>
-__device__ __forceinline__ F qsat_a(F T, F p, F eps, double ice) {
-  const F w = eps * (F)611.0 * (F)exp(((F)17.27 * (T - (F)273.15) / (T - (F)35.85))) / p;
-  if (!(ice > 0) || T >= (F)273.15) return w;
-  const F l = fmin((F)1, fmax((F)0, (T - (F)253.15) / (F)20));
-  const F si = eps * (F)611.0 * (F)exp(((F)21.875 * (T - (F)273.16) / (T - (F)7.66))) / p;
-  return l * w + (1 - l) * si;
-}
-
-// Saturation adjustment at Exner pib and pressure p in one Newton step (air.cu:190-198); returns the condensate.
-template <bool MIX>
-__device__ double condense(double& th, double& q, double pib, double p, const CP& P, const AP& A) {
-  using F = std::conditional_t<MIX, float, double>;
-  const F T = (F)pib * (F)th / (F)P.cp, qs = qsat_a<F>(T, (F)p, (F)A.eps, A.ice_air);
-  if (!(q > qs)) return 0;
-  const double dq = (F)(q - qs) / ((F)1 + (F)P.Lv * (F)P.Lv * qs / ((F)P.cp * (F)A.Rv * T * T));
-  q -= dq;
-  th += P.Lv * dq / pib;
-  return dq;
-}
-
-// The layer's vapor capacity qs over its pressure span [pt, pb] at a uniform theta, and its change with theta dqs
-// (air.cu:334-359): 4-point Gauss-Legendre, nodes colder than 160 K holding nothing, Clausius-Clapeyron's slope at
-// each node. In the build's `fast` type, returned as such.
-template <bool MIX>
-__device__ __forceinline__ void layer_cap(double th, double pt, double pb, const CP& P, const AP& A,
-                                          std::conditional_t<MIX, float, double>& qs,
-                                          std::conditional_t<MIX, float, double>& dqs) {
-  using F = std::conditional_t<MIX, float, double>;
-  constexpr double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
-  constexpr double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
-  qs = 0;
-  dqs = 0;
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
-    const F ex = (F)fpow_<MIX>(p / P.p0, P.kappa), T = (F)th * ex;
-    if (T < 160) continue;
-    const F s = qsat_a<F>(T, (F)p, (F)A.eps, A.ice_air);
-    qs += (F)0.5 * (F)wq[j] * s;
-    dqs += (F)0.5 * (F)wq[j] * s * (F)P.Lv / ((F)A.Rv * T * T) * ex;
-  }
-}
-
-// Saturation adjustment against the layer's capacity over [pt, pb] (air.cu:275-291): one Newton step.
-template <bool MIX>
-__device__ double condense_layer(double& th, double& q, double pib, double pt, double pb, const CP& P, const AP& A) {
-  using F = std::conditional_t<MIX, float, double>;
-  F qs, dqs;
-  layer_cap<MIX>(th, pt, pb, P, A, qs, dqs);
-  if (!(q > qs)) return 0;
-  const double dq = (F)(q - qs) / ((F)1 + (F)P.Lv / (F)pib * dqs);
-  q -= dq;
-  th += P.Lv * dq / pib;
-  return dq;
-}
-
-// Condensation in layer k (air.cu:319-331; sat_prof = sat_base = 0): the layer's capacity, else v1's rule at the
-// pressure whose Exner function is the layer's. k is a runtime index: the layer's values go through selects.
-template <int L, bool MIX>
-__device__ double condense_k(int k, const double* mu, double* th, double* q, const double* pib, const CP& P,
-                             const AP& A) {
-  double pt = 0, mk = 0, tk = 0, qk = 0, pk = 0;
-#pragma unroll
-  for (int j = 0; j < L; j++) {
-    if (j < k) pt += P.grav * mu[j];
-    if (j == k) { mk = mu[j]; tk = th[j]; qk = q[j]; pk = pib[j]; }
-  }
-  double dq;
-  if (A.sat_layer > 0) dq = condense_layer<MIX>(tk, qk, pk, pt, pt + P.grav * mk, P, A);
-  else dq = condense<MIX>(tk, qk, pk, fpow_<MIX>(pk / P.cp, 1 / P.kappa) * P.p0, P, A);
-#pragma unroll
-  for (int j = 0; j < L; j++)
-    if (j == k) { th[j] = tk; q[j] = qk; }
-  return dq;
-}
-
-// A parcel (th, q) lifted to pressure p, condensing what it cannot hold (air.cu:468-473).
-template <bool MIX>
-__device__ __forceinline__ void parcel(double th, double q, double p, const CP& P, const AP& A, double& thp,
-                                       double& qp) {
-  thp = th;
-  qp = q;
-  const double ex = fpow_<MIX>(p / P.p0, P.kappa);
-  condense<MIX>(thp, qp, P.cp * ex, p, P, A);
-}
 
 // The cloud statistics (air.cu:383-388): total water qt spread evenly over +-(1 - rh_crit) qs about its mean; the
 // saturated share f and the mean condensate l there.
@@ -765,7 +877,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double pc = k == bot ? 0.5 * (pb[k] + ptk) : ptk + 0.25 * (pb[k] - ptk);
       liq[k] = fmax(0.0, fmin(1.0, (th[k] * fpow_<MIX>(pc / P.p0, P.kappa) - A.Tf + A.liq_cold) / A.liq_cold));
       F qs, dqs;
-      layer_cap<MIX>(th[k], ptk, ptk + P.grav * mu[k], P, A, qs, dqs);
+      layer_cap_k<L, MIX>(k, th[k], ptk, ptk + P.grav * mu[k], P, A, qs, dqs);
       double l, f;
       cloud_pdf(q[k] + cl[k], qs, A, l, f);
       fc[k] = fmax(f, cover_of(cl[k], mu[k], liq[k], A, qs));
@@ -936,7 +1048,15 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   // ---- the surface: open water at Ts, or under ice thicker than ice_cover a skin balancing conduction through the
   // ice against the surface's fluxes, capped at freezing ----
   const double kex = rough * (A.bulk_c > 0 || sfc ? bulk : A.k_evap * (1 + uabs[i] / A.u0));
-  const double qa = qb;
+  // q_shape: the air at the surface holds the bottom layer's vapor at its relative humidity there (against the
+  // layer's capacity), not the layer's mean; bl_q keeps the layer's vapor even
+  double qa = qb;
+  if (MOIST && A.q_shape > 0 && !(A.bl_q > 0)) {
+    F qsl, dqsl;
+    layer_cap_k<L, MIX>(bot, th[bot], pb[bot - 1], pb[bot], P, A, qsl, dqsl);
+    const double qss = (double)qsat_a<F>((F)Tsa, (F)ps, (F)A.eps, 0.0);
+    if (qsl > 0) qa = fmin(qb * qss / qsl, qss);
+  }
   // soil and plants hold their water back: a surface resistance rs in series with the air's ra (land_rs; rs_max: the
   // stomata open with the sunlight, rs = land_rs (1 + f) / (land_rs / rs_max + f), f the sunlight reaching the ground
   // over rs_sun; lc rows 6, 7 the cover's own land_rs, rs_sun; its bare share, row 3's complement, has none);
@@ -1178,7 +1298,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       // more erodes over tau_erode (air.cu:438-450)
       F qs, dqs;
       double l, f, dl = 0;
-      layer_cap<MIX>(th[k], pt, pbk, P, A, qs, dqs);
+      layer_cap_k<L, MIX>(k, th[k], pt, pbk, P, A, qs, dqs);
       if (qs > 0) {
         cloud_pdf(q[k] + cl[k], qs, A, l, f);
         dl = l > cl[k] ? (l - cl[k]) / (1 + P.Lv / pib[k] * dqs * f) : -(cl[k] - l) * omexp<MIX>(dt / A.tau_erode);
@@ -1188,7 +1308,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       }
       dth[k] += P.Lv * dl / pib[k];
       dlat[k] += P.Lv * dl / pib[k];
-      layer_cap<MIX>(th[k], pt, pbk, P, A, qs, dqs);
+      layer_cap_k<L, MIX>(k, th[k], pt, pbk, P, A, qs, dqs);
       cloud_pdf(q[k] + cl[k], qs, A, l, f);
       f = fmax(f, cover_of(cl[k], mu[k], liq[k], A, qs));
       const double inc = f > 1e-6 ? cl[k] / f * fmax(1.0, mu[k] / A.mu_cloud) : 0.0, r = inc / A.q_auto;
@@ -1243,9 +1363,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       double pt = 0;
 #pragma unroll
       for (int j = 0; j < k; j++) pt += P.grav * mu[j];
-      const double dq = A.sat_layer > 0
-                            ? condense_layer<MIX>(tk, qk, pib[k], pt, pt + P.grav * mu[k], P, A)
-                            : condense<MIX>(tk, qk, pib[k], fpow_<MIX>(pib[k] / P.cp, 1 / P.kappa) * P.p0, P, A);
+      const double dq = condense_at<L, MIX>(k, tk, qk, pib[k], pt, mu[k], P, A);
       th[k] = tk;
       q[k] = qk;
       dth[k] += P.Lv * dq / pib[k];
@@ -1293,7 +1411,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     mu_lcl = lcl_mass<L, MIX>(mu, th[bot], q[bot], P, A);
     P.bl_room = fmax(0.0, mu_lcl - mu[bot]);
   }
-  crossing<L, MOIST, CLOUD>(P, mu, th, q, dth, et, eb, pib, wxo, cl);
+  crossing<L, MOIST, CLOUD, MIX>(P, mu, th, q, dth, et, eb, pib, wxo, cl, &A);
   // the mixed boundary layer: the surface's buoyancy flux sets its depth. Rising it entrains the air above at bl_ent
   // of that flux across the jump to it (Tennekes 1973), a slice from the layer's bottom edge; falling, it gives its
   // top back down to a thin night layer mu_night. bl_reach: dry convection first takes in the air above colder than
@@ -1319,7 +1437,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
                                 fmax(0.0, mu_lcl - mu[bot]));
           if (m > 0) {
             const double fm = m / mu[k], thl = spread ? e + fm * D : th[k];
-            const double qx = q[k];
+            const double qx = A.edge_vap > 0 ? edge_q<L, MIX>(k, false, mu, th, q, m, P, A) : q[k];
             move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, k, bot, m, thl, thl, qx, qx, cl);
 #pragma unroll
             for (int j = k; j < bot; j++) wxo[j] -= m;
@@ -1350,7 +1468,8 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         const bool spread = ka >= 1 && thka < tska && thka > eka;
         const double fm = fmin(1.0, m / mka);
         const double thc = spread ? eka + fm * (thka - eka) : thka;
-        move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, ka, bot, m, thc, thc, qka, qka, cl);
+        const double qx = MOIST && A.edge_vap > 0 ? edge_q<L, MIX>(ka, false, mu, th, q, m, P, A) : qka;
+        move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, ka, bot, m, thc, thc, qx, qx, cl);
 #pragma unroll
         for (int j = 0; j < bot; j++)
           if (j >= ka) wxo[j] -= m;
@@ -1358,14 +1477,14 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     }
     if (!(Bf > 0) && mu[bot] > A.mu_night) {
       const double m = fmin(-Bf * dt / (P.cp * jump), mu[bot] - A.mu_night);
-      const double qx = q[bot];
+      const double qx = MOIST && A.edge_vap > 0 ? edge_q<L, MIX>(bot, true, mu, th, q, m, P, A) : q[bot];
       move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cl);
       wxo[bot - 1] += m;
     }
     if (A.bl_lcl > 0) {
       const double m = mu[bot] - fmax(lcl_mass<L, MIX>(mu, th[bot], q[bot], P, A), A.mu_night);
       if (m > 0) {
-        const double qx = q[bot];
+        const double qx = MOIST && A.edge_vap > 0 ? edge_q<L, MIX>(bot, true, mu, th, q, m, P, A) : q[bot];
         move_mass<L, MOIST, CLOUD>(P, mu, th, q, pib, bot, bot - 1, m, th[bot], th[bot], qx, qx, cl);
         wxo[bot - 1] += m;
       }
@@ -1382,9 +1501,28 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     for (int k = 0; k < bot; k++) pi_ += P.grav * mu[k];
     const double th0 = th[bot];
     double q0 = q[bot];
-    if constexpr (CLOUD) {                           // plumes root in the cell's moist air: its moistest part
+    if (A.plume_root > 0) {
+      // plume_root: plumes root in the air at the surface, the layer's vapor at its relative humidity there (edge_q
+      // at its bottom edge); root_q: plus the thermals' excess b E / (rho w_m), b = 8.5, w_m = (u*^3 + 0.28 w*^3)^(1/3)
+      // (Holtslag & Boville 1993), up to the surface air's saturation; else with clouds its moistest part by the
+      // statistics, against the surface air's capacity
+      const double ps0 = pi_ + P.grav * mu[bot], Ts0 = th0 * fpow_<MIX>(ps0 / P.p0, P.kappa);
+      const double qss = Ts0 < 160 ? 0.0 : (double)qsat_a<F>((F)Ts0, (F)ps0, (F)A.eps, A.ice_air);
+      q0 = edge_q<L, MIX>(bot, false, mu, th, q, 0, P, A);
+      if (A.root_q > 0) {
+        const double rho = ps / (A.rd * Tsa), Ub = uabs[i] + P.bulk_gust;
+        const double us3 = sfc ? fpow_<MIX>(fmax(0.0, ch[i] / rho * Ub), 1.5) : 0.0;
+        const double Bk = fmax(0.0, (H + A.vf * P.cp * Tsa * E) / (rho * P.cp));
+        const double hb = A.rd * Tsa / P.grav * log(ps / fmax(ps - P.grav * mu[bot], 1.0));
+        const double wm = cbrt(us3 + 0.28 * P.grav / Tsa * Bk * hb);
+        const double ex = wm > 1e-3 ? 8.5 * fmax(0.0, E) / (rho * wm) : 0.0;
+        q0 += fmin(ex, fmax(0.0, qss - q0));
+      } else if (CLOUD) {
+        q0 = fmax(q0, fmin(qss, q0 + (1 - A.rh_crit) * qss));
+      }
+    } else if constexpr (CLOUD) {                    // plumes root in the cell's moist air: its moistest part
       F qs, dqs;
-      layer_cap<MIX>(th0, pi_, pi_ + P.grav * mu[bot], P, A, qs, dqs);
+      layer_cap_k<L, MIX>(bot, th0, pi_, pi_ + P.grav * mu[bot], P, A, qs, dqs);
       q0 = fmax(q0, fmin((double)qs, q0 + (1 - A.rh_crit) * qs));
     }
     double thp, qc, thp2, qc2;
@@ -1441,7 +1579,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 #pragma unroll
       for (int k = 0; k < bot; k++) pt_ += P.grav * mu[k];
       F qs, dqs;
-      layer_cap<MIX>(th[bot], pt_, pt_ + P.grav * mu[bot], P, A, qs, dqs);
+      layer_cap_k<L, MIX>(bot, th[bot], pt_, pt_ + P.grav * mu[bot], P, A, qs, dqs);
       const double rc_ = A.revap > 0 ? (isl ? A.revap_rhl : A.revap_rhs) : 1.0;
       if (rc_ * qs > q[bot]) {
         double ev;

@@ -389,6 +389,35 @@ def test_mixed_layer(L, mixed, opts):
             outliers=(2e-2 if mixed else 2e-3) if opts.get('bl_lcl', 0) > 0 and moist else 0.0)
 
 
+@pytest.mark.parametrize('L,mixed,opts', [
+    (3, False, dict(sat_prof=1.0, sat_base=1.0, q_shape=1.0)),
+    (3, False, dict(edge_vap=1.0, freeze_dry=1.0, plume_root=1.0)),
+    (5, False, dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, edge_k=2.0, plume_root=1.0, root_q=1.0, sfc=1.0)),
+    (4, False, dict(sat_prof=1.0, edge_vap=1.0, q_shape=1.0, plume_root=1.0, cloud=1.0, bl_mix=1.0, bl_reach=1.0)),
+    (3, True, dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, q_shape=1.0, plume_root=1.0)),
+    (7, True, dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, edge_k=2.0, plume_root=1.0, cloud=1.0))])
+def test_earth_options(L, mixed, opts):
+    """The earth preset's moist options (the spread layers' capacity, the third layer's base condensation, edge vapor,
+    freeze-drying, the surface air's vapor, surface-rooted plumes with the thermals' excess) alone and together, with
+    clouds and the mixed layer, to the moist column's bars."""
+    from oracle import column as ocol
+    opts = dict(opts)
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column earth options L={L} mixed={mixed} {opts} N={N}', flush=True)
+    cloudy = opts.get('cloud', 0) > 0
+    c = cloudy_columns(N, L, seed=140 + L) if cloudy else moist_columns(N, L, seed=140 + L, polar=30.0)
+    kw = dict(ocean=True, q=c['q'], ice=c['ice'])
+    if cloudy:
+        kw.update(cw=c['cw'], cloudo=True)
+    if opts.get('sfc'):
+        kw['ch'] = np.random.default_rng(150 + L).uniform(1e-3, 3e-2, N)
+    p = par(L, moist=1.0, **opts)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, **kw)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed, **kw)
+    compare(g, o, 1e-5 if mixed else 1e-12, dt, f'earth options L={L} mixed={mixed}',
+            bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed and cloudy else None)
+
+
 def test_aqua_dry_world():
     """The 'aqua_dry' world (three dry air layers over the slab, the sun, gray radiation, the column and the exchange
     every tick) against the oracle's tick, 12 ticks at level 4 in the FP64 form (<= 1e-10, as the core's own
