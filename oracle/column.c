@@ -17,7 +17,8 @@ enum { P_GRAV = 13, P_CP = 14, P_KAPPA = 15, P_P0 = 16, P_FLOOR = 11, P_THETA_ST
        P_MU_BL = 9, P_BL_MIX = 70, P_BL_CAP = 73,
        P_RHO_ICE = 29, P_ICE_COVER = 30, P_BULK_GUST = 75, P_Q_SEA = 97, P_Z0_LAND = 91, P_THS = 132,
        P_EDGE_VAP = 86, P_FREEZE_DRY = 85, P_EDGE_K = 101, P_ICE_AIR = 102, P_SAT_PROF = 51, P_SAT_BASE = 53,
-       P_BL_Q = 89, P_Q_SHAPE = 83, P_PLUME_ROOT = 88, P_ROOT_Q = 107 };
+       P_BL_Q = 89, P_Q_SHAPE = 83, P_PLUME_ROOT = 88, P_ROOT_Q = 107, P_PLUME_DEEP = 47, P_PLUME_ENT = 78,
+       P_ENT_CLOUD = 93, P_ENT_DET = 108, P_L_DET = 65, P_STORM_AREA = 80, P_GRAUPEL_V = 81, P_ICE_SMALL = 82 };
 
 static void layer_exner(int L, const double* mu, const double* par, double* pb, double* pib) {
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP];
@@ -681,12 +682,14 @@ static double cloud_adjust(int L, int k, const double* mu, double* th, double* q
   return dl;
 }
 
-/* base_T (air.cu:476-482): the temperature at layer k's lower edge (its theta there, sat_prof = 0). */
-static double base_T(int k, const double* mu, const double* th, const double* par, int mix) {
+/* base_T (air.cu:476-482): the temperature at layer k's lower edge: its theta there, or for a spread middle layer
+ * (sat_prof) its lower theta*. */
+static double base_T(int L, int k, const double* mu, const double* th, const double* par, int mix) {
   double pt = 0;
   for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
   const double pb = pt + par[P_GRAV] * mu[k];
-  return th[k] * fpow_(pb / par[P_P0], par[P_KAPPA], mix);
+  const int spread = par[P_SAT_PROF] > 0 && k >= (int)par[P_EDGE_K] && k < L - 1 && th[k] < tstar(k - 1, L, par);
+  return (spread ? tstar(k, L, par) : th[k]) * fpow_(pb / par[P_P0], par[P_KAPPA], mix);
 }
 
 /* lcl_mass (air.cu:1413-1421): the mixed bottom layer's mass up to the level where its air, lifted
@@ -708,6 +711,146 @@ static double lcl_mass(int L, const double* mu, const double* th, const double* 
   }
 #undef SAT
   return (ps_ - 0.5 * (lo + hi)) / par[P_GRAV];
+}
+
+/* The liquid share of condensate at T (air.cu:160): 1 at freezing, 0 at -20 C (float32 in the mixed build). */
+static double liq_of(double T, int mix) {
+  if (mix) return fminf(1.0f, fmaxf(0.0f, ((float)T - 253.15f) / 20.0f));
+  return fmin(1.0, fmax(0.0, (T - 253.15) / 20.0));
+}
+
+/* plume_ascent (air.cu:540-666, the deep plume's options plume_ent, ent_cloud, ent_det, ice_air): bottom air (th0,
+ * q0) lifted from the bottom layer's top (pi_) up through the layers, in M sub-levels each, condensing as it goes and
+ * mixing with (plume_ent) and, through the layer above the bottom one, entraining (ent_cloud, its mass growing as
+ * exp(ent_cloud z); ent_det: as much detrained) the air it passes, whose theta at mass share x above the layer's
+ * lower edge a is a + (b - a) ln(1 + x (e^s - 1)) / s. A: its buoyancy summed by mass up to the highest level it
+ * beats (the bottom layer's stretch a triangle); kd, thd, qd: the layer it detrains into, its theta and vapor there;
+ * qx: what its mixing gave each layer per kg; G its mass per kg leaving the bottom layer; FZ its frozen condensate
+ * (ice_air); lt: its condensate freezing between 0 and -40 C, those levels' pressures and its CAPE, for lightning. */
+static void plume_ascent(int L, const double* mu, const double* th, double th0, double q0, double pi_,
+                         const double* par, const og_air* air, int mix, double* A, int* kd, double* thd, double* qd,
+                         const double* q, double* qx, double* lt, double* G, double* FZ) {
+  const int bot = L - 1, M = bot > 2 ? 3 : 8;
+  const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP], Lf = par[P_LF];
+  const double Tf = par[P_TF], rd = air->rd, eps = air->eps;
+  double Mp = 1, tp = th0, qp = q0, fz = 0;
+#define LIFT(p_, ex_)                                                                \
+  do {                                                                               \
+    const double q_ = qp;                                                            \
+    ex_ = fpow_((p_) / p0, kappa, mix);                                              \
+    condense(&tp, &qp, cp * ex_, (p_), par, eps, mix);                               \
+    if (par[P_ICE_AIR] > 0 && q_ > qp) {                                             \
+      const double z_ = (q_ - qp) * (1 - liq_of(tp * ex_, mix));                     \
+      tp += Lf * z_ / (cp * ex_);                                                    \
+      fz += Mp * z_;                                                                 \
+    }                                                                                \
+  } while (0)
+  double ex0;
+  LIFT(pi_, ex0);
+  double sum = 0.5 * mu[bot] * (tp - tstar(bot - 1, L, par)), pt = pi_;
+  *A = 0;
+  *kd = -1;
+  *thd = tp;
+  *qd = qp;
+  double cape = 0, q0c = -1, q40 = -1, p0c = 0, p40 = 0, qa[OG_LMAX];
+  for (int k = 0; k <= bot; k++) {
+    qa[k] = 0;
+    qx[k] = 0;
+  }
+  for (int k = bot - 1; k >= 0; k--) {
+    const double a = tstar(k, L, par), b = k > 0 ? tstar(k - 1, L, par) : 0.0;
+    const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !mix ? expm1(s) : 0.0;
+    const int ek = k > 0 && k == bot - 1;
+    const double f = par[P_PLUME_ENT] > 0 ? omexp(par[P_PLUME_ENT] * grav * mu[k] / M, mix) : 0.0;
+    int last = 0;
+    for (int j = 0; j < M; j++) {
+      const double x = (j + 0.5) / M, pl = pt - x * grav * mu[k];
+      double ex;
+      LIFT(pl, ex);
+      const double te = k == 0 ? th[0] : a + (b - a) * spread_of(x, s, es, mix);
+      if (par[P_ENT_CLOUD] > 0 && ek) {
+        const double dz = rd * te * ex / grav * (grav * mu[k] / M) / pl;
+        const double g_ = expm1(par[P_ENT_CLOUD] * dz), fe = g_ / (1 + g_);
+        if (par[P_ENT_DET] > 0) qa[k] += Mp * fe * (qp - q[k]);
+        tp += fe * (te - tp);
+        qp += fe * (q[k] - qp);
+        if (!(par[P_ENT_DET] > 0)) Mp *= 1 + g_;
+      }
+      if (par[P_PLUME_ENT] > 0) {
+        qa[k] += Mp * f * (qp - q[k]);
+        tp += f * (te - tp);
+        qp += f * (q[k] - qp);
+      }
+      sum += mu[k] / M * (tp - te);
+      last = tp > te;
+      {
+        const double Tp = tp * ex;
+        if (tp > te) cape += rd * (tp - te) * ex * grav * mu[k] / M / pl;
+        if (q0c < 0 && Tp < Tf) {
+          q0c = qp;
+          p0c = pl;
+        }
+        if (q40 < 0 && Tp < Tf - 40) {
+          q40 = qp;
+          p40 = pl;
+        }
+      }
+      if (last) {
+        *A = sum;
+        *kd = k;
+        *thd = tp;
+        *qd = qp;
+        *G = Mp;
+        *FZ = fz;
+        for (int m = 0; m <= bot; m++) qx[m] = qa[m];
+      }
+    }
+    if (!last || k == 0) break;
+    pt -= grav * mu[k];
+    double ext;
+    LIFT(pt, ext);
+    if (!(tp > b)) break;
+  }
+#undef LIFT
+  if (q0c >= 0 && q40 < 0) {
+    q40 = *qd;
+    p40 = pt;
+  }
+  lt[0] = q0c >= 0 ? fmax(0.0, q0c - q40) : 0.0;
+  lt[1] = q0c >= 0 ? p0c : 0.0;
+  lt[2] = q0c >= 0 ? p40 : 0.0;
+  lt[3] = cape;
+}
+
+/* storm_flashes (air.cu:718-729): lightning from one storm's physics (Deierling et al. 2008): 9.0e-15 f_p f_np
+ * flashes per minute, f_p the falling graupel's mass flux and f_np the rising small ice's, the plume's ice flux
+ * shared among storms of updraft area storm_area rising at half sqrt(2 CAPE); only an updraft faster than graupel
+ * falls holds it up to charge. Flashes per second per m^2. */
+static double storm_flashes(double mflux, const double* lt, const double* par, const og_air* air, int mix) {
+  const double qi = lt[0], w = 0.5 * sqrt(2 * fmax(0.0, lt[3]));
+  const double gv = par[P_GRAUPEL_V], Tf = par[P_TF], rd = air->rd;
+  if (!(mflux > 0) || !(qi > 0) || !(w > gv)) return 0;
+  const double rho = lt[1] / (rd * Tf), phi = rho * w * par[P_STORM_AREA], lift = 1 - gv * gv / (w * w);
+  const double lg = mix ? (double)logf((float)(lt[1] / lt[2])) : log(lt[1] / lt[2]);
+  const double H = rd * (Tf - 20) / par[P_GRAV] * lg;
+  const double up = phi * qi * lift, fnp = up * par[P_ICE_SMALL], fp = up * (1 - par[P_ICE_SMALL]) * H / w * gv;
+  return mflux / phi * 9.0e-15 * fp * fnp / 60;
+}
+
+/* wet_bulb (air.cu:453-466): air (th, q) at pressure p evaporating water until saturated, three Newton steps with
+ * Tetens' slope. */
+static void wet_bulb(double* th, double* q, double p, const double* par, double eps, int mix) {
+  const double ex = fpow_(p / par[P_P0], par[P_KAPPA], mix);
+  for (int it = 0; it < 3; it++) {
+    const double T = *th * ex;
+    if (T < 160) return;
+    const double qs = qsat_x(T, p, eps, par[P_ICE_AIR], mix), d = qs - *q;
+    if (!(d > 0)) return;
+    const double dqs = qs * 17.27 * (273.15 - 35.85) / ((T - 35.85) * (T - 35.85));
+    const double e = d / (1 + par[P_LV] / par[P_CP] * dqs);
+    *q += e;
+    *th -= par[P_LV] * e / (par[P_CP] * ex);
+  }
 }
 
 void og_air_column(int N, int L, const double* par_in, const og_air* air, int mode, const double* t,
@@ -1158,7 +1301,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           th[k] += Lf * dz / pib[k];
           dth[k] += Lf * dz / pib[k];
           dlat[k] += Lf * dz / pib[k];
-          if (Fs > 0 && base_T(k, mu, th, par, mix) > Tf) {
+          if (Fs > 0 && base_T(L, k, mu, th, par, mix) > Tf) {
             const double dm = Fs * dt / (mu[k] * pib[k]) * Lf;
             th[k] -= dm;
             dth[k] -= dm;
@@ -1296,7 +1439,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     /* moist convection: a parcel of bottom-layer air lifted to the interface above it; buoyant there (theta_p above
      * theta*), it rises through every interface it stays buoyant at and detrains above the last, its condensate
      * raining; the mass that would remove the buoyancy at the first interface relaxes over tau_conv */
-    double mc = 0;
+    double mc = 0, Scg = 0;
     if (flash) flash[i] = 0;
     if (moist && par[P_THETA_STAR] > 0) {
       double pi_ = 0;
@@ -1328,6 +1471,81 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         q0 = fmax(q0, fmin(qs, q0 + (1 - par[P_RH_CRIT]) * qs));
       }
       double thp, qc, thp2, qc2;
+      if (par[P_PLUME_DEEP] > 0 && bot >= 2) {
+        /* the deep plume: its full ascent (plume_ascent) against the layers' spread air, mixing and entraining; the
+         * air it rises through sinks to make room, warming each level by the theta it had below, so the mass that
+         * removes its buoyancy is A / (theta_d - theta*), relaxed over tau_conv; it takes in the cloud layer's air it
+         * entrained (me), keeps l_det of its condensate as cloud, rains the rest (snow by its frozen share), and its
+         * rain feeds a downdraft of the cloud layer's air, saturated by evaporating it */
+        double A, G = 1, FZ = 0, qx[OG_LMAX], lt[4], xs = 0;
+        int kd;
+        plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thp, &qc, q, qx, lt, &G, &FZ);
+        for (int k = 0; k <= bot; k++) xs += qx[k];
+        if (kd >= 0 && A > 0) {
+          const double rise = thp - tstar(bot - 1, L, par);
+          mc = fmin(A / (rise > 1 ? rise : 1.0), mu[bot] - par[P_FLOOR]) * omexp(dt / par[P_TAU_CONV], mix);
+          if (par[P_PLUME_ROOT] > 0 && q0 > 0) mc = fmin(mc, mu[bot] * q[bot] / q0);
+          if (mc > 0) {
+            if (!(kd < bot - 1)) G = 1;
+            const double me = fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - par[P_FLOOR]));
+            const double qe = q[bot - 1], the = th[bot - 1];
+            G = 1 + me / mc;
+            const double cond_ = q0 + (G - 1) * qe - G * qc - xs;
+            const double cond = fmax(0.0, cond_), det = cloudy ? fmin(cond, par[P_L_DET] * G) : 0.0;
+            if (flash) flash[i] = storm_flashes(mc / dt, lt, par, air, mix);
+            R += mc * (cond - det);
+            Rc = mc * (cond - det);
+            const double qarr = qc + (cond_ - cond) / G;
+            const double Ec = column_energy(L, mu, pib, th, q, par);
+            for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + mc * qx[k] / mu[k]);
+            move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qarr, cloudy ? cl : NULL);
+            if (me > 0) move_mass(L, par, mu, th, q, pib, bot - 1, kd, me, the, thp, qe, qarr, cloudy ? cl : NULL);
+            th[kd] += (Ec - column_energy(L, mu, pib, th, q, par)) / (mu[kd] * pib[kd]);
+            double Sc = 0;
+            if (par[P_ICE_AIR] > 0 && cond > 0 && cond > det) {   /* (its frozen share snows, melting below Tf) */
+              Sc = mc * (cond - det) * fmin(1.0, FZ / cond);
+              th[kd] += Lf * Sc / (mu[kd] * pib[kd]);
+              for (int j = kd + 1; j <= bot && Sc > 0; j++)
+                if (base_T(L, j, mu, th, par, mix) > Tf) {
+                  th[j] -= Lf * Sc / (mu[j] * pib[j]);
+                  Sc = 0;
+                }
+            }
+            for (int j = kd; j < bot; j++) wxo[j] += mc;
+            for (int j = kd; j < bot - 1; j++) wxo[j] += me;
+            if (cloudy) {
+              cl[kd] += mc * det / mu[kd];
+              cloud_adjust(L, kd, mu, th, q, cl, pib, dt, par, aeps, mix);
+            } else {
+              const double rt = mu[kd] * condense_k(L, kd, mu, th, q, pib, par, aeps, mix);
+              R += rt;
+              Rc += rt;
+            }
+            if (par[P_DOWNDRAFT] > 0 && bot >= 2 && Rc - Sc > 0) {
+              const int s_ = bot - 1;
+              double pt_ = 0;
+              for (int k = 0; k < s_; k++) pt_ += grav * mu[k];
+              const double p1 = pt_ + 0.5 * grav * mu[s_], p2 = pt_ + grav * (mu[s_] + mu[bot]);
+              double thd_ = th[s_], qd_ = q[s_];
+              for (int j = 0; j <= 4; j++) wet_bulb(&thd_, &qd_, p1 + (p2 - p1) * j / 4, par, aeps, mix);
+              const double ev = qd_ - q[s_];
+              double md = fmin(par[P_DOWNDRAFT] * mc, fmax(0.0, mu[s_] - par[P_FLOOR]));
+              if (ev > 0) md = fmin(md, (Rc - Sc) / ev);
+              if (md > 0 && thd_ < th[s_]) {
+                move_mass(L, par, mu, th, q, pib, s_, bot, md, th[s_], thd_, q[s_], qd_, cloudy ? cl : NULL);
+                for (int j = s_; j < bot; j++) wxo[j] -= md;
+                const double e_ = md * fmax(0.0, ev);
+                R -= e_;
+                Rc -= e_;
+              }
+            }
+            S += Sc;
+            Scg = Sc;
+          } else {
+            mc = 0;
+          }
+        }
+      } else {
       parcel(th0, q0, pi_, par, aeps, mix, &thp, &qc);
       const double d = thp - tstar(bot - 1, L, par);
       if (d > 0) {
@@ -1360,17 +1578,18 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           mc = 0;
         }
       }
+      }
     }
     /* the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the
-     * convection's area revap_area; else on conv_revap of the cell) */
-    if (cloudy && (par[P_REVAP] > 0 ? Rc > 0 : par[P_CONV_REVAP] > 0 && !(par[P_DOWNDRAFT] > 0) && Rc > 0) && L > 1) {
+     * convection's area revap_area; else on conv_revap of the cell; with downdrafts they evaporate it instead) */
+    if (cloudy && (par[P_REVAP] > 0 ? Rc - Scg > 0 : par[P_CONV_REVAP] > 0 && !(par[P_DOWNDRAFT] > 0) && Rc > 0) && L > 1) {
       double qs, dqs;
       layer_cap(L, bot, mu, th[bot], par, aeps, mix, &qs, &dqs);
       const double rc_ = par[P_REVAP] > 0 ? (isl ? par[P_REVAP_RHL] : par[P_REVAP_RHS]) : 1.0;
       if (rc_ * qs > q[bot]) {
         double ev;
         if (par[P_REVAP] > 0) {
-          const double Fc = Rc / dt;
+          const double Fc = (Rc - Scg) / dt;
           double pt_ = 0;
           for (int k = 0; k < bot; k++) pt_ += grav * mu[k];
           const double pm = pt_ + 0.5 * grav * mu[bot], psf = pt_ + grav * mu[bot];

@@ -901,3 +901,79 @@ def test_freeze_dry_closed_form():
     assert np.all(qsi[up] < o['q'][1][up])
     top = (c['mu'][0] * c['q'][0] + wx * qsi) / (c['mu'][0] + wx)
     np.testing.assert_allclose(o['q'][0][up], top[up], rtol=1e-12)
+
+
+# ---- the deep plume (plume_deep: air.cu plume_ascent with plume_ent, ent_cloud, ent_det, l_det, ice_air, lightning,
+# downdrafts) ----
+
+DEEP_OPTS = [dict(), dict(plume_ent=1e-5), dict(ent_cloud=1e-4), dict(ent_cloud=1e-4, ent_det=1.0),
+             dict(plume_ent=1e-5, ent_cloud=1e-4, cloud=1.0), dict(ice_air=1.0, cloud=1.0, downdraft=0.3),
+             dict(plume_root=1.0, plume_ent=1e-5, downdraft=0.3, revap=1.0, cloud=1.0),
+             dict(plume_root=1.0, root_q=1.0, sfc=1.0, ent_cloud=1e-4, sat_prof=1.0, edge_vap=1.0)]
+
+
+def deep_columns(N=96, L=4, seed=5):
+    """Four layers (interfaces near 150, 400 and 800 hPa), a warm moist bottom layer under a conditionally unstable
+    column, cold enough aloft for ice."""
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(size=(N, 3))
+    pos /= np.linalg.norm(pos, axis=1)[:, None]
+    edges = np.array([0.0, 0.15e5, 0.40e5, 0.80e5, 1.0e5])[:, None] * rng.uniform(0.97, 1.03, (1, N))
+    mu = np.diff(edges, axis=0) / GRAV
+    theta = np.stack([np.full(N, 420.0), np.full(N, 345.0), np.full(N, 318.0), np.full(N, 303.0)])
+    theta += rng.normal(0, 1.0, (4, N))
+    pt, pb, _ = pib_of(mu)
+    pm = 0.5 * (pt + pb)
+    q = np.array([0.01, 0.3, 0.7, 0.9])[:, None] * tetens(theta * (pm / P0) ** KAPPA, pm)
+    Ts = 302.0 + rng.normal(0, 1.0, N)
+    return dict(pos=pos, mu=mu, theta=theta, Ts=Ts, uabs=rng.uniform(0, 15, N), q=q, ice=np.zeros(N))
+
+
+def par4(**kw):
+    from insilica.atmosphere import par_array
+    return par_array(4, dict(dict(theta_stars=[380.0, 330.0, 305.0], edge_k=1.0), **kw))
+
+
+@pytest.mark.parametrize('opts', DEEP_OPTS)
+@pytest.mark.parametrize('mix', [False, True])
+def test_deep_plume_closes(opts, mix):
+    """The deep plume moves air, vapor and cloud water between the layers, rains its condensate (snow by its frozen
+    share), keeps l_det of it as cloud, and its downdraft evaporates rain into the bottom layer: the energy closes,
+    the water changes by evaporation less rain (cloud water included: the plume's air takes its cloud water along)."""
+    c = deep_columns()
+    N = c['Ts'].size
+    cloudy = opts.get('cloud', 0) > 0
+    kw = dict(ch=np.full(N, 0.02)) if opts.get('sfc') else {}
+    if cloudy:
+        c['cw'] = np.full_like(c['q'], 5e-5)
+        kw['cw'] = c['cw']
+    p = par4(moist=1.0, plume_deep=1.0, **opts)
+    dt = 900.0
+    o = run(c, p, q=c['q'], ice=c['ice'], dt=dt, mix=mix, **kw)
+    stored_matches(o, dt)
+    mu1 = new_masses(c, o['wx'])
+    W0 = (c['mu'] * (c['q'] + (c['cw'] if cloudy else 0))).sum(0)
+    W1 = (mu1 * (o['q'] + (o['cw'] if cloudy else 0))).sum(0)
+    np.testing.assert_allclose(W1 - W0, (o['evap'] - o['rain']) * dt, atol=1e-11 * W0.max())
+    assert o['conv'].max() > 0 and o['rainp'][2].max() > 0 and np.all(mu1 >= 10.0 * (1 - 1e-9))
+    assert np.all(o['flash'] >= 0)
+
+
+def test_deep_plume_lightning_and_downdraft():
+    """Lightning needs an updraft (half sqrt(2 CAPE)) faster than graupel falls: none with graupel_v beyond every
+    storm's, some at 1 m/s, and the flash rate grows as the graupel's fall speed's share of the updraft shrinks
+    ((1 - v^2 / w^2) v per storm); a downdraft carries the cloud layer's air down into the bottom layer, moister than
+    it came: the rain it evaporated stays in the column's vapor, which gains exactly what the rain lost."""
+    c = deep_columns()
+    c['cw'] = np.full_like(c['q'], 5e-5)
+    fl = {v: run(c, par4(moist=1.0, plume_deep=1.0, cloud=1.0, graupel_v=v), q=c['q'], ice=c['ice'], cw=c['cw'],
+                 dt=900.0)['flash'] for v in (1e3, 1.0)}
+    assert np.all(fl[1e3] == 0) and fl[1.0].max() > 0
+    d = run(c, par4(moist=1.0, plume_deep=1.0, downdraft=0.3), q=c['q'], ice=c['ice'], dt=900.0)
+    n = run(c, par4(moist=1.0, plume_deep=1.0), q=c['q'], ice=c['ice'], dt=900.0)
+    go = n['conv'] > 0
+    assert go.sum() > 10
+    assert np.all(d['rain'][go] <= n['rain'][go] + 1e-15) and (d['rain'][go] < n['rain'][go]).sum() > go.sum() // 2
+    Wd = (new_masses(c, d['wx']) * d['q']).sum(0)
+    Wn = (new_masses(c, n['wx']) * n['q']).sum(0)
+    np.testing.assert_allclose((Wd - Wn)[go], ((n['rain'] - d['rain']) * 900.0)[go], rtol=1e-9, atol=1e-12)
