@@ -257,7 +257,9 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * slab surface at Ts (N; heat capacity cap (N) when given, else c_slab) under the bulk exchange or the surface
  * layer's rho C_H U (ch (N), with sfc > 0); the heating as theta and the crossing of the interfaces (the k-tables'
  * edge heating). With moist > 0 also the vapor qv (L, N) and sea ice ice (N, kg/m^2): evaporation, the ice's skin
- * and albedo, freezing and melting, condensation per layer, the parcel plume of moist convection (plume_deep = 0),
+ * and albedo, freezing and melting, condensation per layer, moist convection (the parcel plume; with plume_deep > 0
+ * the deep plume's full ascent, mixing (plume_ent) and entraining the cloud layer's air (ent_cloud, ent_det), its
+ * detrained condensate as cloud (l_det), frozen share (ice_air), lightning into flash and downdrafts (downdraft)),
  * and the vapor's longwave part with lw_w0. With clouds (moist, cloud > 0 and the cloud water cloudw (L, N) given,
  * updated in place): the cloud statistics and erosion, Sundqvist's autoconversion, the ice's fall (ice_air: its
  * freezing, melting and snow), the rain's evaporation (k_revap, or Kessler's with revap; conv_revap below the plumes),
@@ -284,8 +286,9 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * qocean, qsw, flash, swalb and firn (the firn's temperature) when given; cap (N) the surface's heat capacity, else
  * c_slab. The argument list is the reference's; elev (N) with theta_rise > 0 raises the lowest interface over high
  * ground. Returns ISW_EBADARG for cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or
- * ice, with land but without soilcap, soil, pond or snow, with plume_deep (the deep and shallow plume family), and
- * with clouds and bl_q: those parts of the reference's column are not in this library.
+ * ice, with land but without soilcap, soil, pond or snow, with clouds and bl_q, and with plume_deep and any of
+ * ent_ifs, ent_loc, det_all, shallow, cu_dth, buoy_v, cu_sub or bl_q: those parts of the reference's column are not
+ * in this library.
  * prec: ISW_FP64, or ISW_MIXED (the reference's mixed build: 1 - e^-x, the edge sub-levels' Exner powers and the
  * saturation and condensation arithmetic in float32). */
 int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* air, int32_t mode, const double* t,

@@ -129,6 +129,8 @@ struct AP {
   // the spread layers' capacity, the base condensation, edge vapor, freeze-drying, the surface air's vapor, the
   // plumes' root
   double sat_prof, sat_base, edge_vap, freeze_dry, q_shape, plume_root, root_q, bl_q;
+  // the deep plume's
+  double plume_deep, plume_ent, ent_cloud, ent_det, l_det, storm_area, graupel_v, ice_small;
 };
 
 static AP make_ap(const double* a, const isw_air* air) {
@@ -152,6 +154,8 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.mu_night = a[71]; p.bl_ent = a[72]; p.bl_reach = a[98]; p.bl_lcl = a[100]; p.ent_lcl = a[130]; p.vf = air->vf;
   p.sat_prof = a[51]; p.sat_base = a[53]; p.edge_vap = a[86]; p.freeze_dry = a[85]; p.q_shape = a[83];
   p.plume_root = a[88]; p.root_q = a[107]; p.bl_q = a[89];
+  p.plume_deep = a[47]; p.plume_ent = a[78]; p.ent_cloud = a[93]; p.ent_det = a[108]; p.l_det = a[65];
+  p.storm_area = a[80]; p.graupel_v = a[81]; p.ice_small = a[82];
   return p;
 }
 
@@ -323,6 +327,14 @@ __device__ double condense_k(int k, const double* mu, double* th, double* q, con
   for (int j = 0; j < L; j++)
     if (j == k) { th[j] = tk; q[j] = qk; }
   return dq;
+}
+
+// The temperature at layer k's lower edge (pressure pb) (air.cu:476-482): its theta there, or for a spread middle
+// layer (sat_prof) its lower theta*.
+template <int L, bool MIX>
+__device__ __forceinline__ double base_T(int k, double th, double pb, const CP& P, const AP& A) {
+  double ea, eb;
+  return (spread_layer<L>(k, th, P, A, ea, eb) ? ea : th) * fpow_<MIX>(pb / P.p0, P.kappa);
 }
 
 // The vapor of layer k's air at its top edge (top) or bottom edge (air.cu:361-378): its relative humidity against
@@ -788,6 +800,179 @@ __device__ __forceinline__ void cloud_pdf(double qt, double qs, const AP& A, dou
 __device__ __forceinline__ double cover_of(double cw, double mu, double liq, const AP& A, double qs) {
   const double s = A.cw_qs > 0 && qs > 0 ? fmin(1.0, A.cw_qs * qs / A.q_auto) : 1.0;
   return fmin(1.0, cw / (s * (liq * A.q_auto + (1 - liq) * A.l_ice)) * fmax(1.0, mu / A.mu_cloud));
+}
+
+// Cloud water in layer k (a runtime index) condensing up to the cloud statistics' condensate with the latent heat's
+// feedback on the capacity, or eroding over tau_erode (air.cu:438-450); returns the condensate.
+template <int L, bool MIX>
+__device__ double cloud_adjust_k(int k, const double* mu, double* th, double* q, double* cl, const double* pib,
+                                 double dt, const CP& P, const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  double pt = 0, mk = 0, tk = 0, qk = 0, ck = 0, pk = 0;
+#pragma unroll
+  for (int j = 0; j < L; j++) {
+    if (j < k) pt += P.grav * mu[j];
+    if (j == k) { mk = mu[j]; tk = th[j]; qk = q[j]; ck = cl[j]; pk = pib[j]; }
+  }
+  F qs, dqs;
+  layer_cap_k<L, MIX>(k, tk, pt, pt + P.grav * mk, P, A, qs, dqs);
+  if (!(qs > 0)) return 0;
+  double l, f;
+  cloud_pdf(qk + ck, qs, A, l, f);
+  const double dl = l > ck ? (l - ck) / (1 + P.Lv / pk * dqs * f) : -(ck - l) * omexp<MIX>(dt / A.tau_erode);
+#pragma unroll
+  for (int j = 0; j < L; j++)
+    if (j == k) {
+      cl[j] = ck + dl;
+      q[j] = qk - dl;
+      th[j] = tk + P.Lv * dl / pk;
+    }
+  return dl;
+}
+
+// The liquid share of condensate at T (air.cu:160): 1 at freezing, 0 at -20 C, in the build's `fast` type.
+template <bool MIX>
+__device__ __forceinline__ double liq_of(double T) {
+  if constexpr (MIX) return fminf(1.0f, fmaxf(0.0f, ((float)T - 253.15f) / 20.0f));
+  else return fmin(1.0, fmax(0.0, (T - 253.15) / 20.0));
+}
+
+// The deep plume's full ascent (air.cu:540-666; its options plume_ent, ent_cloud, ent_det, ice_air): bottom air (th0,
+// q0) lifted from the bottom layer's top (pi_) up through the layers in M sub-levels each, condensing as it goes,
+// mixing with (plume_ent) and, through the layer above the bottom one, entraining (ent_cloud: its mass growing as
+// exp(ent_cloud z); ent_det: as much detrained) the air it passes, whose theta at mass share x above the layer's
+// lower edge a is a + (b - a) ln(1 + x (e^s - 1)) / s. A: its buoyancy summed by mass up to the highest level it beats
+// (the bottom layer's stretch a triangle); kd, thd, qd: the layer it detrains into, its theta and vapor there; qx: what
+// its mixing gave each layer per kg; G: its mass per kg leaving the bottom layer; FZ: its frozen condensate; lt: its
+// condensate freezing between 0 and -40 C, those levels' pressures and its CAPE (for lightning). The layer loop is
+// unrolled with a stop flag, so every per-layer value stays in registers.
+template <int L, bool MIX>
+__device__ void plume_ascent(const double* mu, const double* th, const double* q, double th0, double q0, double pi_,
+                             const CP& P, const AP& A, double& Ab, int& kd, double& thd, double& qd, double* qx,
+                             double* lt, double& G, double& FZ) {
+  constexpr int bot = L - 1, M = bot > 2 ? 3 : 8;
+  double Mp = 1, tp = th0, qp = q0, fz = 0;
+  auto lift = [&](double p_) {
+    const double q_ = qp, ex_ = fpow_<MIX>(p_ / P.p0, P.kappa);
+    condense<MIX>(tp, qp, P.cp * ex_, p_, P, A);
+    if (A.ice_air > 0 && q_ > qp) {
+      const double z_ = (q_ - qp) * (1 - liq_of<MIX>(tp * ex_));
+      tp += A.Lf * z_ / (P.cp * ex_);
+      fz += Mp * z_;
+    }
+    return ex_;
+  };
+  lift(pi_);
+  double sum = 0.5 * mu[bot] * (tp - tstar<L>(bot - 1, P)), pt = pi_;
+  Ab = 0;
+  kd = -1;
+  thd = tp;
+  qd = qp;
+  double cape = 0, q0c = -1, q40 = -1, p0c = 0, p40 = 0, qa[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    qa[k] = 0;
+    qx[k] = 0;
+  }
+  bool stop = false;
+#pragma unroll
+  for (int k = bot - 1; k >= 0; k--) {
+    if (stop) continue;
+    const double a = tstar<L>(k, P), b = k > 0 ? tstar<L>(k - 1, P) : 0.0;
+    const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !MIX ? expm1(s) : 0.0;
+    const bool ek = k > 0 && k == bot - 1;
+    const double f = A.plume_ent > 0 ? omexp<MIX>(A.plume_ent * P.grav * mu[k] / M) : 0.0;
+    bool last = false;
+    for (int j = 0; j < M; j++) {
+      const double x = (j + 0.5) / M, pl = pt - x * P.grav * mu[k];
+      const double ex = lift(pl);
+      const double te = k == 0 ? th[0] : a + (b - a) * spread_of<MIX>(x, s, es);
+      if (A.ent_cloud > 0 && ek) {
+        const double dz = A.rd * te * ex / P.grav * (P.grav * mu[k] / M) / pl;
+        const double g_ = expm1(A.ent_cloud * dz), fe = g_ / (1 + g_);
+        if (A.ent_det > 0) qa[k] += Mp * fe * (qp - q[k]);
+        tp += fe * (te - tp);
+        qp += fe * (q[k] - qp);
+        if (!(A.ent_det > 0)) Mp *= 1 + g_;
+      }
+      if (A.plume_ent > 0) {
+        qa[k] += Mp * f * (qp - q[k]);
+        tp += f * (te - tp);
+        qp += f * (q[k] - qp);
+      }
+      sum += mu[k] / M * (tp - te);
+      last = tp > te;
+      const double Tp = tp * ex;
+      if (tp > te) cape += A.rd * (tp - te) * ex * P.grav * mu[k] / M / pl;
+      if (q0c < 0 && Tp < A.Tf) {
+        q0c = qp;
+        p0c = pl;
+      }
+      if (q40 < 0 && Tp < A.Tf - 40) {
+        q40 = qp;
+        p40 = pl;
+      }
+      if (last) {
+        Ab = sum;
+        kd = k;
+        thd = tp;
+        qd = qp;
+        G = Mp;
+        FZ = fz;
+#pragma unroll
+        for (int m = 0; m < L; m++) qx[m] = qa[m];
+      }
+    }
+    if (!last || k == 0) {
+      stop = true;
+      continue;
+    }
+    pt -= P.grav * mu[k];
+    lift(pt);
+    if (!(tp > b)) stop = true;
+  }
+  if (q0c >= 0 && q40 < 0) {
+    q40 = qd;
+    p40 = pt;
+  }
+  lt[0] = q0c >= 0 ? fmax(0.0, q0c - q40) : 0.0;
+  lt[1] = q0c >= 0 ? p0c : 0.0;
+  lt[2] = q0c >= 0 ? p40 : 0.0;
+  lt[3] = cape;
+}
+
+// Lightning from one storm's physics (air.cu:718-729; Deierling et al. 2008): 9.0e-15 f_p f_np flashes per minute,
+// f_p the falling graupel's mass flux and f_np the rising small ice's, the plume's ice flux shared among storms of
+// updraft area storm_area rising at half sqrt(2 CAPE); only an updraft faster than graupel falls holds it up to
+// charge. Flashes per second per m^2.
+template <bool MIX>
+__device__ double storm_flashes(double mflux, const double* lt, const CP& P, const AP& A) {
+  const double qi = lt[0], w = 0.5 * sqrt(2 * fmax(0.0, lt[3])), gv = A.graupel_v;
+  if (!(mflux > 0) || !(qi > 0) || !(w > gv)) return 0;
+  const double rho = lt[1] / (A.rd * A.Tf), phi = rho * w * A.storm_area, lift = 1 - gv * gv / (w * w);
+  double lg;
+  if constexpr (MIX) lg = (double)logf((float)(lt[1] / lt[2]));
+  else lg = log(lt[1] / lt[2]);
+  const double H = A.rd * (A.Tf - 20) / P.grav * lg;
+  const double up = phi * qi * lift, fnp = up * A.ice_small, fp = up * (1 - A.ice_small) * H / w * gv;
+  return mflux / phi * 9.0e-15 * fp * fnp / 60;
+}
+
+// Air (th, q) at pressure p evaporating water until saturated (air.cu:453-466): three Newton steps, Tetens' slope.
+template <bool MIX>
+__device__ void wet_bulb(double& th, double& q, double p, const CP& P, const AP& A) {
+  using F = std::conditional_t<MIX, float, double>;
+  const double ex = fpow_<MIX>(p / P.p0, P.kappa);
+  for (int it = 0; it < 3; it++) {
+    const double T = th * ex;
+    if (T < 160) return;
+    const double qs = (double)qsat_a<F>((F)T, (F)p, (F)A.eps, A.ice_air), d = qs - q;
+    if (!(d > 0)) return;
+    const double dqs = qs * 17.27 * (273.15 - 35.85) / ((T - 35.85) * (T - 35.85));
+    const double e = d / (1 + P.Lv / P.cp * dqs);
+    q += e;
+    th -= P.Lv * e / (P.cp * ex);
+  }
 }
 
 // The mixed bottom layer's mass up to the level where its air (theta th, vapor q), lifted dry-adiabatically,
@@ -1323,7 +1508,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         th[k] += A.Lf * dz / pib[k];
         dth[k] += A.Lf * dz / pib[k];
         dlat[k] += A.Lf * dz / pib[k];
-        if (Fs > 0 && th[k] * fpow_<MIX>(pbk / P.p0, P.kappa) > A.Tf) {     // (the layer's lower edge above freezing)
+        if (Fs > 0 && base_T<L, MIX>(k, th[k], pbk, P, A) > A.Tf) {          // (the layer's lower edge above freezing)
           const double dm = Fs * dt / (mu[k] * pib[k]) * A.Lf;
           th[k] -= dm;
           dth[k] -= dm;
@@ -1493,7 +1678,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   // ---- moist convection: a parcel of bottom air lifted to the interface above it; buoyant there, it rises through
   // every interface it stays buoyant at and detrains above the last, its condensate raining; the mass that would
   // remove the buoyancy at the first interface relaxes over tau_conv ----
-  double mc = 0;
+  double mc = 0, Scg = 0;
   if (o.flash) o.flash[i] = 0;
   if (MOIST && P.theta_star > 0) {
     double pi_ = 0;
@@ -1526,6 +1711,105 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       q0 = fmax(q0, fmin((double)qs, q0 + (1 - A.rh_crit) * qs));
     }
     double thp, qc, thp2, qc2;
+    if (A.plume_deep > 0 && bot >= 2) {
+      // the deep plume: its full ascent against the layers' spread air, mixing and entraining; the air it rises
+      // through sinks to make room, warming each level by the theta it had below, so the mass that removes its
+      // buoyancy is A / (theta_d - theta*), relaxed over tau_conv; it takes in the cloud layer's air it entrained (me),
+      // keeps l_det of its condensate as cloud, rains the rest (snow by its frozen share), and its rain feeds a
+      // downdraft of the cloud layer's air, saturated by evaporating it
+      double Ab, G = 1, FZ = 0, qx[L], lt[4], xs = 0;
+      int kd;
+      plume_ascent<L, MIX>(mu, th, q, th0, q0, pi_, P, A, Ab, kd, thp, qc, qx, lt, G, FZ);
+#pragma unroll
+      for (int k = 0; k < L; k++) xs += qx[k];
+      if (kd >= 0 && Ab > 0) {
+        const double rise = thp - tstar<L>(bot - 1, P);
+        mc = fmin(Ab / (rise > 1 ? rise : 1.0), mu[bot] - P.floor_) * omexp<MIX>(dt / A.tau_conv);
+        if (A.plume_root > 0 && q0 > 0) mc = fmin(mc, mu[bot] * q[bot] / q0);
+        if (mc > 0) {
+          if (!(kd < bot - 1)) G = 1;
+          const double me = fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - P.floor_));
+          const double qe = q[bot - 1], the = th[bot - 1];
+          G = 1 + me / mc;
+          const double cond_ = q0 + (G - 1) * qe - G * qc - xs;
+          const double cond = fmax(0.0, cond_), det = CLOUD ? fmin(cond, A.l_det * G) : 0.0;
+          if (o.flash) o.flash[i] = storm_flashes<MIX>(mc / dt, lt, P, A);
+          R += mc * (cond - det);
+          Rc = mc * (cond - det);
+          const double qarr = qc + (cond_ - cond) / G;
+          const double Ec = column_energy<L, MOIST>(mu, pib, th, q, P);
+#pragma unroll
+          for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + mc * qx[k] / mu[k]);
+          move_mass<L, true, CLOUD>(P, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qarr, cl);
+          if (me > 0) move_mass<L, true, CLOUD>(P, mu, th, q, pib, bot - 1, kd, me, the, thp, qe, qarr, cl);
+          const double Ea = column_energy<L, MOIST>(mu, pib, th, q, P);
+#pragma unroll
+          for (int k = 0; k < L; k++)
+            if (k == kd) th[k] += (Ec - Ea) / (mu[k] * pib[k]);
+          double Sc = 0;
+          if (A.ice_air > 0 && cond > 0 && cond > det) {   // (its frozen share snows, melting below freezing air)
+            Sc = mc * (cond - det) * fmin(1.0, FZ / cond);
+#pragma unroll
+            for (int k = 0; k < L; k++)
+              if (k == kd) th[k] += A.Lf * Sc / (mu[k] * pib[k]);
+            double ptj = 0;
+#pragma unroll
+            for (int j = 0; j < L; j++) {
+              const double pbj = ptj + P.grav * mu[j];
+              if (j > kd && Sc > 0 && base_T<L, MIX>(j, th[j], pbj, P, A) > A.Tf) {
+                th[j] -= A.Lf * Sc / (mu[j] * pib[j]);
+                Sc = 0;
+              }
+              ptj = pbj;
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < L - 1; j++) {
+            if (j >= kd) wxo[j] += mc;
+            if (j >= kd && j < bot - 1) wxo[j] += me;
+          }
+          if constexpr (CLOUD) {
+#pragma unroll
+            for (int k = 0; k < L; k++)
+              if (k == kd) cl[k] += mc * det / mu[k];
+            cloud_adjust_k<L, MIX>(kd, mu, th, q, cl, pib, dt, P, A);
+          } else {
+            const double rt = [&] {
+              double r = 0;
+#pragma unroll
+              for (int k = 0; k < L - 1; k++)
+                if (k == kd) r = mu[k] * condense_k<L, MIX>(k, mu, th, q, pib, P, A);
+              return r;
+            }();
+            R += rt;
+            Rc += rt;
+          }
+          if (A.downdraft > 0 && bot >= 2 && Rc - Sc > 0) {
+            constexpr int s_ = bot - 1;
+            double pt_ = 0;
+#pragma unroll
+            for (int k = 0; k < s_; k++) pt_ += P.grav * mu[k];
+            const double p1 = pt_ + 0.5 * P.grav * mu[s_], p2 = pt_ + P.grav * (mu[s_] + mu[bot]);
+            double thd_ = th[s_], qd_ = q[s_];
+            for (int j = 0; j <= 4; j++) wet_bulb<MIX>(thd_, qd_, p1 + (p2 - p1) * j / 4, P, A);
+            const double ev = qd_ - q[s_];
+            double md = fmin(A.downdraft * mc, fmax(0.0, mu[s_] - P.floor_));
+            if (ev > 0) md = fmin(md, (Rc - Sc) / ev);
+            if (md > 0 && thd_ < th[s_]) {
+              move_mass<L, true, CLOUD>(P, mu, th, q, pib, s_, bot, md, th[s_], thd_, q[s_], qd_, cl);
+              wxo[s_] -= md;
+              const double e_ = md * fmax(0.0, ev);
+              R -= e_;
+              Rc -= e_;
+            }
+          }
+          S += Sc;
+          Scg = Sc;
+        } else {
+          mc = 0;
+        }
+      }
+    } else {
     parcel<MIX>(th0, q0, pi_, P, A, thp, qc);
     const double d = thp - tstar<L>(bot - 1, P);
     if (d > 0) {
@@ -1570,11 +1854,12 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         mc = 0;
       }
     }
+    }
   }
   // the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the
-  // convection's area revap_area; else on conv_revap of the cell)
+  // convection's area revap_area; else on conv_revap of the cell; with downdrafts they evaporate it instead)
   if constexpr (CLOUD) {
-    if ((A.revap > 0 ? Rc > 0 : A.conv_revap > 0 && !(A.downdraft > 0) && Rc > 0) && L > 1) {
+    if ((A.revap > 0 ? Rc - Scg > 0 : A.conv_revap > 0 && !(A.downdraft > 0) && Rc > 0) && L > 1) {
       double pt_ = 0;
 #pragma unroll
       for (int k = 0; k < bot; k++) pt_ += P.grav * mu[k];
@@ -1583,7 +1868,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double rc_ = A.revap > 0 ? (isl ? A.revap_rhl : A.revap_rhs) : 1.0;
       if (rc_ * qs > q[bot]) {
         double ev;
-        const double Fc = Rc / dt;
+        const double Fc = A.revap > 0 ? (Rc - Scg) / dt : Rc / dt;
         if (A.revap > 0) {
           const double pm = pt_ + 0.5 * P.grav * mu[bot], ps_ = pt_ + P.grav * mu[bot];
           ev = A.revap_area * 5.44e-4 * (rc_ * qs - q[bot]) * pow(sqrt(pm / ps_) / 5.09e-3 * Fc / A.revap_area, 0.577) *

@@ -395,7 +395,8 @@ def test_mixed_layer(L, mixed, opts):
     (5, False, dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, edge_k=2.0, plume_root=1.0, root_q=1.0, sfc=1.0)),
     (4, False, dict(sat_prof=1.0, edge_vap=1.0, q_shape=1.0, plume_root=1.0, cloud=1.0, bl_mix=1.0, bl_reach=1.0)),
     (3, True, dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, q_shape=1.0, plume_root=1.0)),
-    (7, True, dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, edge_k=2.0, plume_root=1.0, cloud=1.0))])
+    (7, True, dict(sat_prof=1.0, edge_vap=1.0, freeze_dry=1.0, edge_k=2.0, plume_root=1.0, cloud=1.0)),
+    (5, False, dict(sat_prof=1.0, ice_air=1.0, cloud=1.0, edge_k=2.0))])
 def test_earth_options(L, mixed, opts):
     """The earth preset's moist options (the spread layers' capacity, the third layer's base condensation, edge vapor,
     freeze-drying, the surface air's vapor, surface-rooted plumes with the thermals' excess) alone and together, with
@@ -415,6 +416,40 @@ def test_earth_options(L, mixed, opts):
     g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, **kw)
     o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed, **kw)
     compare(g, o, 1e-5 if mixed else 1e-12, dt, f'earth options L={L} mixed={mixed}',
+            bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed and cloudy else None)
+
+
+@pytest.mark.parametrize('L,mixed,opts', [
+    (3, False, dict()),
+    (4, False, dict(plume_ent=1e-5, ent_cloud=1e-4)),
+    (4, False, dict(ent_cloud=1e-4, ent_det=1.0, cloud=1.0, ice_air=1.0, downdraft=0.3, graupel_v=1.0)),
+    (5, False, dict(plume_root=1.0, plume_ent=1e-5, downdraft=0.3, revap=1.0, cloud=1.0, sat_prof=1.0, edge_k=2.0)),
+    (3, True, dict(plume_ent=1e-5, ent_cloud=1e-4)),
+    (7, True, dict(plume_ent=1e-5, ent_cloud=1e-4, cloud=1.0, ice_air=1.0, downdraft=0.3, graupel_v=1.0))])
+def test_deep_plume(L, mixed, opts):
+    """The deep plume (its full ascent with mixing and the cloud layer's entrainment, detrainment as cloud, frozen
+    share, lightning, downdrafts) to the moist column's bars."""
+    from oracle import column as ocol
+    opts = dict(opts)
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column deep plume L={L} mixed={mixed} {opts} N={N}', flush=True)
+    cloudy = opts.get('cloud', 0) > 0
+    c = cloudy_columns(N, L, seed=160 + L) if cloudy else moist_columns(N, L, seed=160 + L, polar=30.0)
+    kw = dict(ocean=True, q=c['q'], ice=c['ice'])
+    if cloudy:
+        kw.update(cw=c['cw'], cloudo=True)
+    p = par(L, moist=1.0, plume_deep=1.0, **opts)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, **kw)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed, **kw)
+    assert o['conv'].max() > 0
+    if opts.get('graupel_v') and L == 4:                  # (seven thin layers: the plumes stop below the freezing band)
+        assert o['flash'].max() > 0
+    # lightning: the storm's ice is the condensate between the first sub-levels the plume crosses 0 and -40 C at, a
+    # threshold the mixed form's float32 parcel can cross one sub-level apart (measured: 1 of 40962 columns, 0.8%)
+    flash_g, flash_o = g.pop('flash'), o.pop('flash')
+    np.testing.assert_allclose(flash_g, flash_o, rtol=2e-2 if mixed else 1e-10, atol=1e-12 * max(flash_o.max(), 1e-30))
+    g['flash'] = o['flash'] = np.zeros_like(flash_o)
+    compare(g, o, 1e-5 if mixed else 1e-12, dt, f'deep plume L={L} mixed={mixed}',
             bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed and cloudy else None)
 
 
