@@ -18,7 +18,8 @@ enum { P_GRAV = 13, P_CP = 14, P_KAPPA = 15, P_P0 = 16, P_FLOOR = 11, P_THETA_ST
        P_RHO_ICE = 29, P_ICE_COVER = 30, P_BULK_GUST = 75, P_Q_SEA = 97, P_Z0_LAND = 91, P_THS = 132,
        P_EDGE_VAP = 86, P_FREEZE_DRY = 85, P_EDGE_K = 101, P_ICE_AIR = 102, P_SAT_PROF = 51, P_SAT_BASE = 53,
        P_BL_Q = 89, P_Q_SHAPE = 83, P_PLUME_ROOT = 88, P_ROOT_Q = 107, P_PLUME_DEEP = 47, P_PLUME_ENT = 78,
-       P_ENT_CLOUD = 93, P_ENT_DET = 108, P_L_DET = 65, P_STORM_AREA = 80, P_GRAUPEL_V = 81, P_ICE_SMALL = 82 };
+       P_ENT_CLOUD = 93, P_ENT_DET = 108, P_L_DET = 65, P_STORM_AREA = 80, P_GRAUPEL_V = 81, P_ICE_SMALL = 82,
+       P_ML_CLEAR = 118, P_CU_CLOUD = 119 };
 
 static void layer_exner(int L, const double* mu, const double* par, double* pb, double* pib) {
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP];
@@ -666,14 +667,55 @@ static double cover_of(double cw, double mu, double liq, const double* par, doub
   return fmin(1.0, cw / (s * (liq * par[P_Q_AUTO] + (1 - liq) * par[P_L_ICE])) * fmax(1.0, mu / par[P_MU_CLOUD]));
 }
 
+/* mixed_cloud (air.cu:397-426, bl_q): the mixed bottom layer's vapor is even through its depth, so it is nearest
+ * saturation at its top: its cloud statistics at each of layer_cap's four levels, l the mean condensate, f the cover
+ * where it is most, dqf the condensate's fall per kelvin of warming. ml_clear (unless pdf): only air actually
+ * saturated condenses there. Returns 0 for any other layer (or without bl_q). */
+static int mixed_cloud(int L, int k, const double* mu, double th, double qt, const double* par, double eps, int mix,
+                       double* l, double* f, double* dqf, int pdf) {
+  static const double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  static const double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  if (!(par[P_BL_Q] > 0 && k == L - 1 && k > 0)) return 0;
+  double pt = 0;
+  for (int j = 0; j < k; j++) pt += par[P_GRAV] * mu[j];
+  const double pb = pt + par[P_GRAV] * mu[k];
+  *l = 0;
+  *f = 0;
+  *dqf = 0;
+  for (int j = 0; j < 4; j++) {
+    const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
+    const double ex = fpow_(p / par[P_P0], par[P_KAPPA], mix), T = th * ex;
+    if (T < 160) continue;
+    const double sq = qsat_x(T, p, eps, par[P_ICE_AIR], mix);
+    double lj, fj;
+    if (par[P_ML_CLEAR] > 0 && !pdf) {
+      lj = fmax(0.0, qt - sq);
+      fj = qt > sq ? 1.0 : 0.0;
+    } else {
+      cloud_pdf(qt, sq, par, &lj, &fj);
+    }
+    *l += 0.5 * wq[j] * lj;
+    *f = fmax(*f, fj);
+    *dqf += 0.5 * wq[j] * fj * sq * par[P_LV] / (par[P_RV] * T * T) * ex;
+  }
+  return 1;
+}
+
 /* cloud_adjust (air.cu:438-450): condense up to the statistics' condensate with the latent heat's feedback on the
  * capacity; a cloud holding more erodes over tau_erode. */
+static int mixed_cloud(int L, int k, const double* mu, double th, double qt, const double* par, double eps, int mix,
+                       double* l, double* f, double* dqf, int pdf);
+
 static double cloud_adjust(int L, int k, const double* mu, double* th, double* q, double* cw, const double* pib,
                            double dt, const double* par, double eps, int mix) {
   double qs, dqs, l, f;
-  layer_cap(L, k, mu, th[k], par, eps, mix, &qs, &dqs);
-  if (!(qs > 0)) return 0;
-  cloud_pdf(q[k] + cw[k], qs, par, &l, &f);
+  if (mixed_cloud(L, k, mu, th[k], q[k] + cw[k], par, eps, mix, &l, &f, &dqs, 0)) {
+    f = 1;                                           /* (dqs then already carries each level's share) */
+  } else {
+    layer_cap(L, k, mu, th[k], par, eps, mix, &qs, &dqs);
+    if (!(qs > 0)) return 0;
+    cloud_pdf(q[k] + cw[k], qs, par, &l, &f);
+  }
   const double dl = l > cw[k] ? (l - cw[k]) / (1 + par[P_LV] / pib[k] * dqs * f)
                               : -(cw[k] - l) * omexp(dt / par[P_TAU_ERODE], mix);
   cw[k] += dl;
@@ -909,12 +951,29 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         const double ptk = k ? pb[k - 1] : 0.0;
         const double pc = k == bot ? 0.5 * (pb[k] + ptk) : ptk + 0.25 * (pb[k] - ptk);
         liq[k] = fmax(0.0, fmin(1.0, (th[k] * fpow_(pc / p0, kappa, mix) - Tf + par[P_LIQ_COLD]) / par[P_LIQ_COLD]));
-        double qs, dqs, l, f;
-        layer_cap(L, k, mu, th[k], par, aeps, mix, &qs, &dqs);
-        cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
+        double qs = 0, dqs, l, f;
+        if (!mixed_cloud(L, k, mu, th[k], q[k] + cl[k], par, aeps, mix, &l, &f, &dqs, 0)) {
+          layer_cap(L, k, mu, th[k], par, aeps, mix, &qs, &dqs);
+          cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
+        } else if (par[P_CW_QS] > 0) {
+          double d_;
+          layer_cap(L, k, mu, th[k], par, aeps, mix, &qs, &d_);
+        }
         fc[k] = fmax(f, cover_of(cl[k], mu[k], liq[k], par, qs));
         cwp[k] = fc[k] > 1e-6 ? mu[k] * cl[k] / fc[k] : 0.0;
       }
+    /* cu_cloud (with ml_clear and bl_q): the mixed layer clear below its cloud base, the cloud its thermals make is
+     * above it, in the layer above: the thermals' statistics at the mixed layer's levels give its cover there and its
+     * water (cu_cloud 2: the cover alone), for the radiation */
+    if (cloudy && par[P_CU_CLOUD] > 0 && par[P_ML_CLEAR] > 0 && par[P_BL_Q] > 0 && L > 2) {
+      double l_, f_, d_;
+      if (mixed_cloud(L, bot, mu, th[bot], q[bot] + cl[bot], par, aeps, mix, &l_, &f_, &d_, 1) && f_ > 1e-6) {
+        const int kc = bot - 1;
+        const double w_ = mu[kc] * cl[kc] + (par[P_CU_CLOUD] > 1.5 ? 0.0 : mu[bot] * l_);
+        fc[kc] = fmax(fc[kc], f_);
+        cwp[kc] = w_ / fc[kc];
+      }
+    }
     /* the air touching the ground: the bottom mu_bl of the column (or bl_bot's share of it, the thin bottom layer's
      * own mass at least); the layers above give their air at their lower edge's theta* where warmer */
     double wbl[OG_LMAX], thb = 0, qb = 0, mb = 0;
@@ -1285,7 +1344,9 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         dlat[k] += Lv * dl / pib[k];
         double qs, dqs, l, f;
         layer_cap(L, k, mu, th[k], par, aeps, mix, &qs, &dqs);
-        cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
+        double dqf;
+        if (!mixed_cloud(L, k, mu, th[k], q[k] + cl[k], par, aeps, mix, &l, &f, &dqf, 0))
+          cloud_pdf(q[k] + cl[k], qs, par, &l, &f);
         f = fmax(f, cover_of(cl[k], mu[k], liq[k], par, qs));
         const double inc = f > 1e-6 ? cl[k] / f * fmax(1.0, mu[k] / par[P_MU_CLOUD]) : 0.0, r = inc / par[P_Q_AUTO];
         double pt_ = 0;
@@ -1485,6 +1546,18 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           const double rise = thp - tstar(bot - 1, L, par);
           mc = fmin(A / (rise > 1 ? rise : 1.0), mu[bot] - par[P_FLOOR]) * omexp(dt / par[P_TAU_CONV], mix);
           if (par[P_PLUME_ROOT] > 0 && q0 > 0) mc = fmin(mc, mu[bot] * q[bot] / q0);
+          /* the mixed boundary layer (bl_q): plumes leave it through the clouds at its top, at most rho a w* with a
+           * their share there and w* the free convection's velocity scale (Neggers, Stevens & Neelin 2006) */
+          if (blm && par[P_BL_Q] > 0) {
+            double l_, f_, d_;
+            if (mixed_cloud(L, bot, mu, th[bot], q[bot] + (cloudy ? cl[bot] : 0.0), par, aeps, mix, &l_, &f_, &d_,
+                            par[P_CU_CLOUD] > 0)) {
+              const double rho = ps / (rd * Tsa), Bk = fmax(0.0, (H + air->vf * cp * Tsa * E) / (rho * cp));
+              const double hb = rd * Tsa / grav * log(ps / fmax(ps - grav * mu[bot], 1.0));
+              const double ws = cbrt(grav / Tsa * Bk * hb);
+              mc = fmin(mc, rho * f_ * ws * dt);
+            }
+          }
           if (mc > 0) {
             if (!(kd < bot - 1)) G = 1;
             const double me = fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - par[P_FLOOR]));

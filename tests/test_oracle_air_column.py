@@ -977,3 +977,54 @@ def test_deep_plume_lightning_and_downdraft():
     Wd = (new_masses(c, d['wx']) * d['q']).sum(0)
     Wn = (new_masses(c, n['wx']) * n['q']).sum(0)
     np.testing.assert_allclose((Wd - Wn)[go], ((n['rain'] - d['rain']) * 900.0)[go], rtol=1e-9, atol=1e-12)
+
+
+# ---- bl_q: the mixed bottom layer's even vapor and its cloud statistics (mixed_cloud, ml_clear, cu_cloud) ----
+
+BLQ_OPTS = [dict(), dict(ml_clear=1.0), dict(ml_clear=1.0, cu_cloud=1.0), dict(ml_clear=1.0, cu_cloud=2.0),
+            dict(plume_deep=1.0, downdraft=0.3, ent_cloud=1e-4), dict(plume_deep=1.0, ml_clear=1.0, cu_cloud=1.0,
+                                                                      edge_vap=1.0, q_shape=1.0)]
+
+
+@pytest.mark.parametrize('opts', BLQ_OPTS)
+@pytest.mark.parametrize('mix', [False, True])
+def test_bl_q_closes(opts, mix):
+    """With the mixed layer's cloud statistics the energy closes and the water changes by evaporation less rain
+    (the simple plume held off: it moves its base layer's air without its cloud water, as the reference's does)."""
+    c = deep_columns()
+    N = c['Ts'].size
+    c['cw'] = np.full_like(c['q'], 5e-5)
+    deep = opts.get('plume_deep', 0) > 0
+    p = par4(moist=1.0, cloud=1.0, bl_mix=1.0, bl_q=1.0, **dict(opts, tau_conv=1e12) if not deep else opts)
+    dt = 900.0
+    o = run(c, p, q=c['q'], ice=c['ice'], cw=c['cw'], cloudo=True, dt=dt, mix=mix)
+    stored_matches(o, dt)
+    mu1 = new_masses(c, o['wx'])
+    W0 = (c['mu'] * (c['q'] + c['cw'])).sum(0)
+    W1 = (mu1 * (o['q'] + o['cw'])).sum(0)
+    np.testing.assert_allclose(W1 - W0, (o['evap'] - o['rain']) * dt, atol=1e-11 * W0.max())
+
+
+@pytest.mark.parametrize('ml_clear', [0.0, 1.0])
+def test_mixed_layer_cover(ml_clear):
+    """bl_q: the mixed layer's cover is where its even vapor (and cloud water) is most saturated among its capacity's
+    four Gauss-Legendre levels: by the statistics' share f = (q_t - q_s + b) / 2b, b = (1 - rh_crit) q_s, clipped to
+    [0, 1]; with ml_clear 1 where q_t > q_s, else 0 (no cloud water in it, so nothing else adds cover)."""
+    c = deep_columns(seed=8)
+    N = c['Ts'].size
+    pt, pb, _ = pib_of(c['mu'])
+    x, w = np.polynomial.legendre.leggauss(4)
+    pp = 0.5 * (pt[3] + pb[3])[None] + 0.5 * (pb[3] - pt[3])[None] * x[:, None]
+    qsj = tetens(c['theta'][3][None] * (pp / P0) ** KAPPA, pp)
+    c['q'][3] = qsj.min(0) * np.linspace(0.85, 1.1, N)             # around the top level's saturation
+    cw = np.zeros_like(c['q'])
+    o = run(c, par4(moist=1.0, cloud=1.0, bl_q=1.0, ml_clear=ml_clear, rh_crit=0.9), q=c['q'], ice=c['ice'], cw=cw,
+            cloudo=True, dt=600.0)
+    qt = c['q'][3][None]
+    if ml_clear:
+        f = (qt > qsj).astype(float).max(0)
+    else:
+        b = 0.1 * qsj
+        f = np.clip((qt - qsj + b) / (2 * b), 0.0, 1.0).max(0)
+    assert 0 < f.mean() < 1
+    np.testing.assert_allclose(o['cloudo'][3], f, rtol=1e-12, atol=1e-15)
