@@ -131,6 +131,7 @@ struct AP {
   double sat_prof, sat_base, edge_vap, freeze_dry, q_shape, plume_root, root_q, bl_q;
   // the deep plume's
   double plume_deep, plume_ent, ent_cloud, ent_det, l_det, storm_area, graupel_v, ice_small;
+  double ml_clear, cu_cloud;                           // (the mixed layer's cloud with bl_q)
 };
 
 static AP make_ap(const double* a, const isw_air* air) {
@@ -156,6 +157,7 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.plume_root = a[88]; p.root_q = a[107]; p.bl_q = a[89];
   p.plume_deep = a[47]; p.plume_ent = a[78]; p.ent_cloud = a[93]; p.ent_det = a[108]; p.l_det = a[65];
   p.storm_area = a[80]; p.graupel_v = a[81]; p.ice_small = a[82];
+  p.ml_clear = a[118]; p.cu_cloud = a[119];
   return p;
 }
 
@@ -802,6 +804,44 @@ __device__ __forceinline__ double cover_of(double cw, double mu, double liq, con
   return fmin(1.0, cw / (s * (liq * A.q_auto + (1 - liq) * A.l_ice)) * fmax(1.0, mu / A.mu_cloud));
 }
 
+// The mixed bottom layer's cloud statistics (air.cu:397-426, bl_q): its vapor even through its depth, it is nearest
+// saturation at its top: at each of the capacity's four levels, l the mean condensate, f the cover where it is most,
+// dqf the condensate's fall per kelvin of warming; ml_clear (unless pdf): only air actually saturated condenses.
+// False for any other layer (or without bl_q).
+template <int L, bool MIX>
+__device__ bool mixed_cloud(int k, const double* mu, double th, double qt, const CP& P, const AP& A, double& l,
+                            double& f, double& dqf, bool pdf) {
+  using F = std::conditional_t<MIX, float, double>;
+  if (!(A.bl_q > 0 && k == L - 1 && k > 0)) return false;
+  constexpr double x[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  constexpr double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  double pt = 0;
+#pragma unroll
+  for (int j = 0; j < L - 1; j++) pt += P.grav * mu[j];
+  const double pb = pt + P.grav * mu[L - 1];
+  l = 0;
+  f = 0;
+  dqf = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double p = 0.5 * (pt + pb) + 0.5 * (pb - pt) * x[j];
+    const double ex = fpow_<MIX>(p / P.p0, P.kappa), T = th * ex;
+    if (T < 160) continue;
+    const double sq = (double)qsat_a<F>((F)T, (F)p, (F)A.eps, A.ice_air);
+    double lj, fj;
+    if (A.ml_clear > 0 && !pdf) {
+      lj = fmax(0.0, qt - sq);
+      fj = qt > sq ? 1.0 : 0.0;
+    } else {
+      cloud_pdf(qt, sq, A, lj, fj);
+    }
+    l += 0.5 * wq[j] * lj;
+    f = fmax(f, fj);
+    dqf += 0.5 * wq[j] * fj * sq * P.Lv / (A.Rv * T * T) * ex;
+  }
+  return true;
+}
+
 // Cloud water in layer k (a runtime index) condensing up to the cloud statistics' condensate with the latent heat's
 // feedback on the capacity, or eroding over tau_erode (air.cu:438-450); returns the condensate.
 template <int L, bool MIX>
@@ -814,12 +854,18 @@ __device__ double cloud_adjust_k(int k, const double* mu, double* th, double* q,
     if (j < k) pt += P.grav * mu[j];
     if (j == k) { mk = mu[j]; tk = th[j]; qk = q[j]; ck = cl[j]; pk = pib[j]; }
   }
-  F qs, dqs;
-  layer_cap_k<L, MIX>(k, tk, pt, pt + P.grav * mk, P, A, qs, dqs);
-  if (!(qs > 0)) return 0;
-  double l, f;
-  cloud_pdf(qk + ck, qs, A, l, f);
-  const double dl = l > ck ? (l - ck) / (1 + P.Lv / pk * dqs * f) : -(ck - l) * omexp<MIX>(dt / A.tau_erode);
+  double l, f, dqa, dqm;
+  if (mixed_cloud<L, MIX>(k, mu, tk, qk + ck, P, A, l, f, dqm, false)) {
+    f = 1;
+    dqa = dqm;
+  } else {
+    F qs, dqs;
+    layer_cap_k<L, MIX>(k, tk, pt, pt + P.grav * mk, P, A, qs, dqs);
+    if (!(qs > 0)) return 0;
+    cloud_pdf(qk + ck, qs, A, l, f);
+    dqa = dqs;
+  }
+  const double dl = l > ck ? (l - ck) / (1 + P.Lv / pk * dqa * f) : -(ck - l) * omexp<MIX>(dt / A.tau_erode);
 #pragma unroll
   for (int j = 0; j < L; j++)
     if (j == k) {
@@ -1061,12 +1107,28 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double ptk = k ? pb[k - 1] : 0.0;
       const double pc = k == bot ? 0.5 * (pb[k] + ptk) : ptk + 0.25 * (pb[k] - ptk);
       liq[k] = fmax(0.0, fmin(1.0, (th[k] * fpow_<MIX>(pc / P.p0, P.kappa) - A.Tf + A.liq_cold) / A.liq_cold));
-      F qs, dqs;
-      layer_cap_k<L, MIX>(k, th[k], ptk, ptk + P.grav * mu[k], P, A, qs, dqs);
-      double l, f;
-      cloud_pdf(q[k] + cl[k], qs, A, l, f);
+      F qs = 0, dqs;
+      double l, f, dqm;
+      if (!mixed_cloud<L, MIX>(k, mu, th[k], q[k] + cl[k], P, A, l, f, dqm, false)) {
+        layer_cap_k<L, MIX>(k, th[k], ptk, ptk + P.grav * mu[k], P, A, qs, dqs);
+        cloud_pdf(q[k] + cl[k], qs, A, l, f);
+      } else if (A.cw_qs > 0) {
+        layer_cap_k<L, MIX>(k, th[k], ptk, ptk + P.grav * mu[k], P, A, qs, dqs);
+      }
       fc[k] = fmax(f, cover_of(cl[k], mu[k], liq[k], A, qs));
       cwp[k] = fc[k] > 1e-6 ? mu[k] * cl[k] / fc[k] : 0.0;
+    }
+    // cu_cloud (with ml_clear and bl_q): the mixed layer clear below its cloud base, the cloud its thermals make is
+    // above it, in the layer above: the thermals' statistics at the mixed layer's levels give its cover there and its
+    // water (cu_cloud 2: the cover alone), for the radiation
+    if (A.cu_cloud > 0 && A.ml_clear > 0 && A.bl_q > 0 && L > 2) {
+      double l_, f_, d_;
+      if (mixed_cloud<L, MIX>(bot, mu, th[bot], q[bot] + cl[bot], P, A, l_, f_, d_, true) && f_ > 1e-6) {
+        constexpr int kc = bot - 1;
+        const double w_ = mu[kc] * cl[kc] + (A.cu_cloud > 1.5 ? 0.0 : mu[bot] * l_);
+        fc[kc] = fmax(fc[kc], f_);
+        cwp[kc] = w_ / fc[kc];
+      }
     }
   }
   // the air touching the ground: the bottom mu_bl of the column (bl_bot: the thin bottom layer's share of it); the
@@ -1482,11 +1544,22 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       // condense up to the statistics' condensate with the latent heat's feedback on the capacity; a cloud holding
       // more erodes over tau_erode (air.cu:438-450)
       F qs, dqs;
-      double l, f, dl = 0;
-      layer_cap_k<L, MIX>(k, th[k], pt, pbk, P, A, qs, dqs);
-      if (qs > 0) {
-        cloud_pdf(q[k] + cl[k], qs, A, l, f);
-        dl = l > cl[k] ? (l - cl[k]) / (1 + P.Lv / pib[k] * dqs * f) : -(cl[k] - l) * omexp<MIX>(dt / A.tau_erode);
+      double l, f, dl = 0, dqa = 0, dqm;
+      bool go;
+      if (mixed_cloud<L, MIX>(k, mu, th[k], q[k] + cl[k], P, A, l, f, dqm, false)) {
+        f = 1;                                       // (dqm already carries each level's share)
+        dqa = dqm;
+        go = true;
+      } else {
+        layer_cap_k<L, MIX>(k, th[k], pt, pbk, P, A, qs, dqs);
+        go = qs > 0;
+        if (go) {
+          cloud_pdf(q[k] + cl[k], qs, A, l, f);
+          dqa = dqs;
+        }
+      }
+      if (go) {
+        dl = l > cl[k] ? (l - cl[k]) / (1 + P.Lv / pib[k] * dqa * f) : -(cl[k] - l) * omexp<MIX>(dt / A.tau_erode);
         cl[k] += dl;
         q[k] -= dl;
         th[k] += P.Lv * dl / pib[k];
@@ -1494,7 +1567,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       dth[k] += P.Lv * dl / pib[k];
       dlat[k] += P.Lv * dl / pib[k];
       layer_cap_k<L, MIX>(k, th[k], pt, pbk, P, A, qs, dqs);
-      cloud_pdf(q[k] + cl[k], qs, A, l, f);
+      if (!mixed_cloud<L, MIX>(k, mu, th[k], q[k] + cl[k], P, A, l, f, dqm, false)) cloud_pdf(q[k] + cl[k], qs, A, l, f);
       f = fmax(f, cover_of(cl[k], mu[k], liq[k], A, qs));
       const double inc = f > 1e-6 ? cl[k] / f * fmax(1.0, mu[k] / A.mu_cloud) : 0.0, r = inc / A.q_auto;
       const double pm = pt + 0.5 * P.grav * mu[k], Tk = pib[k] * th[k] / P.cp, depth = mu[k] / (pm / (A.rd * Tk));
@@ -1726,6 +1799,18 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         const double rise = thp - tstar<L>(bot - 1, P);
         mc = fmin(Ab / (rise > 1 ? rise : 1.0), mu[bot] - P.floor_) * omexp<MIX>(dt / A.tau_conv);
         if (A.plume_root > 0 && q0 > 0) mc = fmin(mc, mu[bot] * q[bot] / q0);
+        // the mixed boundary layer (bl_q): plumes leave it through the clouds at its top, at most rho a w* with a
+        // their share there and w* the free convection's velocity scale (Neggers, Stevens & Neelin 2006)
+        if (blm && A.bl_q > 0) {
+          double l_, f_, d_;
+          if (mixed_cloud<L, MIX>(bot, mu, th[bot], q[bot] + (CLOUD ? cl[bot] : 0.0), P, A, l_, f_, d_,
+                                  A.cu_cloud > 0)) {
+            const double rho = ps / (A.rd * Tsa), Bk = fmax(0.0, (H + A.vf * P.cp * Tsa * E) / (rho * P.cp));
+            const double hb = A.rd * Tsa / P.grav * log(ps / fmax(ps - P.grav * mu[bot], 1.0));
+            const double ws = cbrt(P.grav / Tsa * Bk * hb);
+            mc = fmin(mc, rho * f_ * ws * dt);
+          }
+        }
         if (mc > 0) {
           if (!(kd < bot - 1)) G = 1;
           const double me = fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - P.floor_));

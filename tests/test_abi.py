@@ -85,17 +85,12 @@ def test_bad_arguments_rejected_on_host():
     par[22] = 1.0                                            # moist air: no vapor or ice given; the moist options
     assert air_column() == E
     par[47] = 1.0                                            # plume_deep with its IFS entrainment, shallow plume,
-    for k in (109, 125, 116, 110, 111, 113, 114, 89):        # cumulus layer, virtual buoyancy, subsidence, bl_q
+    for k in (109, 125, 116, 110, 111, 113, 114):            # cumulus layer, virtual buoyancy, subsidence
         par[k] = 1.0
         assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf,
                                   600.0, *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
         par[k] = 0.0
     par[47] = 0.0
-    par[55] = par[89] = 1.0                                  # clouds with the evenly mixed bottom layer (bl_q)
-    assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
-                              *([buf] * 8), None, None, *([None] * 7), buf, buf, buf, *([None] * 8), None, 0,
-                              None) == E
-    par[55] = par[89] = 0.0
     # land without its bucket's capacity, soil, ponds or snow
     assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
                               *([buf] * 8), None, None, None, None, None, buf, buf, buf, None, buf, None, None,

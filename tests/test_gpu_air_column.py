@@ -453,6 +453,30 @@ def test_deep_plume(L, mixed, opts):
             bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed and cloudy else None)
 
 
+@pytest.mark.parametrize('L,mixed,opts', [
+    (4, False, dict(ml_clear=1.0, cu_cloud=1.0)),
+    (4, False, dict(plume_deep=1.0, downdraft=0.3, ent_cloud=1e-4)),
+    (5, False, dict(ml_clear=1.0, cu_cloud=2.0, plume_deep=1.0, edge_vap=1.0, q_shape=1.0, sat_prof=1.0, edge_k=2.0)),
+    (4, True, dict(ml_clear=1.0, cu_cloud=1.0, plume_deep=1.0, plume_ent=1e-5))])
+def test_bl_q(L, mixed, opts):
+    """The mixed layer's even vapor and its cloud statistics (bl_q with bl_mix and clouds; ml_clear, cu_cloud), alone
+    and with the deep plume's mixed-layer limit, to the moist column's bars (the mixed form with clouds as the
+    cloudy column's)."""
+    from oracle import column as ocol
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column bl_q L={L} mixed={mixed} {opts} N={N}', flush=True)
+    c = cloudy_columns(N, L, seed=170 + L)
+    kw = dict(ocean=True, q=c['q'], ice=c['ice'], cw=c['cw'], cloudo=True)
+    p = par(L, moist=1.0, cloud=1.0, bl_mix=1.0, bl_q=1.0, **opts)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, **kw)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed, **kw)
+    flash_g, flash_o = g.pop('flash'), o.pop('flash')
+    np.testing.assert_allclose(flash_g, flash_o, rtol=2e-2 if mixed else 1e-10, atol=1e-12 * max(flash_o.max(), 1e-30))
+    g['flash'] = o['flash'] = np.zeros_like(flash_o)
+    compare(g, o, 1e-5 if mixed else 1e-12, dt, f'bl_q L={L} mixed={mixed}',
+            bars=dict(wx=3e-4, q=3e-5, theta=3e-5) if mixed else None)
+
+
 def test_aqua_dry_world():
     """The 'aqua_dry' world (three dry air layers over the slab, the sun, gray radiation, the column and the exchange
     every tick) against the oracle's tick, 12 ticks at level 4 in the FP64 form (<= 1e-10, as the core's own
