@@ -130,7 +130,8 @@ struct AP {
   // plumes' root
   double sat_prof, sat_base, edge_vap, freeze_dry, q_shape, plume_root, root_q, bl_q;
   // the deep plume's
-  double plume_deep, plume_ent, ent_cloud, ent_det, l_det, storm_area, graupel_v, ice_small;
+  double plume_deep, plume_ent, ent_cloud, ent_det, l_det, storm_area, graupel_v, ice_small, ent_ifs, ent_loc, det_all,
+      buoy_v;
   double ml_clear, cu_cloud;                           // (the mixed layer's cloud with bl_q)
 };
 
@@ -156,6 +157,7 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.sat_prof = a[51]; p.sat_base = a[53]; p.edge_vap = a[86]; p.freeze_dry = a[85]; p.q_shape = a[83];
   p.plume_root = a[88]; p.root_q = a[107]; p.bl_q = a[89];
   p.plume_deep = a[47]; p.plume_ent = a[78]; p.ent_cloud = a[93]; p.ent_det = a[108]; p.l_det = a[65];
+  p.ent_ifs = a[109]; p.ent_loc = a[125]; p.det_all = a[116]; p.buoy_v = a[113];
   p.storm_area = a[80]; p.graupel_v = a[81]; p.ice_small = a[82];
   p.ml_clear = a[118]; p.cu_cloud = a[119];
   return p;
@@ -883,14 +885,19 @@ __device__ __forceinline__ double liq_of(double T) {
   else return fmin(1.0, fmax(0.0, (T - 253.15) / 20.0));
 }
 
-// The deep plume's full ascent (air.cu:540-666; its options plume_ent, ent_cloud, ent_det, ice_air): bottom air (th0,
+// The deep plume's full ascent (air.cu:540-666; its options plume_ent, ent_cloud, ent_det, ice_air, ent_ifs, ent_loc,
+// det_all, buoy_v): bottom air (th0,
 // q0) lifted from the bottom layer's top (pi_) up through the layers in M sub-levels each, condensing as it goes,
 // mixing with (plume_ent) and, through the layer above the bottom one, entraining (ent_cloud: its mass growing as
 // exp(ent_cloud z); ent_det: as much detrained) the air it passes, whose theta at mass share x above the layer's
 // lower edge a is a + (b - a) ln(1 + x (e^s - 1)) / s. A: its buoyancy summed by mass up to the highest level it beats
 // (the bottom layer's stretch a triangle); kd, thd, qd: the layer it detrains into, its theta and vapor there; qx: what
 // its mixing gave each layer per kg; G: its mass per kg leaving the bottom layer; FZ: its frozen condensate; lt: its
-// condensate freezing between 0 and -40 C, those levels' pressures and its CAPE (for lightning). The layer loop is
+// condensate freezing between 0 and -40 C, those levels' pressures and its CAPE (for lightning). ent_ifs: the
+// entraining layers' entrainment and detrainment the IFS's, 1.75e-3 /m (1.3 - RH) (q_s / q_s at cloud base)^3 and
+// 0.75e-4 /m, in sub-levels of ~25 hPa (air.cu:578-584, 609-624); det_all: every tropospheric layer (below edge_k's
+// stratosphere) entraining; ent_loc: the air taken in holding the layer's vapor at its relative humidity level by
+// level (air.cu:586-598); buoy_v: buoyancy in virtual theta, theta (1 + 0.608 q) (air.cu:564-567). The layer loop is
 // unrolled with a stop flag, so every per-layer value stays in registers.
 template <int L, bool MIX>
 __device__ void plume_ascent(const double* mu, const double* th, const double* q, double th0, double q0, double pi_,
@@ -909,7 +916,13 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
     return ex_;
   };
   lift(pi_);
-  double sum = 0.5 * mu[bot] * (tp - tstar<L>(bot - 1, P)), pt = pi_;
+  auto vf = [&](double qv_) { return A.buoy_v > 0 ? 1 + 0.608 * qv_ : 1.0; };
+  const bool ifs = A.ent_ifs > 0;
+  double sum = 0.5 * mu[bot] * (tp * vf(qp) - tstar<L>(bot - 1, P) * vf(q[bot - 1])), pt = pi_;
+  // (ent_ifs: the environment's saturation at cloud base, the mixed layer's top)
+  using F = std::conditional_t<MIX, float, double>;
+  auto qsat_ = [&](double T, double p) { return (double)qsat_a<F>((F)T, (F)p, (F)A.eps, A.ice_air); };
+  const double qsb = ifs ? qsat_(th[bot] * fpow_<MIX>(pi_ / P.p0, P.kappa), pi_) : 1.0;
   Ab = 0;
   kd = -1;
   thd = tp;
@@ -926,30 +939,53 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
     if (stop) continue;
     const double a = tstar<L>(k, P), b = k > 0 ? tstar<L>(k - 1, P) : 0.0;
     const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !MIX ? expm1(s) : 0.0;
-    const bool ek = k > 0 && k == bot - 1;
-    const double f = A.plume_ent > 0 ? omexp<MIX>(A.plume_ent * P.grav * mu[k] / M) : 0.0;
+    const bool ek = k > 0 && (k == bot - 1 || (A.det_all > 0 && ifs && k >= (int)A.edge_k));
+    const int Mk = ifs && ek ? min(48, max(M, (int)ceil(P.grav * mu[k] / 2500.0))) : M;
+    const double f = A.plume_ent > 0 ? omexp<MIX>(A.plume_ent * P.grav * mu[k] / Mk) : 0.0;
+    const bool loc = A.ent_loc > 0 && A.ent_cloud > 0 && ifs && ek && k >= (int)A.edge_k && q[k] > 0;
+    double qsl = 0;
+    if (loc)
+      for (int j = 0; j < Mk; j++) {
+        const double x = (j + 0.5) / Mk, pl = pt - x * P.grav * mu[k];
+        const double te = a + (b - a) * spread_of<MIX>(x, s, es);
+        qsl += qsat_(te * fpow_<MIX>(pl / P.p0, P.kappa), pl) / Mk;
+      }
     bool last = false;
-    for (int j = 0; j < M; j++) {
-      const double x = (j + 0.5) / M, pl = pt - x * P.grav * mu[k];
+    for (int j = 0; j < Mk; j++) {
+      const double x = (j + 0.5) / Mk, pl = pt - x * P.grav * mu[k];
       const double ex = lift(pl);
       const double te = k == 0 ? th[0] : a + (b - a) * spread_of<MIX>(x, s, es);
       if (A.ent_cloud > 0 && ek) {
-        const double dz = A.rd * te * ex / P.grav * (P.grav * mu[k] / M) / pl;
-        const double g_ = expm1(A.ent_cloud * dz), fe = g_ / (1 + g_);
-        if (A.ent_det > 0) qa[k] += Mp * fe * (qp - q[k]);
-        tp += fe * (te - tp);
-        qp += fe * (q[k] - qp);
-        if (!(A.ent_det > 0)) Mp *= 1 + g_;
+        double dz = A.rd * te * ex / P.grav * (P.grav * mu[k] / Mk) / pl;
+        if (ifs && j == 0) dz *= 0.5;                  // (from cloud base to the first sub-level's middle)
+        if (ifs) {
+          const double qse = qsat_(te * ex, pl), qen = loc && qsl > 0 ? q[k] * qse / qsl : q[k];
+          const double rh = fmin(1.0, fmax(0.0, qen / qse)), fs = fmin(1.0, qse / qsb);
+          const double eps_ = 1.75e-3 * (1.3 - rh) * fs * fs * fs, del_ = fmin(eps_, 0.75e-4);
+          const double fe_ = -expm1(-eps_ * dz), fd_ = -expm1(-del_ * dz);
+          qa[k] += Mp * fd_ * (qp - qen);
+          if (loc) qa[k] -= Mp * expm1((eps_ - del_) * dz) * (qen - q[k]);   // (the growth taken in at qen)
+          tp += fe_ * (te - tp);
+          qp += fe_ * (qen - qp);
+          Mp *= exp((eps_ - del_) * dz);
+        } else {
+          const double g_ = expm1(A.ent_cloud * dz), fe = g_ / (1 + g_);
+          if (A.ent_det > 0) qa[k] += Mp * fe * (qp - q[k]);
+          tp += fe * (te - tp);
+          qp += fe * (q[k] - qp);
+          if (!(A.ent_det > 0)) Mp *= 1 + g_;
+        }
       }
       if (A.plume_ent > 0) {
         qa[k] += Mp * f * (qp - q[k]);
         tp += f * (te - tp);
         qp += f * (q[k] - qp);
       }
-      sum += mu[k] / M * (tp - te);
-      last = tp > te;
+      const double tpv = tp * vf(qp), tev = te * vf(q[k]);
+      sum += mu[k] / Mk * (tpv - tev);
+      last = tpv > tev;
       const double Tp = tp * ex;
-      if (tp > te) cape += A.rd * (tp - te) * ex * P.grav * mu[k] / M / pl;
+      if (tpv > tev) cape += A.rd * (tpv - tev) * ex * P.grav * mu[k] / Mk / pl;
       if (q0c < 0 && Tp < A.Tf) {
         q0c = qp;
         p0c = pl;
@@ -975,7 +1011,7 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
     }
     pt -= P.grav * mu[k];
     lift(pt);
-    if (!(tp > b)) stop = true;
+    if (!(tp * vf(qp) > b * vf(k > 0 ? q[k - 1] : 0.0))) stop = true;
   }
   if (q0c >= 0 && q40 < 0) {
     q40 = qd;

@@ -19,7 +19,7 @@ enum { P_GRAV = 13, P_CP = 14, P_KAPPA = 15, P_P0 = 16, P_FLOOR = 11, P_THETA_ST
        P_EDGE_VAP = 86, P_FREEZE_DRY = 85, P_EDGE_K = 101, P_ICE_AIR = 102, P_SAT_PROF = 51, P_SAT_BASE = 53,
        P_BL_Q = 89, P_Q_SHAPE = 83, P_PLUME_ROOT = 88, P_ROOT_Q = 107, P_PLUME_DEEP = 47, P_PLUME_ENT = 78,
        P_ENT_CLOUD = 93, P_ENT_DET = 108, P_L_DET = 65, P_STORM_AREA = 80, P_GRAUPEL_V = 81, P_ICE_SMALL = 82,
-       P_ML_CLEAR = 118, P_CU_CLOUD = 119 };
+       P_ML_CLEAR = 118, P_CU_CLOUD = 119, P_ENT_IFS = 109, P_ENT_LOC = 125, P_DET_ALL = 116, P_BUOY_V = 113 };
 
 static void layer_exner(int L, const double* mu, const double* par, double* pb, double* pib) {
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP];
@@ -761,14 +761,18 @@ static double liq_of(double T, int mix) {
   return fmin(1.0, fmax(0.0, (T - 253.15) / 20.0));
 }
 
-/* plume_ascent (air.cu:540-666, the deep plume's options plume_ent, ent_cloud, ent_det, ice_air): bottom air (th0,
+/* plume_ascent (air.cu:540-666, the deep plume's options plume_ent, ent_cloud, ent_det, ice_air, ent_ifs, ent_loc,
+ * det_all, buoy_v; air.cu:564-575, 583-590, 612-624 for the last four): bottom air (th0,
  * q0) lifted from the bottom layer's top (pi_) up through the layers, in M sub-levels each, condensing as it goes and
  * mixing with (plume_ent) and, through the layer above the bottom one, entraining (ent_cloud, its mass growing as
  * exp(ent_cloud z); ent_det: as much detrained) the air it passes, whose theta at mass share x above the layer's
  * lower edge a is a + (b - a) ln(1 + x (e^s - 1)) / s. A: its buoyancy summed by mass up to the highest level it
  * beats (the bottom layer's stretch a triangle); kd, thd, qd: the layer it detrains into, its theta and vapor there;
  * qx: what its mixing gave each layer per kg; G its mass per kg leaving the bottom layer; FZ its frozen condensate
- * (ice_air); lt: its condensate freezing between 0 and -40 C, those levels' pressures and its CAPE, for lightning. */
+ * (ice_air); lt: its condensate freezing between 0 and -40 C, those levels' pressures and its CAPE, for lightning.
+ * ent_ifs: the entraining layers' entrainment and detrainment the IFS's, 1.75e-3 /m (1.3 - RH) (q_s / q_s at cloud
+ * base)^3 and 0.75e-4 /m, in sub-levels of ~25 hPa; det_all: every tropospheric layer it crosses entraining;
+ * ent_loc: the air taken in at the layer's relative humidity level by level; buoy_v: buoyancy in virtual theta. */
 static void plume_ascent(int L, const double* mu, const double* th, double th0, double q0, double pi_,
                          const double* par, const og_air* air, int mix, double* A, int* kd, double* thd, double* qd,
                          const double* q, double* qx, double* lt, double* G, double* FZ) {
@@ -789,7 +793,12 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
   } while (0)
   double ex0;
   LIFT(pi_, ex0);
-  double sum = 0.5 * mu[bot] * (tp - tstar(bot - 1, L, par)), pt = pi_;
+  /* buoy_v: buoyancy in virtual theta, theta (1 + 0.608 q), the plume's vapor against its surroundings' */
+#define VF(qv_) (par[P_BUOY_V] > 0 ? 1 + 0.608 * (qv_) : 1.0)
+  const int ifs = par[P_ENT_IFS] > 0;
+  double sum = 0.5 * mu[bot] * (tp * VF(qp) - tstar(bot - 1, L, par) * VF(q[bot - 1])), pt = pi_;
+  /* (ent_ifs: the environment's saturation at cloud base, the mixed layer's top) */
+  const double qsb = ifs ? qsat_x(th[bot] * fpow_(pi_ / p0, kappa, mix), pi_, eps, par[P_ICE_AIR], mix) : 1.0;
   *A = 0;
   *kd = -1;
   *thd = tp;
@@ -802,32 +811,68 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
   for (int k = bot - 1; k >= 0; k--) {
     const double a = tstar(k, L, par), b = k > 0 ? tstar(k - 1, L, par) : 0.0;
     const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !mix ? expm1(s) : 0.0;
-    const int ek = k > 0 && k == bot - 1;
-    const double f = par[P_PLUME_ENT] > 0 ? omexp(par[P_PLUME_ENT] * grav * mu[k] / M, mix) : 0.0;
+    /* (ent_ifs: the cloud layer's ascent in sub-levels of ~25 hPa; det_all: the IFS's entrainment and detrainment
+     * through every tropospheric layer the plume crosses, below edge_k's stratosphere) */
+    const int ek = k > 0 && (k == bot - 1 || (par[P_DET_ALL] > 0 && ifs && k >= (int)par[P_EDGE_K]));
+    int Mk = M;
+    if (ifs && ek) {
+      const int n = (int)ceil(grav * mu[k] / 2500.0);
+      Mk = n > M ? n : M;
+      if (Mk > 48) Mk = 48;
+    }
+    const double f = par[P_PLUME_ENT] > 0 ? omexp(par[P_PLUME_ENT] * grav * mu[k] / Mk, mix) : 0.0;
+    /* ent_loc: the air taken in at each sub-level holds the layer's vapor at its relative humidity there; qsl the
+     * mean of the sub-levels' saturation */
+    const int loc = par[P_ENT_LOC] > 0 && par[P_ENT_CLOUD] > 0 && ifs && ek && k >= (int)par[P_EDGE_K] && q[k] > 0;
+    double qsl = 0;
+    if (loc)
+      for (int j = 0; j < Mk; j++) {
+        const double x = (j + 0.5) / Mk, pl = pt - x * grav * mu[k];
+        const double te = a + (b - a) * spread_of(x, s, es, mix);
+        qsl += qsat_x(te * fpow_(pl / p0, kappa, mix), pl, eps, par[P_ICE_AIR], mix) / Mk;
+      }
     int last = 0;
-    for (int j = 0; j < M; j++) {
-      const double x = (j + 0.5) / M, pl = pt - x * grav * mu[k];
+    for (int j = 0; j < Mk; j++) {
+      const double x = (j + 0.5) / Mk, pl = pt - x * grav * mu[k];
       double ex;
       LIFT(pl, ex);
       const double te = k == 0 ? th[0] : a + (b - a) * spread_of(x, s, es, mix);
       if (par[P_ENT_CLOUD] > 0 && ek) {
-        const double dz = rd * te * ex / grav * (grav * mu[k] / M) / pl;
-        const double g_ = expm1(par[P_ENT_CLOUD] * dz), fe = g_ / (1 + g_);
-        if (par[P_ENT_DET] > 0) qa[k] += Mp * fe * (qp - q[k]);
-        tp += fe * (te - tp);
-        qp += fe * (q[k] - qp);
-        if (!(par[P_ENT_DET] > 0)) Mp *= 1 + g_;
+        double dz = rd * te * ex / grav * (grav * mu[k] / Mk) / pl;
+        if (ifs && j == 0) dz *= 0.5;                 /* (from cloud base to the first sub-level's middle) */
+        if (ifs) {
+          /* ent_ifs: the IFS's entrainment 1.75e-3 /m (1.3 - RH) (q_s / q_s at cloud base)^3 and mixing detrainment
+           * 0.75e-4 /m (Bechtold et al. 2008, 2014): the detrainment an equal-mass exchange, the rest of the
+           * entrainment growing the plume */
+          const double qse = qsat_x(te * ex, pl, eps, par[P_ICE_AIR], mix), qen = loc && qsl > 0 ? q[k] * qse / qsl : q[k];
+          const double rh = fmin(1.0, fmax(0.0, qen / qse));
+          const double fs = fmin(1.0, qse / qsb);
+          const double eps_ = 1.75e-3 * (1.3 - rh) * fs * fs * fs, del_ = fmin(eps_, 0.75e-4);
+          const double fe_ = -expm1(-eps_ * dz), fd_ = -expm1(-del_ * dz);
+          qa[k] += Mp * fd_ * (qp - qen);
+          if (loc) qa[k] -= Mp * expm1((eps_ - del_) * dz) * (qen - q[k]);
+          tp += fe_ * (te - tp);
+          qp += fe_ * (qen - qp);
+          Mp *= exp((eps_ - del_) * dz);
+        } else {
+          const double g_ = expm1(par[P_ENT_CLOUD] * dz), fe = g_ / (1 + g_);
+          if (par[P_ENT_DET] > 0) qa[k] += Mp * fe * (qp - q[k]);
+          tp += fe * (te - tp);
+          qp += fe * (q[k] - qp);
+          if (!(par[P_ENT_DET] > 0)) Mp *= 1 + g_;
+        }
       }
       if (par[P_PLUME_ENT] > 0) {
         qa[k] += Mp * f * (qp - q[k]);
         tp += f * (te - tp);
         qp += f * (q[k] - qp);
       }
-      sum += mu[k] / M * (tp - te);
-      last = tp > te;
+      const double tpv = tp * VF(qp), tev = te * VF(q[k]);
+      sum += mu[k] / Mk * (tpv - tev);
+      last = tpv > tev;
       {
         const double Tp = tp * ex;
-        if (tp > te) cape += rd * (tp - te) * ex * grav * mu[k] / M / pl;
+        if (tpv > tev) cape += rd * (tpv - tev) * ex * grav * mu[k] / Mk / pl;
         if (q0c < 0 && Tp < Tf) {
           q0c = qp;
           p0c = pl;
@@ -851,9 +896,10 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
     pt -= grav * mu[k];
     double ext;
     LIFT(pt, ext);
-    if (!(tp > b)) break;
+    if (!(tp * VF(qp) > b * VF(q[k - 1]))) break;
   }
 #undef LIFT
+#undef VF
   if (q0c >= 0 && q40 < 0) {
     q40 = *qd;
     p40 = pt;

@@ -425,10 +425,14 @@ def test_earth_options(L, mixed, opts):
     (4, False, dict(ent_cloud=1e-4, ent_det=1.0, cloud=1.0, ice_air=1.0, downdraft=0.3, graupel_v=1.0)),
     (5, False, dict(plume_root=1.0, plume_ent=1e-5, downdraft=0.3, revap=1.0, cloud=1.0, sat_prof=1.0, edge_k=2.0)),
     (3, True, dict(plume_ent=1e-5, ent_cloud=1e-4)),
-    (7, True, dict(plume_ent=1e-5, ent_cloud=1e-4, cloud=1.0, ice_air=1.0, downdraft=0.3, graupel_v=1.0))])
+    (7, True, dict(plume_ent=1e-5, ent_cloud=1e-4, cloud=1.0, ice_air=1.0, downdraft=0.3, graupel_v=1.0)),
+    (4, False, dict(ent_cloud=1e-4, ent_ifs=1.0, buoy_v=1.0, plume_ent=1e-5)),
+    (5, False, dict(ent_cloud=1e-4, ent_ifs=1.0, det_all=1.0, ent_loc=1.0, cloud=1.0, ice_air=1.0, graupel_v=1.0)),
+    (4, True, dict(ent_cloud=1e-4, ent_ifs=1.0, det_all=1.0, ent_loc=1.0, buoy_v=1.0))])
 def test_deep_plume(L, mixed, opts):
     """The deep plume (its full ascent with mixing and the cloud layer's entrainment, detrainment as cloud, frozen
-    share, lightning, downdrafts) to the moist column's bars."""
+    share, lightning, downdrafts; the IFS's entrainment and detrainment through the cloud layer or every tropospheric
+    layer, the entrained air's vapor level by level, virtual buoyancy) to the moist column's bars."""
     from oracle import column as ocol
     opts = dict(opts)
     N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
