@@ -131,7 +131,7 @@ struct AP {
   double sat_prof, sat_base, edge_vap, freeze_dry, q_shape, plume_root, root_q, bl_q;
   // the deep plume's
   double plume_deep, plume_ent, ent_cloud, ent_det, l_det, storm_area, graupel_v, ice_small, ent_ifs, ent_loc, det_all,
-      buoy_v;
+      buoy_v, shallow;
   double ml_clear, cu_cloud;                           // (the mixed layer's cloud with bl_q)
 };
 
@@ -157,7 +157,7 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.sat_prof = a[51]; p.sat_base = a[53]; p.edge_vap = a[86]; p.freeze_dry = a[85]; p.q_shape = a[83];
   p.plume_root = a[88]; p.root_q = a[107]; p.bl_q = a[89];
   p.plume_deep = a[47]; p.plume_ent = a[78]; p.ent_cloud = a[93]; p.ent_det = a[108]; p.l_det = a[65];
-  p.ent_ifs = a[109]; p.ent_loc = a[125]; p.det_all = a[116]; p.buoy_v = a[113];
+  p.ent_ifs = a[109]; p.ent_loc = a[125]; p.det_all = a[116]; p.buoy_v = a[113]; p.shallow = a[110];
   p.storm_area = a[80]; p.graupel_v = a[81]; p.ice_small = a[82];
   p.ml_clear = a[118]; p.cu_cloud = a[119];
   return p;
@@ -902,7 +902,7 @@ __device__ __forceinline__ double liq_of(double T) {
 template <int L, bool MIX>
 __device__ void plume_ascent(const double* mu, const double* th, const double* q, double th0, double q0, double pi_,
                              const CP& P, const AP& A, double& Ab, int& kd, double& thd, double& qd, double* qx,
-                             double* lt, double& G, double& FZ) {
+                             double* lt, double& G, double& FZ, bool shal = false) {
   constexpr int bot = L - 1, M = bot > 2 ? 3 : 8;
   double Mp = 1, tp = th0, qp = q0, fz = 0;
   auto lift = [&](double p_) {
@@ -939,7 +939,7 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
     if (stop) continue;
     const double a = tstar<L>(k, P), b = k > 0 ? tstar<L>(k - 1, P) : 0.0;
     const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !MIX ? expm1(s) : 0.0;
-    const bool ek = k > 0 && (k == bot - 1 || (A.det_all > 0 && ifs && k >= (int)A.edge_k));
+    const bool ek = k > 0 && (k == bot - 1 || (A.det_all > 0 && ifs && !shal && k >= (int)A.edge_k));
     const int Mk = ifs && ek ? min(48, max(M, (int)ceil(P.grav * mu[k] / 2500.0))) : M;
     const double f = A.plume_ent > 0 ? omexp<MIX>(A.plume_ent * P.grav * mu[k] / Mk) : 0.0;
     const bool loc = A.ent_loc > 0 && A.ent_cloud > 0 && ifs && ek && k >= (int)A.edge_k && q[k] > 0;
@@ -961,7 +961,7 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
         if (ifs) {
           const double qse = qsat_(te * ex, pl), qen = loc && qsl > 0 ? q[k] * qse / qsl : q[k];
           const double rh = fmin(1.0, fmax(0.0, qen / qse)), fs = fmin(1.0, qse / qsb);
-          const double eps_ = 1.75e-3 * (1.3 - rh) * fs * fs * fs, del_ = fmin(eps_, 0.75e-4);
+          const double eps_ = (shal ? 3.5e-3 : 1.75e-3) * (1.3 - rh) * fs * fs * fs, del_ = fmin(eps_, 0.75e-4);
           const double fe_ = -expm1(-eps_ * dz), fd_ = -expm1(-del_ * dz);
           qa[k] += Mp * fd_ * (qp - qen);
           if (loc) qa[k] -= Mp * expm1((eps_ - del_) * dz) * (qen - q[k]);   // (the growth taken in at qen)
@@ -1005,7 +1005,7 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
         for (int m = 0; m < L; m++) qx[m] = qa[m];
       }
     }
-    if (!last || k == 0) {
+    if (!last || k == 0 || shal) {                  // (the shallow plume stops in the cloud layer)
       stop = true;
       continue;
     }
@@ -1787,7 +1787,8 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   // ---- moist convection: a parcel of bottom air lifted to the interface above it; buoyant there, it rises through
   // every interface it stays buoyant at and detrains above the last, its condensate raining; the mass that would
   // remove the buoyancy at the first interface relaxes over tau_conv ----
-  double mc = 0, Scg = 0;
+  double mc = 0, Scg = 0, qss_r = 0, ex_r = -1, ws_r = 0, rho_r = 0;   // (the root's saturation, the thermals'
+                                                                         // excess, w* and density: the shallow plume's)
   if (o.flash) o.flash[i] = 0;
   if (MOIST && P.theta_star > 0) {
     double pi_ = 0;
@@ -1811,6 +1812,10 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         const double wm = cbrt(us3 + 0.28 * P.grav / Tsa * Bk * hb);
         const double ex = wm > 1e-3 ? 8.5 * fmax(0.0, E) / (rho * wm) : 0.0;
         q0 += fmin(ex, fmax(0.0, qss - q0));
+        qss_r = qss;
+        ex_r = ex;
+        rho_r = rho;
+        ws_r = cbrt(P.grav / Tsa * Bk * hb);
       } else if (CLOUD) {
         q0 = fmax(q0, fmin(qss, q0 + (1 - A.rh_crit) * qss));
       }
@@ -1975,6 +1980,68 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         mc = 0;
       }
     }
+    }
+    // shallow (air.cu:1730-1835, without the cumulus layer): a second plume from the same root (plume_root with
+    // root_q), entraining twice the deep one's with ent_ifs, stopping in the cloud layer and leaving its air there;
+    // its base mass flux Grant (2001)'s rho a w* (a = shallow), cut to the air and vapor the layers hold. Its
+    // condensate leaves as the deep plume's: l_det as cloud, the rest rains, its frozen share as snow
+    if (A.shallow > 0 && A.plume_deep > 0 && bot >= 2 && ex_r >= 0 && ws_r > 0) {
+      constexpr int ks = bot >= 1 ? bot - 1 : 0;
+      double pis = 0;
+#pragma unroll
+      for (int k = 0; k < bot; k++) pis += P.grav * mu[k];
+      const double ths0 = th[bot];
+      double qs0 = edge_q<L, MIX>(bot, false, mu, th, q, 0, P, A);
+      qs0 += fmin(ex_r, fmax(0.0, qss_r - qs0));
+      double As, Gs = 1, FZs = 0, thps, qcs, qxs[L], lts[4], xss = 0;
+      int kds;
+      plume_ascent<L, MIX>(mu, th, q, ths0, qs0, pis, P, A, As, kds, thps, qcs, qxs, lts, Gs, FZs, true);
+#pragma unroll
+      for (int k = 0; k <= bot; k++) xss += qxs[k];
+      if (kds == ks && As > 0) {
+        // its air taken in from the cloud layer (Gs - 1 per kg leaving the mixed layer) leaves it at the layer's vapor
+        // and comes back at the plume's: at most the layer's air above its floor, and what it and the mixing take at
+        // most half the layer's vapor
+        const double qe = q[ks];
+        double ms = fmin(A.shallow * rho_r * ws_r * dt, mu[bot] - P.floor_);
+        if (qs0 > 0) ms = fmin(ms, mu[bot] * q[bot] / qs0);
+        if (Gs > 1) ms = fmin(ms, (mu[ks] - P.floor_) / (Gs - 1));
+#pragma unroll
+        for (int k = 0; k < bot; k++) {
+          const double take = (k == ks ? (Gs - 1) * qe : 0.0) - qxs[k];
+          if (take > 0) ms = fmin(ms, 0.5 * mu[k] * q[k] / take);
+        }
+        if (ms > 0) {
+          const double cond_ = qs0 + (Gs - 1) * qe - Gs * qcs - xss, cond = fmax(0.0, cond_);
+          const double det = CLOUD ? fmin(cond, A.l_det * Gs) : 0.0, rs = ms * (cond - det);
+          const double qarr = Gs * qcs + (cond_ - cond) - (Gs - 1) * qe;   // (per kg moved: Gs kg arrive)
+          const double Ec = column_energy<L, MOIST>(mu, pib, th, q, P);
+#pragma unroll
+          for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + ms * qxs[k] / mu[k]);
+          move_mass<L, true, CLOUD>(P, mu, th, q, pib, bot, ks, ms, ths0, thps, qs0, qarr, cl);
+          th[ks] += (Ec - column_energy<L, MOIST>(mu, pib, th, q, P)) / (mu[ks] * pib[ks]);
+          R += rs;
+          Rc += rs;
+          if (A.ice_air > 0 && cond > det) {           // (its frozen share snows, melting in a warm bottom layer)
+            const double Ss = rs * fmin(1.0, FZs / cond);
+            th[ks] += A.Lf * Ss / (mu[ks] * pib[ks]);
+            double ps_ = 0;
+#pragma unroll
+            for (int k = 0; k < L; k++) ps_ += P.grav * mu[k];
+            if (base_T<L, MIX>(bot, th[bot], ps_, P, A) > A.Tf) th[bot] -= A.Lf * Ss / (mu[bot] * pib[bot]);
+            else {
+              S += Ss;
+              Scg += Ss;
+            }
+          }
+          wxo[ks] += ms;
+          if constexpr (CLOUD) {
+            cl[ks] += ms * det / mu[ks];
+            cloud_adjust_k<L, MIX>(ks, mu, th, q, cl, pib, dt, P, A);
+          }
+          mc += ms;
+        }
+      }
     }
   }
   // the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the

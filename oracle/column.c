@@ -19,7 +19,8 @@ enum { P_GRAV = 13, P_CP = 14, P_KAPPA = 15, P_P0 = 16, P_FLOOR = 11, P_THETA_ST
        P_EDGE_VAP = 86, P_FREEZE_DRY = 85, P_EDGE_K = 101, P_ICE_AIR = 102, P_SAT_PROF = 51, P_SAT_BASE = 53,
        P_BL_Q = 89, P_Q_SHAPE = 83, P_PLUME_ROOT = 88, P_ROOT_Q = 107, P_PLUME_DEEP = 47, P_PLUME_ENT = 78,
        P_ENT_CLOUD = 93, P_ENT_DET = 108, P_L_DET = 65, P_STORM_AREA = 80, P_GRAUPEL_V = 81, P_ICE_SMALL = 82,
-       P_ML_CLEAR = 118, P_CU_CLOUD = 119, P_ENT_IFS = 109, P_ENT_LOC = 125, P_DET_ALL = 116, P_BUOY_V = 113 };
+       P_ML_CLEAR = 118, P_CU_CLOUD = 119, P_ENT_IFS = 109, P_ENT_LOC = 125, P_DET_ALL = 116, P_BUOY_V = 113,
+       P_SHALLOW = 110 };
 
 static void layer_exner(int L, const double* mu, const double* par, double* pb, double* pib) {
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP];
@@ -775,7 +776,7 @@ static double liq_of(double T, int mix) {
  * ent_loc: the air taken in at the layer's relative humidity level by level; buoy_v: buoyancy in virtual theta. */
 static void plume_ascent(int L, const double* mu, const double* th, double th0, double q0, double pi_,
                          const double* par, const og_air* air, int mix, double* A, int* kd, double* thd, double* qd,
-                         const double* q, double* qx, double* lt, double* G, double* FZ) {
+                         const double* q, double* qx, double* lt, double* G, double* FZ, int shal) {
   const int bot = L - 1, M = bot > 2 ? 3 : 8;
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP], Lf = par[P_LF];
   const double Tf = par[P_TF], rd = air->rd, eps = air->eps;
@@ -813,7 +814,7 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
     const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !mix ? expm1(s) : 0.0;
     /* (ent_ifs: the cloud layer's ascent in sub-levels of ~25 hPa; det_all: the IFS's entrainment and detrainment
      * through every tropospheric layer the plume crosses, below edge_k's stratosphere) */
-    const int ek = k > 0 && (k == bot - 1 || (par[P_DET_ALL] > 0 && ifs && k >= (int)par[P_EDGE_K]));
+    const int ek = k > 0 && (k == bot - 1 || (par[P_DET_ALL] > 0 && ifs && !shal && k >= (int)par[P_EDGE_K]));
     int Mk = M;
     if (ifs && ek) {
       const int n = (int)ceil(grav * mu[k] / 2500.0);
@@ -847,7 +848,7 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
           const double qse = qsat_x(te * ex, pl, eps, par[P_ICE_AIR], mix), qen = loc && qsl > 0 ? q[k] * qse / qsl : q[k];
           const double rh = fmin(1.0, fmax(0.0, qen / qse));
           const double fs = fmin(1.0, qse / qsb);
-          const double eps_ = 1.75e-3 * (1.3 - rh) * fs * fs * fs, del_ = fmin(eps_, 0.75e-4);
+          const double eps_ = (shal ? 3.5e-3 : 1.75e-3) * (1.3 - rh) * fs * fs * fs, del_ = fmin(eps_, 0.75e-4);
           const double fe_ = -expm1(-eps_ * dz), fd_ = -expm1(-del_ * dz);
           qa[k] += Mp * fd_ * (qp - qen);
           if (loc) qa[k] -= Mp * expm1((eps_ - del_) * dz) * (qen - q[k]);
@@ -892,7 +893,7 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
         for (int m = 0; m <= bot; m++) qx[m] = qa[m];
       }
     }
-    if (!last || k == 0) break;
+    if (!last || k == 0 || shal) break;             /* (the shallow plume stops in the cloud layer) */
     pt -= grav * mu[k];
     double ext;
     LIFT(pt, ext);
@@ -1546,7 +1547,8 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     /* moist convection: a parcel of bottom-layer air lifted to the interface above it; buoyant there (theta_p above
      * theta*), it rises through every interface it stays buoyant at and detrains above the last, its condensate
      * raining; the mass that would remove the buoyancy at the first interface relaxes over tau_conv */
-    double mc = 0, Scg = 0;
+    double mc = 0, Scg = 0, qss_r = 0, ex_r = -1, ws_r = 0, rho_r = 0;   /* (the root's saturation, the thermals'
+                                                                           * excess, w* and density: the shallow plume's) */
     if (flash) flash[i] = 0;
     if (moist && par[P_THETA_STAR] > 0) {
       double pi_ = 0;
@@ -1569,6 +1571,10 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           const double wm = cbrt(us3 + 0.28 * grav / Tsa * Bk * hb);
           const double ex = wm > 1e-3 ? 8.5 * fmax(0.0, E) / (rho * wm) : 0.0;
           q0 += fmin(ex, fmax(0.0, qss - q0));
+          qss_r = qss;
+          ex_r = ex;
+          rho_r = rho;
+          ws_r = cbrt(grav / Tsa * Bk * hb);
         } else if (cloudy) {
           q0 = fmax(q0, fmin(qss, q0 + (1 - par[P_RH_CRIT]) * qss));
         }
@@ -1586,7 +1592,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
          * rain feeds a downdraft of the cloud layer's air, saturated by evaporating it */
         double A, G = 1, FZ = 0, qx[OG_LMAX], lt[4], xs = 0;
         int kd;
-        plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thp, &qc, q, qx, lt, &G, &FZ);
+        plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thp, &qc, q, qx, lt, &G, &FZ, 0);
         for (int k = 0; k <= bot; k++) xs += qx[k];
         if (kd >= 0 && A > 0) {
           const double rise = thp - tstar(bot - 1, L, par);
@@ -1697,6 +1703,61 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           mc = 0;
         }
       }
+      }
+      /* shallow (air.cu:1730-1835, without the cumulus layer): a second plume from the same root (plume_root with
+       * root_q), entraining twice the deep one's with ent_ifs, stopping in the cloud layer and leaving its air there;
+       * its base mass flux Grant (2001)'s rho a w* (a = shallow), cut to the air and vapor the layers hold. Its
+       * condensate leaves as the deep plume's: l_det as cloud, the rest rains, its frozen share as snow */
+      if (par[P_SHALLOW] > 0 && par[P_PLUME_DEEP] > 0 && bot >= 2 && ex_r >= 0 && ws_r > 0) {
+        double pis = 0;
+        for (int k = 0; k < bot; k++) pis += grav * mu[k];
+        const double ths0 = th[bot];
+        double qs0 = edge_q(L, bot, 0, mu, th, q, 0, par, aeps, mix);
+        qs0 += fmin(ex_r, fmax(0.0, qss_r - qs0));
+        double As, Gs = 1, FZs = 0, thps, qcs, qxs[OG_LMAX], xss = 0, lts[4];
+        int kds;
+        plume_ascent(L, mu, th, ths0, qs0, pis, par, air, mix, &As, &kds, &thps, &qcs, q, qxs, lts, &Gs, &FZs, 1);
+        for (int k = 0; k <= bot; k++) xss += qxs[k];
+        if (kds == bot - 1 && As > 0) {
+          /* its air taken in from the cloud layer (Gs - 1 per kg leaving the mixed layer) leaves it at the layer's
+           * vapor and comes back at the plume's: at most the layer's air above its floor, and what it and the mixing
+           * take at most half the layer's vapor */
+          const int ks = bot - 1;
+          const double qe = q[ks];
+          double ms = fmin(par[P_SHALLOW] * rho_r * ws_r * dt, mu[bot] - par[P_FLOOR]);
+          if (qs0 > 0) ms = fmin(ms, mu[bot] * q[bot] / qs0);
+          if (Gs > 1) ms = fmin(ms, (mu[ks] - par[P_FLOOR]) / (Gs - 1));
+          for (int k = 0; k < bot; k++) {
+            const double take = (k == ks ? (Gs - 1) * qe : 0.0) - qxs[k];
+            if (take > 0) ms = fmin(ms, 0.5 * mu[k] * q[k] / take);
+          }
+          if (ms > 0) {
+            const double cond_ = qs0 + (Gs - 1) * qe - Gs * qcs - xss, cond = fmax(0.0, cond_);
+            const double det = cloudy ? fmin(cond, par[P_L_DET] * Gs) : 0.0, rs = ms * (cond - det);
+            const double qarr = Gs * qcs + (cond_ - cond) - (Gs - 1) * qe;   /* (per kg moved: Gs kg arrive) */
+            const double Ec = column_energy(L, mu, pib, th, q, par);
+            for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + ms * qxs[k] / mu[k]);
+            move_mass(L, par, mu, th, q, pib, bot, kds, ms, ths0, thps, qs0, qarr, cloudy ? cl : NULL);
+            th[kds] += (Ec - column_energy(L, mu, pib, th, q, par)) / (mu[kds] * pib[kds]);
+            R += rs;
+            Rc += rs;
+            if (par[P_ICE_AIR] > 0 && cond > det) {      /* (its frozen share snows, melting in a warm bottom layer) */
+              const double Ss = rs * fmin(1.0, FZs / cond);
+              th[kds] += Lf * Ss / (mu[kds] * pib[kds]);
+              if (base_T(L, bot, mu, th, par, mix) > Tf) th[bot] -= Lf * Ss / (mu[bot] * pib[bot]);
+              else {
+                S += Ss;
+                Scg += Ss;
+              }
+            }
+            wxo[kds] += ms;
+            if (cloudy) {
+              cl[kds] += ms * det / mu[kds];
+              cloud_adjust(L, kds, mu, th, q, cl, pib, dt, par, aeps, mix);
+            }
+            mc += ms;
+          }
+        }
       }
     }
     /* the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the

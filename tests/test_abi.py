@@ -84,8 +84,8 @@ def test_bad_arguments_rejected_on_host():
     par[111] = 0.0
     par[22] = 1.0                                            # moist air: no vapor or ice given; the moist options
     assert air_column() == E
-    par[47] = 1.0                                            # plume_deep with the shallow plume, the cumulus
-    for k in (110, 111, 114):                                # layer or its subsidence
+    par[47] = 1.0                                            # plume_deep with the cumulus layer or its
+    for k in (111, 114):                                     # subsidence
         par[k] = 1.0
         assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf,
                                   600.0, *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E

@@ -428,11 +428,16 @@ def test_earth_options(L, mixed, opts):
     (7, True, dict(plume_ent=1e-5, ent_cloud=1e-4, cloud=1.0, ice_air=1.0, downdraft=0.3, graupel_v=1.0)),
     (4, False, dict(ent_cloud=1e-4, ent_ifs=1.0, buoy_v=1.0, plume_ent=1e-5)),
     (5, False, dict(ent_cloud=1e-4, ent_ifs=1.0, det_all=1.0, ent_loc=1.0, cloud=1.0, ice_air=1.0, graupel_v=1.0)),
-    (4, True, dict(ent_cloud=1e-4, ent_ifs=1.0, det_all=1.0, ent_loc=1.0, buoy_v=1.0))])
+    (4, True, dict(ent_cloud=1e-4, ent_ifs=1.0, det_all=1.0, ent_loc=1.0, buoy_v=1.0)),
+    (4, False, dict(plume_root=1.0, root_q=1.0, sfc=1.0, shallow=0.03, ent_cloud=1e-4, ent_ifs=1.0, cloud=1.0,
+                    ice_air=1.0)),
+    (5, False, dict(plume_root=1.0, root_q=1.0, sfc=1.0, shallow=0.03, ent_cloud=1e-4, plume_ent=1e-5)),
+    (4, True, dict(plume_root=1.0, root_q=1.0, sfc=1.0, shallow=0.03, ent_cloud=1e-4, ent_ifs=1.0))])
 def test_deep_plume(L, mixed, opts):
     """The deep plume (its full ascent with mixing and the cloud layer's entrainment, detrainment as cloud, frozen
     share, lightning, downdrafts; the IFS's entrainment and detrainment through the cloud layer or every tropospheric
-    layer, the entrained air's vapor level by level, virtual buoyancy) to the moist column's bars."""
+    layer, the entrained air's vapor level by level, virtual buoyancy; the shallow plume) to the moist column's
+    bars."""
     from oracle import column as ocol
     opts = dict(opts)
     N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
@@ -442,6 +447,8 @@ def test_deep_plume(L, mixed, opts):
     kw = dict(ocean=True, q=c['q'], ice=c['ice'])
     if cloudy:
         kw.update(cw=c['cw'], cloudo=True)
+    if opts.get('sfc'):
+        kw['ch'] = np.random.default_rng(165 + L).uniform(1e-3, 3e-2, N)
     p = par(L, moist=1.0, plume_deep=1.0, **opts)
     g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, **kw)
     o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed, **kw)

@@ -912,7 +912,10 @@ DEEP_OPTS = [dict(), dict(plume_ent=1e-5), dict(ent_cloud=1e-4), dict(ent_cloud=
              dict(plume_root=1.0, root_q=1.0, sfc=1.0, ent_cloud=1e-4, sat_prof=1.0, edge_vap=1.0),
              dict(buoy_v=1.0, plume_ent=1e-5), dict(ent_cloud=1e-4, ent_ifs=1.0),
              dict(ent_cloud=1e-4, ent_ifs=1.0, det_all=1.0, ent_loc=1.0, plume_ent=1e-5, buoy_v=1.0, cloud=1.0,
-                  ice_air=1.0)]
+                  ice_air=1.0),
+             dict(plume_root=1.0, root_q=1.0, sfc=1.0, shallow=0.03, ent_cloud=1e-4, ent_ifs=1.0, cloud=1.0,
+                  ice_air=1.0),
+             dict(plume_root=1.0, root_q=1.0, sfc=1.0, shallow=0.03, ent_cloud=1e-4, plume_ent=1e-5)]
 
 
 def deep_columns(N=96, L=4, seed=5):
@@ -1009,6 +1012,32 @@ def test_deep_plume_ifs_and_virtual_buoyancy():
     wet = dict(c, q=c['q'].copy())                   # (plumes through a moister cloud layer reach the layer above)
     wet['q'][2] *= 1.3
     assert not np.array_equal(run_(wet, ent_ifs=1.0, det_all=1.0)['q'], run_(wet, ent_ifs=1.0)['q'])
+
+
+def test_shallow_plume():
+    """The shallow plume (shallow: a second plume from the surface air's root with the thermals' excess, stopping in
+    the cloud layer) moves air only from the bottom layer into the cloud layer above it: the layers above those and
+    the interfaces between them are the deep plume's alone, bit for bit; it adds its base mass flux rho a w* (cut to
+    what the layers hold) to the convection, growing with a; without the root's thermals (root_q off) it is off."""
+    c = deep_columns()
+    N = c['Ts'].size
+    base = dict(moist=1.0, plume_deep=1.0, plume_root=1.0, root_q=1.0, sfc=1.0, ent_cloud=1e-4, ent_ifs=1.0)
+    run_ = lambda **kw: run(c, par4(**dict(base, **kw)), q=c['q'], ice=c['ice'], dt=900.0, ch=np.full(N, 0.02))
+    n, s1, s2 = run_(), run_(shallow=0.03), run_(shallow=0.06)
+    for k in ('q', 'theta'):
+        np.testing.assert_array_equal(s1[k][:2], n[k][:2])
+    np.testing.assert_array_equal(s1['wx'][:2], n['wx'][:2])
+    go = s1['conv'] > n['conv']
+    assert go.sum() > N // 4 and np.all(s1['conv'] >= n['conv'])
+    d1, d2 = s1['conv'] - n['conv'], s2['conv'] - n['conv']
+    assert np.all(d2 >= d1 - 1e-18)
+    # where no cap binds, the flux is linear in a: twice the fraction, twice the flux, to rounding
+    lin = go & np.isclose(d2, 2 * d1, rtol=1e-9, atol=0)
+    assert lin.sum() > go.sum() // 4
+    assert not np.array_equal(s1['q'][2], n['q'][2])
+    off = run(c, par4(**dict(base, root_q=0.0, shallow=0.03)), q=c['q'], ice=c['ice'], dt=900.0, ch=np.full(N, 0.02))
+    ref = run(c, par4(**dict(base, root_q=0.0)), q=c['q'], ice=c['ice'], dt=900.0, ch=np.full(N, 0.02))
+    np.testing.assert_array_equal(off['q'], ref['q'])
 
 
 # ---- bl_q: the mixed bottom layer's even vapor and its cloud statistics (mixed_cloud, ml_clear, cu_cloud) ----
