@@ -1089,7 +1089,7 @@ struct AirIO {
   double *soil, *pond, *snow;
 };
 
-template <int L, bool MIX, bool MOIST, bool CLOUD>
+template <int L, bool MIX, bool MOIST, bool CLOUD, bool FULL>
 __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, const double* __restrict__ t,
                                                    const double* __restrict__ insol, const double* __restrict__ pos,
                                                    const double* __restrict__ uabs, const double* __restrict__ mu_in,
@@ -1099,6 +1099,12 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
                                                    AirIO o) {
   const int i = XTID;
   if (i >= N) return;
+  if constexpr (!FULL) {
+    // (the plain column, launched only when all of these are off: their branches compile away, and the kernel keeps
+    // the registers they would take)
+    P.bl_mix = 0;
+    A.sat_prof = A.sat_base = A.edge_vap = A.freeze_dry = A.q_shape = A.plume_root = A.plume_deep = A.bl_q = 0;
+  }
   if (elev && A.theta_rise > 0 && elev[i] > 0) P.theta_star += A.theta_rise * elev[i];
   const double ts_base = P.theta_star;
   P.bl_cap = omexp<MIX>(dt / 86400.0);                 // (the mixed layer takes in at most a day's share per step)
@@ -1119,7 +1125,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   // the mixed bottom layer: the lowest interface's theta* no colder than the layer, below the air above's mean
   if (blm && L > 2) P.theta_star = fmin(fmax(ts_base, th[L - 1]), th[L - 2] - 1);
   double m_ice = MOIST ? o.ice[i] : 0.0;
-  const bool isl = MOIST && o.land && o.land[i] > 0.5;   // land: the soil bucket, ponds, snow and the ground's heat
+  const bool isl = MOIST && FULL && o.land && o.land[i] > 0.5;   // land: the soil bucket, ponds, snow and the ground's heat
   double* Ts = o.Ts;
   if (o.cap && A.c_wet > 0 && isl) {                     // (c_wet: the ground's capacity by its water, its heat kept)
     const double r = fmin(1.0, o.soil[i] / o.soilcap[i]), Cn = A.c_dry + (A.c_wet - A.c_dry) * r;
@@ -2200,24 +2206,29 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 #if ISW_COLUMN_CLOUDS
 // The cloudy air column's launches, compiled in a unit of their own (ISW_COLUMN_CLOUDS=1) so that its fourteen
 // kernels build in parallel with the rest of this file.
-int air_column_cloudy(int N, int L, int mixed, const CP& P, const AP& A, int mode, const double* t,
+int air_column_cloudy(int N, int L, int mixed, bool full, const CP& P, const AP& A, int mode, const double* t,
                       const double* insol, const double* pos, const double* uabs, const double* mu, double dt,
                       const double* elev, const double* radf, const double* swf, const double* ch, const AirIO& o,
                       double bytes, hipStream_t st) {
 #define CALL(LL)                                                                                                     \
-  if (mixed)                                                                                                         \
-    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, true, true, true>), grid1(N), dim3(BS), N, P, A, mode, t,      \
-               insol, pos, uabs, mu, dt, elev, radf, swf, ch, o);                                               \
-  else                                                                                                               \
-    ISW_LAUNCH("air_column", bytes, (k_air_column<LL, false, true, true>), grid1(N), dim3(BS), N, P, A, mode, t,     \
-               insol, pos, uabs, mu, dt, elev, radf, swf, ch, o);
+  if (mixed) {                                                                                                       \
+    if (full) LAUNCHC(LL, true, true);                                                                               \
+    else LAUNCHC(LL, true, false);                                                                                   \
+  } else {                                                                                                           \
+    if (full) LAUNCHC(LL, false, true);                                                                              \
+    else LAUNCHC(LL, false, false);                                                                                  \
+  }
+#define LAUNCHC(LL, MX, FU)                                                                                          \
+  ISW_LAUNCH("air_column", bytes, (k_air_column<LL, MX, true, true, FU>), grid1(N), dim3(BS), N, P, A, mode, t,      \
+             insol, pos, uabs, mu, dt, elev, radf, swf, ch, o)
   ISW_BY_L(CALL)
 #undef CALL
+#undef LAUNCHC
   return (int)hipGetLastError();
 }
 #else
 // (defined in the ISW_COLUMN_CLOUDS=1 unit)
-int air_column_cloudy(int N, int L, int mixed, const CP& P, const AP& A, int mode, const double* t,
+int air_column_cloudy(int N, int L, int mixed, bool full, const CP& P, const AP& A, int mode, const double* t,
                       const double* insol, const double* pos, const double* uabs, const double* mu, double dt,
                       const double* elev, const double* radf, const double* swf, const double* ch, const AirIO& o,
                       double bytes, hipStream_t st);
@@ -2267,18 +2278,25 @@ int air_column(int N, int L, const double* par, const isw_air* air, int mode, co
                           (cloudy && cloudo ? (radf ? 4.0 * L + 3 : L + 3.0) : 0) + (land ? 3 : 0) +
                           (land && cap ? 1 : 0);
   const double bytes = 8.0 * N * (rows_in + rows_out);
+  // the moist column's options that take runtime branches of their own: with none of them on it runs the plain
+  // kernel (FULL false), their code compiled away (the dry column is always the full one: bl_mix acts there too)
+  const bool full = land || P.bl_mix > 0 || par[51] > 0 || par[53] > 0 || par[86] > 0 || par[85] > 0 ||
+                    par[83] > 0 || par[88] > 0 || par[47] > 0 || par[89] > 0;
   if (cloudy)
-    return air_column_cloudy(N, L, mixed, P, A, mode, t, insol, pos, uabs, mu, dt, elev, radf, swf, ch, o, bytes, st);
-#define LAUNCH(LL, MX, MO)                                                                                         \
-  ISW_LAUNCH("air_column", bytes, (k_air_column<LL, MX, MO, false>), grid1(N), dim3(BS), N, P, A, mode, t, insol,  \
-             pos, uabs, mu, dt, elev, radf, swf, ch, o)
-#define CALL(LL)                           \
-  if (mixed) {                             \
-    if (moist) LAUNCH(LL, true, true);     \
-    else LAUNCH(LL, true, false);          \
-  } else {                                 \
-    if (moist) LAUNCH(LL, false, true);    \
-    else LAUNCH(LL, false, false);         \
+    return air_column_cloudy(N, L, mixed, full, P, A, mode, t, insol, pos, uabs, mu, dt, elev, radf, swf, ch, o,
+                             bytes, st);
+#define LAUNCH(LL, MX, MO, FU)                                                                                     \
+  ISW_LAUNCH("air_column", bytes, (k_air_column<LL, MX, MO, false, FU>), grid1(N), dim3(BS), N, P, A, mode, t,     \
+             insol, pos, uabs, mu, dt, elev, radf, swf, ch, o)
+#define CALL(LL)                                    \
+  if (mixed) {                                      \
+    if (!moist) LAUNCH(LL, true, false, true);      \
+    else if (full) LAUNCH(LL, true, true, true);    \
+    else LAUNCH(LL, true, true, false);             \
+  } else {                                          \
+    if (!moist) LAUNCH(LL, false, false, true);     \
+    else if (full) LAUNCH(LL, false, true, true);   \
+    else LAUNCH(LL, false, true, false);            \
   }
   ISW_BY_L(CALL)
 #undef CALL
