@@ -286,8 +286,8 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * qocean, qsw, flash, swalb and firn (the firn's temperature) when given; cap (N) the surface's heat capacity, else
  * c_slab. The argument list is the reference's; elev (N) with theta_rise > 0 raises the lowest interface over high
  * ground. Returns ISW_EBADARG for cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or
- * ice, with land but without soilcap, soil, pond or snow, and with plume_deep and cu_dth or cu_sub (the cumulus
- * layer): those parts of the reference's column are not in this library. bl_q (the mixed layer's even vapor and its
+ * ice, with land but without soilcap, soil, pond or snow, with cu_sub (the cumulus layer's subsidence), and with
+ * plume_deep and cu_dth (the cumulus layer): those parts of the reference's column are not in this library. bl_q (the mixed layer's even vapor and its
  * cloud statistics; ml_clear, cu_cloud), the deep plume's IFS entrainment and detrainment (ent_ifs, ent_loc, det_all),
  * virtual buoyancy (buoy_v) and the shallow plume (shallow, rooted by plume_root with root_q) as the reference's.
  * prec: ISW_FP64, or ISW_MIXED (the reference's mixed build: 1 - e^-x, the edge sub-levels' Exner powers and the

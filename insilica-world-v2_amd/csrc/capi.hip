@@ -239,12 +239,13 @@ int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* a
     return ISW_EBADARG;
   if (mode == 0 ? (!t || !pos) : !insol) return ISW_EBADARG;
   // the options this library's column does not carry: the cumulus layer's floating edges, the diagnostics rows, the
-  // gas tracers moved with the crossing; with the deep plume the cumulus layer (cu_dth) and its subsidence. Land
+  // gas tracers moved with the crossing; for moist air the cumulus layer's subsidence (cu_sub, which acts with the
+  // mixed layer's condensation level, plume or no plume) and, with the deep plume, the cumulus layer (cu_dth). Land
   // (moist) needs its bucket's capacity, soil, ponds and snow.
   if ((par[111] > 0 && L > 3) || dgo || xfer) return ISW_EBADARG;
   const bool deep = par[47] > 0;
   if (par[22] > 0 && (!qv || !ice ||
-                      (deep && (par[111] > 0 || par[114] > 0)) ||
+                      par[114] > 0 || (deep && par[111] > 0) ||
                       (land && (!soilcap || !soil || !pond || !snow))))
     return ISW_EBADARG;
   return isw_col::air_column(N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, wx, olr, sw_abs,

@@ -91,6 +91,10 @@ def test_bad_arguments_rejected_on_host():
                                   600.0, *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
         par[k] = 0.0
     par[47] = 0.0
+    par[114] = 1.0                                           # cu_sub acts without the deep plume too
+    assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
+                              *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
+    par[114] = 0.0
     # land without its bucket's capacity, soil, ponds or snow
     assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
                               *([buf] * 8), None, None, None, None, None, buf, buf, buf, None, buf, None, None,
