@@ -35,6 +35,7 @@ extern "C" {
 #define ISW_EBADARG 9001
 #define ISW_LMAX 8
 #define ISW_KSOIL 16
+#define ISW_PAIR_THREADS 256 /* threads per block of the two-sub-step patch kernel: the most edges a patch may own */
 
 /* The mesh on the device (reference: src/insilica/gpu.py:55-80 DeviceGrid; tables from src/insilica/grid/trisk.py).
  * rA, rde, rAV: 1/areaCell, 1/de, 1/areaV; wf, d2wtf: FP32 copies of w and d2wt (read by the mixed form only).
@@ -50,14 +51,17 @@ typedef struct isw_grid {
    * no counterpart in the reference). Patch b (of npatch) holds the cells [b pcells, (b + 1) pcells); its halo cells
    * (the second cells of its edges outside it) ph_cell[ph_off[b] .. ph_off[b + 1]); its edges (those whose first cell
    * eI is in it) pe_edge[pe_off[b] .. pe_off[b + 1]), each with its cells' slots pe_slot = sI | sJ << 16 (a cell of
-   * the patch at its offset in it, a halo cell at pcells + its index in the halo); pslots: the most slots a patch uses. */
-  int32_t npatch, pcells, pslots, pad_;
+   * the patch at its offset in it, a halo cell at pcells + its index in the halo); pslots: the most slots a patch uses;
+   * pown: the most edges a patch owns (max of pe_off[b + 1] - pe_off[b]). */
+  int32_t npatch, pcells, pslots, pown;
   const int32_t *ph_off, *ph_cell, *pe_off, *pe_edge, *pe_slot;
   /* Two air fast sub-steps per launch on the same patches (optional: ph2_off NULL keeps one per launch). The first
    * sub-step's cells: patch, halo and the halo's halo ph2_cell[ph2_off[b] ..] (slots after the halo's); its winds on
    * every edge of patch and halo: the patch's edges (slots 0 .. ne - 1) and px_edge[px_off[b] ..] (slots from ne),
    * with their cells' slots px_slot (as pe_slot); pc_eslot[3 pc_off[b] ..]: each patch-then-halo cell's six edge
-   * slots, two per int32 (0xffff: no edge); pslots2, pedges2: the most cell and edge slots a patch uses. */
+   * slots, two per int32 (0xffff: no edge); pslots2, pedges2: the most cell and edge slots a patch uses. The pair
+   * kernel steps each owned edge on its own thread of a 256-thread block, so these tables are refused (ISW_EBADARG)
+   * when a patch owns more than 256 edges (pown > 256). */
   int32_t pslots2, pedges2;
   const int32_t *ph2_off, *ph2_cell, *px_off, *px_edge, *px_slot, *pc_off, *pc_eslot;
 } isw_grid;
@@ -347,6 +351,14 @@ int32_t isw_timing_stop(char* out, int32_t cap);
 
 /* Library version and the gfx target it was built for. */
 const char* isw_version(void);
+
+/* Host-only argument checks (no counterpart in the reference, whose kernels check nothing): each returns 0 where the
+ * entry points above would accept the argument and ISW_EBADARG where they refuse it, and launches nothing.
+ * isw_grid_check: the mesh struct every grid-taking entry point checks first (tables present, patch tables
+ * consistent, at most ISW_PAIR_THREADS owned edges per patch for the two-sub-step tables). isw_air_column_options:
+ * the column options (par in PAR_ORDER, host) that isw_air_column does not carry for L layers. */
+int32_t isw_grid_check(const isw_grid* g);
+int32_t isw_air_column_options(int32_t L, const double* par);
 
 #ifdef __cplusplus
 }

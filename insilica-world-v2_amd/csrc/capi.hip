@@ -73,11 +73,12 @@ bool grid_ok(const isw_grid* g) {
   // the fast sub-steps' patches, when given: every table, a patch count that covers the cells, slots for a patch
   if (g->ph_off && !(g->ph_cell && g->pe_off && g->pe_edge && g->pe_slot && g->pcells > 0 &&
                      g->npatch == (g->N + g->pcells - 1) / g->pcells && g->pslots >= g->pcells &&
-                     g->pslots < (1 << 15)))
+                     g->pslots < (1 << 15) && g->pown > 0))
     return false;
+  // the pair kernel's patches: one thread of its ISW_PAIR_THREADS-thread block per owned edge
   return !g->ph2_off || (g->ph_off && g->ph2_cell && g->px_off && g->px_edge && g->px_slot && g->pc_off &&
                          g->pc_eslot && g->pslots2 >= g->pslots && g->pslots2 < (1 << 15) && g->pedges2 > 0 &&
-                         g->pedges2 < 0xffff);
+                         g->pedges2 < 0xffff && g->pown <= ISW_PAIR_THREADS);
 }
 
 bool layers_ok(const isw_grid* g, const isw_layers* s) {
@@ -105,6 +106,19 @@ bool sw_tables_ok(const isw_sw_tables* t) {
          t->hzf && t->cssa && (!t->cssa_i) == (!t->g_i);
 }
 bool prec_ok(int32_t p) { return p == ISW_FP64 || p == ISW_MIXED; }
+
+// The air column's options (par, the reference's PAR_ORDER) that this library's column does not carry: the cumulus
+// layer's floating edges (cu_dth, par[111], which only acts above three layers) and, for moist air, the deep plume
+// with the cumulus layer, and the cumulus layer's subsidence (cu_sub, par[114]) where it acts: with the mixed layer
+// (bl_mix, par[70]) and its condensation level (bl_lcl, par[100]) in three or more layers (air.cu:1838).
+bool column_options_ok(int32_t L, const double* par) {
+  if (par[111] > 0 && L > 3) return false;
+  if (par[22] > 0) {
+    if (par[47] > 0 && par[111] > 0) return false;
+    if (par[114] > 0 && par[100] > 0 && par[70] > 0 && L >= 3) return false;
+  }
+  return true;
+}
 
 }  // namespace
 
@@ -238,16 +252,9 @@ int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* a
       !Ts || !wx || !olr || !sw_abs || !stored || !rain || !evap || !conv || !tsurf || !(dt > 0) || !prec_ok(prec))
     return ISW_EBADARG;
   if (mode == 0 ? (!t || !pos) : !insol) return ISW_EBADARG;
-  // the options this library's column does not carry: the cumulus layer's floating edges, the diagnostics rows, the
-  // gas tracers moved with the crossing; for moist air the cumulus layer's subsidence (cu_sub, which acts with the
-  // mixed layer's condensation level, plume or no plume) and, with the deep plume, the cumulus layer (cu_dth). Land
-  // (moist) needs its bucket's capacity, soil, ponds and snow.
-  if ((par[111] > 0 && L > 3) || dgo || xfer) return ISW_EBADARG;
-  const bool deep = par[47] > 0;
-  if (par[22] > 0 && (!qv || !ice ||
-                      par[114] > 0 || (deep && par[111] > 0) ||
-                      (land && (!soilcap || !soil || !pond || !snow))))
-    return ISW_EBADARG;
+  if (!column_options_ok(L, par) || dgo || xfer) return ISW_EBADARG;
+  // moist air needs its vapor and ice; land (moist) its bucket's capacity, soil, ponds and snow
+  if (par[22] > 0 && (!qv || !ice || (land && (!soilcap || !soil || !pond || !snow)))) return ISW_EBADARG;
   return isw_col::air_column(N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, wx, olr, sw_abs,
                              stored, rain, evap, conv, tsurf, rainp, cap, qocean, qsw, land, soilcap, soil, pond, snow,
                              elev, flash, radf, swf, swalb, ch, lc, firn, cloudw, cloudo, prec == ISW_MIXED,
@@ -283,6 +290,13 @@ int32_t isw_advance_time(double* t, double dt, void* stream) {
 }
 
 const char* isw_version(void) { return "insilica_hip 0.1 gfx950"; }
+
+int32_t isw_grid_check(const isw_grid* g) { return grid_ok(g) ? 0 : ISW_EBADARG; }
+
+int32_t isw_air_column_options(int32_t L, const double* par) {
+  if (L < 2 || L > ISW_LMAX || !par) return ISW_EBADARG;
+  return column_options_ok(L, par) ? 0 : ISW_EBADARG;
+}
 
 }  // extern "C"
 

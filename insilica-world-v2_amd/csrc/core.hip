@@ -527,7 +527,7 @@ __global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, LP p, in
 // alternate between two buffers (a neighbour block may still read this sub-step's inputs), so in/out never alias.
 // Patch tables (isw_grid): patch b holds cells [b pcells, (b + 1) pcells), its halo cells ph_cell[ph_off[b] ..],
 // its edges pe_edge[pe_off[b] ..] with their cells' LDS slots pe_slot (first | second << 16; halo slots from pcells).
-constexpr int FBS = 256;
+constexpr int FBS = ISW_PAIR_THREADS;
 template <int L>
 __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
                                                   const double* __restrict__ hedge, const double* __restrict__ uin,

@@ -8,6 +8,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / '_lib' / 'libinsilica_hip.so'
 ISW_FP64, ISW_MIXED, ISW_EBADARG, LMAX, KSOIL = 0, 1, 9001, 8, 16
+ISW_PAIR_THREADS = 256            # the most edges a patch may own for the two-sub-step kernel (one thread each)
 
 _i32p, _i8p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int8)
 _f64p, _f32p = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
@@ -20,7 +21,7 @@ class IswGrid(ctypes.Structure):
                 + [(k, _i8p) for k in ('cellSign', 'vEdgeSign', 'vSlot')]
                 + [(k, _f64p) for k in ('le', 'de', 'areaCell', 'areaV', 'fV', 'kite', 'w', 'd2wt', 'rA', 'rde', 'rAV')]
                 + [(k, _f32p) for k in ('wf', 'd2wtf')]
-                + [(k, ctypes.c_int32) for k in ('npatch', 'pcells', 'pslots', 'pad_')]
+                + [(k, ctypes.c_int32) for k in ('npatch', 'pcells', 'pslots', 'pown')]
                 + [(k, _i32p) for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot')]
                 + [(k, ctypes.c_int32) for k in ('pslots2', 'pedges2')]
                 + [(k, _i32p) for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot')])
@@ -114,6 +115,8 @@ def lib():
             'isw_advance_time': [vp, d, vp],
             # N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, then 29 arrays, prec, stream
             'isw_air_column': [i32, i32, vp, vp, i32] + [vp] * 9 + [d] + [vp] * 29 + [i32, vp],
+            'isw_grid_check': [vp],
+            'isw_air_column_options': [i32, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -129,7 +132,8 @@ EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_
            'isw_mass_update', 'isw_transport_step', 'isw_swe_step', 'isw_couple_stress', 'isw_land_column',
            'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_lw_setup', 'isw_lw_column', 'isw_sw_column',
            'isw_hs_column', 'isw_surface_layer', 'isw_cell_speed', 'isw_cell_speed_perot', 'isw_gas_src',
-           'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version')
+           'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version',
+           'isw_grid_check', 'isw_air_column_options')
 
 
 def check(rc, what):

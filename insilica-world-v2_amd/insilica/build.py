@@ -29,8 +29,16 @@ def _run(cmd):
     return r
 
 
-def build_hip(verbose=False):
-    """Compile csrc/*.hip into insilica/_lib/libinsilica_hip.so."""
+def _stale(src, out):
+    """Whether out is older than its source or any header it may include (csrc/*.hpp, include/*.h)."""
+    if not out.exists():
+        return True
+    deps = [CSRC / src, *CSRC.glob('*.hpp'), *INCLUDE.glob('*.h')]
+    return max(d.stat().st_mtime for d in deps) > out.stat().st_mtime
+
+
+def build_hip(verbose=False, force=True):
+    """Compile csrc/*.hip into insilica/_lib/libinsilica_hip.so (force=False recompiles only stale units)."""
     OBJDIR.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
     objs = []
@@ -49,6 +57,8 @@ def build_hip(verbose=False):
     def one(unit):
         src, obj, extra = unit
         out = OBJDIR / obj
+        if not force and not _stale(src, out):
+            return str(out)
         _run([HIPCC, '-c', *HIP_FLAGS, *extra, str(CSRC / src), '-o', str(out)])
         if verbose:
             print('compiled', src, '->', obj, flush=True)
@@ -71,10 +81,11 @@ def build_mesh():
     return LIBDIR / 'libinsilica_mesh.so'
 
 
-def build_all(verbose=False):
+def build_all(verbose=False, force=True):
     build_mesh()
-    return build_hip(verbose)
+    return build_hip(verbose, force)
 
 
 if __name__ == '__main__':
-    print(build_all(verbose=True))
+    import sys
+    print(build_all(verbose=True, force='--changed' not in sys.argv))

@@ -69,7 +69,7 @@ def fast_patches(grid, pcells=64):
                 pe_edge=order, pe_slot=sI | (sJ << 16))
 
 
-def fast_patches2(grid, pt, max_owned=256):
+def fast_patches2(grid, pt, max_owned=_hip.ISW_PAIR_THREADS):
     """Two air fast sub-steps per launch on the same patches (isw_grid's ph2_*, px_*, pc_* tables): the cells of the
     first sub-step are the patch, its halo (pt's) and the halo's own halo, the cells of the edges of patch and halo;
     the winds of the first sub-step are needed on every edge of patch and halo (the patch's own edges first, in pt's
@@ -165,6 +165,7 @@ class DeviceGrid:
         pt = fast_patches(grid, PATCH_CELLS)
         if pt is not None:
             s.npatch, s.pcells, s.pslots = pt['npatch'], pt['pcells'], pt['pslots']
+            s.pown = int(np.max(np.diff(pt['pe_off'])))
             for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):
                 setattr(self, k, put(pt[k], np.int32))
                 setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._i32p))

@@ -84,17 +84,14 @@ def test_bad_arguments_rejected_on_host():
     par[111] = 0.0
     par[22] = 1.0                                            # moist air: no vapor or ice given; the moist options
     assert air_column() == E
-    par[47] = 1.0                                            # plume_deep with the cumulus layer or its
-    for k in (111, 114):                                     # subsidence
-        par[k] = 1.0
-        assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf,
-                                  600.0, *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
-        par[k] = 0.0
-    par[47] = 0.0
-    par[114] = 1.0                                           # cu_sub acts without the deep plume too
+    par[47] = par[111] = 1.0                                 # plume_deep with the cumulus layer
+    assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf,
+                              600.0, *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
+    par[47] = par[111] = 0.0
+    par[114], par[100], par[70] = 1.0, 1.0, 1.0              # cu_sub with the mixed layer's condensation level
     assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
                               *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
-    par[114] = 0.0
+    par[114], par[100], par[70] = 0.0, 0.0, 0.0
     # land without its bucket's capacity, soil, ponds or snow
     assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
                               *([buf] * 8), None, None, None, None, None, buf, buf, buf, None, buf, None, None,
@@ -111,3 +108,65 @@ def test_request_draws_are_seeded():
     assert [(r.arrival, r.level, r.days) for r in a] == [(r.arrival, r.level, r.days) for r in b]
     assert all(5 <= r.level <= 6 and 1 <= r.days <= 30 for r in a)
     assert all(x.arrival < y.arrival for x, y in zip(a, a[1:]))
+
+
+def test_column_options_predicate():
+    """The air column's refusal of the options it does not carry, on the host only (no launch): cu_sub is refused
+    exactly where the reference applies it (cu_sub, bl_lcl, bl_mix all on, three or more layers, moist air;
+    kernels/air.cu:1838), cu_dth above three layers, and cu_dth with the deep plume in moist air."""
+    lib = _hip.lib()
+    E = _hip.ISW_EBADARG
+    par = (ctypes.c_double * 140)()
+    assert lib.isw_air_column_options(3, par) == 0                     # the dry column: nothing refused
+    par[22] = 1.0                                                     # moist
+    assert lib.isw_air_column_options(3, par) == 0
+    for on in range(8):                                               # cu_sub's gates: refused only with all three
+        par[114], par[100], par[70] = (float(on & 1), float(on >> 1 & 1), float(on >> 2 & 1))
+        assert lib.isw_air_column_options(3, par) == (E if on == 7 else 0), on
+        assert lib.isw_air_column_options(2, par) == 0, on            # no mixed layer's cumulus slice in two layers
+    par[114] = par[100] = par[70] = 0.0
+    par[22] = 0.0
+    par[114], par[100], par[70] = 1.0, 1.0, 1.0                       # dry air: cu_sub never acts
+    assert lib.isw_air_column_options(3, par) == 0
+    par[114] = par[100] = par[70] = 0.0
+    par[111] = 1.0                                                    # cu_dth: refused above three layers only
+    assert lib.isw_air_column_options(3, par) == 0 and lib.isw_air_column_options(4, par) == E
+    par[22], par[47] = 1.0, 1.0                                       # and with the deep plume in moist air
+    assert lib.isw_air_column_options(3, par) == E
+    par[111] = 0.0
+    assert lib.isw_air_column_options(3, par) == 0
+    assert lib.isw_air_column_options(1, par) == E and lib.isw_air_column_options(3, None) == E
+
+
+def test_grid_check_pair_tables():
+    """The two-sub-step patch tables are refused when a patch owns more edges than the pair kernel has threads, and
+    accepted at the limit (host check only, no launch)."""
+    lib = _hip.lib()
+    E = _hip.ISW_EBADARG
+    buf = (ctypes.c_int32 * 8)()
+    p32 = ctypes.cast(buf, _hip._i32p)
+    f64 = ctypes.cast((ctypes.c_double * 8)(), _hip._f64p)
+    g = _hip.IswGrid()
+    g.N, g.E, g.NV = 642, 1920, 1280
+    for k in ('eI', 'eJ', 'cellEdge', 'cellEdgeAsc'):
+        setattr(g, k, p32)
+    g.cellSign = ctypes.cast((ctypes.c_int8 * 8)(), _hip._i8p)
+    g.le = g.de = g.areaCell = f64
+    assert lib.isw_grid_check(ctypes.byref(g)) == 0                    # no patches
+    g.npatch, g.pcells, g.pslots, g.pown = 11, 64, 120, 200
+    for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):
+        setattr(g, k, p32)
+    assert lib.isw_grid_check(ctypes.byref(g)) == 0
+    g.pown = 0
+    assert lib.isw_grid_check(ctypes.byref(g)) == E                    # an owned-edge count is required
+    g.pown = 300
+    assert lib.isw_grid_check(ctypes.byref(g)) == 0                    # one sub-step per launch loops over edges
+    g.pslots2, g.pedges2 = 300, 400
+    for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot'):
+        setattr(g, k, p32)
+    assert lib.isw_grid_check(ctypes.byref(g)) == E                    # 300 owned edges > 256 threads
+    g.pown = _hip.ISW_PAIR_THREADS
+    assert lib.isw_grid_check(ctypes.byref(g)) == 0
+    g.pown = _hip.ISW_PAIR_THREADS + 1
+    assert lib.isw_grid_check(ctypes.byref(g)) == E
+    assert lib.isw_grid_check(None) == E

@@ -14,8 +14,9 @@ def test_batch_equals_solo(precision):
     from insilica.world import World
     print(f'case: batch of 3 coupled worlds n=4, {precision}', flush=True)
     worlds = [World(4, 'coupled_stack', precision=precision, params=dict(p)) for p in SPECS]
-    nf = [w.nfast for w in worlds]
-    assert len(set(nf)) > 1 or True
+    nf, nfo = [w.nfast for w in worlds], [w.sea.nfast for w in worlds]
+    print(f'air nfast {nf}, ocean nfast {nfo}', flush=True)
+    assert len(set(nf)) > 1 and len(set(nfo)) > 1, (nf, nfo)       # the batch's per-world sub-step counts differ
     bw = BatchWorld(worlds)
     solo = [World(4, 'coupled_stack', precision=precision, params=dict(p)) for p in SPECS]
     ticks = 2 * bw.ocean_every + 1
