@@ -172,6 +172,19 @@ int32_t isw_ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* c
  * repeated to one block. scratch: at least ceil(n / 2048) + ceil(n / 2048^2) + 2 doubles. */
 int32_t isw_dsum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream);
 
+/* isw_dsum without a workspace argument. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It launches
+ * isw_dsum's kernels in the same fixed-order tree, so out[0] is bit-identical to isw_dsum's, with the workspace in a
+ * file-scope device array of ISW_DSUM_NOALLOC_DOUBLES doubles (131 KB, held on each device the library runs on
+ * until the process ends); hence n <= ISW_DSUM_NOALLOC_MAX_N (else ISW_EBADARG). It calls no allocator. The first
+ * call on a device looks the array's address up there (hipGetSymbolAddress, a host query). Every call on a device
+ * shares that device's array, which keeps the last call's partial sums: calls that overlap on one device (two
+ * streams, two host threads) race, and the caller must order them. Nothing in the array needs to start at zero:
+ * each pass writes its partials before the next reads them. */
+#define ISW_DSUM_NOALLOC_MAX_N 33554432 /* 2^25 values */
+#define ISW_DSUM_NOALLOC_DOUBLES 16400
+int32_t isw_dsum_noalloc(const double* x, const double* y, int64_t n, double* out, void* stream);
+
 /* A compensated, deterministic sum of x (or x*y) into out[0]: per-block Neumaier partial sums with their error terms
  * in a fixed order, then one fixed-order compensated pass over the partials; out[0] the sum, out[1] its remaining
  * correction (out holds 2 doubles). scratch: 2 * ceil(n / 2048) doubles. The budgets' sum (no counterpart in the

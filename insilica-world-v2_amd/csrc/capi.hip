@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -388,13 +389,16 @@ __global__ __launch_bounds__(SUM_THREADS) void k_csum_final(long long nb, const 
 
 }  // namespace
 
-extern "C" int32_t isw_dsum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream) {
-  if (n < 1 || !x || !scratch || !out) return ISW_EBADARG;
-  hipStream_t st = S(stream);
+namespace {
+
+// The passes of the fixed-order tree (block_sum repeated to one block, gpu.py:111-126): each pass's block sums go to
+// one of two halves of ws in turn, the last pass's single sum to out. ws holds at least ceil(n / 2048) + 1 +
+// ceil(n / 2048^2) + 1 doubles; every pass writes its values before the next reads them.
+int32_t dsum_passes(const double* x, const double* y, long long n, double* ws, double* out, hipStream_t st) {
   const long long per = (long long)SUM_THREADS * SUM_CHUNK;
   long long m = n;
   const double *xx = x, *yy = y;
-  double* buf[2] = {scratch, scratch + (n + per - 1) / per + 1};
+  double* buf[2] = {ws, ws + (n + per - 1) / per + 1};
   int which = 0;
   for (;;) {
     const long long blocks = (m + per - 1) / per;
@@ -407,6 +411,48 @@ extern "C" int32_t isw_dsum(const double* x, const double* y, int64_t n, double*
     which ^= 1;
   }
   return (int32_t)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int32_t isw_dsum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream) {
+  if (n < 1 || !x || !scratch || !out) return ISW_EBADARG;
+  return dsum_passes(x, y, n, scratch, out, S(stream));
+}
+
+// isw_dsum's workspace as a file-scope device array: each device the library runs on holds its own copy until the
+// process ends. Host code never passes this name to a kernel (on the host it is a stand-in, not a device address):
+// it uses the address hipGetSymbolAddress gives for the current device, looked up on the first call there.
+static __device__ __align__(256) double scratch[ISW_DSUM_NOALLOC_DOUBLES];
+static_assert((long long)ISW_DSUM_NOALLOC_MAX_N / (SUM_THREADS * SUM_CHUNK) + 1 +
+                      (long long)ISW_DSUM_NOALLOC_MAX_N / ((long long)SUM_THREADS * SUM_CHUNK * SUM_THREADS * SUM_CHUNK) +
+                      1 <=
+                  ISW_DSUM_NOALLOC_DOUBLES,
+              "the static workspace must hold every pass's partials at the largest n");
+
+namespace {
+std::mutex scratch_mu;
+std::vector<double*> scratch_dev;  // per device: the array's device address, nullptr until looked up
+}  // namespace
+
+extern "C" int32_t isw_dsum_noalloc(const double* x, const double* y, int64_t n, double* out, void* stream) {
+  if (n < 1 || n > ISW_DSUM_NOALLOC_MAX_N || !x || !out) return ISW_EBADARG;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int32_t)e;
+  double* ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if ((int)scratch_dev.size() <= dev) scratch_dev.resize(dev + 1, nullptr);
+    if (!scratch_dev[dev]) {
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) return e != hipSuccess ? (int32_t)e : (int32_t)hipErrorInvalidSymbol;
+      scratch_dev[dev] = static_cast<double*>(p);
+    }
+    ws = scratch_dev[dev];
+  }
+  return dsum_passes(x, y, n, ws, out, S(stream));
 }
 
 extern "C" int32_t isw_csum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream) {

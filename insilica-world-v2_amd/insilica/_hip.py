@@ -9,6 +9,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / '_lib' / 'libinsilica_hip.so'
 ISW_FP64, ISW_MIXED, ISW_EBADARG, LMAX, KSOIL = 0, 1, 9001, 8, 16
 ISW_PAIR_THREADS = 256            # the most edges a patch may own for the two-sub-step kernel (one thread each)
+ISW_DSUM_NOALLOC_MAX_N = 1 << 25  # the most values isw_dsum_noalloc sums (its static workspace's limit)
 
 _i32p, _i8p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int8)
 _f64p, _f32p = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
@@ -103,6 +104,7 @@ def lib():
             'isw_ocn_cap': [vp, vp, d, vp, vp, d, vp],
             'isw_dsum': [vp, vp, i64, vp, vp, vp],
             'isw_csum': [vp, vp, i64, vp, vp, vp],
+            'isw_dsum_noalloc': [vp, vp, i64, vp, vp],
             'isw_fill': [vp, i64, d, vp],
             'isw_lw_setup': [i32, i32, vp, vp, vp, d, d, d, vp, vp, vp, vp, vp, d, i32, vp, i32, vp],
             'isw_lw_column': [i32, i32, vp, vp, vp, vp, vp, i32, vp],
@@ -130,7 +132,7 @@ def lib():
 
 EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_exchange', 'isw_vmix_heat',
            'isw_mass_update', 'isw_transport_step', 'isw_swe_step', 'isw_couple_stress', 'isw_land_column',
-           'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_fill', 'isw_lw_setup', 'isw_lw_column', 'isw_sw_column',
+           'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_dsum_noalloc', 'isw_fill', 'isw_lw_setup', 'isw_lw_column', 'isw_sw_column',
            'isw_hs_column', 'isw_surface_layer', 'isw_cell_speed', 'isw_cell_speed_perot', 'isw_gas_src',
            'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version',
            'isw_grid_check', 'isw_air_column_options')

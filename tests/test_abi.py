@@ -32,6 +32,12 @@ def test_bad_arguments_rejected_on_host():
     assert lib.isw_swe_step(None, None, None, None, None, 1.0, None) == E
     assert lib.isw_dsum(None, None, 0, None, None, None) == E
     assert lib.isw_csum(None, None, 10, None, None, None) == E
+    # the static-workspace sum refuses sizes beyond its array (and empty or missing operands) before any HIP call
+    one = ctypes.c_void_p(1)
+    assert lib.isw_dsum_noalloc(one, None, _hip.ISW_DSUM_NOALLOC_MAX_N + 1, one, None) == E
+    assert lib.isw_dsum_noalloc(one, None, 0, one, None) == E
+    assert lib.isw_dsum_noalloc(None, None, 10, one, None) == E
+    assert lib.isw_dsum_noalloc(one, None, 10, None, None) == E
     assert lib.isw_fill(None, 5, 0.0, None) == E
     assert lib.isw_land_column(None, 3, None, None, None, None, None, 1.0, 1.0, 1.0, None) == E
     # a grid struct with sizes but a layer count beyond the maximum, and a transport with rows not a multiple of L
