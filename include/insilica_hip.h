@@ -35,7 +35,8 @@ extern "C" {
 #define ISW_EBADARG 9001
 #define ISW_LMAX 8
 #define ISW_KSOIL 16
-#define ISW_PAIR_THREADS 256 /* threads per block of the two-sub-step patch kernel: the most edges a patch may own */
+#define ISW_PAIR_THREADS 256 /* threads per block of the two-sub-step patch kernel: the most edges a patch may own,
+                              * and the most cells of patch and halo */
 
 /* The mesh on the device (reference: src/insilica/gpu.py:55-80 DeviceGrid; tables from src/insilica/grid/trisk.py).
  * rA, rde, rAV: 1/areaCell, 1/de, 1/areaV; wf, d2wtf: FP32 copies of w and d2wt (read by the mixed form only).
@@ -61,7 +62,7 @@ typedef struct isw_grid {
    * with their cells' slots px_slot (as pe_slot); pc_eslot[3 pc_off[b] ..]: each patch-then-halo cell's six edge
    * slots, two per int32 (0xffff: no edge); pslots2, pedges2: the most cell and edge slots a patch uses. The pair
    * kernel steps each owned edge on its own thread of a 256-thread block, so these tables are refused (ISW_EBADARG)
-   * when a patch owns more than 256 edges (pown > 256). */
+   * when a patch owns more than 256 edges (pown > 256) or its patch and halo hold more than 256 cells (pslots > 256). */
   int32_t pslots2, pedges2;
   const int32_t *ph2_off, *ph2_cell, *px_off, *px_edge, *px_slot, *pc_off, *pc_eslot;
 } isw_grid;
@@ -190,6 +191,60 @@ int32_t isw_dsum_noalloc(const double* x, const double* y, int64_t n, double* ou
  * correction (out holds 2 doubles). scratch: 2 * ceil(n / 2048) doubles. The budgets' sum (no counterpart in the
  * reference, whose totals use dsum). */
 int32_t isw_csum(const double* x, const double* y, int64_t n, double* scratch, double* out, void* stream);
+
+/* ---- the ocean's column processes (reference: src/insilica/kernels/ocean.cu; ocean.py:226-295; FP64 in both forms) ----
+ *
+ * The ocean's settings these read (reference: ocean.py:29-33 OCEAN, an Ocean's cfg; ALPHA_REF ocean.py:27; the
+ * equation of state b = grav (alpha th + alpha2 th^2 / 2 - beta S), th = T - 273.15 K, ocean.cu:43-51). Host struct. */
+typedef struct isw_ocean {
+  double grav, alpha, alpha2, beta, S_ref, alpha_ref, rho_w, cw, m_kt, n_kt, hmin, hmax, dTmin, sw_R, sw_z1, sw_z2,
+      kappa_d, dwf, ri_b, reach, h_ent, det_class, fw_local;
+} isw_ocean;
+
+/* ocn_overturn (reference: ocean.cu:231-237, adjust 55-72): convective adjustment of every sea column's T and S (L, N)
+ * on thicknesses h (L, N): a layer denser than the one below mixes with it completely, repeated until the column is
+ * stable; land (N, nullable) cells > 0.5 are skipped. 2 <= L <= ISW_LMAX. */
+int32_t isw_ocn_overturn(int32_t N, int32_t L, const isw_ocean* o, const double* h, double* T, double* S,
+                         const double* land, void* stream);
+
+/* The mixing interval's forcing (reference: Ocean.step, ocean.py:257-266): tau_acc (E; the sum of tau^2 dt) becomes
+ * the edges' rms stress sqrt(tau_acc (1 / dt)) in place and taum (N) its cell value (air.cu:1978-1989's cell speed);
+ * q_acc, sun_acc, fw_acc (N) are divided by dt in place. */
+int32_t isw_ocn_forcing(const isw_grid* g, double dt, double* tau_acc, double* taum, double* q_acc, double* sun_acc,
+                        double* fw_acc, void* stream);
+
+/* ocn_column (reference: ocean.cu:117-213): the mixed layer over one mixing interval dt: adjustment, Kraus-Turner
+ * entrainment or shoaling on the interval's rms stress taum and mean fluxes qns (non-solar heat), qfw (fresh water),
+ * qsun (sunlight) (N); shear (N, nullable) the base's velocity jump for ri_b; the interior interfaces' deep-water
+ * formation (dwf) and mixing (kappa_d); reach, h_ent; Tc, Sc (L, nullable) the layers' classes for det_class. Writes
+ * T, S (adjusted) and wx (L - 1, N), the water to move up across each interface (for isw_layers_exchange). */
+int32_t isw_ocn_column(int32_t N, int32_t L, const isw_ocean* o, const double* h, double* T, double* S,
+                       const double* taum, const double* qns, const double* qfw, const double* qsun, double dt,
+                       double* wx, const double* land, const double* shear, const double* Tc, const double* Sc,
+                       void* stream);
+
+/* ocn_xch_salt (reference: ocean.cu:352-381): S (L, N) moved with the water wx (L - 1, N) moves, before the exchange
+ * moves the rest; upwind; seq != 0: in passing order. */
+int32_t isw_ocn_xch_salt(int32_t N, int32_t L, const double* wx, const double* h, double* S, int32_t seq, void* stream);
+
+/* ocn_pen (reference: ocean.cu:387-407; Ocean.accumulate, ocean.py:214-224): the sunlight qsw (N, W/m^2) passing the
+ * mixed layer's base heats the layers below over dt; qns_acc += (qocean - qsw) dt, sun_acc += qsw dt; up (N,
+ * nullable): where positive the mixed layer keeps all the sunlight. */
+int32_t isw_ocn_pen(int32_t N, int32_t L, const isw_ocean* o, const double* h, double* T, const double* qocean,
+                    const double* qsw, double dt, double* qns_acc, double* sun_acc, const double* land,
+                    const double* up, void* stream);
+
+/* ocn_fresh (reference: ocean.cu:245-261): rain - evap (kg/m^2/s) and the sea ice's growth (ice - ice_prev, kg/m^2)
+ * freshen or salt the mixed layer's S[0] over dt (at S_ref, or with fw_local > 0 with its volume: h_w, M (N) of the
+ * mixed layer rewritten from area (N)); mouth (N, nullable) emptied into it; ice_prev = ice; fw_acc += the water. */
+int32_t isw_ocn_fresh(int32_t N, const isw_ocean* o, const double* h, double* S, const double* evap,
+                      const double* rain, const double* ice, double* ice_prev, double dt, double* fw_acc,
+                      const double* land, double* mouth, double* hw, double* M, const double* area, void* stream);
+
+/* ocn_diffuse (reference: ocean.cu:268-292): T, S (L, N) mixed along each layer over dt into Tn, Sn (L, N); wall (E,
+ * nullable) closes edges. */
+int32_t isw_ocn_diffuse(const isw_grid* g, int32_t L, const double* h, const double* T, const double* S, double* Tn,
+                        double* Sn, const double* wall, double kappa_h, double dt, void* stream);
 
 /* Fill n doubles with v (a kernel, not hipMemsetAsync, so that captured graphs replay it safely). */
 int32_t isw_fill(double* x, int64_t n, double v, void* stream);
@@ -368,7 +423,7 @@ const char* isw_version(void);
 /* Host-only argument checks (no counterpart in the reference, whose kernels check nothing): each returns 0 where the
  * entry points above would accept the argument and ISW_EBADARG where they refuse it, and launches nothing.
  * isw_grid_check: the mesh struct every grid-taking entry point checks first (tables present, patch tables
- * consistent, at most ISW_PAIR_THREADS owned edges per patch for the two-sub-step tables). isw_air_column_options:
+ * consistent, at most ISW_PAIR_THREADS owned edges and patch-and-halo cells per patch for the two-sub-step tables). isw_air_column_options:
  * the column options (par in PAR_ORDER, host) that isw_air_column does not carry for L layers. */
 int32_t isw_grid_check(const isw_grid* g);
 int32_t isw_air_column_options(int32_t L, const double* par);

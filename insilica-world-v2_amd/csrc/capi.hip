@@ -52,6 +52,20 @@ int air_column(int, int, const double*, const isw_air*, int, const double*, cons
                double*, const double*, const double*, double*, double*, double*, int, hipStream_t);
 }  // namespace isw_col
 
+namespace isw_ocn {
+int overturn(int, int, const isw_ocean*, const double*, double*, double*, const double*, hipStream_t);
+int column(int, int, const isw_ocean*, const double*, double*, double*, const double*, const double*, const double*,
+           const double*, double, double*, const double*, const double*, const double*, const double*, hipStream_t);
+int xch_salt(int, int, const double*, const double*, double*, int, hipStream_t);
+int pen(int, int, const isw_ocean*, const double*, double*, const double*, const double*, double, double*, double*,
+        const double*, const double*, hipStream_t);
+int fresh(int, const isw_ocean*, const double*, double*, const double*, const double*, const double*, double*, double,
+          double*, const double*, double*, double*, double*, const double*, hipStream_t);
+int diffuse(const isw_grid*, int, const double*, const double*, const double*, double*, double*, const double*, double,
+            double, hipStream_t);
+int forcing(const isw_grid*, double, double*, double*, double*, double*, double*, hipStream_t);
+}  // namespace isw_ocn
+
 #define ISW_DECLARE_RAD(NS)                                                                                          \
   namespace NS {                                                                                                     \
   int lw_setup(int, int, const double*, const isw_lw_tables*, const isw_air*, double, double, double, const double*, \
@@ -76,10 +90,11 @@ bool grid_ok(const isw_grid* g) {
                      g->npatch == (g->N + g->pcells - 1) / g->pcells && g->pslots >= g->pcells &&
                      g->pslots < (1 << 15) && g->pown > 0))
     return false;
-  // the pair kernel's patches: one thread of its ISW_PAIR_THREADS-thread block per owned edge
+  // the pair kernel's patches: one thread of its ISW_PAIR_THREADS-thread block per owned edge, and per cell of patch
+  // and halo (at most pslots of them)
   return !g->ph2_off || (g->ph_off && g->ph2_cell && g->px_off && g->px_edge && g->px_slot && g->pc_off &&
                          g->pc_eslot && g->pslots2 >= g->pslots && g->pslots2 < (1 << 15) && g->pedges2 > 0 &&
-                         g->pedges2 < 0xffff && g->pown <= ISW_PAIR_THREADS);
+                         g->pedges2 < 0xffff && g->pown <= ISW_PAIR_THREADS && g->pslots <= ISW_PAIR_THREADS);
 }
 
 bool layers_ok(const isw_grid* g, const isw_layers* s) {
@@ -200,6 +215,67 @@ int32_t isw_ocn_cap(const isw_grid* g, const double* h, double rho_cw, double* c
                     void* stream) {
   if (!g || g->N < 1 || !h || !cap) return ISW_EBADARG;
   return isw_f64::ocn_cap(g, h, rho_cw, cap, land, c_land, S(stream));
+}
+
+// ---- the ocean's column processes (FP64 in both forms) ----
+bool ocean_ok(const isw_ocean* o) {
+  return o && o->grav > 0 && o->rho_w > 0 && o->cw > 0 && o->hmin > 0 && o->hmax >= o->hmin && o->sw_z1 > 0 &&
+         o->sw_z2 > 0;
+}
+
+int32_t isw_ocn_overturn(int32_t N, int32_t L, const isw_ocean* o, const double* h, double* T, double* S,
+                         const double* land, void* stream) {
+  if (N < 1 || L < 2 || L > ISW_LMAX || !ocean_ok(o) || !h || !T || !S) return ISW_EBADARG;
+  return isw_ocn::overturn(N, L, o, h, T, S, land, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t isw_ocn_forcing(const isw_grid* g, double dt, double* tau_acc, double* taum, double* q_acc, double* sun_acc,
+                        double* fw_acc, void* stream) {
+  if (!g || g->N < 1 || g->E < 1 || !g->cellEdge || !g->le || !g->de || !g->areaCell || !(dt > 0) || !tau_acc ||
+      !taum || !q_acc || !sun_acc || !fw_acc)
+    return ISW_EBADARG;
+  return isw_ocn::forcing(g, dt, tau_acc, taum, q_acc, sun_acc, fw_acc, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t isw_ocn_column(int32_t N, int32_t L, const isw_ocean* o, const double* h, double* T, double* S,
+                       const double* taum, const double* qns, const double* qfw, const double* qsun, double dt,
+                       double* wx, const double* land, const double* shear, const double* Tc, const double* Sc,
+                       void* stream) {
+  if (N < 1 || L < 2 || L > ISW_LMAX || !ocean_ok(o) || !h || !T || !S || !taum || !qns || !qfw || !qsun || !wx ||
+      !(dt > 0) || (o->det_class > 0 && !Tc))
+    return ISW_EBADARG;
+  return isw_ocn::column(N, L, o, h, T, S, taum, qns, qfw, qsun, dt, wx, land, shear, Tc, Sc, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t isw_ocn_xch_salt(int32_t N, int32_t L, const double* wx, const double* h, double* S, int32_t seq,
+                         void* stream) {
+  if (N < 1 || L < 2 || L > ISW_LMAX || !wx || !h || !S) return ISW_EBADARG;
+  return isw_ocn::xch_salt(N, L, wx, h, S, seq, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t isw_ocn_pen(int32_t N, int32_t L, const isw_ocean* o, const double* h, double* T, const double* qocean,
+                    const double* qsw, double dt, double* qns_acc, double* sun_acc, const double* land,
+                    const double* up, void* stream) {
+  if (N < 1 || L < 2 || L > ISW_LMAX || !ocean_ok(o) || !h || !T || !qocean || !qsw || !qns_acc || !sun_acc)
+    return ISW_EBADARG;
+  return isw_ocn::pen(N, L, o, h, T, qocean, qsw, dt, qns_acc, sun_acc, land, up, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t isw_ocn_fresh(int32_t N, const isw_ocean* o, const double* h, double* S, const double* evap,
+                      const double* rain, const double* ice, double* ice_prev, double dt, double* fw_acc,
+                      const double* land, double* mouth, double* hw, double* M, const double* area, void* stream) {
+  if (N < 1 || !ocean_ok(o) || !h || !S || !evap || !rain || !ice || !ice_prev || !fw_acc ||
+      (o->fw_local > 0 && (!hw || !M || !area)))
+    return ISW_EBADARG;
+  return isw_ocn::fresh(N, o, h, S, evap, rain, ice, ice_prev, dt, fw_acc, land, mouth, hw, M, area, reinterpret_cast<hipStream_t>(stream));
+}
+
+int32_t isw_ocn_diffuse(const isw_grid* g, int32_t L, const double* h, const double* T, const double* S, double* Tn,
+                        double* Sn, const double* wall, double kappa_h, double dt, void* stream) {
+  if (!g || g->N < 1 || !g->cellEdge || !g->eI || !g->eJ || !g->le || !g->de || !g->areaCell || L < 1 ||
+      L > ISW_LMAX || !h || !T || !S || !Tn || !Sn)
+    return ISW_EBADARG;
+  return isw_ocn::diffuse(g, L, h, T, S, Tn, Sn, wall, kappa_h, dt, reinterpret_cast<hipStream_t>(stream));
 }
 
 int32_t isw_fill(double* x, int64_t n, double v, void* stream) {

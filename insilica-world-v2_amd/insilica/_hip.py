@@ -33,6 +33,12 @@ LAYER_SCRATCH = ('B', 'q', 'Tp', 'F', 'scale', 'G', 'He', 'ubs', 'ub0', 'ub', 'a
                  'uf', 'accL', 'pk', 'ufb', 'ubb', 'ubc')
 
 
+class IswOcean(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ('grav', 'alpha', 'alpha2', 'beta', 'S_ref', 'alpha_ref', 'rho_w', 'cw',
+                                               'm_kt', 'n_kt', 'hmin', 'hmax', 'dTmin', 'sw_R', 'sw_z1', 'sw_z2',
+                                               'kappa_d', 'dwf', 'ri_b', 'reach', 'h_ent', 'det_class', 'fw_local')]
+
+
 class IswBatch(ctypes.Structure):
     _fields_ = ([('nworlds', ctypes.c_int32), ('nmax', ctypes.c_int32 * 3)]
                 + [(k, _i32p) for k in ('wc', 'we', 'nsub')]
@@ -117,6 +123,13 @@ def lib():
             'isw_advance_time': [vp, d, vp],
             # N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, then 29 arrays, prec, stream
             'isw_air_column': [i32, i32, vp, vp, i32] + [vp] * 9 + [d] + [vp] * 29 + [i32, vp],
+            'isw_ocn_overturn': [i32, i32, vp, vp, vp, vp, vp, vp],
+            'isw_ocn_forcing': [vp, d, vp, vp, vp, vp, vp, vp],
+            'isw_ocn_column': [i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, vp, vp, vp],
+            'isw_ocn_xch_salt': [i32, i32, vp, vp, vp, i32, vp],
+            'isw_ocn_pen': [i32, i32, vp, vp, vp, vp, vp, d, vp, vp, vp, vp, vp],
+            'isw_ocn_fresh': [i32, vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, vp, vp, vp, vp],
+            'isw_ocn_diffuse': [vp, i32, vp, vp, vp, vp, vp, vp, d, d, vp],
             'isw_grid_check': [vp],
             'isw_air_column_options': [i32, vp],
         }
@@ -135,7 +148,8 @@ EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_
            'isw_ocn_cap', 'isw_dsum', 'isw_csum', 'isw_dsum_noalloc', 'isw_fill', 'isw_lw_setup', 'isw_lw_column', 'isw_sw_column',
            'isw_hs_column', 'isw_surface_layer', 'isw_cell_speed', 'isw_cell_speed_perot', 'isw_gas_src',
            'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version',
-           'isw_grid_check', 'isw_air_column_options')
+           'isw_grid_check', 'isw_air_column_options', 'isw_ocn_overturn', 'isw_ocn_forcing', 'isw_ocn_column',
+           'isw_ocn_xch_salt', 'isw_ocn_pen', 'isw_ocn_fresh', 'isw_ocn_diffuse')
 
 
 def check(rc, what):

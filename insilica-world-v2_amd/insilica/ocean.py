@@ -1,9 +1,13 @@
 """The ocean under the air: Boussinesq layers on the air's grid with temperature T and salinity S carried by FCT
-transport, the exact air-sea wind stress, and the land column under land cells. The dynamical part of the reference's
-`Ocean` (src/insilica/ocean.py:49-339): construction, `step` without the column mixing, `couple`, `ground`,
-`update_cap`, `heat`, `salt`, `momentum`. The mixed layer's daily column processes (convection, Kraus-Turner
-entrainment, penetrating sunlight, fresh-water fluxes) are not part of this build.
+transport, the exact air-sea wind stress, the land column under land cells, and the mixed layer's column processes:
+the reference's `Ocean` (src/insilica/ocean.py:49-339): construction, `step` (with `mix`, the daily mixing:
+convective adjustment, Kraus-Turner entrainment or shoaling, the interior interfaces' crossings, salt with the water
+and the exchange between layers), `accumulate` (the air step's sunlight below the mixed layer and its fresh water),
+`couple`, `ground`, `update_cap`, `heat`, `salt`, `momentum`. Density classes (T_classes), the currents' own density
+(eos_currents), squeezed-layer flags (shelf_bed), sea-ice drift and the Ekman spreading (ek_depth) are not part of
+this build.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -14,9 +18,25 @@ from .constants import GRAV
 from .layers import Layers
 from .transport import Transport
 
-OCEAN = dict(depth=4000.0, h_mixed=100.0, h_thermo=600.0, T_thermo=293.15, T_deep=273.15, S_ref=35.0, rho=1025.0,
-             cw=3990.0, dt=3600.0, gprime=(0.016, 0.016), hmin=10.0, visc4_U=0.5, bottom_stress=2.5e-4,
-             interface_stress=5.0e-4, thin=2.0, walls=(), land_drag=1.0, h_layers=None, interface_cd=0.0)
+ALPHA_REF = 2.0e-4          # 1/K: the buoyancy of dTmin, below which a mixed layer's base counts as unstratified
+OCEAN = dict(depth=4000.0, h_mixed=100.0, h_thermo=600.0, T_thermo=293.15, T_deep=273.15, alpha=0.52e-4, alpha2=1.0e-5,
+             beta=7.6e-4, S_ref=35.0, rho=1025.0, cw=3990.0, dt=3600.0, mix_dt=86400.0, gprime=(0.016, 0.016),
+             hmin=10.0, hmax=1000.0, dTmin=0.1, m_kt=0.5, n_kt=0.2, visc4_U=0.5, bottom_stress=2.5e-4,
+             interface_stress=5.0e-4, thin=2.0, walls=(), sw_R=0.58, sw_z1=0.35, sw_z2=23.0, land_drag=1.0,
+             h_layers=None, kappa_d=0.0, dwf=0.0, kappa_h=0.0, fw_local=0.0, interface_cd=0.0, ri_b=0.0, reach=0.0,
+             h_ent=0.0, sw_up=0.0, det_class=0.0)
+UNSUPPORTED = ('T_classes', 'S_classes', 'eos_currents', 'shelf_bed', 'ice_drift', 'ek_depth', 'bed')
+
+
+def ocean_struct(c, grav):
+    """The settings the column kernels read (isw_ocean) from an Ocean's cfg."""
+    o = _hip.IswOcean()
+    o.grav, o.alpha, o.alpha2, o.beta = grav, c['alpha'], c['alpha2'], c['beta']
+    o.S_ref, o.alpha_ref, o.rho_w, o.cw = c['S_ref'], ALPHA_REF, c['rho'], c['cw']
+    for k in ('m_kt', 'n_kt', 'hmin', 'hmax', 'dTmin', 'sw_R', 'sw_z1', 'sw_z2', 'kappa_d', 'dwf', 'ri_b', 'reach',
+              'h_ent', 'det_class', 'fw_local'):
+        setattr(o, k, float(c[k]))
+    return o
 
 
 def visc4(g, U):
@@ -42,6 +62,9 @@ class Ocean:
         = dict(dz, c, k): the land column's layers under the skin."""
         self.dg = dg
         self.cfg = c = dict(OCEAN, **(cfg or {}))
+        for k in UNSUPPORTED:
+            if c.get(k):
+                raise NotImplementedError(f'ocean option {k!r} is not part of this build')
         g = dg.host
         N, E = dg.N, dg.E
         nu4 = visc4(g, c['visc4_U'])
@@ -94,8 +117,18 @@ class Ocean:
         self.nfast = int(math.ceil(1.2 * self.core.substeps(self.h, c['dt'])))
         z = dg.zeros
         self._Fm, self._M1 = z(L, E), z(L, N)
-        self.cap = z(N)
-        self.tau_acc, self.diss = z(E), z(E)
+        self.cap = z(N)                                   # rho_w c_w h_0, for the air's surface physics
+        self.qocean, self.qsw = z(N), z(N)                # W/m^2 into the mixed layer, and its sunlight
+        self.q_acc, self.sun_acc = z(N), z(N)             # their non-solar part and the sunlight summed (J/m^2)
+        self.fw_acc, self.ice_prev = z(N), z(N)           # fresh water in (kg/m^2), summed; the sea ice at the last step
+        self.tau_acc, self.taum, self.diss = z(E), z(N), z(E)
+        self.wx = z(L - 1, N)
+        self._Tn, self._Sn = z(L, N), z(L, N)
+        self._du, self.shear = (z(E), z(N)) if c['ri_b'] > 0 else (None, None)
+        self.ostruct = ocean_struct(c, grav)
+        ratio = c['mix_dt'] / c['dt']
+        assert abs(ratio - round(ratio)) < 1e-9 and round(ratio) >= 1, 'mix_dt must be a whole number of ocean steps'
+        self.mix_every = int(round(ratio))
         self.update_cap()
 
     def _land(self):
@@ -121,15 +154,53 @@ class Ocean:
                       _hip.ptr(self.cap), _hip.ptr(self.Tg), _hip.ptr(self.soil_dz), float(self.soil['c']),
                       float(self.soil['k']), float(dt), _hip.stream())
 
-    def step(self, mix=False):
-        """One ocean step: the layers move and T, S are carried (the daily column mixing is not part of this build)."""
-        if mix:
-            raise NotImplementedError('the ocean column mixing is not part of this build')
-        c = self.cfg
+    def accumulate(self, dt, evap=None, rain=None, ice=None, mouth=None):
+        """The air step's sunlight below the mixed layer's base to the layers below, the fluxes the mixed layer keeps
+        into the mixing's sums, and rain, evaporation and the sea ice's change (kg/m^2/s, kg/m^2) freshening or
+        salting the mixed layer (reference: ocean.py:214-224)."""
+        c, N, s = self.cfg, self.dg.N, _hip.stream()
+        o = ctypes.byref(self.ostruct)
+        if evap is not None:
+            _hip.call('isw_ocn_fresh', N, o, _hip.ptr(self.h), _hip.ptr(self.S), _hip.ptr(evap), _hip.ptr(rain),
+                      _hip.ptr(ice), _hip.ptr(self.ice_prev), float(dt), _hip.ptr(self.fw_acc), _hip.ptr(self.land),
+                      _hip.ptr(mouth), _hip.ptr(self.h), _hip.ptr(self.M), _hip.ptr(self.dg.areaCell), s)
+        _hip.call('isw_ocn_pen', N, self.L, o, _hip.ptr(self.h), _hip.ptr(self.T), _hip.ptr(self.qocean),
+                  _hip.ptr(self.qsw), float(dt), _hip.ptr(self.q_acc), _hip.ptr(self.sun_acc), _hip.ptr(self.land),
+                  _hip.ptr(self.wx if c['sw_up'] > 0 else None), s)
+
+    def step(self, mix=True):
+        """One ocean step: convective adjustment, the layers move and T, S are carried; with mix, the mixed layer then
+        overturns, entrains or shoals on the forcing summed since the last mixing (reference: ocean.py:226-295)."""
+        c, dg, N, s = self.cfg, self.dg, self.dg.N, _hip.stream()
+        o = ctypes.byref(self.ostruct)
+        _hip.call('isw_ocn_overturn', N, self.L, o, _hip.ptr(self.h), _hip.ptr(self.T), _hip.ptr(self.S),
+                  _hip.ptr(self.land), s)
         self.core.step(self.h, self.u, c['dt'], self.nfast, self._Fm)
         self.tr.mass_after(self.M, self._Fm, self._M1)
         self.tr.step_rows(self.M, self._M1, self._Fm, self.rows, order=3)
         self.M.copy_(self._M1)
+        self.update_cap()
+        if not mix:
+            return
+        dt = float(c['mix_dt'])
+        _hip.call('isw_ocn_forcing', dg.ref, dt, _hip.ptr(self.tau_acc), _hip.ptr(self.taum), _hip.ptr(self.q_acc),
+                  _hip.ptr(self.sun_acc), _hip.ptr(self.fw_acc), s)
+        if c['kappa_h'] > 0:
+            _hip.call('isw_ocn_diffuse', dg.ref, self.L, _hip.ptr(self.h), _hip.ptr(self.T), _hip.ptr(self.S),
+                      _hip.ptr(self._Tn), _hip.ptr(self._Sn), _hip.ptr(self.core.wall), float(c['kappa_h']), dt, s)
+            self.T.copy_(self._Tn)
+            self.S.copy_(self._Sn)
+        if self.shear is not None:
+            torch.sub(self.u[0], self.u[1], out=self._du)
+            _hip.call('isw_cell_speed', dg.ref, _hip.ptr(self._du), _hip.ptr(self.shear), s)
+        _hip.call('isw_ocn_column', N, self.L, o, _hip.ptr(self.h), _hip.ptr(self.T), _hip.ptr(self.S),
+                  _hip.ptr(self.taum), _hip.ptr(self.q_acc), _hip.ptr(self.fw_acc), _hip.ptr(self.sun_acc), dt,
+                  _hip.ptr(self.wx), _hip.ptr(self.land), _hip.ptr(self.shear), None, None, s)
+        seq = 1 if c['reach'] > 0 else 0
+        _hip.call('isw_ocn_xch_salt', N, self.L, _hip.ptr(self.wx), _hip.ptr(self.h), _hip.ptr(self.S), seq, s)
+        self.core.exchange(self.wx, self.h, self.u, self.M, self.T, seq=bool(seq))
+        for a in (self.tau_acc, self.q_acc, self.sun_acc, self.fw_acc):
+            _hip.call('isw_fill', _hip.ptr(a), a.numel(), 0.0, s)
         self.update_cap()
 
     def heat(self):

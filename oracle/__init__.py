@@ -22,9 +22,9 @@ _i32p, _i8p, _f64p = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_in
 
 
 def build():
-    """Compile liboracle.so from oracle.c, radiation.c and column.c (seconds)."""
+    """Compile liboracle.so from oracle.c, radiation.c, column.c and ocean.c (seconds)."""
     cmd = ['gcc', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(LIB) + '.tmp',
-           str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), str(HERE / 'column.c'), '-lm']
+           str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), str(HERE / 'column.c'), str(HERE / 'ocean.c'), '-lm']
     subprocess.run(cmd, check=True)
     os.replace(str(LIB) + '.tmp', LIB)
 
@@ -68,6 +68,7 @@ def lib():
             'og_land_column': (None, [vp, i, vp, vp, vp, vp, vp, d, d, d]),
             'og_ocn_cap': (None, [vp, vp, d, vp, vp, d]),
             'og_vmix_heat': (None, [vp, i, vp, vp, d, d, d, d, vp]),
+            'og_lay_vmix': (None, [vp, i, vp, d, d, vp, vp, d]),
             'og_dsum': (d, [vp, vp, ctypes.c_int64]),
             'og_lw_setup': (None, [i, i, vp, vp, vp, d, d, d, vp, vp, vp, vp, vp, d, i, vp]),
             'og_lw_column': (None, [i, i, i, vp, vp, vp, vp, vp]),
@@ -251,6 +252,11 @@ def ocn_cap(og, h, rho_cw, cap, land=None, c_land=2.0e6):
 
 def vmix_heat(og, L, h, diss, theta, grav=9.80616, cp=1004.6, kappa=287.04 / 1004.6, p0=1.0e5):
     lib().og_vmix_heat(og.ref, L, _p(h), _p(diss), grav, cp, kappa, p0, _p(theta))
+
+
+def lay_vmix(og, L, h, c, dt, u, diss, cd=0.0):
+    """The interfacial stress on its own (reference: kernels/layers.cu:575-593), in place on u (L, E); diss (E)."""
+    lib().og_lay_vmix(og.ref, L, _p(h), c, dt, _p(u), _p(diss), cd)
 
 
 def dsum(x, y=None):

@@ -483,6 +483,11 @@ static void lay_vmix(const og_grid* g, int L, const double* h, double c, double 
   }
 }
 
+/* lay_vmix on its own, for the tests' pins of the interfacial stress */
+void og_lay_vmix(const og_grid* g, int L, const double* h, double c, double dt, double* u, double* diss, double cd) {
+  lay_vmix(g, L, h, c, dt, u, diss, cd);
+}
+
 /* vmix_heat (layers.cu:599-621) */
 void og_vmix_heat(const og_grid* g, int L, const double* h, const double* diss, double grav, double cp, double kappa,
                   double p0, double* theta) {

@@ -64,6 +64,7 @@ void og_land_column(const og_grid* g, int K, const double* land, double* T, cons
 void og_ocn_cap(const og_grid* g, const double* h, double rho_cw, double* cap, const double* land, double c_land);
 void og_vmix_heat(const og_grid* g, int L, const double* h, const double* diss, double grav, double cp, double kappa,
                   double p0, double* theta);
+void og_lay_vmix(const og_grid* g, int L, const double* h, double c, double dt, double* u, double* diss, double cd);
 
 /* gpu.dsum's fixed-order tree (reference: kernels/trisk.cu:61-81, gpu.py:111-126). */
 double og_dsum(const double* x, const double* y, int64_t n);
@@ -123,5 +124,24 @@ void og_air_column(int N, int L, const double* par, const og_air* air, int mode,
                    double* swalb, const double* ch, double* firn, double* cloudw, double* cloudo, const double* land,
                    const double* soilcap, double* soil, double* pond, double* snow, const double* lc, int mix);
 double og_sun_lon(double t, const double* par, double* fd);
+
+/* ---- the ocean's column processes (ocean.c; reference kernels/ocean.cu) ---- */
+void og_ocn_overturn(int N, int L, const double* h, double* T, double* S, double grav, double alpha, double alpha2,
+                     double beta, const double* land);
+void og_ocn_column(int N, int L, const double* h, double* T, double* S, const double* taum, const double* qns,
+                   const double* qfw, const double* qsun, double grav, double alpha, double alpha2, double beta,
+                   double S_ref, double alpha_ref, double rho_w, double cw, double m_kt, double n_kt, double hmin,
+                   double hmax, double dTmin, double R, double z1, double z2, double dt, double* wx,
+                   const double* land, double kappa_d, double dwf, const double* shear, double ri_b, double reach,
+                   double h_ent, const double* Tc, double det_class, const double* Sc);
+void og_ocn_xch_salt(int N, int L, const double* wx, const double* h, double* S, int seq);
+void og_ocn_pen(int N, int L, const double* h, double* T, const double* qocean, const double* qsw, double rho_cw,
+                double R, double z1, double z2, double dt, double* qns_acc, double* sun_acc, const double* land,
+                const double* up);
+void og_ocn_fresh(int N, const double* h, double* S, const double* evap, const double* rain, const double* ice,
+                  double* ice_prev, double S_ref, double rho_w, double dt, double* fw_acc, const double* land,
+                  double* mouth, double fw_local, double* hw, double* M, const double* area);
+void og_ocn_diffuse(const og_grid* g, int L, const double* h, const double* T, const double* S, double* Tn,
+                    double* Sn, const double* wall, double kappa_h, double dt);
 
 #endif

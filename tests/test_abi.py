@@ -175,4 +175,34 @@ def test_grid_check_pair_tables():
     assert lib.isw_grid_check(ctypes.byref(g)) == 0
     g.pown = _hip.ISW_PAIR_THREADS + 1
     assert lib.isw_grid_check(ctypes.byref(g)) == E
+    g.pown = _hip.ISW_PAIR_THREADS
+    g.pslots, g.pslots2 = _hip.ISW_PAIR_THREADS + 1, 400               # patch and halo: one thread per cell too
+    assert lib.isw_grid_check(ctypes.byref(g)) == E
+    g.pslots = _hip.ISW_PAIR_THREADS
+    assert lib.isw_grid_check(ctypes.byref(g)) == 0
     assert lib.isw_grid_check(None) == E
+
+
+def test_ocean_column_refusals():
+    """The ocean's column entry points refuse bad sizes and settings on the host (no launch)."""
+    lib = _hip.lib()
+    E = _hip.ISW_EBADARG
+    from insilica.ocean import OCEAN, ocean_struct
+    o = ocean_struct(OCEAN, 9.80616)
+    buf = ctypes.c_void_p(1)
+    assert lib.isw_ocn_overturn(10, 1, ctypes.byref(o), buf, buf, buf, None, None) == E          # one layer
+    assert lib.isw_ocn_overturn(10, 9, ctypes.byref(o), buf, buf, buf, None, None) == E          # beyond LMAX
+    assert lib.isw_ocn_overturn(10, 3, None, buf, buf, buf, None, None) == E
+    bad = ocean_struct(dict(OCEAN, hmin=0.0), 9.80616)
+    assert lib.isw_ocn_overturn(10, 3, ctypes.byref(bad), buf, buf, buf, None, None) == E         # no floor
+    col = lambda oo, dt=86400.0, Tc=None: lib.isw_ocn_column(10, 3, ctypes.byref(oo), buf, buf, buf, buf, buf, buf,
+                                                             buf, dt, buf, None, None, Tc, None, None)
+    assert col(o, dt=0.0) == E
+    dc = ocean_struct(dict(OCEAN, det_class=1.0), 9.80616)
+    assert col(dc) == E                                                 # detrainment by class needs the classes
+    fl = ocean_struct(dict(OCEAN, fw_local=1.0), 9.80616)
+    assert lib.isw_ocn_fresh(10, ctypes.byref(fl), buf, buf, buf, buf, buf, buf, 450.0, buf, None, None, None, None,
+                             None, None) == E                           # fresh water with its volume needs h, M, area
+    assert lib.isw_ocn_xch_salt(10, 1, buf, buf, buf, 0, None) == E
+    assert lib.isw_ocn_forcing(None, 86400.0, buf, buf, buf, buf, buf, None) == E
+    assert lib.isw_ocn_diffuse(None, 3, buf, buf, buf, buf, buf, None, 1.0, 1.0, None) == E
