@@ -646,8 +646,12 @@ __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int 
 // step and Phi* of every cell of the patch's edges' cells' edges (patch, halo, the halo's halo); B) the wind of
 // sub-step it on every edge of patch and halo, in LDS; C) the mass step and Phi* of sub-step it + 1 on patch and
 // halo from B's winds; D) the wind of sub-step it + 1 on the patch's edges. Every value is computed as k_af_muphi and
-// k_af_u compute it, so the result is the same to the bit as two sub-steps of one launch each. The patch's own
-// edges' inputs stay in registers from B to D (one thread per own edge: at most FBS of them).
+// k_af_u compute it, so the result is the same to the bit as two sub-steps of one launch each.
+// Latency: the kernel is a chain of dependent gathers (table -> edge -> value) between three barriers, with ~2.5
+// waves per SIMD at level 6, so only A gathers: thread q loads its own edge's inputs (B, D) and the q-th halo edge's
+// (B) before A, and A leaves each patch-and-halo cell's coefficients sign * l_e * h_e per layer in LDS (cf, L <= 4)
+// with its stiffness and area in registers (patch and halo come first in A's order, so thread q does cell q in both),
+// so that C reads no global memory. Halo edges past the block's FBS threads (a few at level 6) load as they go.
 template <int L>
 __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
                                                    const double* __restrict__ hedge, const double* __restrict__ uin,
@@ -655,20 +659,23 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
                                                    const double* __restrict__ S, const double* __restrict__ Tp,
                                                    const double* __restrict__ G, double* __restrict__ acc,
                                                    double* __restrict__ uout, double* __restrict__ un) {
+  constexpr bool CF = L <= 4;                      // C's coefficients from LDS (else C gathers them again)
   extern __shared__ double sm[];
-  const int N = g.N, E = g.E, PC = g.pcells, PS = g.pslots2, PE = g.pedges2;
+  const int N = g.N, E = g.E, PC = g.pcells, PS = g.pslots2, PE = g.pedges2, P1 = g.pslots;
   double* phiA = sm;                               // [L][PS] Phi* of sub-step it
   double* muA = phiA + L * PS;                     // [L][PS] the masses after it (patch and halo)
   double* uB = muA + L * PS;                       // [L][PE] the winds after it
   double* phiC = uB + L * PE;                      // [L][PS] Phi* of sub-step it + 1
+  double* cfs = phiC + L * PS;                     // [L][6][P1] (CF) the coefficients of patch and halo cells
   const int b = xblock();
   const int c0 = b * PC, nown = min(PC, N - c0);
   const int h0 = g.ph_off[b], nh = g.ph_off[b + 1] - h0;
   const int k0 = g.ph2_off[b], nk = g.ph2_off[b + 1] - k0;
   const int e0 = g.pe_off[b], ne = g.pe_off[b + 1] - e0;
   const int x0 = g.px_off[b], nx = g.px_off[b + 1] - x0;
-  // this thread's own edge (if any): its inputs for B and D, loaded first
+  const int nC = nown + nh;                        // <= P1 <= FBS (checked with the tables on the host)
   const int q0 = threadIdx.x;
+  // this thread's own edge (if any): its inputs for B and D
   int eo = 0, slo = 0;
   Sub s1{false, false, 0.0}, s2{false, false, 0.0};
   double v_[L], a_[L], tp_[L], rde_ = 0, G_ = 0;
@@ -689,18 +696,53 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
       }
     }
   }
+  // the q-th halo edge of B: its inputs
+  int xsl = 0;
+  Sub sx{false, false, 0.0};
+  double xv_[L], xtp_[L], xrde = 0, xG = 0;
+  if (q0 < nx) {
+    const int xe = g.px_edge[x0 + q0];
+    xsl = g.px_slot[x0 + q0];
+    sx = sub_at(p, p.we, xe, stage, it, n, tau);
+    if (sx.active) {
+      xrde = dv.de[xe];
+      xG = G[xe];
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        const size_t t = (size_t)l * E + xe;
+        xv_[l] = uin[t];
+        xtp_[l] = Tp[t];
+      }
+    }
+  }
+  // the q-th patch-and-halo cell's edge slots in B's winds (0xffff: no edge)
+  int ces[6] = {0xffff, 0xffff, 0xffff, 0xffff, 0xffff, 0xffff};
+  if (q0 < nC) {
+    const int* ps_ = g.pc_eslot + 3 * g.pc_off[b] + 3 * q0;
+    const int pa = ps_[0], pb2 = ps_[1], pcc = ps_[2];
+    ces[0] = pa & 0xffff; ces[1] = pa >> 16; ces[2] = pb2 & 0xffff;
+    ces[3] = pb2 >> 16; ces[4] = pcc & 0xffff; ces[5] = pcc >> 16;
+  }
   // A: masses and Phi* of sub-step it on patch, halo and second halo
-  for (int s = threadIdx.x; s < nown + nh + nk; s += FBS) {
-    const int i = s < nown ? c0 + s : s < nown + nh ? g.ph_cell[h0 + s - nown] : g.ph2_cell[k0 + s - nown - nh];
+  int ci = 0, cslot = 0;                           // (thread q's cell q, for C)
+  Sub cc{false, false, 0.0};
+  double cS[L], cA = 0;
+  for (int s = q0; s < nown + nh + nk; s += FBS) {
+    const int i = s < nown ? c0 + s : s < nC ? g.ph_cell[h0 + s - nown] : g.ph2_cell[k0 + s - nC];
     const int slot = s < nown ? s : PC + (s - nown);
     const Sub sb = sub_at(p, p.wc, i, stage, it, n, tau);
+    if (s < nC) {
+      ci = i;
+      cslot = slot;
+      cc = sub_at(p, p.wc, i, stage, it + 1, n, tau);
+    }
     if (!sb.active) continue;
     int e[6];
     ring6(g.cellEdge, i, e);
     double sl[6];
 #pragma unroll
     for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[i * 6 + m] * g.le[ok0(e[m])];
-    double sak[L];
+    double sak[L], Sk[L];
 #pragma unroll
     for (int k = 0; k < L; k++) {
       const double* hl = hedge + (size_t)k * E;
@@ -709,7 +751,9 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
 #pragma unroll
       for (int m = 0; m < 6; m++) {
         const int ek = ok0(e[m]);
-        t[m] = sl[m] * hl[ek] * ul[ek];
+        const double c = sl[m] * hl[ek];
+        if (CF && s < nC) cfs[(k * 6 + m) * P1 + s] = c;
+        t[m] = c * ul[ek];
       }
       double div = 0;
 #pragma unroll
@@ -719,14 +763,19 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
       const double m1 = m0 - DIVBY(sb.tau * div, dv.A[i]);
       muA[k * PS + slot] = m1;
       sak[k] = m1 + p.div_damp * (m1 - m0);
+      Sk[k] = S[(size_t)k * N + i];
     }
-    double P[L], Sk[L];
+    if (s < nC) {
+#pragma unroll
+      for (int k = 0; k < L; k++) cS[k] = Sk[k];
+      cA = dv.A[i];
+    }
+    double P[L];
     double a = 0;
 #pragma unroll
     for (int m = 0; m < L; m++) {
       a += sak[m];
       P[m] = a;
-      Sk[m] = S[(size_t)m * N + i];
     }
 #pragma unroll
     for (int k = 0; k < L; k++) {
@@ -737,7 +786,7 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
     }
   }
   __syncthreads();
-  // B: the winds of sub-step it on the patch's edges (registers) and the halo's (loaded here)
+  // B: the winds of sub-step it on the patch's edges (registers) and the halo's
   double u1_[L], acc1_[L];
   if (q0 < ne && s1.active) {
     const int sI = slo & 0xffff, sJ = slo >> 16;
@@ -756,7 +805,13 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
       }
     }
   }
-  for (int w = threadIdx.x; w < nx; w += FBS) {
+  if (q0 < nx && sx.active) {
+    const int sI = xsl & 0xffff, sJ = xsl >> 16;
+#pragma unroll
+    for (int l = 0; l < L; l++)
+      uB[l * PE + ne + q0] = xv_[l] + sx.tau * (DIVBY(-(phiA[l * PS + sJ] - phiA[l * PS + sI]), xrde) + xtp_[l] + xG);
+  }
+  for (int w = q0 + FBS; w < nx; w += FBS) {       // (halo edges past the block's threads)
     const int e = g.px_edge[x0 + w];
     const Sub sb = sub_at(p, p.we, e, stage, it, n, tau);
     if (!sb.active) continue;
@@ -771,55 +826,54 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
     }
   }
   __syncthreads();
-  // C: masses and Phi* of sub-step it + 1 on patch and halo, from B's winds
-  const int cb = 3 * g.pc_off[b];
-  for (int s = threadIdx.x; s < nown + nh; s += FBS) {
-    const int i = s < nown ? c0 + s : g.ph_cell[h0 + s - nown];
-    const int slot = s < nown ? s : PC + (s - nown);
-    const Sub sb = sub_at(p, p.wc, i, stage, it + 1, n, tau);
-    if (!sb.active) continue;
-    int e[6];
-    ring6(g.cellEdge, i, e);
-    const int* ps_ = g.pc_eslot + cb + 3 * s;
-    const int pa = ps_[0], pb2 = ps_[1], pcc = ps_[2];
-    const int es[6] = {pa & 0xffff, pa >> 16, pb2 & 0xffff, pb2 >> 16, pcc & 0xffff, pcc >> 16};
-    double sl[6];
+  // C: masses and Phi* of sub-step it + 1 on patch and halo (thread q's cell q), from B's winds
+  if (q0 < nC && cc.active) {
+    double cf[L][6];
+    if constexpr (CF) {
 #pragma unroll
-    for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[i * 6 + m] * g.le[ok0(e[m])];
+      for (int k = 0; k < L; k++)
+#pragma unroll
+        for (int m = 0; m < 6; m++) cf[k][m] = cfs[(k * 6 + m) * P1 + q0];
+    } else {
+      int e[6];
+      ring6(g.cellEdge, ci, e);
+      double sl[6];
+#pragma unroll
+      for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[ci * 6 + m] * g.le[ok0(e[m])];
+#pragma unroll
+      for (int k = 0; k < L; k++)
+#pragma unroll
+        for (int m = 0; m < 6; m++) cf[k][m] = sl[m] * hedge[(size_t)k * E + ok0(e[m])];
+    }
     double m1k[L], sak[L];
 #pragma unroll
     for (int k = 0; k < L; k++) {
-      const double* hl = hedge + (size_t)k * E;
       double t[6];
 #pragma unroll
-      for (int m = 0; m < 6; m++) {
-        const int ek = ok0(e[m]);
-        t[m] = sl[m] * hl[ek] * uB[k * PE + (e[m] >= 0 ? es[m] : 0)];
-      }
+      for (int m = 0; m < 6; m++) t[m] = cf[k][m] * uB[k * PE + (ces[m] != 0xffff ? ces[m] : 0)];
       double div = 0;
 #pragma unroll
       for (int m = 0; m < 6; m++)
-        if (e[m] >= 0) div += t[m];
-      const double m0 = muA[k * PS + slot];
-      const double m1 = m0 - DIVBY(sb.tau * div, dv.A[i]);
+        if (ces[m] != 0xffff) div += t[m];
+      const double m0 = muA[k * PS + cslot];
+      const double m1 = m0 - DIVBY(cc.tau * div, cA);
       m1k[k] = m1;
       sak[k] = m1 + p.div_damp * (m1 - m0);
     }
-    double P[L], Sk[L];
+    double P[L];
     double a = 0;
 #pragma unroll
     for (int m = 0; m < L; m++) {
       a += sak[m];
       P[m] = a;
-      Sk[m] = S[(size_t)m * N + i];
     }
 #pragma unroll
     for (int k = 0; k < L; k++) {
       double ph = 0;
 #pragma unroll
-      for (int m = L - 1; m >= k; m--) ph += Sk[m] * P[m];
-      phiC[k * PS + slot] = ph;
-      if (s < nown) mu_out[(size_t)k * N + i] = m1k[k];
+      for (int m = L - 1; m >= k; m--) ph += cS[m] * P[m];
+      phiC[k * PS + cslot] = ph;
+      if (q0 < nown) mu_out[(size_t)k * N + ci] = m1k[k];
     }
   }
   __syncthreads();
@@ -1678,7 +1732,9 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
       uin = s->F;
     }
     const double* mus = h0;
-    const size_t pair_lds = g->ph2_off ? (size_t)L * (3 * g->pslots2 + g->pedges2) * sizeof(double) : 0;
+    // LDS of the pair kernel: Phi* twice, masses and winds; with L <= 4 also its patch-and-halo cells' coefficients
+    const size_t pair_lds =
+        g->ph2_off ? (size_t)L * (3 * g->pslots2 + g->pedges2 + (L <= 4 ? 6 * g->pslots : 0)) * sizeof(double) : 0;
     int it = 0;
     int flip = 0;
     if (g->ph2_off && pair_lds <= 65536) {

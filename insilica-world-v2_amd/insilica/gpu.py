@@ -76,12 +76,13 @@ def fast_patches2(grid, pt, max_owned=_hip.ISW_PAIR_THREADS):
     order, then the others); the second sub-step's cells are patch and halo again. Cell slots: the patch at its
     offsets, the halo from pcells, the second halo after it; edge slots: the patch's edges 0 .. ne - 1, the others
     from ne. pc_eslot: each patch-and-halo cell's six edge slots, two per int32 (0xffff for a missing edge). None
-    when a patch owns more than max_owned edges (one thread per owned edge in the kernel)."""
+    when a patch owns more than max_owned edges or its patch and halo hold more than max_owned cells (one thread per
+    owned edge and per patch-and-halo cell in the kernel)."""
     N, PC, nb = int(grid.N), pt['pcells'], pt['npatch']
     eI, eJ = np.asarray(grid.eI, np.int64), np.asarray(grid.eJ, np.int64)
     ce = np.asarray(grid.cellEdge, np.int64)
     ph_off, ph_cell, pe_off, pe_edge = pt['ph_off'], pt['ph_cell'], pt['pe_off'], pt['pe_edge']
-    if np.max(np.diff(pe_off)) > max_owned:
+    if np.max(np.diff(pe_off)) > max_owned or pt['pslots'] > max_owned:
         return None
     h2_off, h2_cell, x_off, x_edge, x_slot, c_off, c_slot = [0], [], [0], [], [], [0], []
     ps2 = pe2 = 0
