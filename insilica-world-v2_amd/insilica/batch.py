@@ -11,6 +11,7 @@ steps exactly as it does alone, bit for bit (the kernels gather only within a wo
 No counterpart in the reference, whose Ensemble runs one CUDA graph per world on its own stream
 (src/insilica/ensemble.py:17-23).
 """
+import ctypes
 import hashlib
 import math
 
@@ -141,6 +142,8 @@ class BatchWorld:
         self.land = cat1(w.sea.land for w in worlds)
         self.soil, self.soil_dz = s0.soil, s0.soil_dz
         self.ocfg, self.c_land = s0.cfg, s0.c_land
+        from .ocean import ocean_struct
+        self.ostruct = ocean_struct(s0.cfg, w0.grav)
         self.surface_drag = w0.surface_drag
         z = dg.zeros
         self._Fm, self._M1 = z(3, dg.E), z(3, dg.N)
@@ -173,6 +176,11 @@ class BatchWorld:
                   _hip.ptr(self.cap), _hip.ptr(self.oTg), _hip.ptr(self.soil_dz), float(self.soil['c']),
                   float(self.soil['k']), float(self.dt), st())
         if kind:
+            # the ocean step's convective adjustment first (Ocean.step); its buoyancy comparisons take the first
+            # world's gravity, which orders every world's layers as its own (g > 0 scales both sides alike)
+            L = self.ocean.L
+            _hip.call('isw_ocn_overturn', dg.N, L, ctypes.byref(self.ostruct), _hip.ptr(self.oh),
+                      _hip.ptr(self.orows[:L]), _hip.ptr(self.orows[L:]), _hip.ptr(self.land), st())
             self.ocean.step(self.oh, self.ou, c['dt'], 1, self._oFm)
             self.otr.mass_after(self.oM, self._oFm, self._oM1)
             self.otr.step_rows(self.oM, self._oM1, self._oFm, self.orows, order=3)
@@ -212,7 +220,8 @@ class BatchWorld:
         cs = lambda t: t[:, c0:c1].contiguous()
         es = lambda t: t[:, e0:e1].contiguous()
         return {'h': cs(self.h), 'u': es(self.u), 'M': cs(self.M), 'rows': cs(self.rows), 'sea.h': cs(self.oh),
-                'sea.u': es(self.ou), 'sea.M': cs(self.oM), 'sea.rows': cs(self.orows), 'sea.Tg': cs(self.oTg)}
+                'sea.u': es(self.ou), 'sea.M': cs(self.oM), 'sea.rows': cs(self.orows),
+                'sea.cap': self.cap[c0:c1].contiguous(), 'sea.Tg': cs(self.oTg)}
 
     def state_hash(self, k):
         h = hashlib.sha1()

@@ -93,6 +93,26 @@ def internal_wave(g, h1, h2, amp=50.0, radius=1.0e6):
     return np.stack([h1 - d, h2 + d]), np.zeros(g.N), np.zeros((2, g.E))
 
 
+def outcrop(g, H=(2000.0, 500.0), U=0.5, ridge=900.0, gprime=0.02, floor=1.0):
+    """Two Boussinesq layers in balanced solid-body rotation, u = U cos(lat) in both, over a meridional ridge at 0 E
+    (crest `ridge` m, e-folding half-width 8 degrees of longitude, fading as cos^2(lat)) that is higher than the lower
+    layer is deep: over the crest the lower layer starts at the floor and the upper layer meets the ridge (reference
+    preset 'outcrop', cases.py:194-201, 216). Balance: a layer's Montgomery potential in solid-body rotation is
+    C_k - (a Om U + U^2 / 2) sin^2(lat); with the same U in both layers the interface between them is level (at H[1]
+    above the flat bottom) and the free surface carries the whole potential. Area means: H[0] + H[1] for the column."""
+    a, Om, G = planet_of(g)
+    wrapped = np.arctan2(np.sin(g.lon), np.cos(g.lon))
+    b = ridge * np.exp(-(np.abs(wrapped) / math.radians(8.0)) ** 2) * np.cos(g.lat) ** 2
+    w = g.areaCell / np.sum(g.areaCell)
+    surf = -(a * Om * U + 0.5 * U * U) * np.sin(g.lat) ** 2 / G
+    eta = surf + (H[0] + H[1]) - np.sum(w * surf)
+    lower = np.maximum(H[1] - b, floor)
+    upper = eta - b - lower
+    lat_e, _, en, _ = edge_east_north(g)
+    ue = U * np.cos(lat_e) * en
+    return np.stack([upper, lower]), b, np.stack([ue, ue])
+
+
 def internal_wave_speed(gprime, h1, h2, grav=GRAV):
     """The internal mode's exact speed for two Boussinesq layers under a free surface."""
     H = h1 + h2
@@ -119,9 +139,11 @@ def zonal_jet_layers(g, H, ujets, gprime, M=20000):
     return np.stack([eta - z1, z1]), np.zeros(g.N), np.stack([uj(lat_e) * en for uj in ujets])
 
 
-def phillips(g, shear_factor, H=(5000.0, 5000.0), Ld=5.0e5, width_deg=7.0):
-    """The balanced (unperturbed) Phillips jet: equal layers, the upper a Gaussian jet at 45N of shear_factor times
-    the critical shear 2 beta Ld^2, the lower at rest. Returns (h, b, u, gprime)."""
+def phillips(g, shear_factor, H=(5000.0, 5000.0), Ld=5.0e5, width_deg=7.0, bump=0.0):
+    """The balanced Phillips jet: equal layers, the upper a Gaussian jet at 45N of shear_factor times the critical
+    shear 2 beta Ld^2, the lower at rest; with bump > 0 the interface is displaced by bump H[1] cos(6 lon) within 10
+    degrees of 45N (the seed of the instability; reference preset 'phillips', cases.py:174-191). Returns
+    (h, b, u, gprime)."""
     lat0 = math.pi / 4
     f0, beta = 2 * OMEGA * math.sin(lat0), 2 * OMEGA * math.cos(lat0) / R_PLANET
     He = H[0] * H[1] / (H[0] + H[1])
@@ -130,6 +152,9 @@ def phillips(g, shear_factor, H=(5000.0, 5000.0), Ld=5.0e5, width_deg=7.0):
     sig = math.radians(width_deg)
     jet = lambda phi: U0 * np.exp(-((np.asarray(phi) - lat0) / sig) ** 2)
     h, b, u = zonal_jet_layers(g, H, (jet, lambda phi: 0.0 * np.asarray(phi)), gprime)
+    if bump:
+        d = bump * H[1] * np.cos(6 * g.lon) * np.exp(-((g.lat - lat0) / math.radians(10.0)) ** 2)
+        h = np.stack([h[0] - d, h[1] + d])
     return h, b, u, gprime
 
 

@@ -34,12 +34,39 @@ from .transport import BETA, Transport
 
 THIN_WIND = 100.0
 GAS_TRACERS = (('co2', 400e-6), ('n2o', 0.32e-6), ('ch4', 1.8e-6))
-PRESETS = ('w2', 'w5', 'jet', 'coupled_stack', 'held_suarez', 'aqua_dry', 'aqua', 'aqua5', 'aqua7')
+PRESETS = ('w2', 'w5', 'jet', 'outcrop', 'phillips', 'coupled_stack', 'held_suarez', 'aqua_dry', 'aqua', 'aqua5',
+           'aqua7', 'coupled', 'basin')
+AQUA = ('aqua_dry', 'aqua', 'aqua5', 'aqua7', 'coupled', 'basin')
+# the layered Boussinesq presets' settings (reference: cases.py:208-217): dt, g', del^4's velocity scale, stress
+LAYERED = {'outcrop': dict(dt=3600.0, gprime=[0.02], visc4_U=0.5, stress=[0.0, 2.5e-4]),
+           'phillips': dict(visc4_U=20.0)}
+
+
+def _jsonable(x):
+    """params as JSON: tuples and arrays become lists, NumPy scalars numbers."""
+    if isinstance(x, dict):
+        return {str(k): _jsonable(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple, np.ndarray)):
+        return [_jsonable(v) for v in (x.tolist() if isinstance(x, np.ndarray) else x)]
+    if isinstance(x, np.generic):
+        return x.item()
+    return x
+
+
+def _mesh_hash(g):
+    """A digest of the mesh's numbering and geometry (its cell positions, edge cells and cell areas)."""
+    h = hashlib.sha1()
+    for a in (g.pos, g.eI, g.eJ, g.areaCell):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
 
 
 class World:
-    def __init__(self, n=4, preset='w2', params=None, precision=None, state=None):
-        self.n, self.preset = n, preset
+    def __init__(self, n=4, preset='w2', params=None, precision=None, state=None, layers=None):
+        """A world on level n from case `preset` (params override its settings). layers=K splits a one-layer case
+        into K equal layers with g' = 0 (they then move as one); state=(h, b, u) starts a layered case from given
+        arrays (reference: world.py:56-58)."""
+        self.n, self.preset, self.precision = n, preset, precision
         P = self.params = dict(params or {})
         if preset not in PRESETS:
             raise ValueError(f'unknown preset {preset!r}; this build has {PRESETS}')
@@ -48,17 +75,18 @@ class World:
         self.grid = gridmod.for_planet(gridmod.get(n), **self.planet)
         self.dg = dg = DeviceGrid(self.grid, precision)
         g = self.grid
-        self.air = preset in ('coupled_stack', 'held_suarez', 'aqua_dry', 'aqua', 'aqua5', 'aqua7')
+        self.air = preset in ('coupled_stack', 'held_suarez') + AQUA
         self.sea = self.physics = None
         self.Ts = self.ice = self.cw = None
         self.rad_every = 0
-        if preset in ('aqua_dry', 'aqua', 'aqua5', 'aqua7'):
+        if preset in AQUA:
             # three air layers at rest over a slab aquaplanet under the diurnal and seasonal sun, gray radiation (or
             # the k-tables with longwave='ktables'), the bottom layer dragged by the surface; 'aqua' with vapor and
             # sea ice (reference presets 'aqua_dry' and 'aqua', cases.py:225-277): the column step then the exchange
             # of the mass it moves
             from .atmosphere import AirPhysics
             moist = preset != 'aqua_dry'
+            coupled = preset in ('coupled', 'basin')
             # with clouds (physics cloud > 0, moist) the cloud water per layer, starting at none and moved as a tracer
             # (reference: world.py:145-148, 294-296)
             cloudy = moist and P.get('physics', {}).get('cloud', 0.0) > 0
@@ -72,8 +100,10 @@ class World:
             self.dt = float(P.get('dt', 900.0 * 2.0 ** (5 - n)))
             nu4 = P.get('nu4', visc4(g, P.get('visc4_U', 15.0)))
             floor = P.get('floor', 10.0)
+            # (coupled: the air's drag on the surface is the wind stress on the ocean, so its own bottom stress is off)
             self.core = Layers(dg, L, b=b, grav=self.grav, gprime=[0.0] * (L - 1), drag=P.get('drag', 0.0),
-                               stress=P.get('stress', [0.0] * (L - 1) + [cases.SURFACE_DRAG]), nu4=nu4, floor=floor,
+                               stress=P.get('stress', [0.0] * L if coupled else
+                                            [0.0] * (L - 1) + [cases.SURFACE_DRAG]), nu4=nu4, floor=floor,
                                rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP),
                                air=dict(cp=CP_DRY, kappa=KAPPA, rd=R_DRY), vmix=P.get('vmix', 0.0),
                                thin_wind=P.get('thin_wind', THIN_WIND))
@@ -85,10 +115,28 @@ class World:
                 self.ice = dg.zeros(g.N)
             if cloudy:
                 self.cw = self.rows[2 * L:3 * L]
-            self.Ts = dg.asarray(extra['Ts'])
             self.psi = dg.asarray(np.tile(cases.cosine_bell(g), (L, 1)))      # moved by the exchange only
+            if coupled:
+                # the ocean under the air (reference presets 'coupled' and 'basin', cases.py:280-287): its mixed layer's
+                # temperature is the air's surface temperature; evaporation recalibrated for the ocean's surface
+                ocfg = dict(P.get('ocean', {}))
+                if preset == 'basin':
+                    ocfg.setdefault('walls', (180.0,))
+                self.sea = Ocean(dg, ocfg, Tsurf=extra['Ts'], grav=self.grav)
+                self.surface_drag = P.get('surface_drag', cases.SURFACE_DRAG)
+                ratio = self.sea.cfg['dt'] / self.dt
+                if abs(ratio - round(ratio)) > 1e-9 or round(ratio) < 1:
+                    raise ValueError(f'the ocean step {self.sea.cfg["dt"]} s must be a whole number of air steps '
+                                     f'({self.dt} s)')
+                self.ocean_every = int(round(ratio))
+                self.Ts = self.sea.T[0]                  # a view: the air's surface physics heats the mixed layer
+                self.sea.ice_prev.copy_(self.ice)
+            else:
+                self.Ts = dg.asarray(extra['Ts'])
             self.tr = Transport(dg, layers=L, beta=P.get('beta', BETA), tracers=len(rows))
             phys = dict(theta_stars=list(stars), floor_=floor, grav=self.grav, moist=1.0 if moist else 0.0)
+            if coupled:
+                phys.update(k_evap=4.8e-3, firn=2.0)
             if self.grav != GRAV:
                 phys.update(mu_ref=1.0e5 / self.grav, mu_bl=1.0e4 / self.grav)
             phys.update(P.get('physics', {}))
@@ -148,10 +196,18 @@ class World:
                 raise ValueError(f'the ocean step {self.sea.cfg["dt"]} s must be a whole number of air steps ({self.dt} s)')
             self.ocean_every = int(round(ratio))
         else:
+            spec = LAYERED.get(preset, {})
             if preset in ('w2', 'w5'):
                 h, b, u = cases.williamson(g, preset)
                 h, u = h[None], u[None]
                 self.dt = float(P.get('dt', cases.default_dt(n)))
+            elif preset == 'outcrop':
+                h, b, u = cases.outcrop(g)
+                self.dt = float(P.get('dt', spec['dt']))
+            elif preset == 'phillips':
+                h, b, u, gp3 = cases.phillips(g, 3.0, bump=0.01)
+                spec = dict(spec, gprime=[gp3])
+                self.dt = float(P.get('dt', 1200.0 * 2.0 ** (5 - n)))
             else:
                 h, b, u = cases.jet_bumps(g, seed=P.get('seed', 0))
                 self.dt = float(P.get('dt', 600.0))
@@ -159,11 +215,16 @@ class World:
                 h, b, u = state[:3]
             h = np.asarray(h, dtype=float).reshape(-1, g.N)
             u = np.asarray(u, dtype=float).reshape(-1, g.E)
+            if layers and layers != h.shape[0]:          # (one layer split into equal layers with g' = 0)
+                assert h.shape[0] == 1, 'only a one-layer case splits into layers'
+                h, u = np.repeat(h / layers, layers, axis=0), np.repeat(u, layers, axis=0)
+                P.setdefault('gprime', [0.0] * (layers - 1))
             self.L = h.shape[0]
-            nu4 = P.get('nu4', visc4(g, P['visc4_U']) if 'visc4_U' in P else 0.0)
-            gp = P.get('gprime', [0.0] * (self.L - 1))
+            U4 = P.get('visc4_U', spec.get('visc4_U'))
+            nu4 = P.get('nu4', visc4(g, U4) if U4 is not None else 0.0)
+            gp = P.get('gprime', spec.get('gprime', [0.0] * (self.L - 1)))
             self.core = Layers(dg, self.L, b=b, grav=self.grav, gprime=gp, drag=P.get('drag', 0.0),
-                               stress=P.get('stress', 0.0), nu4=nu4, floor=P.get('floor', 1.0),
+                               stress=P.get('stress', spec.get('stress', 0.0)), nu4=nu4, floor=P.get('floor', 1.0),
                                rotation=P.get('rotation', 1.0), div_damp=P.get('div_damp', DIV_DAMP))
             self.tracer = P.get('tracer', True)
             self.rows = dg.asarray(np.tile(cases.cosine_bell(g), (self.L, 1)))
@@ -186,44 +247,63 @@ class World:
         # the world's own stream (default priority): its ticks queue there, so an ensemble's members overlap
         self._stream = torch.cuda.Stream()
         self._stream.wait_stream(torch.cuda.current_stream())
-        self.STATE = ('h', 'u', 'M', 'rows') + (('sea.h', 'sea.u', 'sea.M', 'sea.rows', 'sea.Tg')
-                                                 if self.sea is not None else ()) + \
-            (('psi', 'time') if self.physics is not None else ()) + (('Ts',) if self.Ts is not None else ()) + \
-            (('ice',) if self.ice is not None else ())
+        # the ocean mixes its mixed layer daily in the reference's coupled worlds (not in coupled_stack, whose ocean
+        # steps its layers and tracers only)
+        self.sea_mix = self.sea is not None and preset in ('coupled', 'basin')
+        # (the mixed layer's heat capacity is derived from its thickness at the last ocean step, and read by the air
+        # column and the land column until the next: state)
+        sea = () if self.sea is None else (('sea.h', 'sea.u', 'sea.M', 'sea.rows', 'sea.cap') +
+                                           (('sea.Tg',) if self.sea.soil is not None else ()) +
+                                           (('sea.ice_prev', 'sea.tau_acc', 'sea.q_acc', 'sea.sun_acc', 'sea.fw_acc')
+                                            if self.sea_mix else ()))
+        self.STATE = ('h', 'u', 'M', 'rows') + sea + (('psi', 'time') if self.physics is not None else ()) + \
+            (('Ts',) if self.Ts is not None and not self.sea_mix else ()) + (('ice',) if self.ice is not None else ())
 
     # ---- the tick ----
     def _kind(self, tick):
-        """The tick's kind: bit 0 the ocean's step, bit 1 the k-tables' radiation (every rad_every ticks from the
-        first; reference: world.py:283-284)."""
+        """The tick's kind: bit 0 the ocean's step, bit 2 with the mixed layer's daily mixing, bit 1 the k-tables'
+        radiation (every rad_every ticks from the first; reference: world.py:277-284)."""
         k = 0 if self.sea is None or (tick + 1) % self.ocean_every else 1
+        if k and self.sea_mix and (tick + 1) % (self.ocean_every * self.sea.mix_every) == 0:
+            k |= 4
         if self.rad_every and tick % self.rad_every == 0:
             k |= 2
         return k
 
     def _tick(self, kind=0):
-        """One tick's work on fixed buffers (nothing is allocated), so the sequence can be captured as a graph."""
+        """One tick's work on fixed buffers (nothing is allocated), so the sequence can be captured as a graph, in the
+        reference's order (world.py:286-358): the layers' step and transport; the wind stress on the ocean with its
+        heat; the radiation (the k-tables, saved for the column steps until the next), the column (the radiation or
+        Held & Suarez's relaxation, the crossing into wx, the clock), the ground, the ocean's forcing (its sunlight
+        below the mixed layer and fresh water), the exchange of the column's mass between the layers; then the ocean's
+        step (and its mixing)."""
         self.core.step(self.h, self.u, self.dt, self.nfast, self._Fm, theta=self.theta if self.air else None)
         self.tr.mass_after(self.M, self._Fm, self._M1)
         if self.air or self.tracer:
             self.tr.step_rows(self.M, self._M1, self._Fm, self.rows, order=3)
         self.M.copy_(self._M1)
-        if self.sea is not None:
-            self.sea.couple(self.h, self.u, self.surface_drag, self.dt)
-            self.core.vmix_heat(self.h, self.sea.diss, self.theta)
-            self.sea.ground(self.dt)
-            if kind & 1:
-                self.sea.step(mix=False)
+        sea, moist = self.sea, self.ice is not None
+        if sea is not None:
+            sea.couple(self.h, self.u, self.surface_drag, self.dt)
+            self.core.vmix_heat(self.h, sea.diss, self.theta)
         if self.physics is not None:
-            # the radiation step (the k-tables, saved for the column steps until the next), the column (the
-            # radiation or Held & Suarez's relaxation, the crossing into wx, the clock), then the exchange of that
-            # mass with its theta, momentum and tracer between the layers (reference: world.py:319-344)
-            moist = self.ice is not None
             if kind & 2:
                 self.physics.radiate(self.h, self.theta, self.Ts, self.time, self.rad_every * self.dt,
                                      q=self.q if moist else None, cw=self.cw)
             self.physics.column(self.h, self.theta, self.Ts, self.u, self.time, self.dt, elev=self.b,
-                                q=self.q if moist else None, ice=self.ice if moist else None, cw=self.cw)
+                                q=self.q if moist else None, ice=self.ice if moist else None, cw=self.cw,
+                                cap=sea.cap if self.sea_mix else None, qocean=sea.qocean if self.sea_mix else None,
+                                qsw=sea.qsw if self.sea_mix else None)
+        if sea is not None:
+            sea.ground(self.dt)
+            if self.sea_mix:
+                p = self.physics
+                sea.accumulate(self.dt, p.evap if moist else None, p.rain if moist else None,
+                               self.ice if moist else None)
+        if self.physics is not None:
             self.core.exchange(self.physics.wx, self.h, self.u, self.M, self.psi, theta=self.theta)
+        if kind & 1:
+            sea.step(mix=bool(kind & 4))
 
     def launch_tick(self, graph=True):
         """Queue one tick on the world's stream without waiting for it (step() and Ensemble.step() call this)."""
@@ -281,13 +361,76 @@ class World:
         """Total energy (J): kinetic + potential (+ for air the enthalpy; over a slab its heat c_slab Ts; reference:
         world.py:424-441)."""
         e = self.core.energy(self.h, self.u, self.theta if self.air else None)
-        if self.Ts is not None and self.sea is None:
+        if self.sea_mix:                                 # the ocean's heat in place of the slab's
+            e += self.sea.heat()
+        elif self.Ts is not None and self.sea is None:
             e += self.physics.p['c_slab'] * csum(self.dg, self.Ts, self.dg.areaCell)
         if self.ice is not None:                         # the vapor's latent energy, the sea ice's -L_f m
             A, p = self.dg.areaCell, self.physics.p
             e += p['Lv'] * sum(csum(self.dg, self.h[k] * self.q[k], A) for k in range(self.L))
             e -= p['Lf'] * csum(self.dg, self.ice, A)
         return e
+
+    def water(self):
+        """The air's vapor and cloud water (kg): sum A sum_k mu_k (q_k + cw_k) (reference: world.py:420-422)."""
+        if self.ice is None:
+            return 0.0
+        w = self.q if self.cw is None else self.q + self.cw
+        return csum(self.dg, torch.sum(self.h * w, dim=0), self.dg.areaCell)
+
+    def exchange(self, wx):
+        """Move thickness wx (L - 1, N; positive up) between neighbouring layers with the tracer and momentum
+        (reference: world.py:399-403)."""
+        wx = torch.as_tensor(np.asarray(wx, dtype=float), device=self.dg.dev).reshape(self.L - 1, self.dg.N)
+        with torch.cuda.stream(self._stream):
+            self.core.exchange(wx.contiguous(), self.h, self.u, self.M, self.psi)
+        self.sync()
+
+    def fields(self):
+        """The cell fields `field` serves for this world (reference: world.py:445-456, for the parts built here)."""
+        base = ['h', 'eta'] + ([] if self.air else ['psi'])
+        if not self.air:
+            return base
+        out = base + ['theta', 'ps'] + (['Ts'] if self.Ts is not None else [])
+        if self.ice is not None:
+            out += ['q', 'ice']
+        if self.sea is not None:
+            out += ['sst', 'sss']
+        return out
+
+    # ---- save / load (the reference's .iw2 files, world.py:581-611; io/save.py) ----
+    def save(self, path):
+        """Write the world's state, tick and settings to path: the same state gives the same bytes."""
+        from . import savefile
+        self.sync()
+        meta = dict(kind='insilica-world-v2', n=self.n, preset=self.preset, layers=self.L, dt=self.dt, tick=self.tick,
+                    params=_jsonable(self.params), precision=self.dg.precision, order=self.grid.order,
+                    mesh=_mesh_hash(self.grid))
+        arrays = {k: v.detach().cpu().numpy() for k, v in self.state_arrays().items()}
+        arrays['b'] = self.b.detach().cpu().numpy()
+        savefile.write(path, meta, arrays)
+
+    @classmethod
+    def load(cls, path):
+        """A world rebuilt from its settings with its saved state copied into the new world's own buffers (so the
+        graphs it captures later use them); it continues as the saved world would have."""
+        from . import savefile
+        meta, arrays = savefile.read(path)
+        if meta.get('kind') != 'insilica-world-v2':
+            raise ValueError('not a world save file')
+        layered = meta['preset'] not in ('coupled_stack', 'held_suarez') + AQUA
+        w = cls(meta['n'], meta['preset'], params=meta['params'], precision=meta['precision'],
+                state=(arrays['h'], arrays['b'], arrays['u']) if layered else None)
+        if meta.get('mesh') != _mesh_hash(w.grid) or meta.get('order') != w.grid.order:
+            raise ValueError('the save file was made on another mesh or cell order')
+        for k, dst in w.state_arrays().items():
+            a = arrays.get(k)
+            if a is None or tuple(a.shape) != tuple(dst.shape) or a.dtype != np.float64:
+                raise ValueError(f'save file array {k}: expected {tuple(dst.shape)} float64')
+            dst.copy_(torch.as_tensor(a, device=w.dg.dev))
+        w.tick = w._tick0 = int(meta['tick'])
+        torch.cuda.synchronize()
+        return w
 
     def state_arrays(self):
         out = {}

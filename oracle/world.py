@@ -1,9 +1,10 @@
 """The oracle's world tick -- TEST INFRASTRUCTURE ONLY.
 
 Restates the tick order of the reference's World._tick (src/insilica/world.py:286-358) for the parts this build
-covers (the layered core, FCT transport, the air-sea stress with its heat, the land column, the hourly ocean step
-without its column mixing, the Held-Suarez column and the gray radiative air column, dry or moist, with or without clouds) and the
-Ocean.step order (src/insilica/ocean.py:238-252), on the CPU oracle. Built from a
+covers (the layered core, FCT transport, the air-sea stress with its heat, the land column, the hourly ocean step with
+its convective adjustment and, in the coupled worlds, the ocean's forcing from the air column and its daily mixing,
+the Held-Suarez column and the gray radiative air column, dry or moist, with or without clouds) and the Ocean.step
+order (src/insilica/ocean.py:226-295), on the CPU oracle. Built from a
 product World's settings and starting state, so that the two can be stepped side by side.
 """
 import numpy as np
@@ -66,11 +67,30 @@ class OracleWorld:
                             dz=np.asarray(s.soil['dz'], dtype=float) if s.soil is not None else None,
                             c_soil=s.soil['c'] if s.soil is not None else 0.0,
                             k_soil=s.soil['k'] if s.soil is not None else 0.0,
-                            Fm=np.zeros((s.L, og.E)), M1=np.zeros((s.L, og.N)))
+                            Fm=np.zeros((s.L, og.E)), M1=np.zeros((s.L, og.N)),
+                            eos=(s.cfg['alpha'], s.cfg['alpha2'], s.cfg['beta']))
+            self.sea_mix = bool(getattr(w, 'sea_mix', False))
+            if self.sea_mix:                       # the coupled worlds' mixed layer: its forcing sums, its settings
+                from insilica.ocean import ALPHA_REF
+                c = s.cfg
+                par = {k: c[k] for k in ('S_ref', 'hmin', 'hmax', 'dTmin', 'm_kt', 'n_kt', 'sw_R', 'sw_z1', 'sw_z2',
+                                         'kappa_d', 'dwf', 'ri_b', 'reach', 'h_ent', 'det_class', 'fw_local', 'alpha',
+                                         'alpha2', 'beta', 'kappa_h', 'mix_dt')}
+                par.update(grav=sc.grav, alpha_ref=ALPHA_REF, rho_w=c['rho'], cw=c['cw'])
+                for k in ('ice_prev', 'q_acc', 'sun_acc', 'fw_acc'):
+                    self.sea[k] = _np(getattr(s, k))
+                self.sea.update(par=par, mix_every=s.mix_every, taum=np.zeros(og.N), wx=np.zeros((s.L - 1, og.N)),
+                                qocean=np.zeros(og.N), qsw=np.zeros(og.N))
+                if par['ri_b'] > 0 or par['kappa_h'] > 0:
+                    raise NotImplementedError('the oracle world mixes without ri_b and kappa_h')
+                self.Ts = self.sea['rows'][0]      # a view: the mixed layer is the air's surface
 
     def _kind(self, tick):
+        """0 air only; 1 with the ocean's step; 5 with its step and the mixed layer's mixing."""
         if self.sea is None or (tick + 1) % self.ocean_every:
             return 0
+        if getattr(self, 'sea_mix', False) and (tick + 1) % (self.ocean_every * self.sea['mix_every']) == 0:
+            return 5
         return 1
 
     def step(self, k=1):
@@ -80,6 +100,8 @@ class OracleWorld:
         return self
 
     def _tick(self, kind):
+        """The reference's tick order (world.py:286-358): the layers and transport; the wind stress and its heat; the
+        column; the ground; the ocean's forcing; the exchange of the column's mass; the ocean's step and mixing."""
         og, L = self.og, self.L
         theta = self.rows[0:L] if self.air else None   # a view: the air's theta is the first L rows
         self.core.step(self.h, self.u, self.dt, self.nfast, self.Fm, theta=theta)
@@ -87,20 +109,13 @@ class OracleWorld:
         if self.moved:
             transport(og, L, self.M, self.M1, self.Fm, self.rows, beta=self.beta, order=3)
         self.M[...] = self.M1
-        if self.sea is not None:
-            s = self.sea
+        s = self.sea
+        mix = s is not None and getattr(self, 'sea_mix', False)
+        if s is not None:
             couple_stress(og, L, self.h, self.u, s['h'], s['u'], self.surface_drag, self.rho, self.dt, s['diss'],
                           s['tau_acc'], wall=_wall(self.score), land_drag=self.land_drag)
             vmix_heat(og, L, self.h, s['diss'], theta, grav=self.core.grav, cp=self.core.cpa, kappa=self.core.kap)
-            if s['Tg'] is not None and s['land'] is not None:
-                land_column(og, len(s['dz']), s['land'], s['rows'], s['cap'], s['Tg'], s['dz'], s['c_soil'],
-                            s['k_soil'], self.dt)
-            if kind:
-                self.score.step(s['h'], s['u'], s['dt'], s['nfast'], s['Fm'])
-                mass_update(og, s['L'], s['M'], s['Fm'], s['M1'])
-                transport(og, s['L'], s['M'], s['M1'], s['Fm'], s['rows'], order=3)
-                s['M'][...] = s['M1']
-                ocn_cap(og, s['h'], self.rho * self.cw, s['cap'], s['land'], self.c_land)
+        wx = o = None
         if self.physics is not None:
             from .column import air_column, cell_speed, hs_column
             ph = self.physics
@@ -113,15 +128,74 @@ class OracleWorld:
                 o = air_column(ph['par'], ph['mode'], np.array([self.time]), ph['pos'], uabs, self.h, self.rows[0:L],
                                self.Ts, self.dt, insol=ph['insol'], elev=ph['b'],
                                q=self.rows[L:2 * L] if moist else None, ice=self.ice,
-                               cw=self.rows[2 * L:3 * L] if cloudy else None)
-                th, wx, self.Ts = o['theta'], o['wx'], o['Ts']
+                               cw=self.rows[2 * L:3 * L] if cloudy else None, cap=s['cap'] if mix else None,
+                               ocean=mix)
+                th, wx = o['theta'], o['wx']
+                self.Ts[...] = o['Ts']                 # (in place: with an ocean, the mixed layer's row)
                 if moist:
                     self.rows[L:2 * L], self.ice = o['q'], o['ice']
                 if cloudy:
                     self.rows[2 * L:3 * L] = o['cw']
+                if mix:
+                    s['qocean'][...], s['qsw'][...] = o['qocean'], o['qsw']
             self.rows[0:L] = th
             self.time += self.dt
+        if s is not None:
+            if s['Tg'] is not None and s['land'] is not None:
+                land_column(og, len(s['dz']), s['land'], s['rows'], s['cap'], s['Tg'], s['dz'], s['c_soil'],
+                            s['k_soil'], self.dt)
+            if mix:
+                from . import ocean as oo
+                Lo, N, p = s['L'], og.N, s['par']
+                if self.ice is not None:
+                    oo.fresh(N, p, s['h'][0], s['rows'][Lo], o['evap'], o['rain'], self.ice, s['ice_prev'], self.dt,
+                             s['fw_acc'], land=s['land'], hw=s['h'][0], M=s['M'][0], area=og.arr['areaCell'])
+                oo.pen(N, Lo, p, s['h'], s['rows'][:Lo], s['qocean'], s['qsw'], self.dt, s['q_acc'], s['sun_acc'],
+                       land=s['land'])
+        if self.physics is not None:
             self.core.exchange(wx[:L - 1], self.h, self.u, self.M, self.psi, theta=self.rows[0:L])
+        if kind & 1:
+            from .ocean import overturn
+            Lo, N = s['L'], og.N
+            overturn(N, Lo, s['h'], s['rows'][:Lo], s['rows'][Lo:], self.score.grav, s['eos'][0], s['eos'][1],
+                     s['eos'][2], land=s['land'])
+            self.score.step(s['h'], s['u'], s['dt'], s['nfast'], s['Fm'])
+            mass_update(og, s['L'], s['M'], s['Fm'], s['M1'])
+            transport(og, s['L'], s['M'], s['M1'], s['Fm'], s['rows'], order=3)
+            s['M'][...] = s['M1']
+            ocn_cap(og, s['h'], self.rho * self.cw, s['cap'], s['land'], self.c_land)
+            if kind & 4:
+                self._mix()
+
+    def _mix(self):
+        """Ocean.step's mixing (ocean.py:257-295): the interval's forcing, the mixed layer's column step, salt with the
+        water, the exchange between the ocean's layers, the sums cleared."""
+        from . import ocean as oo
+        og, s = self.og, self.sea
+        Lo, N, p = s['L'], og.N, s['par']
+        dt = p['mix_dt']
+        g = og.host
+        tau = np.sqrt(s['tau_acc'] * (1.0 / dt))
+        s['tau_acc'][...] = tau
+        ce = g.cellEdge
+        ke = np.zeros(N)
+        for m in range(6):                            # (the cell speed of the rms stress, in table order)
+            e = ce[:, m]
+            ok = e >= 0
+            ee = np.where(ok, e, 0)
+            ke = ke + np.where(ok, g.le[ee] * g.de[ee] * 0.25 * tau[ee] * tau[ee], 0.0)
+        s['taum'][...] = np.sqrt(2 * ke / g.areaCell)
+        for k in ('q_acc', 'sun_acc', 'fw_acc'):
+            s[k] /= dt
+        T, S = s['rows'][:Lo], s['rows'][Lo:]
+        oo.column(N, Lo, p, s['h'], T, S, s['taum'], s['q_acc'], s['fw_acc'], s['sun_acc'], dt, s['wx'],
+                  land=s['land'])
+        seq = p['reach'] > 0
+        oo.xch_salt(N, Lo, s['wx'], s['h'], S, seq=seq)
+        self.score.exchange(s['wx'], s['h'], s['u'], s['M'], T, seq=seq)
+        for k in ('tau_acc', 'q_acc', 'sun_acc', 'fw_acc'):
+            s[k][...] = 0.0
+        ocn_cap(og, s['h'], self.rho * self.cw, s['cap'], s['land'], self.c_land)
 
     def state(self):
         out = dict(h=self.h, u=self.u, M=self.M, rows=self.rows)
@@ -132,9 +206,11 @@ class OracleWorld:
         if getattr(self, 'ice', None) is not None:
             out['ice'] = self.ice
         if self.sea is not None:
-            for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap'):
-                if self.sea[k] is not None:
+            for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap', 'ice_prev', 'tau_acc', 'q_acc', 'sun_acc', 'fw_acc'):
+                if self.sea.get(k) is not None:
                     out['sea.' + k] = self.sea[k]
+            if getattr(self, 'sea_mix', False):
+                out.pop('Ts', None)                    # (the mixed layer's row)
         return out
 
 
