@@ -73,11 +73,11 @@ def build_hip(verbose=False, force=True):
 
 
 def build_mesh():
-    """Compile the Lloyd loop (host C++, OpenMP) into insilica/_lib/libinsilica_mesh.so."""
+    """Compile the mesh builder (host C++, OpenMP) into insilica/_lib/libinsilica_mesh.so."""
     LIBDIR.mkdir(exist_ok=True)
     tmp = LIBDIR / 'libinsilica_mesh.so.tmp'
     _run(['g++', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(tmp),
-          str(CSRC / 'mesh_lloyd.cpp')])
+          str(CSRC / 'mesh.cpp')])
     os.replace(tmp, LIBDIR / 'libinsilica_mesh.so')
     return LIBDIR / 'libinsilica_mesh.so'
 

@@ -1,50 +1,244 @@
-"""The planet's mesh: icosahedral Voronoi cells smoothed to an SCVT, with TRiSK C-grid geometry, built once per level
-n and cached on disk as .npz (`$INSILICA_CACHE/grids`, default ~/.cache/insilica_amd/grids).
+"""The planet's mesh: the icosahedral Voronoi grid smoothed toward a spherical centroidal Voronoi tessellation, with
+the TRiSK C-grid tables, built by the native mesh library (csrc/mesh.cpp, libinsilica_mesh.so) and cached on disk as
+.npz (`$INSILICA_CACHE/grids`, default ~/.cache/insilica_amd/grids).
 
-Same surface as the reference's grid package (src/insilica/grid/__init__.py:39-127): `get(n)`, `make(n, order)`,
-`for_planet(g, radius, omega, grav)`, `planet_of(g)`, `Grid` with the reference's array names. Cells are renumbered
-along a Hilbert curve by default (order='hilbert'); 'v1' keeps the icosphere's numbering.
+The surface is the reference's grid package's (src/insilica/grid/__init__.py:39-127): `get(n)`, `make(n, order)`,
+`for_planet(g, radius, omega, grav)`, `planet_of(g)`, `Grid` carrying the reference's array names (cells `pos`, `lat`,
+`lon`, `nbr`, `deg`, `area`, `dist`, `edgeLen`, `corners`, ...; edges `eI`, `eJ`, `le`, `de`, ...; vertices `vPos`,
+`areaV`, ...), numbered and valued as the reference's generator numbers and values them (pinned by a level-3
+fixture of its arrays). Cells follow a Hilbert curve by default (order='hilbert'); 'v1' keeps the icosphere's order.
+
+The native stages do the topology, the Lloyd iteration and the algebra of every small triangle; the element-wise
+arcsin and arctan2 that turn chords and triple products into angles run here in NumPy (see csrc/mesh.cpp).
 """
+import ctypes
+import math
 import os
 import time
 from pathlib import Path
 
 import numpy as np
 
-from .mesh import CELL_KEYS, build, hilbert_perm, renumber
-from .trisk import geometry
-
-GRID_VERSION = 1
+GRID_VERSION = 2
 DEFAULT_ORDER = 'hilbert'
 CACHE_DIR = Path(os.environ.get('INSILICA_CACHE', Path.home() / '.cache' / 'insilica_amd'))
+LLOYD_TOL, LLOYD_CAP = 1e-11, 200000
+HILBERT_BITS = 21
+# what scales with the planet's radius (lengths, then areas)
 LENGTHS, AREAS = ('de', 'dist', 'edgeLen', 'le'), ('area', 'areaCell', 'areaV', 'kite', 'totalArea')
-BASE_KEYS = ('n', 'N', 'totalArea', 'lloydIters', 'lloydMoved', 'nbr') + CELL_KEYS
+
+_lib = None
+
+
+def _native():
+    global _lib
+    if _lib is None:
+        path = Path(__file__).resolve().parents[1] / '_lib' / 'libinsilica_mesh.so'
+        if not path.exists():
+            raise RuntimeError(f'{path} is missing: build the native libraries first (python __graft_entry__.py)')
+        L = ctypes.CDLL(str(path))
+        vp, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        p64 = ctypes.POINTER(ctypes.c_int64)
+        for name, args in {
+            'isw_mesh_sizes': [i32, p64, p64, p64, p64],
+            'isw_mesh_icosphere': [i32, vp, vp, vp, vp],
+            'isw_mesh_ring': [i64, vp, vp, vp, vp],
+            'isw_mesh_lloyd': [i64, vp, vp, vp, i32, d, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(d)],
+            'isw_mesh_cells': [i64, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            'isw_mesh_edges': [i64, vp, vp, vp, vp, vp, vp],
+            'isw_mesh_hilbert': [i64, vp, i32, vp],
+            'isw_mesh_trisk': [i64, i64, i64] + [vp] * 25,
+            'isw_mesh_weights': [i64, i64] + [vp] * 14,
+        }.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = ctypes.c_int, args
+        _lib = L
+    return _lib
+
+
+def _call(name, *args):
+    rc = getattr(_native(), name)(*[a.ctypes.data if isinstance(a, np.ndarray) else a for a in args])
+    if rc != 0:
+        raise RuntimeError(f'{name} failed ({rc})')
 
 
 class Grid:
-    """Host (NumPy) copy of one mesh level; attributes carry the reference's array names."""
+    """A mesh level's host arrays as attributes (the reference's names), plus n, N, E, NV and the cell order."""
 
     def __init__(self, arrays):
-        self._keys = sorted(arrays)
+        self._names = sorted(arrays)
         for k, v in arrays.items():
             setattr(self, k, v)
         self.n, self.N, self.E, self.NV = int(self.n), int(self.N), int(self.E), int(self.NV)
         self.order = str(arrays.get('order', 'v1'))
 
     def arrays(self):
-        return {k: getattr(self, k) for k in self._keys}
+        return {k: getattr(self, k) for k in self._names}
 
     def __repr__(self):
         return f'Grid(n={self.n}, N={self.N}, E={self.E}, NV={self.NV}, order={self.order})'
 
 
+def _base_vertices():
+    """The unit icosahedron's 12 vertices before normalisation: the poles and two rings of five at latitude
+    +-atan(1/2), the lower ring turned by half a step (NumPy's trigonometry, as the reference builds them)."""
+    lat = np.arctan(0.5)
+    up = [(np.cos(lat) * np.cos(k * 2 * np.pi / 5), np.cos(lat) * np.sin(k * 2 * np.pi / 5), np.sin(lat))
+          for k in range(5)]
+    lo = [(np.cos(lat) * np.cos((k + 0.5) * 2 * np.pi / 5), np.cos(lat) * np.sin((k + 0.5) * 2 * np.pi / 5),
+           -np.sin(lat)) for k in range(5)]
+    return np.ascontiguousarray([(0.0, 0.0, 1.0)] + up + lo + [(0.0, 0.0, -1.0)], dtype=np.float64)
+
+
+def _edges(nbr, deg, E):
+    e = {k: np.zeros(E, dt) for k, dt in (('eI', np.int32), ('eJ', np.int32), ('eKi', np.uint8), ('eKj', np.uint8))}
+    _call('isw_mesh_edges', len(deg), nbr, deg, e['eI'], e['eJ'], e['eKi'], e['eKj'])
+    return e
+
+
+def _icosahedral(n):
+    """The level-n mesh in the icosphere's own numbering: generators after Lloyd, rings, the Voronoi cells' corners,
+    distances, edge lengths and areas (m, m^2 on the Earth-sized sphere), the edge list."""
+    from ..constants import R_PLANET
+    N, E, NV, ne = (ctypes.c_int64() for _ in range(4))
+    _call('isw_mesh_sizes', n, ctypes.byref(N), ctypes.byref(E), ctypes.byref(NV), ctypes.byref(ne))
+    N, E = N.value, E.value
+    pos = np.zeros((N, 3))
+    adj, deg = np.zeros((N, 6), np.int32), np.zeros(N, np.uint8)
+    _call('isw_mesh_icosphere', n, _base_vertices(), pos, adj, deg)
+    ring = np.zeros((N, 6), np.int32)
+    _call('isw_mesh_ring', N, pos, adj, deg, ring)
+    it, mv = ctypes.c_int(0), ctypes.c_double(0.0)
+    _call('isw_mesh_lloyd', N, pos, ring, deg, LLOYD_CAP, LLOYD_TOL, ctypes.byref(it), ctypes.byref(mv))
+    _call('isw_mesh_ring', N, pos, adj, deg, ring)
+    east, north = np.zeros((N, 3)), np.zeros((N, 3))
+    corners = np.zeros((N, 6, 3))
+    chord_d, chord_e, tri = np.zeros((N, 6)), np.zeros((N, 6)), np.zeros((N, 6, 2))
+    _call('isw_mesh_cells', N, pos, ring, deg, east, north, corners, chord_d, chord_e, tri)
+    live = np.arange(6)[None, :] < deg[:, None]
+    arc = lambda chord: R_PLANET * (2 * np.arcsin(np.minimum(1.0, 0.5 * chord)))
+    fan = 2 * np.arctan2(np.abs(tri[..., 0]), tri[..., 1])
+    area = np.zeros(N)
+    for k in range(6):                                   # each cell's fan in slot order
+        area = area + np.where(live[:, k], fan[:, k], 0.0)
+    area = area * R_PLANET * R_PLANET
+    g = dict(n=n, N=N, pos=pos, lat=np.arcsin(np.clip(pos[:, 2], -1, 1)), lon=np.arctan2(pos[:, 1], pos[:, 0]),
+             east=east, north=north, nbr=ring, deg=deg, corners=corners, area=area,
+             dist=np.where(live, arc(chord_d), 0.0), edgeLen=np.where(live, arc(chord_e), 0.0),
+             totalArea=float(np.sum(area)), lloydIters=it.value, lloydMoved=mv.value, E=E)
+    g.update(_edges(ring, deg, E))
+    return g
+
+
+CELL_ARRAYS = ('pos', 'lat', 'lon', 'area', 'east', 'north', 'deg', 'dist', 'edgeLen', 'corners')
+
+
+def _hilbert(g):
+    """The same mesh with its cells along the Hilbert curve (new cell k = old cell perm[k]); edges listed again."""
+    perm = np.zeros(g['N'], np.int32)
+    _call('isw_mesh_hilbert', g['N'], np.ascontiguousarray(g['pos']), HILBERT_BITS, perm)
+    back = np.empty_like(perm)
+    back[perm] = np.arange(len(perm), dtype=np.int32)
+    out = dict(g)
+    for k in CELL_ARRAYS:
+        out[k] = np.ascontiguousarray(g[k][perm])
+    nb = g['nbr'][perm]
+    out['nbr'] = np.ascontiguousarray(np.where(nb >= 0, back[np.maximum(nb, 0)], -1).astype(np.int32))
+    out.update(_edges(out['nbr'], out['deg'], g['E']))
+    out['perm'] = perm
+    return out
+
+
+def _tables(g):
+    """The TRiSK tables of a mesh in its current cell order (vertices, edge geometry, kites, areas, weights, the
+    ascending gather tables)."""
+    from ..constants import OMEGA, R_PLANET
+    N, E = g['N'], g['E']
+    NV, ne = ctypes.c_int64(), ctypes.c_int64()
+    _call('isw_mesh_sizes', g['n'], ctypes.byref(ctypes.c_int64()), ctypes.byref(ctypes.c_int64()),
+          ctypes.byref(NV), ctypes.byref(ne))
+    NV = NV.value
+    t = dict(NV=NV, vPos=np.zeros((NV, 3)), cornerV=np.zeros((N, 6), np.int32), cellEdge=np.zeros((N, 6), np.int32),
+             cellSign=np.zeros((N, 6), np.int8), ev1=np.zeros(E, np.int32), ev2=np.zeros(E, np.int32),
+             xe=np.zeros((E, 3)), ne=np.zeros((E, 3)), xm=np.zeros((E, 3)), circ1=np.zeros(E, np.int8),
+             circ2=np.zeros(E, np.int8), cellEdgeAsc=np.zeros((N, 6), np.int32), vEdge=np.zeros((NV, 3), np.int32),
+             vEdgeSign=np.zeros((NV, 3), np.int8), vCell=np.zeros((NV, 3), np.int32),
+             vSlot=np.zeros((NV, 3), np.int8))
+    kt = np.zeros((N, 6, 2, 2))
+    _call('isw_mesh_trisk', N, E, NV, g['pos'], g['nbr'], g['deg'], g['corners'], g['eI'], g['eJ'], g['eKi'],
+          g['eKj'], t['vPos'], t['cornerV'], t['cellEdge'], t['cellSign'], t['ev1'], t['ev2'], t['xe'], t['ne'],
+          t['xm'], kt, t['circ1'], t['circ2'], t['cellEdgeAsc'], t['vEdge'], t['vEdgeSign'], t['vCell'], t['vSlot'])
+    live = np.arange(6)[None, :] < g['deg'][:, None]
+    half = 2 * np.arctan2(np.abs(kt[..., 0]), kt[..., 1])
+    kite = np.where(live, (half[..., 0] + half[..., 1]) * R_PLANET * R_PLANET, 0.0)
+    area = np.zeros(N)
+    for k in range(6):
+        area = area + kite[:, k]
+    area_v = np.zeros(NV)
+    np.add.at(area_v, t['cornerV'][live], kite[live])   # (sequential, in (cell, slot) order)
+    eI, eKi = g['eI'].astype(np.int64), g['eKi'].astype(np.int64)
+    de, le = np.ascontiguousarray(g['dist'][eI, eKi]), np.ascontiguousarray(g['edgeLen'][eI, eKi])
+    start, eoe, w = np.zeros(E + 1, np.int32), np.zeros(ne.value, np.int32), np.zeros(ne.value)
+    _call('isw_mesh_weights', N, E, g['deg'], g['eI'], g['eJ'], g['eKi'], g['eKj'], t['cellEdge'], t['cellSign'],
+          np.ascontiguousarray(kite), area, le, de, start, eoe, w)
+    t.update(kite=kite, areaCell=area, areaV=area_v, de=de, le=le, eoeStart=start, eoe=eoe, w=w,
+             fV=2 * OMEGA * t['vPos'][:, 2])
+    return t
+
+
+def make(n, order=None):
+    """Build level n from scratch. Returns (Grid, seconds)."""
+    order = DEFAULT_ORDER if order is None else order
+    if order not in ('v1', 'hilbert'):
+        raise ValueError(f'unknown cell order {order!r}')
+    t0 = time.perf_counter()
+    g = _icosahedral(n)
+    if order == 'hilbert':
+        g = _hilbert(g)
+    g.update(_tables(g))
+    g['order'] = order
+    return Grid(g), time.perf_counter() - t0
+
+
+_loaded = {}
+
+
+def get(n, use_cache=True, order=None):
+    """Level n from memory, the disk cache, or built (and then cached)."""
+    order = DEFAULT_ORDER if order is None else order
+    key = (n, order)
+    if key in _loaded:
+        return _loaded[key]
+    path = CACHE_DIR / 'grids' / f'n{n}-{order}-g{GRID_VERSION}.npz'
+    g = None
+    if use_cache and path.exists():
+        try:
+            with np.load(path, allow_pickle=False) as z:
+                g = Grid({k: (z[k][()] if z[k].ndim == 0 else z[k]) for k in z.files})
+        except (OSError, ValueError):
+            g = None
+    if g is None:
+        g, _ = make(n, order)
+        if use_cache:
+            try:
+                path.parent.mkdir(parents=True, exist_ok=True)
+                tmp = path.with_suffix(f'.{os.getpid()}.tmp.npz')
+                np.savez(tmp, **g.arrays())
+                os.replace(tmp, path)
+            except OSError:
+                pass
+    _loaded[key] = g
+    return g
+
+
 def for_planet(g, radius, omega, grav):
-    """The grid scaled to a planet of this radius (m) and rotation (rad/s); carries radius, omega, grav. Earth's
-    returns g itself (reference: grid/__init__.py:39-53)."""
+    """The mesh scaled to a planet of this radius (m) and rotation (rad/s), carrying radius, omega and grav; Earth's
+    own settings return g itself (reference: grid/__init__.py:39-53)."""
     from ..constants import GRAV, OMEGA, R_PLANET
-    if radius == R_PLANET and omega == OMEGA and grav == GRAV:
+    if (radius, omega, grav) == (R_PLANET, OMEGA, GRAV):
         return g
-    a, s = g.arrays(), radius / R_PLANET
+    a = g.arrays()
+    s = radius / R_PLANET
     for k in LENGTHS:
         a[k] = a[k] * s
     for k in AREAS:
@@ -55,85 +249,32 @@ def for_planet(g, radius, omega, grav):
 
 
 def planet_of(g):
+    """(radius, rotation, gravity) the mesh was scaled for."""
     from ..constants import GRAV, OMEGA, R_PLANET
     return getattr(g, 'radius', R_PLANET), getattr(g, 'omega', OMEGA), getattr(g, 'grav', GRAV)
 
 
-def make(n, order=None):
-    """Build level n from scratch. Returns (Grid, seconds)."""
-    order = DEFAULT_ORDER if order is None else order
-    t0 = time.perf_counter()
-    g = build(n)
-    if order == 'hilbert':
-        g = renumber(g, hilbert_perm(g['pos']))
-    elif order != 'v1':
-        raise ValueError(f'unknown cell order {order!r}')
-    g.update(geometry(g))
-    g['order'] = order
-    return Grid(g), time.perf_counter() - t0
-
-
-_mem = {}
-
-
-def get(n, use_cache=True, order=None):
-    """Level n from memory, the disk cache, or built (and cached)."""
-    order = DEFAULT_ORDER if order is None else order
-    key = (n, order)
-    if key in _mem:
-        return _mem[key]
-    f = CACHE_DIR / 'grids' / f'n{n}-{order}-g{GRID_VERSION}.npz'
-    g = None
-    if use_cache and f.exists():
-        try:
-            with np.load(f, allow_pickle=False) as z:
-                g = Grid({k: (z[k][()] if z[k].ndim == 0 else z[k]) for k in z.files})
-        except (OSError, ValueError):
-            g = None
-    if g is None:
-        if order != 'v1':
-            base = get(n, use_cache, 'v1')
-            b = {k: getattr(base, k) for k in BASE_KEYS}
-            b.update({k: getattr(base, k) for k in ('E', 'eI', 'eJ', 'eKi', 'eKj')})
-            a = renumber(b, hilbert_perm(base.pos))
-            a.update(geometry(a))
-            a['order'] = order
-            g = Grid(a)
-        else:
-            g, _ = make(n, order='v1')
-        if use_cache:
-            try:
-                f.parent.mkdir(parents=True, exist_ok=True)
-                tmp = f.with_suffix(f'.{os.getpid()}.tmp.npz')
-                np.savez(tmp, **g.arrays())
-                os.replace(tmp, f)
-            except OSError:
-                pass
-    _mem[key] = g
-    return g
-
-
 def d2_stencil(g):
-    """Per edge end, weights on four cells (c, j, o1, o2) for the second derivative along the line to the neighbour,
-    scaled by d_e^2/12, for the 3rd-order transport flux (reference: src/insilica/gpu.py:86-108). o1, o2 are the
-    cells opposite the neighbour (one on a hexagon, the two across on a pentagon). Returns idx int32 (E*8), wt (E*8)."""
-    E = g.E
+    """The 3rd-order transport flux's curvature stencil per edge end (reference: src/insilica/gpu.py:86-108): for
+    the side cell c of an edge and its neighbour j across it, the second derivative along the line c-j from c, j and
+    the cell(s) opposite j on c's ring (one on a hexagon, the mean of the two across on a pentagon), times d_e^2 / 12.
+    Returns idx int32 (E * 8) and wt (E * 8): per edge, side eI then eJ, cells (c, j, o1, o2)."""
     nbr, deg, dist = g.nbr.astype(np.int64), g.deg.astype(np.int64), g.dist
-    idx = np.zeros((E, 2, 4), dtype=np.int32)
-    wt = np.zeros((E, 2, 4))
-    for side, (c, k) in enumerate(((g.eI.astype(np.int64), g.eKi.astype(np.int64)),
-                                   (g.eJ.astype(np.int64), g.eKj.astype(np.int64)))):
+    idx = np.zeros((g.E, 2, 4), dtype=np.int32)
+    wt = np.zeros((g.E, 2, 4))
+    sides = ((g.eI.astype(np.int64), g.eKi.astype(np.int64)), (g.eJ.astype(np.int64), g.eKj.astype(np.int64)))
+    for side, (c, k) in enumerate(sides):
         m = deg[c]
-        j = nbr[c, k]
-        d1 = dist[c, k]
-        hexa = m == 6
-        ka = np.where(hexa, (k + 3) % 6, (k + 2) % m)
-        kb = np.where(hexa, (k + 3) % 6, (k + 3) % m)
-        d2 = np.where(hexa, dist[c, (k + 3) % 6], 0.5 * (dist[c, ka] + dist[c, kb]))
-        s = 2 / (d1 + d2) * g.de * g.de / 12
-        idx[:, side] = np.stack([c, j, nbr[c, ka], nbr[c, kb]], axis=1)
-        wt[:, side, 0] = -s * (1 / d1 + 1 / d2)
-        wt[:, side, 1] = s / d1
-        wt[:, side, 2] = 0.5 * s / d2
-        wt[:, side, 3] = 0.5 * s / d2
+        hexagon = m == 6
+        across = (k + 3) % 6
+        ka = np.where(hexagon, across, (k + 2) % m)
+        kb = np.where(hexagon, across, (k + 3) % m)
+        near = dist[c, k]
+        far = np.where(hexagon, dist[c, across % 6], 0.5 * (dist[c, ka] + dist[c, kb]))
+        s = 2 / (near + far) * g.de * g.de / 12
+        idx[:, side] = np.stack([c, nbr[c, k], nbr[c, ka], nbr[c, kb]], axis=1)
+        wt[:, side, 0] = -s * (1 / near + 1 / far)
+        wt[:, side, 1] = s / near
+        wt[:, side, 2] = 0.5 * s / far
+        wt[:, side, 3] = 0.5 * s / far
     return idx.reshape(-1), wt.reshape(-1)
