@@ -911,6 +911,22 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
   lt[3] = cape;
 }
 
+/* plume_ascent on its own, for the tests' closed-form checks of its options: out = (A, kd, thd, qd, G, FZ, lt[0..3]),
+ * qx (L) what its mixing gave each layer per kg. */
+void og_plume_ascent(int L, const double* par, const og_air* air, const double* mu, const double* th, const double* q,
+                     double th0, double q0, double pi_, int shal, int mix, double* out, double* qx) {
+  double A = 0, thd = 0, qd = 0, G = 1, FZ = 0, lt[4] = {0, 0, 0, 0};
+  int kd = -1;
+  plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thd, &qd, q, qx, lt, &G, &FZ, shal);
+  out[0] = A;
+  out[1] = kd;
+  out[2] = thd;
+  out[3] = qd;
+  out[4] = G;
+  out[5] = FZ;
+  for (int m = 0; m < 4; m++) out[6 + m] = lt[m];
+}
+
 /* storm_flashes (air.cu:718-729): lightning from one storm's physics (Deierling et al. 2008): 9.0e-15 f_p f_np
  * flashes per minute, f_p the falling graupel's mass flux and f_np the rising small ice's, the plume's ice flux
  * shared among storms of updraft area storm_area rising at half sqrt(2 CAPE); only an updraft faster than graupel

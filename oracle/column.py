@@ -98,3 +98,15 @@ def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None,
         if cap is not None:
             o['cap'] = cap
     return o
+
+
+def plume_ascent(par, mu, theta, q, th0, q0, pi_, shal=False, mix=False, air=None):
+    """air.cu plume_ascent alone for one column (mu, theta, q: (L,)): a dict of A (buoyancy by mass), kd (the layer it
+    detrains into, -1 if none), thd, qd (its theta and vapor there), G (its mass per kg), FZ (frozen condensate), lt
+    (the lightning inputs) and qx (L) what its mixing gave each layer per kg."""
+    par, mu, theta, q = _d(par), _d(mu), _d(theta), _d(q)
+    L = mu.size
+    out, qx = np.zeros(10), np.zeros(L)
+    lib().og_plume_ascent(L, _p(par), ctypes.byref(air_struct(air)), _p(mu), _p(theta), _p(q), float(th0), float(q0),
+                          float(pi_), int(bool(shal)), int(bool(mix)), _p(out), _p(qx))
+    return dict(A=out[0], kd=int(out[1]), thd=out[2], qd=out[3], G=out[4], FZ=out[5], lt=out[6:10], qx=qx)

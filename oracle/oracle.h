@@ -116,6 +116,10 @@ void og_gas_src(int N, int L, int G, const double* src, const double* h, double 
 /* air_column (air.cu:876-1972): the dry column, and the moist one over the sea without land or the mixed layer; qv
  * (L, N) and ice (N) read and written when moist, the cloud water cloudw (L, N) with cloud > 0 (cloudo ((4L + 3), N)
  * the clouds' cover, effects and optical depths, when given). mix: the reference's mixed build's float32 arithmetic. */
+/* The deep (shal = 0) or shallow (shal = 1) plume's ascent alone (column.c plume_ascent, air.cu:540-666): out = (A,
+ * kd, thd, qd, G, FZ, lt[0..3]), qx (L) what its mixing gave each layer per kg. For the tests' closed-form checks. */
+void og_plume_ascent(int L, const double* par, const og_air* air, const double* mu, const double* th, const double* q,
+                     double th0, double q0, double pi_, int shal, int mix, double* out, double* qx);
 void og_air_column(int N, int L, const double* par, const og_air* air, int mode, const double* t, const double* insol,
                    const double* pos, const double* uabs, const double* mu, double* theta, double* qv, double* Ts,
                    double* ice, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
