@@ -11,7 +11,8 @@
  *   - index tables are int32, padded with -1 to width 6 per cell and 3 per vertex; signs are int8;
  *   - every pointer is a device pointer; optional arrays are NULL, as the reference passes NULL;
  *   - nothing allocates and nothing synchronises: every buffer comes in, all work goes on `stream` (a hipStream_t;
- *     NULL is the default stream), so a caller may capture any sequence of calls in a hipGraph;
+ *     NULL is the default stream), so a caller may capture any sequence of calls in a hipGraph (the one exception is
+ *     the mesh build's isw_mesh_lloyd_device, which reads its convergence back once per chunk of iterations);
  *   - every entry point returns 0, a hipError_t code (launch failure), or ISW_EBADARG for sizes or options it does not
  *     support (checked on the host before anything is launched).
  *
@@ -416,6 +417,18 @@ int32_t isw_advance_time(double* t, double dt, void* stream);
  * it creates events, not device memory. Not thread-safe; not for use while a graph is being captured. */
 int32_t isw_timing_start(void);
 int32_t isw_timing_stop(char* out, int32_t cap);
+
+/* Lloyd's iteration toward the spherical centroidal Voronoi tessellation on the device (reference: grid/icosa.py:194-224
+ * lloyd_scvt, which grid/__init__.py:95-99 runs on the GPU from level 6): pos (N, 3, device) the unit generators,
+ * updated in place; ring (N, 6) int32 each cell's neighbours counter-clockwise (entries below deg[i] in [0, N)), deg (N)
+ * uint8 (5 or 6); generators 0 and 11 stay. Iterates until the largest move (radians) < tol or max_iters; *iters, *moved
+ * (host) the iterations done and the last largest move. work (device): at least 3 N + chunk doubles (the second iterate
+ * and the chunk's moves). Launches chunk iterations at a time and reads their moves back (so it synchronises the
+ * stream once per chunk); 1 <= chunk <= ISW_LLOYD_MAX_CHUNK. The fan triangles' angles use a correctly rounded
+ * small-argument arctangent, so the result agrees with the host builder's to rounding. */
+#define ISW_LLOYD_MAX_CHUNK 4096
+int32_t isw_mesh_lloyd_device(int64_t N, double* pos, const int32_t* ring, const uint8_t* deg, int32_t max_iters, double tol,
+                       double* work, int32_t chunk, int32_t* iters, double* moved, void* stream);
 
 /* Library version and the gfx target it was built for. */
 const char* isw_version(void);

@@ -10,6 +10,7 @@ LIB_PATH = Path(__file__).resolve().parent / '_lib' / 'libinsilica_hip.so'
 ISW_FP64, ISW_MIXED, ISW_EBADARG, LMAX, KSOIL = 0, 1, 9001, 8, 16
 ISW_PAIR_THREADS = 256            # the most edges a patch may own for the two-sub-step kernel (one thread each)
 ISW_DSUM_NOALLOC_MAX_N = 1 << 25  # the most values isw_dsum_noalloc sums (its static workspace's limit)
+ISW_LLOYD_MAX_CHUNK = 4096        # the most Lloyd iterations isw_mesh_lloyd queues between reads of their moves
 
 _i32p, _i8p = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int8)
 _f64p, _f32p = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
@@ -111,6 +112,7 @@ def lib():
             'isw_dsum': [vp, vp, i64, vp, vp, vp],
             'isw_csum': [vp, vp, i64, vp, vp, vp],
             'isw_dsum_noalloc': [vp, vp, i64, vp, vp],
+            'isw_mesh_lloyd_device': [i64, vp, vp, vp, i32, d, vp, i32, vp, vp, vp],
             'isw_fill': [vp, i64, d, vp],
             'isw_lw_setup': [i32, i32, vp, vp, vp, d, d, d, vp, vp, vp, vp, vp, d, i32, vp, i32, vp],
             'isw_lw_column': [i32, i32, vp, vp, vp, vp, vp, i32, vp],
@@ -149,7 +151,7 @@ EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_
            'isw_hs_column', 'isw_surface_layer', 'isw_cell_speed', 'isw_cell_speed_perot', 'isw_gas_src',
            'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version',
            'isw_grid_check', 'isw_air_column_options', 'isw_ocn_overturn', 'isw_ocn_forcing', 'isw_ocn_column',
-           'isw_ocn_xch_salt', 'isw_ocn_pen', 'isw_ocn_fresh', 'isw_ocn_diffuse')
+           'isw_ocn_xch_salt', 'isw_ocn_pen', 'isw_ocn_fresh', 'isw_ocn_diffuse', 'isw_mesh_lloyd_device')
 
 
 def check(rc, what):

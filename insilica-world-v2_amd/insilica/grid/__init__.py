@@ -23,6 +23,8 @@ GRID_VERSION = 2
 DEFAULT_ORDER = 'hilbert'
 CACHE_DIR = Path(os.environ.get('INSILICA_CACHE', Path.home() / '.cache' / 'insilica_amd'))
 LLOYD_TOL, LLOYD_CAP = 1e-11, 200000
+GPU_LLOYD_FROM_N = 6          # levels from which Lloyd runs on the GPU when one is present (as the reference's)
+LLOYD_CHUNK = 512             # device Lloyd iterations queued between reads of their moves
 HILBERT_BITS = 21
 # what scales with the planet's radius (lengths, then areas)
 LENGTHS, AREAS = ('de', 'dist', 'edgeLen', 'le'), ('area', 'areaCell', 'areaV', 'kite', 'totalArea')
@@ -96,9 +98,38 @@ def _edges(nbr, deg, E):
     return e
 
 
-def _icosahedral(n):
-    """The level-n mesh in the icosphere's own numbering: generators after Lloyd, rings, the Voronoi cells' corners,
-    distances, edge lengths and areas (m, m^2 on the Earth-sized sphere), the edge list."""
+def _device_lloyd(pos, ring, deg):
+    """Lloyd's iteration in the HIP library (isw_mesh_lloyd_device) on the current GPU: (pos, iterations, last move)."""
+    import torch
+    from .. import _hip
+    N = len(pos)
+    dev = torch.device('cuda', torch.cuda.current_device())
+    P = torch.from_numpy(np.ascontiguousarray(pos)).to(dev)
+    R = torch.from_numpy(np.ascontiguousarray(ring)).to(dev)
+    D = torch.from_numpy(np.ascontiguousarray(deg)).to(dev)
+    W = torch.empty(3 * N + LLOYD_CHUNK, dtype=torch.float64, device=dev)
+    it, mv = ctypes.c_int32(0), ctypes.c_double(0.0)
+    _hip.check(_hip.lib().isw_mesh_lloyd_device(N, P.data_ptr(), R.data_ptr(), D.data_ptr(), LLOYD_CAP, LLOYD_TOL,
+                                         W.data_ptr(), LLOYD_CHUNK, ctypes.byref(it), ctypes.byref(mv),
+                                         torch.cuda.current_stream(dev).cuda_stream), 'isw_mesh_lloyd_device')
+    return np.ascontiguousarray(P.cpu().numpy()), it.value, mv.value
+
+
+def _lloyd_on(n, lloyd):
+    if lloyd is None:
+        if n < GPU_LLOYD_FROM_N:
+            return 'host'
+        import torch
+        return 'device' if torch.cuda.is_available() else 'host'
+    if lloyd not in ('host', 'device'):
+        raise ValueError(f'unknown Lloyd placement {lloyd!r}')
+    return lloyd
+
+
+def _icosahedral(n, lloyd='host'):
+    """The level-n mesh in the icosphere's own numbering: generators after Lloyd (on the host, or with
+    lloyd='device' on the GPU), rings, the Voronoi cells' corners, distances, edge lengths and areas (m, m^2 on the
+    Earth-sized sphere), the edge list."""
     from ..constants import R_PLANET
     N, E, NV, ne = (ctypes.c_int64() for _ in range(4))
     _call('isw_mesh_sizes', n, ctypes.byref(N), ctypes.byref(E), ctypes.byref(NV), ctypes.byref(ne))
@@ -108,8 +139,12 @@ def _icosahedral(n):
     _call('isw_mesh_icosphere', n, _base_vertices(), pos, adj, deg)
     ring = np.zeros((N, 6), np.int32)
     _call('isw_mesh_ring', N, pos, adj, deg, ring)
-    it, mv = ctypes.c_int(0), ctypes.c_double(0.0)
-    _call('isw_mesh_lloyd', N, pos, ring, deg, LLOYD_CAP, LLOYD_TOL, ctypes.byref(it), ctypes.byref(mv))
+    if lloyd == 'device':
+        pos, it, mv = _device_lloyd(pos, ring, deg)
+    else:
+        it, mv = ctypes.c_int(0), ctypes.c_double(0.0)
+        _call('isw_mesh_lloyd', N, pos, ring, deg, LLOYD_CAP, LLOYD_TOL, ctypes.byref(it), ctypes.byref(mv))
+        it, mv = it.value, mv.value
     _call('isw_mesh_ring', N, pos, adj, deg, ring)
     east, north = np.zeros((N, 3)), np.zeros((N, 3))
     corners = np.zeros((N, 6, 3))
@@ -125,7 +160,7 @@ def _icosahedral(n):
     g = dict(n=n, N=N, pos=pos, lat=np.arcsin(np.clip(pos[:, 2], -1, 1)), lon=np.arctan2(pos[:, 1], pos[:, 0]),
              east=east, north=north, nbr=ring, deg=deg, corners=corners, area=area,
              dist=np.where(live, arc(chord_d), 0.0), edgeLen=np.where(live, arc(chord_e), 0.0),
-             totalArea=float(np.sum(area)), lloydIters=it.value, lloydMoved=mv.value, E=E)
+             totalArea=float(np.sum(area)), lloydIters=it, lloydMoved=mv, lloydOn=lloyd, E=E)
     g.update(_edges(ring, deg, E))
     return g
 
@@ -186,13 +221,15 @@ def _tables(g):
     return t
 
 
-def make(n, order=None):
-    """Build level n from scratch. Returns (Grid, seconds)."""
+def make(n, order=None, lloyd=None):
+    """Build level n from scratch. Returns (Grid, seconds). lloyd: 'host', 'device' (the GPU's), or None for the
+    GPU from level GPU_LLOYD_FROM_N when one is present (the reference's rule; the two agree to rounding)."""
     order = DEFAULT_ORDER if order is None else order
     if order not in ('v1', 'hilbert'):
         raise ValueError(f'unknown cell order {order!r}')
+    lloyd = _lloyd_on(n, lloyd)
     t0 = time.perf_counter()
-    g = _icosahedral(n)
+    g = _icosahedral(n, lloyd)
     if order == 'hilbert':
         g = _hilbert(g)
     g.update(_tables(g))

@@ -38,6 +38,14 @@ def test_bad_arguments_rejected_on_host():
     assert lib.isw_dsum_noalloc(one, None, 0, one, None) == E
     assert lib.isw_dsum_noalloc(None, None, 10, one, None) == E
     assert lib.isw_dsum_noalloc(one, None, 10, None, None) == E
+    # the device Lloyd refuses too few cells, a chunk beyond its limit and a tolerance that never stops, before any HIP call
+    it, mv = ctypes.c_int32(), ctypes.c_double()
+    ok = dict(N=642, max_iters=10, tol=1e-11, chunk=64)
+    for bad in (dict(N=11), dict(chunk=0), dict(chunk=_hip.ISW_LLOYD_MAX_CHUNK + 1), dict(tol=0.0), dict(max_iters=0)):
+        a = dict(ok, **bad)
+        assert lib.isw_mesh_lloyd_device(a['N'], one, one, one, a['max_iters'], a['tol'], one, a['chunk'],
+                                         ctypes.byref(it), ctypes.byref(mv), None) == E, bad
+    assert lib.isw_mesh_lloyd_device(642, one, one, one, 10, 1e-11, one, 64, None, ctypes.byref(mv), None) == E
     assert lib.isw_fill(None, 5, 0.0, None) == E
     assert lib.isw_land_column(None, 3, None, None, None, None, None, 1.0, 1.0, 1.0, None) == E
     # a grid struct with sizes but a layer count beyond the maximum, and a transport with rows not a multiple of L
