@@ -51,3 +51,43 @@ def test_budgets(precision):
     # the air's mass per layer sum h A follows M to rounding
     for k in range(3):
         assert abs(t1[f'volume{k}'] - t1[f'M{k}']) / t1[f'M{k}'] <= 1e-13
+
+
+def test_accuracy_contract_level6_30_days():
+    """The mixed form's contract on the benchmark's own world (coupled_stack at level 6, nfast 15): after 30 model
+    days (5,760 ticks) the relative L2 error of each air layer's thickness against the FP64 form is <= 1e-5, and in
+    both forms every per-layer mass and every conserved tracer's total has drifted by <= 1e-13 relative. The other
+    fields' errors are reported beside it."""
+    from insilica.world import World
+    print('case: mixed vs fp64 coupled_stack n=6, 30 days', flush=True)
+    a = World(6, 'coupled_stack', precision='double', params=dict(nfast=15))
+    b = World(6, 'coupled_stack', precision='mixed', params=dict(nfast=15))
+    keys = ([f'M{k}' for k in range(3)] + [f'ocean_M{k}' for k in range(2)] + [f'tracer{y}' for y in range(3, 15)]
+            + [f'ocean_tracer{y}' for y in (2, 3)])
+    t0 = {w.dg.precision: w.totals() for w in (a, b)}
+    ticks = int(round(30 * 86400 / a.dt))
+    assert ticks == 5760
+    for day in range(30):
+        a.step(ticks // 30)
+        b.step(ticks // 30)
+        if day in (0, 9, 19, 29):
+            ha, hb = a.h.cpu().numpy(), b.h.cpu().numpy()
+            print(f'day {day + 1}: h L2 per air layer',
+                  [f'{_l2rel(a.grid.areaCell, hb[k], ha[k]):.2e}' for k in range(3)], flush=True)
+    A = a.grid.areaCell
+    sa, sb = a.state_arrays(), b.state_arrays()
+    report = {}
+    for name, x in sa.items():
+        xa, xb = x.detach().cpu().numpy(), sb[name].detach().cpu().numpy()
+        if xa.ndim == 2 and xa.shape[1] == a.grid.N:
+            report[name] = [_l2rel(A, xb[k], xa[k]) if np.any(xa[k]) else 0.0 for k in range(xa.shape[0])]
+        else:
+            report[name] = [float(np.sqrt(np.sum((xb - xa) ** 2) / max(np.sum(xa ** 2), 1e-300)))]
+    for name, e in report.items():
+        print(f'  {name:12s} relative L2 after 30 days: ' + ' '.join(f'{v:.2e}' for v in e), flush=True)
+    assert max(report['h']) <= 1e-5
+    for w in (a, b):
+        t1, s = w.totals(), t0[w.dg.precision]
+        worst = max(abs(t1[k] - s[k]) / abs(s[k]) for k in keys)
+        print(f'{w.dg.precision}: worst budget drift over 30 days {worst:.2e}', flush=True)
+        assert worst <= 1e-13

@@ -38,3 +38,15 @@ def test_budget_queues_requests():
     done = sorted(done, key=lambda r: r.admitted)
     for a, b in zip(done, done[1:]):
         assert b.admitted >= a.finished
+
+
+def test_levels_5_to_7_same_alone():
+    """Requests at levels 5, 6 and 7 served together give the same states as each run alone."""
+    from insilica.serve import Request, Server, draw_requests
+    print('case: serve 4 requests, levels 5-7', flush=True)
+    planets = draw_requests(4, seed=11, levels=(5, 6, 7))
+    reqs = [Request(k, 0.05 * k, lev, 1, dict(planets[k].params)) for k, lev in enumerate((5, 7, 6, 5))]
+    done, stats = Server(keep_hashes=True, lead_hours=2).run(reqs)
+    print(f'served {stats["model_hours"]} model hours at {stats["model_hours_per_s"]:.1f} model-hours/s', flush=True)
+    for r in done:
+        assert r.hash == _alone(r), (r.rid, r.level)
