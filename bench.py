@@ -180,10 +180,11 @@ def serve_main(a):
                 vs_baseline=None, dtype='f64 state, f32 stencils (mixed)', data='synthetic',
                 config=dict(workload=f'{len(done)} coupled_stack requests, Poisson arrivals 2/s (seed {a.seed}), levels '
                                      f'{a.serve_levels}, 1-30 model days, seeded planets; world construction inside the '
-                                     f'timed loop',
+                                     f'timed loop (host part on worker threads)',
                             parallelism='1 GPU, continuous admission, one stream and graph per world'),
                 day_p50_s=round(st['day_p50_s'], 4), day_p99_s=round(st['day_p99_s'], 4),
-                model_hours=st['model_hours'], seconds=round(st['seconds'], 3))
+                model_hours=st['model_hours'], seconds=round(st['seconds'], 3),
+                **{k: {q: round(v, 4) for q, v in st[k].items()} for k in ('queue_s', 'prep_s', 'build_s', 'compute_s')})
     print(json.dumps(line), flush=True)
 
 

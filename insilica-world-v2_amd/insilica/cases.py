@@ -228,13 +228,32 @@ _LEGENDRE = {}
 
 
 def _legendre(pos, lmax):
-    """The normalised associated Legendre functions P_lm(sin lat) at unit positions (cached per position set)."""
+    """The orthonormal associated Legendre functions P_lm(sin lat) (Condon-Shortley phase, so that P_lm(cos theta)
+    e^(i m lon) is the unit-norm spherical harmonic Y_l^m) at unit positions, cached per position set, by the standard
+    recurrences in l at fixed m: P_mm = -sqrt((2m + 1) / 2m) sqrt(1 - x^2) P_(m-1)(m-1) from P_00 = 1 / sqrt(4 pi),
+    P_(m+1)m = sqrt(2m + 3) x P_mm, P_lm = a_lm (x P_(l-1)m - b_lm P_(l-2)m) with a_lm = sqrt((4l^2 - 1) / (l^2 - m^2))
+    and b_lm = sqrt(((l - 1)^2 - m^2) / (4 (l - 1)^2 - 1)); x = sin lat."""
     key = (len(pos), lmax, pos[0].tobytes(), pos[-1].tobytes())
     if key not in _LEGENDRE:
-        from scipy.special import sph_harm_y
-        lat = np.arcsin(np.clip(pos[:, 2], -1, 1))
-        theta_c = math.pi / 2 - lat
-        P = {(l, m): np.real(sph_harm_y(l, m, theta_c, 0.0)) for l in range(1, lmax + 1) for m in range(0, l + 1)}
+        x = np.clip(pos[:, 2], -1.0, 1.0)
+        s = np.sqrt((1.0 - x) * (1.0 + x))
+        P, pmm = {}, np.full(len(pos), 1.0 / math.sqrt(4 * math.pi))
+        for m in range(0, lmax + 1):
+            if m > 0:
+                pmm = -math.sqrt((2 * m + 1) / (2.0 * m)) * s * pmm
+            prev2, prev = None, pmm
+            if m >= 1:
+                P[(m, m)] = pmm
+            if m + 1 <= lmax:
+                cur = math.sqrt(2 * m + 3) * x * pmm
+                P[(m + 1, m)] = cur
+                prev2, prev = pmm, cur
+            for l in range(m + 2, lmax + 1):
+                a = math.sqrt((4.0 * l * l - 1) / (l * l - m * m))
+                b = math.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1) ** 2 - 1))
+                cur = a * (x * prev - b * prev2)
+                P[(l, m)] = cur
+                prev2, prev = prev, cur
         if len(_LEGENDRE) > 8:
             _LEGENDRE.clear()
         _LEGENDRE[key] = (P, np.arctan2(pos[:, 1], pos[:, 0]))

@@ -125,6 +125,25 @@ def fast_patches2(grid, pt, max_owned=_hip.ISW_PAIR_THREADS):
     return out
 
 
+_PATCHES = {}
+
+
+def patch_tables(grid):
+    """fast_patches and fast_patches2 of a mesh, built once per mesh topology: meshes scaled to other planets share
+    their index arrays with the level's mesh, so they share these tables too. Host work only (safe to call from a
+    worker thread)."""
+    key = (id(grid.eI), int(grid.N), PATCH_CELLS, PATCH_PAIRS)
+    hit = _PATCHES.get(key)
+    if hit is None or hit[0] is not grid.eI:
+        pt = fast_patches(grid, PATCH_CELLS)
+        p2 = fast_patches2(grid, pt) if pt is not None and PATCH_PAIRS else None
+        hit = (grid.eI, pt, p2)
+        if len(_PATCHES) > 16:
+            _PATCHES.clear()
+        _PATCHES[key] = hit
+    return hit[1], hit[2]
+
+
 class DeviceGrid:
     """A Grid's arrays on the GPU: indices int32/int8, geometry float64, reciprocals and FP32 weight copies."""
 
@@ -163,14 +182,13 @@ class DeviceGrid:
             setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._f64p))
         for k in ('wf', 'd2wtf'):
             setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._f32p))
-        pt = fast_patches(grid, PATCH_CELLS)
+        pt, p2 = patch_tables(grid)
         if pt is not None:
             s.npatch, s.pcells, s.pslots = pt['npatch'], pt['pcells'], pt['pslots']
             s.pown = int(np.max(np.diff(pt['pe_off'])))
             for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):
                 setattr(self, k, put(pt[k], np.int32))
                 setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._i32p))
-            p2 = fast_patches2(grid, pt) if PATCH_PAIRS else None
             if p2 is not None:
                 s.pslots2, s.pedges2 = p2['pslots2'], p2['pedges2']
                 for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot'):

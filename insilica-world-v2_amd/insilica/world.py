@@ -62,17 +62,34 @@ def _mesh_hash(g):
 
 
 class World:
-    def __init__(self, n=4, preset='w2', params=None, precision=None, state=None, layers=None):
+    @staticmethod
+    def prepare(n=4, preset='w2', params=None):
+        """The host-side part of building World(n, preset, params): the planet's mesh, the fast sub-steps' patch
+        tables and, for 'coupled_stack', the seeded initial state. NumPy only (nothing touches the GPU), so a serving
+        loop can run it on a worker thread; World(n, preset, params, prepared=this) then only uploads and sets up."""
+        from .gpu import patch_tables
+        P = dict(params or {})
+        planet = dict(PLANET, **P.get('planet', {}))
+        g = gridmod.for_planet(gridmod.get(n), **planet)
+        patch_tables(g)
+        out = dict(n=n, preset=preset, params=P, grid=g)
+        if preset == 'coupled_stack':
+            out['built'] = cases.coupled_stack(g, seed=P.get('seed', 0), land_frac=P.get('land_frac', 0.3))
+        return out
+
+    def __init__(self, n=4, preset='w2', params=None, precision=None, state=None, layers=None, prepared=None):
         """A world on level n from case `preset` (params override its settings). layers=K splits a one-layer case
         into K equal layers with g' = 0 (they then move as one); state=(h, b, u) starts a layered case from given
-        arrays (reference: world.py:56-58)."""
+        arrays (reference: world.py:56-58). prepared: World.prepare(n, preset, params)'s result, reused."""
         self.n, self.preset, self.precision = n, preset, precision
         P = self.params = dict(params or {})
         if preset not in PRESETS:
             raise ValueError(f'unknown preset {preset!r}; this build has {PRESETS}')
         self.planet = dict(PLANET, **P.get('planet', {}))
         self.grav = self.planet['grav']
-        self.grid = gridmod.for_planet(gridmod.get(n), **self.planet)
+        if prepared is not None and (prepared['n'], prepared['preset'], prepared['params']) != (n, preset, P):
+            raise ValueError('prepared for another level, preset or settings')
+        self.grid = prepared['grid'] if prepared is not None else gridmod.for_planet(gridmod.get(n), **self.planet)
         self.dg = dg = DeviceGrid(self.grid, precision)
         g = self.grid
         self.air = preset in ('coupled_stack', 'held_suarez') + AQUA
@@ -170,7 +187,8 @@ class World:
             self.physics = AirPhysics(dg, 2, phys, forcing='held_suarez')
             self.time = dg.zeros(1)
         elif self.air:
-            built = cases.coupled_stack(g, seed=P.get('seed', 0), land_frac=P.get('land_frac', 0.3))
+            built = prepared['built'] if prepared is not None and 'built' in prepared else \
+                cases.coupled_stack(g, seed=P.get('seed', 0), land_frac=P.get('land_frac', 0.3))
             h, b, u = built['h'] * (GRAV / self.grav), built['b'], built['u']
             self.L = 3
             self.dt = float(P.get('dt', 900.0 * 2.0 ** (5 - n)))
