@@ -358,11 +358,14 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * (kg/m^2/s), tsurf; rainp (4, N: rain from the layers above the bottom one, from the bottom one, the plumes', snow),
  * qocean, qsw, flash, swalb and firn (the firn's temperature) when given; cap (N) the surface's heat capacity, else
  * c_slab. The argument list is the reference's; elev (N) with theta_rise > 0 raises the lowest interface over high
- * ground. Returns ISW_EBADARG for cu_dth > 0 with L > 3, a non-NULL dgo or xfer, and for moist air without qv or
- * ice, with land but without soilcap, soil, pond or snow, with cu_sub (the cumulus layer's subsidence), and with
- * plume_deep and cu_dth (the cumulus layer): those parts of the reference's column are not in this library. bl_q (the mixed layer's even vapor and its
+ * ground. Returns ISW_EBADARG for a non-NULL dgo (the diagnostics rows are not in this library), and for moist air
+ * without qv or ice, or with land but without soilcap, soil, pond or snow. bl_q (the mixed layer's even vapor and its
  * cloud statistics; ml_clear, cu_cloud), the deep plume's IFS entrainment and detrainment (ent_ifs, ent_loc, det_all),
- * virtual buoyancy (buoy_v) and the shallow plume (shallow, rooted by plume_root with root_q) as the reference's.
+ * virtual buoyancy (buoy_v), the shallow plume (shallow, rooted by plume_root with root_q) and the cumulus layer
+ * (cu_dth's floating edges with cu_dp and cl_float, the plumes' air taken in through it moved with them, cu_sub's
+ * subsidence with sub_mix; air.cu:668-713, 913-922, 1601-1700, 1748-1861) as the reference's. xfer (3 L, N,
+ * nullable): the gas tracers' mole fractions per layer, carried by every move of the column's air (the reference's
+ * -DGASX column, air.cu:503-507), updated in place.
  * prec: ISW_FP64, or ISW_MIXED (the reference's mixed build: 1 - e^-x, the edge sub-levels' Exner powers and the
  * saturation and condensation arithmetic in float32). */
 int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* air, int32_t mode, const double* t,

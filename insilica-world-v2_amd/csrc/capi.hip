@@ -49,7 +49,7 @@ int air_column(int, int, const double*, const isw_air*, int, const double*, cons
                const double*, const double*, double*, double*, double*, double*, double, double*, double*, double*,
                double*, double*, double*, double*, double*, double*, double*, double*, double*, const double*,
                const double*, double*, double*, double*, const double*, double*, const double*, const double*,
-               double*, const double*, const double*, double*, double*, double*, int, hipStream_t);
+               double*, const double*, const double*, double*, double*, double*, double*, int, hipStream_t);
 }  // namespace isw_col
 
 namespace isw_ocn {
@@ -123,16 +123,12 @@ bool sw_tables_ok(const isw_sw_tables* t) {
 }
 bool prec_ok(int32_t p) { return p == ISW_FP64 || p == ISW_MIXED; }
 
-// The air column's options (par, the reference's PAR_ORDER) that this library's column does not carry: the cumulus
-// layer's floating edges (cu_dth, par[111], which only acts above three layers) and, for moist air, the deep plume
-// with the cumulus layer, and the cumulus layer's subsidence (cu_sub, par[114]) where it acts: with the mixed layer
-// (bl_mix, par[70]) and its condensation level (bl_lcl, par[100]) in three or more layers (air.cu:1838).
+// The air column's options (par, the reference's PAR_ORDER) that this library's column does not carry: none now
+// (the cumulus layer's options cu_dth, cu_dp, cl_float, cu_sub and sub_mix are the reference's); kept as the one place
+// the entry point and isw_air_column_options ask.
 bool column_options_ok(int32_t L, const double* par) {
-  if (par[111] > 0 && L > 3) return false;
-  if (par[22] > 0) {
-    if (par[47] > 0 && par[111] > 0) return false;
-    if (par[114] > 0 && par[100] > 0 && par[70] > 0 && L >= 3) return false;
-  }
+  (void)L;
+  (void)par;
   return true;
 }
 
@@ -329,12 +325,12 @@ int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* a
       !Ts || !wx || !olr || !sw_abs || !stored || !rain || !evap || !conv || !tsurf || !(dt > 0) || !prec_ok(prec))
     return ISW_EBADARG;
   if (mode == 0 ? (!t || !pos) : !insol) return ISW_EBADARG;
-  if (!column_options_ok(L, par) || dgo || xfer) return ISW_EBADARG;
+  if (!column_options_ok(L, par) || dgo) return ISW_EBADARG;
   // moist air needs its vapor and ice; land (moist) its bucket's capacity, soil, ponds and snow
   if (par[22] > 0 && (!qv || !ice || (land && (!soilcap || !soil || !pond || !snow)))) return ISW_EBADARG;
   return isw_col::air_column(N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, wx, olr, sw_abs,
                              stored, rain, evap, conv, tsurf, rainp, cap, qocean, qsw, land, soilcap, soil, pond, snow,
-                             elev, flash, radf, swf, swalb, ch, lc, firn, cloudw, cloudo, prec == ISW_MIXED,
+                             elev, flash, radf, swf, swalb, ch, lc, firn, cloudw, cloudo, xfer, prec == ISW_MIXED,
                              S(stream));
 }
 

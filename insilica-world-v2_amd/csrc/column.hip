@@ -34,6 +34,9 @@ struct CP {
       q_sea, z0_land;
   double bl_room;                                      // (the mixed layer's room up to its condensation level)
   double ths[ISW_LMAX];
+  // the column's gas tracers (3 x L mole fractions, xfer) while it is stepped, else null: every move of its air
+  // carries them (air.cu:503-507)
+  double* xf;
 };
 
 static CP make_cp(const double* a) {
@@ -43,6 +46,7 @@ static CP make_cp(const double* a) {
   p.bulk_gust = a[75]; p.q_sea = a[97]; p.z0_land = a[91];
   p.bl_room = 1e30;
   for (int j = 0; j < ISW_LMAX; j++) p.ths[j] = a[132 + j];
+  p.xf = nullptr;
   return p;
 }
 
@@ -133,6 +137,7 @@ struct AP {
   double plume_deep, plume_ent, ent_cloud, ent_det, l_det, storm_area, graupel_v, ice_small, ent_ifs, ent_loc, det_all,
       buoy_v, shallow;
   double ml_clear, cu_cloud;                           // (the mixed layer's cloud with bl_q)
+  double cu_dth, cu_dp, cu_sub, cl_float, sub_mix;     // (the cumulus layer's)
 };
 
 static AP make_ap(const double* a, const isw_air* air) {
@@ -160,6 +165,7 @@ static AP make_ap(const double* a, const isw_air* air) {
   p.ent_ifs = a[109]; p.ent_loc = a[125]; p.det_all = a[116]; p.buoy_v = a[113]; p.shallow = a[110];
   p.storm_area = a[80]; p.graupel_v = a[81]; p.ice_small = a[82];
   p.ml_clear = a[118]; p.cu_cloud = a[119];
+  p.cu_dth = a[111]; p.cu_dp = a[112]; p.cu_sub = a[114]; p.cl_float = a[117]; p.sub_mix = a[131];
   return p;
 }
 
@@ -415,6 +421,18 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* q, double
   for (int k = 0; k < L; k++) {
     if (k == from) { mf = mu[k]; tf = th[k]; mu2[k] -= m; if constexpr (MOIST) qf = q[k]; }
     if (k == to) { mt = mu[k]; tt = th[k]; mu2[k] += m; if constexpr (MOIST) qt = q[k]; }
+  }
+  if (P.xf) {
+    const int d = m > 0 ? from : to, r = m > 0 ? to : from;
+    const double am = fabs(m);
+    double mur = 0, mur2 = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++)
+      if (k == r) {
+        mur = mu[k];
+        mur2 = mu2[k];
+      }
+    for (int g = 0; g < 3; g++) P.xf[g * L + r] = (mur * P.xf[g * L + r] + am * P.xf[g * L + d]) / mur2;
   }
   layer_exner<L>(mu2, P, pb2, pib2);
   double muf2 = 0, mut2 = 0;
@@ -897,14 +915,18 @@ __device__ __forceinline__ double liq_of(double T) {
 // entraining layers' entrainment and detrainment the IFS's, 1.75e-3 /m (1.3 - RH) (q_s / q_s at cloud base)^3 and
 // 0.75e-4 /m, in sub-levels of ~25 hPa (air.cu:578-584, 609-624); det_all: every tropospheric layer (below edge_k's
 // stratosphere) entraining; ent_loc: the air taken in holding the layer's vapor at its relative humidity level by
-// level (air.cu:586-598); buoy_v: buoyancy in virtual theta, theta (1 + 0.608 q) (air.cu:564-567). The layer loop is
-// unrolled with a stop flag, so every per-layer value stays in registers.
+// level (air.cu:586-598); buoy_v: buoyancy in virtual theta, theta (1 + 0.608 q) (air.cu:564-567); cu_dth (the
+// cumulus layer, air.cu:580-583, 609-611, 654): the cloud layer above the cumulus layer entraining too, each entraining
+// layer's first sub-level taking the rest of the last one below, the shallow plume going on into that cloud layer; Gl
+// (L): what it took in from each layer with ent_ifs, per kg leaving the bottom layer. The layer loop is unrolled with a
+// stop flag, so every per-layer value stays in registers.
 template <int L, bool MIX>
 __device__ void plume_ascent(const double* mu, const double* th, const double* q, double th0, double q0, double pi_,
                              const CP& P, const AP& A, double& Ab, int& kd, double& thd, double& qd, double* qx,
-                             double* lt, double& G, double& FZ, bool shal = false) {
+                             double* lt, double& G, double& FZ, double* Gl, bool shal = false) {
   constexpr int bot = L - 1, M = bot > 2 ? 3 : 8;
-  double Mp = 1, tp = th0, qp = q0, fz = 0;
+  const bool cu = A.cu_dth > 0;
+  double Mp = 1, tp = th0, qp = q0, fz = 0, gk[L], dzc = 0;
   auto lift = [&](double p_) {
     const double q_ = qp, ex_ = fpow_<MIX>(p_ / P.p0, P.kappa);
     condense<MIX>(tp, qp, P.cp * ex_, p_, P, A);
@@ -932,6 +954,8 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
   for (int k = 0; k < L; k++) {
     qa[k] = 0;
     qx[k] = 0;
+    gk[k] = 0;
+    Gl[k] = 0;
   }
   bool stop = false;
 #pragma unroll
@@ -939,7 +963,8 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
     if (stop) continue;
     const double a = tstar<L>(k, P), b = k > 0 ? tstar<L>(k - 1, P) : 0.0;
     const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !MIX ? expm1(s) : 0.0;
-    const bool ek = k > 0 && (k == bot - 1 || (A.det_all > 0 && ifs && !shal && k >= (int)A.edge_k));
+    const bool ek = k > 0 && (k == bot - 1 || (cu && ifs && k == bot - 2) ||
+                              (A.det_all > 0 && ifs && !shal && k >= (int)A.edge_k));
     const int Mk = ifs && ek ? min(48, max(M, (int)ceil(P.grav * mu[k] / 2500.0))) : M;
     const double f = A.plume_ent > 0 ? omexp<MIX>(A.plume_ent * P.grav * mu[k] / Mk) : 0.0;
     const bool loc = A.ent_loc > 0 && A.ent_cloud > 0 && ifs && ek && k >= (int)A.edge_k && q[k] > 0;
@@ -958,6 +983,8 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
       if (A.ent_cloud > 0 && ek) {
         double dz = A.rd * te * ex / P.grav * (P.grav * mu[k] / Mk) / pl;
         if (ifs && j == 0) dz *= 0.5;                  // (from cloud base to the first sub-level's middle)
+        if (cu && j == 0) dz += dzc;                   // (and the rest of the layer below's last sub-level)
+        if (cu && j == Mk - 1) dzc = 0.5 * A.rd * te * ex / P.grav * (P.grav * mu[k] / Mk) / pl;
         if (ifs) {
           const double qse = qsat_(te * ex, pl), qen = loc && qsl > 0 ? q[k] * qse / qsl : q[k];
           const double rh = fmin(1.0, fmax(0.0, qen / qse)), fs = fmin(1.0, qse / qsb);
@@ -967,6 +994,7 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
           if (loc) qa[k] -= Mp * expm1((eps_ - del_) * dz) * (qen - q[k]);   // (the growth taken in at qen)
           tp += fe_ * (te - tp);
           qp += fe_ * (qen - qp);
+          gk[k] += Mp * expm1((eps_ - del_) * dz);
           Mp *= exp((eps_ - del_) * dz);
         } else {
           const double g_ = expm1(A.ent_cloud * dz), fe = g_ / (1 + g_);
@@ -1002,10 +1030,14 @@ __device__ void plume_ascent(const double* mu, const double* th, const double* q
         G = Mp;
         FZ = fz;
 #pragma unroll
-        for (int m = 0; m < L; m++) qx[m] = qa[m];
+        for (int m = 0; m < L; m++) {
+          qx[m] = qa[m];
+          Gl[m] = gk[m];
+        }
       }
     }
-    if (!last || k == 0 || shal) {                  // (the shallow plume stops in the cloud layer)
+    // (the shallow plume stops in the cloud layer; with cu_dth in the cumulus layer or the cloud layer above it)
+    if (!last || k == 0 || (shal && !(cu && k == bot - 1))) {
       stop = true;
       continue;
     }
@@ -1080,6 +1112,106 @@ __device__ double lcl_mass(const double* mu, double th, double q, const CP& P, c
   return (ps_ - 0.5 * (lo + hi)) / P.grav;
 }
 
+// cu_top (air.cu:668-680): the theta the bottom layer's air reaches lifted from its top by cu_dp (Pa), condensing on
+// the way, in eight equal steps (no higher than 1000 Pa).
+template <int L, bool MIX>
+__device__ double cu_top(const double* mu, const double* th, const double* q, const CP& P, const AP& A) {
+  constexpr int bot = L - 1;
+  double pb = 0;
+#pragma unroll
+  for (int k = 0; k < bot; k++) pb += P.grav * mu[k];
+  double thp = th[bot], qp = q[bot];
+  for (int s = 1; s <= 8; s++) {
+    const double p = fmax(pb - A.cu_dp * s / 8, 1000.0);
+    condense<MIX>(thp, qp, P.cp * fpow_<MIX>(p / P.p0, P.kappa), p, P, A);
+  }
+  return thp;
+}
+
+// The cumulus layer's floating edges (air.cu:913-922, 1403-1407): its top (interface L - 3) at cu_dth above theta*,
+// or with cu_dp at cu_top, kept 0.5 K above theta* and below the layer's mean less 1 K and the edge above less 0.5 K;
+// with cl_float the cloud layer's lower edge (interface L - 4) cl_float above cu_top where that is above its fixed
+// value ths_cl, below that layer's mean less 1 K and the edge above less 0.5 K.
+template <int L, bool MIX>
+__device__ void cu_edges(const double* mu, const double* th, const double* q, CP& P, const AP& A, double ths_cl) {
+  if constexpr (L > 4) {
+    if (A.cl_float > 0 && A.cu_dth > 0 && A.cu_dp > 0)
+      P.ths[L - 4] = fmax(ths_cl, fmin(cu_top<L, MIX>(mu, th, q, P, A) + A.cl_float,
+                                       fmin(th[L - 4] - 1, P.ths[L - 5] - 0.5)));
+  }
+  if constexpr (L > 3) {
+    if (A.cu_dth > 0) {
+      const double top = A.cu_dp > 0 ? cu_top<L, MIX>(mu, th, q, P, A) : P.theta_star + A.cu_dth;
+      P.ths[L - 3] = fmax(P.theta_star + 0.5, fmin(top, fmin(th[L - 3] - 1, P.ths[L - 4] - 0.5)));
+    }
+  }
+}
+
+// plume_out (air.cu:682-713): a plume of base mass m from the bottom layer (theta th0, vapor q0) detraining into layer
+// kd at (thp, qc), having taken in Gl[k] per kg from each layer k and given qx[k] by mixing. m is first cut to the
+// layers' air above the floor and half their vapor. The air taken in below kd rises with it; kd's own leaves at the
+// layer's vapor and comes back at the plume's. cond: its condensate per kg (det of it left as cloud, at most ldet per
+// kg of plume); G its mass at kd per kg; the column's energy kept, the residual into kd.
+template <int L, bool CL>
+__device__ void plume_out(const CP& P, double* mu, double* th, double* q, double* pib, double* cl, int kd, double& m,
+                          double th0, double q0, double thp, double qc, const double* Gl, const double* qx,
+                          double* wxo, double ldet, double& cond, double& det, double& G) {
+  constexpr int bot = L - 1;
+  m = fmin(m, mu[bot] - P.floor_);
+#pragma unroll
+  for (int k = 0; k < bot; k++) {
+    if (Gl[k] > 0) m = fmin(m, fmax(0.0, mu[k] - P.floor_) / Gl[k]);
+    const double take = Gl[k] * q[k] - qx[k];
+    if (take > 0) m = fmin(m, 0.5 * mu[k] * q[k] / take);
+  }
+  G = 1;
+  cond = 0;
+  det = 0;
+  if (!(m > 0)) {
+    m = 0;
+    return;
+  }
+  double qin = 0, xs = 0;
+#pragma unroll
+  for (int k = 0; k < bot; k++) {
+    G += Gl[k];
+    qin += Gl[k] * q[k];
+    xs += qx[k];
+  }
+  const double cond_ = q0 + qin - G * qc - xs;
+  cond = fmax(0.0, cond_);
+  det = CL ? fmin(cond, ldet * G) : 0.0;
+  const double qarr = qc + (cond_ - cond) / G;
+  double tk[L], qk[L];
+#pragma unroll
+  for (int k = 0; k < L; k++) {
+    tk[k] = th[k];
+    qk[k] = q[k];
+  }
+  const double Ec = column_energy<L, true>(mu, pib, th, q, P);
+#pragma unroll
+  for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + m * qx[k] / mu[k]);
+  move_mass<L, true, CL>(P, mu, th, q, pib, bot, kd, m, th0, thp, q0, qarr, cl);
+#pragma unroll
+  for (int j = 0; j < L - 1; j++)
+    if (j >= kd) wxo[j] += m;
+#pragma unroll
+  for (int k = 0; k < bot; k++)
+    if (k > kd && Gl[k] > 0) {
+      move_mass<L, true, CL>(P, mu, th, q, pib, k, kd, m * Gl[k], tk[k], thp, qk[k], qarr, cl);
+#pragma unroll
+      for (int j = 0; j < L - 1; j++)
+        if (j >= kd && j < k) wxo[j] += m * Gl[k];
+    }
+#pragma unroll
+  for (int k = 0; k < L; k++)
+    if (k == kd && Gl[k] > 0) q[k] = fmax(0.0, q[k] + m * Gl[k] * (qarr - qk[k]) / mu[k]);
+  const double Ea = column_energy<L, true>(mu, pib, th, q, P);
+#pragma unroll
+  for (int k = 0; k < L; k++)
+    if (k == kd) th[k] += (Ec - Ea) / (mu[k] * pib[k]);
+}
+
 struct AirIO {
   double *theta, *qv, *Ts, *ice;
   double *wx, *olr, *sw_abs, *stored, *rain, *evap, *conv, *tsurf, *rainp, *qocean, *qsw, *flash, *swalb, *firn;
@@ -1087,6 +1219,7 @@ struct AirIO {
   double* cap;                                         // (rewritten over land with c_wet)
   const double *land, *soilcap, *lc;                   // the land mask (> 0.5 land), the bucket's capacity, the cover
   double *soil, *pond, *snow;
+  double* xfer;                                        // (3 L, N) the gas tracers' mole fractions, nullable
 };
 
 template <int L, bool MIX, bool MOIST, bool CLOUD, bool FULL>
@@ -1104,6 +1237,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     // the registers they would take)
     P.bl_mix = 0;
     A.sat_prof = A.sat_base = A.edge_vap = A.freeze_dry = A.q_shape = A.plume_root = A.plume_deep = A.bl_q = 0;
+    A.cu_dth = A.cu_sub = 0;
   }
   if (elev && A.theta_rise > 0 && elev[i] > 0) P.theta_star += A.theta_rise * elev[i];
   const double ts_base = P.theta_star;
@@ -1124,6 +1258,16 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   const double exb = pow(pb[bot] / P.p0, P.kappa);
   // the mixed bottom layer: the lowest interface's theta* no colder than the layer, below the air above's mean
   if (blm && L > 2) P.theta_star = fmin(fmax(ts_base, th[L - 1]), th[L - 2] - 1);
+  const double ths_cl = L > 4 ? P.ths[L > 4 ? L - 4 : 0] : 0.0;   // (cl_float: the cloud layer's fixed lower edge)
+  if constexpr (FULL) cu_edges<L, MIX>(mu, th, q, P, A, ths_cl);
+  // (the column's gas tracers, carried by every move of its air)
+  double xfl[3 * L];
+  if (o.xfer) {
+    for (int g = 0; g < 3; g++)
+#pragma unroll
+      for (int k = 0; k < L; k++) xfl[g * L + k] = o.xfer[((size_t)g * L + k) * N + i];
+    P.xf = xfl;
+  }
   double m_ice = MOIST ? o.ice[i] : 0.0;
   const bool isl = MOIST && FULL && o.land && o.land[i] > 0.5;   // land: the soil bucket, ponds, snow and the ground's heat
   double* Ts = o.Ts;
@@ -1706,6 +1850,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     }
   }
   if (blm && L > 2) P.theta_star = fmin(fmax(ts_base, th[bot]), th[bot - 1] - 1);
+  if constexpr (FULL) cu_edges<L, MIX>(mu, th, q, P, A, ths_cl);
   double mu_lcl = 1e30;
   if (A.bl_lcl > 0 && blm && L > 2) {                  // (bl_lcl: the mixed layer ends at its condensation level)
     mu_lcl = lcl_mass<L, MIX>(mu, th[bot], q[bot], P, A);
@@ -1795,6 +1940,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   // remove the buoyancy at the first interface relaxes over tau_conv ----
   double mc = 0, Scg = 0, qss_r = 0, ex_r = -1, ws_r = 0, rho_r = 0;   // (the root's saturation, the thermals'
                                                                          // excess, w* and density: the shallow plume's)
+  const double mu_pre = mu[bot];                       // (cu_sub: the mixed layer's mass before the plumes)
   if (o.flash) o.flash[i] = 0;
   if (MOIST && P.theta_star > 0) {
     double pi_ = 0;
@@ -1837,9 +1983,9 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       // buoyancy is A / (theta_d - theta*), relaxed over tau_conv; it takes in the cloud layer's air it entrained (me),
       // keeps l_det of its condensate as cloud, rains the rest (snow by its frozen share), and its rain feeds a
       // downdraft of the cloud layer's air, saturated by evaporating it
-      double Ab, G = 1, FZ = 0, qx[L], lt[4], xs = 0;
+      double Ab, G = 1, FZ = 0, qx[L], lt[4], xs = 0, Gl[L];
       int kd;
-      plume_ascent<L, MIX>(mu, th, q, th0, q0, pi_, P, A, Ab, kd, thp, qc, qx, lt, G, FZ);
+      plume_ascent<L, MIX>(mu, th, q, th0, q0, pi_, P, A, Ab, kd, thp, qc, qx, lt, G, FZ, Gl);
 #pragma unroll
       for (int k = 0; k < L; k++) xs += qx[k];
       if (kd >= 0 && Ab > 0) {
@@ -1858,26 +2004,36 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
             mc = fmin(mc, rho * f_ * ws * dt);
           }
         }
+        // cu_dth: through the cumulus and cloud layers, the air it took in from each (the layer it stops in included)
+        // moved by plume_out
+        double cond_cu = 0, det_cu = 0, G_cu = 1;
+        const bool cu = A.cu_dth > 0;
+        if (mc > 0 && cu)
+          plume_out<L, CLOUD>(P, mu, th, q, pib, cl, kd, mc, th0, q0, thp, qc, Gl, qx, wxo, A.l_det, cond_cu, det_cu,
+                              G_cu);
         if (mc > 0) {
           if (!(kd < bot - 1)) G = 1;
-          const double me = fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - P.floor_));
+          const double me = cu ? mc * (G_cu - 1) : fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - P.floor_));
           const double qe = q[bot - 1], the = th[bot - 1];
-          G = 1 + me / mc;
-          const double cond_ = q0 + (G - 1) * qe - G * qc - xs;
-          const double cond = fmax(0.0, cond_), det = CLOUD ? fmin(cond, A.l_det * G) : 0.0;
+          G = cu ? G_cu : 1 + me / mc;
+          const double cond_ = cu ? cond_cu : q0 + (G - 1) * qe - G * qc - xs;
+          const double cond = cu ? cond_cu : fmax(0.0, cond_);
+          const double det = cu ? det_cu : CLOUD ? fmin(cond, A.l_det * G) : 0.0;
           if (o.flash) o.flash[i] = storm_flashes<MIX>(mc / dt, lt, P, A);
           R += mc * (cond - det);
           Rc = mc * (cond - det);
-          const double qarr = qc + (cond_ - cond) / G;
-          const double Ec = column_energy<L, MOIST>(mu, pib, th, q, P);
+          if (!cu) {
+            const double qarr = qc + (cond_ - cond) / G;
+            const double Ec = column_energy<L, MOIST>(mu, pib, th, q, P);
 #pragma unroll
-          for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + mc * qx[k] / mu[k]);
-          move_mass<L, true, CLOUD>(P, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qarr, cl);
-          if (me > 0) move_mass<L, true, CLOUD>(P, mu, th, q, pib, bot - 1, kd, me, the, thp, qe, qarr, cl);
-          const double Ea = column_energy<L, MOIST>(mu, pib, th, q, P);
+            for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + mc * qx[k] / mu[k]);
+            move_mass<L, true, CLOUD>(P, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qarr, cl);
+            if (me > 0) move_mass<L, true, CLOUD>(P, mu, th, q, pib, bot - 1, kd, me, the, thp, qe, qarr, cl);
+            const double Ea = column_energy<L, MOIST>(mu, pib, th, q, P);
 #pragma unroll
-          for (int k = 0; k < L; k++)
-            if (k == kd) th[k] += (Ec - Ea) / (mu[k] * pib[k]);
+            for (int k = 0; k < L; k++)
+              if (k == kd) th[k] += (Ec - Ea) / (mu[k] * pib[k]);
+          }
           double Sc = 0;
           if (A.ice_air > 0 && cond > 0 && cond > det) {   // (its frozen share snows, melting below freezing air)
             Sc = mc * (cond - det) * fmin(1.0, FZ / cond);
@@ -1895,10 +2051,12 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
               ptj = pbj;
             }
           }
+          if (!cu) {
 #pragma unroll
-          for (int j = 0; j < L - 1; j++) {
-            if (j >= kd) wxo[j] += mc;
-            if (j >= kd && j < bot - 1) wxo[j] += me;
+            for (int j = 0; j < L - 1; j++) {
+              if (j >= kd) wxo[j] += mc;
+              if (j >= kd && j < bot - 1) wxo[j] += me;
+            }
           }
           if constexpr (CLOUD) {
 #pragma unroll
@@ -1917,22 +2075,52 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
             Rc += rt;
           }
           if (A.downdraft > 0 && bot >= 2 && Rc - Sc > 0) {
-            constexpr int s_ = bot - 1;
-            double pt_ = 0;
+            // the downdraft from the layer above the bottom one; with cu_dth from the cumulus layer or the cloud layer
+            // above it, whichever has the lower equivalent potential theta exp(Lv q / (cp T)) at its middle
+            auto downdraft = [&](auto S_) {
+              constexpr int s_ = decltype(S_)::value;
+              double pt_ = 0;
 #pragma unroll
-            for (int k = 0; k < s_; k++) pt_ += P.grav * mu[k];
-            const double p1 = pt_ + 0.5 * P.grav * mu[s_], p2 = pt_ + P.grav * (mu[s_] + mu[bot]);
-            double thd_ = th[s_], qd_ = q[s_];
-            for (int j = 0; j <= 4; j++) wet_bulb<MIX>(thd_, qd_, p1 + (p2 - p1) * j / 4, P, A);
-            const double ev = qd_ - q[s_];
-            double md = fmin(A.downdraft * mc, fmax(0.0, mu[s_] - P.floor_));
-            if (ev > 0) md = fmin(md, (Rc - Sc) / ev);
-            if (md > 0 && thd_ < th[s_]) {
-              move_mass<L, true, CLOUD>(P, mu, th, q, pib, s_, bot, md, th[s_], thd_, q[s_], qd_, cl);
-              wxo[s_] -= md;
-              const double e_ = md * fmax(0.0, ev);
-              R -= e_;
-              Rc -= e_;
+              for (int k = 0; k < s_; k++) pt_ += P.grav * mu[k];
+              const double p1 = pt_ + 0.5 * P.grav * mu[s_];
+              const double p2 = s_ == bot - 1 ? pt_ + P.grav * (mu[s_] + mu[bot])
+                                              : pt_ + P.grav * (mu[s_] + mu[s_ + 1 < L ? s_ + 1 : bot] + mu[bot]);
+              double thd_ = th[s_], qd_ = q[s_];
+              for (int j = 0; j <= 4; j++) wet_bulb<MIX>(thd_, qd_, p1 + (p2 - p1) * j / 4, P, A);
+              const double ev = qd_ - q[s_];
+              double md = fmin(A.downdraft * mc, fmax(0.0, mu[s_] - P.floor_));
+              if (ev > 0) md = fmin(md, (Rc - Sc) / ev);
+              if (md > 0 && thd_ < th[s_]) {
+                move_mass<L, true, CLOUD>(P, mu, th, q, pib, s_, bot, md, th[s_], thd_, q[s_], qd_, cl);
+#pragma unroll
+                for (int j = 0; j < L - 1; j++)
+                  if (j >= s_) wxo[j] -= md;
+                const double e_ = md * fmax(0.0, ev);
+                R -= e_;
+                Rc -= e_;
+              }
+            };
+            bool lower = false;
+            if constexpr (L >= 4) {
+              if (cu) {
+                double pc = 0, te_[2];
+#pragma unroll
+                for (int k = 0; k < bot - 2; k++) pc += P.grav * mu[k];
+#pragma unroll
+                for (int m = 0; m < 2; m++) {
+                  const int k = bot - 2 + m;
+                  const double pm = pc + 0.5 * P.grav * mu[k], Tk = th[k] * fpow_<MIX>(pm / P.p0, P.kappa);
+                  te_[m] = th[k] * exp(P.Lv * q[k] / (P.cp * Tk));
+                  pc += P.grav * mu[k];
+                }
+                lower = te_[0] < te_[1];
+              }
+            }
+            if constexpr (L >= 4) {
+              if (lower) downdraft(std::integral_constant<int, (L >= 4 ? bot - 2 : bot - 1)>{});
+              else downdraft(std::integral_constant<int, bot - 1>{});
+            } else {
+              downdraft(std::integral_constant<int, bot - 1>{});
             }
           }
           S += Sc;
@@ -1999,12 +2187,59 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       const double ths0 = th[bot];
       double qs0 = edge_q<L, MIX>(bot, false, mu, th, q, 0, P, A);
       qs0 += fmin(ex_r, fmax(0.0, qss_r - qs0));
-      double As, Gs = 1, FZs = 0, thps, qcs, qxs[L], lts[4], xss = 0;
+      double As, Gs = 1, FZs = 0, thps, qcs, qxs[L], lts[4], xss = 0, Gls[L];
       int kds;
-      plume_ascent<L, MIX>(mu, th, q, ths0, qs0, pis, P, A, As, kds, thps, qcs, qxs, lts, Gs, FZs, true);
+      plume_ascent<L, MIX>(mu, th, q, ths0, qs0, pis, P, A, As, kds, thps, qcs, qxs, lts, Gs, FZs, Gls, true);
 #pragma unroll
       for (int k = 0; k <= bot; k++) xss += qxs[k];
-      if (kds == ks && As > 0) {
+      if (A.cu_dth > 0) {
+        // cu_dth: through the cumulus layer and into the cloud layer above it, stopping in either; what it took in
+        // from each moved by plume_out; with buoy_v only through the cloud at the mixed layer's top, its share there
+        // at most
+        if ((kds == bot - 1 || kds == bot - 2) && As > 0) {
+          double ms = A.shallow * rho_r * ws_r * dt, cond, det, Gx;
+          if (A.buoy_v > 0) {
+            double l_, f_, d_;
+            ms = mixed_cloud<L, MIX>(bot, mu, th[bot], q[bot] + (CLOUD ? cl[bot] : 0.0), P, A, l_, f_, d_,
+                                     A.cu_cloud > 0)
+                     ? fmin(ms, rho_r * f_ * ws_r * dt)
+                     : 0.0;
+          }
+          if (qs0 > 0) ms = fmin(ms, mu[bot] * q[bot] / qs0);
+          plume_out<L, CLOUD>(P, mu, th, q, pib, cl, kds, ms, ths0, qs0, thps, qcs, Gls, qxs, wxo, A.l_det, cond, det,
+                              Gx);
+          if (ms > 0) {
+            const double rs = ms * (cond - det);
+            R += rs;
+            Rc += rs;
+            if (A.ice_air > 0 && cond > det) {         // (its frozen share snows, melting below where warm)
+              double Ss = rs * fmin(1.0, FZs / cond);
+#pragma unroll
+              for (int k = 0; k < L; k++)
+                if (k == kds) th[k] += A.Lf * Ss / (mu[k] * pib[k]);
+              double ptj = 0;
+#pragma unroll
+              for (int j = 0; j < L; j++) {
+                const double pbj = ptj + P.grav * mu[j];
+                if (j > kds && Ss > 0 && base_T<L, MIX>(j, th[j], pbj, P, A) > A.Tf) {
+                  th[j] -= A.Lf * Ss / (mu[j] * pib[j]);
+                  Ss = 0;
+                }
+                ptj = pbj;
+              }
+              S += Ss;
+              Scg += Ss;
+            }
+            if constexpr (CLOUD) {
+#pragma unroll
+              for (int k = 0; k < L; k++)
+                if (k == kds) cl[k] += ms * det / mu[k];
+              cloud_adjust_k<L, MIX>(kds, mu, th, q, cl, pib, dt, P, A);
+            }
+            mc += ms;
+          }
+        }
+      } else if (kds == ks && As > 0) {
         // its air taken in from the cloud layer (Gs - 1 per kg leaving the mixed layer) leaves it at the layer's vapor
         // and comes back at the plume's: at most the layer's air above its floor, and what it and the mixing take at
         // most half the layer's vapor
@@ -2049,6 +2284,28 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         }
       }
     }
+    // cu_sub (air.cu:1834-1861): the air the plumes took out of the mixed layer comes back from the cumulus layer,
+    // sinking between the clouds, up to the condensation level: the slice from that layer's bottom edge, its theta
+    // and vapor as the mixed layer's entrainment's; sub_mix stirs it toward the cumulus layer's mean
+    if constexpr (L >= 3) {
+      if (A.cu_sub > 0 && A.bl_lcl > 0 && blm) {
+        constexpr int ka = bot - 1;
+        const double m = fmin(fmin(mu_pre - mu[bot], lcl_mass<L, MIX>(mu, th[bot], q[bot], P, A) - mu[bot]),
+                              mu[ka] - P.floor_);
+        if (m > 0) {
+          const double e = tstar<L>(ka, P), fm = fmin(1.0, m / mu[ka]);
+          const bool spread = th[ka] < tstar<L>(ka - 1, P) && th[ka] > e;
+          double thc = spread ? e + fm * (th[ka] - e) : th[ka];
+          double qx = A.edge_vap > 0 ? edge_q<L, MIX>(ka, false, mu, th, q, m, P, A) : q[ka];
+          if (A.sub_mix > 0) {
+            thc += A.sub_mix * (th[ka] - thc);
+            qx += A.sub_mix * (q[ka] - qx);
+          }
+          move_mass<L, true, CLOUD>(P, mu, th, q, pib, ka, bot, m, thc, thc, qx, qx, cl);
+          wxo[ka] -= m;
+        }
+      }
+    }
   }
   // the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the
   // convection's area revap_area; else on conv_revap of the cell; with downdrafts they evaporate it instead)
@@ -2082,6 +2339,10 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   if (o.conv) o.conv[i] = mc;
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) o.wx[(size_t)j * N + i] = wxo[j];
+  if (o.xfer)
+    for (int g = 0; g < 3; g++)
+#pragma unroll
+      for (int k = 0; k < L; k++) o.xfer[((size_t)g * L + k) * N + i] = xfl[g * L + k];
   o.rain[i] = R / dt;
   if (!isl && S > 0) {                               // snow into the sea melts with the slab's heat
     Ts[i] -= A.Lf * S / Cw;
@@ -2261,14 +2522,14 @@ int air_column(int N, int L, const double* par, const isw_air* air, int mode, co
                double* evap, double* conv, double* tsurf, double* rainp, double* cap, double* qocean,
                double* qsw, const double* land, const double* soilcap, double* soil, double* pond, double* snow,
                const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
-               const double* ch, const double* lc, double* firn, double* cloudw, double* cloudo, int mixed,
-               hipStream_t st) {
+               const double* ch, const double* lc, double* firn, double* cloudw, double* cloudo, double* xfer,
+               int mixed, hipStream_t st) {
   const CP P = make_cp(par);
   const AP A = make_ap(par, air);
   const bool moist = P.moist > 0, cloudy = moist && par[55] > 0 && cloudw;
   if (!moist) land = nullptr;                          // (the dry column has no land)
   const AirIO o{theta, qv, Ts, ice, wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, qocean, qsw, flash,
-                swalb, firn, cloudw, cloudo, cap, land, soilcap, lc, soil, pond, snow};
+                swalb, firn, cloudw, cloudo, cap, land, soilcap, lc, soil, pond, snow, xfer};
   const int RW = L + 1;
   const double rows_in = 1 + 2.0 * L + 1 + (mode == 0 ? 3 : 1) + (cap ? 1 : 0) + (elev ? 1 : 0) + (ch ? 1 : 0) +
                          (radf ? 3.0 * RW + 6 * L : 0) + (swf ? 3.0 * L + 1 : 0) + (firn ? 1 : 0) +
@@ -2277,11 +2538,11 @@ int air_column(int N, int L, const double* par, const isw_air* air, int mode, co
                           (swalb ? 1 : 0) + (firn ? 1 : 0) + (moist ? L + 1.0 : 0) + (cloudy ? L : 0) +
                           (cloudy && cloudo ? (radf ? 4.0 * L + 3 : L + 3.0) : 0) + (land ? 3 : 0) +
                           (land && cap ? 1 : 0);
-  const double bytes = 8.0 * N * (rows_in + rows_out);
+  const double bytes = 8.0 * N * (rows_in + rows_out + (xfer ? 6.0 * L : 0));
   // the moist column's options that take runtime branches of their own: with none of them on it runs the plain
   // kernel (FULL false), their code compiled away (the dry column is always the full one: bl_mix acts there too)
   const bool full = land || P.bl_mix > 0 || par[51] > 0 || par[53] > 0 || par[86] > 0 || par[85] > 0 ||
-                    par[83] > 0 || par[88] > 0 || par[47] > 0 || par[89] > 0;
+                    par[83] > 0 || par[88] > 0 || par[47] > 0 || par[89] > 0 || par[111] > 0 || par[114] > 0;
   if (cloudy)
     return air_column_cloudy(N, L, mixed, full, P, A, mode, t, insol, pos, uabs, mu, dt, elev, radf, swf, ch, o,
                              bytes, st);

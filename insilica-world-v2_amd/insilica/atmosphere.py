@@ -195,8 +195,9 @@ class AirPhysics:
         optical depths into self.cloudo for the next radiate(); over land (the mask land (N), the bucket's capacity
         soilcap; soil, pond and snow updated in place; self.cover the land cover when set) the ground's water,
         snow and heat. Into theta, Ts, self.wx, self.olr, self.sw_abs, self.stored, self.rain, self.evap,
-        self.conv, self.tsurf, self.rainp, self.flash. The column's gas tracers (gx) are not in this library and
-        raise."""
+        self.conv, self.tsurf, self.rainp, self.flash. gx (3, L, N; the gas tracers' mole fractions per layer, the
+        reference's -DGASX column, world.py:124-125), when given, is carried by every move of the column's air
+        (updated in place)."""
         from . import _hip
         if self.forcing == 'held_suarez':
             _hip.call('isw_hs_column', self.dg.N, self.L, self._par, self.hs.ctypes.data_as(ctypes.c_void_p),
@@ -206,8 +207,8 @@ class AirPhysics:
             return
         if self.moist and (q is None or ice is None):
             raise ValueError('the moist column needs the vapor q and the sea ice')
-        if gx is not None:
-            raise NotImplementedError('the gas tracers carried by the column\'s crossing are not in this library')
+        if gx is not None and (not gx.is_contiguous() or tuple(gx.shape) != (3, self.L, self.dg.N)):
+            raise ValueError('gx: a contiguous (3, L, N) array of the gas tracers per layer')
         self._speed(u)
         P = _hip.ptr
         lw, sw = self.lw, self.sw
@@ -219,7 +220,7 @@ class AirPhysics:
                   P(snow), P(cw), P(self.cloudo) if cw is not None else None, P(elev), P(self.flash),
                   P(lw.radf) if lw is not None else None,
                   P(sw.swf) if sw is not None else None, P(self.swalb) if sw is not None else None,
-                  P(self.ch) if self.sfc else None, P(self.cover), P(firn), None, self._mixed, _hip.stream())
+                  P(self.ch) if self.sfc else None, P(self.cover), P(firn), P(gx), self._mixed, _hip.stream())
         self.advance_time(t, dt)
 
     @staticmethod

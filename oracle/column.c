@@ -20,7 +20,7 @@ enum { P_GRAV = 13, P_CP = 14, P_KAPPA = 15, P_P0 = 16, P_FLOOR = 11, P_THETA_ST
        P_BL_Q = 89, P_Q_SHAPE = 83, P_PLUME_ROOT = 88, P_ROOT_Q = 107, P_PLUME_DEEP = 47, P_PLUME_ENT = 78,
        P_ENT_CLOUD = 93, P_ENT_DET = 108, P_L_DET = 65, P_STORM_AREA = 80, P_GRAUPEL_V = 81, P_ICE_SMALL = 82,
        P_ML_CLEAR = 118, P_CU_CLOUD = 119, P_ENT_IFS = 109, P_ENT_LOC = 125, P_DET_ALL = 116, P_BUOY_V = 113,
-       P_SHALLOW = 110 };
+       P_SHALLOW = 110, P_CU_DTH = 111, P_CU_DP = 112, P_CU_SUB = 114, P_CL_FLOAT = 117, P_SUB_MIX = 131 };
 
 static void layer_exner(int L, const double* mu, const double* par, double* pb, double* pib) {
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP];
@@ -95,6 +95,10 @@ static double column_energy(int L, const double* mu, const double* pib, const do
 }
 
 /* Move m (> 0) kg/m^2 from layer `from` to `to`, re-dividing the column; the energy residual to the receiver. */
+/* The column's gas tracers (OG_GASN x OG_LMAX mole fractions, xfer) while a column is stepped, else NULL: every move of
+ * its air carries them, the donor's fraction into the receiver's (air.cu:503-507). One per OpenMP thread. */
+static _Thread_local double* xf_col;
+
 static void move_mass(int L, const double* par, double* mu, double* th, double* q, double* pib, int from, int to,
                       double m, double th_leave, double th_arrive, double q_leave, double q_arrive, double* cl) {
   if (m == 0) return;
@@ -107,6 +111,12 @@ static void move_mass(int L, const double* par, double* mu, double* th, double* 
   }
   mu2[from] -= m;
   mu2[to] += m;
+  if (xf_col) {
+    const int d = m > 0 ? from : to, r = m > 0 ? to : from;
+    const double am = fabs(m);
+    for (int g = 0; g < OG_GASN; g++)
+      xf_col[g * OG_LMAX + r] = (mu[r] * xf_col[g * OG_LMAX + r] + am * xf_col[g * OG_LMAX + d]) / mu2[r];
+  }
   layer_exner(L, mu2, par, pb2, pib2);
   th2[from] = (mu[from] * th[from] - m * th_leave) / mu2[from];
   th2[to] = (mu[to] * th[to] + m * th_arrive) / mu2[to];
@@ -776,11 +786,14 @@ static double liq_of(double T, int mix) {
  * ent_loc: the air taken in at the layer's relative humidity level by level; buoy_v: buoyancy in virtual theta. */
 static void plume_ascent(int L, const double* mu, const double* th, double th0, double q0, double pi_,
                          const double* par, const og_air* air, int mix, double* A, int* kd, double* thd, double* qd,
-                         const double* q, double* qx, double* lt, double* G, double* FZ, int shal) {
-  const int bot = L - 1, M = bot > 2 ? 3 : 8;
+                         const double* q, double* qx, double* lt, double* G, double* FZ, int shal, double* Gl,
+                         double* ptop) {
+  const int bot = L - 1, M = bot > 2 ? 3 : 8, cu = par[P_CU_DTH] > 0;
   const double grav = par[P_GRAV], p0 = par[P_P0], kappa = par[P_KAPPA], cp = par[P_CP], Lf = par[P_LF];
   const double Tf = par[P_TF], rd = air->rd, eps = air->eps;
   double Mp = 1, tp = th0, qp = q0, fz = 0;
+  /* (cu_dth: the growth it takes in from each layer; the half sub-level a layer's top leaves to the next) */
+  double gk[OG_LMAX], dzc = 0;
 #define LIFT(p_, ex_)                                                                \
   do {                                                                               \
     const double q_ = qp;                                                            \
@@ -808,13 +821,17 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
   for (int k = 0; k <= bot; k++) {
     qa[k] = 0;
     qx[k] = 0;
+    gk[k] = 0;
+    if (Gl) Gl[k] = 0;
   }
   for (int k = bot - 1; k >= 0; k--) {
     const double a = tstar(k, L, par), b = k > 0 ? tstar(k - 1, L, par) : 0.0;
     const double s = k > 0 ? edge_shape(a, b, th[k]) : 0.0, es = k > 0 && !mix ? expm1(s) : 0.0;
     /* (ent_ifs: the cloud layer's ascent in sub-levels of ~25 hPa; det_all: the IFS's entrainment and detrainment
      * through every tropospheric layer the plume crosses, below edge_k's stratosphere) */
-    const int ek = k > 0 && (k == bot - 1 || (par[P_DET_ALL] > 0 && ifs && !shal && k >= (int)par[P_EDGE_K]));
+    /* (cu_dth: and through the cloud layer above the cumulus layer) */
+    const int ek = k > 0 && (k == bot - 1 || (cu && ifs && k == bot - 2) ||
+                             (par[P_DET_ALL] > 0 && ifs && !shal && k >= (int)par[P_EDGE_K]));
     int Mk = M;
     if (ifs && ek) {
       const int n = (int)ceil(grav * mu[k] / 2500.0);
@@ -841,6 +858,9 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
       if (par[P_ENT_CLOUD] > 0 && ek) {
         double dz = rd * te * ex / grav * (grav * mu[k] / Mk) / pl;
         if (ifs && j == 0) dz *= 0.5;                 /* (from cloud base to the first sub-level's middle) */
+        /* (cu_dth: a layer's first sub-level also takes the rest of the layer below's last one; its last leaves half) */
+        if (cu && j == 0) dz += dzc;
+        if (cu && j == Mk - 1) dzc = 0.5 * rd * te * ex / grav * (grav * mu[k] / Mk) / pl;
         if (ifs) {
           /* ent_ifs: the IFS's entrainment 1.75e-3 /m (1.3 - RH) (q_s / q_s at cloud base)^3 and mixing detrainment
            * 0.75e-4 /m (Bechtold et al. 2008, 2014): the detrainment an equal-mass exchange, the rest of the
@@ -854,6 +874,7 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
           if (loc) qa[k] -= Mp * expm1((eps_ - del_) * dz) * (qen - q[k]);
           tp += fe_ * (te - tp);
           qp += fe_ * (qen - qp);
+          gk[k] += Mp * expm1((eps_ - del_) * dz);
           Mp *= exp((eps_ - del_) * dz);
         } else {
           const double g_ = expm1(par[P_ENT_CLOUD] * dz), fe = g_ / (1 + g_);
@@ -891,9 +912,13 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
         *G = Mp;
         *FZ = fz;
         for (int m = 0; m <= bot; m++) qx[m] = qa[m];
+        if (Gl)
+          for (int m = 0; m <= bot; m++) Gl[m] = gk[m];
+        if (ptop) *ptop = pl;
       }
     }
-    if (!last || k == 0 || shal) break;             /* (the shallow plume stops in the cloud layer) */
+    /* (the shallow plume stops in the cloud layer; with cu_dth in the cumulus layer or the cloud layer above it) */
+    if (!last || k == 0 || (shal && !(cu && k == bot - 1))) break;
     pt -= grav * mu[k];
     double ext;
     LIFT(pt, ext);
@@ -911,13 +936,98 @@ static void plume_ascent(int L, const double* mu, const double* th, double th0, 
   lt[3] = cape;
 }
 
-/* plume_ascent on its own, for the tests' closed-form checks of its options: out = (A, kd, thd, qd, G, FZ, lt[0..3]),
- * qx (L) what its mixing gave each layer per kg. */
+/* cu_top (air.cu:668-680): the theta the bottom layer's air reaches lifted from its top by cu_dp (Pa), condensing on
+ * the way, in eight equal steps (no higher than 1000 Pa). */
+static double cu_top(int L, const double* mu, const double* th, const double* q, const double* par, double eps,
+                     int mix) {
+  const int bot = L - 1;
+  double pb = 0;
+  for (int k = 0; k < bot; k++) pb += par[P_GRAV] * mu[k];
+  double thp = th[bot], qp = q[bot];
+  for (int s = 1; s <= 8; s++) {
+    const double p = fmax(pb - par[P_CU_DP] * s / 8, 1000.0);
+    condense(&thp, &qp, par[P_CP] * fpow_(p / par[P_P0], par[P_KAPPA], mix), p, par, eps, mix);
+  }
+  return thp;
+}
+
+/* The cumulus layer's floating edges (air.cu:913-922, again before the crossing at 1403-1407): the cumulus layer's
+ * top (interface L - 3) at cu_dth above theta*, or with cu_dp at cu_top, kept 0.5 K above theta* and below both the
+ * layer's mean less 1 K and the edge above less 0.5 K; with cl_float the cloud layer's lower edge (interface L - 4)
+ * cl_float above cu_top where that is above its fixed value ths_cl, below the layer's mean less 1 K and the edge
+ * above less 0.5 K. */
+static void cu_edges(int L, const double* mu, const double* th, const double* q, double* par, double ths_cl,
+                     double eps, int mix) {
+  if (par[P_CL_FLOAT] > 0 && par[P_CU_DTH] > 0 && par[P_CU_DP] > 0 && L > 4)
+    par[P_THS + L - 4] = fmax(ths_cl, fmin(cu_top(L, mu, th, q, par, eps, mix) + par[P_CL_FLOAT],
+                                           fmin(th[L - 4] - 1, par[P_THS + L - 5] - 0.5)));
+  if (par[P_CU_DTH] > 0 && L > 3) {
+    const double top = par[P_CU_DP] > 0 ? cu_top(L, mu, th, q, par, eps, mix) : par[P_THETA_STAR] + par[P_CU_DTH];
+    par[P_THS + L - 3] = fmax(par[P_THETA_STAR] + 0.5, fmin(top, fmin(th[L - 3] - 1, par[P_THS + L - 4] - 0.5)));
+  }
+}
+
+/* plume_out (air.cu:682-713): a plume of base mass *m from the bottom layer (theta th0, vapor q0) detraining into
+ * layer kd at (thp, qc), having taken in Gl[k] per kg from each layer k and given qx[k] by mixing. *m is first cut
+ * to the layers' air above the floor and half their vapor. The air taken in below kd rises with it; kd's own leaves
+ * at the layer's vapor and comes back at the plume's. cond: its condensate per kg (det of it left as cloud, at most
+ * ldet per kg of plume); G its mass at kd per kg; the column's energy kept, the residual into kd. */
+static void plume_out(int L, const double* par, double* mu, double* th, double* q, double* pib, double* cl, int kd,
+                      double* m, double th0, double q0, double thp, double qc, const double* Gl, const double* qx,
+                      double* wxo, double ldet, double* cond, double* det, double* G) {
+  const int bot = L - 1;
+  const double floor_ = par[P_FLOOR];
+  double mm = fmin(*m, mu[bot] - floor_);
+  for (int k = 0; k < bot; k++) {
+    if (Gl[k] > 0) mm = fmin(mm, fmax(0.0, mu[k] - floor_) / Gl[k]);
+    const double take = Gl[k] * q[k] - qx[k];
+    if (take > 0) mm = fmin(mm, 0.5 * mu[k] * q[k] / take);
+  }
+  *G = 1;
+  *cond = 0;
+  *det = 0;
+  if (!(mm > 0)) {
+    *m = 0;
+    return;
+  }
+  *m = mm;
+  double qin = 0, xs = 0, Gt = 1;
+  for (int k = 0; k < bot; k++) {
+    Gt += Gl[k];
+    qin += Gl[k] * q[k];
+    xs += qx[k];
+  }
+  *G = Gt;
+  const double cond_ = q0 + qin - Gt * qc - xs;
+  *cond = fmax(0.0, cond_);
+  *det = cl ? fmin(*cond, ldet * Gt) : 0.0;
+  const double qarr = qc + (cond_ - *cond) / Gt;
+  double tk[OG_LMAX], qk[OG_LMAX];
+  for (int k = 0; k < bot; k++) {
+    tk[k] = th[k];
+    qk[k] = q[k];
+  }
+  const double Ec = column_energy(L, mu, pib, th, q, par);
+  for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + mm * qx[k] / mu[k]);
+  move_mass(L, par, mu, th, q, pib, bot, kd, mm, th0, thp, q0, qarr, cl);
+  for (int j = kd; j < bot; j++) wxo[j] += mm;
+  for (int k = kd + 1; k < bot; k++)
+    if (Gl[k] > 0) {
+      move_mass(L, par, mu, th, q, pib, k, kd, mm * Gl[k], tk[k], thp, qk[k], qarr, cl);
+      for (int j = kd; j < k; j++) wxo[j] += mm * Gl[k];
+    }
+  if (Gl[kd] > 0) q[kd] = fmax(0.0, q[kd] + mm * Gl[kd] * (qarr - qk[kd]) / mu[kd]);
+  th[kd] += (Ec - column_energy(L, mu, pib, th, q, par)) / (mu[kd] * pib[kd]);
+}
+
+/* plume_ascent on its own, for the tests' closed-form checks of its options: out = (A, kd, thd, qd, G, FZ, lt[0..3],
+ * ptop), qx (L) what its mixing gave each layer per kg, Gl (L) what it took in from each. */
 void og_plume_ascent(int L, const double* par, const og_air* air, const double* mu, const double* th, const double* q,
-                     double th0, double q0, double pi_, int shal, int mix, double* out, double* qx) {
-  double A = 0, thd = 0, qd = 0, G = 1, FZ = 0, lt[4] = {0, 0, 0, 0};
+                     double th0, double q0, double pi_, int shal, int mix, double* out, double* qx, double* Gl) {
+  double A = 0, thd = 0, qd = 0, G = 1, FZ = 0, lt[4] = {0, 0, 0, 0}, ptop = pi_;
   int kd = -1;
-  plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thd, &qd, q, qx, lt, &G, &FZ, shal);
+  plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thd, &qd, q, qx, lt, &G, &FZ, shal, Gl, &ptop);
+  out[10] = ptop;
   out[0] = A;
   out[1] = kd;
   out[2] = thd;
@@ -965,11 +1075,15 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
                    double* cap, double* qocean, double* qsw, const double* elev, double* flash,
                    const double* radf, const double* swf, double* swalb, const double* ch, double* firn,
                    double* cloudw, double* cloudo, const double* land, const double* soilcap, double* soil,
-                   double* pond, double* snow, const double* lc, int mix) {
+                   double* pond, double* snow, const double* lc, double* xfer, int mix) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++) {
     double par[132 + OG_LMAX];
     memcpy(par, par_in, sizeof(par));
+    double xfl[OG_GASN * OG_LMAX];                    /* the column's gas tracers, carried by every move of its air */
+    for (int g = 0; g < OG_GASN; g++)
+      for (int k = 0; k < L; k++) xfl[g * OG_LMAX + k] = xfer ? xfer[((size_t)g * L + k) * N + i] : 0.0;
+    xf_col = xfer ? xfl : NULL;
     if (elev && par[P_THETA_RISE] > 0 && elev[i] > 0) par[P_THETA_STAR] += par[P_THETA_RISE] * elev[i];
     const double ts_base = par[P_THETA_STAR];
     par[P_BL_CAP] = omexp(dt / 86400.0, mix);       /* (the mixed layer takes in at most a day's share per step) */
@@ -995,6 +1109,8 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     const double exb = pow(pb[bot] / p0, kappa);     /* (p_bot / p0)^kappa at the column's lower edge */
     /* the mixed bottom layer: the lowest interface's theta* no colder than the layer, below the air above's mean */
     if (blm && L > 2) par[P_THETA_STAR] = fmin(fmax(ts_base, th[L - 1]), th[L - 2] - 1);
+    const double ths_cl = L > 4 ? par[P_THS + L - 4] : 0.0;   /* (cl_float: the cloud layer's fixed lower edge) */
+    cu_edges(L, mu, th, q, par, ths_cl, aeps, mix);
     double m_ice = moist ? ice[i] : 0.0;
     const int isl = moist && land && land[i] > 0.5;  /* land: the soil bucket, ponds, snow and the ground's heat */
     if (cap && par[P_C_WET] > 0 && isl) {            /* (c_wet: the ground's capacity by its water, its heat kept) */
@@ -1467,6 +1583,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       }
     /* the crossing, with the k-tables' edge sub-levels' own heating at the layers' edges */
     if (blm && L > 2) par[P_THETA_STAR] = fmin(fmax(ts_base, th[bot]), th[bot - 1] - 1);
+    cu_edges(L, mu, th, q, par, ths_cl, aeps, mix);
     /* (bl_lcl: the mixed layer ends at its condensation level; lcl_mass: the layer's mass up to where its air, lifted
      * dry-adiabatically, saturates, by bisection in pressure) */
     double mu_lcl = 1e30, bl_room = 1e30;
@@ -1565,6 +1682,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
      * raining; the mass that would remove the buoyancy at the first interface relaxes over tau_conv */
     double mc = 0, Scg = 0, qss_r = 0, ex_r = -1, ws_r = 0, rho_r = 0;   /* (the root's saturation, the thermals'
                                                                            * excess, w* and density: the shallow plume's) */
+    const double mu_pre = mu[bot];                    /* (cu_sub: the mixed layer's mass before the plumes) */
     if (flash) flash[i] = 0;
     if (moist && par[P_THETA_STAR] > 0) {
       double pi_ = 0;
@@ -1606,9 +1724,9 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
          * removes its buoyancy is A / (theta_d - theta*), relaxed over tau_conv; it takes in the cloud layer's air it
          * entrained (me), keeps l_det of its condensate as cloud, rains the rest (snow by its frozen share), and its
          * rain feeds a downdraft of the cloud layer's air, saturated by evaporating it */
-        double A, G = 1, FZ = 0, qx[OG_LMAX], lt[4], xs = 0;
+        double A, G = 1, FZ = 0, qx[OG_LMAX], lt[4], xs = 0, Gl[OG_LMAX];
         int kd;
-        plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thp, &qc, q, qx, lt, &G, &FZ, 0);
+        plume_ascent(L, mu, th, th0, q0, pi_, par, air, mix, &A, &kd, &thp, &qc, q, qx, lt, &G, &FZ, 0, Gl, NULL);
         for (int k = 0; k <= bot; k++) xs += qx[k];
         if (kd >= 0 && A > 0) {
           const double rise = thp - tstar(bot - 1, L, par);
@@ -1626,22 +1744,32 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
               mc = fmin(mc, rho * f_ * ws * dt);
             }
           }
+          /* cu_dth: through the cumulus and cloud layers, the air it took in from each (the layer it stops in
+           * included) moved by plume_out */
+          double cond_cu = 0, det_cu = 0, G_cu = 1;
+          const int cu = par[P_CU_DTH] > 0;
+          if (mc > 0 && cu)
+            plume_out(L, par, mu, th, q, pib, cloudy ? cl : NULL, kd, &mc, th0, q0, thp, qc, Gl, qx, wxo, par[P_L_DET],
+                      &cond_cu, &det_cu, &G_cu);
           if (mc > 0) {
             if (!(kd < bot - 1)) G = 1;
-            const double me = fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - par[P_FLOOR]));
+            const double me = cu ? mc * (G_cu - 1) : fmin(mc * (G - 1), fmax(0.0, mu[bot - 1] - par[P_FLOOR]));
             const double qe = q[bot - 1], the = th[bot - 1];
-            G = 1 + me / mc;
-            const double cond_ = q0 + (G - 1) * qe - G * qc - xs;
-            const double cond = fmax(0.0, cond_), det = cloudy ? fmin(cond, par[P_L_DET] * G) : 0.0;
+            G = cu ? G_cu : 1 + me / mc;
+            const double cond_ = cu ? cond_cu : q0 + (G - 1) * qe - G * qc - xs;
+            const double cond = cu ? cond_cu : fmax(0.0, cond_);
+            const double det = cu ? det_cu : cloudy ? fmin(cond, par[P_L_DET] * G) : 0.0;
             if (flash) flash[i] = storm_flashes(mc / dt, lt, par, air, mix);
             R += mc * (cond - det);
             Rc = mc * (cond - det);
-            const double qarr = qc + (cond_ - cond) / G;
-            const double Ec = column_energy(L, mu, pib, th, q, par);
-            for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + mc * qx[k] / mu[k]);
-            move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qarr, cloudy ? cl : NULL);
-            if (me > 0) move_mass(L, par, mu, th, q, pib, bot - 1, kd, me, the, thp, qe, qarr, cloudy ? cl : NULL);
-            th[kd] += (Ec - column_energy(L, mu, pib, th, q, par)) / (mu[kd] * pib[kd]);
+            if (!cu) {
+              const double qarr = qc + (cond_ - cond) / G;
+              const double Ec = column_energy(L, mu, pib, th, q, par);
+              for (int k = 0; k < bot; k++) q[k] = fmax(0.0, q[k] + mc * qx[k] / mu[k]);
+              move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qarr, cloudy ? cl : NULL);
+              if (me > 0) move_mass(L, par, mu, th, q, pib, bot - 1, kd, me, the, thp, qe, qarr, cloudy ? cl : NULL);
+              th[kd] += (Ec - column_energy(L, mu, pib, th, q, par)) / (mu[kd] * pib[kd]);
+            }
             double Sc = 0;
             if (par[P_ICE_AIR] > 0 && cond > 0 && cond > det) {   /* (its frozen share snows, melting below Tf) */
               Sc = mc * (cond - det) * fmin(1.0, FZ / cond);
@@ -1652,8 +1780,10 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
                   Sc = 0;
                 }
             }
-            for (int j = kd; j < bot; j++) wxo[j] += mc;
-            for (int j = kd; j < bot - 1; j++) wxo[j] += me;
+            if (!cu) {
+              for (int j = kd; j < bot; j++) wxo[j] += mc;
+              for (int j = kd; j < bot - 1; j++) wxo[j] += me;
+            }
             if (cloudy) {
               cl[kd] += mc * det / mu[kd];
               cloud_adjust(L, kd, mu, th, q, cl, pib, dt, par, aeps, mix);
@@ -1663,10 +1793,24 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
               Rc += rt;
             }
             if (par[P_DOWNDRAFT] > 0 && bot >= 2 && Rc - Sc > 0) {
-              const int s_ = bot - 1;
+              int s_ = bot - 1;
+              if (cu && bot >= 3) {
+                /* (cu_dth: it sinks from the cumulus layer or the cloud layer above it, whichever has the lower
+                 * equivalent potential temperature theta exp(Lv q / (cp T)) at its middle) */
+                double pc = 0, te_[2];
+                for (int k = 0; k < bot - 2; k++) pc += grav * mu[k];
+                for (int m = 0; m < 2; m++) {
+                  const int k = bot - 2 + m;
+                  const double pm = pc + 0.5 * grav * mu[k], Tk = th[k] * fpow_(pm / p0, kappa, mix);
+                  te_[m] = th[k] * exp(Lv * q[k] / (cp * Tk));
+                  pc += grav * mu[k];
+                }
+                if (te_[0] < te_[1]) s_ = bot - 2;
+              }
               double pt_ = 0;
               for (int k = 0; k < s_; k++) pt_ += grav * mu[k];
-              const double p1 = pt_ + 0.5 * grav * mu[s_], p2 = pt_ + grav * (mu[s_] + mu[bot]);
+              const double p1 = pt_ + 0.5 * grav * mu[s_];
+              const double p2 = s_ == bot - 1 ? pt_ + grav * (mu[s_] + mu[bot]) : pt_ + grav * (mu[s_] + mu[s_ + 1] + mu[bot]);
               double thd_ = th[s_], qd_ = q[s_];
               for (int j = 0; j <= 4; j++) wet_bulb(&thd_, &qd_, p1 + (p2 - p1) * j / 4, par, aeps, mix);
               const double ev = qd_ - q[s_];
@@ -1730,11 +1874,50 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         const double ths0 = th[bot];
         double qs0 = edge_q(L, bot, 0, mu, th, q, 0, par, aeps, mix);
         qs0 += fmin(ex_r, fmax(0.0, qss_r - qs0));
-        double As, Gs = 1, FZs = 0, thps, qcs, qxs[OG_LMAX], xss = 0, lts[4];
+        double As, Gs = 1, FZs = 0, thps, qcs, qxs[OG_LMAX], xss = 0, lts[4], Gls[OG_LMAX];
         int kds;
-        plume_ascent(L, mu, th, ths0, qs0, pis, par, air, mix, &As, &kds, &thps, &qcs, q, qxs, lts, &Gs, &FZs, 1);
+        plume_ascent(L, mu, th, ths0, qs0, pis, par, air, mix, &As, &kds, &thps, &qcs, q, qxs, lts, &Gs, &FZs, 1, Gls,
+                     NULL);
         for (int k = 0; k <= bot; k++) xss += qxs[k];
-        if (kds == bot - 1 && As > 0) {
+        if (par[P_CU_DTH] > 0) {
+          /* cu_dth: through the cumulus layer and into the cloud layer above it, stopping in either; what it took in
+           * from each moved by plume_out; with buoy_v only through the cloud at the mixed layer's top, its share
+           * there at most */
+          if ((kds == bot - 1 || kds == bot - 2) && As > 0) {
+            double ms = par[P_SHALLOW] * rho_r * ws_r * dt, cond, det, Gx;
+            if (par[P_BUOY_V] > 0) {
+              double l_, f_, d_;
+              ms = mixed_cloud(L, bot, mu, th[bot], q[bot] + (cloudy ? cl[bot] : 0.0), par, aeps, mix, &l_, &f_, &d_,
+                               par[P_CU_CLOUD] > 0)
+                       ? fmin(ms, rho_r * f_ * ws_r * dt)
+                       : 0.0;
+            }
+            if (qs0 > 0) ms = fmin(ms, mu[bot] * q[bot] / qs0);
+            plume_out(L, par, mu, th, q, pib, cloudy ? cl : NULL, kds, &ms, ths0, qs0, thps, qcs, Gls, qxs, wxo,
+                      par[P_L_DET], &cond, &det, &Gx);
+            if (ms > 0) {
+              const double rs = ms * (cond - det);
+              R += rs;
+              Rc += rs;
+              if (par[P_ICE_AIR] > 0 && cond > det) {    /* (its frozen share snows, melting below where warm) */
+                double Ss = rs * fmin(1.0, FZs / cond);
+                th[kds] += Lf * Ss / (mu[kds] * pib[kds]);
+                for (int j = kds + 1; j <= bot && Ss > 0; j++)
+                  if (base_T(L, j, mu, th, par, mix) > Tf) {
+                    th[j] -= Lf * Ss / (mu[j] * pib[j]);
+                    Ss = 0;
+                  }
+                S += Ss;
+                Scg += Ss;
+              }
+              if (cloudy) {
+                cl[kds] += ms * det / mu[kds];
+                cloud_adjust(L, kds, mu, th, q, cl, pib, dt, par, aeps, mix);
+              }
+              mc += ms;
+            }
+          }
+        } else if (kds == bot - 1 && As > 0) {
           /* its air taken in from the cloud layer (Gs - 1 per kg leaving the mixed layer) leaves it at the layer's
            * vapor and comes back at the plume's: at most the layer's air above its floor, and what it and the mixing
            * take at most half the layer's vapor */
@@ -1775,6 +1958,26 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           }
         }
       }
+      /* cu_sub (air.cu:1834-1861): the air the plumes took out of the mixed layer comes back from the cumulus layer,
+       * sinking between the clouds, up to the condensation level: the slice from that layer's bottom edge, its theta
+       * and vapor as the mixed layer's entrainment's; sub_mix stirs it toward the cumulus layer's mean */
+      if (par[P_CU_SUB] > 0 && par[P_BL_LCL] > 0 && blm && bot >= 2) {
+        const int ka = bot - 1;
+        const double m = fmin(fmin(mu_pre - mu[bot], lcl_mass(L, mu, th, q, par, aeps, mix) - mu[bot]),
+                              mu[ka] - par[P_FLOOR]);
+        if (m > 0) {
+          const double e = tstar(ka, L, par), fm = fmin(1.0, m / mu[ka]);
+          const int spread = th[ka] < tstar(ka - 1, L, par) && th[ka] > e;
+          double thc = spread ? e + fm * (th[ka] - e) : th[ka];
+          double qx = par[P_EDGE_VAP] > 0 ? edge_q(L, ka, 0, mu, th, q, m, par, aeps, mix) : q[ka];
+          if (par[P_SUB_MIX] > 0) {
+            thc += par[P_SUB_MIX] * (th[ka] - thc);
+            qx += par[P_SUB_MIX] * (q[ka] - qx);
+          }
+          move_mass(L, par, mu, th, q, pib, ka, bot, m, thc, thc, qx, qx, cloudy ? cl : NULL);
+          wxo[ka] -= m;
+        }
+      }
     }
     /* the plumes' rain evaporates as it falls below their base through the bottom layer's clear air (revap: on the
      * convection's area revap_area; else on conv_revap of the cell; with downdrafts they evaporate it instead) */
@@ -1805,6 +2008,10 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     }
     if (conv) conv[i] = mc;
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
+    if (xfer)
+      for (int g = 0; g < OG_GASN; g++)
+        for (int k = 0; k < L; k++) xfer[((size_t)g * L + k) * N + i] = xfl[g * OG_LMAX + k];
+    xf_col = NULL;
     rain[i] = R / dt;
     if (!isl && S > 0) {                             /* snow into the sea melts with the slab's heat */
       Ts[i] -= Lf * S / Cw;

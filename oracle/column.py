@@ -59,12 +59,13 @@ def gas_src(src, h, dt, x, L):
 
 def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None, elev=None, radf=None, swf=None,
                ch=None, firn=None, air=None, mix=False, ocean=False, q=None, ice=None, cw=None, cloudo=False,
-               land=None, soilcap=None, soil=None, pond=None, snow=None, lc=None):
+               land=None, soilcap=None, soil=None, pond=None, snow=None, lc=None, xfer=None):
     """air.cu air_column (the dry column; the moist one, par's moist > 0, with q (L, N) and ice (N); clouds with cloud
     > 0 and the cloud water cw (L, N)): returns a dict of theta, Ts, wx, olr, sw_abs, stored, rain, evap, conv, tsurf,
     rainp, flash, swalb (and qocean, qsw with ocean=True; firn when given; q and ice when moist; cw, and cloudo
     ((4L + 3), N) with cloudo=True, with clouds; over land (moist, the mask land (N) > 0.5, soilcap (N)) soil, pond and
-    snow (N), and cap when given (c_wet rewrites it); lc (8, N) the land cover's rows). mu is read only."""
+    snow (N), and cap when given (c_wet rewrites it); lc (8, N) the land cover's rows; xfer (3 L, N) the gas
+    tracers' mole fractions per layer, carried by the column's moves of air). mu is read only."""
     par, t, pos, uabs, mu, insol, elev, radf, swf, ch, land, soilcap, lc = (
         _d(x) for x in (par, t, pos, uabs, mu, insol, elev, radf, swf, ch, land, soilcap, lc))
     cap, soil, pond, snow = (None if x is None else _d(x).copy() for x in (cap, soil, pond, snow))
@@ -73,6 +74,7 @@ def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None,
     ice = None if ice is None else _d(ice).copy()
     cw = None if cw is None else _d(cw).copy()
     firn = None if firn is None else _d(firn).copy()
+    xfer = None if xfer is None else _d(xfer).copy()
     L, N = mu.shape
     z = lambda *s: np.zeros(s)
     o = dict(theta=theta, Ts=Ts, wx=z(max(L - 1, 1), N), olr=z(N), sw_abs=z(N), stored=z(N), rain=z(N), evap=z(N),
@@ -86,7 +88,9 @@ def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None,
                         _p(o['stored']), _p(o['rain']), _p(o['evap']), _p(o['conv']), _p(o['tsurf']), _p(o['rainp']),
                         _p(cap), _p(o.get('qocean')), _p(o.get('qsw')), _p(elev), _p(o['flash']), _p(radf), _p(swf),
                         _p(o['swalb']), _p(ch), _p(firn), _p(cw), _p(o.get('cloudo')), _p(land), _p(soilcap),
-                        _p(soil), _p(pond), _p(snow), _p(lc), int(bool(mix)))
+                        _p(soil), _p(pond), _p(snow), _p(lc), _p(xfer), int(bool(mix)))
+    if xfer is not None:
+        o['xfer'] = xfer
     if firn is not None:
         o['firn'] = firn
     if q is not None:
@@ -103,10 +107,12 @@ def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None,
 def plume_ascent(par, mu, theta, q, th0, q0, pi_, shal=False, mix=False, air=None):
     """air.cu plume_ascent alone for one column (mu, theta, q: (L,)): a dict of A (buoyancy by mass), kd (the layer it
     detrains into, -1 if none), thd, qd (its theta and vapor there), G (its mass per kg), FZ (frozen condensate), lt
-    (the lightning inputs) and qx (L) what its mixing gave each layer per kg."""
+    (the lightning inputs), ptop (the pressure of its last buoyant sub-level), qx (L) what its mixing gave each layer
+    per kg and Gl (L) what it took in from each (ent_ifs)."""
     par, mu, theta, q = _d(par), _d(mu), _d(theta), _d(q)
     L = mu.size
-    out, qx = np.zeros(10), np.zeros(L)
+    out, qx, Gl = np.zeros(11), np.zeros(L), np.zeros(L)
     lib().og_plume_ascent(L, _p(par), ctypes.byref(air_struct(air)), _p(mu), _p(theta), _p(q), float(th0), float(q0),
-                          float(pi_), int(bool(shal)), int(bool(mix)), _p(out), _p(qx))
-    return dict(A=out[0], kd=int(out[1]), thd=out[2], qd=out[3], G=out[4], FZ=out[5], lt=out[6:10], qx=qx)
+                          float(pi_), int(bool(shal)), int(bool(mix)), _p(out), _p(qx), _p(Gl))
+    return dict(A=out[0], kd=int(out[1]), thd=out[2], qd=out[3], G=out[4], FZ=out[5], lt=out[6:10], ptop=out[10], qx=qx,
+                Gl=Gl)

@@ -13,6 +13,7 @@
 #define ISW_ORACLE_H
 #include <stdint.h>
 
+#define OG_GASN 3 /* the gas tracers the air column carries (CO2, N2O, CH4) */
 #define OG_LMAX 8
 
 typedef struct og_grid {
@@ -117,16 +118,18 @@ void og_gas_src(int N, int L, int G, const double* src, const double* h, double 
  * (L, N) and ice (N) read and written when moist, the cloud water cloudw (L, N) with cloud > 0 (cloudo ((4L + 3), N)
  * the clouds' cover, effects and optical depths, when given). mix: the reference's mixed build's float32 arithmetic. */
 /* The deep (shal = 0) or shallow (shal = 1) plume's ascent alone (column.c plume_ascent, air.cu:540-666): out = (A,
- * kd, thd, qd, G, FZ, lt[0..3]), qx (L) what its mixing gave each layer per kg. For the tests' closed-form checks. */
+ * kd, thd, qd, G, FZ, lt[0..3], ptop), qx (L) what its mixing gave each layer per kg, Gl (L) what it took in from each.
+ * For the tests' closed-form checks. */
 void og_plume_ascent(int L, const double* par, const og_air* air, const double* mu, const double* th, const double* q,
-                     double th0, double q0, double pi_, int shal, int mix, double* out, double* qx);
+                     double th0, double q0, double pi_, int shal, int mix, double* out, double* qx, double* Gl);
 void og_air_column(int N, int L, const double* par, const og_air* air, int mode, const double* t, const double* insol,
                    const double* pos, const double* uabs, const double* mu, double* theta, double* qv, double* Ts,
                    double* ice, double dt, double* wx, double* olr, double* sw_abs, double* stored, double* rain,
                    double* evap, double* conv, double* tsurf, double* rainp, double* cap, double* qocean,
                    double* qsw, const double* elev, double* flash, const double* radf, const double* swf,
                    double* swalb, const double* ch, double* firn, double* cloudw, double* cloudo, const double* land,
-                   const double* soilcap, double* soil, double* pond, double* snow, const double* lc, int mix);
+                   const double* soilcap, double* soil, double* pond, double* snow, const double* lc, double* xfer,
+                   int mix);
 double og_sun_lon(double t, const double* par, double* fd);
 
 /* ---- the ocean's column processes (ocean.c; reference kernels/ocean.cu) ---- */

@@ -93,25 +93,14 @@ def test_bad_arguments_rejected_on_host():
         # cloudo; elev, flash, radf, swf, swalb, ch, lc, firn; xfer, prec, stream
         return lib.isw_air_column(10, L, par, a, mode, t, insol, buf, buf, buf, buf, None, buf, None, dt,
                                   *([buf] * 8), None, dgo, *([None] * 10), *([None] * 8), xfer, prec, None)
-    par[111] = 1.0                                           # cu_dth (with L > 3)
-    assert air_column(L=4) == E
-    par[111] = 0.0
-    par[22] = 1.0                                            # moist air: no vapor or ice given; the moist options
+    par[22] = 1.0                                            # moist air: no vapor or ice given
     assert air_column() == E
-    par[47] = par[111] = 1.0                                 # plume_deep with the cumulus layer
-    assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf,
-                              600.0, *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
-    par[47] = par[111] = 0.0
-    par[114], par[100], par[70] = 1.0, 1.0, 1.0              # cu_sub with the mixed layer's condensation level
-    assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
-                              *([buf] * 8), None, None, *([None] * 10), *([None] * 8), None, 0, None) == E
-    par[114], par[100], par[70] = 0.0, 0.0, 0.0
     # land without its bucket's capacity, soil, ponds or snow
     assert lib.isw_air_column(10, 3, par, ctypes.byref(air), 0, buf, None, buf, buf, buf, buf, buf, buf, buf, 600.0,
                               *([buf] * 8), None, None, None, None, None, buf, buf, buf, None, buf, None, None,
                               *([None] * 8), None, 0, None) == E
     par[22] = 0.0
-    assert air_column(dgo=buf) == E and air_column(xfer=buf) == E and air_column(a=None) == E
+    assert air_column(dgo=buf) == E and air_column(a=None) == E
     assert air_column(t=None) == E and air_column(mode=1) == E and air_column(mode=2, insol=buf) == E
     assert air_column(prec=5) == E and air_column(L=1) == E and air_column(L=9) == E and air_column(dt=0.0) == E
 
@@ -125,31 +114,21 @@ def test_request_draws_are_seeded():
 
 
 def test_column_options_predicate():
-    """The air column's refusal of the options it does not carry, on the host only (no launch): cu_sub is refused
-    exactly where the reference applies it (cu_sub, bl_lcl, bl_mix all on, three or more layers, moist air;
-    kernels/air.cu:1838), cu_dth above three layers, and cu_dth with the deep plume in moist air."""
+    """The air column carries every option of the reference's column (the cumulus layer's cu_dth, cu_dp, cl_float,
+    cu_sub and sub_mix included): the predicate accepts them all in two to eight layers, dry or moist, and refuses only
+    a layer count out of range or a missing parameter array (host only, no launch)."""
     lib = _hip.lib()
     E = _hip.ISW_EBADARG
     par = (ctypes.c_double * 140)()
-    assert lib.isw_air_column_options(3, par) == 0                     # the dry column: nothing refused
-    par[22] = 1.0                                                     # moist
-    assert lib.isw_air_column_options(3, par) == 0
-    for on in range(8):                                               # cu_sub's gates: refused only with all three
-        par[114], par[100], par[70] = (float(on & 1), float(on >> 1 & 1), float(on >> 2 & 1))
-        assert lib.isw_air_column_options(3, par) == (E if on == 7 else 0), on
-        assert lib.isw_air_column_options(2, par) == 0, on            # no mixed layer's cumulus slice in two layers
-    par[114] = par[100] = par[70] = 0.0
-    par[22] = 0.0
-    par[114], par[100], par[70] = 1.0, 1.0, 1.0                       # dry air: cu_sub never acts
-    assert lib.isw_air_column_options(3, par) == 0
-    par[114] = par[100] = par[70] = 0.0
-    par[111] = 1.0                                                    # cu_dth: refused above three layers only
-    assert lib.isw_air_column_options(3, par) == 0 and lib.isw_air_column_options(4, par) == E
-    par[22], par[47] = 1.0, 1.0                                       # and with the deep plume in moist air
-    assert lib.isw_air_column_options(3, par) == E
-    par[111] = 0.0
-    assert lib.isw_air_column_options(3, par) == 0
-    assert lib.isw_air_column_options(1, par) == E and lib.isw_air_column_options(3, None) == E
+    for moist in (0.0, 1.0):
+        par[22] = moist
+        for idx in (111, 112, 114, 117, 131, 47):                   # cu_dth, cu_dp, cu_sub, cl_float, sub_mix, plume_deep
+            par[idx] = 1.0
+        par[100] = par[70] = 1.0                                    # with the mixed layer and its condensation level
+        for L in range(2, 9):
+            assert lib.isw_air_column_options(L, par) == 0, (moist, L)
+    assert lib.isw_air_column_options(1, par) == E and lib.isw_air_column_options(9, par) == E
+    assert lib.isw_air_column_options(3, None) == E
 
 
 def test_grid_check_pair_tables():

@@ -51,7 +51,7 @@ def columns(N, L, seed):
 
 def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None, swf=None, ch=None, firn=None,
                    mixed=False, ocean=False, q=None, ice=None, cw=None, cloudo=False, land=None, soilcap=None,
-                   soil=None, pond=None, snow=None, lc=None):
+                   soil=None, pond=None, snow=None, lc=None, xfer=None):
     from insilica import _hip
     from insilica.radiation import _air
     L, N = c['mu'].shape
@@ -73,6 +73,8 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
         o['soil'], o['pond'], o['snow'] = _dev(soil), _dev(pond), _dev(snow)
         if cap is not None:
             o['cap'] = _dev(cap)
+    if xfer is not None:
+        o['xfer'] = _dev(xfer)
     tl, tsc, tlc = _dev(land), _dev(soilcap), _dev(lc)
     ins = [_dev(x) for x in (np.array([t]), insol, c['pos'], c['uabs'], c['mu'], cap, elev, radf, swf, ch)]
     tt, ti, tp, tu, tm, tc, te, tr, ts, tch = ins
@@ -84,7 +86,7 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
               P(o['sw_abs']), P(o['stored']), P(o['rain']), P(o['evap']), P(o['conv']), P(o['tsurf']),
               P(o['rainp']), None, P(o.get('cap', tc)), P(o.get('qocean')), P(o.get('qsw')), P(tl), P(tsc),
               P(o.get('soil')), P(o.get('pond')), P(o.get('snow')), P(o.get('cw')), P(o.get('cloudo')), P(te),
-              P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), P(tlc), P(o.get('firn')), None,
+              P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), P(tlc), P(o.get('firn')), P(o.get('xfer')),
               1 if mixed else 0, _hip.stream())
     torch.cuda.synchronize()
     return {k: _np(v) for k, v in o.items()}
@@ -100,7 +102,7 @@ def compare(g, o, tol, dt, label, bars=None, outliers=0.0):
     """outliers: the share of columns allowed past the bars (columns on a knife edge of the reference's own
     design, named by the caller), every other column within them."""
     keys = [k for k in ('theta', 'Ts', 'wx', 'olr', 'sw_abs', 'tsurf', 'qocean', 'qsw', 'swalb', 'firn', 'q', 'ice',
-                        'cw', 'cloudo', 'soil', 'pond', 'snow', 'cap') if k in o] + \
+                        'cw', 'cloudo', 'soil', 'pond', 'snow', 'cap', 'xfer') if k in o] + \
         (['rain', 'evap', 'conv', 'rainp'] if 'q' in o else [])
     keys.append('stored')
     ce = {k: cell_errors(g[k], o[k]) for k in keys}
@@ -542,3 +544,49 @@ def test_aqua_dry_world_ktables_and_mixed():
     l2 = float(np.linalg.norm(hb - ha) / np.linalg.norm(ha))
     print('aqua_dry mixed vs FP64 after a day: rel L2 h', l2, 'theta', relmax(_np(b.theta), _np(a.theta)), flush=True)
     assert l2 <= 1e-5
+
+
+@pytest.mark.parametrize('L,mixed,case', [(5, False, 0), (5, False, 1), (5, False, 2), (7, False, 3), (7, False, 4),
+                                          (5, True, 1), (7, True, 3)])
+def test_cumulus_layer(L, mixed, case):
+    """The cumulus layer (cu_dth's floating edges with cu_dp and cl_float, the plumes' air taken in through it moved
+    with them, the shallow plume stopping in it or the cloud layer above, the mixed layer refilled from it with
+    sub_mix; the earth7 column's options together) with the gas tracers carried by the column's moves.
+
+    FP64 bars from the rounding alone: the oracle against itself built with exp, expm1 and pow one ulp off for about
+    half their arguments (a C library that rounds differently, as the device's does) differs on these columns by up
+    to 3.3e-11 in vapor, 4.4e-11 in the gas tracers, 1.4e-11 in rain, 1.6e-11 in the plume mass, 1.3e-10 in cloud
+    water and 3.6e-12 in the masses crossing: the condensate is a difference of the plume's vapor terms, q0 + sum Gl q
+    - G qc - sum qx, and each term carries the ulps of a dozen sub-levels' exponentials. Hence q and xfer 1e-10, rain
+    3e-11, cloud water and its rows 1e-9, wx 1e-11 (the oracle against its perturbed self passes these in every
+    column but 2 of 40962 at the mixed layer's condensation level). The mixed form as the cloudy column's; with
+    bl_lcl the condensation level's knife edge as in the mixed-layer test."""
+    from oracle import column as ocol
+    from test_oracle_air_column import CU_OPTS, cu_par, cumulus_columns
+    L_, opts = CU_OPTS[case]
+    assert L_ == L
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column cumulus L={L} mixed={mixed} case {case} N={N}', flush=True)
+    c = cumulus_columns(N, L, seed=230 + L)
+    cloudy = opts.get('cloud', 0) > 0
+    kw = dict(ocean=True, q=c['q'], ice=c['ice'])
+    if cloudy:
+        kw.update(cw=c['cw'], cloudo=True)
+    if opts.get('sfc'):
+        kw['ch'] = np.random.default_rng(235 + L).uniform(1e-3, 3e-2, N)
+    rng = np.random.default_rng(236 + L)
+    kw['xfer'] = rng.uniform(0.5, 1.5, (3 * L, N)) * np.repeat([400e-6, 0.32e-6, 1.8e-6], L)[:, None]
+    p = cu_par(L, **opts)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, **kw)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed, **kw)
+    assert o['conv'].max() > 0 and np.any(o['xfer'] != kw['xfer'])
+    flash_g, flash_o = g.pop('flash'), o.pop('flash')
+    np.testing.assert_allclose(flash_g, flash_o, rtol=2e-2 if mixed else 1e-10, atol=1e-12 * max(flash_o.max(), 1e-30))
+    g['flash'] = o['flash'] = np.zeros_like(flash_o)
+    lcl = opts.get('bl_lcl', 0) > 0
+    fp64 = dict(q=1e-10, xfer=1e-10, rain=3e-11, rainp=3e-11, cw=1e-9, cloudo=1e-9, wx=1e-11, conv=1e-10)
+    # (the mixed form lifts cu_top's parcel in float32, so the floating edges sit at float32 precision and a few
+    # crossings at them switch: measured 2 of 40962 columns with cu_dp; allowed 0.1%)
+    compare(g, o, 1e-5 if mixed else 1e-12, dt, f'cumulus L={L} mixed={mixed} case {case}',
+            bars=(dict(wx=3e-4, q=3e-5, theta=3e-5, xfer=3e-5) if cloudy else None) if mixed else fp64,
+            outliers=(2e-2 if mixed else 2e-3) if lcl else (1e-3 if mixed else 0.0))

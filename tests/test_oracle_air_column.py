@@ -1089,3 +1089,107 @@ def test_mixed_layer_cover(ml_clear):
         f = np.clip((qt - qsj + b) / (2 * b), 0.0, 1.0).max(0)
     assert 0 < f.mean() < 1
     np.testing.assert_allclose(o['cloudo'][3], f, rtol=1e-12, atol=1e-15)
+
+
+# ---- the cumulus layer (cu_dth: air.cu's floating cumulus layer above the mixed layer, its top at cu_dth above theta*
+# or where cu_dp lifts the mixed layer's air; cl_float: the cloud layer's lower edge above that; plume_out: the plumes'
+# air taken in on the way moved with them; cu_sub: the mixed layer refilled from the cumulus layer; the gas tracers
+# (xfer) carried by every move of the column's air) ----
+
+EARTH7 = dict(moist=1.0, plume_deep=1.0, sat_prof=1.0, bl_bot=0.25, cloud=1.0, q_shape=1.0, freeze_dry=1.0,
+              edge_vap=1.0, plume_root=1.0, plume_ent=1e-5, conv_revap=0.05, cw_qs=0.025, bl_mix=1.0, bl_q=1.0,
+              sfc=1.0, downdraft=0.3, ent_cloud=1e-4, q_sea=0.98, bl_reach=1.0, dry_edge=1.0, bl_lcl=1.0, edge_k=2.0,
+              ice_air=1.0, revap=1.0, root_q=1.0, ent_det=1.0, ent_ifs=1.0, shallow=0.03, cu_dth=4.0, cu_dp=2.0e4,
+              buoy_v=1.0, cu_sub=1.0, det_all=1.0, ml_clear=1.0, cu_cloud=2.0)
+EARTH7_STARS = [500.0, 380.0, 330.0, 305.0, 204.0, 200.0]
+
+
+def cumulus_columns(N, L, seed):
+    """Columns with a mixed layer under a cumulus layer under a cloud layer: seven layers (the earth7 layering's
+    stratosphere, upper troposphere, two tropospheric layers, the cloud layer, the cumulus layer, the mixed layer) or
+    five (the stratosphere, the troposphere above 305 K, the cloud, cumulus and mixed layers), a moist warm bottom,
+    cloud water in most layers."""
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(size=(N, 3))
+    pos /= np.linalg.norm(pos, axis=1)[:, None]
+    if L == 7:
+        share = [0.0, 0.05, 0.13, 0.27, 0.5, 0.78, 0.88, 1.0]
+        th = [560.0, 395.0, 345.0, 318.0, 306.0, 301.5, 299.0]
+        rh = [0.01, 0.05, 0.3, 0.45, 0.7, 0.8, 0.85]
+    else:
+        share = [0.0, 0.12, 0.4, 0.75, 0.87, 1.0]
+        th = [420.0, 340.0, 306.0, 301.5, 299.0]
+        rh = [0.02, 0.35, 0.7, 0.8, 0.85]
+    edges = np.array(share)[:, None] * 1.0e5 * rng.uniform(0.98, 1.02, (1, N))
+    mu = np.diff(edges, axis=0) / GRAV
+    theta = np.array(th)[:, None] + rng.normal(0, 0.6, (L, N))
+    theta[-1] -= 8.0 * pos[:, 2] ** 2                      # (colder toward the poles)
+    pt, pb, _ = pib_of(mu)
+    pm = 0.5 * (pt + pb)
+    q = np.array(rh)[:, None] * rng.uniform(0.9, 1.1, (L, N)) * tetens(theta * (pm / P0) ** KAPPA, pm)
+    cw = np.where(rng.uniform(size=(L, N)) < 0.6, rng.uniform(0, 4e-4, (L, N)), 0.0)
+    Ts = 301.0 - 10.0 * pos[:, 2] ** 2 + rng.normal(0, 0.5, N)
+    return dict(pos=pos, mu=mu, theta=theta, Ts=Ts, uabs=rng.uniform(0, 15, N), q=q, ice=np.zeros(N), cw=cw)
+
+
+def cu_par(L, **kw):
+    from insilica.atmosphere import par_array
+    stars = EARTH7_STARS if L == 7 else [380.0, 305.0, 204.0, 200.0]
+    return par_array(L, dict(dict(theta_stars=stars), **kw))
+
+
+CU_OPTS = [(5, dict(moist=1.0, plume_deep=1.0, cu_dth=4.0, ent_cloud=1e-4, ent_ifs=1.0, bl_mix=1.0)),
+           (5, dict(moist=1.0, plume_deep=1.0, cu_dth=4.0, cu_dp=2.0e4, cl_float=2.0, ent_cloud=1e-4, ent_ifs=1.0,
+                    buoy_v=1.0, cloud=1.0, ice_air=1.0, downdraft=0.3, bl_mix=1.0)),
+           (5, dict(moist=1.0, plume_deep=1.0, cu_dth=4.0, cu_sub=1.0, sub_mix=0.5, bl_mix=1.0, bl_lcl=1.0,
+                    plume_root=1.0, root_q=1.0, sfc=1.0, shallow=0.03, ent_cloud=1e-4, ent_ifs=1.0, cloud=1.0)),
+           (7, EARTH7), (7, dict(EARTH7, cl_float=2.0, sub_mix=0.5))]
+
+
+@pytest.mark.parametrize('L,opts', CU_OPTS)
+@pytest.mark.parametrize('mix', [False, True])
+def test_cumulus_layer_closes(L, opts, mix):
+    """With the cumulus layer (its floating edges, the plumes' air taken in through it moved with them, the shallow
+    plume stopping in it or the cloud layer above, the mixed layer refilled from it) the energy closes, the water
+    changes by evaporation less rain (cloud water included), the mass is only moved between layers, every layer keeps
+    its floor, and the gas tracers carried by the moves keep each gas's column amount (sum mu x) while the cumulus
+    options move air."""
+    c = cumulus_columns(96, L, seed=200 + L)
+    N = c['Ts'].size
+    cloudy = opts.get('cloud', 0) > 0
+    kw = dict(ch=np.full(N, 0.02)) if opts.get('sfc') else {}
+    if cloudy:
+        kw['cw'] = c['cw']
+    rng = np.random.default_rng(7)
+    xf = rng.uniform(0.5, 1.5, (3 * L, N)) * np.repeat([400e-6, 0.32e-6, 1.8e-6], L)[:, None]
+    dt = 900.0
+    o = run(c, cu_par(L, **opts), q=c['q'], ice=c['ice'], dt=dt, mix=mix, xfer=xf, **kw)
+    stored_matches(o, dt)
+    mu1 = new_masses(c, o['wx'])
+    np.testing.assert_allclose(mu1.sum(0), c['mu'].sum(0), rtol=1e-14)
+    assert np.all(mu1 >= 10.0 * (1 - 1e-9))
+    W0 = (c['mu'] * (c['q'] + (c['cw'] if cloudy else 0))).sum(0)
+    W1 = (mu1 * (o['q'] + (o['cw'] if cloudy else 0))).sum(0)
+    np.testing.assert_allclose(W1 - W0, (o['evap'] - o['rain']) * dt, atol=1e-11 * W0.max())
+    assert o['conv'].max() > 0
+    for g in range(3):
+        x0 = (c['mu'] * xf[g * L:(g + 1) * L]).sum(0)
+        x1 = (mu1 * o['xfer'][g * L:(g + 1) * L]).sum(0)
+        np.testing.assert_allclose(x1, x0, rtol=1e-13)
+    assert np.any(o['xfer'] != xf)
+
+
+def test_cumulus_options_act():
+    """Each cumulus option changes the column: the floating edges (cu_dth, cu_dp, cl_float), the cumulus layer's
+    subsidence (cu_sub, sub_mix), against the same columns without it."""
+    c = cumulus_columns(96, 5, seed=211)
+    base = dict(moist=1.0, plume_deep=1.0, ent_cloud=1e-4, ent_ifs=1.0, bl_mix=1.0, bl_lcl=1.0, cloud=1.0)
+    out = lambda **kw: run(c, cu_par(5, **dict(base, **kw)), q=c['q'], ice=c['ice'], cw=c['cw'], dt=900.0)
+    ref = out()
+    steps = [dict(cu_dth=4.0), dict(cu_dth=4.0, cu_dp=2.0e4), dict(cu_dth=4.0, cu_dp=2.0e4, cl_float=2.0),
+             dict(cu_dth=4.0, cu_sub=1.0), dict(cu_dth=4.0, cu_sub=1.0, sub_mix=0.5)]
+    prev = ref
+    for kw in steps:
+        o = out(**kw)
+        assert np.any(o['wx'] != prev['wx']) or np.any(o['theta'] != prev['theta']), kw
+        prev = o
