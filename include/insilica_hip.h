@@ -358,8 +358,10 @@ int32_t isw_sw_column(int32_t N, int32_t L, const double* par, const isw_sw_tabl
  * (kg/m^2/s), tsurf; rainp (4, N: rain from the layers above the bottom one, from the bottom one, the plumes', snow),
  * qocean, qsw, flash, swalb and firn (the firn's temperature) when given; cap (N) the surface's heat capacity, else
  * c_slab. The argument list is the reference's; elev (N) with theta_rise > 0 raises the lowest interface over high
- * ground. Returns ISW_EBADARG for a non-NULL dgo (the diagnostics rows are not in this library), and for moist air
- * without qv or ice, or with land but without soilcap, soil, pond or snow. bl_q (the mixed layer's even vapor and its
+ * ground. dgo (5 L - 2, N, nullable): the diagnostics rows (air.cu:1286-1290, 1944-1950): each layer's heating Q and
+ * latent heating (W/m^2), each interface's gross up and down dry crossing (kg/m^2 per step, before netting and
+ * limiting), each layer's convective mass delivered into it. Returns ISW_EBADARG for moist air without qv or ice, or
+ * with land but without soilcap, soil, pond or snow. bl_q (the mixed layer's even vapor and its
  * cloud statistics; ml_clear, cu_cloud), the deep plume's IFS entrainment and detrainment (ent_ifs, ent_loc, det_all),
  * virtual buoyancy (buoy_v), the shallow plume (shallow, rooted by plume_root with root_q) and the cumulus layer
  * (cu_dth's floating edges with cu_dp and cl_float, the plumes' air taken in through it moved with them, cu_sub's

@@ -49,7 +49,8 @@ int air_column(int, int, const double*, const isw_air*, int, const double*, cons
                const double*, const double*, double*, double*, double*, double*, double, double*, double*, double*,
                double*, double*, double*, double*, double*, double*, double*, double*, double*, const double*,
                const double*, double*, double*, double*, const double*, double*, const double*, const double*,
-               double*, const double*, const double*, double*, double*, double*, double*, int, hipStream_t);
+               double*, const double*, const double*, double*, double*, double*, double*, double*, int,
+               hipStream_t);
 }  // namespace isw_col
 
 namespace isw_ocn {
@@ -325,13 +326,13 @@ int32_t isw_air_column(int32_t N, int32_t L, const double* par, const isw_air* a
       !Ts || !wx || !olr || !sw_abs || !stored || !rain || !evap || !conv || !tsurf || !(dt > 0) || !prec_ok(prec))
     return ISW_EBADARG;
   if (mode == 0 ? (!t || !pos) : !insol) return ISW_EBADARG;
-  if (!column_options_ok(L, par) || dgo) return ISW_EBADARG;
+  if (!column_options_ok(L, par)) return ISW_EBADARG;
   // moist air needs its vapor and ice; land (moist) its bucket's capacity, soil, ponds and snow
   if (par[22] > 0 && (!qv || !ice || (land && (!soilcap || !soil || !pond || !snow)))) return ISW_EBADARG;
   return isw_col::air_column(N, L, par, air, mode, t, insol, pos, uabs, mu, theta, qv, Ts, ice, dt, wx, olr, sw_abs,
                              stored, rain, evap, conv, tsurf, rainp, cap, qocean, qsw, land, soilcap, soil, pond, snow,
-                             elev, flash, radf, swf, swalb, ch, lc, firn, cloudw, cloudo, xfer, prec == ISW_MIXED,
-                             S(stream));
+                             elev, flash, radf, swf, swalb, ch, lc, firn, cloudw, cloudo, xfer, dgo,
+                             prec == ISW_MIXED, S(stream));
 }
 
 int32_t isw_surface_layer(int32_t N, int32_t L, const double* par, const isw_air* air, const double* uabs,

@@ -484,7 +484,8 @@ __device__ void move_mass(const CP& P, double* mu, double* th, double* q, double
 // air takes its cloud water cl along.
 template <int L, bool MOIST = false, bool CL = false, bool MIX = false>
 __device__ void crossing(const CP& P, double* mu, double* th, double* q, const double* dth, const double* et,
-                         const double* eb, double* pib, double* wxo, double* cl = nullptr, const AP* A = nullptr) {
+                         const double* eb, double* pib, double* wxo, double* cl = nullptr, const AP* A = nullptr,
+                         double* gross = nullptr, size_t gstride = 0) {
 #pragma unroll
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (P.theta_star <= 0) return;
@@ -511,7 +512,7 @@ __device__ void crossing(const CP& P, double* mu, double* th, double* q, const d
   for (int j = 0; j + 1 < L; j++) {
     const int a = j, b = j + 1;
     const double ts_ = tstar<L>(j, P);
-    double dm = 0, thc = ts_, thd = ts_;
+    double dm = 0, thc = ts_, thd = ts_, gdown = 0;   // (gdown: the gross down part, for the diagnostics rows)
     if (dth[b] > 0 && blm && b == L - 1) {
       if (th[b] >= ts_) {
         double fa, fb;
@@ -540,6 +541,7 @@ __device__ void crossing(const CP& P, double* mu, double* th, double* q, const d
       const double down = mu[a] - P.floor_;
       thd = th[a];
       dm -= down;
+      gdown = down;
     } else if (da < 0) {
       double down;
       if (th[a] <= ts_) {
@@ -553,6 +555,11 @@ __device__ void crossing(const CP& P, double* mu, double* th, double* q, const d
         down = mu[a] * (-da) * fa;
       }
       dm -= down;
+      gdown = down;
+    }
+    if (gross) {
+      gross[(size_t)(2 * j) * gstride] = dm + gdown;
+      gross[(size_t)(2 * j + 1) * gstride] = gdown;
     }
     dmj[j] = dm;
     thcj[j] = dm > 0 ? thc : thd;
@@ -1220,6 +1227,9 @@ struct AirIO {
   const double *land, *soilcap, *lc;                   // the land mask (> 0.5 land), the bucket's capacity, the cover
   double *soil, *pond, *snow;
   double* xfer;                                        // (3 L, N) the gas tracers' mole fractions, nullable
+  // (5 L - 2, N) the diagnostics rows, nullable: heating and latent heating per layer (W/m^2), the crossing's gross
+  // up and down per interface, the convective mass into each layer
+  double* dgo;
 };
 
 template <int L, bool MIX, bool MOIST, bool CLOUD, bool FULL>
@@ -1260,6 +1270,15 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
   if (blm && L > 2) P.theta_star = fmin(fmax(ts_base, th[L - 1]), th[L - 2] - 1);
   const double ths_cl = L > 4 ? P.ths[L > 4 ? L - 4 : 0] : 0.0;   // (cl_float: the cloud layer's fixed lower edge)
   if constexpr (FULL) cu_edges<L, MIX>(mu, th, q, P, A, ths_cl);
+  // (the diagnostics rows: each accumulates in the caller's array at this column, in the oracle's order)
+  if (o.dgo)
+    for (int r = 0; r < 5 * L - 2; r++) o.dgo[(size_t)r * N + i] = 0;
+  auto dgl = [&](int k, double v) {
+    if (o.dgo) o.dgo[(size_t)(L + k) * N + i] += v;
+  };
+  auto cin = [&](int k, double v) {
+    if (o.dgo) o.dgo[(size_t)(4 * L - 2 + k) * N + i] += v;
+  };
   // (the column's gas tracers, carried by every move of its air)
   double xfl[3 * L];
   if (o.xfer) {
@@ -1752,6 +1771,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       }
       dth[k] += P.Lv * dl / pib[k];
       dlat[k] += P.Lv * dl / pib[k];
+      dgl(k, P.Lv * mu[k] * dl / dt);
       layer_cap_k<L, MIX>(k, th[k], pt, pbk, P, A, qs, dqs);
       if (!mixed_cloud<L, MIX>(k, mu, th[k], q[k] + cl[k], P, A, l, f, dqm, false)) cloud_pdf(q[k] + cl[k], qs, A, l, f);
       f = fmax(f, cover_of(cl[k], mu[k], liq[k], A, qs));
@@ -1767,11 +1787,13 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         th[k] += A.Lf * dz / pib[k];
         dth[k] += A.Lf * dz / pib[k];
         dlat[k] += A.Lf * dz / pib[k];
+        dgl(k, A.Lf * mu[k] * dz / dt);
         if (Fs > 0 && base_T<L, MIX>(k, th[k], pbk, P, A) > A.Tf) {          // (the layer's lower edge above freezing)
           const double dm = Fs * dt / (mu[k] * pib[k]) * A.Lf;
           th[k] -= dm;
           dth[k] -= dm;
           dlat[k] -= dm;
+          dgl(k, -(A.Lf * Fs));
           Fs = 0;
         }
       }
@@ -1794,6 +1816,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         th[k] -= Le * dt / (mu[k] * pib[k]);
         dth[k] -= Le * dt / (mu[k] * pib[k]);
         dlat[k] -= Le * dt / (mu[k] * pib[k]);
+        dgl(k, -Le);
       }
       if (k == bot) Rk1 += mu[k] * loss - ev * dt;
       else Rk0 += mu[k] * loss - ev * dt;
@@ -1812,6 +1835,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
       q[k] = qk;
       dth[k] += P.Lv * dq / pib[k];
       dlat[k] += P.Lv * dq / pib[k];
+      dgl(k, P.Lv * mu[k] * dq / dt);
       R += mu[k] * dq;
       if (k == bot) Rk1 += mu[k] * dq;
       else Rk0 += mu[k] * dq;
@@ -1856,7 +1880,8 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     mu_lcl = lcl_mass<L, MIX>(mu, th[bot], q[bot], P, A);
     P.bl_room = fmax(0.0, mu_lcl - mu[bot]);
   }
-  crossing<L, MOIST, CLOUD, MIX>(P, mu, th, q, dth, et, eb, pib, wxo, cl, &A);
+  crossing<L, MOIST, CLOUD, MIX>(P, mu, th, q, dth, et, eb, pib, wxo, cl, &A,
+                                 o.dgo ? o.dgo + (size_t)2 * L * N + i : nullptr, (size_t)N);
   // the mixed boundary layer: the surface's buoyancy flux sets its depth. Rising it entrains the air above at bl_ent
   // of that flux across the jump to it (Tennekes 1973), a slice from the layer's bottom edge; falling, it gives its
   // top back down to a thin night layer mu_night. bl_reach: dry convection first takes in the air above colder than
@@ -2040,12 +2065,14 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 #pragma unroll
             for (int k = 0; k < L; k++)
               if (k == kd) th[k] += A.Lf * Sc / (mu[k] * pib[k]);
+            dgl(kd, A.Lf * Sc / dt);
             double ptj = 0;
 #pragma unroll
             for (int j = 0; j < L; j++) {
               const double pbj = ptj + P.grav * mu[j];
               if (j > kd && Sc > 0 && base_T<L, MIX>(j, th[j], pbj, P, A) > A.Tf) {
                 th[j] -= A.Lf * Sc / (mu[j] * pib[j]);
+                dgl(j, -(A.Lf * Sc / dt));
                 Sc = 0;
               }
               ptj = pbj;
@@ -2058,6 +2085,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
               if (j >= kd && j < bot - 1) wxo[j] += me;
             }
           }
+          cin(kd, mc + me);
           if constexpr (CLOUD) {
 #pragma unroll
             for (int k = 0; k < L; k++)
@@ -2098,6 +2126,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
                 const double e_ = md * fmax(0.0, ev);
                 R -= e_;
                 Rc -= e_;
+                dgl(bot, -(P.Lv * e_ / dt));
               }
             };
             bool lower = false;
@@ -2161,6 +2190,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 #pragma unroll
         for (int j = 0; j < L - 1; j++)
           if (j >= kd) wxo[j] += mc;
+        cin(kd, mc);
         const double rt = [&] {
           double r = 0;
 #pragma unroll
@@ -2217,12 +2247,14 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
 #pragma unroll
               for (int k = 0; k < L; k++)
                 if (k == kds) th[k] += A.Lf * Ss / (mu[k] * pib[k]);
+              dgl(kds, A.Lf * Ss / dt);
               double ptj = 0;
 #pragma unroll
               for (int j = 0; j < L; j++) {
                 const double pbj = ptj + P.grav * mu[j];
                 if (j > kds && Ss > 0 && base_T<L, MIX>(j, th[j], pbj, P, A) > A.Tf) {
                   th[j] -= A.Lf * Ss / (mu[j] * pib[j]);
+                  dgl(j, -(A.Lf * Ss / dt));
                   Ss = 0;
                 }
                 ptj = pbj;
@@ -2230,6 +2262,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
               S += Ss;
               Scg += Ss;
             }
+            cin(kds, ms * Gx);
             if constexpr (CLOUD) {
 #pragma unroll
               for (int k = 0; k < L; k++)
@@ -2266,16 +2299,20 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
           if (A.ice_air > 0 && cond > det) {           // (its frozen share snows, melting in a warm bottom layer)
             const double Ss = rs * fmin(1.0, FZs / cond);
             th[ks] += A.Lf * Ss / (mu[ks] * pib[ks]);
+            dgl(ks, A.Lf * Ss / dt);
             double ps_ = 0;
 #pragma unroll
             for (int k = 0; k < L; k++) ps_ += P.grav * mu[k];
-            if (base_T<L, MIX>(bot, th[bot], ps_, P, A) > A.Tf) th[bot] -= A.Lf * Ss / (mu[bot] * pib[bot]);
-            else {
+            if (base_T<L, MIX>(bot, th[bot], ps_, P, A) > A.Tf) {
+              th[bot] -= A.Lf * Ss / (mu[bot] * pib[bot]);
+              dgl(bot, -(A.Lf * Ss / dt));
+            } else {
               S += Ss;
               Scg += Ss;
             }
           }
           wxo[ks] += ms;
+          cin(ks, ms);
           if constexpr (CLOUD) {
             cl[ks] += ms * det / mu[ks];
             cloud_adjust_k<L, MIX>(ks, mu, th, q, cl, pib, dt, P, A);
@@ -2331,6 +2368,7 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
         }
         q[bot] += ev * dt / mu[bot];
         th[bot] -= P.Lv * ev * dt / (mu[bot] * pib[bot]);
+        dgl(bot, -(P.Lv * ev));
         R -= ev * dt;
         Rc -= ev * dt;
       }
@@ -2343,6 +2381,10 @@ __global__ void __launch_bounds__(BS) k_air_column(int N, CP P, AP A, int mode, 
     for (int g = 0; g < 3; g++)
 #pragma unroll
       for (int k = 0; k < L; k++) o.xfer[((size_t)g * L + k) * N + i] = xfl[g * L + k];
+  if (o.dgo) {
+#pragma unroll
+    for (int k = 0; k < L; k++) o.dgo[(size_t)k * N + i] = Q[k];
+  }
   o.rain[i] = R / dt;
   if (!isl && S > 0) {                               // snow into the sea melts with the slab's heat
     Ts[i] -= A.Lf * S / Cw;
@@ -2523,13 +2565,13 @@ int air_column(int N, int L, const double* par, const isw_air* air, int mode, co
                double* qsw, const double* land, const double* soilcap, double* soil, double* pond, double* snow,
                const double* elev, double* flash, const double* radf, const double* swf, double* swalb,
                const double* ch, const double* lc, double* firn, double* cloudw, double* cloudo, double* xfer,
-               int mixed, hipStream_t st) {
+               double* dgo, int mixed, hipStream_t st) {
   const CP P = make_cp(par);
   const AP A = make_ap(par, air);
   const bool moist = P.moist > 0, cloudy = moist && par[55] > 0 && cloudw;
   if (!moist) land = nullptr;                          // (the dry column has no land)
   const AirIO o{theta, qv, Ts, ice, wx, olr, sw_abs, stored, rain, evap, conv, tsurf, rainp, qocean, qsw, flash,
-                swalb, firn, cloudw, cloudo, cap, land, soilcap, lc, soil, pond, snow, xfer};
+                swalb, firn, cloudw, cloudo, cap, land, soilcap, lc, soil, pond, snow, xfer, dgo};
   const int RW = L + 1;
   const double rows_in = 1 + 2.0 * L + 1 + (mode == 0 ? 3 : 1) + (cap ? 1 : 0) + (elev ? 1 : 0) + (ch ? 1 : 0) +
                          (radf ? 3.0 * RW + 6 * L : 0) + (swf ? 3.0 * L + 1 : 0) + (firn ? 1 : 0) +
@@ -2538,7 +2580,7 @@ int air_column(int N, int L, const double* par, const isw_air* air, int mode, co
                           (swalb ? 1 : 0) + (firn ? 1 : 0) + (moist ? L + 1.0 : 0) + (cloudy ? L : 0) +
                           (cloudy && cloudo ? (radf ? 4.0 * L + 3 : L + 3.0) : 0) + (land ? 3 : 0) +
                           (land && cap ? 1 : 0);
-  const double bytes = 8.0 * N * (rows_in + rows_out + (xfer ? 6.0 * L : 0));
+  const double bytes = 8.0 * N * (rows_in + rows_out + (xfer ? 6.0 * L : 0) + (dgo ? 5.0 * L - 2 : 0));
   // the moist column's options that take runtime branches of their own: with none of them on it runs the plain
   // kernel (FULL false), their code compiled away (the dry column is always the full one: bl_mix acts there too)
   const bool full = land || P.bl_mix > 0 || par[51] > 0 || par[53] > 0 || par[86] > 0 || par[85] > 0 ||

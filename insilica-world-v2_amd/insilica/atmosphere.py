@@ -128,6 +128,7 @@ class AirPhysics:
         self.uabs, self.wx, self.qnet, self.stored = z(N), z(max(layers - 1, 1), N), z(N), z(N)
         self.olr, self.sw_abs, self.rain, self.evap, self.conv, self.tsurf = (z(N) for _ in range(6))
         self.rainp, self.flash = z(4, N), z(N)
+        self.dgo = None                     # the diagnostics rows ((5 L - 2, N), air_column), set by a caller
         self.insol = None
         if mean_sun:
             lat = np.arcsin(np.clip(np.asarray(dg.host.pos)[:, 2], -1, 1))
@@ -209,6 +210,9 @@ class AirPhysics:
             raise ValueError('the moist column needs the vapor q and the sea ice')
         if gx is not None and (not gx.is_contiguous() or tuple(gx.shape) != (3, self.L, self.dg.N)):
             raise ValueError('gx: a contiguous (3, L, N) array of the gas tracers per layer')
+        if self.dgo is not None and (not self.dgo.is_contiguous() or
+                                     tuple(self.dgo.shape) != (5 * self.L - 2, self.dg.N)):
+            raise ValueError('dgo: a contiguous (5 L - 2, N) array for the diagnostics rows')
         self._speed(u)
         P = _hip.ptr
         lw, sw = self.lw, self.sw
@@ -216,7 +220,7 @@ class AirPhysics:
                   P(self.insol), P(self.pos), P(self.uabs), P(mu), P(theta), P(q) if self.moist else None, P(Ts),
                   P(ice) if self.moist else None, float(dt),
                   P(self.wx), P(self.olr), P(self.sw_abs), P(self.stored), P(self.rain), P(self.evap), P(self.conv),
-                  P(self.tsurf), P(self.rainp), None, P(cap), P(qocean), P(qsw), P(land), P(soilcap), P(soil), P(pond),
+                  P(self.tsurf), P(self.rainp), P(self.dgo), P(cap), P(qocean), P(qsw), P(land), P(soilcap), P(soil), P(pond),
                   P(snow), P(cw), P(self.cloudo) if cw is not None else None, P(elev), P(self.flash),
                   P(lw.radf) if lw is not None else None,
                   P(sw.swf) if sw is not None else None, P(self.swalb) if sw is not None else None,

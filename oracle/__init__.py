@@ -79,7 +79,7 @@ def lib():
             'og_cell_speed': (None, [i, vp, vp, vp, vp, vp, vp]),
             'og_cell_speed_perot': (None, [i, vp, vp, vp, vp]),
             'og_gas_src': (None, [i, i, i, vp, vp, d, vp]),
-            'og_air_column': (None, [i, i, vp, vp, i] + [vp] * 9 + [d] + [vp] * 28 + [i]),
+            'og_air_column': (None, [i, i, vp, vp, i] + [vp] * 9 + [d] + [vp] * 29 + [i]),
             'og_plume_ascent': (None, [i, vp, vp, vp, vp, vp, d, d, d, i, i, vp, vp, vp]),
             'og_sun_lon': (d, [d, vp, vp]),
         }

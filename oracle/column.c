@@ -144,7 +144,7 @@ static double fpow_(double x, double y, int mix);
 
 static void crossing(int L, const double* par, double* mu, double* th, double* q, const double* dth, double dth_up,
                      double* pib, double* wxo, const double* dtop, const double* dbot, double* cl, double bl_room,
-                     double eps, int mix) {
+                     double eps, int mix, double* gross) {
   const double floor_ = par[P_FLOOR];
   for (int j = 0; j + 1 < L; j++) wxo[j] = 0;
   if (par[P_THETA_STAR] <= 0) return;
@@ -164,6 +164,7 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
     const double ts_ = tstar(j, L, par);
     double dm = 0, thc = ts_, thd = ts_;
     const int blm = par[P_BL_MIX] > 0;
+    if (gross) gross[2 * j] = gross[2 * j + 1] = 0;
     if (dth[b] > 0 && blm && b == L - 1) {
       if (th[b] >= ts_) {                              /* the mixed bottom layer takes in the air above */
         double fa, fb;
@@ -192,6 +193,7 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
       const double down = mu[a] - floor_;
       thd = th[a];
       dm -= down;
+      if (gross) gross[2 * j + 1] = down;
     } else if (da < 0) {
       double down;
       if (th[a] <= ts_) {
@@ -205,7 +207,9 @@ static void crossing(int L, const double* par, double* mu, double* th, double* q
         down = mu[a] * (-da) * fa;
       }
       dm -= down;
+      if (gross) gross[2 * j + 1] = down;
     }
+    if (gross) gross[2 * j] = dm + gross[2 * j + 1];   /* (the up and down parts before netting and limiting) */
     dmj[j] = dm;
     thcj[j] = dm > 0 ? thc : thd;
   }
@@ -297,7 +301,7 @@ void og_hs_column(int N, int L, const double* par, const double* hs, const doubl
       const double ki = hs[0] + (hs[1] - hs[0]) * (si > 0.7 ? (si - 0.7) / 0.3 : 0.0) * c2 * c2;
       dth_up = (thi - par[P_THETA_STAR]) * (1 - exp(-ki * dt));
     }
-    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL, NULL, 1e30, 0.622, 0);   /* (dry: no vapor moves) */
+    crossing(L, par, mu, th, q, dth, dth_up, pib, wxo, NULL, NULL, NULL, 1e30, 0.622, 0, NULL);   /* (dry: no vapor moves) */
     for (int j = 0; j + 1 < L; j++) wx[(size_t)j * N + i] = wxo[j];
     double E1 = 0;
     for (int k = 0; k < L; k++) {
@@ -1075,7 +1079,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
                    double* cap, double* qocean, double* qsw, const double* elev, double* flash,
                    const double* radf, const double* swf, double* swalb, const double* ch, double* firn,
                    double* cloudw, double* cloudo, const double* land, const double* soilcap, double* soil,
-                   double* pond, double* snow, const double* lc, double* xfer, int mix) {
+                   double* pond, double* snow, const double* lc, double* xfer, double* dgo, int mix) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++) {
     double par[132 + OG_LMAX];
@@ -1468,6 +1472,10 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     evap[i] = E;
     tsurf[i] = tsk;
     double dth[OG_LMAX], dlat[OG_LMAX];
+    /* the diagnostics (dgo, air.cu:1286-1290): each layer's latent heating (W/m^2), the crossing's gross up and down
+     * mass per interface, the convective mass delivered into each layer */
+    double dgl[OG_LMAX], gross[2 * OG_LMAX], cin[OG_LMAX];
+    for (int k = 0; k < L; k++) dgl[k] = cin[k] = gross[2 * k] = gross[2 * k + 1] = 0;
     for (int k = 0; k < L; k++) {
       dth[k] = Q[k] * dt / (mu[k] * pib[k]);
       th[k] += dth[k];
@@ -1521,6 +1529,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         const double dl = cloud_adjust(L, k, mu, th, q, cl, pib, dt, par, aeps, mix);
         dth[k] += Lv * dl / pib[k];
         dlat[k] += Lv * dl / pib[k];
+        dgl[k] += Lv * mu[k] * dl / dt;
         double qs, dqs, l, f;
         layer_cap(L, k, mu, th[k], par, aeps, mix, &qs, &dqs);
         double dqf;
@@ -1541,11 +1550,13 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           th[k] += Lf * dz / pib[k];
           dth[k] += Lf * dz / pib[k];
           dlat[k] += Lf * dz / pib[k];
+          dgl[k] += Lf * mu[k] * dz / dt;
           if (Fs > 0 && base_T(L, k, mu, th, par, mix) > Tf) {
             const double dm = Fs * dt / (mu[k] * pib[k]) * Lf;
             th[k] -= dm;
             dth[k] -= dm;
             dlat[k] -= dm;
+            dgl[k] -= Lf * Fs;
             Fs = 0;
           }
         }
@@ -1568,6 +1579,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           th[k] -= Le * dt / (mu[k] * pib[k]);
           dth[k] -= Le * dt / (mu[k] * pib[k]);
           dlat[k] -= Le * dt / (mu[k] * pib[k]);
+          dgl[k] -= Le;
         }
         Rk[k == bot ? 1 : 0] += mu[k] * loss - ev * dt;
       }
@@ -1578,6 +1590,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         const double dq = condense_k(L, k, mu, th, q, pib, par, aeps, mix);
         dth[k] += Lv * dq / pib[k];
         dlat[k] += Lv * dq / pib[k];
+        dgl[k] += Lv * mu[k] * dq / dt;
         R += mu[k] * dq;
         Rk[k == bot ? 1 : 0] += mu[k] * dq;
       }
@@ -1614,7 +1627,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
     }
     crossing(L, par, mu, th, q, dth, dth[top], pib, wxo,
              radf ? (edges ? dtop : (L > 1 ? dtop1 : NULL)) : NULL, edges && radf ? dbot : NULL, cloudy ? cl : NULL,
-             bl_room, aeps, mix);
+             bl_room, aeps, mix, gross);
     /* the mixed boundary layer: the surface's buoyancy flux sets its depth. Rising it entrains the air above at bl_ent
      * of that flux across the jump to it (Tennekes 1973), a slice from the layer's bottom edge; falling, it gives its
      * top back down to a thin night layer mu_night. bl_reach: dry convection first takes in the air above colder
@@ -1774,9 +1787,11 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
             if (par[P_ICE_AIR] > 0 && cond > 0 && cond > det) {   /* (its frozen share snows, melting below Tf) */
               Sc = mc * (cond - det) * fmin(1.0, FZ / cond);
               th[kd] += Lf * Sc / (mu[kd] * pib[kd]);
+              dgl[kd] += Lf * Sc / dt;
               for (int j = kd + 1; j <= bot && Sc > 0; j++)
                 if (base_T(L, j, mu, th, par, mix) > Tf) {
                   th[j] -= Lf * Sc / (mu[j] * pib[j]);
+                  dgl[j] -= Lf * Sc / dt;
                   Sc = 0;
                 }
             }
@@ -1784,6 +1799,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
               for (int j = kd; j < bot; j++) wxo[j] += mc;
               for (int j = kd; j < bot - 1; j++) wxo[j] += me;
             }
+            cin[kd] += mc + me;
             if (cloudy) {
               cl[kd] += mc * det / mu[kd];
               cloud_adjust(L, kd, mu, th, q, cl, pib, dt, par, aeps, mix);
@@ -1822,6 +1838,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
                 const double e_ = md * fmax(0.0, ev);
                 R -= e_;
                 Rc -= e_;
+                dgl[bot] -= Lv * e_ / dt;
               }
             }
             S += Sc;
@@ -1856,6 +1873,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
           Rc = mc * (q0 - qc);
           move_mass(L, par, mu, th, q, pib, bot, kd, mc, th0, thp, q0, qc, NULL);
           for (int j = kd; j < bot; j++) wxo[j] += mc;
+          cin[kd] += mc;
           const double rt = mu[kd] * condense_k(L, kd, mu, th, q, pib, par, aeps, mix);
           R += rt;
           Rc += rt;
@@ -1902,14 +1920,17 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
               if (par[P_ICE_AIR] > 0 && cond > det) {    /* (its frozen share snows, melting below where warm) */
                 double Ss = rs * fmin(1.0, FZs / cond);
                 th[kds] += Lf * Ss / (mu[kds] * pib[kds]);
+                dgl[kds] += Lf * Ss / dt;
                 for (int j = kds + 1; j <= bot && Ss > 0; j++)
                   if (base_T(L, j, mu, th, par, mix) > Tf) {
                     th[j] -= Lf * Ss / (mu[j] * pib[j]);
+                    dgl[j] -= Lf * Ss / dt;
                     Ss = 0;
                   }
                 S += Ss;
                 Scg += Ss;
               }
+              cin[kds] += ms * Gx;
               if (cloudy) {
                 cl[kds] += ms * det / mu[kds];
                 cloud_adjust(L, kds, mu, th, q, cl, pib, dt, par, aeps, mix);
@@ -1943,13 +1964,17 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
             if (par[P_ICE_AIR] > 0 && cond > det) {      /* (its frozen share snows, melting in a warm bottom layer) */
               const double Ss = rs * fmin(1.0, FZs / cond);
               th[kds] += Lf * Ss / (mu[kds] * pib[kds]);
-              if (base_T(L, bot, mu, th, par, mix) > Tf) th[bot] -= Lf * Ss / (mu[bot] * pib[bot]);
-              else {
+              dgl[kds] += Lf * Ss / dt;
+              if (base_T(L, bot, mu, th, par, mix) > Tf) {
+                th[bot] -= Lf * Ss / (mu[bot] * pib[bot]);
+                dgl[bot] -= Lf * Ss / dt;
+              } else {
                 S += Ss;
                 Scg += Ss;
               }
             }
             wxo[kds] += ms;
+            cin[kds] += ms;
             if (cloudy) {
               cl[kds] += ms * det / mu[kds];
               cloud_adjust(L, kds, mu, th, q, cl, pib, dt, par, aeps, mix);
@@ -2002,6 +2027,7 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
         }
         q[bot] += ev * dt / mu[bot];
         th[bot] -= Lv * ev * dt / (mu[bot] * pib[bot]);
+        dgl[bot] -= Lv * ev;
         R -= ev * dt;
         Rc -= ev * dt;
       }
@@ -2012,6 +2038,17 @@ void og_air_column(int N, int L, const double* par_in, const og_air* air, int mo
       for (int g = 0; g < OG_GASN; g++)
         for (int k = 0; k < L; k++) xfer[((size_t)g * L + k) * N + i] = xfl[g * OG_LMAX + k];
     xf_col = NULL;
+    if (dgo) {                                       /* rows: Q (L), latent heating (L), gross up and down (2 (L - 1)),
+                                                      * convective mass (L) */
+      int r = 0;
+      for (int k = 0; k < L; k++) dgo[(size_t)(r++) * N + i] = Q[k];
+      for (int k = 0; k < L; k++) dgo[(size_t)(r++) * N + i] = dgl[k];
+      for (int j = 0; j + 1 < L; j++) {
+        dgo[(size_t)(r++) * N + i] = gross[2 * j];
+        dgo[(size_t)(r++) * N + i] = gross[2 * j + 1];
+      }
+      for (int k = 0; k < L; k++) dgo[(size_t)(r++) * N + i] = cin[k];
+    }
     rain[i] = R / dt;
     if (!isl && S > 0) {                             /* snow into the sea melts with the slab's heat */
       Ts[i] -= Lf * S / Cw;

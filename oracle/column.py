@@ -59,13 +59,15 @@ def gas_src(src, h, dt, x, L):
 
 def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None, elev=None, radf=None, swf=None,
                ch=None, firn=None, air=None, mix=False, ocean=False, q=None, ice=None, cw=None, cloudo=False,
-               land=None, soilcap=None, soil=None, pond=None, snow=None, lc=None, xfer=None):
+               land=None, soilcap=None, soil=None, pond=None, snow=None, lc=None, xfer=None, dgo=False):
     """air.cu air_column (the dry column; the moist one, par's moist > 0, with q (L, N) and ice (N); clouds with cloud
     > 0 and the cloud water cw (L, N)): returns a dict of theta, Ts, wx, olr, sw_abs, stored, rain, evap, conv, tsurf,
     rainp, flash, swalb (and qocean, qsw with ocean=True; firn when given; q and ice when moist; cw, and cloudo
     ((4L + 3), N) with cloudo=True, with clouds; over land (moist, the mask land (N) > 0.5, soilcap (N)) soil, pond and
     snow (N), and cap when given (c_wet rewrites it); lc (8, N) the land cover's rows; xfer (3 L, N) the gas
-    tracers' mole fractions per layer, carried by the column's moves of air). mu is read only."""
+    tracers' mole fractions per layer, carried by the column's moves of air; dgo=True: the diagnostics rows (5 L - 2,
+    N): each layer's heating Q and latent heating (W/m^2), each interface's gross up and down crossing, each layer's
+    convective mass). mu is read only."""
     par, t, pos, uabs, mu, insol, elev, radf, swf, ch, land, soilcap, lc = (
         _d(x) for x in (par, t, pos, uabs, mu, insol, elev, radf, swf, ch, land, soilcap, lc))
     cap, soil, pond, snow = (None if x is None else _d(x).copy() for x in (cap, soil, pond, snow))
@@ -83,12 +85,14 @@ def air_column(par, mode, t, pos, uabs, mu, theta, Ts, dt, insol=None, cap=None,
         o['qocean'], o['qsw'] = z(N), z(N)
     if cw is not None and cloudo:
         o['cloudo'] = z(4 * L + 3, N)
+    if dgo:
+        o['dgo'] = z(5 * L - 2, N)
     lib().og_air_column(N, L, _p(par), ctypes.byref(air_struct(air)), int(mode), _p(t), _p(insol), _p(pos), _p(uabs),
                         _p(mu), _p(theta), _p(q), _p(Ts), _p(ice), float(dt), _p(o['wx']), _p(o['olr']), _p(o['sw_abs']),
                         _p(o['stored']), _p(o['rain']), _p(o['evap']), _p(o['conv']), _p(o['tsurf']), _p(o['rainp']),
                         _p(cap), _p(o.get('qocean')), _p(o.get('qsw')), _p(elev), _p(o['flash']), _p(radf), _p(swf),
                         _p(o['swalb']), _p(ch), _p(firn), _p(cw), _p(o.get('cloudo')), _p(land), _p(soilcap),
-                        _p(soil), _p(pond), _p(snow), _p(lc), _p(xfer), int(bool(mix)))
+                        _p(soil), _p(pond), _p(snow), _p(lc), _p(xfer), _p(o.get('dgo')), int(bool(mix)))
     if xfer is not None:
         o['xfer'] = xfer
     if firn is not None:

@@ -129,7 +129,7 @@ void og_air_column(int N, int L, const double* par, const og_air* air, int mode,
                    double* qsw, const double* elev, double* flash, const double* radf, const double* swf,
                    double* swalb, const double* ch, double* firn, double* cloudw, double* cloudo, const double* land,
                    const double* soilcap, double* soil, double* pond, double* snow, const double* lc, double* xfer,
-                   int mix);
+                   double* dgo, int mix);
 double og_sun_lon(double t, const double* par, double* fd);
 
 /* ---- the ocean's column processes (ocean.c; reference kernels/ocean.cu) ---- */

@@ -100,7 +100,7 @@ def test_bad_arguments_rejected_on_host():
                               *([buf] * 8), None, None, None, None, None, buf, buf, buf, None, buf, None, None,
                               *([None] * 8), None, 0, None) == E
     par[22] = 0.0
-    assert air_column(dgo=buf) == E and air_column(a=None) == E
+    assert air_column(a=None) == E
     assert air_column(t=None) == E and air_column(mode=1) == E and air_column(mode=2, insol=buf) == E
     assert air_column(prec=5) == E and air_column(L=1) == E and air_column(L=9) == E and air_column(dt=0.0) == E
 

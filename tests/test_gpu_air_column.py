@@ -51,7 +51,7 @@ def columns(N, L, seed):
 
 def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None, swf=None, ch=None, firn=None,
                    mixed=False, ocean=False, q=None, ice=None, cw=None, cloudo=False, land=None, soilcap=None,
-                   soil=None, pond=None, snow=None, lc=None, xfer=None):
+                   soil=None, pond=None, snow=None, lc=None, xfer=None, dgo=False):
     from insilica import _hip
     from insilica.radiation import _air
     L, N = c['mu'].shape
@@ -75,6 +75,8 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
             o['cap'] = _dev(cap)
     if xfer is not None:
         o['xfer'] = _dev(xfer)
+    if dgo:
+        o['dgo'] = torch.full((5 * L - 2, N), np.nan, dtype=torch.float64, device='cuda')
     tl, tsc, tlc = _dev(land), _dev(soilcap), _dev(lc)
     ins = [_dev(x) for x in (np.array([t]), insol, c['pos'], c['uabs'], c['mu'], cap, elev, radf, swf, ch)]
     tt, ti, tp, tu, tm, tc, te, tr, ts, tch = ins
@@ -84,7 +86,7 @@ def gpu_air_column(p, mode, t, c, dt, insol=None, cap=None, elev=None, radf=None
               P(tp), P(tu), P(tm), P(o['theta']), P(o.get('q')), P(o['Ts']), P(o.get('ice')), float(dt), P(o['wx']),
               P(o['olr']),
               P(o['sw_abs']), P(o['stored']), P(o['rain']), P(o['evap']), P(o['conv']), P(o['tsurf']),
-              P(o['rainp']), None, P(o.get('cap', tc)), P(o.get('qocean')), P(o.get('qsw')), P(tl), P(tsc),
+              P(o['rainp']), P(o.get('dgo')), P(o.get('cap', tc)), P(o.get('qocean')), P(o.get('qsw')), P(tl), P(tsc),
               P(o.get('soil')), P(o.get('pond')), P(o.get('snow')), P(o.get('cw')), P(o.get('cloudo')), P(te),
               P(o['flash']), P(tr), P(ts), P(o['swalb']), P(tch), P(tlc), P(o.get('firn')), P(o.get('xfer')),
               1 if mixed else 0, _hip.stream())
@@ -102,7 +104,7 @@ def compare(g, o, tol, dt, label, bars=None, outliers=0.0):
     """outliers: the share of columns allowed past the bars (columns on a knife edge of the reference's own
     design, named by the caller), every other column within them."""
     keys = [k for k in ('theta', 'Ts', 'wx', 'olr', 'sw_abs', 'tsurf', 'qocean', 'qsw', 'swalb', 'firn', 'q', 'ice',
-                        'cw', 'cloudo', 'soil', 'pond', 'snow', 'cap', 'xfer') if k in o] + \
+                        'cw', 'cloudo', 'soil', 'pond', 'snow', 'cap', 'xfer', 'dgo') if k in o] + \
         (['rain', 'evap', 'conv', 'rainp'] if 'q' in o else [])
     keys.append('stored')
     ce = {k: cell_errors(g[k], o[k]) for k in keys}
@@ -590,3 +592,32 @@ def test_cumulus_layer(L, mixed, case):
     compare(g, o, 1e-5 if mixed else 1e-12, dt, f'cumulus L={L} mixed={mixed} case {case}',
             bars=(dict(wx=3e-4, q=3e-5, theta=3e-5, xfer=3e-5) if cloudy else None) if mixed else fp64,
             outliers=(2e-2 if mixed else 2e-3) if lcl else (1e-3 if mixed else 0.0))
+
+
+@pytest.mark.parametrize('L,mixed,case', [(5, False, 2), (7, False, 3), (7, True, 3)])
+def test_diagnostics_rows(L, mixed, case):
+    """The diagnostics rows (dgo: heating and latent heating per layer, the crossing's gross up and down, the convective
+    mass per layer) on the cumulus columns, to the same bars as the fields they come from (latent heating as rain and
+    cloud water, the crossing and convective masses as wx and conv)."""
+    from oracle import column as ocol
+    from test_oracle_air_column import CU_OPTS, cu_par, cumulus_columns
+    _, opts = CU_OPTS[case]
+    N, dt, t = 40962, 900.0, 0.45 * 86400.0 + 30 * 86400.0
+    print(f'case: air_column diagnostics L={L} mixed={mixed} case {case} N={N}', flush=True)
+    c = cumulus_columns(N, L, seed=240 + L)
+    kw = dict(ocean=True, q=c['q'], ice=c['ice'], cw=c['cw'], cloudo=True)
+    if opts.get('sfc'):
+        kw['ch'] = np.random.default_rng(245 + L).uniform(1e-3, 3e-2, N)
+    p = cu_par(L, **opts)
+    g = gpu_air_column(p, 0, t, c, dt, mixed=mixed, dgo=True, **kw)
+    o = ocol.air_column(p, 0, np.array([t]), c['pos'], c['uabs'], c['mu'], c['theta'], c['Ts'], dt, mix=mixed,
+                        dgo=True, **kw)
+    assert np.all(np.isfinite(g['dgo'])) and np.any(o['dgo'][4 * L - 2:] != 0)
+    for name, rows, fp64, mix in (('Q', slice(0, L), 1e-12, 1e-5), ('latent', slice(L, 2 * L), 1e-9, 3e-4),
+                                  ('gross', slice(2 * L, 4 * L - 2), 1e-11, 3e-4),
+                                  ('convective', slice(4 * L - 2, 5 * L - 2), 1e-10, 1e-3)):
+        ce = cell_errors(g['dgo'][rows], o['dgo'][rows])
+        bar = mix if mixed else fp64
+        past = ce > bar
+        print(f'  {name}: worst {ce.max():.3g}, past {int(past.sum())}', flush=True)
+        assert past.sum() <= (2e-2 if mixed else 2e-3) * N, name

@@ -1193,3 +1193,44 @@ def test_cumulus_options_act():
         o = out(**kw)
         assert np.any(o['wx'] != prev['wx']) or np.any(o['theta'] != prev['theta']), kw
         prev = o
+
+
+# ---- the diagnostics rows (dgo: air.cu:1286-1290, 1944-1950): per layer the heating Q and the latent heating (W/m^2),
+# per interface the dry crossing's gross up and down masses, per layer the convective mass delivered into it ----
+
+def test_diagnostics_rows():
+    """The rows add up to what the column did: without clouds or plumes the latent heating sums to Lv times the rain
+    (every gram condensed rains, each releasing Lv); where no floor limits the crossing each interface's net crossing
+    is gross up less gross down; the parcel plume's (and the deep plume's, without the cloud layer's entrainment)
+    delivered mass sums to the convective mass conv; the rows are zero where nothing happens (a dry column's latent
+    heating and convective mass)."""
+    c = moist_aqua(rh=(0.1, 0.8, 1.3), warm=12.0)
+    N, dt = c['Ts'].size, 900.0
+    o = run(c, par3(moist=1.0, tau_conv=1e30), q=c['q'], ice=c['ice'], dt=dt, dgo=True)
+    L = 3
+    assert o['dgo'].shape == (5 * L - 2, N)
+    Q, dgl, gross, cin = o['dgo'][:L], o['dgo'][L:2 * L], o['dgo'][2 * L:4 * L - 2], o['dgo'][4 * L - 2:]
+    assert o['rain'].max() > 0 and o['conv'].max() == 0
+    from insilica.atmosphere import PAR_ORDER
+    lv = par3()[PAR_ORDER.index('Lv')]
+    np.testing.assert_allclose(dgl.sum(0), lv * o['rain'], rtol=1e-12, atol=1e-12 * lv * o['rain'].max())
+    assert np.all(cin == 0)
+    net = gross[0::2] - gross[1::2]
+    mu1 = new_masses(c, o['wx'])
+    free = np.all(mu1 > 10.0 * 1.01, axis=0)            # (no layer at its floor: the crossing was not limited)
+    assert free.sum() > N // 2
+    np.testing.assert_allclose(o['wx'][:, free], net[:, free], rtol=1e-12, atol=1e-12 * np.abs(net).max())
+    # the plumes: the parcel plume, then the deep plume without the cloud layer's entrainment
+    for extra in (dict(), dict(plume_deep=1.0)):
+        d = deep_columns()
+        p = par4(moist=1.0, **extra)
+        o = run(d, p, q=d['q'], ice=d['ice'], dt=dt, dgo=True)
+        L = 4
+        cin = o['dgo'][4 * L - 2:]
+        assert o['conv'].max() > 0
+        np.testing.assert_allclose(cin.sum(0), o['conv'], rtol=1e-13, atol=0)
+    # a dry column: no latent heating, no convective mass
+    a = aqua()
+    o = run(a, par3(), dt=dt, dgo=True)
+    L = 3
+    assert np.all(o['dgo'][L:2 * L] == 0) and np.all(o['dgo'][4 * L - 2:] == 0) and np.any(o['dgo'][:L] != 0)
