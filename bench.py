@@ -53,6 +53,19 @@ def parse():
 TRAFFIC_PROFILE = ROOT / 'profiles' / 'coupled_stack_level6_mixed_hbm_traffic.md'
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and its logical CPU count."""
+    name = 'unknown'
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                name = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f'{name}, {os.cpu_count()} logical CPUs'
+
+
 def pmc_traffic(name):
     """HBM bytes per launch of kernel `name` at level 6, mixed form, from the committed FETCH_SIZE / WRITE_SIZE
     profile (copy-calibrated counters; the table's last column, MB per launch), or None if it has no row for it."""
@@ -330,7 +343,7 @@ def radiation_main(a):
         orad.sw_column(n, L, par, swt, sub, sub_c['alb'], mode=0, tnow=2.0e6, interval=3600.0,
                        pos=np.ascontiguousarray(pos[cols]), cloudo=sub_c['cloudo'])
         el = time.perf_counter() - t0
-        cpu = dict(value=round(n / N / el, 5), unit='passes/s', cores=cores, kind='port',
+        cpu = dict(value=round(n / N / el, 5), unit='passes/s', cores=cores, kind='port', model=cpu_model(),
                    sample=(f'{n} of the {N} columns (every {N // n}th), ' if n < N else f'all {N} columns, ') +
                           f'one FP64 oracle pass (C, OpenMP), {el:.1f} s' + (', scaled to the whole pass' if n < N else ''))
         if n == N:
@@ -382,6 +395,7 @@ def main():
         print(f'bench.py runs on one GPU only (asked for --gpus {a.gpus}): this build has no multi-GPU path; '
               f'run with --gpus 1', file=sys.stderr, flush=True)
         sys.exit(2)
+    import numpy as np
     import torch
     torch.cuda.set_device(0)
     if a.calibrate_copy:
@@ -424,13 +438,20 @@ def main():
         o = OracleWorld(wm)
         tph = int(round(wm.ticks_per_hour))
         hours = a.cpu_hours or (48.0 if a.preset == 'coupled_stack' else 150.0)     # (about 10-20 s of CPU work)
-        ticks = max(1, int(round(hours * tph)))
-        t0 = time.perf_counter()
-        o.step(ticks)
-        el = time.perf_counter() - t0
-        cpu = dict(value=round(ticks / tph / el, 4), unit='model-hours/s', cores=cores, kind='port',
-                   sample=f'{ticks} ticks ({ticks / tph:g} model hours) of the same level-{a.level} {a.preset} world, '
-                          f'FP64 oracle (C, OpenMP), {el:.1f} s')
+        ticks = max(5, int(round(hours * tph)) // 5 * 5)
+        o.step(tph)                                                              # (one model hour of warm-up)
+        rates, el = [], 0.0
+        for _ in range(5):                                                       # the median of five runs
+            t0 = time.perf_counter()
+            o.step(ticks // 5)
+            dt_ = time.perf_counter() - t0
+            el += dt_
+            rates.append(ticks // 5 / tph / dt_)
+        cpu = dict(value=round(float(np.median(rates)), 4), unit='model-hours/s', cores=cores, kind='port',
+                   model=cpu_model(), runs=[round(r, 4) for r in rates],
+                   sample=f'5 runs of {ticks // 5} ticks ({ticks // 5 / tph:g} model hours) each of the same level-'
+                          f'{a.level} {a.preset} world after a model hour of warm-up, median; FP64 oracle (C, '
+                          f'OpenMP), {el:.1f} s in all')
     ms = el_m / a.steps * 1e3
     what, published = PRESET_LINES[a.preset]
     value = a.steps / el_m
