@@ -423,6 +423,26 @@ int32_t isw_advance_time(double* t, double dt, void* stream);
 int32_t isw_timing_start(void);
 int32_t isw_timing_stop(char* out, int32_t cap);
 
+/* Water over and through the land (reference: LandWater.step, land.py:33-46; kernels/land.cu:38-101): pond (kg/m^2)
+ * runs toward lower water surfaces z = b + pond / rho_w + ice / rho_ice at 3/4 of the speed U over a depth
+ * min(head, pond / rho_w), at most 1/12 of the head per edge; soil water seeps between water tables
+ * zt = soil / soilcap b by Darcy's law (kseep); a donor gives at most half its pond and all its soil in a step; a sea
+ * cell (land <= 0.5) receives into mouth (kg/m^2, accumulated). b, land, soilcap, ice (N); pond, soil (N) updated in
+ * place; u_dt = U dt (m), kseep_dt = K dt (m^2); work: 4 N doubles; qout (N, nullable): each land cell's surface
+ * outflow, accumulated (kg/m^2). */
+int32_t isw_land_route(const isw_grid* g, const double* b, const double* land, const double* soilcap, double* pond,
+                       double* soil, const double* ice, double rho_w, double rho_ice, double u_dt, double kseep_dt,
+                       double* work, double* mouth, double* qout, void* stream);
+
+/* The ice sheets' flow (reference: LandWater.ice_step, land.py:48-60; kernels/land.cu:104-173): Glen's shallow-ice law
+ * without sliding, through each edge gam H^5 |grad s|^2 (head / de) le per step with s = b + ice / rho_ice (0 at sea)
+ * and |grad s|^2 the TRiSK kinetic-energy rule's, the two ends' mean between land cells; at most 1/12 of the head per
+ * edge and all the donor's ice. ice (N, kg/m^2) updated in place; gam_dt = 2 A (rho_ice g)^3 / 5 dt; work: 3 N doubles;
+ * a sea cell's gain is added to calved (N) and to ice_prev (N, nullable: the ocean's sea-ice record); iout (N,
+ * nullable): each land cell's outflow, accumulated. */
+int32_t isw_ice_flow(const isw_grid* g, const double* b, const double* land, double* ice, double rho_ice,
+                     double gam_dt, double* work, double* ice_prev, double* calved, double* iout, void* stream);
+
 /* Lloyd's iteration toward the spherical centroidal Voronoi tessellation on the device (reference: grid/icosa.py:194-224
  * lloyd_scvt, which grid/__init__.py:95-99 runs on the GPU from level 6): pos (N, 3, device) the unit generators,
  * updated in place; ring (N, 6) int32 each cell's neighbours counter-clockwise (entries below deg[i] in [0, N)), deg (N)

@@ -53,6 +53,13 @@ int air_column(int, int, const double*, const isw_air*, int, const double*, cons
                hipStream_t);
 }  // namespace isw_col
 
+namespace isw_land {
+int route(const isw_grid*, const double*, const double*, const double*, double*, double*, const double*, double, double,
+          double, double, double*, double*, double*, hipStream_t);
+int ice(const isw_grid*, const double*, const double*, double*, double, double, double*, double*, double*, double*,
+        hipStream_t);
+}  // namespace isw_land
+
 namespace isw_ocn {
 int overturn(int, int, const isw_ocean*, const double*, double*, double*, const double*, hipStream_t);
 int column(int, int, const isw_ocean*, const double*, double*, double*, const double*, const double*, const double*,
@@ -361,6 +368,22 @@ int32_t isw_gas_src(int32_t N, int32_t L, int32_t G, const double* src, const do
 int32_t isw_advance_time(double* t, double dt, void* stream) {
   if (!t) return ISW_EBADARG;
   return isw_col::advance_time(t, dt, S(stream));
+}
+
+int32_t isw_land_route(const isw_grid* g, const double* b, const double* land, const double* soilcap, double* pond,
+                       double* soil, const double* ice, double rho_w, double rho_ice, double u_dt, double kseep_dt,
+                       double* work, double* mouth, double* qout, void* stream) {
+  if (!grid_ok(g) || !b || !land || !soilcap || !pond || !soil || !ice || !work || !mouth || !(rho_w > 0) ||
+      !(rho_ice > 0) || !(u_dt >= 0) || !(kseep_dt >= 0))
+    return ISW_EBADARG;
+  return isw_land::route(g, b, land, soilcap, pond, soil, ice, rho_w, rho_ice, u_dt, kseep_dt, work, mouth, qout,
+                         S(stream));
+}
+
+int32_t isw_ice_flow(const isw_grid* g, const double* b, const double* land, double* ice, double rho_ice,
+                     double gam_dt, double* work, double* ice_prev, double* calved, double* iout, void* stream) {
+  if (!grid_ok(g) || !b || !land || !ice || !work || !calved || !(rho_ice > 0) || !(gam_dt >= 0)) return ISW_EBADARG;
+  return isw_land::ice(g, b, land, ice, rho_ice, gam_dt, work, ice_prev, calved, iout, S(stream));
 }
 
 const char* isw_version(void) { return "insilica_hip 0.1 gfx950"; }

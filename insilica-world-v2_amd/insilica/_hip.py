@@ -113,6 +113,8 @@ def lib():
             'isw_csum': [vp, vp, i64, vp, vp, vp],
             'isw_dsum_noalloc': [vp, vp, i64, vp, vp],
             'isw_mesh_lloyd_device': [i64, vp, vp, vp, i32, d, vp, i32, vp, vp, vp],
+            'isw_land_route': [vp, vp, vp, vp, vp, vp, vp, d, d, d, d, vp, vp, vp, vp],
+            'isw_ice_flow': [vp, vp, vp, vp, d, d, vp, vp, vp, vp, vp],
             'isw_fill': [vp, i64, d, vp],
             'isw_lw_setup': [i32, i32, vp, vp, vp, d, d, d, vp, vp, vp, vp, vp, d, i32, vp, i32, vp],
             'isw_lw_column': [i32, i32, vp, vp, vp, vp, vp, i32, vp],
@@ -151,7 +153,8 @@ EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_
            'isw_hs_column', 'isw_surface_layer', 'isw_cell_speed', 'isw_cell_speed_perot', 'isw_gas_src',
            'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version',
            'isw_grid_check', 'isw_air_column_options', 'isw_ocn_overturn', 'isw_ocn_forcing', 'isw_ocn_column',
-           'isw_ocn_xch_salt', 'isw_ocn_pen', 'isw_ocn_fresh', 'isw_ocn_diffuse', 'isw_mesh_lloyd_device')
+           'isw_ocn_xch_salt', 'isw_ocn_pen', 'isw_ocn_fresh', 'isw_ocn_diffuse', 'isw_mesh_lloyd_device',
+           'isw_land_route', 'isw_ice_flow')
 
 
 def check(rc, what):

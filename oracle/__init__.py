@@ -22,9 +22,10 @@ _i32p, _i8p, _f64p = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_in
 
 
 def build():
-    """Compile liboracle.so from oracle.c, radiation.c, column.c and ocean.c (seconds)."""
+    """Compile liboracle.so from oracle.c, radiation.c, column.c, ocean.c and land.c (seconds)."""
     cmd = ['gcc', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(LIB) + '.tmp',
-           str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), str(HERE / 'column.c'), str(HERE / 'ocean.c'), '-lm']
+           str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), str(HERE / 'column.c'), str(HERE / 'ocean.c'),
+           str(HERE / 'land.c'), '-lm']
     subprocess.run(cmd, check=True)
     os.replace(str(LIB) + '.tmp', LIB)
 
@@ -81,6 +82,8 @@ def lib():
             'og_gas_src': (None, [i, i, i, vp, vp, d, vp]),
             'og_air_column': (None, [i, i, vp, vp, i] + [vp] * 9 + [d] + [vp] * 29 + [i]),
             'og_plume_ascent': (None, [i, vp, vp, vp, vp, vp, d, d, d, i, i, vp, vp, vp]),
+            'og_land_route': (None, [vp, vp, vp, vp, vp, vp, vp, d, d, d, d, vp, vp]),
+            'og_ice_flow': (None, [vp, vp, vp, vp, d, d, vp, vp, vp]),
             'og_sun_lon': (d, [d, vp, vp]),
         }
         for name, (res, args) in sig.items():

@@ -151,4 +151,11 @@ void og_ocn_fresh(int N, const double* h, double* S, const double* evap, const d
 void og_ocn_diffuse(const og_grid* g, int L, const double* h, const double* T, const double* S, double* Tn,
                     double* Sn, const double* wall, double kappa_h, double dt);
 
+/* ---- the land's water routing and ice flow (reference: kernels/land.cu, land.py:33-60); land.c ---- */
+void og_land_route(const og_grid* g, const double* b, const double* land, const double* soilcap, double* pond,
+                   double* soil, const double* ice, double rho_w, double rho_ice, double u, double ks, double* mouth,
+                   double* qout);
+void og_ice_flow(const og_grid* g, const double* b, const double* land, double* ice, double rho_ice, double gam,
+                 double* ice_prev, double* calved, double* iout);
+
 #endif
