@@ -466,6 +466,40 @@ const char* isw_version(void);
 int32_t isw_grid_check(const isw_grid* g);
 int32_t isw_air_column_options(int32_t L, const double* par);
 
+/* The mountains' drag on the air (reference: OroDrag.apply, orodrag.py:62-68; kernels/drag.cu:40-193, Lott & Miller
+ * 1997 on the isentropic layers): per cell, from its sub-grid orography sso (4, N: the heights' standard deviation mu,
+ * the slope sigma, the anisotropy gamma and the axis's angle from east), the low-level flow between mu and 2 mu, the
+ * blocked layer below Z_blk (its form drag, implicit: beta (L, N) = dt x rate) and the gravity waves launched at
+ * 2 mu and deposited going up (Lindzen's saturation, critical levels; F (2L, N): east and north force per area per
+ * layer); then per edge and layer u <- (u + dt fn / m) / (1 + beta_e), fn the cells' forces along the edge's normal
+ * (nrm (E, 4): the normal's east and north at its first and second cell), m the layer's mass at the edge, the
+ * kinetic energy removed added to diss (E, J/m^2). mu, theta (L, N) the air layers; u (L, E) updated in place; op
+ * (N, 6, 2) Perot's coefficients for a cell's wind (east, north) per edge slot; beta, F: outputs (work). par (host):
+ * grav, p0, kappa, rd, G, cd, hnc, ric, dt (equal to dt), mu_min (the reference's par vector, orodrag.py:53). L >= 2. */
+typedef struct isw_oro {
+  double grav, p0, kappa, rd, G, cd, hnc, ric, dt, mu_min;
+} isw_oro;
+int32_t isw_oro_drag(const isw_grid* g, int32_t L, const isw_oro* par, const double* mu, const double* theta,
+                     double* u, const double* op, const double* nrm, const double* sso, double* beta, double* F,
+                     double dt, double* diss, void* stream);
+
+/* The gravity waves not tied to the ground (reference: WaveDrag.apply, orodrag.py:86-96; kernels/drag.cu:205-293,
+ * Alexander & Dunkerton 1999): per cell a spectrum of waves, phase speeds -cmax .. cmax by dc (at most
+ * ISW_GW_MAX_SPEEDS), launched at the pressure psrc centred on the source layer's wind, each deposited where it
+ * breaks or meets its critical level, a layer taking no more than brings its wind to the wave's speed in a step and
+ * the source layer keeping what none could take (each column's momentum exact); F (2L, N) east rows the force per
+ * area, north rows 0; then the edges as isw_oro_drag's with beta = zero ((L, N) zeros). conv (N, nullable): the
+ * column's convective rain for the convective source share fc; cmean (1 double, device) receives its area mean
+ * (required with conv). par (host): grav, p0, kappa, rd, bt, cw, kw = 2 pi / lambda, psrc, cmax, dc, dt (equal to
+ * dt), fc (orodrag.py:76-77). L >= 2. */
+#define ISW_GW_MAX_SPEEDS 4096
+typedef struct isw_gw {
+  double grav, p0, kappa, rd, bt, cw, kw, psrc, cmax, dc, dt, fc;
+} isw_gw;
+int32_t isw_wave_drag(const isw_grid* g, int32_t L, const isw_gw* par, const double* mu, const double* theta,
+                      double* u, const double* op, const double* nrm, const double* conv, double* cmean,
+                      const double* zero, double* F, double dt, double* diss, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -719,9 +719,10 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
   int ces[6] = {0xffff, 0xffff, 0xffff, 0xffff, 0xffff, 0xffff};
   if (q0 < nC) {
     const int* ps_ = g.pc_eslot + 3 * g.pc_off[b] + 3 * q0;
-    const int pa = ps_[0], pb2 = ps_[1], pcc = ps_[2];
-    ces[0] = pa & 0xffff; ces[1] = pa >> 16; ces[2] = pb2 & 0xffff;
-    ces[3] = pb2 >> 16; ces[4] = pcc & 0xffff; ces[5] = pcc >> 16;
+    // (unsigned: a missing edge's 0xffff in the high half makes the word negative as an int32)
+    const unsigned pa = (unsigned)ps_[0], pb2 = (unsigned)ps_[1], pcc = (unsigned)ps_[2];
+    ces[0] = (int)(pa & 0xffffu); ces[1] = (int)(pa >> 16); ces[2] = (int)(pb2 & 0xffffu);
+    ces[3] = (int)(pb2 >> 16); ces[4] = (int)(pcc & 0xffffu); ces[5] = (int)(pcc >> 16);
   }
   // A: masses and Phi* of sub-step it on patch, halo and second halo
   int ci = 0, cslot = 0;                           // (thread q's cell q, for C)

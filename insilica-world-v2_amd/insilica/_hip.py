@@ -81,6 +81,18 @@ class IswSwTables(ctypes.Structure):
                 + [(k, _f32p) for k in ('logk', 'sun', 'ray', 'hzf', 'cssa', 'cssa_i', 'g_i', 'cia')])
 
 
+class IswOro(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ('grav', 'p0', 'kappa', 'rd', 'G', 'cd', 'hnc', 'ric', 'dt', 'mu_min')]
+
+
+class IswGw(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ('grav', 'p0', 'kappa', 'rd', 'bt', 'cw', 'kw', 'psrc', 'cmax', 'dc',
+                                               'dt', 'fc')]
+
+
+ISW_GW_MAX_SPEEDS = 4096          # the most phase speeds isw_wave_drag's spectrum may hold
+
+
 class HipError(RuntimeError):
     pass
 
@@ -136,6 +148,8 @@ def lib():
             'isw_ocn_diffuse': [vp, i32, vp, vp, vp, vp, vp, vp, d, d, vp],
             'isw_grid_check': [vp],
             'isw_air_column_options': [i32, vp],
+            'isw_oro_drag': [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp, vp],
+            'isw_wave_drag': [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -154,7 +168,7 @@ EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_
            'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version',
            'isw_grid_check', 'isw_air_column_options', 'isw_ocn_overturn', 'isw_ocn_forcing', 'isw_ocn_column',
            'isw_ocn_xch_salt', 'isw_ocn_pen', 'isw_ocn_fresh', 'isw_ocn_diffuse', 'isw_mesh_lloyd_device',
-           'isw_land_route', 'isw_ice_flow')
+           'isw_land_route', 'isw_ice_flow', 'isw_oro_drag', 'isw_wave_drag')
 
 
 def check(rc, what):

@@ -53,7 +53,10 @@ DEFAULTS = dict(
     revap_area=0.05, root_q=0.0, ent_det=0.0, ent_ifs=0.0, shallow=0.0, cu_dth=0.0, cu_dp=0.0, buoy_v=0.0,
     cu_sub=0.0, bare_fx=0.0, det_all=0.0, cl_float=0.0, ml_clear=0.0, cu_cloud=0.0, firn=0.0, k_firn=0.34,
     rho_firn=450.0, c_ice=2090.0, lw_refl=0.0, ent_loc=0.0, ecc=0.0, lam_peri=0.0, liq_cold=20.0, cw_qs=0.0,
-    ent_lcl=0.0, sub_mix=0.0, cover=0.0, sfc_wind=0.0)
+    ent_lcl=0.0, sub_mix=0.0, cover=0.0, sfc_wind=0.0,
+    # the drags (insilica.orodrag; not in PAR_ORDER: the column does not read them)
+    oro=0.0, oro_g=0.3, oro_cd=1.0, oro_hnc=0.5, oro_ric=0.25,
+    gw=0.0, gw_bt=0.004, gw_cw=40.0, gw_lambda=3.0e5, gw_psrc=31500.0, gw_cmax=99.6, gw_dc=1.2, gw_conv=0.0)
 assert len(PAR_ORDER) == 132 and set(PAR_ORDER) <= set(DEFAULTS)
 
 

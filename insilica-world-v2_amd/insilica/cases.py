@@ -285,6 +285,25 @@ def coupled_stack(g, seed=0, lmax=20, dtheta=2.0, du=5.0, land_frac=0.3):
     return dict(h=h, b=b, u=u, theta=theta, q=x['q'], Ts=x['Ts'], land=land)
 
 
+def synthetic_sso(g, seed=0, land=None):
+    """Seeded sub-grid orography statistics per cell for the mountains' drag (the reference reads Earth's from
+    ETOPO1, cases.earth_sso): (mu, sigma, gamma, theta) -- the heights' standard deviation (m; 0 at sea, up to about
+    1.5 km over a few ranges), the slope along the principal axis (0.005-0.05), the anisotropy (0.1-1) and the axis's
+    angle from east (radians, -pi/2 .. pi/2). land: the land mask (mountains only where > 0.5), else every cell."""
+    rng = np.random.default_rng(seed)
+    ranges = rng.normal(size=(6, 3))
+    ranges /= np.linalg.norm(ranges, axis=1)[:, None]
+    # each range a Gaussian ridge of 15 degrees' width around a seeded centre
+    d = np.arccos(np.clip(g.pos @ ranges.T, -1.0, 1.0))
+    mu = 1500.0 * np.max(np.exp(-(d / math.radians(15.0)) ** 2), axis=1)
+    if land is not None:
+        mu = np.where(np.asarray(land) > 0.5, mu, 0.0)
+    sigma = np.where(mu > 0, 0.005 + 0.045 * rng.random(g.N), 0.0)
+    gamma = 0.1 + 0.9 * rng.random(g.N)
+    theta = (rng.random(g.N) - 0.5) * math.pi
+    return np.stack([mu, sigma, gamma, theta])
+
+
 def held_suarez(g, seed=1):
     """Held & Suarez (1994) in two air layers, started as their protocol starts (reference: cases.py:657-670): at
     rest and horizontally uniform, each layer at the area-mean equilibrium theta of its half's mass-mean pressure

@@ -250,6 +250,32 @@ class World:
             self.tr = Transport(dg, layers=self.L, beta=P.get('beta', BETA), tracers=1)
         self.b = self.core.b
         self.h, self.u = dg.asarray(h), dg.asarray(u)
+        # the gravity waves' drag on the air (reference: world.py:209-224): the mountains' (the air physics' oro > 0,
+        # with the sub-grid orography params['sso'], (4, N) or 'synthetic') and the waves not tied to the ground
+        # (gw > 0); both book their heat with the wind stress, so they need the ocean under the air
+        self.oro = self.gw = None
+        if self.air:
+            from .atmosphere import DEFAULTS
+            dp = self.physics.p if self.physics is not None else dict(DEFAULTS, grav=self.grav, **P.get('physics', {}))
+            if dp['oro'] > 0:
+                from .orodrag import OroDrag
+                if self.sea is None or P.get('sso') is None:
+                    raise ValueError('the mountain drag (oro) needs the coupled ocean and the sub-grid orography (sso)')
+                sso = P['sso']
+                if isinstance(sso, str):
+                    if sso != 'synthetic':
+                        raise ValueError("sso: a (4, N) array or 'synthetic'")
+                    sso = cases.synthetic_sso(g, seed=P.get('seed', 0), land=getattr(self, 'land', None))
+                self.oro = OroDrag(dg, self.L, sso, dp, self.dt)
+            if dp['gw'] > 0:
+                from .orodrag import WaveDrag
+                if self.sea is None:
+                    raise ValueError('the wave drag (gw) books its heat with the wind stress: it needs the coupled ocean')
+                self.gw = WaveDrag(dg, self.L, dp, self.dt)
+                if dp.get('gw_conv', 0.0) > 0:
+                    if self.physics is None or getattr(self, 'ice', None) is None:
+                        raise ValueError('the waves\' convective source (gw_conv) needs the moist column')
+                    self.gw.conv = self.physics.rainp[2]
         self.M = self.h * dg.areaCell[None, :]
         if nu4:
             lam = self.core.lap_max()
@@ -303,6 +329,10 @@ class World:
         sea, moist = self.sea, self.ice is not None
         if sea is not None:
             sea.couple(self.h, self.u, self.surface_drag, self.dt)
+            if self.oro is not None:
+                self.oro.apply(self.h, self.theta, self.u, self.dt, sea.diss)
+            if self.gw is not None:
+                self.gw.apply(self.h, self.theta, self.u, self.dt, sea.diss)
             self.core.vmix_heat(self.h, sea.diss, self.theta)
         if self.physics is not None:
             if kind & 2:

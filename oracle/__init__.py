@@ -22,10 +22,10 @@ _i32p, _i8p, _f64p = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_in
 
 
 def build():
-    """Compile liboracle.so from oracle.c, radiation.c, column.c, ocean.c and land.c (seconds)."""
+    """Compile liboracle.so from oracle.c, radiation.c, column.c, ocean.c, land.c and drag.c (seconds)."""
     cmd = ['gcc', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(LIB) + '.tmp',
            str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), str(HERE / 'column.c'), str(HERE / 'ocean.c'),
-           str(HERE / 'land.c'), '-lm']
+           str(HERE / 'land.c'), str(HERE / 'drag.c'), '-lm']
     subprocess.run(cmd, check=True)
     os.replace(str(LIB) + '.tmp', LIB)
 
@@ -85,6 +85,10 @@ def lib():
             'og_land_route': (None, [vp, vp, vp, vp, vp, vp, vp, d, d, d, d, vp, vp]),
             'og_ice_flow': (None, [vp, vp, vp, vp, d, d, vp, vp, vp]),
             'og_sun_lon': (d, [d, vp, vp]),
+            'og_oro_drag': (None, [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp]),
+            'og_wave_drag': (None, [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp]),
+            'og_gw_mean': (d, [i, vp, vp]),
+            'og_blk_int': (d, [d, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -50,6 +50,18 @@ class OracleWorld:
                 self.ice = _np(w.ice) if w.ice is not None else None
                 self.cloudy = getattr(w, 'cw', None) is not None      # (the cloud water: rows 2L..3L)
             self.psi, self.time = _np(w.psi), float(w.time[0])
+        # the drags (the product World's settings and tables: Perot's coefficients, the edges' normals, the sub-grid
+        # orography, the settings in the kernels' order)
+        self.oro = self.gw = None
+        if getattr(w, 'oro', None) is not None:
+            o = w.oro
+            self.oro = dict(op=_np(o.op), nrm=_np(o.nrm), sso=_np(o.sso),
+                            par=np.array([getattr(o.par, k) for k, _ in o.par._fields_]))
+        if getattr(w, 'gw', None) is not None:
+            o = w.gw
+            if o.cmean is not None:
+                raise NotImplementedError('the oracle world steps the waves without the convective source')
+            self.gw = dict(op=_np(o.op), nrm=_np(o.nrm), par=np.array([getattr(o.par, k) for k, _ in o.par._fields_]))
         if w.sea is not None:
             s = w.sea
             self.surface_drag = w.surface_drag
@@ -114,6 +126,16 @@ class OracleWorld:
         if s is not None:
             couple_stress(og, L, self.h, self.u, s['h'], s['u'], self.surface_drag, self.rho, self.dt, s['diss'],
                           s['tau_acc'], wall=_wall(self.score), land_drag=self.land_drag)
+            if self.oro is not None or self.gw is not None:
+                from .drag import oro_drag, wave_drag
+                if self.oro is not None:
+                    d = self.oro
+                    self.u, s['diss'], _, _ = oro_drag(og, self.h, theta, self.u, d['op'], d['nrm'], d['sso'],
+                                                       d['par'], self.dt, s['diss'])
+                if self.gw is not None:
+                    d = self.gw
+                    self.u, s['diss'], _ = wave_drag(og, self.h, theta, self.u, d['op'], d['nrm'], d['par'], self.dt,
+                                                     s['diss'])
             vmix_heat(og, L, self.h, s['diss'], theta, grav=self.core.grav, cp=self.core.cpa, kappa=self.core.kap)
         wx = o = None
         if self.physics is not None:
@@ -206,7 +228,9 @@ class OracleWorld:
         if getattr(self, 'ice', None) is not None:
             out['ice'] = self.ice
         if self.sea is not None:
-            for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap', 'ice_prev', 'tau_acc', 'q_acc', 'sun_acc', 'fw_acc'):
+            # (the forcing sums are state only where the mixed layer mixes, as in the product's STATE)
+            mixed = ('ice_prev', 'tau_acc', 'q_acc', 'sun_acc', 'fw_acc') if getattr(self, 'sea_mix', False) else ()
+            for k in ('h', 'u', 'M', 'rows', 'Tg', 'cap') + mixed:
                 if self.sea.get(k) is not None:
                     out['sea.' + k] = self.sea[k]
             if getattr(self, 'sea_mix', False):

@@ -193,3 +193,36 @@ def test_ocean_column_refusals():
     assert lib.isw_ocn_xch_salt(10, 1, buf, buf, buf, 0, None) == E
     assert lib.isw_ocn_forcing(None, 86400.0, buf, buf, buf, buf, buf, None) == E
     assert lib.isw_ocn_diffuse(None, 3, buf, buf, buf, buf, buf, None, 1.0, 1.0, None) == E
+
+
+def test_drag_refusals():
+    """The drags' entry points refuse, on the host, a single layer, a step that is not the settings' dt, a spectrum
+    beyond ISW_GW_MAX_SPEEDS, non-positive widths and missing arrays (no launch is made for a refused call)."""
+    lib = _hip.lib()
+    E = _hip.ISW_EBADARG
+    buf = ctypes.c_void_p(1)
+    p32 = ctypes.cast(buf, _hip._i32p)
+    f64 = ctypes.cast((ctypes.c_double * 8)(), _hip._f64p)
+    g = _hip.IswGrid()
+    g.N, g.E, g.NV = 642, 1920, 1280
+    g.eI = g.eJ = g.cellEdge = p32
+    g.areaCell = f64
+    oro = _hip.IswOro(grav=9.8, p0=1e5, kappa=0.2857, rd=287.0, G=0.3, cd=1.0, hnc=0.5, ric=0.25, dt=900.0, mu_min=1.0)
+    o = lambda L=3, dt=900.0, par=oro, sso=buf: lib.isw_oro_drag(ctypes.byref(g), L, ctypes.byref(par) if par else None,
+                                                                buf, buf, buf, buf, buf, sso, buf, buf, dt, buf, None)
+    assert o(L=1) == E and o(L=_hip.LMAX + 1) == E
+    assert o(dt=600.0) == E and o(par=None) == E and o(sso=None) == E
+    bad = _hip.IswOro(**{k: getattr(oro, k) for k, _ in oro._fields_})
+    bad.ric = 0.0
+    assert o(par=bad) == E
+    gw = _hip.IswGw(grav=9.8, p0=1e5, kappa=0.2857, rd=287.0, bt=0.004, cw=40.0, kw=2e-5, psrc=31500.0, cmax=99.6,
+                    dc=1.2, dt=900.0, fc=0.0)
+    w = lambda L=3, dt=900.0, par=gw, conv=None, cmean=None: lib.isw_wave_drag(
+        ctypes.byref(g), L, ctypes.byref(par), buf, buf, buf, buf, buf, conv, cmean, buf, buf, dt, buf, None)
+    assert w(L=1) == E and w(dt=450.0) == E
+    assert w(conv=buf, cmean=None) == E                                # the mean's output is required with conv
+    wide = _hip.IswGw(**{k: getattr(gw, k) for k, _ in gw._fields_})
+    wide.dc = 2 * wide.cmax / _hip.ISW_GW_MAX_SPEEDS                  # one speed more than the limit
+    assert w(par=wide) == E
+    wide.dc = 0.0
+    assert w(par=wide) == E
