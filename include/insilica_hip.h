@@ -66,6 +66,13 @@ typedef struct isw_grid {
    * when a patch owns more than 256 edges (pown > 256) or its patch and halo hold more than 256 cells (pslots > 256). */
   int32_t pslots2, pedges2;
   const int32_t *ph2_off, *ph2_cell, *px_off, *px_edge, *px_slot, *pc_off, *pc_eslot;
+  /* The transport's fused flux-and-stage passes on the same patches (optional: pi_off NULL keeps a flux pass and a
+   * stage pass; needs pcells 64). A patch's edges are its own (pe_edge, slots 0 .. ne - 1) and those whose second
+   * cell only is in it, pi_edge[pi_off[b] .. pi_off[b + 1]) (slots from ne); tc_slot (N x 6): for each cell, in
+   * cellEdgeAsc order, its edge's slot in its patch times 2, plus 1 where the cell is the edge's first (-1: no edge);
+   * ptedges: the most edges a patch uses (ne + its incoming edges). */
+  int32_t ptedges, pad3_;
+  const int32_t *pi_off, *pi_edge, *tc_slot;
 } isw_grid;
 
 /* A batch of worlds stepped together on one mesh: the disjoint union of the worlds' meshes (all of one level, so of
@@ -112,6 +119,8 @@ typedef struct isw_transport {
   double beta;
   double *td, *p1, *p2, *Rp, *Rm, *Hf, *A;
   float *pf;
+  float *pf2; /* (rows,N) floats: the mixed form's second stencil copy, which the fused stages alternate with pf
+               * (NULL: the flux and stage passes run apart) */
 } isw_transport;
 
 /* Layers.step (reference: src/insilica/layers.py:311-338; kernels/layers.cu): one time-split RK3 step in place on

@@ -98,6 +98,9 @@ bool grid_ok(const isw_grid* g) {
                      g->npatch == (g->N + g->pcells - 1) / g->pcells && g->pslots >= g->pcells &&
                      g->pslots < (1 << 15) && g->pown > 0))
     return false;
+  // the transport's fused stages: their incoming edges, each cell's slots, the most edges a patch uses
+  if (g->pi_off && !(g->ph_off && g->pi_edge && g->tc_slot && g->ptedges > 0 && g->ptedges < (1 << 14)))
+    return false;
   // the pair kernel's patches: one thread of its ISW_PAIR_THREADS-thread block per owned edge, and per cell of patch
   // and halo (at most pslots of them)
   return !g->ph2_off || (g->ph_off && g->ph2_cell && g->px_off && g->px_edge && g->px_slot && g->pc_off &&

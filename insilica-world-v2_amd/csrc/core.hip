@@ -1379,6 +1379,70 @@ __global__ __launch_bounds__(BS) void k_tr_stage(isw_grid g, int Lm, int rows, d
   if (pf) pf[o] = (float)r;
 }
 
+// One RK stage of the transport in one launch on the grid's 64-cell patches (k_tr_hflux then k_tr_stage): block
+// (patch, four rows) first computes the third-order flux of every edge of its cells -- its own edges and those
+// whose second cell only is in the patch, the latter also computed by the neighbour that owns them, from the same
+// inputs, so every copy is the same to the bit -- into LDS, then steps its 64 cells x 4 rows from there. The flux
+// array's write and re-read (rows x E doubles each way) and one kernel boundary go away; the arithmetic and every
+// sum's order are k_tr_hflux's and k_tr_stage's. pfo: the output's FP32 copy (mixed form; the stencil reads srcf,
+// a different buffer, so no block overwrites what another still reads).
+constexpr int TR_EDGES_PER_THREAD = 6;             // a patch's edges per thread of k_tr_hstage (ptedges <= 384)
+__global__ __launch_bounds__(BS) void k_tr_hstage(isw_grid g, int Lm, int rows, double frac, double beta,
+                                                  const double* __restrict__ src, const fast* __restrict__ srcf,
+                                                  const fast* __restrict__ wt, const double* __restrict__ Fm,
+                                                  const double* __restrict__ M0, const double* __restrict__ psi,
+                                                  double* __restrict__ out, float* __restrict__ pfo) {
+  extern __shared__ double hs[];                   // [4][ptedges] the stage flux of the patch's edges, per row
+  const int PT = g.ptedges, N = g.N, E = g.E;
+  const int b = xblock();
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int y = blockIdx.y * 4 + ty;
+  const bool rowok = y < rows;
+  const int l = y % Lm;
+  const int e0 = g.pe_off[b], ne = g.pe_off[b + 1] - e0;
+  const int i0 = g.pi_off[b], ni = g.pi_off[b + 1] - i0;
+  if (rowok) {
+    // (a compile-time count of edges per thread, so that every edge's chain of loads -- its index, its stencil, the
+    // values it names -- is in flight at once rather than one edge after another)
+    const double* Fml = Fm + (size_t)l * E;
+    const double* sp = src + (size_t)y * N;
+    const fast* spf = srcf + (size_t)y * N;
+    int ev[TR_EDGES_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < TR_EDGES_PER_THREAD; j++) {
+      const int q = tx + j * BS1;
+      ev[j] = q < ne ? g.pe_edge[e0 + q] : q < ne + ni ? g.pi_edge[i0 + q - ne] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < TR_EDGES_PER_THREAD; j++)
+      if (ev[j] >= 0) hs[ty * PT + tx + j * BS1] = frac * hoflux(Fml, sp, spf, wt, g, ev[j], beta);
+  }
+  __syncthreads();
+  const int i = b * BS1 + tx;
+  if (!rowok || i >= N) return;
+  const size_t o = (size_t)y * N + i, om = (size_t)l * N + i;
+  const double* Fml = Fm + (size_t)l * E;
+  int e[6], sl[6];
+  ring6(g.cellEdgeAsc, i, e);
+  ring6(g.tc_slot, i, sl);
+  double fv[6], mv[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    fv[k] = hs[ty * PT + (sl[k] < 0 ? 0 : sl[k] >> 1)];
+    mv[k] = frac * Fml[ok0(e[k])];
+  }
+  double t = M0[om] * psi[o];
+  double mx = M0[om];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (e[k] < 0) break;
+    if (sl[k] & 1) { t -= fv[k]; mx -= mv[k]; } else { t += fv[k]; mx += mv[k]; }
+  }
+  const double r = t / mx;
+  out[o] = r;
+  if (pfo) pfo[o] = (float)r;
+}
+
 __global__ __launch_bounds__(BS) void k_tr_aflux(isw_grid g, int Lm, int rows, double beta, const double* __restrict__ psi,
                                                  const double* __restrict__ p2, const fast* __restrict__ p2f,
                                                  const fast* __restrict__ wt, const double* __restrict__ Fm,
@@ -1959,16 +2023,42 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
   const fast* p1f = tr->p1;
   const fast* p2f = tr->p2;
 #endif
-  ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_hflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf);
-  ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
-                             mixed * r_ * n_ * 4,
-             k_tr_stage, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf);
-  ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_hflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm, tr->Hf);
-  ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
-                             mixed * r_ * n_ * 4,
-             k_tr_stage, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
+  const bool fused = g->pi_off && g->pcells == BS1 && (!ISW_FORM_MIXED || tr->pf2) &&
+                     g->ptedges <= TR_EDGES_PER_THREAD * BS1;
+  if (fused) {
+    // (algorithmic bytes: the stencil's field and its FP32 copy, the stencil and its weights, Fm, M0, psi, the
+    // output and its copy, the cell rings and the patch tables; the flux stays in LDS)
+    const double tabs = 6 * n_ * I + 2 * (n_ / BS1 + 1) * I + 2 * e_ * I + 6 * n_ * I;
+    const size_t shm = (size_t)4 * g->ptedges * sizeof(double);
+#if ISW_FORM_MIXED
+    float* pfb = tr->pf2;
+    const fast* p1s = tr->pf2;                     // (stage 1 writes p1's copy to pf2; stage 2 reads it, writes pf)
+#else
+    float* pfb = nullptr;
+    const fast* p1s = tr->p1;
+#endif
+    const dim3 gp((unsigned)g->npatch, (unsigned)((rows + 3) / 4));
+    ISW_LAUNCH_SHM("tr_hstage", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + l_ * n_ * D + r_ * n_ * D + tabs +
+                                    r_ * n_ * D + mixed * r_ * n_ * 4,
+                   k_tr_hstage, gp, dim3(BS1, 4), shm, *g, Lm, rows, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, M0, psi,
+                   tr->p1, pfb);
+    ISW_LAUNCH_SHM("tr_hstage", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + l_ * n_ * D + r_ * n_ * D + tabs +
+                                    r_ * n_ * D + mixed * r_ * n_ * 4,
+                   k_tr_hstage, gp, dim3(BS1, 4), shm, *g, Lm, rows, 1.0 / 2.0, tr->beta, tr->p1, p1s, wt, Fm, M0,
+                   psi, tr->p2, pf);
+  } else {
+    ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
+               k_tr_hflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 3.0, tr->beta, psi, psf, wt, Fm, tr->Hf);
+    ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
+                               mixed * r_ * n_ * 4,
+               k_tr_stage, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 3.0, M0, psi, tr->Hf, Fm, tr->p1, pf);
+    ISW_LAUNCH("tr_hflux", r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
+               k_tr_hflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 2.0, tr->beta, tr->p1, p1f, wt, Fm,
+               tr->Hf);
+    ISW_LAUNCH("tr_stage", l_ * n_ * D + r_ * n_ * D + r_ * e_ * D + l_ * e_ * D + cellE + r_ * n_ * D +
+                               mixed * r_ * n_ * 4,
+               k_tr_stage, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
+  }
   ISW_LAUNCH("tr_aflux", 2 * r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
              k_tr_aflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
   ISW_LAUNCH("tr_ratios", 2 * r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 6 * n_ * I + cellE + 2 * r_ * n_ * D,

@@ -26,7 +26,9 @@ class IswGrid(ctypes.Structure):
                 + [(k, ctypes.c_int32) for k in ('npatch', 'pcells', 'pslots', 'pown')]
                 + [(k, _i32p) for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot')]
                 + [(k, ctypes.c_int32) for k in ('pslots2', 'pedges2')]
-                + [(k, _i32p) for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot')])
+                + [(k, _i32p) for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot')]
+                + [(k, ctypes.c_int32) for k in ('ptedges', 'pad3_')]
+                + [(k, _i32p) for k in ('pi_off', 'pi_edge', 'tc_slot')])
 
 
 LAYER_SCRATCH = ('B', 'q', 'Tp', 'F', 'scale', 'G', 'He', 'ubs', 'ub0', 'ub', 'acc', 'eta0', 'eta', 'eta_prev', 'h1',
@@ -60,7 +62,7 @@ class IswTransport(ctypes.Structure):
     _fields_ = ([(k, ctypes.c_int32) for k in ('L', 'rows_max', 'prec', 'pad_')]
                 + [('beta', ctypes.c_double)]
                 + [(k, _f64p) for k in ('td', 'p1', 'p2', 'Rp', 'Rm', 'Hf', 'A')]
-                + [('pf', _f32p)])
+                + [('pf', _f32p), ('pf2', _f32p)])
 
 
 class IswAir(ctypes.Structure):

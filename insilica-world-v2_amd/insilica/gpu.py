@@ -125,6 +125,45 @@ def fast_patches2(grid, pt, max_owned=_hip.ISW_PAIR_THREADS):
     return out
 
 
+def transport_patches(grid, pt):
+    """The transport's fused flux-and-stage tables on fast_patches' patches (isw_grid's pi_*, tc_slot): a patch's
+    edges are its own (pt's pe_edge, slots 0 .. ne - 1) then its incoming ones -- those whose second cell only is in
+    it, ascending (slots from ne); tc_slot (N, 6): each cell's edges in cellEdgeAsc order as slot * 2 + 1 where the
+    cell is the edge's first, slot * 2 where it is the second, -1 for none. Every invariant the kernel relies on is
+    checked here."""
+    N, PC, nb = int(grid.N), pt['pcells'], pt['npatch']
+    eI, eJ = np.asarray(grid.eI, np.int64), np.asarray(grid.eJ, np.int64)
+    asc = np.asarray(grid.cellEdgeAsc, np.int64)
+    bI, bJ = eI // PC, eJ // PC
+    inc = np.flatnonzero(bJ != bI)
+    order = inc[np.lexsort((inc, bJ[inc]))]                           # incoming edges by patch, then ascending
+    pi_off = np.searchsorted(bJ[order], np.arange(nb + 1)).astype(np.int64)
+    ne = np.diff(pt['pe_off'])
+    slot = np.full(len(eI), -1, np.int64)                             # an edge's slot in its first cell's patch
+    slot[pt['pe_edge']] = np.arange(len(eI)) - np.repeat(pt['pe_off'][:-1], ne)
+    islot = np.full(len(eI), -1, np.int64)                            # its slot in its second cell's patch
+    islot[order] = ne[bJ[order]] + np.arange(len(order)) - pi_off[bJ[order]]
+    cells = np.repeat(np.arange(N), 6)
+    e = asc.reshape(-1)
+    ok = e >= 0
+    ec = np.where(ok, e, 0)
+    first = eI[ec] == cells
+    s_ = np.where(first, slot[ec], np.where(bJ[ec] == bI[ec], slot[ec], islot[ec]))
+    tc = np.where(ok, 2 * s_ + first, -1).reshape(N, 6)
+    ptedges = int(np.max(ne + np.diff(pi_off)))
+    # the kernel's assumptions: every cell's edge has a slot in the cell's own patch naming that edge
+    own_edges = [np.concatenate([pt['pe_edge'][pt['pe_off'][b]:pt['pe_off'][b + 1]], order[pi_off[b]:pi_off[b + 1]]])
+                 for b in range(nb)]
+    for b in range(nb):
+        rows = np.arange(b * PC, min((b + 1) * PC, N))
+        t = tc[rows]
+        m = t >= 0
+        assert np.array_equal(own_edges[b][t[m] >> 1], asc[rows][m])
+        assert np.array_equal((t[m] & 1) == 1, eI[asc[rows][m]] == np.repeat(rows, 6).reshape(-1, 6)[m])
+    assert np.all((asc >= 0) == (tc >= 0)) and ptedges < 1 << 14
+    return dict(pi_off=pi_off, pi_edge=order, tc_slot=tc, ptedges=ptedges)
+
+
 _PATCHES = {}
 
 
@@ -137,6 +176,8 @@ def patch_tables(grid):
     if hit is None or hit[0] is not grid.eI:
         pt = fast_patches(grid, PATCH_CELLS)
         p2 = fast_patches2(grid, pt) if pt is not None and PATCH_PAIRS else None
+        if pt is not None and pt['pcells'] == 64:
+            pt.update(transport_patches(grid, pt))
         hit = (grid.eI, pt, p2)
         if len(_PATCHES) > 16:
             _PATCHES.clear()
@@ -189,6 +230,11 @@ class DeviceGrid:
             for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):
                 setattr(self, k, put(pt[k], np.int32))
                 setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._i32p))
+            if 'pi_off' in pt:
+                s.ptedges = pt['ptedges']
+                for k in ('pi_off', 'pi_edge', 'tc_slot'):
+                    setattr(self, k, put(pt[k], np.int32))
+                    setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._i32p))
             if p2 is not None:
                 s.pslots2, s.pedges2 = p2['pslots2'], p2['pedges2']
                 for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot'):

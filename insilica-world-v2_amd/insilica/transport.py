@@ -23,11 +23,13 @@ class Transport:
         self.td, self.p1, self.p2, self.Rp, self.Rm = (z(R, N) for _ in range(5))
         self.Hf, self.A = z(R, E), z(R, E)
         self.pf = z(R, N, dtype=torch.float32) if dg.mixed else None
+        self.pf2 = z(R, N, dtype=torch.float32) if dg.mixed else None
         s = _hip.IswTransport()
         s.L, s.rows_max, s.prec, s.beta = layers, R, dg.prec, beta
         for k in ('td', 'p1', 'p2', 'Rp', 'Rm', 'Hf', 'A'):
             setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._f64p))
         s.pf = ctypes.cast(self.pf.data_ptr(), _hip._f32p) if self.pf is not None else None
+        s.pf2 = ctypes.cast(self.pf2.data_ptr(), _hip._f32p) if self.pf2 is not None else None
         self.s = s
         self.ref = ctypes.byref(s)
 

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from insilica import grid as gridmod
-from insilica.gpu import fast_patches, fast_patches2
+from insilica.gpu import fast_patches, fast_patches2, transport_patches
 
 
 def unpack(v):
@@ -68,3 +68,29 @@ def test_two_step_tables(n):
         assert np.array_equal(slots_cells[sI], eI[x_e]) and np.array_equal(slots_cells[sJ], eJ[x_e])
         assert pc + len(h1) + len(h2) <= t2['pslots2'] and len(edges) <= t2['pedges2']
         assert len(np.intersect1d(h2, c1)) == 0 and len(np.unique(h2)) == len(h2)
+
+
+@pytest.mark.parametrize('n', [3, 4])
+def test_transport_tables(n):
+    """The fused transport stage's tables: a patch's edge list is every edge touching its cells, once, its own edges
+    first in pe order; each cell's slots name its cellEdgeAsc edges in its own patch's list, odd where it is the
+    edge's first cell; every incoming edge's first cell lies outside the patch and its second inside."""
+    g = gridmod.get(n)
+    t = fast_patches(g, 64)
+    r = transport_patches(g, t)
+    N, pc = g.N, 64
+    eI, eJ, asc = np.asarray(g.eI), np.asarray(g.eJ), np.asarray(g.cellEdgeAsc)
+    for b in range(t['npatch']):
+        cells = np.arange(b * pc, min((b + 1) * pc, N))
+        own = t['pe_edge'][t['pe_off'][b]:t['pe_off'][b + 1]]
+        inc = r['pi_edge'][r['pi_off'][b]:r['pi_off'][b + 1]]
+        assert np.all(eI[inc] // pc != b) and np.all(eJ[inc] // pc == b) and np.all(np.diff(inc) > 0)
+        edges = np.concatenate([own, inc])
+        touching = np.unique(asc[cells][asc[cells] >= 0])
+        assert np.array_equal(np.sort(edges), touching)
+        tc = r['tc_slot'][cells]
+        m = asc[cells] >= 0
+        assert np.array_equal(edges[tc[m] >> 1], asc[cells][m])
+        assert np.array_equal((tc[m] & 1) == 1, eI[asc[cells][m]] == np.repeat(cells, 6).reshape(-1, 6)[m])
+        assert len(edges) <= r['ptedges']
+    assert np.all((r['tc_slot'] < 0) == (asc < 0))
