@@ -110,6 +110,12 @@ typedef struct isw_layers {
   double *D1, *Z1, *L1, *L2, *hold, *hedge, *diss, *pibar, *S, *muf, *mup, *phi, *uf, *accL, *pk, *ufb, *ubb, *ubc;
   float *pkf;
   const isw_batch* batch;
+  /* Tides on Boussinesq layers (reference: layers.py:57-58, set by tides.Tides; NULL / 0 without): tide (N), the
+   * tide-generating potential (m^2/s^2) subtracted from every layer's Bernoulli function; cbot (E), the internal
+   * tide's drag (m/s) and cd_bot, the quadratic drag coefficient, both added to the bottom layer's stress (cd_bot
+   * times |u| from the edge's normal and TRiSK tangential winds). */
+  const double *tide, *cbot;
+  double cd_bot;
 } isw_layers;
 
 /* FCT transport scratch for up to rows_max rows (reference: src/insilica/transport.py:19-30 Transport.__init__).
@@ -508,6 +514,38 @@ typedef struct isw_gw {
 int32_t isw_wave_drag(const isw_grid* g, int32_t L, const isw_gw* par, const double* mu, const double* theta,
                       double* u, const double* op, const double* nrm, const double* conv, double* cmean,
                       const double* zero, double* F, double dt, double* diss, void* stream);
+
+/* The Sun's and the Moon's places for the tides (reference: tides.sky_par, tides.py:42-51, the same order): the
+ * planet's tilt, year and day (s), orbital eccentricity and longitude of perihelion (the air's own orbit), the Sun's
+ * GM and semi-major axis; the Moon's GM, semi-major axis, eccentricity, inclination to the ecliptic, sidereal month,
+ * node regression and perigee advance periods (s), its longitude, node and perigee at time 0 (radians); the Love
+ * factor 1 + k2 - h2; the planet's radius (m). A Moon taken away is GM 0. */
+typedef struct isw_sky {
+  double tilt, year_s, day_s, ecc, lam_peri, sun_gm, sun_a, moon_gm, moon_a, moon_e, moon_i, moon_T, node_T, aps_T,
+      moon_L0, node0, aps0, love, radius;
+} isw_sky;
+
+/* The tide-generating potential (reference: Tides.force, tides.py:70-73; kernels/tides.cu:58-71): at each cell
+ * (pos (N, 3) unit vectors) V = love sum_b GM_b R^2 / r_b^3 (3 cos^2 psi_b - 1) / 2 over the Sun and the Moon at
+ * time[0] - back (time: 1 double on the device, the world's clock), into V (N). */
+int32_t isw_tide_pot(const isw_grid* g, const isw_sky* tp, const double* time, double back, const double* pos,
+                     double* V, void* stream);
+
+/* The internal tide's drag (reference: Tides.force, tides.py:74-76; kernels/tides.cu:128-144): per edge
+ * cbot = mean over its two cells of kappa h^2 N / 2 (rough (N): the floor's roughness h^2, m^2; N the buoyancy
+ * frequency above the floor, nbot (N, nullable) or from the two lowest layers of h, T, S (L, N) with o's equation of
+ * state). */
+int32_t isw_tide_itd(const isw_grid* g, int32_t L, const isw_ocean* o, const double* h, const double* T,
+                     const double* S, const double* rough, double kappa, const double* nbot, double* cbot,
+                     void* stream);
+
+/* Tidal mixing after an ocean step (reference: Tides.mix, tides.py:78-87; kernels/tides.cu:85-122): each sea cell's
+ * bottom dissipation D = rho (c_bot + cd |u_b| + mean cbot) |u_b|^2 (u (L, E) the bottom layer's winds), D dt added to
+ * dsum (N); gamma D dt exp(-z / zeta) mixes T and S (L, N) across each interface z above the floor in equal volumes
+ * (heat and salt kept exactly); land (N, nullable) cells skipped. */
+int32_t isw_tide_mix(const isw_grid* g, int32_t L, const isw_ocean* o, const double* h, const double* u, double* T,
+                     double* S, const double* land, double c_bot, double rho, double gamma, double dt, double* dsum,
+                     double cd, const double* cbot, double zeta, void* stream);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,10 @@ struct LP {
   // fast sub-steps, their length and 1/n per stage (isw_batch)
   const int32_t *wc, *we, *wn;
   const double *wgrav, *wnu4, *wtau, *winv;
+  // tides (Boussinesq layers; NULL / 0 without): the potential per cell, the internal tide's drag per edge, the
+  // bottom's quadratic drag coefficient
+  const double *tide, *cbot;
+  double cd_bot;
 };
 
 static LP make_lp(const isw_layers* s) {
@@ -87,6 +91,9 @@ static LP make_lp(const isw_layers* s) {
   p.wc = b ? b->wc : nullptr; p.we = b ? b->we : nullptr; p.wn = b ? b->nsub : nullptr;
   p.wgrav = b ? b->grav : nullptr; p.wnu4 = b ? b->nu4 : nullptr; p.wtau = b ? b->tau : nullptr;
   p.winv = b ? b->inv_n : nullptr;
+  p.tide = s->air ? nullptr : s->tide;
+  p.cbot = s->air ? nullptr : s->cbot;
+  p.cd_bot = s->air ? 0.0 : s->cd_bot;
   return p;
 }
 
@@ -309,7 +316,9 @@ __global__ __launch_bounds__(512) void k_bern_pv(isw_grid g, LP p, Div dv, int n
       for (int m = L - 1; m >= 0; m--) { s += h[(size_t)m * N + i]; z[m] = s; }
       double Mp = 0;
       for (int m = 1; m <= k; m++) Mp += p.gp[m - 1] * z[m];
-      B[(size_t)k * N + i] = Mp + ke / g.areaCell[i];
+      double Bk = Mp + ke / g.areaCell[i];
+      if (p.tide) Bk -= p.tide[i];                 // (the tide's force grad V on every layer, layers.cu:106)
+      B[(size_t)k * N + i] = Bk;
     }
   } else {
     const int v = ((int)blockIdx.x - nbc) * BS1 + j;
@@ -409,7 +418,22 @@ __global__ __launch_bounds__(BS) void k_slow_e(isw_grid g, LP p, Div dv, double 
     const double sqv = th ? 1.0 : sq0;
     const double* Bl = B + (size_t)k * N;
     double d = Q - DIVBY(sqv * (Bl[eJ] - Bl[eI]), rde) - (p.drag[k] + (1 - sqv) / 21600.0) * ue;
-    const double ck = p.stress[k];
+    double ck = p.stress[k];
+    if (k == L - 1 && (p.cd_bot > 0 || p.cbot)) {
+      // (tides, layers.cu:305-315) the bottom layer's quadratic stress cd |u|, |u| from the normal wind and the TRiSK
+      // tangential wind of the packed neighbours' u; the internal tide's drag
+      if (p.cd_bot > 0) {
+        fast vt = 0;
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+          if (m0 + m >= m1) break;
+          vt += w[m] * pk[((size_t)nb[m] * L + k) * 3 + 1];
+        }
+        for (int m = m0 + NB; m < m1; m++) vt += wf[m] * pk[((size_t)g.eoe[m] * L + k) * 3 + 1];
+        ck += p.cd_bot * sqrt(ue * ue + (double)vt * (double)vt);
+      }
+      if (p.cbot) ck += p.cbot[e];
+    }
     if (ck != 0) {
       const double r = ck / (he > p.floor_ ? he : p.floor_);
       d = d - ((r * dts > 1e-8) ? (1 - exp(-r * dts)) / dts : r) * ue;

@@ -55,7 +55,8 @@ class IswLayers(ctypes.Structure):
                 + [(k, ctypes.c_double * LMAX) for k in ('gp', 'drag', 'stress')]
                 + [(k, _f64p) for k in ('fV', 'b', 'wall', 'Gc')]
                 + [(k, _f64p) for k in LAYER_SCRATCH]
-                + [('pkf', _f32p), ('batch', ctypes.POINTER(IswBatch))])
+                + [('pkf', _f32p), ('batch', ctypes.POINTER(IswBatch))]
+                + [(k, _f64p) for k in ('tide', 'cbot')] + [('cd_bot', ctypes.c_double)])
 
 
 class IswTransport(ctypes.Structure):
@@ -90,6 +91,12 @@ class IswOro(ctypes.Structure):
 class IswGw(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in ('grav', 'p0', 'kappa', 'rd', 'bt', 'cw', 'kw', 'psrc', 'cmax', 'dc',
                                                'dt', 'fc')]
+
+
+class IswSky(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ('tilt', 'year_s', 'day_s', 'ecc', 'lam_peri', 'sun_gm', 'sun_a',
+                                               'moon_gm', 'moon_a', 'moon_e', 'moon_i', 'moon_T', 'node_T', 'aps_T',
+                                               'moon_L0', 'node0', 'aps0', 'love', 'radius')]
 
 
 ISW_GW_MAX_SPEEDS = 4096          # the most phase speeds isw_wave_drag's spectrum may hold
@@ -152,6 +159,9 @@ def lib():
             'isw_air_column_options': [i32, vp],
             'isw_oro_drag': [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp, vp],
             'isw_wave_drag': [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp, vp],
+            'isw_tide_pot': [vp, vp, vp, d, vp, vp, vp],
+            'isw_tide_itd': [vp, i32, vp, vp, vp, vp, vp, d, vp, vp, vp],
+            'isw_tide_mix': [vp, i32, vp, vp, vp, vp, vp, vp, d, d, d, d, vp, d, vp, d, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -170,7 +180,8 @@ EXPORTS = ('isw_layers_step', 'isw_layers_lap', 'isw_layers_gfast', 'isw_layers_
            'isw_advance_time', 'isw_air_column', 'isw_timing_start', 'isw_timing_stop', 'isw_version',
            'isw_grid_check', 'isw_air_column_options', 'isw_ocn_overturn', 'isw_ocn_forcing', 'isw_ocn_column',
            'isw_ocn_xch_salt', 'isw_ocn_pen', 'isw_ocn_fresh', 'isw_ocn_diffuse', 'isw_mesh_lloyd_device',
-           'isw_land_route', 'isw_ice_flow', 'isw_oro_drag', 'isw_wave_drag')
+           'isw_land_route', 'isw_ice_flow', 'isw_oro_drag', 'isw_wave_drag', 'isw_tide_pot',
+           'isw_tide_itd', 'isw_tide_mix')
 
 
 def check(rc, what):

@@ -3,7 +3,7 @@
 core.hip and radiation.hip are compiled twice each: the plain FP64 form with -ffp-contract=off (every product rounds on its own) and the
 mixed form with -ffp-contract=fast (radiation.hip: fast-honor-pragmas, so that its set-up kernel's own
 `fp contract(off)` holds). column.hip is compiled twice too, its cloudy air-column kernels (ISW_COLUMN_CLOUDS=1) apart
-from the rest, so that the two build in parallel; ocean.hip (the ocean's column processes) mesh.hip (Lloyd's iteration) and land.hip (the land's water and ice) and drag.hip (the gravity waves' drag) once, FP64. All HIP code is compiled with packed-FP32 instructions disabled.
+from the rest, so that the two build in parallel; ocean.hip (the ocean's column processes) mesh.hip (Lloyd's iteration) and land.hip (the land's water and ice) drag.hip (the gravity waves' drag) and tides.hip (the tides) once, FP64. All HIP code is compiled with packed-FP32 instructions disabled.
 """
 import os
 import subprocess
@@ -52,6 +52,7 @@ def build_hip(verbose=False, force=True):
              ('mesh.hip', 'mesh.o', ['-ffp-contract=off']),
              ('land.hip', 'land.o', ['-ffp-contract=off']),
              ('drag.hip', 'drag.o', ['-ffp-contract=off']),
+             ('tides.hip', 'tides.o', ['-ffp-contract=off']),
              ('capi.hip', 'capi.o', ['-ffp-contract=off'])]
     # the units compile in parallel (each hipcc is one process; at most ISW_BUILD_JOBS at a time, default one per unit
     # up to the CPU count)

@@ -80,6 +80,37 @@ class Layers:
             self._set_batch(batch)
         self.dmin = float(np.min(dg.host.de))
         self.nu4dt = 0.0
+        self._tide = self._cbot = None
+        self._cd_bot = 0.0
+
+    # tides (reference: layers.py:57-58, set by tides.Tides): the potential per cell, the internal tide's drag per
+    # edge and the bottom's quadratic drag coefficient, read by the step's kernels through the struct
+    @property
+    def tide(self):
+        return self._tide
+
+    @tide.setter
+    def tide(self, t):
+        self._tide = t
+        self.s.tide = ctypes.cast(t.data_ptr(), _hip._f64p) if t is not None else None
+
+    @property
+    def cbot(self):
+        return self._cbot
+
+    @cbot.setter
+    def cbot(self, t):
+        self._cbot = t
+        self.s.cbot = ctypes.cast(t.data_ptr(), _hip._f64p) if t is not None else None
+
+    @property
+    def cd_bot(self):
+        return self._cd_bot
+
+    @cd_bot.setter
+    def cd_bot(self, v):
+        self._cd_bot = float(v)
+        self.s.cd_bot = float(v)
 
     def _set_batch(self, b):
         """A batch of worlds on a union mesh (insilica.batch): per-world gravity, del^4 coefficient and fast

@@ -276,6 +276,16 @@ class World:
                     if self.physics is None or getattr(self, 'ice', None) is None:
                         raise ValueError('the waves\' convective source (gw_conv) needs the moist column')
                     self.gw.conv = self.physics.rainp[2]
+        # the Moon's and the Sun's tides (reference: world.py:199-208): params['tides'] True / {} for Earth's, or a dict
+        # of changes; params 'rough' (N, the floor's roughness h^2) adds the internal tide's drag, 'nbot' its N
+        self.tides = None
+        tides = P.get('tides')
+        if tides is not None and tides is not False:
+            if self.sea is None or self.physics is None:
+                raise ValueError('tides need the coupled ocean under air')
+            from .tides import Tides
+            self.tides = Tides(dg, self.sea, self.physics.p, self.planet['radius'], tides, rough=P.get('rough'),
+                               nbot=P.get('nbot'))
         self.M = self.h * dg.areaCell[None, :]
         if nu4:
             lam = self.core.lap_max()
@@ -351,7 +361,11 @@ class World:
         if self.physics is not None:
             self.core.exchange(self.physics.wx, self.h, self.u, self.M, self.psi, theta=self.theta)
         if kind & 1:
+            if self.tides is not None:             # (the force at the ocean step's middle)
+                self.tides.force(self.time, 0.5 * sea.cfg['dt'])
             sea.step(mix=bool(kind & 4))
+            if self.tides is not None:
+                self.tides.mix(sea.cfg['dt'])
 
     def launch_tick(self, graph=True):
         """Queue one tick on the world's stream without waiting for it (step() and Ensemble.step() call this)."""

@@ -22,10 +22,10 @@ _i32p, _i8p, _f64p = (ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_in
 
 
 def build():
-    """Compile liboracle.so from oracle.c, radiation.c, column.c, ocean.c, land.c and drag.c (seconds)."""
+    """Compile liboracle.so from oracle.c, radiation.c, column.c, ocean.c, land.c, drag.c and tides.c (seconds)."""
     cmd = ['gcc', '-O2', '-ffp-contract=off', '-fopenmp', '-fPIC', '-shared', '-o', str(LIB) + '.tmp',
            str(HERE / 'oracle.c'), str(HERE / 'radiation.c'), str(HERE / 'column.c'), str(HERE / 'ocean.c'),
-           str(HERE / 'land.c'), str(HERE / 'drag.c'), '-lm']
+           str(HERE / 'land.c'), str(HERE / 'drag.c'), str(HERE / 'tides.c'), '-lm']
     subprocess.run(cmd, check=True)
     os.replace(str(LIB) + '.tmp', LIB)
 
@@ -43,7 +43,8 @@ class OGLayersCfg(ctypes.Structure):
                 + [(k, ctypes.c_double) for k in ('grav', 'cp', 'kappa', 'p0', 'rd', 'nu4', 'floor_', 'div_damp', 'thin',
                                                   'shelf', 'vmix', 'vmix_cd', 'thin_wind')]
                 + [(k, ctypes.c_double * LMAX) for k in ('gp', 'drag', 'stress')]
-                + [(k, _f64p) for k in ('fV', 'b', 'wall')])
+                + [(k, _f64p) for k in ('fV', 'b', 'wall', 'tide', 'cbot')]
+                + [('cd_bot', ctypes.c_double)])
 
 
 _lib = None
@@ -58,6 +59,7 @@ def lib():
         vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
         sig = {
             'og_layers_new': (vp, [vp, vp]), 'og_layers_free': (None, [vp]), 'og_layers_Gc': (vp, [vp]),
+            'og_layers_tides': (None, [vp, vp, vp, d]),
             'og_layers_L2': (vp, [vp]), 'og_layers_diss': (vp, [vp]),
             'og_layers_gfast': (None, [vp, vp, vp]), 'og_layers_substeps': (i, [vp, vp, d]),
             'og_layers_step': (None, [vp, vp, vp, d, i, vp, vp]), 'og_layers_lap': (None, [vp, vp, vp]),
@@ -89,6 +91,9 @@ def lib():
             'og_wave_drag': (None, [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp]),
             'og_gw_mean': (d, [i, vp, vp]),
             'og_blk_int': (d, [d, d]),
+            'og_tide_pot': (None, [i, vp, d, d, vp, vp]),
+            'og_tide_mix': (None, [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, d, d, d, d, d, d, d, vp, d, vp, d]),
+            'og_tide_itd': (None, [i, i, i, vp, vp, vp, vp, vp, vp, d, d, d, d, d, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -207,6 +212,13 @@ class Layers:
     @property
     def diss(self):
         return self._view(lib().og_layers_diss, self.og.E)
+
+    def set_tides(self, tide, cbot, cd_bot):
+        """The tides' hooks (reference: layers.py:57-58, set by tides.Tides): the potential per cell (N, read at each
+        stage), the internal tide's drag per edge (E or None) and the bottom's quadratic drag coefficient. The arrays
+        are held, not copied: the caller updates them in place."""
+        self._tide, self._cbot = tide, cbot
+        lib().og_layers_tides(self.h_, _p(tide), _p(cbot), float(cd_bot))
 
     def update_gfast(self, h, theta):
         lib().og_layers_gfast(self.h_, _p(h), _p(theta))

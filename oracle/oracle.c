@@ -134,9 +134,9 @@ static inline double cell_ke(const og_grid* g, const double* ul, int i) {
   return ke;
 }
 
-/* lay_bern (layers.cu:83-109), sig/tide NULL */
+/* lay_bern (layers.cu:83-109), sig NULL; tide (N) the tide-generating potential or NULL */
 static void lay_bern(const og_grid* g, int L, const double* h, const double* u, const double* b, const double* gp,
-                     double* B) {
+                     const double* tide, double* B) {
   const int N = g->N;
   PAR FOR_N(i, N) {
     double z[OG_LMAX];
@@ -146,7 +146,9 @@ static void lay_bern(const og_grid* g, int L, const double* h, const double* u, 
     for (int k = 0; k < L; k++) {
       if (k > 0) Mp += gp[k - 1] * z[k];
       const double ke = cell_ke(g, u + (size_t)k * g->E, i);
-      B[(size_t)k * N + i] = Mp + ke / g->areaCell[i];
+      double Bk = Mp + ke / g->areaCell[i];
+      if (tide) Bk -= tide[i];
+      B[(size_t)k * N + i] = Bk;
     }
   }
 }
@@ -290,11 +292,12 @@ static void lay_edge_hq(const og_grid* g, int L, const double* h, const double* 
   }
 }
 
-/* lay_slow (layers.cu:269-359) with sig, zm, ob, cbot NULL and cd_bot 0 */
+/* lay_slow (layers.cu:269-359) with sig, zm, ob NULL; the bottom layer's quadratic drag cd_bot and the internal
+ * tide's drag cbot (E or NULL) as the reference's tides set them */
 static void lay_slow(const og_grid* g, int L, const double* pk, const double* hedge, const double* u, const double* B,
                      const double* drag, const double* stress, double floor_, double nu4, const double* lap4,
                      const double* theta, const double* pibar, double dts, double thin, double shelf, double* Tp,
-                     double* G, double* He, double* ubs) {
+                     double* G, double* He, double* ubs, double cd_bot, const double* cbot) {
   const int N = g->N, E = g->E;
   PAR FOR_N(e, E) {
     double T[OG_LMAX], hev[OG_LMAX], sqk[OG_LMAX];
@@ -319,7 +322,16 @@ static void lay_slow(const og_grid* g, int L, const double* pk, const double* he
       const double sq = theta ? 1.0 : sq0;
       sqk[k] = sq0;
       double d = Q - sq * (Bl[eJ] - Bl[eI]) / g->de[e] - (drag[k] + (1 - sq) / 21600.0) * ul[e];
-      const double ck = stress[k];
+      double ck = stress[k];
+      if (k == L - 1 && (cd_bot > 0 || cbot)) {
+        /* |u| from the edge's normal wind and its TRiSK tangential wind (the packed neighbours' u) */
+        if (cd_bot > 0) {
+          double vt = 0;
+          for (int m = g->eoeStart[e]; m < g->eoeStart[e + 1]; m++) vt += g->w[m] * pk[((size_t)g->eoe[m] * L + k) * 3 + 1];
+          ck += cd_bot * sqrt(ul[e] * ul[e] + vt * vt);
+        }
+        if (cbot) ck += cbot[e];
+      }
       if (ck != 0) {
         const double r = ck / (he > floor_ ? he : floor_);
         d = d - ((r * dts > 1e-8) ? (1 - exp(-r * dts)) / dts : r) * ul[e];
@@ -618,6 +630,12 @@ static void xch_edge(const og_grid* g, int L, const double* wx, const double* ho
 
 /* ---- Layers (src/insilica/layers.py) ---- */
 
+void og_layers_tides(og_layers* s, const double* tide, const double* cbot, double cd_bot) {
+  s->c.tide = tide;
+  s->c.cbot = cbot;
+  s->c.cd_bot = cd_bot;
+}
+
 og_layers* og_layers_new(const og_grid* g, const og_layers_cfg* cfg) {
   og_layers* s = (og_layers*)calloc(1, sizeof(og_layers));
   s->g = *g;
@@ -690,11 +708,12 @@ static void stage(og_layers* s, const double* hs, const double* us, const double
   if (C->air)
     air_bern(g, L, hs, us, theta, C->b, C->grav, C->cp, C->kappa, C->p0, s->S, s->B, s->pibar);
   else
-    lay_bern(g, L, hs, us, C->b, C->gp, s->B);
+    lay_bern(g, L, hs, us, C->b, C->gp, C->tide, s->B);
   swe_vertex(g, L, hs, us, C->fV, s->q);
   lay_edge_hq(g, L, hs, s->q, C->shelf, us, s->hedge, s->pk);
   lay_slow(g, L, s->pk, s->hedge, us, s->B, C->drag, C->stress, C->floor_, C->nu4, C->nu4 != 0 ? s->L2 : 0,
-           C->air ? theta : 0, C->air ? s->pibar : 0, c * dt, C->thin, C->shelf, s->Tp, s->G, s->He, s->ubs);
+           C->air ? theta : 0, C->air ? s->pibar : 0, c * dt, C->thin, C->shelf, s->Tp, s->G, s->He, s->ubs,
+           C->air ? 0.0 : C->cd_bot, C->air ? 0 : C->cbot);
   int n = (int)ceil(c * nfast);
   if (n < 1) n = 1;
   const double tau = c * dt / n;

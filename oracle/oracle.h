@@ -31,11 +31,17 @@ typedef struct og_layers_cfg {
   const double *fV;   /* (NV) the Coriolis parameter with rotation applied */
   const double *b;    /* (N) bottom */
   const double *wall; /* (E) or NULL */
+  /* tides (reference: layers.py:57-58): the tide-generating potential per cell (N) subtracted from every Boussinesq
+   * layer's Bernoulli function, the bottom layer's quadratic drag coefficient and the internal tide's drag per edge
+   * (E, m/s); NULL / 0 without tides */
+  const double *tide, *cbot;
+  double cd_bot;
 } og_layers_cfg;
 
 typedef struct og_layers og_layers; /* opaque: the settings plus preallocated scratch */
 
 og_layers* og_layers_new(const og_grid* g, const og_layers_cfg* cfg);
+void og_layers_tides(og_layers* s, const double* tide, const double* cbot, double cd_bot);
 void og_layers_free(og_layers* s);
 double* og_layers_Gc(og_layers* s);  /* (N) the fast mode's coefficient (grav at creation) */
 double* og_layers_L2(og_layers* s);  /* (L,E) del^4 of the last step's start wind */

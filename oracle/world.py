@@ -80,8 +80,20 @@ class OracleWorld:
                             c_soil=s.soil['c'] if s.soil is not None else 0.0,
                             k_soil=s.soil['k'] if s.soil is not None else 0.0,
                             Fm=np.zeros((s.L, og.E)), M1=np.zeros((s.L, og.N)),
-                            eos=(s.cfg['alpha'], s.cfg['alpha2'], s.cfg['beta']))
+                            eos=(s.cfg['alpha'], s.cfg['alpha2'], s.cfg['beta']), c_bot=s.cfg['bottom_stress'])
             self.sea_mix = bool(getattr(w, 'sea_mix', False))
+            # the tides (the product World's settings): the sky, the cells' positions, the potential the ocean's
+            # layers read, the internal tide's drag per edge
+            self.tides = None
+            if getattr(w, 'tides', None) is not None:
+                from insilica.tides import sky_array
+                t = w.tides
+                self.tides = dict(tp=sky_array(t.sky), pos=np.ascontiguousarray(w.grid.pos, dtype=float),
+                                  V=np.zeros(og.N), dsum=_np(t.dsum), cfg=dict(t.cfg),
+                                  h2=_np(t.h2) if t.h2 is not None else None,
+                                  nbot=_np(t.nbot) if t.nbot is not None else None,
+                                  cbot=np.zeros(og.E) if t.h2 is not None else None)
+                self.score.set_tides(self.tides['V'], self.tides['cbot'], t.cfg['cd'])
             if self.sea_mix:                       # the coupled worlds' mixed layer: its forcing sums, its settings
                 from insilica.ocean import ALPHA_REF
                 c = s.cfg
@@ -179,6 +191,13 @@ class OracleWorld:
         if kind & 1:
             from .ocean import overturn
             Lo, N = s['L'], og.N
+            td = getattr(self, 'tides', None)
+            if td is not None:
+                from .tides import tide_itd, tide_pot
+                tide_pot(og, td['tp'], self.time, 0.5 * s['dt'], td['pos'], td['V'])
+                if td['h2'] is not None:
+                    tide_itd(og, Lo, s['h'], s['rows'][:Lo], s['rows'][Lo:], td['h2'], td['cfg']['kappa_it'],
+                             (self.score.grav,) + tuple(s['eos']), td['cbot'], td['nbot'])
             overturn(N, Lo, s['h'], s['rows'][:Lo], s['rows'][Lo:], self.score.grav, s['eos'][0], s['eos'][1],
                      s['eos'][2], land=s['land'])
             self.score.step(s['h'], s['u'], s['dt'], s['nfast'], s['Fm'])
@@ -188,6 +207,11 @@ class OracleWorld:
             ocn_cap(og, s['h'], self.rho * self.cw, s['cap'], s['land'], self.c_land)
             if kind & 4:
                 self._mix()
+            if td is not None:
+                from .tides import tide_mix
+                tide_mix(og, Lo, s['h'], s['u'], s['rows'][:Lo], s['rows'][Lo:], s['land'], s['c_bot'], self.rho,
+                         td['cfg']['gamma'], s['dt'], (self.score.grav,) + tuple(s['eos']), td['dsum'],
+                         td['cfg']['cd'], td['cbot'], td['cfg']['zeta'])
 
     def _mix(self):
         """Ocean.step's mixing (ocean.py:257-295): the interval's forcing, the mixed layer's column step, salt with the
