@@ -36,6 +36,8 @@ def parse():
     ap.add_argument('--cpu-hours', type=float, default=0.0,
                     help='model hours of the CPU sample (0: 48 for coupled_stack, 150 for the aquaplanets)')
     ap.add_argument('--ensemble', type=int, default=0, help='run the batched-ensemble workload with this many worlds')
+    ap.add_argument('--no-ensemble', action='store_true',
+                    help='skip the default line\'s ensemble leg (64 batched worlds at levels 4-6, one model day)')
     ap.add_argument('--ens-days', type=int, default=1, help='model days the ensemble advances')
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--batched', action='store_true', help='ensemble: batch the worlds of each level on one mesh')
@@ -117,23 +119,23 @@ def kernel_profile(w, hours=1):
     return _hip.timing_stop()
 
 
-def ensemble_main(a):
-    """The batched-ensemble workload: `a.ensemble` coupled worlds with seeded levels (4/5/6), planet radius, rotation
-    and gravity, mixed form, all advanced together by `a.ens_days` model days; one JSON line with the aggregate
-    model-hours/s and the p50/p99 wall time per model day over the members' days."""
+def ensemble_run(n, days, seed, batched, roofline_leg=True):
+    """The batched-ensemble workload: n coupled worlds with seeded levels (4/5/6), planet radius, rotation and gravity,
+    mixed form, all advanced together by `days` model days. Returns (worlds, levels, hours, seconds, the per-day
+    latencies, the largest batch's dominant-kernel roofline or None)."""
     import numpy as np
     import torch
     from insilica.ensemble import Ensemble, draw_planets
     from insilica.world import World
     torch.cuda.set_device(0)
-    specs = draw_planets(a.ensemble, seed=a.seed)
+    specs = draw_planets(n, seed=seed)
     t0 = time.perf_counter()
     worlds = [World(lev, 'coupled_stack', precision='mixed', params=p) for lev, p in specs]
     print(f'built {len(worlds)} worlds in {time.perf_counter() - t0:.1f} s', file=sys.stderr, flush=True)
-    ens = Ensemble(worlds, batch=a.batched)
+    ens = Ensemble(worlds, batch=batched)
     ens.advance(1)                                   # warm-up: the first tick of each kind runs directly, then capture
     torch.cuda.synchronize()
-    hours = 24 * a.ens_days
+    hours = 24 * days
     t0 = time.perf_counter()
     ens.advance(hours, day_marks=True)
     torch.cuda.synchronize()
@@ -141,7 +143,7 @@ def ensemble_main(a):
     lat = np.concatenate([np.asarray(x) for x in ens.day_latencies()])
     levels = [lev for lev, _ in specs]
     roofline = None
-    if a.batched:
+    if batched and roofline_leg:
         # the largest batch's dominant kernel over one model hour of direct launches (the library's event timing)
         from insilica import _hip
         unit = max(ens.units, key=lambda u: u.grid.N if hasattr(u, 'grid') else 0)
@@ -158,15 +160,28 @@ def ensemble_main(a):
                         frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
                         bytes_per_launch=k['bytes'] / k['launches'], avg_us=round(avg_s * 1e6, 2),
                         batch=f'{len(unit)} worlds, {unit.grid.N} cells')
+    del ens
+    return worlds, levels, hours, el, lat, roofline
+
+
+def ensemble_workload(n, levels, days, batched):
+    return (f'{n} coupled_stack worlds, levels {sorted(set(levels))} ({levels.count(4)}/{levels.count(5)}/'
+            f'{levels.count(6)} at 4/5/6), seeded planets (radius 0.5-1.5 Earth, rotation 0.5-2x, gravity 3.7-11), '
+            f'{days} model days, ' + ('one batch (union mesh) per level, one launch per phase' if batched
+                                       else 'one stream and graph per world'))
+
+
+def ensemble_main(a):
+    """The batched-ensemble workload on its own: one JSON line with the aggregate model-hours/s and the p50/p99 wall
+    time per model day over the members' days."""
+    import numpy as np
+    worlds, levels, hours, el, lat, roofline = ensemble_run(a.ensemble, a.ens_days, a.seed, a.batched)
     line = dict(metric='ensemble model-hours/s and wall seconds per model day (p50/p99), 1 GPU',
                 value=round(len(worlds) * hours / el, 2), unit='model-hours/s', n_gpus=1, steps=hours, warmup=1,
                 ms_per_step=round(el / hours * 1e3, 3), higher_is_better=True, scaling='weak', vs_baseline=None,
                 dtype='f64 state, f32 stencils (mixed)', data='synthetic',
-                config=dict(workload=f'{len(worlds)} coupled_stack worlds, levels {sorted(set(levels))} '
-                                     f'({levels.count(4)}/{levels.count(5)}/{levels.count(6)} at 4/5/6), seeded planets '
-                                     f'(radius 0.5-1.5 Earth, rotation 0.5-2x, gravity 3.7-11), {a.ens_days} model days',
-                            parallelism='1 GPU, ' + ('one batch (union mesh) per level, one launch per phase'
-                                                     if a.batched else 'one stream and graph per world')),
+                config=dict(workload=ensemble_workload(len(worlds), levels, a.ens_days, a.batched),
+                            parallelism='1 GPU'),
                 day_p50_s=round(float(np.percentile(lat, 50)), 4), day_p99_s=round(float(np.percentile(lat, 99)), 4),
                 days_measured=int(lat.size), roofline=roofline)
     print(json.dumps(line), flush=True)
@@ -452,6 +467,22 @@ def main():
                    sample=f'5 runs of {ticks // 5} ticks ({ticks // 5 / tph:g} model hours) each of the same level-'
                           f'{a.level} {a.preset} world after a model hour of warm-up, median; FP64 oracle (C, '
                           f'OpenMP), {el:.1f} s in all')
+    # the metric's second half: a 64-world ensemble at levels 4/5/6, batched, one model day, p50/p99 per model day
+    ens = None
+    tph_w, cells_w, edges_w = int(round(wm.ticks_per_hour)), wm.grid.N, wm.grid.E
+    if not a.no_ensemble and a.level == 6 and a.preset == 'coupled_stack':
+        del wm
+        import gc
+        gc.collect()
+        torch.cuda.empty_cache()
+        worlds, levels, hours, el_e, lat, _ = ensemble_run(64, 1, a.seed, True, roofline_leg=False)
+        ens = dict(worlds=len(worlds), model_hours_per_s=round(len(worlds) * hours / el_e, 2),
+                   day_p50_s=round(float(np.percentile(lat, 50)), 4), day_p99_s=round(float(np.percentile(lat, 99)), 4),
+                   days_measured=int(lat.size), seconds=round(el_e, 3),
+                   workload=ensemble_workload(len(worlds), levels, 1, True))
+        print(f'ensemble: {len(worlds)} worlds, a model day in {el_e:.2f} s', file=sys.stderr, flush=True)
+        del worlds
+        wm = None
     ms = el_m / a.steps * 1e3
     what, published = PRESET_LINES[a.preset]
     value = a.steps / el_m
@@ -462,13 +493,13 @@ def main():
                 vs_baseline=round(value / published, 3) if published and a.level == 6 else None,
                 dtype='f64 state, f32 stencils (mixed)', data='synthetic',
                 config=dict(workload=f'{a.preset} level {a.level}: {what}; step = 1 model hour '
-                                     f'({int(round(wm.ticks_per_hour))} ticks)',
-                            cells=wm.grid.N, edges=wm.grid.E, parallelism='1 GPU'),
+                                     f'({tph_w} ticks)',
+                            cells=cells_w, edges=edges_w, parallelism='1 GPU'),
                 baseline=(dict(value=published, unit='model-hours/s', hardware='RTX 4070 (the reference, mixed)',
                                source='SPEC.md:560') if published and a.level == 6 else None),
                 fp64_value=round(fp64, 3) if fp64 else None,
                 mixed_over_fp64=round(a.steps / el_m / fp64, 3) if fp64 else None,
-                roofline=roofline, cpu_baseline=cpu,
+                roofline=roofline, cpu_baseline=cpu, ensemble=ens,
                 kernels={k2: dict(launches=v['launches'], ms=round(v['ms'], 4),
                                   GBs=round(v['bytes'] / (v['ms'] / 1e3) / 1e9, 1) if v['ms'] > 0 else None)
                          for k2, v in sorted(prof.items(), key=lambda kv: -kv[1]['ms'])})

@@ -116,6 +116,9 @@ typedef struct isw_layers {
    * times |u| from the edge's normal and TRiSK tangential winds). */
   const double *tide, *cbot;
   double cd_bot;
+  /* (L,E) floats, the mixed form's FP32 copy of the stage's edge thicknesses that the air's fast sub-steps gather
+   * (NULL: they gather the FP64 hedge; always NULL in the plain FP64 form) */
+  float *hedgef;
 } isw_layers;
 
 /* FCT transport scratch for up to rows_max rows (reference: src/insilica/transport.py:19-30 Transport.__init__).

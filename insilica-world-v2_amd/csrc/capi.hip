@@ -112,6 +112,7 @@ bool layers_ok(const isw_grid* g, const isw_layers* s) {
   if (!grid_ok(g) || !s || s->L < 1 || s->L > ISW_LMAX) return false;
   if (s->prec != ISW_FP64 && s->prec != ISW_MIXED) return false;
   if (s->prec == ISW_MIXED && (!s->pkf || !g->wf || !g->rA || !g->rde || !g->rAV)) return false;
+  if (s->prec == ISW_MIXED && s->air && g->ph_off && !s->hedgef) return false;   // (the patch kernels gather it)
   if (s->prec == ISW_FP64 && !s->pk) return false;
   const isw_batch* b = s->batch;
   if (b && (b->nworlds < 1 || !b->wc || !b->we || !b->nsub || !b->grav || !b->nu4 || !b->tau || !b->inv_n ||

@@ -345,7 +345,8 @@ __global__ __launch_bounds__(512) void k_bern_pv(isw_grid g, LP p, Div dv, int n
 // the PV flux's neighbour gathers.
 __global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf, const double* __restrict__ h,
                                                 const double* __restrict__ q, const double* __restrict__ u,
-                                                double* __restrict__ he, fast* __restrict__ pk) {
+                                                double* __restrict__ he, fast* __restrict__ pk,
+                                                float* __restrict__ hef) {
   const int t = XTID;
   if (t >= g.E * L) return;
   const int l = t / g.E, e = t - l * g.E;
@@ -353,6 +354,7 @@ __global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf,
   const double hv = edge_h(h[(size_t)l * N + g.eI[e]], h[(size_t)l * N + g.eJ[e]], shelf);
   const double qv = 0.5 * (q[(size_t)l * NV + g.ev1[e]] + q[(size_t)l * NV + g.ev2[e]]);
   he[t] = hv;
+  if (hef) hef[t] = (float)hv;                     // (mixed form: what the air's fast sub-steps gather)
   fast* pp = pk + ((size_t)e * L + l) * 3;
   pp[0] = (fast)hv; pp[1] = (fast)u[t]; pp[2] = (fast)qv;
 }
@@ -554,7 +556,7 @@ __global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, LP p, in
 constexpr int FBS = ISW_PAIR_THREADS;
 template <int L>
 __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
-                                                  const double* __restrict__ hedge, const double* __restrict__ uin,
+                                                  const fast* __restrict__ hedge, const double* __restrict__ uin,
                                                   const double* __restrict__ mu_src, double* __restrict__ mu_out,
                                                   const double* __restrict__ S, const double* __restrict__ Tp,
                                                   const double* __restrict__ G, double* __restrict__ acc,
@@ -600,7 +602,7 @@ __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int 
     double m1k[L], sak[L];
 #pragma unroll
     for (int k = 0; k < L; k++) {
-      const double* hl = hedge + (size_t)k * E;
+      const fast* hl = hedge + (size_t)k * E;
       const double* ul = uin + (size_t)k * E;
       double t[6];
 #pragma unroll
@@ -678,7 +680,7 @@ __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int 
 // so that C reads no global memory. Halo edges past the block's FBS threads (a few at level 6) load as they go.
 template <int L>
 __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
-                                                   const double* __restrict__ hedge, const double* __restrict__ uin,
+                                                   const fast* __restrict__ hedge, const double* __restrict__ uin,
                                                    const double* __restrict__ mu_src, double* __restrict__ mu_out,
                                                    const double* __restrict__ S, const double* __restrict__ Tp,
                                                    const double* __restrict__ G, double* __restrict__ acc,
@@ -770,7 +772,7 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
     double sak[L], Sk[L];
 #pragma unroll
     for (int k = 0; k < L; k++) {
-      const double* hl = hedge + (size_t)k * E;
+      const fast* hl = hedge + (size_t)k * E;
       const double* ul = uin + (size_t)k * E;
       double t[6];
 #pragma unroll
@@ -1773,12 +1775,18 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
 #if ISW_FORM_MIXED
   fast* pk = s->pkf;
   const fast* wf = g->wf;
+  // the air's fast sub-steps gather the edge thicknesses in FP32 (the limiter's flux keeps the FP64 ones)
+  float* fast_he = s->air ? s->hedgef : nullptr;
+  const fast* fhedge = s->hedgef;
 #else
   fast* pk = s->pk;
   const fast* wf = g->w;
+  float* fast_he = nullptr;
+  const fast* fhedge = s->hedge;
 #endif
-  ISW_LAUNCH("edge_hq", l_ * n_ * D + l_ * v_ * D + l_ * e_ * D + 4 * e_ * I + l_ * e_ * D + 3 * l_ * e_ * R,
-             k_edge_hq, g1((long)E * L), BS, *g, L, s->shelf, hs, s->q, us, s->hedge, pk);
+  ISW_LAUNCH("edge_hq", l_ * n_ * D + l_ * v_ * D + l_ * e_ * D + 4 * e_ * I + l_ * e_ * D + 3 * l_ * e_ * R +
+                            (fast_he ? l_ * e_ * 4 : 0),
+             k_edge_hq, g1((long)E * L), BS, *g, L, s->shelf, hs, s->q, us, s->hedge, pk, fast_he);
   const double neoe = g->n_eoe;
   const double slow_bytes = 3 * l_ * e_ * R + neoe * (R + I) + (e_ + 1) * I + 2 * l_ * e_ * D + l_ * n_ * D +
                             2 * e_ * I + e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + air * 2 * l_ * n_ * D + l_ * e_ * D +
@@ -1831,12 +1839,12 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
       for (; it + 1 < n; it += 2, flip ^= 1) {
         double* uo = flip ? s->ufb : s->uf;
         double* mo = flip ? s->mup : s->muf;
-        const double bytes = 2 * l_ * e_ * D + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
+        const double bytes = l_ * e_ * (D + sizeof(fast)) + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
                              (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D +
                              e_ * D + 2 * e_ * I;
 #define PAIR(LL)                                                                                                  \
   ISW_LAUNCH_SHM("af_patch2", bytes, k_af_patch2<LL>, dim3((unsigned)g->npatch), dim3(FBS), pair_lds, *g, p, dv,  \
-                 si, it, n, tau, s->hedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
+                 si, it, n, tau, fhedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
         switch (L) {
           case 1: PAIR(1); break;
           case 2: PAIR(2); break;
@@ -1857,12 +1865,12 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
       double* mo = flip ? s->mup : s->muf;
       // (algorithmic bytes: hedge, uin, mu in and out, S, Tp, G, acc, the wind out, the cell ring and signs, le,
       // 1/A, de, the patch's edge list and slots; the halo's re-reads and Phi* in LDS are free)
-      const double bytes = 2 * l_ * e_ * D + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
+      const double bytes = l_ * e_ * (D + sizeof(fast)) + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
                            (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D +
                            e_ * D + 2 * e_ * I;
 #define PATCH(LL)                                                                                                 \
   ISW_LAUNCH_SHM("af_patch", bytes, k_af_patch<LL>, dim3((unsigned)g->npatch), dim3(FBS), patch_lds, *g, p, dv, si, \
-                 it, n, tau, s->hedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
+                 it, n, tau, fhedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
       switch (L) {
         case 1: PATCH(1); break;
         case 2: PATCH(2); break;

@@ -56,7 +56,8 @@ class IswLayers(ctypes.Structure):
                 + [(k, _f64p) for k in ('fV', 'b', 'wall', 'Gc')]
                 + [(k, _f64p) for k in LAYER_SCRATCH]
                 + [('pkf', _f32p), ('batch', ctypes.POINTER(IswBatch))]
-                + [(k, _f64p) for k in ('tide', 'cbot')] + [('cd_bot', ctypes.c_double)])
+                + [(k, _f64p) for k in ('tide', 'cbot')] + [('cd_bot', ctypes.c_double)]
+                + [('hedgef', _f32p)])
 
 
 class IswTransport(ctypes.Structure):

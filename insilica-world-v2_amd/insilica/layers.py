@@ -73,6 +73,9 @@ class Layers:
         for k in _hip.LAYER_SCRATCH:
             setattr(s, k, f64(getattr(self, k)))
         s.pkf = ctypes.cast(self.pkf.data_ptr(), _hip._f32p) if self.pkf is not None else None
+        # (the mixed form's FP32 edge thicknesses for the air's fast sub-steps)
+        self.hedgef = z(L, E, dtype=torch.float32) if dg.mixed and air else None
+        s.hedgef = ctypes.cast(self.hedgef.data_ptr(), _hip._f32p) if self.hedgef is not None else None
         self.s = s
         self.ref = ctypes.byref(s)
         self.batch = None
