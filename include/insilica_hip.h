@@ -73,6 +73,9 @@ typedef struct isw_grid {
    * ptedges: the most edges a patch uses (ne + its incoming edges). */
   int32_t ptedges, pad3_;
   const int32_t *pi_off, *pi_edge, *tc_slot;
+  /* eslot (E x 2): each edge's slot in its first and its second cell's cellEdge ring (required with ph_off: the
+   * stage's coefficients are written at those slots) */
+  const int8_t *eslot;
 } isw_grid;
 
 /* A batch of worlds stepped together on one mesh: the disjoint union of the worlds' meshes (all of one level, so of
@@ -116,9 +119,11 @@ typedef struct isw_layers {
    * times |u| from the edge's normal and TRiSK tangential winds). */
   const double *tide, *cbot;
   double cd_bot;
-  /* (L,E) floats, the mixed form's FP32 copy of the stage's edge thicknesses that the air's fast sub-steps gather
-   * (NULL: they gather the FP64 hedge; always NULL in the plain FP64 form) */
-  float *hedgef;
+  /* The air's fast sub-steps' coefficients per cell slot and layer, sign * l_e * h_e at [(layer * N + cell) * 6 +
+   * slot] (L*N*6), written once per stage with the edge thicknesses: cfc (doubles, plain FP64 form) or cfcf (floats,
+   * mixed form); the form's one is required for air layers when the grid has patch tables. */
+  double *cfc;
+  float *cfcf;
 } isw_layers;
 
 /* FCT transport scratch for up to rows_max rows (reference: src/insilica/transport.py:19-30 Transport.__init__).

@@ -94,7 +94,7 @@ bool grid_ok(const isw_grid* g) {
         g->le && g->de && g->areaCell))
     return false;
   // the fast sub-steps' patches, when given: every table, a patch count that covers the cells, slots for a patch
-  if (g->ph_off && !(g->ph_cell && g->pe_off && g->pe_edge && g->pe_slot && g->pcells > 0 &&
+  if (g->ph_off && !(g->ph_cell && g->pe_off && g->pe_edge && g->pe_slot && g->eslot && g->pcells > 0 &&
                      g->npatch == (g->N + g->pcells - 1) / g->pcells && g->pslots >= g->pcells &&
                      g->pslots < (1 << 15) && g->pown > 0))
     return false;
@@ -112,7 +112,8 @@ bool layers_ok(const isw_grid* g, const isw_layers* s) {
   if (!grid_ok(g) || !s || s->L < 1 || s->L > ISW_LMAX) return false;
   if (s->prec != ISW_FP64 && s->prec != ISW_MIXED) return false;
   if (s->prec == ISW_MIXED && (!s->pkf || !g->wf || !g->rA || !g->rde || !g->rAV)) return false;
-  if (s->prec == ISW_MIXED && s->air && g->ph_off && !s->hedgef) return false;   // (the patch kernels gather it)
+  // (the patch kernels read the stage's coefficients per cell slot)
+  if (s->air && g->ph_off && (s->prec == ISW_MIXED ? !s->cfcf : !s->cfc)) return false;
   if (s->prec == ISW_FP64 && !s->pk) return false;
   const isw_batch* b = s->batch;
   if (b && (b->nworlds < 1 || !b->wc || !b->we || !b->nsub || !b->grav || !b->nu4 || !b->tau || !b->inv_n ||

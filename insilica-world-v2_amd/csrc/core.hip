@@ -66,6 +66,19 @@ __device__ __forceinline__ void ring6(const int* __restrict__ tab, int i, int (&
 }
 __device__ __forceinline__ int ok0(int e) { return e < 0 ? 0 : e; }
 
+// A cell's six per-slot coefficients of one layer (contiguous, 16-byte aligned as doubles, 8-byte as floats), in
+// three vector loads.
+__device__ __forceinline__ void load6(const double* p, double (&c)[6]) {
+  const double2* q = reinterpret_cast<const double2*>(p);
+  const double2 a = q[0], b = q[1], d = q[2];
+  c[0] = a.x; c[1] = a.y; c[2] = b.x; c[3] = b.y; c[4] = d.x; c[5] = d.y;
+}
+__device__ __forceinline__ void load6(const float* p, double (&c)[6]) {
+  const float2* q = reinterpret_cast<const float2*>(p);
+  const float2 a = q[0], b = q[1], d = q[2];
+  c[0] = a.x; c[1] = a.y; c[2] = b.x; c[3] = b.y; c[4] = d.x; c[5] = d.y;
+}
+
 // Settings of a layer stack passed by value (reference: Layers.__init__, src/insilica/layers.py:161-218).
 struct LP {
   int L, air;
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(512) void k_bern_pv(isw_grid g, LP p, Div dv, int n
 __global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf, const double* __restrict__ h,
                                                 const double* __restrict__ q, const double* __restrict__ u,
                                                 double* __restrict__ he, fast* __restrict__ pk,
-                                                float* __restrict__ hef) {
+                                                fast* __restrict__ cf) {
   const int t = XTID;
   if (t >= g.E * L) return;
   const int l = t / g.E, e = t - l * g.E;
@@ -354,7 +367,14 @@ __global__ __launch_bounds__(BS) void k_edge_hq(isw_grid g, int L, double shelf,
   const double hv = edge_h(h[(size_t)l * N + g.eI[e]], h[(size_t)l * N + g.eJ[e]], shelf);
   const double qv = 0.5 * (q[(size_t)l * NV + g.ev1[e]] + q[(size_t)l * NV + g.ev2[e]]);
   he[t] = hv;
-  if (hef) hef[t] = (float)hv;                     // (mixed form: what the air's fast sub-steps gather)
+  if (cf) {
+    // the air's fast sub-steps' coefficient sign * l_e * h_e at this edge's slot in each of its two cells (the
+    // product the mass step forms, in its order: (sign * l_e) * h_e)
+    const int a = g.eI[e], c = g.eJ[e], sa = g.eslot[2 * e], sc = g.eslot[2 * e + 1];
+    const double le = g.le[e];
+    cf[((size_t)l * N + a) * 6 + sa] = (fast)((double)g.cellSign[a * 6 + sa] * le * hv);
+    cf[((size_t)l * N + c) * 6 + sc] = (fast)((double)g.cellSign[c * 6 + sc] * le * hv);
+  }
   fast* pp = pk + ((size_t)e * L + l) * 3;
   pp[0] = (fast)hv; pp[1] = (fast)u[t]; pp[2] = (fast)qv;
 }
@@ -556,7 +576,7 @@ __global__ __launch_bounds__(BS) void k_af_u(isw_grid g, int L, Div dv, LP p, in
 constexpr int FBS = ISW_PAIR_THREADS;
 template <int L>
 __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
-                                                  const fast* __restrict__ hedge, const double* __restrict__ uin,
+                                                  const fast* __restrict__ coef, const double* __restrict__ uin,
                                                   const double* __restrict__ mu_src, double* __restrict__ mu_out,
                                                   const double* __restrict__ S, const double* __restrict__ Tp,
                                                   const double* __restrict__ G, double* __restrict__ acc,
@@ -596,20 +616,15 @@ __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int 
     if (!sb.active) continue;
     int e[6];
     ring6(g.cellEdge, i, e);
-    double sl[6];
-#pragma unroll
-    for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[i * 6 + m] * g.le[ok0(e[m])];
     double m1k[L], sak[L];
 #pragma unroll
     for (int k = 0; k < L; k++) {
-      const fast* hl = hedge + (size_t)k * E;
+      double c6[6];                                // (sign * l_e * h_e per slot, written with the stage's h_e)
+      load6(coef + ((size_t)k * N + i) * 6, c6);
       const double* ul = uin + (size_t)k * E;
       double t[6];
 #pragma unroll
-      for (int m = 0; m < 6; m++) {
-        const int ek = ok0(e[m]);
-        t[m] = sl[m] * hl[ek] * ul[ek];
-      }
+      for (int m = 0; m < 6; m++) t[m] = c6[m] * ul[ok0(e[m])];
       double div = 0;
 #pragma unroll
       for (int m = 0; m < 6; m++)
@@ -680,7 +695,7 @@ __global__ __launch_bounds__(FBS) void k_af_patch(isw_grid g, LP p, Div dv, int 
 // so that C reads no global memory. Halo edges past the block's FBS threads (a few at level 6) load as they go.
 template <int L>
 __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int stage, int it, int n, double tau,
-                                                   const fast* __restrict__ hedge, const double* __restrict__ uin,
+                                                   const fast* __restrict__ coef, const double* __restrict__ uin,
                                                    const double* __restrict__ mu_src, double* __restrict__ mu_out,
                                                    const double* __restrict__ S, const double* __restrict__ Tp,
                                                    const double* __restrict__ G, double* __restrict__ acc,
@@ -766,21 +781,17 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
     if (!sb.active) continue;
     int e[6];
     ring6(g.cellEdge, i, e);
-    double sl[6];
-#pragma unroll
-    for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[i * 6 + m] * g.le[ok0(e[m])];
     double sak[L], Sk[L];
 #pragma unroll
     for (int k = 0; k < L; k++) {
-      const fast* hl = hedge + (size_t)k * E;
+      double c6[6];                                // (sign * l_e * h_e per slot, written with the stage's h_e)
+      load6(coef + ((size_t)k * N + i) * 6, c6);
       const double* ul = uin + (size_t)k * E;
       double t[6];
 #pragma unroll
       for (int m = 0; m < 6; m++) {
-        const int ek = ok0(e[m]);
-        const double c = sl[m] * hl[ek];
-        if (CF && s < nC) cfs[(k * 6 + m) * P1 + s] = c;
-        t[m] = c * ul[ek];
+        if (CF && s < nC) cfs[(k * 6 + m) * P1 + s] = c6[m];
+        t[m] = c6[m] * ul[ok0(e[m])];
       }
       double div = 0;
 #pragma unroll
@@ -862,15 +873,8 @@ __global__ __launch_bounds__(FBS) void k_af_patch2(isw_grid g, LP p, Div dv, int
 #pragma unroll
         for (int m = 0; m < 6; m++) cf[k][m] = cfs[(k * 6 + m) * P1 + q0];
     } else {
-      int e[6];
-      ring6(g.cellEdge, ci, e);
-      double sl[6];
 #pragma unroll
-      for (int m = 0; m < 6; m++) sl[m] = (double)g.cellSign[ci * 6 + m] * g.le[ok0(e[m])];
-#pragma unroll
-      for (int k = 0; k < L; k++)
-#pragma unroll
-        for (int m = 0; m < 6; m++) cf[k][m] = sl[m] * hedge[(size_t)k * E + ok0(e[m])];
+      for (int k = 0; k < L; k++) load6(coef + ((size_t)k * N + ci) * 6, cf[k]);
     }
     double m1k[L], sak[L];
 #pragma unroll
@@ -1775,18 +1779,16 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
 #if ISW_FORM_MIXED
   fast* pk = s->pkf;
   const fast* wf = g->wf;
-  // the air's fast sub-steps gather the edge thicknesses in FP32 (the limiter's flux keeps the FP64 ones)
-  float* fast_he = s->air ? s->hedgef : nullptr;
-  const fast* fhedge = s->hedgef;
+  // the air's fast sub-steps' coefficients per cell slot (FP32 in the mixed form)
+  fast* cf = s->air && g->ph_off ? s->cfcf : nullptr;
 #else
   fast* pk = s->pk;
   const fast* wf = g->w;
-  float* fast_he = nullptr;
-  const fast* fhedge = s->hedge;
+  fast* cf = s->air && g->ph_off ? s->cfc : nullptr;
 #endif
   ISW_LAUNCH("edge_hq", l_ * n_ * D + l_ * v_ * D + l_ * e_ * D + 4 * e_ * I + l_ * e_ * D + 3 * l_ * e_ * R +
-                            (fast_he ? l_ * e_ * 4 : 0),
-             k_edge_hq, g1((long)E * L), BS, *g, L, s->shelf, hs, s->q, us, s->hedge, pk, fast_he);
+                            (cf ? 6 * l_ * n_ * sizeof(fast) + 2 * e_ + 6 * n_ + e_ * D : 0),
+             k_edge_hq, g1((long)E * L), BS, *g, L, s->shelf, hs, s->q, us, s->hedge, pk, cf);
   const double neoe = g->n_eoe;
   const double slow_bytes = 3 * l_ * e_ * R + neoe * (R + I) + (e_ + 1) * I + 2 * l_ * e_ * D + l_ * n_ * D +
                             2 * e_ * I + e_ * D + (s->nu4 != 0 ? l_ * e_ * D : 0) + air * 2 * l_ * n_ * D + l_ * e_ * D +
@@ -1839,12 +1841,12 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
       for (; it + 1 < n; it += 2, flip ^= 1) {
         double* uo = flip ? s->ufb : s->uf;
         double* mo = flip ? s->mup : s->muf;
-        const double bytes = l_ * e_ * (D + sizeof(fast)) + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
-                             (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D +
-                             e_ * D + 2 * e_ * I;
+        const double bytes = l_ * e_ * D + 6 * l_ * n_ * sizeof(fast) + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
+                             (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * I + n_ * D + e_ * D +
+                             2 * e_ * I;
 #define PAIR(LL)                                                                                                  \
   ISW_LAUNCH_SHM("af_patch2", bytes, k_af_patch2<LL>, dim3((unsigned)g->npatch), dim3(FBS), pair_lds, *g, p, dv,  \
-                 si, it, n, tau, fhedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
+                 si, it, n, tau, cf, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
         switch (L) {
           case 1: PAIR(1); break;
           case 2: PAIR(2); break;
@@ -1863,14 +1865,14 @@ static void stage(const isw_grid* g, const isw_layers* s, const LP& p, const Div
     for (; it < n; it++, flip ^= 1) {
       double* uo = flip ? s->ufb : s->uf;
       double* mo = flip ? s->mup : s->muf;
-      // (algorithmic bytes: hedge, uin, mu in and out, S, Tp, G, acc, the wind out, the cell ring and signs, le,
-      // 1/A, de, the patch's edge list and slots; the halo's re-reads and Phi* in LDS are free)
-      const double bytes = l_ * e_ * (D + sizeof(fast)) + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
-                           (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * (I + C) + e_ * D + n_ * D +
-                           e_ * D + 2 * e_ * I;
+      // (algorithmic bytes: uin, the cells' slot coefficients, mu in and out, S, Tp, G, acc, the wind out, the
+      // cell ring, 1/A, de, the patch's edge list and slots; the halo's re-reads and Phi* in LDS are free)
+      const double bytes = l_ * e_ * D + 6 * l_ * n_ * sizeof(fast) + 3 * l_ * n_ * D + l_ * e_ * D + e_ * D +
+                           (it == 0 ? 1 : 2) * l_ * e_ * D + l_ * e_ * D + 6 * n_ * I + n_ * D + e_ * D +
+                           2 * e_ * I;
 #define PATCH(LL)                                                                                                 \
   ISW_LAUNCH_SHM("af_patch", bytes, k_af_patch<LL>, dim3((unsigned)g->npatch), dim3(FBS), patch_lds, *g, p, dv, si, \
-                 it, n, tau, fhedge, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
+                 it, n, tau, cf, uin, mus, mo, s->S, s->Tp, s->G, s->accL, uo, un)
       switch (L) {
         case 1: PATCH(1); break;
         case 2: PATCH(2); break;

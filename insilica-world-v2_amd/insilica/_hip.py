@@ -28,7 +28,8 @@ class IswGrid(ctypes.Structure):
                 + [(k, ctypes.c_int32) for k in ('pslots2', 'pedges2')]
                 + [(k, _i32p) for k in ('ph2_off', 'ph2_cell', 'px_off', 'px_edge', 'px_slot', 'pc_off', 'pc_eslot')]
                 + [(k, ctypes.c_int32) for k in ('ptedges', 'pad3_')]
-                + [(k, _i32p) for k in ('pi_off', 'pi_edge', 'tc_slot')])
+                + [(k, _i32p) for k in ('pi_off', 'pi_edge', 'tc_slot')]
+                + [('eslot', _i8p)])
 
 
 LAYER_SCRATCH = ('B', 'q', 'Tp', 'F', 'scale', 'G', 'He', 'ubs', 'ub0', 'ub', 'acc', 'eta0', 'eta', 'eta_prev', 'h1',
@@ -57,7 +58,7 @@ class IswLayers(ctypes.Structure):
                 + [(k, _f64p) for k in LAYER_SCRATCH]
                 + [('pkf', _f32p), ('batch', ctypes.POINTER(IswBatch))]
                 + [(k, _f64p) for k in ('tide', 'cbot')] + [('cd_bot', ctypes.c_double)]
-                + [('hedgef', _f32p)])
+                + [('cfc', _f64p), ('cfcf', _f32p)])
 
 
 class IswTransport(ctypes.Structure):

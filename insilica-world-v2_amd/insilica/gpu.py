@@ -164,6 +164,18 @@ def transport_patches(grid, pt):
     return dict(pi_off=pi_off, pi_edge=order, tc_slot=tc, ptedges=ptedges)
 
 
+def edge_slots(grid):
+    """(E, 2): each edge's slot in its first cell's and its second cell's cellEdge ring."""
+    ce = np.asarray(grid.cellEdge, np.int64)
+    eI, eJ = np.asarray(grid.eI, np.int64), np.asarray(grid.eJ, np.int64)
+    out = np.full((len(eI), 2), -1, np.int64)
+    for col, cell in ((0, eI), (1, eJ)):
+        hit = ce[cell] == np.arange(len(eI))[:, None]
+        assert np.all(hit.sum(1) == 1), 'every edge once in each of its cells\' rings'
+        out[:, col] = np.argmax(hit, axis=1)
+    return out
+
+
 _PATCHES = {}
 
 
@@ -225,6 +237,9 @@ class DeviceGrid:
             setattr(s, k, ctypes.cast(getattr(self, k).data_ptr(), _hip._f32p))
         pt, p2 = patch_tables(grid)
         if pt is not None:
+            # each edge's slot in its two cells' rings (where the stage's coefficients of its cells are written)
+            self.eslot = put(edge_slots(grid), np.int8)
+            s.eslot = ctypes.cast(self.eslot.data_ptr(), _hip._i8p)
             s.npatch, s.pcells, s.pslots = pt['npatch'], pt['pcells'], pt['pslots']
             s.pown = int(np.max(np.diff(pt['pe_off'])))
             for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):

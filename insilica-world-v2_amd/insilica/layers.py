@@ -73,9 +73,12 @@ class Layers:
         for k in _hip.LAYER_SCRATCH:
             setattr(s, k, f64(getattr(self, k)))
         s.pkf = ctypes.cast(self.pkf.data_ptr(), _hip._f32p) if self.pkf is not None else None
-        # (the mixed form's FP32 edge thicknesses for the air's fast sub-steps)
-        self.hedgef = z(L, E, dtype=torch.float32) if dg.mixed and air else None
-        s.hedgef = ctypes.cast(self.hedgef.data_ptr(), _hip._f32p) if self.hedgef is not None else None
+        # (the air's fast sub-steps' coefficients sign * l_e * h_e per cell slot, written with each stage's h_e:
+        # FP64 in the plain form, FP32 in the mixed form)
+        self.cfc = z(L * N * 6) if air and not dg.mixed else None
+        self.cfcf = z(L * N * 6, dtype=torch.float32) if air and dg.mixed else None
+        s.cfc = f64(self.cfc)
+        s.cfcf = ctypes.cast(self.cfcf.data_ptr(), _hip._f32p) if self.cfcf is not None else None
         self.s = s
         self.ref = ctypes.byref(s)
         self.batch = None

@@ -149,6 +149,8 @@ def test_grid_check_pair_tables():
     g.npatch, g.pcells, g.pslots, g.pown = 11, 64, 120, 200
     for k in ('ph_off', 'ph_cell', 'pe_off', 'pe_edge', 'pe_slot'):
         setattr(g, k, p32)
+    assert lib.isw_grid_check(ctypes.byref(g)) == E                    # each edge's slots in its cells' rings
+    g.eslot = g.cellSign
     assert lib.isw_grid_check(ctypes.byref(g)) == 0
     g.pown = 0
     assert lib.isw_grid_check(ctypes.byref(g)) == E                    # an owned-edge count is required
