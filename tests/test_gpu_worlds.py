@@ -119,11 +119,21 @@ def test_coupled_with_mixing():
     assert drift <= 1e-13
 
 
-@pytest.mark.parametrize('preset', ['jet', 'coupled'])
-def test_save_load_continues_identically(tmp_path, preset):
+SAVE_CASES = {'jet': ('jet', None), 'coupled': ('coupled', None),
+              # (the coupled world with both drags over seeded orography and the tides over a seeded rough floor:
+              # their settings, arrays included, go into the file and rebuild the same world)
+              'coupled+drags+tides': ('coupled', {'physics': {'oro': 1.0, 'gw': 1.0}, 'tides': True,
+                                                  'sso': 'synthetic',
+                                                  'rough': [float(x) for x in np.random.default_rng(9).uniform(
+                                                      0, 4e4, 2562)]})}
+
+
+@pytest.mark.parametrize('case', sorted(SAVE_CASES))
+def test_save_load_continues_identically(tmp_path, case):
     from insilica.world import World
-    print(f'case: save/load {preset}', flush=True)
-    w = World(4, preset, precision='mixed')
+    preset, params = SAVE_CASES[case]
+    print(f'case: save/load {case}', flush=True)
+    w = World(4, preset, precision='mixed', params=params)
     w.step(7)
     a, b = tmp_path / 'a.iw2', tmp_path / 'b.iw2'
     w.save(a)
