@@ -1327,16 +1327,28 @@ __device__ __forceinline__ double upwind(const double* Fml, const double* p, con
 
 // 3rd-order flux with upwind weight beta (fused.cu:131-138): f (mean - a - b) + |f| (b - a) beta; the curvature
 // stencil on `psf` (the field itself in the plain form, its FP32 copy in the mixed form) with weights wt.
+// The high-order flux of one edge from its stencil (eight cells ix with weights w, its two cells I, J), for the
+// tracer row ps (psf its stencil copy) and the edge's mass flux f.
+__device__ __forceinline__ double hoflux_at(double f, const double* ps, const fast* psf, const int (&ix)[8],
+                                            const fast (&w)[8], int I, int J, double beta) {
+  const fast a = w[0] * psf[ix[0]] + w[1] * psf[ix[1]] + w[2] * psf[ix[2]] + w[3] * psf[ix[3]];
+  const fast b = w[4] * psf[ix[4]] + w[5] * psf[ix[5]] + w[6] * psf[ix[6]] + w[7] * psf[ix[7]];
+  return f * (0.5 * (ps[I] + ps[J]) - (double)(a + b)) + fabs(f) * ((double)(b - a) * beta);
+}
+__device__ __forceinline__ void stencil_at(const isw_grid& g, const fast* wt, int e, int (&ix)[8], fast (&w)[8]) {
+  const int o = e * 8;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    ix[k] = g.d2idx[o + k];
+    w[k] = wt[o + k];
+  }
+}
 __device__ __forceinline__ double hoflux(const double* Fml, const double* ps, const fast* psf, const fast* wt,
                                          const isw_grid& g, int e, double beta) {
-  const double f = Fml[e];
-  const int o = e * 8;
-  const int* idx = g.d2idx;
-  const fast a = wt[o] * psf[idx[o]] + wt[o + 1] * psf[idx[o + 1]] + wt[o + 2] * psf[idx[o + 2]] +
-                 wt[o + 3] * psf[idx[o + 3]];
-  const fast b = wt[o + 4] * psf[idx[o + 4]] + wt[o + 5] * psf[idx[o + 5]] + wt[o + 6] * psf[idx[o + 6]] +
-                 wt[o + 7] * psf[idx[o + 7]];
-  return f * (0.5 * (ps[g.eI[e]] + ps[g.eJ[e]]) - (double)(a + b)) + fabs(f) * ((double)(b - a) * beta);
+  int ix[8];
+  fast w[8];
+  stencil_at(g, wt, e, ix, w);
+  return hoflux_at(Fml[e], ps, psf, ix, w, g.eI[e], g.eJ[e], beta);
 }
 
 __global__ __launch_bounds__(BS) void k_tr_td(isw_grid g, int Lm, int rows, const double* __restrict__ M0,
@@ -1416,7 +1428,6 @@ __global__ __launch_bounds__(BS) void k_tr_stage(isw_grid g, int Lm, int rows, d
 // array's write and re-read (rows x E doubles each way) and one kernel boundary go away; the arithmetic and every
 // sum's order are k_tr_hflux's and k_tr_stage's. pfo: the output's FP32 copy (mixed form; the stencil reads srcf,
 // a different buffer, so no block overwrites what another still reads).
-constexpr int TR_EDGES_PER_THREAD = 6;             // a patch's edges per thread of k_tr_hstage (ptedges <= 384)
 __global__ __launch_bounds__(BS) void k_tr_hstage(isw_grid g, int Lm, int rows, double frac, double beta,
                                                   const double* __restrict__ src, const fast* __restrict__ srcf,
                                                   const fast* __restrict__ wt, const double* __restrict__ Fm,
@@ -1431,21 +1442,23 @@ __global__ __launch_bounds__(BS) void k_tr_hstage(isw_grid g, int Lm, int rows, 
   const int l = y % Lm;
   const int e0 = g.pe_off[b], ne = g.pe_off[b + 1] - e0;
   const int i0 = g.pi_off[b], ni = g.pi_off[b + 1] - i0;
-  if (rowok) {
-    // (a compile-time count of edges per thread, so that every edge's chain of loads -- its index, its stencil, the
-    // values it names -- is in flight at once rather than one edge after another)
-    const double* Fml = Fm + (size_t)l * E;
-    const double* sp = src + (size_t)y * N;
-    const fast* spf = srcf + (size_t)y * N;
-    int ev[TR_EDGES_PER_THREAD];
+  // the stage flux of the patch's edges for the block's four rows: thread t takes edges t, t + BS, ..., each edge's
+  // stencil (its eight cells and weights, its two cells) loaded once for the four rows
+  const int y0 = blockIdx.y * 4, nq = ne + ni;
+  for (int q = ty * BS1 + tx; q < nq; q += BS) {
+    const int e = q < ne ? g.pe_edge[e0 + q] : g.pi_edge[i0 + q - ne];
+    int ix[8];
+    fast w[8];
+    stencil_at(g, wt, e, ix, w);
+    const int I = g.eI[e], J = g.eJ[e];
 #pragma unroll
-    for (int j = 0; j < TR_EDGES_PER_THREAD; j++) {
-      const int q = tx + j * BS1;
-      ev[j] = q < ne ? g.pe_edge[e0 + q] : q < ne + ni ? g.pi_edge[i0 + q - ne] : -1;
+    for (int r = 0; r < 4; r++) {
+      const int yr = y0 + r;
+      if (yr < rows) {
+        const double f = Fm[(size_t)(yr % Lm) * E + e];
+        hs[r * PT + q] = frac * hoflux_at(f, src + (size_t)yr * N, srcf + (size_t)yr * N, ix, w, I, J, beta);
+      }
     }
-#pragma unroll
-    for (int j = 0; j < TR_EDGES_PER_THREAD; j++)
-      if (ev[j] >= 0) hs[ty * PT + tx + j * BS1] = frac * hoflux(Fml, sp, spf, wt, g, ev[j], beta);
   }
   __syncthreads();
   const int i = b * BS1 + tx;
@@ -1477,12 +1490,24 @@ __global__ __launch_bounds__(BS) void k_tr_aflux(isw_grid g, int Lm, int rows, d
                                                  const double* __restrict__ p2, const fast* __restrict__ p2f,
                                                  const fast* __restrict__ wt, const double* __restrict__ Fm,
                                                  double* __restrict__ A) {
-  const int e = XTID;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y, l = y % Lm;
-  if (e >= g.E || y >= rows) return;
-  const double* Fml = Fm + (size_t)l * g.E;
-  A[(size_t)y * g.E + e] = hoflux(Fml, p2 + (size_t)y * g.N, p2f + (size_t)y * g.N, wt, g, e, beta) -
-                           upwind(Fml, psi + (size_t)y * g.N, g, e);
+  // one edge per thread (256 per block), its stencil loaded once for the block's four rows
+  const int e = xblock() * BS + threadIdx.y * BS1 + threadIdx.x;
+  if (e >= g.E) return;
+  int ix[8];
+  fast w[8];
+  stencil_at(g, wt, e, ix, w);
+  const int I = g.eI[e], J = g.eJ[e];
+  const int y0 = blockIdx.y * 4;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int y = y0 + r;
+    if (y < rows) {
+      const double f = Fm[(size_t)(y % Lm) * g.E + e];
+      const double* p = psi + (size_t)y * g.N;
+      A[(size_t)y * g.E + e] = hoflux_at(f, p2 + (size_t)y * g.N, p2f + (size_t)y * g.N, ix, w, I, J, beta) -
+                               f * (f >= 0 ? p[I] : p[J]);
+    }
+  }
 }
 
 __global__ __launch_bounds__(BS) void k_tr_ratios(isw_grid g, int Lm, int rows, const double* __restrict__ psi,
@@ -2057,8 +2082,9 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
   const fast* p1f = tr->p1;
   const fast* p2f = tr->p2;
 #endif
+  // (the flux of a patch's edges for four rows in LDS: at most 64 KiB)
   const bool fused = g->pi_off && g->pcells == BS1 && (!ISW_FORM_MIXED || tr->pf2) &&
-                     g->ptedges <= TR_EDGES_PER_THREAD * BS1;
+                     (size_t)4 * g->ptedges * sizeof(double) <= 65536;
   if (fused) {
     // (algorithmic bytes: the stencil's field and its FP32 copy, the stencil and its weights, Fm, M0, psi, the
     // output and its copy, the cell rings and the patch tables; the flux stays in LDS)
@@ -2094,7 +2120,8 @@ int transport_step(const isw_grid* g, const isw_transport* tr, const double* M0,
                k_tr_stage, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, 1.0 / 2.0, M0, psi, tr->Hf, Fm, tr->p2, pf);
   }
   ISW_LAUNCH("tr_aflux", 2 * r_ * n_ * D + mixed * r_ * n_ * 4 + sten + r_ * e_ * D,
-             k_tr_aflux, c4(E, rows), dim3(BS1, 4), *g, Lm, rows, tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
+             k_tr_aflux, dim3((unsigned)((E + BS - 1) / BS), (unsigned)((rows + 3) / 4)), dim3(BS1, 4), *g, Lm, rows,
+             tr->beta, psi, tr->p2, p2f, wt, Fm, tr->A);
   ISW_LAUNCH("tr_ratios", 2 * r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 6 * n_ * I + cellE + 2 * r_ * n_ * D,
              k_tr_ratios, c4(N, rows), dim3(BS1, 4), *g, Lm, rows, psi, tr->td, M1, tr->A, tr->Rp, tr->Rm);
   ISW_LAUNCH("tr_apply", r_ * n_ * D + l_ * n_ * D + r_ * e_ * D + 2 * r_ * n_ * D + cellE + r_ * n_ * D,
