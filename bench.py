@@ -75,7 +75,7 @@ def pmc_traffic(name):
     if not TRAFFIC_PROFILE.exists():
         return None
     for line in TRAFFIC_PROFILE.read_text().splitlines():
-        m = re.match(r'\|\s*`k_(\w+)`\s*\|.*\|\s*([0-9.]+)\s*\|\s*$', line)
+        m = re.match(r'\|\s*`k_(\w+)(?:<\d+>)?`\s*\|.*\|\s*([0-9.]+)\s*\|\s*$', line)
         if m and m.group(1) == name:
             return float(m.group(2)) * 1e6
     return None
